@@ -1,0 +1,191 @@
+"""Benchmark of the static linear-elastic solve on one MI355X.
+
+Workload: the 100^3 hex box (1,030,301 nodes, 3,090,903 DOF, 245,438,109 nonzeros),
+E = 1e7 * lognormal(sigma = 1.0) per element, nu ~ U[0.25, 0.35], bottom face clamped,
+uniform downward traction 1e4 Pa on the top face (synthetic, seeded).  One step is the
+whole device path with the mesh and materials already in HBM: CRS graph build, element
+stiffness assembly, traction RHS, Dirichlet elimination and BiCGSTAB to a relative
+residual ||b - Ax|| / ||b|| of 1e-8 (FP64).  value = seconds per step (time to solve).
+
+The same line reports the mixed-precision solve on the same matrix, the SpMV kernels'
+bandwidth, the roofline of the dominant kernel (y = A p fused with r0.y, timed live with
+hipEvents around every launch) and the CPU baseline (the oracle's BiCGSTAB iterations on
+the host cores, extrapolated to the GPU iteration count).
+
+    python bench.py [--steps K] [--warmup W] [--size 100] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "time-to-solve @relres 1e-8, 3M-DOF hex; SpMV GB/s; mixed vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(kind: int, n: int, nnz: int) -> float:
+    """Algorithmic HBM bytes of one y = A x (x read once, y written once)."""
+    nn = n // 3
+    nblocks = nnz // 9
+    nslices = (nn + 63) // 64
+    if kind == 1:  # CRS: int64 row_ptr, int32 col_idx, FP64 values
+        return 12.0 * nnz + 8.0 * (n + 1) + 16.0 * n
+    if kind == 2:  # blocked FP64: one int32 node index per 3x3 block
+        return 8.0 * nnz + 4.0 * nblocks + 12.0 * nslices + 8 + 16.0 * n
+    return 4.0 * nnz + 4.0 * nblocks + 12.0 * nslices + 8 + 8.0 * n  # blocked FP32
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=4)
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; the multi-GPU (z-slab sharded) path is a design "
+              "note only, so --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    from plantfem_amd import _lib as L
+    from plantfem_amd import workloads as W
+    from plantfem_amd.fem import FEMDomain, FEMSolver
+
+    t0 = time.perf_counter()
+    w = W.box(a.size, sigma=1.0, seed=a.seed, nu_range=(0.25, 0.35))
+    n = w.ndof
+    log(f"workload {w.name}: n={n} ne={w.ne} (mesh built in {time.perf_counter() - t0:.1f}s)")
+    ctx = L.default_context()
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setCRS(DOF=3)
+    s.setStiffness(w.E, w.nu)  # materials resident from here on
+    ctx.synchronize()
+    nnz = s.nnz
+    log(f"nnz={nnz}")
+
+    s.itrmax = 40000
+
+    def step(precision=64, timing=0):
+        s.setCRS(DOF=3)                       # ZeroMatrix graph build (device)
+        s.setStiffness()                      # element stiffness assembly (device)
+        s.addTraction(w.traction_box, w.pressure)
+        s.fix(1, w.fixed_dofs, FixValue=0.0)
+        s.relative_er = 1e-8
+        s.solve(precision=precision, stop_mode=1, download=False, kernel_timing=timing, target_relres=1e-8)
+        return dict(s.last_stats)
+
+    # ---- FP64 time to solve (timed steps carry per-kernel hipEvent timing) -------------
+    for i in range(a.warmup):
+        st = step(64, 1)
+        log(f"warmup {i}: {st['iterations']} it, {st['solve_ms']:.1f} ms, relres {st['true_relres']:.2e}")
+    ctx.synchronize()
+    times, stats = [], []
+    for i in range(a.steps):
+        ctx.synchronize()
+        t = time.perf_counter()
+        st = step(64, 1)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t)
+        stats.append(st)
+        log(f"step {i}: {times[-1]:.3f} s, {st['iterations']} it, solve {st['solve_ms']:.1f} ms, "
+            f"relres {st['true_relres']:.2e}, K1 {st['spmv_p_ms']:.3f} ms")
+    t_step = float(np.median(times))
+    st64 = stats[-1]
+
+    # graph-captured solve without per-kernel events, for the overhead of the timing mode
+    t = time.perf_counter()
+    st_graph = step(64, 0)
+    ctx.synchronize()
+    t_graph = time.perf_counter() - t
+
+    # ---- mixed precision on the same matrix --------------------------------------------
+    mixed_times, st32 = [], None
+    for i in range(max(1, a.steps)):
+        ctx.synchronize()
+        t = time.perf_counter()
+        st32 = step(32, 0)
+        ctx.synchronize()
+        mixed_times.append(time.perf_counter() - t)
+        log(f"mixed {i}: {mixed_times[-1]:.3f} s, outer {st32['outer_iterations']}, inner {st32['iterations']}, "
+            f"relres {st32['true_relres']:.2e}")
+    t_mixed = float(np.median(mixed_times))
+    x32 = s.getX()
+    step(64, 0)
+    x64 = s.getX()
+    mixed_err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+
+    # ---- SpMV kernels on the eliminated matrix ----------------------------------------
+    spmv = {}
+    xr = np.random.default_rng(1).standard_normal(n)
+    for kind, name in ((1, "crs_f64"), (2, "blocked_f64"), (3, "blocked_f32")):
+        s.spmv(kind, xr, repeat=3, download=False)
+        _, ms = s.spmv(kind, None, repeat=20, download=False)
+        gbs = spmv_bytes(kind, n, nnz) / (ms * 1e-3) / 1e9
+        spmv[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)}
+        log(f"spmv {name}: {ms:.3f} ms {gbs:.0f} GB/s")
+
+    # ---- roofline of the dominant kernel: y = A p (+ r0.y), blocked FP64 ---------------
+    k1_bytes = spmv_bytes(2, n, nnz) + 8.0 * n  # + r0 read for the fused dot
+    k1_ms = float(np.mean([x["spmv_p_ms"] for x in stats]))
+    k1_gbs = k1_bytes / (k1_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_spmv_p (blocked FP64 y=A p fused with r0.y)", "bytes_per_launch": k1_bytes,
+                "avg_launch_ms": round(k1_ms, 4)}
+
+    # ---- CPU baseline: oracle BiCGSTAB iterations on the host ---------------------------
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+        crs = s.getCRS()   # eliminated system of the last solve
+        rhs = s.getRHS()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        O.bicgstab(crs.row_ptr, crs.col_idx, crs.val, rhs, itrmax=1, nthreads=threads)  # warm
+        t = time.perf_counter()
+        O.bicgstab(crs.row_ptr, crs.col_idx, crs.val, rhs, itrmax=a.cpu_iters, nthreads=threads)
+        t_it = (time.perf_counter() - t) / a.cpu_iters
+        cpu = {"value": round(t_it * st64["iterations"], 2), "unit": "s", "cores": threads, "kind": "port",
+               "sample": f"{a.cpu_iters} bicgstab_CRS_2 iterations (OpenMP row SpMV with per-nonzero atomics, "
+                         f"serial dots) of the same eliminated {n}-DOF system, {t_it:.3f} s/iteration, "
+                         f"extrapolated to the GPU's {st64['iterations']} iterations"}
+        log(f"cpu: {t_it:.3f} s/iteration on {threads} threads")
+
+    iters = st64["iterations"]
+    line = {
+        "metric": METRIC, "value": round(t_step, 4), "unit": "s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * t_step, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"hex box {a.size}^3 (3.09M DOF at 100^3), E=1e7*lognormal(1.0), nu~U[0.25,0.35], "
+                               "clamped bottom, 1e4 Pa top traction; graph+assembly+BC+BiCGSTAB to relres 1e-8",
+                   "n_dof": n, "nnz": nnz, "seed": a.seed},
+        "iterations": iters, "solve_ms": round(st64["solve_ms"], 2), "true_relres": st64["true_relres"],
+        "dof_iterations_per_s": round(n * iters / (st64["solve_ms"] * 1e-3), 1),
+        "solve_ms_graph_mode": round(st_graph["solve_ms"], 2), "step_s_graph_mode": round(t_graph, 4),
+        "mixed": {"time_to_solve_s": round(t_mixed, 4), "solve_ms": round(st32["solve_ms"], 2),
+                  "outer": st32["outer_iterations"], "inner_iterations": st32["iterations"],
+                  "true_relres": st32["true_relres"], "rel_err_vs_fp64": mixed_err,
+                  "speedup_vs_fp64": round(t_step / t_mixed, 3)},
+        "spmv": spmv, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,209 @@
+/*
+ * pfgpu.h -- C-ABI of the MI355X (gfx950) implementation of plantFEM's static
+ * linear-elastic solve path: hex8 CRS graph build, element stiffness assembly,
+ * Dirichlet elimination and BiCGSTAB over CRS.
+ *
+ * Plain pointers and sizes only.  Array conventions are the reference's Fortran ones:
+ *   NodCoord(nn,3) real64 column-major      nodcoord[d*nn + i]
+ *   ElemNod(ne,nne) int32 column-major      elemnod[a*ne + e], node ids 1-based
+ *   CRS_: row_ptr int64 (n+1, 1-based), col_idx int32 (nnz, 1-based), val real64
+ *         (SparseClass.f90:128-131, FEMSolverClass.f90:24-27)
+ *   element matrices Ke(24,24) real64 column-major   ke[c*24 + r]
+ * Functions named pf_dev_* take DEVICE pointers (e.g. torch.cuda tensors); all
+ * others take HOST pointers unless stated.
+ *
+ * Every function returns a status (PF_OK = 0).  The message of the last failure on
+ * the calling thread is available from pf_last_error().
+ */
+#ifndef PFGPU_H
+#define PFGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------------- */
+#define PF_OK 0
+#define PF_BREAKDOWN_VV 1    /* v.v == 0: reference stops "Bicgstab devide by zero" (FEMSolverClass.f90:2266) */
+#define PF_BREAKDOWN_C2C3 2  /* c2*c3 == 0: same stop (FEMSolverClass.f90:2295) */
+#define PF_NOT_CONVERGED 3   /* itrmax reached; the reference ends silently (FEMSolverClass.f90:2241) */
+#define PF_INVALID 4         /* bad ids/sizes, missing CRS slot ("fill-in request" :1040-1047), call order */
+#define PF_ERR_HIP -1        /* HIP runtime error */
+#define PF_ERR_NOMEM -2      /* device allocation failed */
+
+typedef struct pf_ctx pf_ctx;       /* one device + one HIP stream */
+typedef struct pf_system pf_system; /* device-resident FEMSolver_ state (mesh, CRS, RHS, BCs, x) */
+
+/* Solver options; defaults from FEMSolverClass.f90:69-71.  er0 and relative_er default
+ * to dble(1.0e-10): a REAL(4) literal widened to REAL(8), i.e. 1.00000001335e-10. */
+typedef struct {
+  int32_t itrmax;        /* 100000 */
+  int32_t precision;     /* 64: plain FP64 BiCGSTAB (bicgstab_CRS_2); 32: FP32 inner + FP64 refinement */
+  double er0;            /* absolute stop sqrt(r.r) < er0 */
+  double relative_er;    /* stop sqrt(r.r / r1.r1) < relative_er (r1 = residual after iteration 1) */
+  double inner_tol;      /* precision 32: inner FP32 solve relative tolerance (default 1e-5) */
+  double target_relres;  /* precision 32: stop when ||b-Ax||/||b|| (FP64) < target (default 1e-8) */
+  int32_t max_outer;     /* precision 32: refinement steps cap (default 50) */
+  int32_t spmv_kind;     /* 0 auto (blocked when the CRS has DOF=3 node blocks), 1 CRS row kernel, 2 blocked */
+  int32_t check_every;   /* iterations enqueued between host polls of the stop flag (default 32) */
+  int32_t use_graph;     /* capture the polled chunk in a hipGraph (default 1) */
+  int32_t debug;         /* print sqrt(r.r) per poll */
+  int32_t stop_mode;     /* 0: the reference's tests (default); 1: sqrt(r.r) < relative_er * ||b|| */
+  int32_t kernel_timing; /* 1: bracket every kernel of the iteration loop with hipEvents (no hipGraph)
+                            and report the average device time of each in pf_stats */
+  int32_t jacobi;        /* precision 32: solve the FP32 inner system symmetrically scaled,
+                            D^-1/2 A D^-1/2 with D = diag(A) (default 1) */
+} pf_solve_opts;
+
+typedef struct {
+  int32_t status;           /* PF_* code of the solve */
+  int32_t iterations;       /* BiCGSTAB iterations (FP64) or total inner iterations (mixed) */
+  int32_t outer_iterations; /* refinement steps (mixed), 0 for FP64 */
+  int32_t spmv_kind_used;   /* 1 CRS row kernel, 2 blocked */
+  double rr;                /* final r.r of the recurrence (FP64 form) */
+  double rr1;               /* r.r after iteration 1 */
+  double true_relres;       /* ||b - A x|| / ||b|| in FP64 after the solve (eliminated system) */
+  double solve_ms;          /* device time of the solve, hipEvents on the context stream */
+  double setup_ms;          /* device time of per-solve setup (blocked layout, FP32 copy) */
+  /* kernel_timing = 1: average device ms of y = A p, v = A e, the x/r update and the p update
+     (FP32 inner kernels for precision 32); 0 otherwise */
+  double spmv_p_ms, spmv_e_ms, update_ms, pupdate_ms;
+  int64_t timed_iterations;
+} pf_stats;
+
+void pf_solve_opts_default(pf_solve_opts *opts);
+const char *pf_last_error(void);
+int pf_version(void);
+
+/* ---- context --------------------------------------------------------------------- */
+int pf_ctx_create(int device, pf_ctx **out);
+int pf_ctx_destroy(pf_ctx *ctx);
+int pf_ctx_synchronize(pf_ctx *ctx);
+/* The context's HIP stream (hipStream_t as void*), for callers that order their own work. */
+void *pf_ctx_stream(pf_ctx *ctx);
+
+/* ---- stateless host-pointer entry points ---------------------------------------- */
+
+/* ZeroMatrix_as_CRS_FEMDomain(DOF) (FEMDomainClass.f90:10581-10840): CRS graph of the
+ * mesh in the reference's first-appearance column order.  Two-phase: call with
+ * row_ptr == col_idx == NULL to get *nnz, then with arrays of n*dof+1 and *nnz. */
+int pf_crs_graph(pf_ctx *ctx, int64_t nn, int64_t ne, int32_t nne, const int32_t *elemnod,
+                 int32_t dof, int64_t *nnz, int64_t *row_ptr, int32_t *col_idx);
+
+/* StiffnessMatrixFEMDomain(ElementID,E,v) for elements first..first+count-1 (1-based)
+ * (FEMDomainClass.f90:11457-11515).  E, nu indexed by element (length ne).
+ * jmode 0 = reference B = J^-T dN/dxi (FEMDomainClass.f90:12289); 1 = textbook J^-1 dN/dxi.
+ * ke receives count matrices of 24x24, column-major. */
+int pf_hex8_stiffness(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord,
+                      const int32_t *elemnod, const double *E, const double *nu, int32_t jmode,
+                      int64_t first, int64_t count, double *ke);
+
+/* StiffnessMatrix_as_CRS_FEMDomain(YoungModulus(:),PoissonRatio(:))
+ * (FEMDomainClass.f90:11123-11232): val (nnz) of the ZeroMatrix pattern given by
+ * row_ptr/col_idx, summed in ascending element order. */
+int pf_hex8_stiffness_crs(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord,
+                          const int32_t *elemnod, const double *E, const double *nu,
+                          int32_t jmode, const int64_t *row_ptr, const int32_t *col_idx,
+                          double *val);
+
+/* sub_crs_matvec (LinearSolverClass.f90:3728-3776): y = A x, 1-based CRS, host arrays. */
+int pf_spmv_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                const double *val, const double *x, double *y);
+
+/* bicgstab_CRS_2 (FEMSolverClass.f90:2199-2298) on host arrays; x holds x0 on entry and
+ * the solution on exit.  This is the plug-in LinearSolver(row_ptr,col_idx,val,rhs,x) of
+ * solveFEMSolver_UserDefinedLinearSolver (FEMSolverClass.f90:1638-1648); like that hook
+ * it applies no Dirichlet elimination.  opts NULL = defaults; stats may be NULL. */
+int pf_bicgstab_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                    const double *val, const double *rhs, double *x, const pf_solve_opts *opts,
+                    pf_stats *stats);
+
+/* ---- device-pointer entry points (torch interop) ---------------------------------- */
+/* y = A x on device arrays; 1-based CRS as above. */
+int pf_dev_spmv_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                    const double *val, const double *x, double *y);
+int pf_dev_spmv_crs_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                        const float *val, const float *x, float *y);
+
+/* ---- FEMSolver_ state on the device (FEMSolverClass.f90:13-164) ------------------ */
+int pf_system_create(pf_ctx *ctx, pf_system **out);
+int pf_system_destroy(pf_system *sys);
+
+/* FEMDomain mesh upload (setDomain, FEMSolverClass.f90:329-419).  nne = 8 for hex8. */
+int pf_system_set_mesh(pf_system *sys, int64_t nn, int64_t ne, int32_t nne,
+                       const double *nodcoord, const int32_t *elemnod);
+
+/* setCRS(DOF) single-domain branch (FEMSolverClass.f90:679-694): ZeroMatrix on the device,
+ * zero RHS and x.  Returns nnz. */
+int pf_system_zero_matrix(pf_system *sys, int32_t dof, int64_t *nnz);
+
+/* setCRS(CRS_) (FEMSolverClass.f90:421-429): upload an existing CRS (host arrays). */
+int pf_system_set_crs(pf_system *sys, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                      const int32_t *col_idx, const double *val);
+
+/* Sizes of the current CRS. */
+int pf_system_sizes(pf_system *sys, int64_t *n, int64_t *nnz, int64_t *nn, int64_t *ne);
+
+/* getCRS: download (any pointer may be NULL). */
+int pf_system_get_crs(pf_system *sys, int64_t *row_ptr, int32_t *col_idx, double *val);
+
+/* StiffnessMatrix(YoungModulus(:),PoissonRatio(:)) assembled into the CRS values on the
+ * device (overwrites val).  E, nu: host arrays of length ne, kept resident on the device;
+ * E == nu == NULL reuses the resident arrays of the previous call. */
+int pf_system_stiffness(pf_system *sys, const double *E, const double *nu, int32_t jmode);
+
+/* setMatrix(DomainID=1,ElementID,DOF,Matrix) for a batch (FEMSolverClass.f90:901-909,
+ * :1020-1057): elem_ids 1-based, ke count matrices of (nne*dof)^2 column-major.  Adds into
+ * val; a missing slot returns PF_INVALID like the reference's fill-in stop. */
+int pf_system_add_element_matrices(pf_system *sys, int64_t count, const int32_t *elem_ids,
+                                   const double *ke);
+
+/* setRHS (FEMSolverClass.f90:431-437) / setVector batch (:913-920, :1059-1069) / getRHS */
+int pf_system_set_rhs(pf_system *sys, const double *rhs);
+int pf_system_add_element_vectors(pf_system *sys, int64_t count, const int32_t *elem_ids,
+                                  const double *vec);
+int pf_system_get_rhs(pf_system *sys, double *rhs);
+
+/* TractionVector_by_elemFEMDomain summed over elements, added to the RHS
+ * (FEMDomainClass.f90:17210-17292).  box = xmin,xmax,ymin,ymax,zmin,zmax (open box, as
+ * Range_%inside).  pressure > 0 pushes along the outward facet normal. */
+int pf_system_add_traction(pf_system *sys, const double *box, double pressure);
+
+/* fix(DomainID=1,IDs,FixValue|FixValues) (FEMSolverClass.f90:1073-1136): ids are 1-based
+ * DOF ids; ids < 1 or > n are skipped; values NULL -> fixvalue for all. */
+int pf_system_fix(pf_system *sys, int64_t nids, const int32_t *ids, const double *values,
+                  double fixvalue);
+
+/* solve() (FEMSolverClass.f90:1697-1825): Dirichlet elimination on the device CRS and RHS
+ * (destructive, as the reference), BiCGSTAB from x0 (NULL = zeros), x_fixed = ubar.
+ * x_out (host, length n) may be NULL to keep x on the device. */
+int pf_system_solve(pf_system *sys, const pf_solve_opts *opts, const double *x0, double *x_out,
+                    pf_stats *stats);
+int pf_system_get_x(pf_system *sys, double *x);
+
+/* y = A x with the system's current CRS (elimination included once solve() has run).
+ * kind 1: CRS row kernel, FP64; 2: blocked kernel, FP64; 3: blocked kernel, FP32 values and
+ * vectors (x rounded to FP32, y widened back).  x, y are host arrays of length n (x NULL:
+ * reuse the last x; y NULL: keep on device).  The kernel is launched `repeat` times and *ms
+ * (may be NULL) receives the average device time of one launch (hipEvents). */
+int pf_system_spmv(pf_system *sys, int32_t kind, const double *x, double *y, int32_t repeat, double *ms);
+
+/* Device pointers of the system's arrays, for torch interop (valid until the next call
+ * that reallocates: set_mesh, zero_matrix, set_crs, destroy). */
+typedef struct {
+  int64_t n, nnz, nn, ne;
+  int64_t *row_ptr;
+  int32_t *col_idx;
+  double *val;
+  double *rhs;
+  double *x;
+  double *nodcoord;
+  int32_t *elemnod;
+} pf_system_view;
+int pf_system_view_get(pf_system *sys, pf_system_view *view);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFGPU_H */

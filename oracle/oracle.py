@@ -1,0 +1,205 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+_lib = None
+
+P64, P32, PD, PU8 = C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_uint8)
+I64, I32, D = C.c_int64, C.c_int, C.c_double
+
+_SIG = {
+    "orc_cube3d": (None, [I32, I32, I32, D, D, D, PD, P32]),
+    "orc_zero_matrix": (I64, [I64, I64, I32, P32, I32, P64, P32]),
+    "orc_zero_matrix_legacy": (I64, [I64, I64, I32, P32, I32, P64, P32]),
+    "orc_hex8_stiffness": (None, [PD, D, D, I32, PD]),
+    "orc_hex8_stiffness_all": (None, [I64, I64, PD, P32, PD, PD, I32, PD]),
+    "orc_assemble_stiffness": (I32, [I64, I64, PD, P32, PD, PD, I32, P64, P32, PD]),
+    "orc_set_element_matrix": (I32, [I64, I32, P32, I64, I32, PD, P64, P32, PD]),
+    "orc_traction_box": (None, [I64, I64, PD, P32, PD, D, PD]),
+    "orc_select_box": (I64, [I64, PD, PD, P32]),
+    "orc_apply_dirichlet": (None, [I64, P64, P32, PD, PD, PU8, PD]),
+    "orc_spmv": (None, [I64, P64, P32, PD, PD, PD, I32]),
+    "orc_bicgstab_crs2": (I32, [I64, P64, P32, PD, PD, PD, I32, D, D, I32, P32, PD, PD, PD]),
+    "orc_solve": (I32, [I64, P64, P32, PD, PD, PU8, PD, PD, I32, D, D, I32, P32, PD, PD]),
+}
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        lib = C.CDLL(str(LIB))
+        for k, (r, a) in _SIG.items():
+            f = getattr(lib, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = lib
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _fl(a, dt):
+    return np.asfortranarray(a, dtype=dt).ravel(order="F")
+
+
+def cube3d(xn, yn, zn, lx=1.0, ly=1.0, lz=1.0):
+    nn = (xn + 1) * (yn + 1) * (zn + 1)
+    ne = xn * yn * zn
+    xyz = np.empty(nn * 3)
+    en = np.empty(ne * 8, dtype=np.int32)
+    load().orc_cube3d(xn, yn, zn, lx, ly, lz, _p(xyz, C.c_double), _p(en, C.c_int32))
+    return xyz.reshape(3, nn).T.copy(order="F"), en.reshape(8, ne).T.copy(order="F")
+
+
+def zero_matrix(elemnod, nn, dof=3, legacy=False):
+    en = _fl(elemnod, np.int32)
+    ne, nne = elemnod.shape
+    f = load().orc_zero_matrix_legacy if legacy else load().orc_zero_matrix
+    nnz = f(nn, ne, nne, _p(en, C.c_int32), dof, None, None)
+    if nnz < 0:
+        raise ValueError("invalid connectivity")
+    rp = np.empty(dof * nn + 1, dtype=np.int64)
+    ci = np.empty(nnz, dtype=np.int32)
+    f(nn, ne, nne, _p(en, C.c_int32), dof, _p(rp, C.c_int64), _p(ci, C.c_int32))
+    return rp, ci
+
+
+def hex8_stiffness(X, E, nu, jmode=0):
+    """Ke[r, c] (24x24) of one element with coordinates X (8,3)."""
+    x = _c(X, np.float64).ravel()
+    ke = np.empty(576)
+    load().orc_hex8_stiffness(_p(x, C.c_double), float(E), float(nu), int(jmode), _p(ke, C.c_double))
+    return ke.reshape(24, 24).T.copy()
+
+
+def hex8_stiffness_all(nodcoord, elemnod, E, nu, jmode=0):
+    xyz = _fl(nodcoord, np.float64)
+    en = _fl(elemnod, np.int32)
+    ne = elemnod.shape[0]
+    ke = np.empty(ne * 576)
+    load().orc_hex8_stiffness_all(nodcoord.shape[0], ne, _p(xyz, C.c_double), _p(en, C.c_int32),
+                                  _p(_c(E, np.float64), C.c_double), _p(_c(nu, np.float64), C.c_double), jmode,
+                                  _p(ke, C.c_double))
+    return ke.reshape(ne, 24, 24).transpose(0, 2, 1)
+
+
+def assemble(nodcoord, elemnod, E, nu, row_ptr, col_idx, jmode=0):
+    xyz = _fl(nodcoord, np.float64)
+    en = _fl(elemnod, np.int32)
+    val = np.empty(col_idx.size)
+    rc = load().orc_assemble_stiffness(nodcoord.shape[0], elemnod.shape[0], _p(xyz, C.c_double), _p(en, C.c_int32),
+                                       _p(_c(E, np.float64), C.c_double), _p(_c(nu, np.float64), C.c_double), jmode,
+                                       _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                                       _p(val, C.c_double))
+    if rc:
+        raise ValueError("assembly: missing CRS slot")
+    return val
+
+
+def set_element_matrix(elemnod, elem, dof, ke, row_ptr, col_idx, val):
+    en = _fl(elemnod, np.int32)
+    k = np.asfortranarray(ke, dtype=np.float64).ravel(order="F")
+    return load().orc_set_element_matrix(elemnod.shape[0], elemnod.shape[1], _p(en, C.c_int32), int(elem), dof,
+                                         _p(k, C.c_double), _p(row_ptr, C.c_int64), _p(col_idx, C.c_int32),
+                                         _p(val, C.c_double))
+
+
+def traction_box(nodcoord, elemnod, box, pressure, rhs=None):
+    nn = nodcoord.shape[0]
+    out = np.zeros(3 * nn) if rhs is None else np.array(rhs, dtype=np.float64)
+    load().orc_traction_box(nn, elemnod.shape[0], _p(_fl(nodcoord, np.float64), C.c_double),
+                            _p(_fl(elemnod, np.int32), C.c_int32), _p(_c(box, np.float64), C.c_double),
+                            float(pressure), _p(out, C.c_double))
+    return out
+
+
+def select_box(nodcoord, box):
+    xyz = _fl(nodcoord, np.float64)
+    b = _c(box, np.float64)
+    m = load().orc_select_box(nodcoord.shape[0], _p(xyz, C.c_double), _p(b, C.c_double), None)
+    ids = np.empty(m, dtype=np.int32)
+    load().orc_select_box(nodcoord.shape[0], _p(xyz, C.c_double), _p(b, C.c_double), _p(ids, C.c_int32))
+    return ids
+
+
+def apply_dirichlet(row_ptr, col_idx, val, rhs, fixed, ubar):
+    val = np.array(val, dtype=np.float64)
+    rhs = np.array(rhs, dtype=np.float64)
+    load().orc_apply_dirichlet(rhs.size, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                               _p(val, C.c_double), _p(rhs, C.c_double), _p(_c(fixed, np.uint8), C.c_uint8),
+                               _p(_c(ubar, np.float64), C.c_double))
+    return val, rhs
+
+
+def spmv(row_ptr, col_idx, val, x, nthreads=0):
+    n = row_ptr.size - 1
+    y = np.empty(n)
+    load().orc_spmv(n, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                    _p(_c(val, np.float64), C.c_double), _p(_c(x, np.float64), C.c_double), _p(y, C.c_double),
+                    nthreads)
+    return y
+
+
+def bicgstab(row_ptr, col_idx, val, b, x0=None, itrmax=100000, er0=None, relative_er=None, nthreads=0, hist=False):
+    """bicgstab_CRS_2; returns (x, status, iters, rr, rr1[, hist])."""
+    er = float(np.float32(1e-10))
+    er0 = er if er0 is None else er0
+    relative_er = er if relative_er is None else relative_er
+    n = b.size
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = C.c_int(0)
+    rr = C.c_double(0)
+    rr1 = C.c_double(0)
+    h = np.zeros(max(itrmax, 1)) if hist else None
+    st = load().orc_bicgstab_crs2(n, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                                  _p(_c(val, np.float64), C.c_double), _p(_c(b, np.float64), C.c_double),
+                                  _p(x, C.c_double), itrmax, er0, relative_er, nthreads, C.byref(it), C.byref(rr),
+                                  C.byref(rr1), _p(h, C.c_double))
+    out = (x, st, it.value, rr.value, rr1.value)
+    return out + (h[:it.value],) if hist else out
+
+
+def solve(row_ptr, col_idx, val, rhs, fixed, ubar, x0=None, itrmax=100000, er0=None, relative_er=None, nthreads=0):
+    """solveFEMSolver on copies; returns (x, status, iters, rr, rr1)."""
+    er = float(np.float32(1e-10))
+    er0 = er if er0 is None else er0
+    relative_er = er if relative_er is None else relative_er
+    val = np.array(val, dtype=np.float64)
+    rhs = np.array(rhs, dtype=np.float64)
+    n = rhs.size
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = C.c_int(0)
+    rr = C.c_double(0)
+    rr1 = C.c_double(0)
+    st = load().orc_solve(n, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                          _p(val, C.c_double), _p(rhs, C.c_double), _p(_c(fixed, np.uint8), C.c_uint8),
+                          _p(_c(ubar, np.float64), C.c_double), _p(x, C.c_double), itrmax, er0, relative_er,
+                          nthreads, C.byref(it), C.byref(rr), C.byref(rr1))
+    return x, st, it.value, rr.value, rr1.value
+
+
+def build_system(w):
+    """Oracle CRS system of a Workload: (row_ptr, col_idx, val, rhs, fixed(uint8), ubar)."""
+    rp, ci = zero_matrix(w.elemnod, w.nn, 3)
+    val = assemble(w.nodcoord, w.elemnod, w.E, w.nu, rp, ci)
+    rhs = traction_box(w.nodcoord, w.elemnod, w.traction_box, w.pressure)
+    fixed = np.zeros(3 * w.nn, dtype=np.uint8)
+    fixed[w.fixed_dofs - 1] = 1
+    return rp, ci, val, rhs, fixed, np.zeros(3 * w.nn)

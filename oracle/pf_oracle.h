@@ -1,0 +1,103 @@
+/*
+ * pf_oracle.h -- CPU restatement of plantFEM's static linear-elastic solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path
+ * (plantfem_amd/csrc) and the timed CPU baseline in bench.py.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product library never links, loads or calls it.
+ *
+ * Conventions follow the reference's Fortran storage:
+ *   - node coordinates NodCoord(nn,3) column-major  -> nodcoord[d*nn + i]
+ *   - connectivity   ElemNod(ne,nne) column-major    -> elemnod[a*ne + e], 1-based node ids
+ *   - CRS: row_ptr int64 (n+1, 1-based), col_idx int32 (nnz, 1-based), val double
+ *   - element matrices Ke(24,24) column-major        -> ke[c*24 + r]
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * the plantFEM source tree).
+ */
+#ifndef PF_ORACLE_H
+#define PF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes shared with the product library */
+#define ORC_OK 0
+#define ORC_BREAKDOWN_VV 1      /* "Bicgstab devide by zero" at v.v == 0  */
+#define ORC_BREAKDOWN_C2C3 2    /* "Bicgstab devide by zero" at c2*c3 == 0 */
+#define ORC_NOT_CONVERGED 3     /* loop ran to itrmax (silent in the reference) */
+#define ORC_INVALID 4
+
+/* A1: Cube3D structured box (MeshClass.f90:6348-6361, :6745-6808, :6992-7090) */
+void orc_cube3d(int xn, int yn, int zn, double lx, double ly, double lz,
+                double *nodcoord, int32_t *elemnod);
+
+/* A3: ZeroMatrix_as_CRS_FEMDomain, non-legacy branch (FEMDomainClass.f90:10665-10838).
+ * Returns nnz; row_ptr/col_idx may be NULL (count only).  Returns -1 on invalid input. */
+int64_t orc_zero_matrix(int64_t nn, int64_t ne, int nne, const int32_t *elemnod, int dof,
+                        int64_t *row_ptr, int32_t *col_idx);
+
+/* A3 legacy branch: COO append + to_CRS + DOF extension (FEMDomainClass.f90:10597-10655,
+ * SparseClass.f90:358-409, :450-500).  Independent second route to the same arrays. */
+int64_t orc_zero_matrix_legacy(int64_t nn, int64_t ne, int nne, const int32_t *elemnod, int dof,
+                               int64_t *row_ptr, int32_t *col_idx);
+
+/* A4-A8: StiffnessMatrixFEMDomain for one hex8 element (FEMDomainClass.f90:11457-11515).
+ * X is the element coordinate table ElemCoord(8,3) row-major x[a*3+d].
+ * jmode 0 = reference B (J^-T dN/dxi, FEMDomainClass.f90:12289), 1 = textbook J^-1 dN/dxi.
+ * ke is 24x24 column-major (Fortran layout). */
+void orc_hex8_stiffness(const double *X, double E, double nu, int jmode, double *ke);
+
+/* batched form over a mesh: ke[(e*24*24) + c*24 + r] */
+void orc_hex8_stiffness_all(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                            const double *E, const double *nu, int jmode, double *ke);
+
+/* A9: StiffnessMatrix_as_CRS_FEMDomain serial order (FEMDomainClass.f90:11193-11229):
+ * val += Ke in ascending element order with a linear search of the row.  val must hold
+ * the ZeroMatrix pattern's nnz entries (it is zeroed here).  Returns 0 or ORC_INVALID. */
+int orc_assemble_stiffness(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                           const double *E, const double *nu, int jmode,
+                           const int64_t *row_ptr, const int32_t *col_idx, double *val);
+
+/* A10: setValueFEMSolver matrix branch for one element (FEMSolverClass.f90:1020-1057). */
+int orc_set_element_matrix(int64_t ne, int nne, const int32_t *elemnod, int64_t elem /*1-based*/,
+                           int dof, const double *ke /* (nne*dof)^2 col-major */,
+                           const int64_t *row_ptr, const int32_t *col_idx, double *val);
+
+/* A11: TractionVector_by_elemFEMDomain summed over all elements (FEMDomainClass.f90:17210-17292,
+ * facets :8192-8204).  Facets whose 4 nodes lie in the inclusive box get p*area*n/4 per node. */
+void orc_traction_box(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                      const double *box /* xmin,xmax,ymin,ymax,zmin,zmax */,
+                      double pressure, double *rhs /* 3*nn, accumulated */);
+
+/* A12: selectFEMDomain box form (FEMDomainClass.f90:12854-12926); returns count, fills ids (1-based)
+ * when ids != NULL. */
+int64_t orc_select_box(int64_t nn, const double *nodcoord, const double *box, int32_t *ids);
+
+/* A14 steps 1-4: Dirichlet elimination of solveFEMSolver (FEMSolverClass.f90:1707-1756). */
+void orc_apply_dirichlet(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
+                         double *rhs, const uint8_t *fixed, const double *ubar);
+
+/* A16: sub_crs_matvec_generic (LinearSolverClass.f90:3728-3776): y = A x with an OpenMP
+ * row loop and an atomic add per nonzero, as in the reference.  nthreads<=0: OpenMP default. */
+void orc_spmv(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+              const double *x, double *y, int nthreads);
+
+/* A15: bicgstab_CRS_2 (FEMSolverClass.f90:2199-2298).  x holds x0 on entry.  Returns a status
+ * code; *iters = iterations run, *rr_out = final r.r, *rr1_out = r.r after iteration 1.
+ * hist (may be NULL, length >= itrmax) receives sqrt(rr) per iteration. */
+int orc_bicgstab_crs2(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                      const double *b, double *x, int itrmax, double er0, double relative_er,
+                      int nthreads, int *iters, double *rr_out, double *rr1_out, double *hist);
+
+/* Whole solveFEMSolver (FEMSolverClass.f90:1697-1825) on a copy: elimination, BiCGSTAB, x_fixed = ubar. */
+int orc_solve(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, double *rhs,
+              const uint8_t *fixed, const double *ubar, double *x, int itrmax, double er0,
+              double relative_er, int nthreads, int *iters, double *rr_out, double *rr1_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,477 @@
+// assemble.hip -- hex8 element stiffness and its CRS assembly on the device.
+//
+// Element matrix (StiffnessMatrixFEMDomain, FEMDomainClass.f90:11457-11515):
+//   Ke = sum_{gp=1..8} (B^T D) B * detJ,  2x2x2 Gauss points in the reference's table order
+//   (ShapeFunctionClass.f90:503-529), unit weights, J = dN/dxi * X (:1476),
+//   B from G = J^-T dN/dxi (reference, FEMDomainClass.f90:12289) or J^-1 dN/dxi (textbook).
+// The kernels evaluate one 24-wide ROW of Ke per thread and keep the reference's
+// operation order: (B^T D)(r,k) and ((B^T D) B)(r,c) are Fortran matmul sums in ascending
+// k whose only nonzero terms are written out below in that order, accumulated over Gauss
+// points as K = K + (..)*detJ.  This file is compiled with -ffp-contract=off, so every
+// rounding matches a plain IEEE evaluation of the reference formulas.
+//
+// Assembly (StiffnessMatrix_as_CRS_FEMDomain, FEMDomainClass.f90:11193-11229) is a
+// deterministic gather instead of the reference's OpenMP reduction: one thread owns one
+// CRS row, walks the row node's incident elements in ascending id and adds their Ke rows
+// into its own row, so every entry is summed in ascending element order with no atomics.
+#include "pf_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr double GPV = 0.57735026918962576;
+// local node sign pattern of the hex8 (ShapeFunctionClass.f90:781-788)
+__device__ __constant__ double SX[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+__device__ __constant__ double SY[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+__device__ __constant__ double SZ[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+
+// dN/dxi of node a at (x,y,z), ShapeFunctionClass.f90:1094-1119, evaluated as
+// s*(0.125*(1 + t*u))*(1 + w*v) with the same grouping as the reference literals
+__device__ inline void dN_node(int a, double x, double y, double z, double &d0, double &d1, double &d2) {
+  const double sx = SX[a], sy = SY[a], sz = SZ[a];
+  const double ty = 1.0 + sy * y, tz = 1.0 + sz * z, tx = 1.0 + sx * x;
+  // +-(0.125*(1+-y))*(1+-z): the sign is applied to an exact product, so it is exact
+  d0 = sx * ((0.125 * ty) * tz);
+  d1 = sy * ((0.125 * tx) * tz);
+  d2 = sz * ((0.125 * tx) * ty);
+}
+
+// det_mat_real64 cofactor expansion along column 1 (MathClass.f90:1460-1476)
+__device__ inline double det3(const double J[3][3]) {
+  double m1 = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+  double m2 = J[0][1] * J[2][2] - J[2][1] * J[0][2];
+  double m3 = J[0][1] * J[1][2] - J[1][1] * J[0][2];
+  double r = 0.0;
+  r = r + J[0][0] * m1;
+  r = r + (-J[1][0]) * m2;
+  r = r + J[2][0] * m3;
+  return r;
+}
+
+// inverse_rank_2 (MathClass.f90:1567-1598)
+__device__ inline void inv3(const double A[3][3], double Ai[3][3]) {
+  const double d1 = 1.0 / det3(A);
+  Ai[0][0] = d1 * (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
+  Ai[0][1] = d1 * (A[0][2] * A[2][1] - A[0][1] * A[2][2]);
+  Ai[0][2] = d1 * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+  Ai[1][0] = d1 * (A[1][2] * A[2][0] - A[1][0] * A[2][2]);
+  Ai[1][1] = d1 * (A[0][0] * A[2][2] - A[0][2] * A[2][0]);
+  Ai[1][2] = d1 * (A[0][2] * A[1][0] - A[0][0] * A[1][2]);
+  Ai[2][0] = d1 * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  Ai[2][1] = d1 * (A[0][1] * A[2][0] - A[0][0] * A[2][1]);
+  Ai[2][2] = d1 * (A[0][0] * A[1][1] - A[0][1] * A[1][0]);
+}
+
+// Row r = 3a+d of Ke into acc[24] (acc[3b+d'] = Ke(3a+d, 3b+d')).
+// D (FEMDomainClass.f90:11607-11636): mu = E/2/(1+v), lam = v*E/(1+v)/(1-2v).
+__device__ inline void hex8_ke_row(const double (&X)[8][3], double E, double nu, int jmode, int a,
+                                   int d, double (&acc)[24]) {
+  const double mu = E / 2.0 / (1.0 + nu);
+  const double lam = nu * E / (1.0 + nu) / (1.0 - 2.0 * nu);
+  const double dg = 2.0 * mu + lam;  // D(1,1) = D(2,2) = D(3,3)
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    const double gx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? GPV : -GPV;
+    const double gy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? GPV : -GPV;
+    const double gz = (gp >= 4) ? GPV : -GPV;
+    double dN[3][8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) dN_node(b, gx, gy, gz, dN[0][b], dN[1][b], dN[2][b]);
+    double J[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) s = s + dN[i][b] * X[b][j];
+        J[i][j] = s;
+      }
+    const double detJ = det3(J);
+    double Ji[3][3];
+    inv3(J, Ji);
+    double G[3][8];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        double s = 0.0;
+        if (jmode == 0) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) s = s + dN[i][b] * Ji[i][p];
+        } else {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) s = s + Ji[p][i] * dN[i][b];
+        }
+        G[p][b] = s;
+      }
+    double ga0 = 0.0, ga1 = 0.0, ga2 = 0.0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      if (b == a) {
+        ga0 = G[0][b];
+        ga1 = G[1][b];
+        ga2 = G[2][b];
+      }
+    // (B^T D)(r, 0..5) for r = 3a+d: single nonzero product each
+    const double gd = (d == 0) ? ga0 : ((d == 1) ? ga1 : ga2);
+    const double T0 = gd * ((d == 0) ? dg : lam);
+    const double T1 = gd * ((d == 1) ? dg : lam);
+    const double T2 = gd * ((d == 2) ? dg : lam);
+    const double T3 = (d == 0) ? ga1 * mu : ((d == 1) ? ga0 * mu : 0.0);
+    const double T4 = (d == 0) ? 0.0 : ((d == 1) ? ga2 * mu : ga1 * mu);
+    const double T5 = (d == 0) ? ga2 * mu : ((d == 1) ? 0.0 : ga0 * mu);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      // ((B^T D) B)(r, 3b+d'), nonzero k in ascending order
+      const double k0 = (T0 * G[0][b] + T3 * G[1][b]) + T5 * G[2][b];
+      const double k1 = (T1 * G[1][b] + T3 * G[0][b]) + T4 * G[2][b];
+      const double k2 = (T2 * G[2][b] + T4 * G[1][b]) + T5 * G[0][b];
+      if (gp == 0) {
+        acc[3 * b + 0] = k0 * detJ;
+        acc[3 * b + 1] = k1 * detJ;
+        acc[3 * b + 2] = k2 * detJ;
+      } else {
+        acc[3 * b + 0] = acc[3 * b + 0] + k0 * detJ;
+        acc[3 * b + 1] = acc[3 * b + 1] + k1 * detJ;
+        acc[3 * b + 2] = acc[3 * b + 2] + k2 * detJ;
+      }
+    }
+  }
+}
+
+__device__ inline void load_element(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                                    const int32_t *__restrict__ en, int64_t e, int32_t (&nodes)[8],
+                                    double (&X)[8][3]) {
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    nodes[b] = en[(int64_t)b * ne + e];
+    const int64_t k = nodes[b] - 1;
+    X[b][0] = xyz[k];
+    X[b][1] = xyz[nn + k];
+    X[b][2] = xyz[2 * nn + k];
+  }
+}
+
+// one thread per CRS row r = 3i + d of a hex8 / DOF=3 mesh
+__global__ void __launch_bounds__(128) k_assemble_rows(
+    int64_t nn, int64_t ne, const double *__restrict__ xyz, const int32_t *__restrict__ en,
+    const double *__restrict__ E, const double *__restrict__ nu, int jmode,
+    const int64_t *__restrict__ eoff, const int32_t *__restrict__ elist,
+    const int64_t *__restrict__ noff, const int32_t *__restrict__ ncol,
+    const int64_t *__restrict__ row_ptr, double *__restrict__ val) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= 3 * nn) return;
+  const int64_t i = r / 3;
+  const int d = (int)(r - 3 * i);
+  const int64_t base = row_ptr[r] - 1;
+  const int64_t len = row_ptr[r + 1] - 1 - base;
+  for (int64_t k = 0; k < len; ++k) val[base + k] = 0.0;
+  const int64_t c0 = noff[i];
+  const int32_t deg = (int32_t)(noff[i + 1] - c0);
+  const int64_t s0 = eoff[i], s1 = eoff[i + 1];
+  for (int64_t s = s0; s < s1; ++s) {
+    const int64_t e = elist[s];
+    if (s > s0 && elist[s - 1] == e) continue;  // node listed twice in one element
+    int32_t nodes[8];
+    double X[8][3];
+    load_element(nn, ne, xyz, en, e, nodes, X);
+    const double Ee = E[e], ve = nu[e];
+    for (int a = 0; a < 8; ++a) {
+      if (nodes[a] != (int32_t)(i + 1)) continue;
+      double acc[24];
+      hex8_ke_row(X, Ee, ve, jmode, a, d, acc);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        int32_t p = 0;
+        const int32_t target = nodes[b] - 1;
+        for (int32_t q = 0; q < deg; ++q)
+          if (ncol[c0 + q] == target) {
+            p = q;
+            break;
+          }
+        double *dst = val + base + 3 * (int64_t)p;
+        dst[0] = dst[0] + acc[3 * b + 0];
+        dst[1] = dst[1] + acc[3 * b + 1];
+        dst[2] = dst[2] + acc[3 * b + 2];
+      }
+    }
+  }
+}
+
+// dense Ke of elements first..first+count-1: one thread per (element, row)
+__global__ void __launch_bounds__(128) k_ke_batch(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                                                  const int32_t *__restrict__ en,
+                                                  const double *__restrict__ E,
+                                                  const double *__restrict__ nu, int jmode,
+                                                  int64_t first, int64_t count,
+                                                  double *__restrict__ ke) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= count * 24) return;
+  const int64_t q = t / 24;
+  const int r = (int)(t - q * 24);
+  const int64_t e = first + q;
+  int32_t nodes[8];
+  double X[8][3];
+  load_element(nn, ne, xyz, en, e, nodes, X);
+  double acc[24];
+  hex8_ke_row(X, E[e], nu[e], jmode, r / 3, r % 3, acc);
+  double *out = ke + q * 576;
+#pragma unroll
+  for (int c = 0; c < 24; ++c) out[c * 24 + r] = acc[c];
+}
+
+// ---- batched setMatrix / setVector: node -> (call index, local node) lists ---------
+__global__ void k_batch_count(int64_t count, int nne, int64_t ne, const int32_t *__restrict__ en,
+                              const int32_t *__restrict__ ids, int32_t *__restrict__ cnt) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= count * nne) return;
+  const int64_t q = t / nne;
+  const int a = (int)(t - q * nne);
+  atomicAdd(&cnt[en[(int64_t)a * ne + (ids[q] - 1)] - 1], 1);
+}
+
+__global__ void k_batch_fill(int64_t count, int nne, int64_t ne, const int32_t *__restrict__ en,
+                             const int32_t *__restrict__ ids, const int64_t *__restrict__ off,
+                             int32_t *__restrict__ fill, int64_t *__restrict__ list) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= count * nne) return;
+  const int64_t q = t / nne;
+  const int a = (int)(t - q * nne);
+  const int32_t node = en[(int64_t)a * ne + (ids[q] - 1)] - 1;
+  const int32_t slot = atomicAdd(&fill[node], 1);
+  list[off[node] + slot] = t;  // t = q*nne + a sorts by call order, then local node
+}
+
+__global__ void k_batch_sort(int64_t nn, const int64_t *__restrict__ off, int64_t *__restrict__ list) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int64_t s0 = off[i], s1 = off[i + 1];
+  for (int64_t a = s0 + 1; a < s1; ++a) {
+    const int64_t v = list[a];
+    int64_t b = a - 1;
+    while (b >= s0 && list[b] > v) {
+      list[b + 1] = list[b];
+      --b;
+    }
+    list[b + 1] = v;
+  }
+}
+
+// setValueFEMSolver matrix branch (FEMSolverClass.f90:1020-1057) for the rows of one
+// node: linear search of the row for every column, add in call order.
+__global__ void k_batch_add_mats(int64_t nn, int nne, int dof, int64_t ne,
+                                 const int32_t *__restrict__ en, const int32_t *__restrict__ ids,
+                                 const int64_t *__restrict__ off, const int64_t *__restrict__ list,
+                                 const double *__restrict__ ke, const int64_t *__restrict__ row_ptr,
+                                 const int32_t *__restrict__ col_idx, double *__restrict__ val,
+                                 int32_t *__restrict__ err) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= (int64_t)dof * nn) return;
+  const int64_t i = r / dof;
+  const int j = (int)(r - i * dof);
+  const int nk = nne * dof;
+  const int64_t rb = row_ptr[r] - 1, re = row_ptr[r + 1] - 1;
+  for (int64_t s = off[i]; s < off[i + 1]; ++s) {
+    const int64_t q = list[s] / nne;
+    const int a = (int)(list[s] - q * nne);
+    const int64_t e = ids[q] - 1;
+    const double *K = ke + q * (int64_t)nk * nk;
+    for (int b = 0; b < nne; ++b)
+      for (int jj = 0; jj < dof; ++jj) {
+        const int32_t col = (int32_t)((int64_t)dof * (en[(int64_t)b * ne + e] - 1) + jj + 1);
+        int64_t id = -1;
+        for (int64_t k = rb; k < re; ++k)
+          if (col_idx[k] == col) {
+            id = k;
+            break;
+          }
+        if (id < 0) {
+          atomicOr(err, 1);
+          continue;
+        }
+        val[id] = val[id] + K[(int64_t)(dof * b + jj) * nk + (dof * a + j)];
+      }
+  }
+}
+
+__global__ void k_batch_add_vecs(int64_t nn, int nne, int dof, const int64_t *__restrict__ off,
+                                 const int64_t *__restrict__ list, const double *__restrict__ vec,
+                                 double *__restrict__ rhs) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= (int64_t)dof * nn) return;
+  const int64_t i = r / dof;
+  const int j = (int)(r - i * dof);
+  double acc = rhs[r];
+  for (int64_t s = off[i]; s < off[i + 1]; ++s) {
+    const int64_t q = list[s] / nne;
+    const int a = (int)(list[s] - q * nne);
+    acc = acc + vec[q * (int64_t)nne * dof + dof * a + j];
+  }
+  rhs[r] = acc;
+}
+
+// ---- traction (TractionVector_by_elemFEMDomain, FEMDomainClass.f90:17210-17292) ----
+__device__ __constant__ int FACET[6][4] = {{4, 3, 2, 1}, {1, 2, 6, 5}, {2, 3, 7, 6},
+                                           {3, 4, 8, 7}, {4, 1, 5, 8}, {5, 6, 7, 8}};
+
+__device__ inline double dot3(const double *a, const double *b) {
+  double s = 0.0;
+  for (int i = 0; i < 3; ++i) s = s + a[i] * b[i];
+  return s;
+}
+
+__device__ inline double tri_area(const double *A, const double *B, const double *C) {
+  double ab[3], ac[3];
+  for (int i = 0; i < 3; ++i) {
+    ab[i] = B[i] - A[i];
+    ac[i] = C[i] - A[i];
+  }
+  return 0.5 * sqrt(dot3(ab, ab) * dot3(ac, ac) - dot3(ab, ac) * dot3(ab, ac));
+}
+
+// one thread per node: sum, in ascending element then facet order, the quarter facet
+// forces of every facet (fully inside the open box) that holds this node
+__global__ void k_traction(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                           const int32_t *__restrict__ en, const int64_t *__restrict__ eoff,
+                           const int32_t *__restrict__ elist, double bx0, double bx1, double by0,
+                           double by1, double bz0, double bz1, double pressure,
+                           double *__restrict__ rhs) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const double box[6] = {bx0, bx1, by0, by1, bz0, bz1};
+  double f0 = rhs[3 * i], f1 = rhs[3 * i + 1], f2 = rhs[3 * i + 2];
+  const int64_t s0 = eoff[i], s1 = eoff[i + 1];
+  for (int64_t s = s0; s < s1; ++s) {
+    const int64_t e = elist[s];
+    if (s > s0 && elist[s - 1] == e) continue;
+    int32_t nodes[8];
+    double P[8][3];
+    load_element(nn, ne, xyz, en, e, nodes, P);
+    int in[8];
+    for (int a = 0; a < 8; ++a) {
+      in[a] = 1;
+      for (int dd = 0; dd < 3; ++dd)
+        if (box[2 * dd] >= P[a][dd] || box[2 * dd + 1] <= P[a][dd]) in[a] = 0;
+    }
+    for (int f = 0; f < 6; ++f) {
+      int cnt = 0;
+      for (int j = 0; j < 4; ++j) cnt += in[FACET[f][j] - 1];
+      if (cnt != 4) continue;
+      const double *A = P[FACET[f][0] - 1], *B = P[FACET[f][1] - 1];
+      const double *C = P[FACET[f][2] - 1], *Dq = P[FACET[f][3] - 1];
+      const double area = tri_area(A, B, C) + tri_area(B, C, Dq);
+      double u[3], v[3], n[3];
+      for (int k = 0; k < 3; ++k) {
+        u[k] = B[k] - A[k];
+        v[k] = C[k] - A[k];
+      }
+      n[0] = u[1] * v[2] - u[2] * v[1];
+      n[1] = u[2] * v[0] - u[0] * v[2];
+      n[2] = u[0] * v[1] - u[1] * v[0];
+      const double nrm = sqrt(dot3(n, n));
+      const double fx = pressure * area * (n[0] / nrm);
+      const double fy = pressure * area * (n[1] / nrm);
+      const double fz = pressure * area * (n[2] / nrm);
+      for (int j = 0; j < 4; ++j)
+        if (nodes[FACET[f][j] - 1] == (int32_t)(i + 1)) {
+          f0 = f0 + 0.25 * fx;
+          f1 = f1 + 0.25 * fy;
+          f2 = f2 + 0.25 * fz;
+        }
+    }
+  }
+  rhs[3 * i] = f0;
+  rhs[3 * i + 1] = f1;
+  rhs[3 * i + 2] = f2;
+}
+
+}  // namespace
+
+int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
+                     const int64_t *row_ptr, double *val) {
+  if (g.nne != 8 || !g.cols_built || !g.has_coords) {
+    pf_set_error("hex8 assembly needs an 8-node mesh with coordinates and its graph built");
+    return PF_INVALID;
+  }
+  if (g.nn == 0) return PF_OK;
+  k_assemble_rows<<<pf_blocks(3 * g.nn, 128), 128, 0, ctx->stream>>>(
+      g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p,
+      row_ptr, val);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_hex8_ke_batch(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
+                     int64_t first, int64_t count, double *ke) {
+  if (g.nne != 8 || !g.has_coords) {
+    pf_set_error("hex8 element stiffness needs nne = 8 and coordinates");
+    return PF_INVALID;
+  }
+  if (count <= 0) return PF_OK;
+  k_ke_batch<<<pf_blocks(count * 24, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p,
+                                                                  E, nu, jmode, first, count, ke);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+static int batch_lists(pf_ctx *ctx, const NodeGraph &g, int64_t count, const int32_t *ids,
+                       DevBuf<int64_t> &off, DevBuf<int64_t> &list) {
+  DevBuf<int32_t> cnt;
+  DevBuf<int64_t> tmp;
+  const int64_t total = count * g.nne;
+  PF_CHECK(cnt.ensure((size_t)g.nn + 1));
+  PF_CHECK(off.ensure((size_t)g.nn + 1));
+  PF_CHECK(list.ensure((size_t)total + 1));
+  PF_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (g.nn + 1), ctx->stream));
+  k_batch_count<<<pf_blocks(total, 256), 256, 0, ctx->stream>>>(count, g.nne, g.ne, g.elemnod.p, ids, cnt.p);
+  PF_CHECK(pf_scan_i32_to_i64(ctx, cnt.p, g.nn, off.p, tmp));
+  PF_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (g.nn + 1), ctx->stream));
+  k_batch_fill<<<pf_blocks(total, 256), 256, 0, ctx->stream>>>(count, g.nne, g.ne, g.elemnod.p, ids,
+                                                               off.p, cnt.p, list.p);
+  k_batch_sort<<<pf_blocks(g.nn, 256), 256, 0, ctx->stream>>>(g.nn, off.p, list.p);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int pf_add_element_mats(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count, const int32_t *ids,
+                        const double *ke, const int64_t *row_ptr, const int32_t *col_idx, double *val,
+                        int32_t *err) {
+  if (count <= 0 || g.nn == 0) return PF_OK;
+  DevBuf<int64_t> off, list;
+  PF_CHECK(batch_lists(ctx, g, count, ids, off, list));
+  const int64_t rows = (int64_t)dof * g.nn;
+  k_batch_add_mats<<<pf_blocks(rows, 128), 128, 0, ctx->stream>>>(g.nn, g.nne, dof, g.ne, g.elemnod.p, ids,
+                                                                  off.p, list.p, ke, row_ptr,
+                                                                  col_idx, val, err);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int pf_add_element_vecs(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count, const int32_t *ids,
+                        const double *vec, double *rhs) {
+  if (count <= 0 || g.nn == 0) return PF_OK;
+  DevBuf<int64_t> off, list;
+  PF_CHECK(batch_lists(ctx, g, count, ids, off, list));
+  const int64_t rows = (int64_t)dof * g.nn;
+  k_batch_add_vecs<<<pf_blocks(rows, 256), 256, 0, ctx->stream>>>(g.nn, g.nne, dof, off.p, list.p, vec, rhs);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double pressure, double *rhs) {
+  if (g.nne != 8 || !g.built || !g.has_coords) {
+    pf_set_error("traction needs an 8-node mesh with coordinates and its node->element lists");
+    return PF_INVALID;
+  }
+  if (g.nn == 0) return PF_OK;
+  k_traction<<<pf_blocks(g.nn, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, g.eoff.p,
+                                                            g.elist.p, box[0], box[1], box[2], box[3],
+                                                            box[4], box[5], pressure, rhs);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}

@@ -1,0 +1,95 @@
+// bc.hip -- Dirichlet conditions of FEMSolver_ on the device CRS.
+//   fix()   : flags and prescribed values (FEMSolverClass.f90:1073-1136); with repeated ids
+//             the last one wins, as in the reference's sequential loop.
+//   solve() : steps 1-4 of the elimination (FEMSolverClass.f90:1707-1756) and the final
+//             x_fixed = ubar overwrite (:1814-1823).
+#include "pf_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+__global__ void k_fix_last(int64_t nids, const int32_t *__restrict__ ids, int64_t n,
+                           int32_t *__restrict__ last) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nids) return;
+  const int32_t id = ids[t];
+  if (id < 1 || (int64_t)id > n) return;
+  atomicMax(&last[id - 1], (int32_t)t);
+}
+
+__global__ void k_fix_write(int64_t nids, const int32_t *__restrict__ ids, int64_t n,
+                            const int32_t *__restrict__ last, const double *__restrict__ values,
+                            double fixvalue, uint8_t *__restrict__ fixed, double *__restrict__ ubar) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nids) return;
+  const int32_t id = ids[t];
+  if (id < 1 || (int64_t)id > n) return;
+  if (last[id - 1] != (int32_t)t) return;
+  fixed[id - 1] = 1;
+  ubar[id - 1] = values ? values[t] : fixvalue;
+}
+
+// one thread per row: (1) b_i -= A_ij*ubar_j over fixed columns in CRS order,
+// (2) b_i = ubar_i on fixed rows, (3) zero fixed rows and columns, (4) diagonal 1 on fixed rows
+__global__ void k_bc_rows(int64_t n, const int64_t *__restrict__ row_ptr,
+                          const int32_t *__restrict__ col_idx, double *__restrict__ val,
+                          double *__restrict__ rhs, const uint8_t *__restrict__ fixed,
+                          const double *__restrict__ ubar) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t k0 = row_ptr[i] - 1, k1 = row_ptr[i + 1] - 1;
+  const bool fr = fixed[i] != 0;
+  double b = rhs[i];
+  for (int64_t k = k0; k < k1; ++k) {
+    const int64_t c = col_idx[k] - 1;
+    if (fixed[c]) b = b - val[k] * ubar[c];
+  }
+  if (fr) b = ubar[i];
+  rhs[i] = b;
+  for (int64_t k = k0; k < k1; ++k) {
+    const int64_t c = col_idx[k] - 1;
+    if (fr || fixed[c]) val[k] = (fr && c == i) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void k_set_fixed_x(int64_t n, const uint8_t *__restrict__ fixed,
+                              const double *__restrict__ ubar, double *__restrict__ x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n && fixed[i]) x[i] = ubar[i];
+}
+
+}  // namespace
+
+int pf_bc_fix(pf_ctx *ctx, int64_t n, int64_t nids, const int32_t *ids, const double *values,
+              double fixvalue, uint8_t *fixed, double *ubar) {
+  if (nids <= 0) return PF_OK;
+  if (nids >= (int64_t)INT32_MAX) {
+    pf_set_error("too many fixed ids");
+    return PF_INVALID;
+  }
+  DevBuf<int32_t> last;
+  PF_CHECK(last.ensure((size_t)n + 1));
+  PF_HIP(hipMemsetAsync(last.p, 0xff, sizeof(int32_t) * (n + 1), ctx->stream));  // -1
+  k_fix_last<<<pf_blocks(nids, 256), 256, 0, ctx->stream>>>(nids, ids, n, last.p);
+  k_fix_write<<<pf_blocks(nids, 256), 256, 0, ctx->stream>>>(nids, ids, n, last.p, values, fixvalue,
+                                                             fixed, ubar);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int pf_bc_apply(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
+                double *rhs, const uint8_t *fixed, const double *ubar) {
+  if (n <= 0) return PF_OK;
+  k_bc_rows<<<pf_blocks(n, 256), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs, fixed, ubar);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_bc_set_fixed_x(pf_ctx *ctx, int64_t n, const uint8_t *fixed, const double *ubar, double *x) {
+  if (n <= 0) return PF_OK;
+  k_set_fixed_x<<<pf_blocks(n, 256), 256, 0, ctx->stream>>>(n, fixed, ubar, x);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}

@@ -1,0 +1,690 @@
+// capi.hip -- the extern "C" entry points declared in include/pfgpu.h.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pf_internal.hpp"
+
+static thread_local char g_err[1024] = "";
+
+void pf_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(const pf_ctx *ctx) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != ctx->device) (void)hipSetDevice(ctx->device);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+template <typename T>
+int upload(pf_ctx *ctx, DevBuf<T> &dst, const T *src, size_t count) {
+  PF_CHECK(dst.ensure(count));
+  if (count) PF_HIP(hipMemcpyAsync(dst.p, src, sizeof(T) * count, hipMemcpyHostToDevice, ctx->stream));
+  return PF_OK;
+}
+
+template <typename T>
+int download(pf_ctx *ctx, T *dst, const T *src, size_t count) {
+  if (count) PF_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int check_ctx(const pf_ctx *ctx) {
+  if (!ctx) {
+    pf_set_error("null pf_ctx");
+    return PF_INVALID;
+  }
+  return PF_OK;
+}
+
+// host-side validation of an uploaded CRS (1-based, nondecreasing row_ptr, columns in range)
+int validate_crs(int64_t n, int64_t nnz, const int64_t *row_ptr, const int32_t *col_idx) {
+  if (n < 0 || nnz < 0 || !row_ptr || (nnz > 0 && !col_idx)) {
+    pf_set_error("invalid CRS arguments");
+    return PF_INVALID;
+  }
+  if (row_ptr[0] != 1 || row_ptr[n] != nnz + 1) {
+    pf_set_error("row_ptr must start at 1 and end at nnz+1");
+    return PF_INVALID;
+  }
+  for (int64_t i = 0; i < n; ++i)
+    if (row_ptr[i + 1] < row_ptr[i]) {
+      pf_set_error("row_ptr decreases at row %lld", (long long)i + 1);
+      return PF_INVALID;
+    }
+  for (int64_t k = 0; k < nnz; ++k)
+    if (col_idx[k] < 1 || (int64_t)col_idx[k] > n) {
+      pf_set_error("col_idx(%lld) = %d outside 1..n", (long long)k + 1, col_idx[k]);
+      return PF_INVALID;
+    }
+  return PF_OK;
+}
+
+int ensure_work(pf_ctx *ctx, SolverWork **w) {
+  if (!*w) PF_CHECK(pf_solver_work_create(w));
+  (void)ctx;
+  return PF_OK;
+}
+
+int set_mesh(pf_ctx *ctx, NodeGraph &g, int64_t nn, int64_t ne, int32_t nne, const double *nodcoord,
+             const int32_t *elemnod) {
+  if (nn < 0 || ne < 0 || nne < 1 || (ne > 0 && !elemnod)) {
+    pf_set_error("invalid mesh arguments");
+    return PF_INVALID;
+  }
+  g.nn = nn;
+  g.ne = ne;
+  g.nne = nne;
+  g.built = g.cols_built = false;
+  g.has_coords = nodcoord != nullptr;
+  if (nodcoord) PF_CHECK(upload(ctx, g.nodcoord, nodcoord, (size_t)nn * 3));
+  PF_CHECK(upload(ctx, g.elemnod, elemnod, (size_t)ne * nne));
+  return PF_OK;
+}
+
+int64_t nnz_of(const NodeGraph &g, int dof) { return (int64_t)dof * dof * g.nblocks; }
+
+}  // namespace
+
+// ================================================================================== misc
+extern "C" const char *pf_last_error(void) { return g_err; }
+
+extern "C" int pf_version(void) { return 10000; }
+
+extern "C" void pf_solve_opts_default(pf_solve_opts *o) {
+  std::memset(o, 0, sizeof(*o));
+  o->itrmax = 100000;
+  o->precision = 64;
+  o->er0 = (double)1.0e-10f;  // dble(1.0e-10): REAL(4) literal (FEMSolverClass.f90:70-71)
+  o->relative_er = (double)1.0e-10f;
+  o->inner_tol = 1e-5;
+  o->target_relres = 1e-8;
+  o->max_outer = 50;
+  o->spmv_kind = 0;
+  o->check_every = 32;
+  o->use_graph = 1;
+  o->debug = 0;
+  o->stop_mode = 0;
+  o->kernel_timing = 0;
+  o->jacobi = 1;
+}
+
+// =============================================================================== context
+extern "C" int pf_ctx_create(int device, pf_ctx **out) {
+  if (!out) return PF_INVALID;
+  *out = nullptr;
+  int count = 0;
+  PF_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) {
+    pf_set_error("device %d not present (%d devices)", device, count);
+    return PF_INVALID;
+  }
+  pf_ctx *c = new pf_ctx();
+  c->device = device;
+  DeviceGuard guard(c);
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) {
+    pf_set_error("context setup failed: %s", hipGetErrorString(e));
+    delete c;
+    return PF_ERR_HIP;
+  }
+  *out = c;
+  return PF_OK;
+}
+
+extern "C" int pf_ctx_destroy(pf_ctx *ctx) {
+  if (!ctx) return PF_OK;
+  {
+    DeviceGuard guard(ctx);
+    (void)hipStreamSynchronize(ctx->stream);
+    pf_solver_work_destroy(ctx->work);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return PF_OK;
+}
+
+extern "C" int pf_ctx_synchronize(pf_ctx *ctx) {
+  PF_CHECK(check_ctx(ctx));
+  DeviceGuard guard(ctx);
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+extern "C" void *pf_ctx_stream(pf_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+// ====================================================================== stateless calls
+extern "C" int pf_crs_graph(pf_ctx *ctx, int64_t nn, int64_t ne, int32_t nne, const int32_t *elemnod, int32_t dof,
+                            int64_t *nnz, int64_t *row_ptr, int32_t *col_idx) {
+  PF_CHECK(check_ctx(ctx));
+  if (dof < 1 || !nnz) {
+    pf_set_error("pf_crs_graph: dof >= 1 and nnz required");
+    return PF_INVALID;
+  }
+  DeviceGuard guard(ctx);
+  NodeGraph g;
+  PF_CHECK(set_mesh(ctx, g, nn, ne, nne, nullptr, elemnod));
+  PF_CHECK(pf_graph_node_cols(ctx, g));
+  *nnz = nnz_of(g, dof);
+  if (!row_ptr && !col_idx) return PF_OK;
+  const int64_t n = (int64_t)dof * nn;
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> ci;
+  PF_CHECK(rp.ensure((size_t)n + 1));
+  PF_CHECK(ci.ensure((size_t)*nnz + 1));
+  PF_CHECK(pf_graph_expand(ctx, g, dof, rp.p, ci.p));
+  if (row_ptr) PF_CHECK(download(ctx, row_ptr, rp.p, (size_t)n + 1));
+  if (col_idx) PF_CHECK(download(ctx, col_idx, ci.p, (size_t)*nnz));
+  return PF_OK;
+}
+
+extern "C" int pf_hex8_stiffness(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                                 const double *E, const double *nu, int32_t jmode, int64_t first, int64_t count,
+                                 double *ke) {
+  PF_CHECK(check_ctx(ctx));
+  if (first < 1 || count < 0 || first - 1 + count > ne || !E || !nu || !ke || !nodcoord) {
+    pf_set_error("pf_hex8_stiffness: element range %lld..%lld outside 1..%lld", (long long)first,
+                 (long long)(first + count - 1), (long long)ne);
+    return PF_INVALID;
+  }
+  DeviceGuard guard(ctx);
+  NodeGraph g;
+  PF_CHECK(set_mesh(ctx, g, nn, ne, 8, nodcoord, elemnod));
+  PF_CHECK(pf_graph_check_mesh(ctx, g));
+  DevBuf<double> dE, dnu, dke;
+  PF_CHECK(upload(ctx, dE, E, (size_t)ne));
+  PF_CHECK(upload(ctx, dnu, nu, (size_t)ne));
+  PF_CHECK(dke.ensure((size_t)count * 576));
+  PF_CHECK(pf_hex8_ke_batch(ctx, g, dE.p, dnu.p, jmode, first - 1, count, dke.p));
+  return download(ctx, ke, dke.p, (size_t)count * 576);
+}
+
+extern "C" int pf_hex8_stiffness_crs(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord,
+                                     const int32_t *elemnod, const double *E, const double *nu, int32_t jmode,
+                                     const int64_t *row_ptr, const int32_t *col_idx, double *val) {
+  PF_CHECK(check_ctx(ctx));
+  if (!E || !nu || !row_ptr || !col_idx || !val || !nodcoord) {
+    pf_set_error("pf_hex8_stiffness_crs: null argument");
+    return PF_INVALID;
+  }
+  DeviceGuard guard(ctx);
+  NodeGraph g;
+  PF_CHECK(set_mesh(ctx, g, nn, ne, 8, nodcoord, elemnod));
+  PF_CHECK(pf_graph_node_cols(ctx, g));
+  const int64_t n = 3 * nn, nnz = nnz_of(g, 3);
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> ci;
+  DevBuf<double> dv, dE, dnu;
+  PF_CHECK(rp.ensure((size_t)n + 1));
+  PF_CHECK(ci.ensure((size_t)nnz + 1));
+  PF_CHECK(pf_graph_expand(ctx, g, 3, rp.p, ci.p));
+  // the given pattern must be this mesh's ZeroMatrix(DOF=3)
+  std::vector<int64_t> hrp((size_t)n + 1);
+  std::vector<int32_t> hci((size_t)nnz);
+  PF_CHECK(download(ctx, hrp.data(), rp.p, (size_t)n + 1));
+  PF_CHECK(download(ctx, hci.data(), ci.p, (size_t)nnz));
+  if (std::memcmp(hrp.data(), row_ptr, sizeof(int64_t) * (n + 1)) != 0 ||
+      std::memcmp(hci.data(), col_idx, sizeof(int32_t) * nnz) != 0) {
+    pf_set_error("pf_hex8_stiffness_crs: row_ptr/col_idx are not ZeroMatrix(DOF=3) of this mesh");
+    return PF_INVALID;
+  }
+  PF_CHECK(upload(ctx, dE, E, (size_t)ne));
+  PF_CHECK(upload(ctx, dnu, nu, (size_t)ne));
+  PF_CHECK(dv.ensure((size_t)nnz + 1));
+  PF_CHECK(pf_assemble_hex8(ctx, g, dE.p, dnu.p, jmode, rp.p, dv.p));
+  return download(ctx, val, dv.p, (size_t)nnz);
+}
+
+extern "C" int pf_spmv_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                           const double *x, double *y) {
+  PF_CHECK(check_ctx(ctx));
+  if (n < 0 || !row_ptr) return PF_INVALID;
+  const int64_t nnz = row_ptr[n] - 1;
+  PF_CHECK(validate_crs(n, nnz, row_ptr, col_idx));
+  DeviceGuard guard(ctx);
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> ci;
+  DevBuf<double> dv, dx, dy;
+  PF_CHECK(upload(ctx, rp, row_ptr, (size_t)n + 1));
+  PF_CHECK(upload(ctx, ci, col_idx, (size_t)nnz));
+  PF_CHECK(upload(ctx, dv, val, (size_t)nnz));
+  PF_CHECK(upload(ctx, dx, x, (size_t)n));
+  PF_CHECK(dy.ensure((size_t)n + 1));
+  PF_CHECK(pf_spmv_crs_launch<double>(ctx, n, rp.p, ci.p, dv.p, dx.p, dy.p, n ? (double)nnz / n : 1.0));
+  return download(ctx, y, dy.p, (size_t)n);
+}
+
+static void fill_stats(pf_stats *st, const SolveResult &r) {
+  if (!st) return;
+  std::memset(st, 0, sizeof(*st));
+  st->spmv_p_ms = r.t_ms[0];
+  st->spmv_e_ms = r.t_ms[1];
+  st->update_ms = r.t_ms[2];
+  st->pupdate_ms = r.t_ms[3];
+  st->timed_iterations = r.timed;
+  st->status = r.status;
+  st->iterations = r.iterations;
+  st->outer_iterations = r.outer;
+  st->spmv_kind_used = r.kind;
+  st->rr = r.rr;
+  st->rr1 = r.rr1;
+  st->true_relres = r.true_relres;
+  st->solve_ms = r.solve_ms;
+  st->setup_ms = r.setup_ms;
+}
+
+extern "C" int pf_bicgstab_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                               const double *val, const double *rhs, double *x, const pf_solve_opts *opts,
+                               pf_stats *stats) {
+  PF_CHECK(check_ctx(ctx));
+  if (n < 0 || !row_ptr || !rhs || !x) return PF_INVALID;
+  const int64_t nnz = row_ptr[n] - 1;
+  PF_CHECK(validate_crs(n, nnz, row_ptr, col_idx));
+  pf_solve_opts o;
+  if (opts) o = *opts; else pf_solve_opts_default(&o);
+  DeviceGuard guard(ctx);
+  PF_CHECK(ensure_work(ctx, &ctx->work));
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> ci;
+  DevBuf<double> dv, db, dx;
+  PF_CHECK(upload(ctx, rp, row_ptr, (size_t)n + 1));
+  PF_CHECK(upload(ctx, ci, col_idx, (size_t)nnz));
+  PF_CHECK(upload(ctx, dv, val, (size_t)nnz));
+  PF_CHECK(upload(ctx, db, rhs, (size_t)n));
+  PF_CHECK(upload(ctx, dx, x, (size_t)n));
+  SolveResult res;
+  PF_CHECK(pf_bicgstab_device(ctx, ctx->work, n, rp.p, ci.p, dv.p, nnz, db.p, dx.p, o, true, res));
+  fill_stats(stats, res);
+  PF_CHECK(download(ctx, x, dx.p, (size_t)n));
+  // graphs captured on these temporary buffers must not be replayed later
+  pf_solver_work_destroy(ctx->work);
+  ctx->work = nullptr;
+  return res.status;
+}
+
+extern "C" int pf_dev_spmv_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                               const double *val, const double *x, double *y) {
+  PF_CHECK(check_ctx(ctx));
+  if (n <= 0) return n == 0 ? PF_OK : PF_INVALID;
+  DeviceGuard guard(ctx);
+  int64_t end = 0;
+  PF_CHECK(pf_read_i64(ctx, row_ptr + n, &end));
+  PF_CHECK(pf_spmv_crs_launch<double>(ctx, n, row_ptr, col_idx, val, x, y, (double)(end - 1) / n));
+  return PF_OK;
+}
+
+extern "C" int pf_dev_spmv_crs_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                                   const float *val, const float *x, float *y) {
+  PF_CHECK(check_ctx(ctx));
+  if (n <= 0) return n == 0 ? PF_OK : PF_INVALID;
+  DeviceGuard guard(ctx);
+  int64_t end = 0;
+  PF_CHECK(pf_read_i64(ctx, row_ptr + n, &end));
+  PF_CHECK(pf_spmv_crs_launch<float>(ctx, n, row_ptr, col_idx, val, x, y, (double)(end - 1) / n));
+  return PF_OK;
+}
+
+// ============================================================================ pf_system
+struct pf_system {
+  pf_ctx *ctx = nullptr;
+  NodeGraph g;
+  bool has_mesh = false;
+  int dof = 0;
+  int64_t n = 0, nnz = 0;
+  bool graph_crs = false;  // CRS is ZeroMatrix(dof) of the mesh
+  DevBuf<int64_t> row_ptr;
+  DevBuf<int32_t> col_idx;
+  DevBuf<double> val, rhs, x, ubar;
+  DevBuf<uint8_t> fixed;
+  bool has_fix = false;
+  bool matrix_changed = true;
+  SolverWork *work = nullptr;
+  DevBuf<double> sx, sy;  // pf_system_spmv operands
+  DevBuf<double> E, nu;   // resident materials
+  bool has_mat = false;
+  ~pf_system() { pf_solver_work_destroy(work); }
+};
+
+namespace {
+int sys_check(pf_system *s, bool need_crs) {
+  if (!s || !s->ctx) {
+    pf_set_error("null pf_system");
+    return PF_INVALID;
+  }
+  if (need_crs && s->n == 0 && s->nnz == 0 && !s->row_ptr.p) {
+    pf_set_error("no CRS: call pf_system_zero_matrix or pf_system_set_crs first");
+    return PF_INVALID;
+  }
+  return PF_OK;
+}
+
+int sys_alloc_vectors(pf_system *s, bool zero) {
+  pf_ctx *ctx = s->ctx;
+  PF_CHECK(s->rhs.ensure((size_t)s->n + 1));
+  PF_CHECK(s->x.ensure((size_t)s->n + 1));
+  if (zero) {
+    PF_HIP(hipMemsetAsync(s->rhs.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
+    PF_HIP(hipMemsetAsync(s->x.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
+  }
+  s->has_fix = false;
+  return PF_OK;
+}
+}  // namespace
+
+extern "C" int pf_system_create(pf_ctx *ctx, pf_system **out) {
+  PF_CHECK(check_ctx(ctx));
+  if (!out) return PF_INVALID;
+  pf_system *s = new pf_system();
+  s->ctx = ctx;
+  *out = s;
+  return PF_OK;
+}
+
+extern "C" int pf_system_destroy(pf_system *s) {
+  if (!s) return PF_OK;
+  DeviceGuard guard(s->ctx);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return PF_OK;
+}
+
+extern "C" int pf_system_set_mesh(pf_system *s, int64_t nn, int64_t ne, int32_t nne, const double *nodcoord,
+                                  const int32_t *elemnod) {
+  PF_CHECK(sys_check(s, false));
+  DeviceGuard guard(s->ctx);
+  PF_CHECK(set_mesh(s->ctx, s->g, nn, ne, nne, nodcoord, elemnod));
+  PF_CHECK(pf_graph_check_mesh(s->ctx, s->g));
+  s->has_mesh = true;
+  s->has_mat = false;
+  s->graph_crs = false;
+  return PF_OK;
+}
+
+extern "C" int pf_system_zero_matrix(pf_system *s, int32_t dof, int64_t *nnz) {
+  PF_CHECK(sys_check(s, false));
+  if (!s->has_mesh || dof < 1) {
+    pf_set_error("pf_system_zero_matrix: mesh required and dof >= 1");
+    return PF_INVALID;
+  }
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_CHECK(pf_graph_node_cols(ctx, s->g));
+  s->dof = dof;
+  s->n = (int64_t)dof * s->g.nn;
+  s->nnz = nnz_of(s->g, dof);
+  if (s->nnz >= (int64_t)1 << 40) {
+    pf_set_error("CRS too large");
+    return PF_INVALID;
+  }
+  PF_CHECK(s->row_ptr.ensure((size_t)s->n + 1));
+  PF_CHECK(s->col_idx.ensure((size_t)s->nnz + 1));
+  PF_CHECK(s->val.ensure((size_t)s->nnz + 1));
+  PF_CHECK(pf_graph_expand(ctx, s->g, dof, s->row_ptr.p, s->col_idx.p));
+  PF_HIP(hipMemsetAsync(s->val.p, 0, sizeof(double) * (s->nnz + 1), ctx->stream));
+  PF_CHECK(sys_alloc_vectors(s, true));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  s->graph_crs = true;
+  s->matrix_changed = true;
+  if (nnz) *nnz = s->nnz;
+  return PF_OK;
+}
+
+extern "C" int pf_system_set_crs(pf_system *s, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                                 const int32_t *col_idx, const double *val) {
+  PF_CHECK(sys_check(s, false));
+  PF_CHECK(validate_crs(n, nnz, row_ptr, col_idx));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  const bool resize = (n != s->n);
+  s->n = n;
+  s->nnz = nnz;
+  s->dof = 0;
+  PF_CHECK(upload(ctx, s->row_ptr, row_ptr, (size_t)n + 1));
+  PF_CHECK(upload(ctx, s->col_idx, col_idx, (size_t)nnz));
+  if (val) {
+    PF_CHECK(upload(ctx, s->val, val, (size_t)nnz));
+  } else {
+    PF_CHECK(s->val.ensure((size_t)nnz + 1));
+    PF_HIP(hipMemsetAsync(s->val.p, 0, sizeof(double) * (nnz + 1), ctx->stream));
+  }
+  if (resize || !s->rhs.p) PF_CHECK(sys_alloc_vectors(s, true));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  s->graph_crs = false;
+  s->matrix_changed = true;
+  return PF_OK;
+}
+
+extern "C" int pf_system_sizes(pf_system *s, int64_t *n, int64_t *nnz, int64_t *nn, int64_t *ne) {
+  PF_CHECK(sys_check(s, false));
+  if (n) *n = s->n;
+  if (nnz) *nnz = s->nnz;
+  if (nn) *nn = s->g.nn;
+  if (ne) *ne = s->g.ne;
+  return PF_OK;
+}
+
+extern "C" int pf_system_get_crs(pf_system *s, int64_t *row_ptr, int32_t *col_idx, double *val) {
+  PF_CHECK(sys_check(s, true));
+  DeviceGuard guard(s->ctx);
+  if (row_ptr) PF_CHECK(download(s->ctx, row_ptr, s->row_ptr.p, (size_t)s->n + 1));
+  if (col_idx) PF_CHECK(download(s->ctx, col_idx, s->col_idx.p, (size_t)s->nnz));
+  if (val) PF_CHECK(download(s->ctx, val, s->val.p, (size_t)s->nnz));
+  return PF_OK;
+}
+
+extern "C" int pf_system_stiffness(pf_system *s, const double *E, const double *nu, int32_t jmode) {
+  PF_CHECK(sys_check(s, false));
+  if (!s->has_mesh || s->g.nne != 8 || (!E != !nu) || (!E && !s->has_mat)) {
+    pf_set_error("pf_system_stiffness: hex8 mesh and E, nu (or resident ones) required");
+    return PF_INVALID;
+  }
+  if (!(s->graph_crs && s->dof == 3)) PF_CHECK(pf_system_zero_matrix(s, 3, nullptr));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  if (E) {
+    PF_CHECK(upload(ctx, s->E, E, (size_t)s->g.ne));
+    PF_CHECK(upload(ctx, s->nu, nu, (size_t)s->g.ne));
+    s->has_mat = true;
+  }
+  PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  s->matrix_changed = true;
+  return PF_OK;
+}
+
+static int check_elem_ids(const pf_system *s, int64_t count, const int32_t *ids) {
+  for (int64_t q = 0; q < count; ++q)
+    if (ids[q] < 1 || (int64_t)ids[q] > s->g.ne) {
+      pf_set_error("element id %d outside 1..%lld", ids[q], (long long)s->g.ne);
+      return PF_INVALID;
+    }
+  return PF_OK;
+}
+
+extern "C" int pf_system_add_element_matrices(pf_system *s, int64_t count, const int32_t *elem_ids,
+                                              const double *ke) {
+  PF_CHECK(sys_check(s, true));
+  if (!s->has_mesh || s->dof < 1 || count < 0 || (count > 0 && (!elem_ids || !ke))) {
+    pf_set_error("pf_system_add_element_matrices: needs a mesh and a CRS from pf_system_zero_matrix");
+    return PF_INVALID;
+  }
+  if (count == 0) return PF_OK;
+  PF_CHECK(check_elem_ids(s, count, elem_ids));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  const int64_t nk = (int64_t)s->g.nne * s->dof;
+  DevBuf<int32_t> dids, err;
+  DevBuf<double> dke;
+  PF_CHECK(upload(ctx, dids, elem_ids, (size_t)count));
+  PF_CHECK(upload(ctx, dke, ke, (size_t)(count * nk * nk)));
+  PF_CHECK(err.ensure(1));
+  PF_HIP(hipMemsetAsync(err.p, 0, sizeof(int32_t), ctx->stream));
+  if (!s->g.built) PF_CHECK(pf_graph_node_elems(ctx, s->g));
+  PF_CHECK(pf_add_element_mats(ctx, s->g, s->dof, count, dids.p, dke.p, s->row_ptr.p, s->col_idx.p, s->val.p,
+                               err.p));
+  int32_t herr = 0;
+  PF_CHECK(download(ctx, &herr, err.p, 1));
+  s->matrix_changed = true;
+  if (herr) {
+    pf_set_error("setMatrix: fill-in request, no CRS slot for an element entry");
+    return PF_INVALID;
+  }
+  return PF_OK;
+}
+
+extern "C" int pf_system_set_rhs(pf_system *s, const double *rhs) {
+  PF_CHECK(sys_check(s, true));
+  DeviceGuard guard(s->ctx);
+  PF_CHECK(upload(s->ctx, s->rhs, rhs, (size_t)s->n));
+  PF_HIP(hipStreamSynchronize(s->ctx->stream));
+  return PF_OK;
+}
+
+extern "C" int pf_system_add_element_vectors(pf_system *s, int64_t count, const int32_t *elem_ids,
+                                             const double *vec) {
+  PF_CHECK(sys_check(s, true));
+  if (!s->has_mesh || s->dof < 1 || count < 0 || (count > 0 && (!elem_ids || !vec))) {
+    pf_set_error("pf_system_add_element_vectors: needs a mesh and a CRS from pf_system_zero_matrix");
+    return PF_INVALID;
+  }
+  if (count == 0) return PF_OK;
+  PF_CHECK(check_elem_ids(s, count, elem_ids));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  const int64_t nk = (int64_t)s->g.nne * s->dof;
+  DevBuf<int32_t> dids;
+  DevBuf<double> dvec;
+  PF_CHECK(upload(ctx, dids, elem_ids, (size_t)count));
+  PF_CHECK(upload(ctx, dvec, vec, (size_t)(count * nk)));
+  if (!s->g.built) PF_CHECK(pf_graph_node_elems(ctx, s->g));
+  return pf_add_element_vecs(ctx, s->g, s->dof, count, dids.p, dvec.p, s->rhs.p);
+}
+
+extern "C" int pf_system_get_rhs(pf_system *s, double *rhs) {
+  PF_CHECK(sys_check(s, true));
+  DeviceGuard guard(s->ctx);
+  return download(s->ctx, rhs, s->rhs.p, (size_t)s->n);
+}
+
+extern "C" int pf_system_add_traction(pf_system *s, const double *box, double pressure) {
+  PF_CHECK(sys_check(s, true));
+  if (!s->has_mesh || s->g.nne != 8 || s->dof != 3 || !box) {
+    pf_set_error("pf_system_add_traction: hex8 mesh with a DOF=3 CRS required");
+    return PF_INVALID;
+  }
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  if (!s->g.built) PF_CHECK(pf_graph_node_elems(ctx, s->g));
+  PF_CHECK(pf_traction_box(ctx, s->g, box, pressure, s->rhs.p));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+extern "C" int pf_system_fix(pf_system *s, int64_t nids, const int32_t *ids, const double *values,
+                             double fixvalue) {
+  PF_CHECK(sys_check(s, true));
+  if (nids < 0 || (nids > 0 && !ids)) return PF_INVALID;
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  if (!s->has_fix) {
+    PF_CHECK(s->fixed.ensure((size_t)s->n + 1));
+    PF_CHECK(s->ubar.ensure((size_t)s->n + 1));
+    PF_HIP(hipMemsetAsync(s->fixed.p, 0, s->n + 1, ctx->stream));
+    PF_HIP(hipMemsetAsync(s->ubar.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
+    s->has_fix = true;
+  }
+  if (nids == 0) return PF_OK;
+  DevBuf<int32_t> dids;
+  DevBuf<double> dvals;
+  PF_CHECK(upload(ctx, dids, ids, (size_t)nids));
+  if (values) PF_CHECK(upload(ctx, dvals, values, (size_t)nids));
+  return pf_bc_fix(ctx, s->n, nids, dids.p, values ? dvals.p : nullptr, fixvalue, s->fixed.p, s->ubar.p);
+}
+
+extern "C" int pf_system_solve(pf_system *s, const pf_solve_opts *opts, const double *x0, double *x_out,
+                               pf_stats *stats) {
+  PF_CHECK(sys_check(s, true));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  pf_solve_opts o;
+  if (opts) o = *opts; else pf_solve_opts_default(&o);
+  PF_CHECK(ensure_work(ctx, &s->work));
+  if (s->has_fix) {
+    PF_CHECK(pf_bc_apply(ctx, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->rhs.p, s->fixed.p, s->ubar.p));
+    s->matrix_changed = true;
+  }
+  if (x0) {
+    PF_CHECK(upload(ctx, s->x, x0, (size_t)s->n));
+  } else {
+    PF_HIP(hipMemsetAsync(s->x.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
+  }
+  SolveResult res;
+  PF_CHECK(pf_bicgstab_device(ctx, s->work, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->rhs.p, s->x.p,
+                              o, s->matrix_changed, res));
+  s->matrix_changed = false;
+  if (s->has_fix) PF_CHECK(pf_bc_set_fixed_x(ctx, s->n, s->fixed.p, s->ubar.p, s->x.p));
+  fill_stats(stats, res);
+  if (x_out) PF_CHECK(download(ctx, x_out, s->x.p, (size_t)s->n));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return res.status;
+}
+
+extern "C" int pf_system_get_x(pf_system *s, double *x) {
+  PF_CHECK(sys_check(s, true));
+  DeviceGuard guard(s->ctx);
+  return download(s->ctx, x, s->x.p, (size_t)s->n);
+}
+
+extern "C" int pf_system_view_get(pf_system *s, pf_system_view *v) {
+  PF_CHECK(sys_check(s, false));
+  if (!v) return PF_INVALID;
+  v->n = s->n;
+  v->nnz = s->nnz;
+  v->nn = s->g.nn;
+  v->ne = s->g.ne;
+  v->row_ptr = s->row_ptr.p;
+  v->col_idx = s->col_idx.p;
+  v->val = s->val.p;
+  v->rhs = s->rhs.p;
+  v->x = s->x.p;
+  v->nodcoord = s->g.nodcoord.p;
+  v->elemnod = s->g.elemnod.p;
+  return PF_OK;
+}
+
+extern "C" int pf_system_spmv(pf_system *s, int32_t kind, const double *x, double *y, int32_t repeat, double *ms) {
+  PF_CHECK(sys_check(s, true));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_CHECK(ensure_work(ctx, &s->work));
+  PF_CHECK(s->sy.ensure((size_t)s->n + 1));
+  if (!s->sx.p || x) {
+    PF_CHECK(s->sx.ensure((size_t)s->n + 1));
+    if (x) PF_CHECK(upload(ctx, s->sx, x, (size_t)s->n));
+    else PF_HIP(hipMemsetAsync(s->sx.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
+  }
+  PF_CHECK(pf_work_spmv(ctx, s->work, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->matrix_changed, kind,
+                        s->sx.p, s->sy.p, repeat, ms));
+  s->matrix_changed = false;
+  if (y) PF_CHECK(download(ctx, y, s->sy.p, (size_t)s->n));
+  return PF_OK;
+}

@@ -1,0 +1,184 @@
+// pf_internal.hpp -- shared declarations of the gfx950 kernels behind include/pfgpu.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/pfgpu.h"
+
+// ---------------------------------------------------------------------------------
+// error plumbing: every host wrapper returns a PF_* status and records a message
+void pf_set_error(const char *fmt, ...);
+
+#define PF_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      pf_set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return PF_ERR_HIP;                                                             \
+    }                                                                                \
+  } while (0)
+
+#define PF_CHECK(expr)         \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != PF_OK) return rc_; \
+  } while (0)
+
+struct SolverWork;
+
+struct pf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cus = 256;
+  SolverWork *work = nullptr;  // workspace of the stateless solver entry point
+};
+
+// ---------------------------------------------------------------------------------
+// device buffer: owns a hipMalloc'd array; grows, never shrinks
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int ensure(size_t count) {
+    if (count <= n && p) return PF_OK;
+    release();
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      pf_set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+      return PF_ERR_NOMEM;
+    }
+    n = count;
+    return PF_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+};
+
+// ---------------------------------------------------------------------------------
+// mesh + node-level graph (row structure shared by the DOF rows of one node)
+struct NodeGraph {
+  int64_t nn = 0, ne = 0;
+  int32_t nne = 0;
+  DevBuf<double> nodcoord;   // nn*3 column-major
+  DevBuf<int32_t> elemnod;   // ne*nne column-major, 1-based
+  // node -> incident elements, ascending element id
+  DevBuf<int64_t> eoff;      // nn+1
+  DevBuf<int32_t> elist;     // ne*nne (0-based element ids)
+  // node -> distinct neighbour nodes in first-appearance order
+  DevBuf<int32_t> deg;       // nn
+  DevBuf<int64_t> noff;      // nn+1
+  DevBuf<int32_t> ncol;      // noff[nn] (0-based node ids)
+  int64_t nblocks = 0;       // noff[nn]
+  int32_t maxdeg = 0;
+  bool has_coords = false;   // nodcoord uploaded
+  bool built = false;        // eoff/elist valid
+  bool cols_built = false;   // deg/noff/ncol valid
+};
+
+// blocked sliced-ELL copy of a DOF=3 CRS: 64 nodes per slice (one wavefront), slot t of
+// node l of slice s at block index soff[s] + t*64 + l; 3x3 values stored component-major
+// inside the slot (q = 3*row_dof + col_dof): val[(soff[s] + t*64)*9 + q*64 + l].
+template <typename V>
+struct Sell3 {
+  int64_t nn = 0, nslices = 0, nslots = 0;  // nslots = soff[nslices]
+  DevBuf<int64_t> soff;                    // nslices+1
+  DevBuf<int32_t> width;                   // nslices
+  DevBuf<int32_t> col;                     // nslots*64 padded node ids (-1 = pad)
+  DevBuf<V> val;                           // nslots*64*9
+};
+
+// ---------------------------------------------------------------------------------
+// host wrappers implemented in the .hip files (all enqueue on ctx->stream)
+
+// scan.hip: exclusive scan of n counts into out[0..n] (out[n] = total)
+int pf_scan_i32_to_i64(pf_ctx *ctx, const int32_t *in, int64_t n, int64_t *out, DevBuf<int64_t> &tmp);
+int pf_scan_i64(pf_ctx *ctx, const int64_t *in, int64_t n, int64_t *out, DevBuf<int64_t> &tmp);
+int pf_read_i64(pf_ctx *ctx, const int64_t *dev, int64_t *host);
+
+// graph.hip
+int pf_graph_node_elems(pf_ctx *ctx, NodeGraph &g);                 // eoff/elist (sorted)
+int pf_graph_node_cols(pf_ctx *ctx, NodeGraph &g);                  // deg/noff/ncol
+int pf_graph_expand(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t *row_ptr, int32_t *col_idx);
+int pf_graph_check_mesh(pf_ctx *ctx, const NodeGraph &g);           // node ids in range
+
+// assemble.hip
+int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
+                     const int64_t *row_ptr, double *val);
+int pf_hex8_ke_batch(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
+                     int64_t first, int64_t count, double *ke);
+// batched setMatrix / setVector: node gathers over a batch in call order
+int pf_add_element_mats(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count, const int32_t *ids,
+                        const double *ke, const int64_t *row_ptr, const int32_t *col_idx, double *val,
+                        int32_t *err);
+int pf_add_element_vecs(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count, const int32_t *ids,
+                        const double *vec, double *rhs);
+int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double pressure, double *rhs);
+
+// bc.hip
+int pf_bc_apply(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
+                double *rhs, const uint8_t *fixed, const double *ubar);
+int pf_bc_fix(pf_ctx *ctx, int64_t n, int64_t nids, const int32_t *ids, const double *values,
+              double fixvalue, uint8_t *fixed, double *ubar);
+int pf_bc_set_fixed_x(pf_ctx *ctx, int64_t n, const uint8_t *fixed, const double *ubar, double *x);
+
+// spmv.hip
+template <typename V>
+int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                       const V *val, const V *x, V *y, double avg_row);
+int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                             int64_t nn, int32_t *ok_dev, int *ok_host);
+// scale (may be NULL): values stored as s_i a_ij s_j
+template <typename V>
+int pf_sell3_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                   const double *val, Sell3<V> &s, const double *scale = nullptr);
+template <typename V>
+int pf_sell3_values(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const double *val, Sell3<V> &s,
+                    const double *scale = nullptr);
+int pf_jacobi_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                    double *scale);
+int pf_crs_to_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                  const double *scale, float *out);
+template <typename V>
+int pf_spmv_sell3_launch(pf_ctx *ctx, const Sell3<V> &s, const V *x, V *y);
+
+// bicgstab.hip
+struct SolveResult {
+  int status = 0, iterations = 0, outer = 0, kind = 0;
+  double rr = 0, rr1 = 0, true_relres = 0, solve_ms = 0, setup_ms = 0;
+  double t_ms[4] = {0, 0, 0, 0};  // kernel_timing averages: spmv_p, spmv_e, update, pupdate
+  int64_t timed = 0;
+};
+// SolverWork: device workspace, defined in bicgstab.hip
+int pf_solver_work_create(SolverWork **w);
+void pf_solver_work_destroy(SolverWork *w);
+// Solve A x = b on the device; A given as CRS (always) and optionally as blocked layout
+// built inside.  x holds x0 on entry.
+int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr,
+                       const int32_t *col_idx, const double *val, int64_t nnz, const double *b,
+                       double *x, const pf_solve_opts &o, bool matrix_changed, SolveResult &res);
+// y = A x on device vectors with layout kind (1 CRS FP64, 2 blocked FP64, 3 blocked FP32);
+// `repeat` timed launches, *ms = average device ms per launch
+int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                 const double *val, int64_t nnz, bool matrix_changed, int kind, const double *x, double *y,
+                 int repeat, double *ms);
+
+// common launch geometry
+static inline unsigned pf_blocks(int64_t work, int threads, int64_t cap = 1 << 20) {
+  int64_t b = (work + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}

@@ -1,0 +1,268 @@
+// spmv.hip -- y = A x kernels and the blocked (Sell3) layout built from a DOF=3 CRS.
+#include "spmv_dev.hpp"
+
+namespace {
+
+template <typename V, int SUB>
+__global__ void __launch_bounds__(256) k_spmv_crs(int64_t n, const int64_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ col_idx,
+                                                  const V *__restrict__ val, const V *__restrict__ x,
+                                                  V *__restrict__ y) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = g / SUB;
+  const int sub = (int)(g % SUB);
+  if (row >= n) return;  // whole subgroups leave together (SUB divides 64)
+  V s = pfdev::crs_row<V, V, SUB>(row_ptr, col_idx, val, row, sub, pfdev::PlainX<V>{x});
+  if (sub == 0) y[row] = s;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_spmv_sell3(int64_t nn, int64_t nslices,
+                                                    const int64_t *__restrict__ soff,
+                                                    const int32_t *__restrict__ width,
+                                                    const int32_t *__restrict__ col,
+                                                    const V *__restrict__ val, const V *__restrict__ x,
+                                                    V *__restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t s = i >> 6;
+  if (s >= nslices) return;
+  V y0, y1, y2;
+  pfdev::sell3_node<V, V>(soff, width, col, val, s, (int)(i & 63), pfdev::PlainX<V>{x}, y0, y1, y2);
+  if (i < nn) {
+    y[3 * i] = y0;
+    y[3 * i + 1] = y1;
+    y[3 * i + 2] = y2;
+  }
+}
+
+// rows 3i..3i+2 share one node-column list and each node column c appears as the aligned
+// triple 3c+1, 3c+2, 3c+3 (1-based) -- the layout ZeroMatrix(DOF=3) produces
+__global__ void k_block_check(int64_t nn, const int64_t *__restrict__ row_ptr,
+                              const int32_t *__restrict__ col_idx, int32_t *__restrict__ bad) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int64_t r0 = row_ptr[3 * i] - 1;
+  const int64_t L = row_ptr[3 * i + 1] - 1 - r0;
+  bool ok = (L % 3) == 0;
+  for (int d = 1; d < 3 && ok; ++d)
+    ok = (row_ptr[3 * i + d + 1] - row_ptr[3 * i + d]) == L;
+  const int64_t m = L / 3;
+  for (int d = 0; d < 3 && ok; ++d) {
+    const int64_t rb = row_ptr[3 * i + d] - 1;
+    for (int64_t t = 0; t < m && ok; ++t) {
+      const int32_t c0 = col_idx[rb + 3 * t];
+      ok = ((c0 - 1) % 3 == 0) && col_idx[rb + 3 * t + 1] == c0 + 1 && col_idx[rb + 3 * t + 2] == c0 + 2 &&
+           (d == 0 || c0 == col_idx[r0 + 3 * t]);
+    }
+  }
+  if (!ok) atomicOr(bad, 1);
+}
+
+__global__ void k_sell3_width(int64_t nn, int64_t nslices, const int64_t *__restrict__ row_ptr,
+                              int32_t *__restrict__ width) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= nslices) return;
+  int32_t w = 0;
+  for (int l = 0; l < 64; ++l) {
+    const int64_t i = s * 64 + l;
+    if (i >= nn) break;
+    const int32_t dg = (int32_t)((row_ptr[3 * i + 1] - row_ptr[3 * i]) / 3);
+    w = dg > w ? dg : w;
+  }
+  width[s] = w;
+}
+
+// one thread per cell (slot, lane): node column index, or -1 for padding
+__global__ void k_sell3_cols(int64_t nn, int64_t nslices, int64_t ncells, const int64_t *__restrict__ soff,
+                             const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
+                             int32_t *__restrict__ col) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ncells;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t slot = g >> 6;
+    const int lane = (int)(g & 63);
+    int64_t lo = 0, hi = nslices;  // largest s with soff[s] <= slot
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (soff[mid] <= slot) lo = mid; else hi = mid;
+    }
+    const int64_t i = lo * 64 + lane;
+    const int64_t t = slot - soff[lo];
+    int32_t c = -1;
+    if (i < nn) {
+      const int64_t rb = row_ptr[3 * i] - 1;
+      if (3 * t < row_ptr[3 * i + 1] - 1 - rb) c = (col_idx[rb + 3 * t] - 1) / 3;
+    }
+    col[g] = c;
+  }
+}
+
+template <typename V>
+__global__ void k_sell3_vals(int64_t nn, int64_t nslices, int64_t ncells, const int64_t *__restrict__ soff,
+                             const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                             const double *__restrict__ val, const double *__restrict__ scale,
+                             V *__restrict__ out) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ncells;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t slot = g >> 6;
+    const int lane = (int)(g & 63);
+    int64_t lo = 0, hi = nslices;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (soff[mid] <= slot) lo = mid; else hi = mid;
+    }
+    const int64_t i = lo * 64 + lane;
+    const int64_t t = slot - soff[lo];
+    V *o = out + slot * 64 * 9 + lane;
+    bool live = false;
+    if (i < nn) {
+      const int64_t len = row_ptr[3 * i + 1] - row_ptr[3 * i];
+      live = 3 * t < len;
+    }
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 3; ++k) {
+        double a = live ? val[row_ptr[3 * i + d] - 1 + 3 * t + k] : 0.0;
+        if (live && scale) a = (a * scale[3 * i + d]) * scale[3 * (int64_t)col[g] + k];
+        o[(3 * d + k) * 64] = (V)a;
+      }
+  }
+}
+
+// s_i = 1/sqrt(|a_ii|) (1 where the diagonal is missing or zero)
+__global__ void k_jacobi_scale(int64_t n, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
+                               const double *__restrict__ val, double *__restrict__ scale) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int64_t k = row_ptr[i] - 1; k < row_ptr[i + 1] - 1; ++k)
+    if (col_idx[k] == i + 1) {
+      d = val[k];
+      break;
+    }
+  scale[i] = d != 0.0 ? 1.0 / sqrt(fabs(d)) : 1.0;
+}
+
+// FP32 copy of CRS values, optionally scaled s_i a_ij s_j (one thread per row)
+__global__ void k_crs_f32(int64_t n, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
+                          const double *__restrict__ val, const double *__restrict__ scale, float *__restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int64_t k = row_ptr[i] - 1; k < row_ptr[i + 1] - 1; ++k) {
+    double a = val[k];
+    if (scale) a = (a * scale[i]) * scale[col_idx[k] - 1];
+    out[k] = (float)a;
+  }
+}
+
+}  // namespace
+
+template <typename V>
+int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                       const V *val, const V *x, V *y, double avg_row) {
+  if (n <= 0) return PF_OK;
+  // subgroup width ~ average row length, between 4 and 64 lanes
+  int sub = 4;
+  while (sub < 64 && sub * 2 <= avg_row) sub *= 2;
+  const int64_t threads = n * sub;
+  const unsigned blocks = pf_blocks(threads, 256, INT32_MAX);
+  switch (sub) {
+    case 4: k_spmv_crs<V, 4><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
+    case 8: k_spmv_crs<V, 8><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
+    case 16: k_spmv_crs<V, 16><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
+    case 32: k_spmv_crs<V, 32><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
+    default: k_spmv_crs<V, 64><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
+  }
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+template int pf_spmv_crs_launch<double>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const double *,
+                                        const double *, double *, double);
+template int pf_spmv_crs_launch<float>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const float *,
+                                       const float *, float *, double);
+
+int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                             int64_t nn, int32_t *ok_dev, int *ok_host) {
+  *ok_host = 0;
+  if (n % 3 != 0 || nn * 3 != n) return PF_OK;
+  if (nn == 0) {
+    *ok_host = 1;
+    return PF_OK;
+  }
+  PF_HIP(hipMemsetAsync(ok_dev, 0, sizeof(int32_t), ctx->stream));
+  k_block_check<<<pf_blocks(nn, 256), 256, 0, ctx->stream>>>(nn, row_ptr, col_idx, ok_dev);
+  int32_t bad = 1;
+  PF_HIP(hipMemcpyAsync(&bad, ok_dev, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  *ok_host = bad ? 0 : 1;
+  return PF_OK;
+}
+
+int pf_jacobi_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                    double *scale) {
+  if (n <= 0) return PF_OK;
+  k_jacobi_scale<<<pf_blocks(n, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, scale);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_crs_to_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                  const double *scale, float *out) {
+  if (n <= 0) return PF_OK;
+  k_crs_f32<<<pf_blocks(n, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, scale, out);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+template <typename V>
+int pf_sell3_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                   const double *val, Sell3<V> &s, const double *scale) {
+  const int64_t nn = n / 3;
+  s.nn = nn;
+  s.nslices = (nn + 63) / 64;
+  DevBuf<int64_t> tmp;
+  PF_CHECK(s.width.ensure((size_t)s.nslices + 1));
+  PF_CHECK(s.soff.ensure((size_t)s.nslices + 1));
+  if (s.nslices > 0)
+    k_sell3_width<<<pf_blocks(s.nslices, 256), 256, 0, ctx->stream>>>(nn, s.nslices, row_ptr, s.width.p);
+  PF_CHECK(pf_scan_i32_to_i64(ctx, s.width.p, s.nslices, s.soff.p, tmp));
+  PF_CHECK(pf_read_i64(ctx, s.soff.p + s.nslices, &s.nslots));
+  const int64_t ncells = s.nslots * 64;
+  PF_CHECK(s.col.ensure((size_t)ncells + 1));
+  PF_CHECK(s.val.ensure((size_t)ncells * 9 + 1));
+  if (ncells > 0)
+    k_sell3_cols<<<pf_blocks(ncells, 256, 65536), 256, 0, ctx->stream>>>(nn, s.nslices, ncells, s.soff.p,
+                                                                       row_ptr, col_idx, s.col.p);
+  PF_HIP(hipGetLastError());
+  return pf_sell3_values<V>(ctx, n, row_ptr, val, s, scale);
+}
+
+template <typename V>
+int pf_sell3_values(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const double *val, Sell3<V> &s,
+                    const double *scale) {
+  (void)n;
+  const int64_t ncells = s.nslots * 64;
+  if (ncells > 0)
+    k_sell3_vals<V><<<pf_blocks(ncells, 256, 65536), 256, 0, ctx->stream>>>(s.nn, s.nslices, ncells, s.soff.p,
+                                                                          row_ptr, s.col.p, val, scale, s.val.p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+template <typename V>
+int pf_spmv_sell3_launch(pf_ctx *ctx, const Sell3<V> &s, const V *x, V *y) {
+  if (s.nslices == 0) return PF_OK;
+  k_spmv_sell3<V><<<pf_blocks(s.nslices * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+      s.nn, s.nslices, s.soff.p, s.width.p, s.col.p, s.val.p, x, y);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+template int pf_sell3_build<double>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const double *,
+                                    Sell3<double> &, const double *);
+template int pf_sell3_build<float>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const double *,
+                                   Sell3<float> &, const double *);
+template int pf_sell3_values<double>(pf_ctx *, int64_t, const int64_t *, const double *, Sell3<double> &,
+                                     const double *);
+template int pf_sell3_values<float>(pf_ctx *, int64_t, const int64_t *, const double *, Sell3<float> &,
+                                    const double *);
+template int pf_spmv_sell3_launch<double>(pf_ctx *, const Sell3<double> &, const double *, double *);
+template int pf_spmv_sell3_launch<float>(pf_ctx *, const Sell3<float> &, const float *, float *);

@@ -1,0 +1,378 @@
+"""Python mirror of the reference's FEMDomain_/FEMSolver_ surface for the static
+linear-elastic path, dispatching every compute step to libpfgpu.so.
+
+Names, argument meanings and error behaviour follow plantFEM:
+  FEMDomain_%create("Cube3D", x_num, y_num, z_num, x_len, y_len, z_len)  FEMDomainClass.f90:7215-7310
+  FEMDomain_%ZeroMatrix(DOF)                                             FEMDomainClass.f90:10581-10840
+  FEMDomain_%StiffnessMatrix(ElementID, E, v)                            FEMDomainClass.f90:11457-11515
+  FEMDomain_%StiffnessMatrix(YoungModulus(:), PoissonRatio(:))           FEMDomainClass.f90:11123-11232
+  FEMDomain_%select(x_min, x_max, ...)                                   FEMDomainClass.f90:12854-12926
+  FEMSolver_%init/setDomain/setCRS/setRHS/setMatrix/setVector/fix/solve  FEMSolverClass.f90:84-163
+Indices are 1-based as in the reference (node ids, element ids, DOF ids).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ----------------------------------------------------------------------------- meshes
+def cube3d(x_num: int = 10, y_num: int = 10, z_num: int = 10, x_len: float = 1.0, y_len: float = 1.0,
+           z_len: float = 1.0):
+    """Cube3D numbering (MeshClass.f90:6348-6361, :6745-6808, :6992-7090).
+
+    Returns (NodCoord(nn,3) float64 Fortran-ordered, ElemNod(ne,8) int32 Fortran-ordered, 1-based).
+    Node id = i + (xn+1)(j-1) + (xn+1)(yn+1)(k-1), x fastest; coordinates lx/dble(xn)*dble(i-1).
+    """
+    xn, yn, zn = int(x_num), int(y_num), int(z_num)
+    i = np.arange(xn + 1, dtype=np.float64)
+    j = np.arange(yn + 1, dtype=np.float64)
+    k = np.arange(zn + 1, dtype=np.float64)
+    xs = (float(x_len) / float(xn)) * i
+    ys = (float(y_len) / float(yn)) * j
+    zs = (float(z_len) / float(zn)) * k
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    nod = np.empty((X.size, 3), dtype=np.float64, order="F")
+    nod[:, 0] = X.ravel()
+    nod[:, 1] = Y.ravel()
+    nod[:, 2] = Z.ravel()
+    nn2 = (xn + 1) * (yn + 1)
+    ii, jj = np.meshgrid(np.arange(1, xn + 1), np.arange(1, yn + 1), indexing="xy")  # x fastest
+    ii = ii.ravel()
+    jj = jj.ravel()
+    q = np.stack([ii + (jj - 1) * (xn + 1), ii + 1 + (jj - 1) * (xn + 1),
+                  xn + 2 + ii + (jj - 1) * (xn + 1), xn + 1 + ii + (jj - 1) * (xn + 1)], axis=1)
+    layers = np.arange(zn)[:, None, None] * nn2
+    bottom = (q[None, :, :] + layers).reshape(-1, 4)
+    top = bottom + nn2
+    elem = np.asfortranarray(np.concatenate([bottom, top], axis=1).astype(np.int32))
+    return nod, elem
+
+
+@dataclass
+class CRS:
+    """CRS_ (SparseClass.f90:128-131): 1-based int64 row_ptr, int32 col_idx, float64 val."""
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+    val: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.row_ptr.size - 1)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col_idx.size)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col_idx.astype(np.int64) - 1, self.row_ptr - 1),
+                             shape=(self.n, self.n))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class FEMDomain:
+    """Subset of FEMDomain_ used by the static solve path."""
+
+    def __init__(self, nodcoord=None, elemnod=None, ctx: L.Context | None = None):
+        self.ctx = ctx
+        self.nodcoord = None if nodcoord is None else np.asfortranarray(nodcoord, dtype=np.float64)
+        self.elemnod = None if elemnod is None else np.asfortranarray(elemnod, dtype=np.int32)
+
+    def _ctx(self):
+        return self.ctx or L.default_context()
+
+    def create(self, meshtype: str = "Cube3D", x_num=None, y_num=None, z_num=None, x_len=None, y_len=None,
+               z_len=None):
+        if meshtype not in ("Cube3D", "Cube", "cube3D"):
+            raise ValueError(f"meshtype {meshtype!r}: only Cube3D is on this path")
+        self.nodcoord, self.elemnod = cube3d(x_num or 10, y_num or 10, z_num or 10, x_len or 1.0, y_len or 1.0,
+                                             z_len or 1.0)
+        return self
+
+    # sizes (FEMDomainClass.f90:10117-10173)
+    def nn(self) -> int:
+        return int(self.nodcoord.shape[0])
+
+    def ne(self) -> int:
+        return int(self.elemnod.shape[0])
+
+    def nne(self) -> int:
+        return int(self.elemnod.shape[1])
+
+    def nd(self) -> int:
+        return int(self.nodcoord.shape[1])
+
+    def x_min(self): return float(self.nodcoord[:, 0].min())
+    def x_max(self): return float(self.nodcoord[:, 0].max())
+    def y_min(self): return float(self.nodcoord[:, 1].min())
+    def y_max(self): return float(self.nodcoord[:, 1].max())
+    def z_min(self): return float(self.nodcoord[:, 2].min())
+    def z_max(self): return float(self.nodcoord[:, 2].max())
+
+    def select(self, x_min=None, x_max=None, y_min=None, y_max=None, z_min=None, z_max=None) -> np.ndarray:
+        """Node ids inside the closed box (InOrOut, ArrayClass.f90:2458-2487); defaults +-1e14."""
+        lo = [-1.0e14 if v is None else v for v in (x_min, y_min, z_min)]
+        hi = [1.0e14 if v is None else v for v in (x_max, y_max, z_max)]
+        X = self.nodcoord
+        inside = np.ones(X.shape[0], dtype=bool)
+        for d in range(3):
+            inside &= (lo[d] <= X[:, d]) & (X[:, d] <= hi[d])
+        return (np.nonzero(inside)[0] + 1).astype(np.int32)
+
+    def ZeroMatrix(self, DOF: int = 3) -> CRS:
+        lib = L.load()
+        en = np.asfortranarray(self.elemnod, dtype=np.int32).ravel(order="F")
+        nnz = C.c_int64(0)
+        L.check(lib.pf_crs_graph(self._ctx().h, self.nn(), self.ne(), self.nne(), L.ptr(en, C.c_int32), DOF,
+                                 C.byref(nnz), None, None), "ZeroMatrix")
+        rp = np.empty(DOF * self.nn() + 1, dtype=np.int64)
+        ci = np.empty(nnz.value, dtype=np.int32)
+        L.check(lib.pf_crs_graph(self._ctx().h, self.nn(), self.ne(), self.nne(), L.ptr(en, C.c_int32), DOF,
+                                 C.byref(nnz), L.ptr(rp, C.c_int64), L.ptr(ci, C.c_int32)), "ZeroMatrix")
+        return CRS(rp, ci, np.zeros(nnz.value, dtype=np.float64))
+
+    def _flat(self):
+        return (np.asfortranarray(self.nodcoord).ravel(order="F"), np.asfortranarray(self.elemnod).ravel(order="F"))
+
+    def StiffnessMatrix(self, ElementID=None, E=None, v=None, YoungModulus=None, PoissonRatio=None, jmode: int = 0):
+        """Element form (ElementID, E, v) -> Ke(24,24); array form (YoungModulus(:), PoissonRatio(:)) -> CRS."""
+        lib = L.load()
+        xyz, en = self._flat()
+        if YoungModulus is not None:
+            Ea = _f64(YoungModulus)
+            va = _f64(PoissonRatio)
+            crs = self.ZeroMatrix(3)
+            L.check(lib.pf_hex8_stiffness_crs(self._ctx().h, self.nn(), self.ne(), L.ptr(xyz, C.c_double),
+                                              L.ptr(en, C.c_int32), L.ptr(Ea, C.c_double), L.ptr(va, C.c_double),
+                                              jmode, L.ptr(crs.row_ptr, C.c_int64), L.ptr(crs.col_idx, C.c_int32),
+                                              L.ptr(crs.val, C.c_double)), "StiffnessMatrix")
+            return crs
+        ids = np.atleast_1d(np.asarray(ElementID, dtype=np.int64))
+        Ea = np.zeros(self.ne())
+        va = np.zeros(self.ne())
+        Ea[ids - 1] = E
+        va[ids - 1] = v
+        out = np.empty((ids.size, 24, 24), dtype=np.float64)
+        # contiguous runs of element ids share one launch
+        for q, e in enumerate(ids):
+            ke = np.empty(576, dtype=np.float64)
+            L.check(lib.pf_hex8_stiffness(self._ctx().h, self.nn(), self.ne(), L.ptr(xyz, C.c_double),
+                                          L.ptr(en, C.c_int32), L.ptr(Ea, C.c_double), L.ptr(va, C.c_double), jmode,
+                                          int(e), 1, L.ptr(ke, C.c_double)), "StiffnessMatrix")
+            out[q] = ke.reshape(24, 24).T  # column-major -> Ke[r, c]
+        return out[0] if np.ndim(ElementID) == 0 else out
+
+    def StiffnessMatrices(self, E, v, first: int = 1, count=None, jmode: int = 0) -> np.ndarray:
+        """Batched element form: Ke of elements first..first+count-1 as (count,24,24) with Ke[q, r, c]."""
+        lib = L.load()
+        xyz, en = self._flat()
+        count = self.ne() - first + 1 if count is None else int(count)
+        ke = np.empty(count * 576, dtype=np.float64)
+        L.check(lib.pf_hex8_stiffness(self._ctx().h, self.nn(), self.ne(), L.ptr(xyz, C.c_double), L.ptr(en, C.c_int32),
+                                      L.ptr(_f64(E), C.c_double), L.ptr(_f64(v), C.c_double), jmode, int(first), count,
+                                      L.ptr(ke, C.c_double)), "StiffnessMatrices")
+        return ke.reshape(count, 24, 24).transpose(0, 2, 1)
+
+
+class FEMSolver:
+    """FEMSolver_ (FEMSolverClass.f90:13-164) for one domain, state resident on the GPU."""
+
+    def __init__(self, ctx: L.Context | None = None):
+        self.ctx = ctx or L.default_context()
+        lib = L.load()
+        h = C.c_void_p()
+        L.check(lib.pf_system_create(self.ctx.h, C.byref(h)), "pf_system_create")
+        self.h = h
+        self.itrmax = 100000
+        self.er0 = L.ER0_DEFAULT
+        self.relative_er = L.ER0_DEFAULT
+        self.debug = False
+        self.domain: FEMDomain | None = None
+        self.DOF = 0
+        self.n = 0
+        self.nnz = 0
+        self.last_stats: dict | None = None
+        self._pending_ids: list = []
+        self._pending_mats: list = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.load().pf_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # (1)-(2)
+    def init(self, NumDomain: int = 1):
+        if NumDomain != 1:
+            raise NotImplementedError("multi-domain assembly is outside the single-domain static path")
+        return self
+
+    def setDomain(self, FEMDomain: FEMDomain, DomainID: int = 1):
+        if DomainID != 1:
+            raise NotImplementedError("only DomainID=1")
+        self.domain = FEMDomain
+        xyz, en = FEMDomain._flat()
+        L.check(L.load().pf_system_set_mesh(self.h, FEMDomain.nn(), FEMDomain.ne(), FEMDomain.nne(),
+                                            L.ptr(xyz, C.c_double), L.ptr(en, C.c_int32)), "setDomain")
+        return self
+
+    # (3)
+    def setCRS(self, DOF: int | None = None, CRS: CRS | None = None):
+        lib = L.load()
+        if CRS is not None:
+            L.check(lib.pf_system_set_crs(self.h, CRS.n, CRS.nnz, L.ptr(CRS.row_ptr, C.c_int64),
+                                          L.ptr(CRS.col_idx, C.c_int32), L.ptr(_f64(CRS.val), C.c_double)), "setCRS")
+            self.n, self.nnz, self.DOF = CRS.n, CRS.nnz, 0
+            return self
+        nnz = C.c_int64(0)
+        L.check(lib.pf_system_zero_matrix(self.h, int(DOF), C.byref(nnz)), "setCRS")
+        self.DOF = int(DOF)
+        self.n = self.DOF * self.domain.nn()
+        self.nnz = nnz.value
+        return self
+
+    def setRHS(self, RHS):
+        self._flush()
+        L.check(L.load().pf_system_set_rhs(self.h, L.ptr(_f64(RHS), C.c_double)), "setRHS")
+
+    def getRHS(self) -> np.ndarray:
+        out = np.empty(self.n)
+        L.check(L.load().pf_system_get_rhs(self.h, L.ptr(out, C.c_double)), "getRHS")
+        return out
+
+    def getX(self) -> np.ndarray:
+        out = np.empty(self.n)
+        L.check(L.load().pf_system_get_x(self.h, L.ptr(out, C.c_double)), "getX")
+        return out
+
+    def spmv(self, kind: int, x=None, repeat: int = 1, download: bool = True):
+        """y = A x with kernel kind (1 CRS FP64, 2 blocked FP64, 3 blocked FP32); returns (y, ms/launch)."""
+        y = np.empty(self.n) if download else None
+        ms = C.c_double(0)
+        L.check(L.load().pf_system_spmv(self.h, int(kind), L.ptr(None if x is None else _f64(x), C.c_double),
+                                        L.ptr(y, C.c_double), int(repeat), C.byref(ms)), "spmv")
+        return y, ms.value
+
+    def getCRS(self) -> CRS:
+        self._flush()
+        rp = np.empty(self.n + 1, dtype=np.int64)
+        ci = np.empty(self.nnz, dtype=np.int32)
+        va = np.empty(self.nnz, dtype=np.float64)
+        L.check(L.load().pf_system_get_crs(self.h, L.ptr(rp, C.c_int64), L.ptr(ci, C.c_int32), L.ptr(va, C.c_double)),
+                "getCRS")
+        return CRS(rp, ci, va)
+
+    # (4) element contributions are buffered and scattered on the device in call order
+    def setMatrix(self, DomainID: int, ElementID: int, DOF: int, Matrix):
+        if self.n == 0:
+            self.setCRS(DOF=DOF)
+        self._pending_ids.append(int(ElementID))
+        self._pending_mats.append(np.asfortranarray(Matrix, dtype=np.float64).ravel(order="F"))
+        if len(self._pending_ids) >= 65536:
+            self._flush()
+
+    def _flush(self):
+        if not self._pending_ids:
+            return
+        ids = np.asarray(self._pending_ids, dtype=np.int32)
+        ke = np.ascontiguousarray(np.concatenate(self._pending_mats))
+        self._pending_ids, self._pending_mats = [], []
+        L.check(L.load().pf_system_add_element_matrices(self.h, ids.size, L.ptr(ids, C.c_int32),
+                                                        L.ptr(ke, C.c_double)), "setMatrix")
+
+    def setVector(self, DomainID: int, ElementID, DOF: int, Vector):
+        self._flush()
+        ids = np.atleast_1d(np.asarray(ElementID, dtype=np.int32))
+        vec = np.ascontiguousarray(np.asarray(Vector, dtype=np.float64).reshape(ids.size, -1))
+        L.check(L.load().pf_system_add_element_vectors(self.h, ids.size, L.ptr(ids, C.c_int32),
+                                                       L.ptr(vec, C.c_double)), "setVector")
+
+    def setStiffness(self, YoungModulus=None, PoissonRatio=None, jmode: int = 0):
+        """Device assembly of StiffnessMatrix(YoungModulus(:),PoissonRatio(:)) into this solver's CRS.
+        Without arguments the materials of the previous call (resident on the device) are reused."""
+        self._flush()
+        E = None if YoungModulus is None else _f64(YoungModulus)
+        v = None if PoissonRatio is None else _f64(PoissonRatio)
+        L.check(L.load().pf_system_stiffness(self.h, L.ptr(E, C.c_double), L.ptr(v, C.c_double), jmode),
+                "setStiffness")
+        self.DOF = 3
+        self.n = 3 * self.domain.nn()
+        n = C.c_int64(0)
+        nnz = C.c_int64(0)
+        L.check(L.load().pf_system_sizes(self.h, C.byref(n), C.byref(nnz), None, None), "sizes")
+        self.nnz = nnz.value
+
+    def addTraction(self, box, pressure: float):
+        """Sum of TractionVector_by_elem over elements (open box) added to the RHS."""
+        b = _f64(box)
+        L.check(L.load().pf_system_add_traction(self.h, L.ptr(b, C.c_double), float(pressure)), "addTraction")
+
+    # (5)
+    def fix(self, DomainID: int = 1, IDs=None, FixValue: float | None = None, FixValues=None):
+        ids = _i32(IDs)
+        vals = None if FixValues is None else _f64(FixValues)
+        L.check(L.load().pf_system_fix(self.h, ids.size, L.ptr(ids, C.c_int32), L.ptr(vals, C.c_double),
+                                       0.0 if FixValue is None else float(FixValue)), "fix")
+
+    # (7-2)
+    def solve(self, x0=None, precision: int = 64, raise_on_breakdown: bool = True, download: bool = True,
+              **opts):
+        """solve() (FEMSolverClass.f90:1697-1825).  download=False keeps x on the device (returns None)."""
+        self._flush()
+        o = L.default_opts(itrmax=int(self.itrmax), er0=float(self.er0), relative_er=float(self.relative_er),
+                           debug=int(bool(self.debug)), precision=int(precision), **opts)
+        st = L.Stats()
+        x = np.empty(self.n) if download else None
+        code = L.load().pf_system_solve(self.h, C.byref(o), L.ptr(None if x0 is None else _f64(x0), C.c_double),
+                                        L.ptr(x, C.c_double), C.byref(st))
+        self.last_stats = st.as_dict()
+        ok = (L.PF_OK, L.PF_NOT_CONVERGED) if raise_on_breakdown else (
+            L.PF_OK, L.PF_NOT_CONVERGED, L.PF_BREAKDOWN_VV, L.PF_BREAKDOWN_C2C3)
+        L.check(code, "solve (Bicgstab devide by zero)" if code in (1, 2) else "solve", ok=ok)
+        return x
+
+
+def bicgstab_crs(row_ptr, col_idx, val, rhs, x0=None, ctx: L.Context | None = None, **opts):
+    """Plug-in LinearSolver(row_ptr, col_idx, val, rhs, x) (FEMSolverClass.f90:1638-1648) on the GPU.
+
+    Returns (x, stats dict).  No Dirichlet elimination, like the reference hook."""
+    ctx = ctx or L.default_context()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+    va = _f64(val)
+    b = _f64(rhs)
+    x = np.zeros(b.size) if x0 is None else np.array(x0, dtype=np.float64)
+    o = L.default_opts(**opts)
+    st = L.Stats()
+    code = L.load().pf_bicgstab_crs(ctx.h, b.size, L.ptr(rp, C.c_int64), L.ptr(ci, C.c_int32), L.ptr(va, C.c_double),
+                                    L.ptr(b, C.c_double), L.ptr(x, C.c_double), C.byref(o), C.byref(st))
+    L.check(code, "bicgstab_crs", ok=(L.PF_OK, L.PF_NOT_CONVERGED, L.PF_BREAKDOWN_VV, L.PF_BREAKDOWN_C2C3))
+    return x, st.as_dict()
+
+
+def spmv_crs(row_ptr, col_idx, val, x, ctx: L.Context | None = None) -> np.ndarray:
+    """sub_crs_matvec (LinearSolverClass.f90:3728-3776) on host arrays."""
+    ctx = ctx or L.default_context()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    y = np.empty(rp.size - 1)
+    L.check(L.load().pf_spmv_crs(ctx.h, rp.size - 1, L.ptr(rp, C.c_int64), L.ptr(_i32(col_idx), C.c_int32),
+                                 L.ptr(_f64(val), C.c_double), L.ptr(_f64(x), C.c_double), L.ptr(y, C.c_double)),
+            "spmv_crs")
+    return y

@@ -1,0 +1,91 @@
+"""Synthetic static-elasticity workloads on Cube3D hex meshes (seeded, no data files).
+
+box(n, sigma, nu, seed): unit cube with n^3 hex8 elements, E = 1e7 * lognormal(0, sigma)
+per element, bottom face (z = 0) fully clamped and a uniform downward traction of 1e4 Pa
+on the top face, as the static-analysis callers set it up (select(z_max=z_min()) for the
+fixed nodes, TractionVector_by_elem over the top facets).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .fem import cube3d
+
+TOP_TRACTION = 1.0e4  # Pa, downward
+
+
+@dataclass
+class Workload:
+    name: str
+    nodcoord: np.ndarray   # (nn,3) Fortran order
+    elemnod: np.ndarray    # (ne,8) Fortran order, 1-based
+    E: np.ndarray          # (ne,)
+    nu: np.ndarray         # (ne,)
+    fixed_dofs: np.ndarray  # 1-based DOF ids
+    traction_box: np.ndarray  # open box (xmin,xmax,ymin,ymax,zmin,zmax)
+    pressure: float
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nn(self) -> int:
+        return int(self.nodcoord.shape[0])
+
+    @property
+    def ne(self) -> int:
+        return int(self.elemnod.shape[0])
+
+    @property
+    def ndof(self) -> int:
+        return 3 * self.nn
+
+
+def lognormal_E(ne: int, sigma: float, rng: np.random.Generator, base: float = 1.0e7,
+                clip_ratio: float | None = None) -> np.ndarray:
+    f = np.exp(sigma * rng.standard_normal(ne))
+    if clip_ratio is not None:
+        half = np.sqrt(clip_ratio)
+        f = np.clip(f, 1.0 / half, half)
+    return base * f
+
+
+def box(n: int, sigma: float = 0.5, nu=0.3, seed: int = 0, clip_ratio: float | None = None,
+        nu_range: tuple | None = None, nx=None, ny=None, nz=None) -> Workload:
+    nx = nx or n
+    ny = ny or n
+    nz = nz or n
+    nod, elem = cube3d(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    ne = elem.shape[0]
+    E = lognormal_E(ne, sigma, rng, clip_ratio=clip_ratio)
+    if nu_range is not None:
+        nuv = rng.uniform(nu_range[0], nu_range[1], ne)
+    else:
+        nuv = np.full(ne, float(nu))
+    bottom = np.nonzero(nod[:, 2] <= nod[:, 2].min())[0] + 1
+    fixed = np.sort(np.concatenate([3 * bottom - 2, 3 * bottom - 1, 3 * bottom])).astype(np.int32)
+    zt = nod[:, 2].max()
+    eps = 1e-9 * max(1.0, abs(zt))
+    tbox = np.array([-1e300, 1e300, -1e300, 1e300, zt - eps, zt + eps])
+    return Workload(f"box{nx}x{ny}x{nz}", nod, elem, E, nuv, fixed, tbox, -TOP_TRACTION,
+                    {"sigma": sigma, "seed": seed, "clip_ratio": clip_ratio, "nu_range": nu_range})
+
+
+def config(k: int, seed: int = 0) -> Workload:
+    """The four single-mesh configurations of the measurement plan (1-based like the list)."""
+    if k == 1:   # 20^3, sigma 0.5, nu 0.3 -- the CPU-runnable case
+        return box(20, sigma=0.5, nu=0.3, seed=seed)
+    if k == 2:   # 100^3, sigma 1.0, nu ~ U[0.25, 0.35]
+        return box(100, sigma=1.0, seed=seed, nu_range=(0.25, 0.35))
+    if k == 3:   # 100^3, sigma 2.0 clipped to a 1e4 contrast, nu ~ U[0.25, 0.35]
+        return box(100, sigma=2.0, seed=seed, nu_range=(0.25, 0.35), clip_ratio=1.0e4)
+    if k == 5:   # 60^3 (adjoint case)
+        return box(60, sigma=0.5, nu=0.3, seed=seed)
+    raise ValueError(k)
+
+
+def crs_sizes(nx: int, ny: int, nz: int) -> tuple[int, int]:
+    """(n, nnz) of ZeroMatrix(DOF=3) on an nx x ny x nz Cube3D: nnz = 9 prod(3 m_i - 2), m_i = n_i + 1."""
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    return 3 * nn, 9 * (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)

@@ -1,0 +1,269 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Integer/index work must be bit-exact (graph); the element matrices and their assembly are
+written to follow the oracle's operation order and are compared bit-exactly too; the
+solver is compared within the tolerance each test states.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plantfem_amd import _lib as L
+from plantfem_amd import workloads as W
+from plantfem_amd.fem import CRS, FEMDomain, FEMSolver, bicgstab_crs, cube3d, spmv_crs
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def permuted_mesh(dims, seed, jitter=0.0):
+    rng = np.random.default_rng(seed)
+    nod, el = cube3d(*dims)
+    perm = rng.permutation(nod.shape[0])
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    nod2 = np.asfortranarray(nod[perm] + jitter * rng.standard_normal(nod.shape) / max(dims))
+    el2 = np.asfortranarray((inv[el - 1] + 1)[rng.permutation(el.shape[0])].astype(np.int32))
+    return nod2, el2
+
+
+# ------------------------------------------------------------------------------ graph
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 1), (3, 2, 4), (20, 20, 20)])
+@pytest.mark.parametrize("dof", [1, 3])
+def test_graph_bit_exact_cube(ctx, dims, dof):
+    say("graph", dims, dof)
+    nod, el = cube3d(*dims)
+    crs = FEMDomain(nod, el, ctx).ZeroMatrix(dof)
+    rp, ci = O.zero_matrix(el, nod.shape[0], dof)
+    assert np.array_equal(crs.row_ptr, rp) and np.array_equal(crs.col_idx, ci)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_graph_bit_exact_permuted(ctx, seed):
+    nod, el = permuted_mesh((7, 5, 6), seed)
+    say("graph permuted", seed)
+    for dof in (1, 2, 3):
+        crs = FEMDomain(nod, el, ctx).ZeroMatrix(dof)
+        rp, ci = O.zero_matrix(el, nod.shape[0], dof)
+        assert np.array_equal(crs.row_ptr, rp) and np.array_equal(crs.col_idx, ci)
+
+
+def test_graph_general_element_and_unused_node(ctx):
+    """nne = 4 connectivity with repeated incidences and a node no element uses."""
+    el = np.asfortranarray(np.array([[1, 2, 3, 4], [2, 3, 5, 6], [6, 5, 1, 7], [3, 7, 6, 2]], dtype=np.int32))
+    nn = 8  # node 8 is unused -> empty rows
+    rp, ci = O.zero_matrix(el, nn, 2)
+    lib = L.load()
+    nnz = C.c_int64(0)
+    e = el.ravel(order="F")
+    L.check(lib.pf_crs_graph(ctx.h, nn, 4, 4, L.ptr(e, C.c_int32), 2, C.byref(nnz), None, None), "graph")
+    grp = np.empty(2 * nn + 1, np.int64)
+    gci = np.empty(nnz.value, np.int32)
+    L.check(lib.pf_crs_graph(ctx.h, nn, 4, 4, L.ptr(e, C.c_int32), 2, C.byref(nnz), L.ptr(grp, C.c_int64),
+                             L.ptr(gci, C.c_int32)), "graph")
+    assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
+
+
+def test_graph_rejects_bad_ids(ctx):
+    el = np.asfortranarray(np.array([[1, 2, 3, 9]], dtype=np.int32))
+    nnz = C.c_int64(0)
+    code = L.load().pf_crs_graph(ctx.h, 4, 1, 4, L.ptr(el.ravel(order="F"), C.c_int32), 1, C.byref(nnz), None, None)
+    assert code == L.PF_INVALID
+
+
+# ---------------------------------------------------------------------------- element
+@pytest.mark.parametrize("jmode", [0, 1])
+def test_element_stiffness_bit_exact(ctx, jmode):
+    nod, el = permuted_mesh((4, 3, 3), 7, jitter=0.2)
+    rng = np.random.default_rng(1)
+    ne = el.shape[0]
+    E = 1e7 * np.exp(rng.standard_normal(ne))
+    nu = rng.uniform(0.1, 0.45, ne)
+    say("Ke", jmode)
+    ke = FEMDomain(nod, el, ctx).StiffnessMatrices(E, nu, jmode=jmode)
+    want = O.hex8_stiffness_all(nod, el, E, nu, jmode)
+    assert np.array_equal(ke, want), np.abs(ke - want).max()
+    one = FEMDomain(nod, el, ctx).StiffnessMatrix(ElementID=5, E=E[4], v=nu[4], jmode=jmode)
+    assert np.array_equal(one, want[4])
+
+
+@pytest.mark.parametrize("jmode", [0, 1])
+def test_assembly_bit_exact(ctx, jmode):
+    nod, el = permuted_mesh((6, 5, 4), 3, jitter=0.15)
+    rng = np.random.default_rng(2)
+    ne = el.shape[0]
+    E = 1e7 * np.exp(0.8 * rng.standard_normal(ne))
+    nu = rng.uniform(0.25, 0.35, ne)
+    say("assembly", jmode)
+    crs = FEMDomain(nod, el, ctx).StiffnessMatrix(YoungModulus=E, PoissonRatio=nu, jmode=jmode)
+    rp, ci = O.zero_matrix(el, nod.shape[0], 3)
+    val = O.assemble(nod, el, E, nu, rp, ci, jmode)
+    assert np.array_equal(crs.row_ptr, rp) and np.array_equal(crs.col_idx, ci)
+    assert np.array_equal(crs.val, val), np.abs(crs.val - val).max()
+
+
+def test_setmatrix_setvector_call_order(ctx):
+    """setMatrix/setVector batches add in call order, like setValueFEMSolver."""
+    nod, el = cube3d(3, 3, 2)
+    rng = np.random.default_rng(5)
+    ne = el.shape[0]
+    order = np.concatenate([rng.permutation(ne) + 1, rng.integers(1, ne + 1, 7)])  # repeats included
+    mats = rng.standard_normal((order.size, 24, 24))
+    vecs = rng.standard_normal((order.size, 24))
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(nod, el, ctx))
+    s.setCRS(DOF=3)
+    for q, e in enumerate(order):
+        s.setMatrix(1, int(e), 3, mats[q])
+    s.setVector(1, order, 3, vecs)
+    got = s.getCRS()
+    rhs = s.getRHS()
+    rp, ci = O.zero_matrix(el, nod.shape[0], 3)
+    val = np.zeros(ci.size)
+    want_rhs = np.zeros(3 * nod.shape[0])
+    for q, e in enumerate(order):
+        assert O.set_element_matrix(el, e, 3, mats[q], rp, ci, val) == 0
+        dofs = (3 * (el[e - 1][:, None] - 1) + np.arange(3)).ravel()
+        for k, d in enumerate(dofs):
+            want_rhs[d] = want_rhs[d] + vecs[q, k]
+    assert np.array_equal(got.val, val)
+    assert np.array_equal(rhs, want_rhs)
+
+
+# -------------------------------------------------------------------------- traction, BC
+def test_traction_and_dirichlet_bit_exact(ctx):
+    w = W.box(5, sigma=0.7, seed=4)
+    rng = np.random.default_rng(6)
+    ubar_vals = rng.uniform(-1e-4, 1e-4, w.fixed_dofs.size)
+    say("traction + dirichlet")
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    rp, ci, val, rhs, fixed, _ = O.build_system(w)
+    assert np.array_equal(s.getRHS(), rhs)
+    s.fix(1, w.fixed_dofs, FixValues=ubar_vals)
+    ubar = np.zeros(rhs.size)
+    ubar[w.fixed_dofs - 1] = ubar_vals
+    s.itrmax = 1
+    s.solve(raise_on_breakdown=False)  # applies the elimination on the device
+    v2, r2 = O.apply_dirichlet(rp, ci, val, rhs, fixed, ubar)
+    got = s.getCRS()
+    assert np.array_equal(got.val, v2)
+    assert np.array_equal(s.getRHS(), r2)
+
+
+# ------------------------------------------------------------------------------ SpMV
+@pytest.mark.parametrize("kind", [1, 2])
+def test_spmv_matches_oracle(ctx, kind):
+    w = W.box(12, sigma=1.0, seed=8)
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    x = np.random.default_rng(3).standard_normal(rhs.size)
+    say("spmv crs")
+    y = spmv_crs(rp, ci, val, x, ctx)
+    ref = O.spmv(rp, ci, val, x)
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------------------- solver
+import json  # noqa: E402
+from pathlib import Path  # noqa: E402
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "known_answers.json").read_text())
+
+
+@pytest.mark.parametrize("case", GOLD["systems"], ids=[c["name"] for c in GOLD["systems"]])
+def test_tutorial_systems(ctx, case):
+    import scipy.sparse as sp
+    M = sp.csr_matrix(np.array(case["A"], float))
+    rp, ci = (M.indptr + 1).astype(np.int64), (M.indices + 1).astype(np.int32)
+    b = np.array(case["b"], float)
+    want = np.linalg.solve(M.toarray(), b) if case["x"] is None else np.array(case["x"])
+    say("tutorial", case["name"])
+    x, st = bicgstab_crs(rp, ci, M.data, b, ctx=ctx, relative_er=1e-13, er0=1e-14)
+    assert st["status"] in (0, 1)
+    assert np.allclose(x, want, rtol=1e-9, atol=1e-9)
+    # same iterate sequence as the oracle on these tiny systems
+    xo, sto, ito, *_ = O.bicgstab(rp, ci, M.data, b, relative_er=1e-13, er0=1e-14)
+    assert st["status"] == sto and st["iterations"] == ito
+
+
+def test_breakdown_and_itrmax_statuses(ctx):
+    n = 5
+    rp = np.arange(1, n + 2, dtype=np.int64)
+    ci = np.arange(1, n + 1, dtype=np.int32)
+    say("breakdown")
+    x, st = bicgstab_crs(rp, ci, np.full(n, 2.0), np.arange(1.0, n + 1), ctx=ctx)
+    assert st["status"] == L.PF_BREAKDOWN_VV and st["iterations"] == 1
+    w = W.box(3, seed=2)
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    v2, r2 = O.apply_dirichlet(rp, ci, val, rhs, fixed, ub)
+    x, st = bicgstab_crs(rp, ci, v2, r2, ctx=ctx, itrmax=3)
+    assert st["status"] == L.PF_NOT_CONVERGED and st["iterations"] == 3
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_c1_solve_matches_oracle(ctx, kind):
+    """Config 1 end to end on the GPU vs the oracle, both converged tightly: 1e-10 relative."""
+    w = W.config(1)
+    say("C1 solve kind", kind)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    s.relative_er = 1e-14
+    s.er0 = 1e-30
+    x = s.solve(spmv_kind=kind)
+    st = s.last_stats
+    assert st["status"] == 0 and st["spmv_kind_used"] == kind
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    xo, sto, ito, *_ = O.solve(rp, ci, val, rhs, fixed, ub, relative_er=1e-14, er0=1e-30)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    say("C1 iters gpu", st["iterations"], "oracle", ito, "rel err", err, "relres", st["true_relres"])
+    assert err <= 1e-10
+    assert st["true_relres"] < 1e-12
+
+
+def test_c1_reference_tolerance(ctx):
+    """At the reference's default stop (dble(1.0e-10), r1-relative) the GPU stops within a few
+    iterations of the oracle and both solutions agree to the accuracy that stop implies."""
+    w = W.config(1)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    say("C1 default tolerance")
+    x = s.solve()
+    st = s.last_stats
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    xo, sto, ito, rro, rr1o = O.solve(rp, ci, val, rhs, fixed, ub)
+    say("iters", st["iterations"], ito)
+    assert st["status"] == 0 and abs(st["iterations"] - ito) <= max(5, ito // 20)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-7
+
+
+def test_mixed_precision_c1(ctx):
+    w = W.config(1)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    say("C1 mixed")
+    x = s.solve(precision=32, target_relres=1e-10)
+    st = s.last_stats
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    xo, *_ = O.solve(rp, ci, val, rhs, fixed, ub, relative_er=1e-14, er0=1e-30)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    say("mixed outer", st["outer_iterations"], "inner", st["iterations"], "relres", st["true_relres"], "err", err)
+    assert st["status"] == 0 and st["true_relres"] < 1e-10
+    assert err <= 1e-8
