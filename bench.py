@@ -51,12 +51,16 @@ def spmv_bytes(kind: int, n: int, nnz: int) -> float:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=100)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--no-graph", action="store_true", help="direct kernel launches only (for profilers)")
+    ap.add_argument("--profile", action="store_true",
+                    help="short profiling workload: 40 FP64 iterations, the SpMV kernels and a 512 MiB "
+                         "device copy (the reference kernel for HBM counter calibration)")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the multi-GPU (z-slab sharded) path is a design "
@@ -82,13 +86,36 @@ def main(argv=None):
 
     s.itrmax = 40000
 
+    if a.profile:
+        s.setCRS(DOF=3)
+        s.setStiffness()
+        s.addTraction(w.traction_box, w.pressure)
+        s.fix(1, w.fixed_dofs, FixValue=0.0)
+        s.itrmax = 40
+        s.relative_er = 1e-30
+        s.solve(stop_mode=1, download=False, use_graph=0, raise_on_breakdown=False)
+        log(f"profile: {s.last_stats['iterations']} FP64 iterations")
+        xr = np.random.default_rng(1).standard_normal(n)
+        for kind in (1, 2, 3):
+            _, ms = s.spmv(kind, xr, repeat=5, download=False)
+            log(f"profile: spmv kind {kind} {ms:.3f} ms")
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("profile: 3 x 512 MiB device copies done")
+        return
+
     def step(precision=64, timing=0):
         s.setCRS(DOF=3)                       # ZeroMatrix graph build (device)
         s.setStiffness()                      # element stiffness assembly (device)
         s.addTraction(w.traction_box, w.pressure)
         s.fix(1, w.fixed_dofs, FixValue=0.0)
         s.relative_er = 1e-8
-        s.solve(precision=precision, stop_mode=1, download=False, kernel_timing=timing, target_relres=1e-8)
+        s.solve(precision=precision, stop_mode=1, download=False, kernel_timing=timing, target_relres=1e-8,
+                raise_on_breakdown=False, use_graph=0 if a.no_graph else 1)
         return dict(s.last_stats)
 
     # ---- FP64 time to solve (timed steps carry per-kernel hipEvent timing) -------------
@@ -123,8 +150,8 @@ def main(argv=None):
         st32 = step(32, 0)
         ctx.synchronize()
         mixed_times.append(time.perf_counter() - t)
-        log(f"mixed {i}: {mixed_times[-1]:.3f} s, outer {st32['outer_iterations']}, inner {st32['iterations']}, "
-            f"relres {st32['true_relres']:.2e}")
+        log(f"mixed {i}: {mixed_times[-1]:.3f} s, status {st32['status']}, outer {st32['outer_iterations']}, "
+            f"inner {st32['iterations']}, relres {st32['true_relres']:.2e}")
     t_mixed = float(np.median(mixed_times))
     x32 = s.getX()
     step(64, 0)
@@ -178,7 +205,7 @@ def main(argv=None):
         "iterations": iters, "solve_ms": round(st64["solve_ms"], 2), "true_relres": st64["true_relres"],
         "dof_iterations_per_s": round(n * iters / (st64["solve_ms"] * 1e-3), 1),
         "solve_ms_graph_mode": round(st_graph["solve_ms"], 2), "step_s_graph_mode": round(t_graph, 4),
-        "mixed": {"time_to_solve_s": round(t_mixed, 4), "solve_ms": round(st32["solve_ms"], 2),
+        "mixed": {"status": st32["status"], "time_to_solve_s": round(t_mixed, 4), "solve_ms": round(st32["solve_ms"], 2),
                   "outer": st32["outer_iterations"], "inner_iterations": st32["iterations"],
                   "true_relres": st32["true_relres"], "rel_err_vs_fp64": mixed_err,
                   "speedup_vs_fp64": round(t_step / t_mixed, 3)},

@@ -39,12 +39,16 @@ typedef struct pf_system pf_system; /* device-resident FEMSolver_ state (mesh, C
  * to dble(1.0e-10): a REAL(4) literal widened to REAL(8), i.e. 1.00000001335e-10. */
 typedef struct {
   int32_t itrmax;        /* 100000 */
-  int32_t precision;     /* 64: plain FP64 BiCGSTAB (bicgstab_CRS_2); 32: FP32 inner + FP64 refinement */
+  int32_t precision;     /* 64: plain FP64 BiCGSTAB (bicgstab_CRS_2); 32: FP32 BiCGSTAB with FP64 reliable
+                            updates (FP64 solution accumulation and true-residual replacement) */
   double er0;            /* absolute stop sqrt(r.r) < er0 */
   double relative_er;    /* stop sqrt(r.r / r1.r1) < relative_er (r1 = residual after iteration 1) */
-  double inner_tol;      /* precision 32: inner FP32 solve relative tolerance (default 1e-5) */
-  double target_relres;  /* precision 32: stop when ||b-Ax||/||b|| (FP64) < target (default 1e-8) */
-  int32_t max_outer;     /* precision 32: refinement steps cap (default 50) */
+  double inner_tol;      /* precision 32: replace the FP32 recursive residual by the FP64 true residual
+                            once its norm fell below inner_tol times its largest value since the last
+                            replacement (default 0.1) */
+  double target_relres;  /* precision 32: stop when ||b-Ax||/||b|| (FP64, at a replacement) < target
+                            (default 1e-8) */
+  int32_t max_outer;     /* unused, kept for layout (0) */
   int32_t spmv_kind;     /* 0 auto (blocked when the CRS has DOF=3 node blocks), 1 CRS row kernel, 2 blocked */
   int32_t check_every;   /* iterations enqueued between host polls of the stop flag (default 32) */
   int32_t use_graph;     /* capture the polled chunk in a hipGraph (default 1) */
@@ -54,12 +58,15 @@ typedef struct {
                             and report the average device time of each in pf_stats */
   int32_t jacobi;        /* precision 32: solve the FP32 inner system symmetrically scaled,
                             D^-1/2 A D^-1/2 with D = diag(A) (default 1) */
+  int32_t inner_max;     /* precision 32: also replace the residual after this many iterations without
+                            a replacement (default 256; 0 = never) */
+  int32_t reserved;
 } pf_solve_opts;
 
 typedef struct {
   int32_t status;           /* PF_* code of the solve */
-  int32_t iterations;       /* BiCGSTAB iterations (FP64) or total inner iterations (mixed) */
-  int32_t outer_iterations; /* refinement steps (mixed), 0 for FP64 */
+  int32_t iterations;       /* BiCGSTAB iterations (FP64 or, for precision 32, FP32) */
+  int32_t outer_iterations; /* FP64 residual replacements (mixed), 0 for FP64 */
   int32_t spmv_kind_used;   /* 1 CRS row kernel, 2 blocked */
   double rr;                /* final r.r of the recurrence (FP64 form) */
   double rr1;               /* r.r after iteration 1 */

@@ -49,6 +49,8 @@ class SolveOpts(C.Structure):
         ("stop_mode", C.c_int32),
         ("kernel_timing", C.c_int32),
         ("jacobi", C.c_int32),
+        ("inner_max", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

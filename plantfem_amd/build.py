@@ -85,7 +85,7 @@ def build_fortran() -> Path | None:
     mod_dir = LIB / "mod"
     mod_dir.mkdir(parents=True, exist_ok=True)
     out = LIB / "libpfgpu_f.so"
-    _run([FLANG, "-O2", "-fPIC", "-shared", f"-module-dir={mod_dir}", str(src), f"-L{LIB}", "-lpfgpu",
+    _run([FLANG, "-O2", "-fPIC", "-shared", "-J", str(mod_dir), str(src), f"-L{LIB}", "-lpfgpu",
           "-Wl,-rpath,$ORIGIN", "-o", str(out)])
     ex_dir = FSRC / "examples"
     bin_dir = LIB / "bin"

@@ -4,22 +4,28 @@
 //   c1 = r0.r ; y = A p ; c2 = r0.y ; alp = c1/c2 ; e = r - alp y ; v = A e ;
 //   c3 = (e.v)/(v.v) ; x += alp p + c3 e ; r = e - c3 v ; rr = r.r ; stop tests ;
 //   c1 = r0.r ; bet = c1/(c2 c3) ; p = r + bet (p - c3 y)
-// and is mapped onto four kernels that keep every scalar on the device:
-//   K1  y = A p            + partial r0.y          -> last block: c2, alpha
-//   K2  v = A e, e formed on the fly as r - alpha y while gathering
-//                          + partials e.v, v.v     -> last block: omega (or v.v == 0 stop)
-//   K3  x, r update        + partials r.r, r0.r    -> last block: stop tests, beta
-//   K4  p = r + beta (p - omega y)
-// e is never stored.  Each reduction writes one partial per block; the last block to
-// arrive (atomic ticket) sums the partials in a fixed order, so results are bitwise
-// reproducible run to run.  After a stop every kernel returns at entry, so the host can
-// enqueue iterations in chunks (optionally one hipGraph per chunk) and poll a pinned copy
-// of the scalars once per chunk.  The redundant loop-top c1 = r0.r of the reference equals
-// the c1 of the previous iteration's end and is not recomputed.
+// and runs as four kernels:
+//   K1  y = A p                                      partials r0.y
+//   K2  c2, alpha from K1's partials; v = A e with e = r - alpha y formed while gathering
+//                                                    partials e.v, v.v
+//   K3  omega (or the v.v == 0 stop) from K2's partials; x += alpha p + omega e;
+//       r = e - omega v                              partials r.r, r0.r
+//   K4  rr, stop tests, beta from K3's partials; p = r + beta (p - omega y)
+// Reductions need no atomics and no fences: every block writes one partial sum per dot
+// product, and every block of the NEXT kernel reads all partials and sums them in the same
+// fixed order, so each block holds bitwise the same scalar (and results are reproducible
+// run to run).  Kernel boundaries make the partials visible across the chip's XCDs.  A
+// scalar is written by block 0 of one kernel and read only by later kernels; the iteration
+// counter alternates between two slots by iteration parity (a kernel argument) so no block
+// reads a value its own launch writes.  After a stop every kernel returns at entry, so the
+// host enqueues iterations in chunks (one hipGraph per chunk) and polls a pinned copy of
+// the scalars once per chunk.  The redundant loop-top c1 = r0.r of the reference equals
+// the c1 of the previous iteration's end and is not recomputed.  e is never stored.
 //
-// The same kernels, instantiated for float vectors and matrix values with double partial
-// sums, are the inner solver of the mixed-precision form: FP32 SpMV with FP32 accumulation,
-// FP64 global dot products, FP64 residual and solution update in the refinement loop.
+// The same kernels, instantiated for float vectors and matrix values (FP32 SpMV with FP32
+// accumulation) with double partial sums, are the inner solver of the mixed-precision
+// form; the refinement loop keeps the residual r = b - Ax, the solution update and all
+// global dot products in FP64.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -34,22 +40,27 @@ constexpr int BT = 256;  // threads per block of every fused kernel (4 wavefront
 
 struct Scal {
   double c1, c2, alpha, omega, beta;
-  double rr, rr1, bnorm2;
-  double er0, rel;
-  double rnorm2;  // refinement residual norm^2
-  int32_t itr, itrmax, done, status;
-  int32_t stop_mode, hist_cap;
-  uint32_t arrive[4];
+  double rr, rr1, bnorm2, er0, rel;
+  double rnorm2;       // refinement: |b - A x|^2
+  double rr_best[2];   // stagnation watch: smallest r.r so far (slot by parity)
+  double maxr[2];      // reliable updates: largest |r| since the last replacement (slot by parity)
+  double relres;       // reliable updates: FP64 |b - A x| / |b| at the last replacement
+  double target, delta;
+  int32_t pending;     // reliable updates: a replacement is due after this iteration
+  int32_t nupd;        // replacements done
+  int32_t itr_rep;     // iteration of the last replacement
+  int32_t itr[2];      // completed iterations, slot by parity (written by K4)
+  int32_t itr_best[2]; // iteration of rr_best
+  int32_t itr_last;    // host-facing copy of the latest count
+  int32_t itrmax, done, status, stop_mode, hist_cap, guard, stall, pad;
 };
 
 template <typename V>
 struct Mat {
-  // CRS
   int64_t n;
   const int64_t *row_ptr;
   const int32_t *col_idx;
   const V *val;
-  // Sell3
   int64_t nn, nslices;
   const int64_t *soff;
   const int32_t *width;
@@ -87,66 +98,49 @@ __device__ inline void for_rows(const Mat<V> &m, XF xf, OP op) {
   }
 }
 
+// block sum of NV values, result in every thread
 template <int NV>
-__device__ inline void block_partials(double (&v)[NV], double *part) {
+__device__ inline void block_sum(double (&v)[NV]) {
   __shared__ double sh[NV][BT / 64];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
+  for (int k = 0; k < NV; ++k)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
-  }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
     for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
   __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+  __syncthreads();
+}
+
+// one partial per block: part[k*gridDim.x + blockIdx.x]
+template <int NV>
+__device__ inline void write_partials(double (&v)[NV], double *part) {
+  block_sum<NV>(v);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) part[(int64_t)k * gridDim.x + blockIdx.x] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    for (int k = 0; k < NV; ++k) part[(int64_t)k * gridDim.x + blockIdx.x] = v[k];
 }
 
-// true in every thread of the last block to finish; its partial writes are visible
-__device__ inline bool last_block(uint32_t *ticket) {
-  __shared__ bool am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t prev = atomicAdd(ticket, 1u);
-    am_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (am_last) __threadfence();
-  return am_last;
-}
-
+// all G partials of the previous kernel, summed in a fixed order (same result in every block)
 template <int NV>
-__device__ inline void final_sums(const double *part, double (&out)[NV]) {
-  __shared__ double sh[NV][BT / 64];
-  double v[NV];
+__device__ inline void read_partials(const double *part, int G, double (&out)[NV]) {
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  for (int64_t b = threadIdx.x; b < gridDim.x; b += BT)
+  for (int k = 0; k < NV; ++k) out[k] = 0.0;
+  for (int b = threadIdx.x; b < G; b += BT)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] += __builtin_nontemporal_load(&part[(int64_t)k * gridDim.x + b]);
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
-  }
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) out[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    for (int k = 0; k < NV; ++k) out[k] += part[(int64_t)k * G + b];
+  block_sum<NV>(out);
 }
 
-// ---- K0: r = b - A x (or r = b when x0 = 0); r0 = p = r; c1 = r.r; |b|^2 -----------
+// ---- K0: r = b - A x (or r = b when x0 = 0); r0 = p = r  -> partials r.r, b.b ----------
 template <typename V, int L, int SUB, bool ZEROX>
 __global__ void __launch_bounds__(BT) k_init(Mat<V> m, const V *__restrict__ b, V *__restrict__ x,
                                              V *__restrict__ r, V *__restrict__ r0, V *__restrict__ p,
-                                             Scal *__restrict__ sc, double *__restrict__ part) {
+                                             double *__restrict__ part) {
   double acc[2] = {0.0, 0.0};
   if (ZEROX) {
     for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < m.n; i += (int64_t)gridDim.x * BT) {
@@ -170,26 +164,30 @@ __global__ void __launch_bounds__(BT) k_init(Mat<V> m, const V *__restrict__ b, 
     };
     for_rows<V, L, SUB>(m, pfdev::PlainX<V>{x}, op);
   }
-  block_partials<2>(acc, part);
-  if (last_block(&sc->arrive[0])) {
-    double t[2];
-    final_sums<2>(part, t);
-    if (threadIdx.x == 0) {
-      sc->c1 = t[0];
-      sc->rr = t[0];
-      sc->bnorm2 = t[1];
-      sc->itr = 0;
-      sc->done = sc->itrmax <= 0 ? 1 : 0;
-      sc->status = PF_NOT_CONVERGED;
-      sc->arrive[0] = 0;
-    }
+  write_partials<2>(acc, part);
+}
+
+// one block: c1 = r.r, |b|^2, counters
+__global__ void __launch_bounds__(BT) k_init_finish(const double *__restrict__ part, int G, Scal *__restrict__ sc) {
+  double t[2];
+  read_partials<2>(part, G, t);
+  if (threadIdx.x == 0) {
+    sc->c1 = t[0];
+    sc->rr = t[0];
+    sc->bnorm2 = t[1];
+    sc->itr[0] = sc->itr[1] = 0;
+    sc->itr_best[0] = sc->itr_best[1] = 0;
+    sc->rr_best[0] = sc->rr_best[1] = t[0];
+    sc->itr_last = 0;
+    sc->done = sc->itrmax <= 0 ? 1 : 0;
+    sc->status = PF_NOT_CONVERGED;
   }
 }
 
-// ---- K1: y = A p ; c2 = r0.y ; alpha = c1/c2 ------------------------------------------
+// ---- K1: y = A p -> partials r0.y ------------------------------------------------------
 template <typename V, int L, int SUB>
 __global__ void __launch_bounds__(BT) k_spmv_p(Mat<V> m, const V *__restrict__ p, V *__restrict__ y,
-                                               const V *__restrict__ r0, Scal *__restrict__ sc,
+                                               const V *__restrict__ r0, const Scal *__restrict__ sc,
                                                double *__restrict__ part) {
   if (sc->done) return;
   double acc[1] = {0.0};
@@ -198,16 +196,7 @@ __global__ void __launch_bounds__(BT) k_spmv_p(Mat<V> m, const V *__restrict__ p
     acc[0] += (double)r0[row] * (double)yv;
   };
   for_rows<V, L, SUB>(m, pfdev::PlainX<V>{p}, op);
-  block_partials<1>(acc, part);
-  if (last_block(&sc->arrive[1])) {
-    double t[1];
-    final_sums<1>(part, t);
-    if (threadIdx.x == 0) {
-      sc->c2 = t[0];
-      sc->alpha = sc->c1 / t[0];
-      sc->arrive[1] = 0;
-    }
-  }
+  write_partials<1>(acc, part);
 }
 
 template <typename V>
@@ -218,14 +207,30 @@ struct EFromRY {
   __device__ V operator()(int64_t j) const { return r[j] - alpha * y[j]; }
 };
 
-// ---- K2: v = A e with e = r - alpha y ; e.v, v.v ; omega -------------------------------
+// ---- K2: c2, alpha ; v = A e, e = r - alpha y -> partials e.v, v.v ----------------------
 template <typename V, int L, int SUB>
 __global__ void __launch_bounds__(BT) k_spmv_e(Mat<V> m, const V *__restrict__ r, const V *__restrict__ y,
                                                V *__restrict__ v, Scal *__restrict__ sc,
+                                               const double *__restrict__ pin, int Gin,
                                                double *__restrict__ part) {
   if (sc->done) return;
-  const V alpha = (V)sc->alpha;
-  const EFromRY<V> ef{r, y, alpha};
+  double t[1];
+  read_partials<1>(pin, Gin, t);
+  const double c2 = t[0];
+  const double alpha_d = sc->c1 / c2;
+  const bool first = threadIdx.x == 0 && blockIdx.x == 0;
+  if (first) {
+    sc->c2 = c2;
+    sc->alpha = alpha_d;
+  }
+  if (sc->guard && !isfinite(alpha_d)) {  // r0.Ap lost: stop before x is touched
+    if (first) {
+      sc->status = PF_BREAKDOWN_C2C3;
+      sc->done = 1;
+    }
+    return;
+  }
+  const EFromRY<V> ef{r, y, (V)alpha_d};
   double acc[2] = {0.0, 0.0};
   auto op = [&](int64_t row, V vv) {
     v[row] = vv;
@@ -234,31 +239,37 @@ __global__ void __launch_bounds__(BT) k_spmv_e(Mat<V> m, const V *__restrict__ r
     acc[1] += (double)vv * (double)vv;
   };
   for_rows<V, L, SUB>(m, ef, op);
-  block_partials<2>(acc, part);
-  if (last_block(&sc->arrive[2])) {
-    double t[2];
-    final_sums<2>(part, t);
-    if (threadIdx.x == 0) {
-      if (t[1] == 0.0) {
-        sc->status = PF_BREAKDOWN_VV;
-        sc->done = 1;
-      } else {
-        sc->omega = t[0] / t[1];
-      }
-      sc->arrive[2] = 0;
-    }
-  }
+  write_partials<2>(acc, part);
 }
 
-// ---- K3: x += alpha p + omega e ; r = e - omega v ; stop tests ; beta ----------------
+// ---- K3: omega ; x += alpha p + omega e ; r = e - omega v -> partials r.r, r0.r --------
 template <typename V>
 __global__ void __launch_bounds__(BT) k_update(int64_t n, V *__restrict__ x, V *__restrict__ r,
                                                const V *__restrict__ r0, const V *__restrict__ p,
                                                const V *__restrict__ y, const V *__restrict__ v,
-                                               Scal *__restrict__ sc, double *__restrict__ part,
-                                               double *__restrict__ hist) {
+                                               Scal *__restrict__ sc, const double *__restrict__ pin, int Gin,
+                                               double *__restrict__ part) {
   if (sc->done) return;
-  const V alpha = (V)sc->alpha, omega = (V)sc->omega;
+  double t[2];
+  read_partials<2>(pin, Gin, t);
+  const bool first = threadIdx.x == 0 && blockIdx.x == 0;
+  if (t[1] == 0.0) {  // "Bicgstab devide by zero" (FEMSolverClass.f90:2266)
+    if (first) {
+      sc->status = PF_BREAKDOWN_VV;
+      sc->done = 1;
+    }
+    return;
+  }
+  const double omega_d = t[0] / t[1];
+  if (sc->guard && !isfinite(omega_d)) {
+    if (first) {
+      sc->status = PF_BREAKDOWN_VV;
+      sc->done = 1;
+    }
+    return;
+  }
+  if (first) sc->omega = omega_d;
+  const V alpha = (V)sc->alpha, omega = (V)omega_d;
   double acc[2] = {0.0, 0.0};
   for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) {
     const V e = r[i] - alpha * y[i];
@@ -268,58 +279,116 @@ __global__ void __launch_bounds__(BT) k_update(int64_t n, V *__restrict__ x, V *
     acc[0] += (double)rn * (double)rn;
     acc[1] += (double)r0[i] * (double)rn;
   }
-  block_partials<2>(acc, part);
-  if (last_block(&sc->arrive[3])) {
-    double t[2];
-    final_sums<2>(part, t);
-    if (threadIdx.x == 0) {
-      const int itr = sc->itr + 1;
-      const double rr = t[0];
-      sc->itr = itr;
-      sc->rr = rr;
-      if (itr == 1) sc->rr1 = rr;
-      if (hist && itr <= sc->hist_cap) hist[itr - 1] = sqrt(rr);
-      bool stop;
-      if (sc->stop_mode == 0)
-        stop = (sqrt(rr / sc->rr1) < sc->rel) || (sqrt(rr) < sc->er0);
-      else
-        stop = sqrt(rr) < sc->rel * sqrt(sc->bnorm2);
-      if (stop) {
-        sc->status = PF_OK;
-        sc->done = 1;
-      } else {
-        sc->c1 = t[1];
-        const double c2w = sc->c2 * sc->omega;
-        sc->beta = t[1] / c2w;
-        if (c2w == 0.0) {
-          sc->status = PF_BREAKDOWN_C2C3;
-          sc->done = 1;
-        } else if (itr >= sc->itrmax) {
-          sc->status = PF_NOT_CONVERGED;
-          sc->done = 1;
-        }
-      }
-      sc->arrive[3] = 0;
+  write_partials<2>(acc, part);
+}
+
+// ---- K4: rr, stop tests, beta ; p = r + beta (p - omega y) -----------------------------
+template <typename V>
+__global__ void __launch_bounds__(BT) k_pupdate(int64_t n, V *__restrict__ p, const V *__restrict__ r,
+                                                const V *__restrict__ y, Scal *__restrict__ sc,
+                                                const double *__restrict__ pin, int Gin, int par,
+                                                double *__restrict__ hist, double *__restrict__ xhi,
+                                                V *__restrict__ xlo) {
+  if (sc->done) return;
+  double t[2];
+  read_partials<2>(pin, Gin, t);
+  const bool first = threadIdx.x == 0 && blockIdx.x == 0;
+  const double rr = t[0];
+  const int itr = sc->itr[par] + 1;
+  const double rr1 = (itr == 1) ? rr : sc->rr1;
+  if (first) {
+    sc->itr[par ^ 1] = itr;
+    sc->itr_last = itr;
+    sc->rr = rr;
+    if (itr == 1) sc->rr1 = rr;
+    if (hist && itr <= sc->hist_cap) hist[itr - 1] = sqrt(rr);
+  }
+  bool stop;
+  bool replace = false;
+  if (sc->stop_mode == 0) {  // FEMSolverClass.f90:2275-2289
+    stop = (sqrt(rr / rr1) < sc->rel) || (sqrt(rr) < sc->er0);
+  } else if (sc->stop_mode == 1) {
+    stop = sqrt(rr) < sc->rel * sqrt(sc->bnorm2);
+  } else {
+    // reliable updates: convergence is judged on the FP64 true residual after a
+    // replacement; here only decide whether one is due (|r| fell by delta since its max)
+    stop = false;
+    const double nr = sqrt(rr);
+    const double mx = fmax(sc->maxr[par], nr);
+    if (first) sc->maxr[par ^ 1] = mx;
+    // also replace when the FP32 recurrence has run `stall` iterations since the last one
+    replace = nr < sc->delta * mx || (sc->stall > 0 && itr - sc->itr_rep >= sc->stall);
+  }
+  if (stop) {
+    if (first) {
+      sc->status = PF_OK;
+      sc->done = 1;
     }
+    return;
+  }
+  if (sc->stall > 0 && sc->stop_mode != 2) {
+    // inner FP32 solves: end the solve once |r| has not dropped by 10% for `stall`
+    // iterations (the recurrence sits at its rounding floor and BiCGSTAB then tends to
+    // diverge) or jumps 100x above its best; the FP64 refinement restarts from there
+    double best = sc->rr_best[par];
+    int ib = sc->itr_best[par];
+    if (rr < 0.81 * best) {
+      best = rr;
+      ib = itr;
+    }
+    if (first) {
+      sc->rr_best[par ^ 1] = best;
+      sc->itr_best[par ^ 1] = ib;
+    }
+    if (itr - ib > sc->stall || rr > 1e4 * best) {
+      if (first) {
+        sc->status = PF_NOT_CONVERGED;
+        sc->done = 1;
+      }
+      return;
+    }
+  }
+  const double c2w = sc->c2 * sc->omega;
+  const double beta_d = t[1] / c2w;
+  if (c2w == 0.0 || (sc->guard && !(isfinite(beta_d) && isfinite(rr)))) {  // FEMSolverClass.f90:2295
+    if (first) {
+      sc->status = PF_BREAKDOWN_C2C3;
+      sc->done = 1;
+    }
+    return;
+  }
+  if (itr >= sc->itrmax) {
+    if (first) {
+      sc->status = PF_NOT_CONVERGED;
+      sc->done = 1;
+    }
+    return;
+  }
+  if (first) {
+    sc->c1 = t[1];
+    sc->beta = beta_d;
+    if (replace) sc->pending = 1;
+  }
+  const V beta = (V)beta_d, omega = (V)sc->omega;
+  if (replace) {
+    // fold the FP32 correction into the FP64 solution before the residual is replaced
+    for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) {
+      p[i] = r[i] + beta * (p[i] - omega * y[i]);
+      xhi[i] += (double)xlo[i];
+      xlo[i] = (V)0;
+    }
+  } else {
+    for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT)
+      p[i] = r[i] + beta * (p[i] - omega * y[i]);
   }
 }
 
-// ---- K4: p = r + beta (p - omega y) ----------------------------------------------------
-template <typename V>
-__global__ void __launch_bounds__(BT) k_pupdate(int64_t n, V *__restrict__ p, const V *__restrict__ r,
-                                                const V *__restrict__ y, const Scal *__restrict__ sc) {
-  if (sc->done) return;
-  const V beta = (V)sc->beta, omega = (V)sc->omega;
-  for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT)
-    p[i] = r[i] + beta * (p[i] - omega * y[i]);
-}
-
 // ---- refinement (mixed precision) -----------------------------------------------------
-// r64 = b - A64 x64 ; rnorm2 = |r64|^2 (bnorm2 too on the first call)
+// r64 = b - A64 x64 -> partials |r|^2, |b|^2
 template <int L, int SUB>
 __global__ void __launch_bounds__(BT) k_residual64(Mat<double> m, const double *__restrict__ b,
                                                    const double *__restrict__ x, double *__restrict__ r,
-                                                   Scal *__restrict__ sc, double *__restrict__ part) {
+                                                   double *__restrict__ part) {
   double acc[2] = {0.0, 0.0};
   auto op = [&](int64_t row, double ax) {
     const double rv = b[row] - ax;
@@ -328,32 +397,107 @@ __global__ void __launch_bounds__(BT) k_residual64(Mat<double> m, const double *
     acc[1] += b[row] * b[row];
   };
   for_rows<double, L, SUB>(m, pfdev::PlainX<double>{x}, op);
-  block_partials<2>(acc, part);
-  if (last_block(&sc->arrive[0])) {
-    double t[2];
-    final_sums<2>(part, t);
-    if (threadIdx.x == 0) {
-      sc->rnorm2 = t[0];
-      sc->bnorm2 = t[1];
-      sc->arrive[0] = 0;
-    }
+  write_partials<2>(acc, part);
+}
+
+__global__ void __launch_bounds__(BT) k_residual_finish(const double *__restrict__ part, int G, Scal *__restrict__ sc) {
+  double t[2];
+  read_partials<2>(part, G, t);
+  if (threadIdx.x == 0) {
+    sc->rnorm2 = t[0];
+    sc->bnorm2 = t[1];
   }
 }
 
-// inner right-hand side: (S r) / |r| in FP32, S = diag(scale) or identity
-__global__ void k_scale_to_f32(int64_t n, const double *__restrict__ r, const Scal *__restrict__ sc,
-                               const double *__restrict__ scale, float *__restrict__ r32) {
-  const double s = 1.0 / sqrt(sc->rnorm2);
+// ---- mixed precision: FP32 BiCGSTAB on S A S with FP64 reliable updates -------------
+// The FP32 recurrence solves (S A S) z = S b, S = diag(scale).  The solution is kept as
+// z_hi (FP64) + z_lo (FP32); whenever the FP32 residual norm has dropped by `delta` since
+// its last maximum, z_lo is folded into z_hi and the recursive residual is replaced by the
+// FP64 true residual S (b - A S z_hi).  This is iterative refinement without restarting
+// the Krylov process; x = S (z_hi + z_lo).
+struct ScaledX {
+  const double *__restrict__ scale;
+  const double *__restrict__ z;
+  __device__ double operator()(int64_t j) const { return scale[j] * z[j]; }
+};
+
+__global__ void k_rel_setz(int64_t n, const double *__restrict__ x, const double *__restrict__ scale,
+                           double *__restrict__ zhi) {
   for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT)
-    r32[i] = (float)(r[i] * s * (scale ? scale[i] : 1.0));
+    zhi[i] = x[i] / scale[i];
 }
 
-// x += |r| S d  (FP64 solution update)
-__global__ void k_correct64(int64_t n, double *__restrict__ x, const float *__restrict__ d,
-                            const Scal *__restrict__ sc, const double *__restrict__ scale) {
-  const double s = sqrt(sc->rnorm2);
+// r' = S (b - A S z_hi) in FP64, stored as FP32 r (INIT: also r0 = p = r, z_lo = 0)
+// partials: |r'|^2, |b - A x|^2, r0.r', |b|^2
+template <int L, int SUB, bool INIT>
+__global__ void __launch_bounds__(BT) k_rel_residual(Mat<double> m, const double *__restrict__ b,
+                                                     const double *__restrict__ scale, const double *__restrict__ zhi,
+                                                     float *__restrict__ r, float *__restrict__ r0,
+                                                     float *__restrict__ p, float *__restrict__ zlo,
+                                                     const Scal *__restrict__ sc, double *__restrict__ part) {
+  if (!INIT && !sc->pending) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  auto op = [&](int64_t row, double ax) {
+    const double bu = b[row] - ax;
+    const double rp = scale[row] * bu;
+    const float rf = (float)rp;
+    r[row] = rf;
+    if (INIT) {
+      r0[row] = rf;
+      p[row] = rf;
+      zlo[row] = 0.0f;
+      acc[2] += (double)rf * (double)rf;
+      acc[3] += b[row] * b[row];
+    } else {
+      acc[2] += (double)r0[row] * (double)rf;
+    }
+    acc[0] += (double)rf * (double)rf;
+    acc[1] += bu * bu;
+  };
+  for_rows<double, L, SUB>(m, ScaledX{scale, zhi}, op);
+  write_partials<4>(acc, part);
+}
+
+// one block: new c1 and |r|, FP64 relative residual, convergence on the true residual
+template <bool INIT>
+__global__ void __launch_bounds__(BT) k_rel_finish(const double *__restrict__ part, int G, Scal *__restrict__ sc) {
+  if (!INIT && !sc->pending) return;
+  double t[4];
+  read_partials<4>(part, G, t);
+  if (threadIdx.x != 0) return;
+  sc->itr_rep = INIT ? 0 : sc->itr_last;
+  if (INIT) {
+    sc->bnorm2 = t[3];
+    sc->itr[0] = sc->itr[1] = 0;
+    sc->itr_last = 0;
+    sc->nupd = 0;
+    sc->done = sc->itrmax <= 0 ? 1 : 0;
+    sc->status = PF_NOT_CONVERGED;
+  } else {
+    sc->nupd += 1;
+  }
+  const double bn = sqrt(sc->bnorm2);
+  const double relres = bn > 0.0 ? sqrt(t[1]) / bn : sqrt(t[1]);
+  sc->relres = relres;
+  sc->c1 = t[2];
+  sc->rr = t[0];
+  sc->maxr[0] = sc->maxr[1] = sqrt(t[0]);
+  sc->pending = 0;
+  if (!(relres >= sc->target)) {  // converged, or NaN
+    sc->status = isfinite(relres) ? PF_OK : PF_BREAKDOWN_C2C3;
+    sc->done = 1;
+  }
+}
+
+// x = S (z_hi + z_lo)
+__global__ void k_rel_final(int64_t n, const double *__restrict__ zhi, const float *__restrict__ zlo,
+                            const double *__restrict__ scale, double *__restrict__ x) {
   for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT)
-    x[i] = x[i] + s * (double)d[i] * (scale ? scale[i] : 1.0);
+    x[i] = scale[i] * (zhi[i] + (double)zlo[i]);
+}
+
+__global__ void k_ones(int64_t n, double *__restrict__ a) {
+  for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) a[i] = 1.0;
 }
 
 __global__ void k_f64_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
@@ -361,13 +505,19 @@ __global__ void k_f64_to_f32(int64_t n, const double *__restrict__ a, float *__r
     b[i] = (float)a[i];
 }
 
+__global__ void k_f32_to_f64(int64_t n, const float *__restrict__ a, double *__restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
 }  // namespace
 
 // ========================================================================================
 struct SolverWork {
   int64_t n = 0;
-  DevBuf<double> r, r0, p, y, v, rref;  // FP64 recurrence vectors (+ refinement residual)
-  DevBuf<float> x32, b32, r32, r032, p32, y32, v32, val32;
+  DevBuf<double> r, r0, p, y, v, rref;  // FP64 recurrence vectors (+ FP64 residual)
+  DevBuf<double> zhi;                   // mixed: FP64 part of the scaled solution
+  DevBuf<float> x32, r32, r032, p32, y32, v32, val32;
   DevBuf<double> part, hist;
   DevBuf<Scal> sc;  // [0] FP64 / outer, [1] FP32 inner
   Scal *h_sc = nullptr;
@@ -380,7 +530,7 @@ struct SolverWork {
   int block_ok = -1;  // -1 unknown
   const void *key_val = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  uint64_t gkey[2][16] = {};  // everything a captured chunk bakes in
+  uint64_t gkey[2][20] = {};  // everything a captured chunk bakes in
   std::vector<hipEvent_t> evs;  // kernel_timing: 5 per iteration of a chunk
   double t_sum[4] = {0, 0, 0, 0};
   int64_t t_count[4] = {0, 0, 0, 0};
@@ -410,14 +560,18 @@ void pf_solver_work_destroy(SolverWork *w) { delete w; }
 namespace {
 
 struct Launch {
-  unsigned grid;
+  unsigned grid;    // SpMV kernels
+  unsigned grid_e;  // element-wise kernels
   int sub;
   int layout;  // 0 CRS, 1 Sell3
 };
 
+// partial-sum slots of G doubles: K1 (1 dot), K2 (2), K3 (2), init / residual (2)
+constexpr int PART_P1 = 0, PART_P2 = 1, PART_P3 = 3, PART_PI = 5, PART_SLOTS = 7;
+
 unsigned fused_grid(pf_ctx *ctx, int64_t work_threads) {
-  // enough resident waves to cover HBM latency on every CU, capped so the last-block
-  // reduction reads at most a few thousand partials
+  // enough resident waves to cover HBM latency on every CU; the next kernel re-reads one
+  // partial per block, so the grid is capped at 8 blocks per CU
   int64_t cap = (int64_t)ctx->num_cus * 8;
   int64_t b = (work_threads + BT - 1) / BT;
   if (b > cap) b = cap;
@@ -431,8 +585,8 @@ int choose_sub(double avg_row) {
   return sub;
 }
 
-#define PF_DISPATCH_SUB(SUBV, ...)                       \
-  switch (SUBV) {                                        \
+#define PF_DISPATCH_SUB(SUBV, ...)                          \
+  switch (SUBV) {                                           \
     case 4: { constexpr int S_ = 4; __VA_ARGS__; } break;   \
     case 8: { constexpr int S_ = 8; __VA_ARGS__; } break;   \
     case 16: { constexpr int S_ = 16; __VA_ARGS__; } break; \
@@ -440,42 +594,73 @@ int choose_sub(double avg_row) {
     default: { constexpr int S_ = 64; __VA_ARGS__; } break; \
   }
 
+// FP64 side of the mixed solve: matrix, launch shape and vectors of the replacements
+struct Rel {
+  Mat<double> m;
+  Launch L;
+  const double *b, *scale;
+  double *zhi;
+  float *r, *r0, *p, *zlo;
+};
+
+template <bool INIT>
+int enqueue_replacement(pf_ctx *ctx, const Rel &R, Scal *sc, double *part) {
+  hipStream_t st = ctx->stream;
+  if (R.L.layout == 1) {
+    k_rel_residual<1, 64, INIT><<<R.L.grid, BT, 0, st>>>(R.m, R.b, R.scale, R.zhi, R.r, R.r0, R.p, R.zlo, sc, part);
+  } else {
+    PF_DISPATCH_SUB(R.L.sub, (k_rel_residual<0, S_, INIT><<<R.L.grid, BT, 0, st>>>(R.m, R.b, R.scale, R.zhi, R.r,
+                                                                                    R.r0, R.p, R.zlo, sc, part)));
+  }
+  k_rel_finish<INIT><<<1, BT, 0, st>>>(part, (int)R.L.grid, sc);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
 template <typename V>
 int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V> &m, const V *b, V *x, V *r, V *r0, V *p, Scal *sc,
                  double *part, bool zerox) {
   hipStream_t st = ctx->stream;
+  double *pi = part + (size_t)PART_PI * ctx->num_cus * 8;
+  const unsigned g = zerox ? L.grid_e : L.grid;
   if (zerox) {
-    k_init<V, 0, 4, true><<<L.grid, BT, 0, st>>>(m, b, x, r, r0, p, sc, part);
+    k_init<V, 0, 4, true><<<g, BT, 0, st>>>(m, b, x, r, r0, p, pi);
   } else if (L.layout == 1) {
-    k_init<V, 1, 64, false><<<L.grid, BT, 0, st>>>(m, b, x, r, r0, p, sc, part);
+    k_init<V, 1, 64, false><<<g, BT, 0, st>>>(m, b, x, r, r0, p, pi);
   } else {
-    PF_DISPATCH_SUB(L.sub, (k_init<V, 0, S_, false><<<L.grid, BT, 0, st>>>(m, b, x, r, r0, p, sc, part)));
+    PF_DISPATCH_SUB(L.sub, (k_init<V, 0, S_, false><<<g, BT, 0, st>>>(m, b, x, r, r0, p, pi)));
   }
+  k_init_finish<<<1, BT, 0, st>>>(pi, (int)g, sc);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
 
 template <typename V>
 int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V> &m, V *x, V *r, const V *r0, V *p, V *y, V *v,
-                  Scal *sc, double *part, double *hist, int count, hipEvent_t *ev = nullptr) {
+                  Scal *sc, double *part, double *hist, int count, const Rel *rel, hipEvent_t *ev = nullptr) {
   hipStream_t st = ctx->stream;
-  const unsigned ge = fused_grid(ctx, m.n);
+  const size_t G = (size_t)ctx->num_cus * 8;
+  double *p1 = part + PART_P1 * G, *p2 = part + PART_P2 * G, *p3 = part + PART_P3 * G;
+  const int g1 = (int)L.grid, ge = (int)L.grid_e;
   for (int it = 0; it < count; ++it) {
+    const int par = it & 1;
     if (ev) PF_HIP(hipEventRecord(ev[5 * it + 0], st));
     if (L.layout == 1) {
-      k_spmv_p<V, 1, 64><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, part);
+      k_spmv_p<V, 1, 64><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, p1);
       if (ev) PF_HIP(hipEventRecord(ev[5 * it + 1], st));
-      k_spmv_e<V, 1, 64><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, part);
+      k_spmv_e<V, 1, 64><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, p1, g1, p2);
     } else {
-      PF_DISPATCH_SUB(L.sub, (k_spmv_p<V, 0, S_><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, part)));
+      PF_DISPATCH_SUB(L.sub, (k_spmv_p<V, 0, S_><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, p1)));
       if (ev) PF_HIP(hipEventRecord(ev[5 * it + 1], st));
-      PF_DISPATCH_SUB(L.sub, (k_spmv_e<V, 0, S_><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, part)));
+      PF_DISPATCH_SUB(L.sub, (k_spmv_e<V, 0, S_><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, p1, g1, p2)));
     }
     if (ev) PF_HIP(hipEventRecord(ev[5 * it + 2], st));
-    k_update<V><<<ge, BT, 0, st>>>(m.n, x, r, r0, p, y, v, sc, part, hist);
+    k_update<V><<<L.grid_e, BT, 0, st>>>(m.n, x, r, r0, p, y, v, sc, p2, g1, p3);
     if (ev) PF_HIP(hipEventRecord(ev[5 * it + 3], st));
-    k_pupdate<V><<<ge, BT, 0, st>>>(m.n, p, r, y, sc);
+    k_pupdate<V><<<L.grid_e, BT, 0, st>>>(m.n, p, r, y, sc, p3, ge, par, hist, rel ? rel->zhi : nullptr,
+                                          rel ? x : nullptr);
     if (ev) PF_HIP(hipEventRecord(ev[5 * it + 4], st));
+    if (rel) PF_CHECK(enqueue_replacement<false>(ctx, *rel, sc, part + PART_PI * G));
   }
   PF_HIP(hipGetLastError());
   return PF_OK;
@@ -484,24 +669,27 @@ int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V> &m, V *x, V *r, con
 // Run the iteration loop of one BiCGSTAB (scalars slot `slot`) to its stop.
 template <typename V>
 int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V> &m, V *x, V *r, const V *r0,
-             V *p, V *y, V *v, double *hist, const pf_solve_opts &o) {
+             V *p, V *y, V *v, double *hist, const pf_solve_opts &o, const Rel *rel = nullptr) {
   Scal *sc = w->sc.p + slot;
   Scal *hs = w->h_sc + slot;
-  const int chunk = o.check_every > 0 ? o.check_every : 32;
+  int chunk = o.check_every > 0 ? o.check_every : 32;
+  chunk += chunk & 1;  // even: the parity of the iteration counter slots carries across chunks
   const bool timing = o.kernel_timing != 0;
   bool use_graph = o.use_graph != 0 && !timing;
-  if (timing && (int)w->evs.size() < 5 * chunk) {
+  if (timing) {
     while ((int)w->evs.size() < 5 * chunk) {
       hipEvent_t e;
       PF_HIP(hipEventCreate(&e));
       w->evs.push_back(e);
     }
   }
-  uint64_t key[16] = {(uint64_t)L.grid, (uint64_t)L.sub, (uint64_t)L.layout, (uint64_t)chunk,
-                      (uint64_t)m.n, (uint64_t)(uintptr_t)m.val, (uint64_t)(uintptr_t)m.sval,
+  uint64_t key[20] = {(uint64_t)L.grid, (uint64_t)L.grid_e * 131 + (uint64_t)L.sub, (uint64_t)L.layout,
+                      (uint64_t)chunk, (uint64_t)m.n, (uint64_t)(uintptr_t)m.val, (uint64_t)(uintptr_t)m.sval,
                       (uint64_t)(uintptr_t)m.col, (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)r,
                       (uint64_t)(uintptr_t)r0, (uint64_t)(uintptr_t)p, (uint64_t)(uintptr_t)y,
-                      (uint64_t)(uintptr_t)v, (uint64_t)(uintptr_t)w->part.p, (uint64_t)(uintptr_t)hist};
+                      (uint64_t)(uintptr_t)v, (uint64_t)(uintptr_t)w->part.p, (uint64_t)(uintptr_t)hist,
+                      rel ? (uint64_t)(uintptr_t)rel->m.sval : 0, rel ? (uint64_t)(uintptr_t)rel->m.val : 0,
+                      rel ? (uint64_t)(uintptr_t)rel->zhi : 0, rel ? (uint64_t)(uintptr_t)rel->b : 0};
   if (use_graph && (w->gexec[slot] == nullptr || std::memcmp(w->gkey[slot], key, sizeof(key)) != 0)) {
     if (w->gexec[slot]) {
       PF_HIP(hipGraphExecDestroy(w->gexec[slot]));
@@ -509,7 +697,7 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V>
     }
     hipGraph_t graph;
     PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk);
+    int rc = enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel);
     hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
     if (rc != PF_OK) return rc;
     if (ce != hipSuccess) {
@@ -520,41 +708,42 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V>
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) {
       w->gexec[slot] = nullptr;
-      use_graph = false;  // fall back to direct launches of the same kernels
+      use_graph = false;  // direct launches of the same kernels
     } else {
       std::memcpy(w->gkey[slot], key, sizeof(key));
     }
   }
   for (;;) {
-    const int itr_before = hs->itr;
+    const int itr_before = hs->itr_last;
     if (use_graph) {
       PF_HIP(hipGraphLaunch(w->gexec[slot], ctx->stream));
     } else {
-      PF_CHECK(enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk,
+      PF_CHECK(enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel,
                                 timing ? w->evs.data() : nullptr));
     }
     PF_HIP(hipMemcpyAsync(hs, sc, sizeof(Scal), hipMemcpyDeviceToHost, ctx->stream));
     PF_HIP(hipStreamSynchronize(ctx->stream));
     if (timing) {
-      // only iterations that ran (kernels after the stop return at entry)
-      const int live = hs->itr - itr_before;
+      // iterations whose four kernels ran (kernels after a stop return at entry)
+      const int live = hs->itr_last - itr_before;
       for (int it = 0; it < live && it < chunk; ++it)
         for (int k = 0; k < 4; ++k) {
-          if (k == 3 && hs->done && it == live - 1) continue;  // p update skipped after a stop
           float ms = 0.f;
           PF_HIP(hipEventElapsedTime(&ms, w->evs[5 * it + k], w->evs[5 * it + k + 1]));
           w->t_sum[k] += ms;
           w->t_count[k] += 1;
         }
     }
-    if (o.debug) std::printf("BiCGSTAB >> itr %d  sqrt(rr) %.6e\n", hs->itr, std::sqrt(hs->rr));
+    if (o.debug)
+      std::printf("BiCGSTAB >> itr %d  sqrt(rr) %.6e%s\n", hs->itr_last, std::sqrt(hs->rr), rel ? "" : "");
+    if (o.debug && rel) std::printf("            replacements %d  FP64 relres %.6e\n", hs->nupd, hs->relres);
     if (hs->done) break;
   }
   return PF_OK;
 }
 
 int prepare_scalars(pf_ctx *ctx, SolverWork *w, int slot, const pf_solve_opts &o, int itrmax, double rel,
-                    int stop_mode, int hist_cap) {
+                    int stop_mode, int hist_cap, int guard, int stall = 0) {
   Scal h;
   std::memset(&h, 0, sizeof(h));
   h.er0 = o.er0;
@@ -562,9 +751,49 @@ int prepare_scalars(pf_ctx *ctx, SolverWork *w, int slot, const pf_solve_opts &o
   h.itrmax = itrmax;
   h.stop_mode = stop_mode;
   h.hist_cap = hist_cap;
+  h.guard = guard;
+  h.stall = stall;
   w->h_sc[slot] = h;
   PF_HIP(hipMemcpyAsync(w->sc.p + slot, w->h_sc + slot, sizeof(Scal), hipMemcpyHostToDevice, ctx->stream));
   return PF_OK;
+}
+
+// reset cached layouts when the matrix (or its size) changed
+void note_matrix(SolverWork *w, int64_t n, const double *val, bool &matrix_changed) {
+  if (n != w->n) {
+    w->n = n;
+    matrix_changed = true;
+  }
+  if (val != w->key_val) matrix_changed = true;
+  w->key_val = val;
+  if (matrix_changed) {
+    w->have_s64 = w->have_s32 = w->have_v32 = w->have_scale = false;
+    w->block_ok = -1;
+    std::memset(w->gkey, 0, sizeof(w->gkey));
+  }
+}
+
+int check_blocks(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx) {
+  if (w->block_ok < 0) {
+    int ok = 0;
+    PF_CHECK(pf_block_structure_check(ctx, n, row_ptr, col_idx, n / 3, w->flag.p, &ok));
+    w->block_ok = ok;
+  }
+  return PF_OK;
+}
+
+template <typename V>
+Mat<V> make_mat(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const V *val, const Sell3<V> *s) {
+  Mat<V> m{n, row_ptr, col_idx, val, 0, 0, nullptr, nullptr, nullptr, nullptr};
+  if (s) {
+    m.nn = s->nn;
+    m.nslices = s->nslices;
+    m.soff = s->soff.p;
+    m.width = s->width.p;
+    m.col = s->col.p;
+    m.sval = s->val.p;
+  }
+  return m;
 }
 
 }  // namespace
@@ -578,12 +807,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     w->t_count[k] = 0;
   }
   if (n <= 0) return PF_OK;
-  if (n != w->n) {
-    w->n = n;
-    matrix_changed = true;
-  }
-  if (val != w->key_val) matrix_changed = true;
-  w->key_val = val;
+  note_matrix(w, n, val, matrix_changed);
   PF_CHECK(w->r.ensure(n));
   PF_CHECK(w->r0.ensure(n));
   PF_CHECK(w->p.ensure(n));
@@ -591,23 +815,13 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   PF_CHECK(w->v.ensure(n));
   PF_CHECK(w->sc.ensure(2));
   PF_CHECK(w->flag.ensure(1));
-  const int grid_cap = ctx->num_cus * 8;
-  PF_CHECK(w->part.ensure((size_t)grid_cap * 2 + 16));
+  PF_CHECK(w->part.ensure((size_t)ctx->num_cus * 8 * PART_SLOTS + 16));
 
   PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-  if (matrix_changed) {
-    w->have_s64 = w->have_s32 = w->have_v32 = w->have_scale = false;
-    w->block_ok = -1;
-    std::memset(w->gkey, 0, sizeof(w->gkey));
-  }
   const double avg_row = (double)nnz / (double)n;
   int layout = 0;
   if (o.spmv_kind != 1) {
-    if (w->block_ok < 0) {
-      int ok = 0;
-      PF_CHECK(pf_block_structure_check(ctx, n, row_ptr, col_idx, n / 3, w->flag.p, &ok));
-      w->block_ok = ok;
-    }
+    PF_CHECK(check_blocks(ctx, w, n, row_ptr, col_idx));
     layout = w->block_ok ? 1 : 0;
     if (o.spmv_kind == 2 && !layout) {
       pf_set_error("blocked SpMV requested but the CRS has no DOF=3 node-block structure");
@@ -623,7 +837,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   }
   const double *scale = scaled ? w->jscale.p : nullptr;
   if (layout == 1) {
-    if (!mixed && !w->have_s64) {
+    if (!w->have_s64) {  // FP64 products (mixed: the residual replacements)
       PF_CHECK(pf_sell3_build<double>(ctx, n, row_ptr, col_idx, val, w->s64));
       w->have_s64 = true;
     }
@@ -644,105 +858,76 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   PF_HIP(hipEventElapsedTime(&setup_ms, ctx->ev0, ctx->ev1));
   res.setup_ms = setup_ms;
 
-  Mat<double> m64{n, row_ptr, col_idx, val, 0, 0, nullptr, nullptr, nullptr, nullptr};
-  Launch L64{0, choose_sub(avg_row), layout};
-  if (layout == 1 && w->have_s64) {
-    m64.nn = w->s64.nn;
-    m64.nslices = w->s64.nslices;
-    m64.soff = w->s64.soff.p;
-    m64.width = w->s64.width.p;
-    m64.col = w->s64.col.p;
-    m64.sval = w->s64.val.p;
-  }
-  // FP64 matrix for the mixed refinement residual: CRS rows (the blocked copy is FP32 only)
-  Launch Lres{fused_grid(ctx, n * (int64_t)choose_sub(avg_row)), choose_sub(avg_row), 0};
-  L64.grid = fused_grid(ctx, layout == 1 ? ((n / 3 + 63) / 64) * 64 : n * (int64_t)L64.sub);
+  const int sub = choose_sub(avg_row);
+  const int64_t sell_threads = ((n / 3 + 63) / 64) * 64;
+  // FP64 CRS rows: refinement residual and the final true residual
+  Mat<double> mcrs = make_mat<double>(n, row_ptr, col_idx, val, nullptr);
+  Launch Lres{fused_grid(ctx, n * (int64_t)sub), fused_grid(ctx, n), sub, 0};
   res.kind = layout == 1 ? 2 : 1;
+  const size_t G = (size_t)ctx->num_cus * 8;
+  double *pres = w->part.p + PART_PI * G;
 
   PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   if (!mixed) {
-    double *hist = nullptr;
-    PF_CHECK(prepare_scalars(ctx, w, 0, o, o.itrmax, o.relative_er, o.stop_mode == 1 ? 1 : 0, 0));
+    Mat<double> m64 = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
+    Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), fused_grid(ctx, n), sub, layout};
+    const int guard = o.stop_mode == 1 ? 1 : 0;  // mode 0 keeps the reference's unguarded arithmetic
+    PF_CHECK(prepare_scalars(ctx, w, 0, o, o.itrmax, o.relative_er, o.stop_mode == 1 ? 1 : 0, 0, guard));
     PF_CHECK(enqueue_init<double>(ctx, L64, m64, b, x, w->r.p, w->r0.p, w->p.p, w->sc.p, w->part.p, false));
-    PF_CHECK(run_loop<double>(ctx, w, 0, L64, m64, x, w->r.p, w->r0.p, w->p.p, w->y.p, w->v.p, hist, o));
+    PF_CHECK(run_loop<double>(ctx, w, 0, L64, m64, x, w->r.p, w->r0.p, w->p.p, w->y.p, w->v.p, nullptr, o));
     const Scal &h = w->h_sc[0];
     res.status = h.status;
     // the reference's loop counter: a v.v == 0 stop happens inside iteration itr+1
-    res.iterations = h.itr + (h.status == PF_BREAKDOWN_VV ? 1 : 0);
+    res.iterations = h.itr_last + (h.status == PF_BREAKDOWN_VV ? 1 : 0);
     res.rr = h.rr;
     res.rr1 = h.rr1;
   } else {
-    PF_CHECK(w->rref.ensure(n));
+    // FP32 BiCGSTAB on S A S (S = Jacobi scaling or identity) with FP64 reliable updates
+    PF_CHECK(w->zhi.ensure(n));
     PF_CHECK(w->x32.ensure(n));
-    PF_CHECK(w->b32.ensure(n));
     PF_CHECK(w->r32.ensure(n));
     PF_CHECK(w->r032.ensure(n));
     PF_CHECK(w->p32.ensure(n));
     PF_CHECK(w->y32.ensure(n));
     PF_CHECK(w->v32.ensure(n));
-    Mat<float> m32{n, row_ptr, col_idx, w->have_v32 ? w->val32.p : nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr};
-    Launch L32{0, choose_sub(avg_row), layout};
-    if (layout == 1) {
-      m32.nn = w->s32.nn;
-      m32.nslices = w->s32.nslices;
-      m32.soff = w->s32.soff.p;
-      m32.width = w->s32.width.p;
-      m32.col = w->s32.col.p;
-      m32.sval = w->s32.val.p;
-    }
-    L32.grid = fused_grid(ctx, layout == 1 ? ((n / 3 + 63) / 64) * 64 : n * (int64_t)L32.sub);
     const unsigned ge = fused_grid(ctx, n);
-    PF_CHECK(prepare_scalars(ctx, w, 0, o, 0, 0.0, 1, 0));
-    int total_inner = 0, outer = 0, inner_status = PF_OK;
-    double relres = 0.0;
-    const int max_outer = o.max_outer > 0 ? o.max_outer : 50;
-    for (;;) {
-      PF_DISPATCH_SUB(Lres.sub, (k_residual64<0, S_><<<Lres.grid, BT, 0, ctx->stream>>>(
-                                    m64, b, x, w->rref.p, w->sc.p, w->part.p)));
+    if (!scale) {  // unscaled: S = I
+      PF_CHECK(w->jscale.ensure(n));
+      k_ones<<<ge, BT, 0, ctx->stream>>>(n, w->jscale.p);
       PF_HIP(hipGetLastError());
-      PF_HIP(hipMemcpyAsync(w->h_sc, w->sc.p, sizeof(Scal), hipMemcpyDeviceToHost, ctx->stream));
-      PF_HIP(hipStreamSynchronize(ctx->stream));
-      const double bn = std::sqrt(w->h_sc[0].bnorm2);
-      relres = bn > 0 ? std::sqrt(w->h_sc[0].rnorm2) / bn : std::sqrt(w->h_sc[0].rnorm2);
-      if (o.debug) std::printf("refinement %d  relres %.6e  inner %d\n", outer, relres, total_inner);
-      if (!(relres >= o.target_relres) || w->h_sc[0].rnorm2 == 0.0) break;  // also stops on NaN
-      if (outer >= max_outer || total_inner >= o.itrmax) {
-        inner_status = PF_NOT_CONVERGED;
-        break;
-      }
-      k_scale_to_f32<<<ge, BT, 0, ctx->stream>>>(n, w->rref.p, w->sc.p, scale, w->b32.p);
-      PF_HIP(hipGetLastError());
-      // restart cap: a stagnating FP32 recurrence is better restarted from the FP64 residual
-      const int inner_cap = std::min(o.itrmax - total_inner, 20000);
-      PF_CHECK(prepare_scalars(ctx, w, 1, o, inner_cap, o.inner_tol, 1, 0));
-      PF_CHECK(enqueue_init<float>(ctx, L32, m32, w->b32.p, w->x32.p, w->r32.p, w->r032.p, w->p32.p,
-                                   w->sc.p + 1, w->part.p, true));
-      PF_CHECK(run_loop<float>(ctx, w, 1, L32, m32, w->x32.p, w->r32.p, w->r032.p, w->p32.p, w->y32.p, w->v32.p,
-                               nullptr, o));
-      total_inner += w->h_sc[1].itr;
-      const int ist = w->h_sc[1].status;
-      ++outer;
-      if (!std::isfinite(w->h_sc[1].rr)) {
-        // the FP32 recurrence broke down numerically: keep x, report the stop
-        inner_status = PF_BREAKDOWN_C2C3;
-        break;
-      }
-      k_correct64<<<ge, BT, 0, ctx->stream>>>(n, x, w->x32.p, w->sc.p, scale);
-      PF_HIP(hipGetLastError());
-      if (ist == PF_BREAKDOWN_VV || ist == PF_BREAKDOWN_C2C3) {
-        // the correction made so far is kept; a breakdown right after a restart is final
-        if (w->h_sc[1].itr <= 1) {
-          inner_status = ist;
-          break;
-        }
-      }
+      scale = w->jscale.p;
+      w->have_scale = false;
     }
-    res.status = inner_status;
-    res.iterations = total_inner;
-    res.outer = outer;
-    res.rr = w->h_sc[0].rnorm2;
+    Mat<float> m32 = make_mat<float>(n, row_ptr, col_idx, w->have_v32 ? w->val32.p : nullptr,
+                                     layout == 1 ? &w->s32 : nullptr);
+    Launch L32{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), ge, sub, layout};
+    Rel R;
+    R.m = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
+    R.L = Launch{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), ge, sub, layout};
+    R.b = b;
+    R.scale = scale;
+    R.zhi = w->zhi.p;
+    R.r = w->r32.p;
+    R.r0 = w->r032.p;
+    R.p = w->p32.p;
+    R.zlo = w->x32.p;
+    PF_CHECK(prepare_scalars(ctx, w, 1, o, o.itrmax, 0.0, 2, 0, 1, o.inner_max > 0 ? o.inner_max : 0));
+    w->h_sc[1].target = o.target_relres;
+    w->h_sc[1].delta = o.inner_tol > 0.0 && o.inner_tol < 1.0 ? o.inner_tol : 0.1;
+    PF_HIP(hipMemcpyAsync(w->sc.p + 1, w->h_sc + 1, sizeof(Scal), hipMemcpyHostToDevice, ctx->stream));
+    k_rel_setz<<<ge, BT, 0, ctx->stream>>>(n, x, scale, w->zhi.p);
+    PF_HIP(hipGetLastError());
+    PF_CHECK(enqueue_replacement<true>(ctx, R, w->sc.p + 1, w->part.p + PART_PI * G));
+    PF_CHECK(run_loop<float>(ctx, w, 1, L32, m32, w->x32.p, w->r32.p, w->r032.p, w->p32.p, w->y32.p, w->v32.p,
+                             nullptr, o, &R));
+    k_rel_final<<<ge, BT, 0, ctx->stream>>>(n, w->zhi.p, w->x32.p, scale, x);
+    PF_HIP(hipGetLastError());
+    const Scal &hi = w->h_sc[1];
+    res.status = hi.status;
+    res.iterations = hi.itr_last;
+    res.outer = hi.nupd;
+    res.rr = hi.rr;
     res.rr1 = 0.0;
-    res.true_relres = relres;
   }
   PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   PF_HIP(hipEventSynchronize(ctx->ev1));
@@ -753,10 +938,10 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   res.timed = w->t_count[0];
 
   // true residual of the solved system in FP64 (CRS rows, independent of the solve layout)
-  if (!mixed) {
+  {
     PF_CHECK(w->rref.ensure(n));
-    PF_DISPATCH_SUB(Lres.sub, (k_residual64<0, S_><<<Lres.grid, BT, 0, ctx->stream>>>(
-                                  m64, b, x, w->rref.p, w->sc.p, w->part.p)));
+    PF_DISPATCH_SUB(Lres.sub, (k_residual64<0, S_><<<Lres.grid, BT, 0, ctx->stream>>>(mcrs, b, x, w->rref.p, pres)));
+    k_residual_finish<<<1, BT, 0, ctx->stream>>>(pres, (int)Lres.grid, w->sc.p);
     PF_HIP(hipGetLastError());
     PF_HIP(hipMemcpyAsync(w->h_sc, w->sc.p, sizeof(Scal), hipMemcpyDeviceToHost, ctx->stream));
     PF_HIP(hipStreamSynchronize(ctx->stream));
@@ -765,13 +950,6 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   }
   return PF_OK;
 }
-
-namespace {
-__global__ void k_f32_to_f64(int64_t n, const float *__restrict__ a, double *__restrict__ b) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = (double)a[i];
-}
-}  // namespace
 
 int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                  const double *val, int64_t nnz, bool matrix_changed, int kind, const double *x, double *y,
@@ -782,24 +960,10 @@ int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, 
     pf_set_error("spmv kind must be 1, 2 or 3");
     return PF_INVALID;
   }
-  if (n != w->n) {
-    w->n = n;
-    matrix_changed = true;
-  }
-  if (val != w->key_val) matrix_changed = true;
-  w->key_val = val;
+  note_matrix(w, n, val, matrix_changed);
   PF_CHECK(w->flag.ensure(1));
-  if (matrix_changed) {
-    w->have_s64 = w->have_s32 = w->have_v32 = w->have_scale = false;
-    w->block_ok = -1;
-    std::memset(w->gkey, 0, sizeof(w->gkey));
-  }
   if (kind != 1) {
-    if (w->block_ok < 0) {
-      int ok = 0;
-      PF_CHECK(pf_block_structure_check(ctx, n, row_ptr, col_idx, n / 3, w->flag.p, &ok));
-      w->block_ok = ok;
-    }
+    PF_CHECK(check_blocks(ctx, w, n, row_ptr, col_idx));
     if (!w->block_ok) {
       pf_set_error("blocked SpMV requested but the CRS has no DOF=3 node-block structure");
       return PF_INVALID;

@@ -111,9 +111,9 @@ extern "C" void pf_solve_opts_default(pf_solve_opts *o) {
   o->precision = 64;
   o->er0 = (double)1.0e-10f;  // dble(1.0e-10): REAL(4) literal (FEMSolverClass.f90:70-71)
   o->relative_er = (double)1.0e-10f;
-  o->inner_tol = 1e-5;
+  o->inner_tol = 0.1;
   o->target_relres = 1e-8;
-  o->max_outer = 50;
+  o->max_outer = 0;
   o->spmv_kind = 0;
   o->check_every = 32;
   o->use_graph = 1;
@@ -121,6 +121,8 @@ extern "C" void pf_solve_opts_default(pf_solve_opts *o) {
   o->stop_mode = 0;
   o->kernel_timing = 0;
   o->jacobi = 1;
+  o->inner_max = 256;
+  o->reserved = 0;
 }
 
 // =============================================================================== context

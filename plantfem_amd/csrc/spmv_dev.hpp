@@ -9,10 +9,20 @@
 //    a wavefront per node, 64 nodes per slice, one int32 node index per 3x3 block and the
 //    nine values of a block component-major across the 64 lanes, so every load of the
 //    inner loop is a contiguous 64-lane access and no cross-lane reduction is needed.
+// Matrix values and indices are read once per product and never reused, so they are
+// loaded non-temporally; this keeps the gathered vector entries resident in L2/MALL
+// (measured on MI355X: 5.6 -> 6.1 TB/s for the FP64 blocked product at 100^3).
+// Padding slots hold zero values and index -1; they are read like live slots (no branch
+// before the value loads) and multiply x[0].
 #pragma once
 #include "pf_internal.hpp"
 
 namespace pfdev {
+
+template <typename T>
+__device__ inline T ldnt(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
 
 template <typename V, int SUB>
 __device__ inline V subgroup_sum(V v) {
@@ -27,7 +37,7 @@ __device__ inline A crs_row(const int64_t *__restrict__ row_ptr, const int32_t *
                             const V *__restrict__ val, int64_t row, int sub, XF xf) {
   const int64_t k0 = row_ptr[row] - 1, k1 = row_ptr[row + 1] - 1;
   A acc = 0;
-  for (int64_t k = k0 + sub; k < k1; k += SUB) acc += (A)val[k] * (A)xf((int64_t)col_idx[k] - 1);
+  for (int64_t k = k0 + sub; k < k1; k += SUB) acc += (A)ldnt(val + k) * (A)xf((int64_t)ldnt(col_idx + k) - 1);
   return subgroup_sum<A, SUB>(acc);
 }
 
@@ -43,15 +53,16 @@ __device__ inline void sell3_node(const int64_t *__restrict__ soff, const int32_
   const V *vp = val + so * 64 * 9 + lane;
 #pragma unroll 2
   for (int t = 0; t < w; ++t) {
-    const int32_t c = cp[(int64_t)t * 64];
-    if (c >= 0) {
-      const V *v = vp + (int64_t)t * 64 * 9;
-      const int64_t j = 3 * (int64_t)c;
-      const A x0 = (A)xf(j), x1 = (A)xf(j + 1), x2 = (A)xf(j + 2);
-      a0 += (A)v[0 * 64] * x0 + (A)v[1 * 64] * x1 + (A)v[2 * 64] * x2;
-      a1 += (A)v[3 * 64] * x0 + (A)v[4 * 64] * x1 + (A)v[5 * 64] * x2;
-      a2 += (A)v[6 * 64] * x0 + (A)v[7 * 64] * x1 + (A)v[8 * 64] * x2;
-    }
+    const int32_t c = ldnt(cp + (int64_t)t * 64);
+    const V *v = vp + (int64_t)t * 64 * 9;
+    V q[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q[k] = ldnt(v + 64 * k);
+    const int64_t j = 3 * (int64_t)(c < 0 ? 0 : c);
+    const A x0 = (A)xf(j), x1 = (A)xf(j + 1), x2 = (A)xf(j + 2);
+    a0 += (A)q[0] * x0 + (A)q[1] * x1 + (A)q[2] * x2;
+    a1 += (A)q[3] * x0 + (A)q[4] * x1 + (A)q[5] * x2;
+    a2 += (A)q[6] * x0 + (A)q[7] * x1 + (A)q[8] * x2;
   }
   y0 = a0;
   y1 = a1;

@@ -1,0 +1,741 @@
+! plantfem_gpu -- Fortran FEMDomain_/FEMSolver_ surface of plantFEM's static linear-elastic
+! path, calling the gfx950 library libpfgpu.so through iso_c_binding (include/pfgpu.h).
+!
+! The type-bound names and argument meanings follow the reference:
+!   FEMDomain_%create("Cube3D",x_num,y_num,z_num,x_len,y_len,z_len)  FEMDomainClass.f90:7215-7310
+!   FEMDomain_%ZeroMatrix(DOF)                                      FEMDomainClass.f90:10581-10840
+!   FEMDomain_%StiffnessMatrix(ElementID,E,v)                       FEMDomainClass.f90:11457-11515
+!   FEMDomain_%StiffnessMatrix(YoungModulus,PoissonRatio)           FEMDomainClass.f90:11123-11232
+!   FEMDomain_%select(x_min,...,z_max)                              FEMDomainClass.f90:12854-12926
+!   FEMSolver_%init/setDomain/setCRS/setRHS/setMatrix/setVector/fix/solve
+!                                                                   FEMSolverClass.f90:84-163
+! plus pf_gpu_bicgstab, a subroutine with the exact LinearSolver interface of
+! solveFEMSolver_UserDefinedLinearSolver (FEMSolverClass.f90:1638-1648), so the reference's own
+! FEMSolver_ can hand its CRS system to the GPU: x = solver%solve(LinearSolver=pf_gpu_bicgstab).
+module plantfem_gpu
+   use iso_c_binding
+   use iso_fortran_env, only: int32, int64, real64, error_unit
+   implicit none
+   private
+
+   public :: Mesh_, CRS_, FEMDomain_, FEMSolver_, pf_solve_opts, pf_stats
+   public :: pf_gpu_bicgstab, pf_gpu_bicgstab_func, pf_gpu_spmv, pf_gpu_version
+
+   integer(c_int), parameter, public :: PF_OK = 0, PF_BREAKDOWN_VV = 1, PF_BREAKDOWN_C2C3 = 2, &
+                                        PF_NOT_CONVERGED = 3, PF_INVALID = 4
+
+   type, bind(C) :: pf_solve_opts
+      integer(c_int32_t) :: itrmax, precision
+      real(c_double) :: er0, relative_er, inner_tol, target_relres
+      integer(c_int32_t) :: max_outer, spmv_kind, check_every, use_graph, debug, stop_mode, &
+                            kernel_timing, jacobi, inner_max, reserved
+   end type
+
+   type, bind(C) :: pf_stats
+      integer(c_int32_t) :: status, iterations, outer_iterations, spmv_kind_used
+      real(c_double) :: rr, rr1, true_relres, solve_ms, setup_ms, spmv_p_ms, spmv_e_ms, update_ms, pupdate_ms
+      integer(c_int64_t) :: timed_iterations
+   end type
+
+   interface
+      subroutine c_pf_solve_opts_default(o) bind(C, name="pf_solve_opts_default")
+         import :: pf_solve_opts
+         type(pf_solve_opts), intent(inout) :: o
+      end subroutine
+      function c_pf_last_error() bind(C, name="pf_last_error")
+         import :: c_ptr
+         type(c_ptr) :: c_pf_last_error
+      end function
+      integer(c_int) function c_pf_version() bind(C, name="pf_version")
+         import :: c_int
+      end function
+      integer(c_int) function c_pf_ctx_create(device, ctx) bind(C, name="pf_ctx_create")
+         import :: c_int, c_ptr
+         integer(c_int), value :: device
+         type(c_ptr), intent(out) :: ctx
+      end function
+      integer(c_int) function c_pf_crs_graph(ctx, nn, ne, nne, elemnod, dof, nnz, row_ptr, col_idx) &
+         bind(C, name="pf_crs_graph")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: nn, ne
+         integer(c_int32_t), value :: nne, dof
+         integer(c_int32_t), intent(in) :: elemnod(*)
+         integer(c_int64_t), intent(out) :: nnz
+         type(c_ptr), value :: row_ptr, col_idx
+      end function
+      integer(c_int) function c_pf_hex8_stiffness(ctx, nn, ne, nodcoord, elemnod, E, nu, jmode, first, count, ke) &
+         bind(C, name="pf_hex8_stiffness")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: nn, ne, first, count
+         real(c_double), intent(in) :: nodcoord(*), E(*), nu(*)
+         integer(c_int32_t), intent(in) :: elemnod(*)
+         integer(c_int32_t), value :: jmode
+         real(c_double), intent(out) :: ke(*)
+      end function
+      integer(c_int) function c_pf_hex8_stiffness_crs(ctx, nn, ne, nodcoord, elemnod, E, nu, jmode, row_ptr, &
+                                                      col_idx, val) bind(C, name="pf_hex8_stiffness_crs")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: nn, ne
+         real(c_double), intent(in) :: nodcoord(*), E(*), nu(*)
+         integer(c_int32_t), intent(in) :: elemnod(*), col_idx(*)
+         integer(c_int64_t), intent(in) :: row_ptr(*)
+         integer(c_int32_t), value :: jmode
+         real(c_double), intent(out) :: val(*)
+      end function
+      integer(c_int) function c_pf_spmv_crs(ctx, n, row_ptr, col_idx, val, x, y) bind(C, name="pf_spmv_crs")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: n
+         integer(c_int64_t), intent(in) :: row_ptr(*)
+         integer(c_int32_t), intent(in) :: col_idx(*)
+         real(c_double), intent(in) :: val(*), x(*)
+         real(c_double), intent(out) :: y(*)
+      end function
+      integer(c_int) function c_pf_bicgstab_crs(ctx, n, row_ptr, col_idx, val, rhs, x, opts, stats) &
+         bind(C, name="pf_bicgstab_crs")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double, pf_solve_opts, pf_stats
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: n
+         integer(c_int64_t), intent(in) :: row_ptr(*)
+         integer(c_int32_t), intent(in) :: col_idx(*)
+         real(c_double), intent(in) :: val(*), rhs(*)
+         real(c_double), intent(inout) :: x(*)
+         type(pf_solve_opts), intent(in) :: opts
+         type(pf_stats), intent(out) :: stats
+      end function
+      integer(c_int) function c_pf_system_create(ctx, sys) bind(C, name="pf_system_create")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         type(c_ptr), intent(out) :: sys
+      end function
+      integer(c_int) function c_pf_system_destroy(sys) bind(C, name="pf_system_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: sys
+      end function
+      integer(c_int) function c_pf_system_set_mesh(sys, nn, ne, nne, nodcoord, elemnod) &
+         bind(C, name="pf_system_set_mesh")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: sys
+         integer(c_int64_t), value :: nn, ne
+         integer(c_int32_t), value :: nne
+         real(c_double), intent(in) :: nodcoord(*)
+         integer(c_int32_t), intent(in) :: elemnod(*)
+      end function
+      integer(c_int) function c_pf_system_zero_matrix(sys, dof, nnz) bind(C, name="pf_system_zero_matrix")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t
+         type(c_ptr), value :: sys
+         integer(c_int32_t), value :: dof
+         integer(c_int64_t), intent(out) :: nnz
+      end function
+      integer(c_int) function c_pf_system_set_crs(sys, n, nnz, row_ptr, col_idx, val) bind(C, name="pf_system_set_crs")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: sys
+         integer(c_int64_t), value :: n, nnz
+         integer(c_int64_t), intent(in) :: row_ptr(*)
+         integer(c_int32_t), intent(in) :: col_idx(*)
+         real(c_double), intent(in) :: val(*)
+      end function
+      integer(c_int) function c_pf_system_get_crs(sys, row_ptr, col_idx, val) bind(C, name="pf_system_get_crs")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: sys, row_ptr, col_idx, val
+      end function
+      integer(c_int) function c_pf_system_stiffness(sys, E, nu, jmode) bind(C, name="pf_system_stiffness")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: sys
+         real(c_double), intent(in) :: E(*), nu(*)
+         integer(c_int32_t), value :: jmode
+      end function
+      integer(c_int) function c_pf_system_add_element_matrices(sys, count, ids, ke) &
+         bind(C, name="pf_system_add_element_matrices")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: sys
+         integer(c_int64_t), value :: count
+         integer(c_int32_t), intent(in) :: ids(*)
+         real(c_double), intent(in) :: ke(*)
+      end function
+      integer(c_int) function c_pf_system_add_element_vectors(sys, count, ids, vec) &
+         bind(C, name="pf_system_add_element_vectors")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: sys
+         integer(c_int64_t), value :: count
+         integer(c_int32_t), intent(in) :: ids(*)
+         real(c_double), intent(in) :: vec(*)
+      end function
+      integer(c_int) function c_pf_system_set_rhs(sys, rhs) bind(C, name="pf_system_set_rhs")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: sys
+         real(c_double), intent(in) :: rhs(*)
+      end function
+      integer(c_int) function c_pf_system_get_rhs(sys, rhs) bind(C, name="pf_system_get_rhs")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: sys
+         real(c_double), intent(out) :: rhs(*)
+      end function
+      integer(c_int) function c_pf_system_add_traction(sys, box, pressure) bind(C, name="pf_system_add_traction")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: sys
+         real(c_double), intent(in) :: box(6)
+         real(c_double), value :: pressure
+      end function
+      integer(c_int) function c_pf_system_fix(sys, nids, ids, values, fixvalue) bind(C, name="pf_system_fix")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: sys
+         integer(c_int64_t), value :: nids
+         integer(c_int32_t), intent(in) :: ids(*)
+         type(c_ptr), value :: values
+         real(c_double), value :: fixvalue
+      end function
+      integer(c_int) function c_pf_system_solve(sys, opts, x0, x_out, stats) bind(C, name="pf_system_solve")
+         import :: c_int, c_ptr, pf_solve_opts, pf_stats
+         type(c_ptr), value :: sys
+         type(pf_solve_opts), intent(in) :: opts
+         type(c_ptr), value :: x0, x_out
+         type(pf_stats), intent(out) :: stats
+      end function
+      function c_strlen(s) bind(C, name="strlen")
+         import :: c_ptr, c_size_t
+         type(c_ptr), value :: s
+         integer(c_size_t) :: c_strlen
+      end function
+   end interface
+
+   type :: Mesh_
+      real(real64), allocatable :: NodCoord(:, :)
+      integer(int32), allocatable :: ElemNod(:, :)
+   end type
+
+   type :: CRS_
+      integer(int32), allocatable :: col_idx(:)
+      integer(int64), allocatable :: row_ptr(:)
+      real(real64), allocatable :: val(:)
+   end type
+
+   type :: FEMDomain_
+      type(Mesh_) :: mesh
+   contains
+      procedure :: create => createFEMDomain
+      procedure :: nn => nnFEMDomain
+      procedure :: ne => neFEMDomain
+      procedure :: nne => nneFEMDomain
+      procedure :: nd => ndFEMDomain
+      procedure :: z_min => z_minFEMDomain
+      procedure :: z_max => z_maxFEMDomain
+      procedure :: x_min => x_minFEMDomain
+      procedure :: x_max => x_maxFEMDomain
+      procedure :: select => selectFEMDomain
+      procedure :: ZeroMatrix => ZeroMatrixFEMDomain
+      procedure, pass :: StiffnessMatrixElementFEMDomain
+      procedure, pass :: StiffnessMatrixCRSFEMDomain
+      generic :: StiffnessMatrix => StiffnessMatrixElementFEMDomain, StiffnessMatrixCRSFEMDomain
+   end type
+
+   type :: FEMSolver_
+      type(c_ptr) :: sys = c_null_ptr
+      type(FEMDomain_), pointer :: domain => null()
+      logical :: initialized = .false.
+      logical :: debug = .false.
+      integer(int32) :: itrmax = 100000
+      real(real64) :: er0 = dble(1.0e-10)
+      real(real64) :: relative_er = dble(1.0e-10)
+      integer(int32) :: DOF = 0
+      integer(int64) :: n = 0, nnz = 0
+      ! host copies, filled by getCRS() like the reference's public CRS_* components
+      real(real64), allocatable :: CRS_val(:), CRS_RHS(:), CRS_x(:)
+      integer(int32), allocatable :: CRS_Index_Col(:)
+      integer(int64), allocatable :: CRS_Index_Row(:)
+      type(pf_stats) :: stats
+      ! setMatrix calls staged here and scattered on the device in call order
+      integer(int32), allocatable, private :: pend_ids(:)
+      real(real64), allocatable, private :: pend_mat(:)
+      integer(int64), private :: npend = 0, mat_len = 0
+   contains
+      procedure :: init => initFEMSolver
+      procedure :: setDomain => setDomainFEMSolver
+      procedure, pass :: setCRSFEMSolver
+      procedure, pass :: setCRS_CRSobjFEMSolver
+      generic :: setCRS => setCRSFEMSolver, setCRS_CRSobjFEMSolver
+      procedure :: setRHS => setRHSFEMSolver
+      procedure :: getCRS => getCRSFEMSolver
+      procedure :: setMatrix => setMatrixFEMSolver
+      procedure :: setVector => setVectorFEMSolver
+      procedure :: setStiffness => setStiffnessFEMSolver
+      procedure :: addTraction => addTractionFEMSolver
+      procedure :: fix => fixFEMSolver
+      procedure :: solve => solveFEMSolver
+      procedure :: flush => flushFEMSolver
+      procedure :: remove => removeFEMSolver
+   end type
+
+   type(c_ptr), save :: g_ctx = c_null_ptr
+
+contains
+
+   ! ------------------------------------------------------------------------ plumbing
+   function last_error() result(msg)
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: chars(:)
+      integer(c_size_t) :: k, m
+      p = c_pf_last_error()
+      m = c_strlen(p)
+      call c_f_pointer(p, chars, [m])
+      allocate (character(len=m) :: msg)
+      do k = 1, m
+         msg(k:k) = chars(k)
+      end do
+   end function
+
+   subroutine check(code, where)
+      integer(c_int), intent(in) :: code
+      character(*), intent(in) :: where
+      if (code == PF_OK) return
+      write (error_unit, '(a,a,a,i0,a,a)') "[plantfem_gpu] ", where, " failed (", code, "): ", last_error()
+      error stop 1
+   end subroutine
+
+   function ctx() result(c)
+      type(c_ptr) :: c
+      integer(c_int) :: dev, code
+      character(len=16) :: env
+      integer :: stat, ln
+      if (.not. c_associated(g_ctx)) then
+         dev = 0
+         call get_environment_variable("PF_DEVICE", env, ln, stat)
+         if (stat == 0 .and. ln > 0) read (env, *) dev
+         code = c_pf_ctx_create(dev, g_ctx)
+         call check(code, "pf_ctx_create")
+      end if
+      c = g_ctx
+   end function
+
+   integer function pf_gpu_version()
+      pf_gpu_version = int(c_pf_version())
+   end function
+
+   function default_opts() result(o)
+      type(pf_solve_opts) :: o
+      call c_pf_solve_opts_default(o)
+   end function
+
+   ! ------------------------------------------------------------------------ FEMDomain_
+   ! Cube3D numbering (MeshClass.f90:6745-6808, :6992-7090): x fastest, lx/dble(xn)*dble(i-1)
+   subroutine createFEMDomain(this, meshtype, x_num, y_num, z_num, x_len, y_len, z_len)
+      class(FEMDomain_), intent(inout) :: this
+      character(*), intent(in) :: meshtype
+      integer(int32), optional, intent(in) :: x_num, y_num, z_num
+      real(real64), optional, intent(in) :: x_len, y_len, z_len
+      integer(int32) :: xn, yn, zn, i, j, k, n, e, a
+      integer(int64) :: nn2, ne2
+      real(real64) :: lx, ly, lz
+      integer(int32) :: q(4)
+      if (meshtype /= "Cube3D" .and. meshtype /= "Cube") stop "plantfem_gpu: only Cube3D is on this path"
+      xn = 10; yn = 10; zn = 10; lx = 1.0d0; ly = 1.0d0; lz = 1.0d0
+      if (present(x_num)) xn = x_num
+      if (present(y_num)) yn = y_num
+      if (present(z_num)) zn = z_num
+      if (present(x_len)) lx = x_len
+      if (present(y_len)) ly = y_len
+      if (present(z_len)) lz = z_len
+      nn2 = int(xn + 1, int64)*(yn + 1)
+      ne2 = int(xn, int64)*yn
+      if (allocated(this%mesh%NodCoord)) deallocate (this%mesh%NodCoord)
+      if (allocated(this%mesh%ElemNod)) deallocate (this%mesh%ElemNod)
+      allocate (this%mesh%NodCoord(nn2*(zn + 1), 3), this%mesh%ElemNod(ne2*zn, 8))
+      n = 0
+      do k = 1, zn + 1
+         do j = 1, yn + 1
+            do i = 1, xn + 1
+               n = n + 1
+               this%mesh%NodCoord(n, 1) = lx/dble(xn)*dble(i - 1)
+               this%mesh%NodCoord(n, 2) = ly/dble(yn)*dble(j - 1)
+               this%mesh%NodCoord(n, 3) = lz/dble(zn)*dble(k - 1)
+            end do
+         end do
+      end do
+      e = 0
+      do k = 1, zn
+         do j = 1, yn
+            do i = 1, xn
+               e = e + 1
+               q = [i + (j - 1)*(xn + 1), i + 1 + (j - 1)*(xn + 1), xn + 2 + i + (j - 1)*(xn + 1), &
+                    xn + 1 + i + (j - 1)*(xn + 1)]
+               do a = 1, 4
+                  this%mesh%ElemNod(e, a) = q(a) + int(nn2, int32)*(k - 1)
+                  this%mesh%ElemNod(e, a + 4) = q(a) + int(nn2, int32)*k
+               end do
+            end do
+         end do
+      end do
+   end subroutine
+
+   integer(int32) function nnFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      nnFEMDomain = size(this%mesh%NodCoord, 1)
+   end function
+
+   integer(int32) function neFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      neFEMDomain = size(this%mesh%ElemNod, 1)
+   end function
+
+   integer(int32) function nneFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      nneFEMDomain = size(this%mesh%ElemNod, 2)
+   end function
+
+   integer(int32) function ndFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      ndFEMDomain = size(this%mesh%NodCoord, 2)
+   end function
+
+   real(real64) function x_minFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      x_minFEMDomain = minval(this%mesh%NodCoord(:, 1))
+   end function
+
+   real(real64) function x_maxFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      x_maxFEMDomain = maxval(this%mesh%NodCoord(:, 1))
+   end function
+
+   real(real64) function z_minFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      z_minFEMDomain = minval(this%mesh%NodCoord(:, 3))
+   end function
+
+   real(real64) function z_maxFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      z_maxFEMDomain = maxval(this%mesh%NodCoord(:, 3))
+   end function
+
+   ! selectFEMDomain box form with InOrOut's closed box (FEMDomainClass.f90:12854-12926)
+   function selectFEMDomain(this, x_min, x_max, y_min, y_max, z_min, z_max) result(NodeList)
+      class(FEMDomain_), intent(in) :: this
+      real(real64), optional, intent(in) :: x_min, x_max, y_min, y_max, z_min, z_max
+      integer(int32), allocatable :: NodeList(:)
+      real(real64) :: lo(3), hi(3)
+      logical, allocatable :: inside(:)
+      integer(int32) :: i, d, m
+      lo = -1.0d14; hi = 1.0d14
+      if (present(x_min)) lo(1) = x_min
+      if (present(y_min)) lo(2) = y_min
+      if (present(z_min)) lo(3) = z_min
+      if (present(x_max)) hi(1) = x_max
+      if (present(y_max)) hi(2) = y_max
+      if (present(z_max)) hi(3) = z_max
+      allocate (inside(this%nn()))
+      inside = .true.
+      do d = 1, 3
+         inside = inside .and. (lo(d) <= this%mesh%NodCoord(:, d)) .and. (this%mesh%NodCoord(:, d) <= hi(d))
+      end do
+      allocate (NodeList(count(inside)))
+      m = 0
+      do i = 1, this%nn()
+         if (inside(i)) then
+            m = m + 1
+            NodeList(m) = i
+         end if
+      end do
+   end function
+
+   function ZeroMatrixFEMDomain(this, DOF) result(ZeroMatrix)
+      class(FEMDomain_), intent(in) :: this
+      integer(int32), intent(in) :: DOF
+      type(CRS_) :: ZeroMatrix
+      integer(c_int64_t) :: nnz
+      integer(c_int) :: code
+      integer(int64), allocatable, target :: rp(:)
+      integer(int32), allocatable, target :: ci(:)
+      code = c_pf_crs_graph(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), &
+                            int(this%nne(), c_int32_t), this%mesh%ElemNod, int(DOF, c_int32_t), nnz, &
+                            c_null_ptr, c_null_ptr)
+      call check(code, "ZeroMatrix")
+      allocate (rp(int(DOF, int64)*this%nn() + 1), ci(nnz))
+      code = c_pf_crs_graph(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), &
+                            int(this%nne(), c_int32_t), this%mesh%ElemNod, int(DOF, c_int32_t), nnz, &
+                            c_loc(rp), c_loc(ci))
+      call check(code, "ZeroMatrix")
+      call move_alloc(rp, ZeroMatrix%row_ptr)
+      call move_alloc(ci, ZeroMatrix%col_idx)
+      allocate (ZeroMatrix%val(nnz))
+      ZeroMatrix%val(:) = 0.0d0
+   end function
+
+   function StiffnessMatrixElementFEMDomain(this, ElementID, E, v) result(StiffnessMatrix)
+      class(FEMDomain_), intent(in) :: this
+      integer(int32), intent(in) :: ElementID
+      real(real64), intent(in) :: E, v
+      real(real64), allocatable :: StiffnessMatrix(:, :)
+      real(real64), allocatable :: Ea(:), va(:)
+      integer(c_int) :: code
+      allocate (StiffnessMatrix(24, 24), Ea(this%ne()), va(this%ne()))
+      Ea = 0.0d0; va = 0.0d0
+      Ea(ElementID) = E
+      va(ElementID) = v
+      code = c_pf_hex8_stiffness(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), this%mesh%NodCoord, &
+                                 this%mesh%ElemNod, Ea, va, 0_c_int32_t, int(ElementID, c_int64_t), 1_c_int64_t, &
+                                 StiffnessMatrix)
+      call check(code, "StiffnessMatrix(ElementID,E,v)")
+   end function
+
+   function StiffnessMatrixCRSFEMDomain(this, YoungModulus, PoissonRatio) result(StiffnessMatrix)
+      class(FEMDomain_), intent(in) :: this
+      real(real64), intent(in) :: YoungModulus(:), PoissonRatio(:)
+      type(CRS_) :: StiffnessMatrix
+      integer(c_int) :: code
+      StiffnessMatrix = this%ZeroMatrix(DOF=3)
+      code = c_pf_hex8_stiffness_crs(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), &
+                                     this%mesh%NodCoord, this%mesh%ElemNod, YoungModulus, PoissonRatio, 0_c_int32_t, &
+                                     StiffnessMatrix%row_ptr, StiffnessMatrix%col_idx, StiffnessMatrix%val)
+      call check(code, "StiffnessMatrix(YoungModulus,PoissonRatio)")
+   end function
+
+   ! ------------------------------------------------------------------------ FEMSolver_
+   subroutine initFEMSolver(this, NumDomain)
+      class(FEMSolver_), intent(inout) :: this
+      integer(int32), optional, intent(in) :: NumDomain
+      integer(c_int) :: code
+      if (present(NumDomain)) then
+         if (NumDomain /= 1) stop "plantfem_gpu: only the single-domain static path is supported"
+      end if
+      if (.not. c_associated(this%sys)) then
+         code = c_pf_system_create(ctx(), this%sys)
+         call check(code, "init")
+      end if
+      this%initialized = .true.
+   end subroutine
+
+   subroutine setDomainFEMSolver(this, FEMDomain, DomainID)
+      class(FEMSolver_), intent(inout) :: this
+      type(FEMDomain_), target, intent(in) :: FEMDomain
+      integer(int32), optional, intent(in) :: DomainID
+      integer(c_int) :: code
+      if (.not. this%initialized) call this%init(NumDomain=1)
+      if (present(DomainID)) then
+         if (DomainID /= 1) stop "plantfem_gpu: only DomainID=1"
+      end if
+      this%domain => FEMDomain
+      code = c_pf_system_set_mesh(this%sys, int(FEMDomain%nn(), c_int64_t), int(FEMDomain%ne(), c_int64_t), &
+                                  int(FEMDomain%nne(), c_int32_t), FEMDomain%mesh%NodCoord, FEMDomain%mesh%ElemNod)
+      call check(code, "setDomain")
+   end subroutine
+
+   ! setCRS(DOF) single-domain branch (FEMSolverClass.f90:679-694)
+   subroutine setCRSFEMSolver(this, DOF)
+      class(FEMSolver_), intent(inout) :: this
+      integer(int32), intent(in) :: DOF
+      integer(c_int64_t) :: nnz
+      integer(c_int) :: code
+      code = c_pf_system_zero_matrix(this%sys, int(DOF, c_int32_t), nnz)
+      call check(code, "setCRS")
+      this%DOF = DOF
+      this%n = int(DOF, int64)*this%domain%nn()
+      this%nnz = nnz
+      this%npend = 0
+   end subroutine
+
+   ! setCRS(CRS_) (FEMSolverClass.f90:421-429)
+   subroutine setCRS_CRSobjFEMSolver(this, CRS)
+      class(FEMSolver_), intent(inout) :: this
+      type(CRS_), intent(in) :: CRS
+      integer(c_int) :: code
+      if (.not. this%initialized) call this%init(NumDomain=1)
+      code = c_pf_system_set_crs(this%sys, int(size(CRS%row_ptr) - 1, c_int64_t), int(size(CRS%col_idx), c_int64_t), &
+                                 CRS%row_ptr, CRS%col_idx, CRS%val)
+      call check(code, "setCRS(CRS)")
+      this%n = size(CRS%row_ptr) - 1
+      this%nnz = size(CRS%col_idx)
+   end subroutine
+
+   subroutine setRHSFEMSolver(this, RHS)
+      class(FEMSolver_), intent(inout) :: this
+      real(real64), intent(in) :: RHS(:)
+      integer(c_int) :: code
+      code = c_pf_system_set_rhs(this%sys, RHS)
+      call check(code, "setRHS")
+   end subroutine
+
+   subroutine flushFEMSolver(this)
+      class(FEMSolver_), intent(inout) :: this
+      integer(c_int) :: code
+      if (this%npend == 0) return
+      code = c_pf_system_add_element_matrices(this%sys, int(this%npend, c_int64_t), this%pend_ids, this%pend_mat)
+      call check(code, "setMatrix (fill-in request)")
+      this%npend = 0
+   end subroutine
+
+   ! setMatrix(DomainID,ElementID,DOF,Matrix) (FEMSolverClass.f90:901-909, :1020-1057)
+   subroutine setMatrixFEMSolver(this, DomainID, ElementID, DOF, Matrix)
+      class(FEMSolver_), intent(inout) :: this
+      integer(int32), intent(in) :: DomainID, ElementID, DOF
+      real(real64), intent(in) :: Matrix(:, :)
+      integer(int64), parameter :: batch = 8192
+      integer(int64) :: m
+      if (DomainID /= 1) stop "plantfem_gpu: only DomainID=1"
+      if (this%n == 0) call this%setCRS(DOF=DOF)
+      m = size(Matrix, kind=int64)
+      if (.not. allocated(this%pend_ids) .or. this%mat_len /= m) then
+         call this%flush()
+         if (allocated(this%pend_ids)) deallocate (this%pend_ids, this%pend_mat)
+         allocate (this%pend_ids(batch), this%pend_mat(batch*m))
+         this%mat_len = m
+      end if
+      this%npend = this%npend + 1
+      this%pend_ids(this%npend) = ElementID
+      this%pend_mat((this%npend - 1)*m + 1:this%npend*m) = reshape(Matrix, [m])
+      if (this%npend == batch) call this%flush()
+   end subroutine
+
+   subroutine setVectorFEMSolver(this, DomainID, ElementID, DOF, Vector)
+      class(FEMSolver_), intent(inout) :: this
+      integer(int32), intent(in) :: DomainID, ElementID, DOF
+      real(real64), intent(in) :: Vector(:)
+      integer(c_int) :: code
+      integer(int32) :: ids(1)
+      if (DomainID /= 1) stop "plantfem_gpu: only DomainID=1"
+      call this%flush()
+      ids(1) = ElementID
+      code = c_pf_system_add_element_vectors(this%sys, 1_c_int64_t, ids, Vector)
+      call check(code, "setVector")
+   end subroutine
+
+   ! device assembly of StiffnessMatrix(YoungModulus(:),PoissonRatio(:)) into the solver's CRS
+   subroutine setStiffnessFEMSolver(this, YoungModulus, PoissonRatio)
+      class(FEMSolver_), intent(inout) :: this
+      real(real64), intent(in) :: YoungModulus(:), PoissonRatio(:)
+      integer(c_int) :: code
+      call this%flush()
+      code = c_pf_system_stiffness(this%sys, YoungModulus, PoissonRatio, 0_c_int32_t)
+      call check(code, "setStiffness")
+      this%DOF = 3
+      this%n = 3_int64*this%domain%nn()
+   end subroutine
+
+   ! TractionVector_by_elem summed over elements (open box as Range_), added to the RHS
+   subroutine addTractionFEMSolver(this, box, SurfacePressure)
+      class(FEMSolver_), intent(inout) :: this
+      real(real64), intent(in) :: box(6), SurfacePressure
+      integer(c_int) :: code
+      code = c_pf_system_add_traction(this%sys, box, SurfacePressure)
+      call check(code, "addTraction")
+   end subroutine
+
+   ! fix(DomainID,IDs,FixValue|FixValues) (FEMSolverClass.f90:1073-1136)
+   subroutine fixFEMSolver(this, DomainID, IDs, FixValue, FixValues)
+      class(FEMSolver_), intent(inout) :: this
+      integer(int32), optional, intent(in) :: DomainID
+      integer(int32), intent(in) :: IDs(:)
+      real(real64), optional, intent(in) :: FixValue
+      real(real64), optional, target, intent(in) :: FixValues(:)
+      integer(c_int) :: code
+      real(real64) :: fv
+      type(c_ptr) :: pv
+      if (present(DomainID)) then
+         if (DomainID /= 1) stop "plantfem_gpu: only DomainID=1"
+      end if
+      fv = 0.0d0
+      if (present(FixValue)) fv = FixValue
+      pv = c_null_ptr
+      if (present(FixValues)) pv = c_loc(FixValues)
+      code = c_pf_system_fix(this%sys, int(size(IDs), c_int64_t), IDs, pv, fv)
+      call check(code, "fix")
+   end subroutine
+
+   ! solve() (FEMSolverClass.f90:1697-1825): elimination, bicgstab_CRS_2, x_fixed = ubar
+   function solveFEMSolver(this, x0, precision) result(x)
+      class(FEMSolver_), intent(inout) :: this
+      real(real64), optional, target, intent(in) :: x0(:)
+      integer(int32), optional, intent(in) :: precision
+      real(real64), allocatable :: x(:)
+      real(real64), allocatable, target :: xl(:)
+      type(pf_solve_opts) :: o
+      type(c_ptr) :: px0
+      integer(c_int) :: code
+      call this%flush()
+      o = default_opts()
+      o%itrmax = this%itrmax
+      o%er0 = this%er0
+      o%relative_er = this%relative_er
+      o%debug = merge(1, 0, this%debug)
+      if (present(precision)) o%precision = precision
+      allocate (xl(this%n))
+      px0 = c_null_ptr
+      if (present(x0)) px0 = c_loc(x0)
+      code = c_pf_system_solve(this%sys, o, px0, c_loc(xl), this%stats)
+      if (code == PF_BREAKDOWN_VV .or. code == PF_BREAKDOWN_C2C3) stop "Bicgstab devide by zero"
+      if (code /= PF_NOT_CONVERGED) call check(code, "solve")
+      this%CRS_x = xl
+      call move_alloc(xl, x)
+   end function
+
+   ! getCRS: host copies of the device CRS and RHS into the public CRS_* components
+   subroutine getCRSFEMSolver(this)
+      class(FEMSolver_), target, intent(inout) :: this
+      integer(c_int) :: code
+      call this%flush()
+      if (allocated(this%CRS_Index_Row)) deallocate (this%CRS_Index_Row, this%CRS_Index_Col, this%CRS_val)
+      allocate (this%CRS_Index_Row(this%n + 1), this%CRS_Index_Col(this%nnz), this%CRS_val(this%nnz))
+      code = c_pf_system_get_crs(this%sys, c_loc(this%CRS_Index_Row), c_loc(this%CRS_Index_Col), c_loc(this%CRS_val))
+      call check(code, "getCRS")
+      if (allocated(this%CRS_RHS)) deallocate (this%CRS_RHS)
+      allocate (this%CRS_RHS(this%n))
+      code = c_pf_system_get_rhs(this%sys, this%CRS_RHS)
+      call check(code, "getRHS")
+   end subroutine
+
+   subroutine removeFEMSolver(this)
+      class(FEMSolver_), intent(inout) :: this
+      integer(c_int) :: code
+      if (c_associated(this%sys)) then
+         code = c_pf_system_destroy(this%sys)
+         this%sys = c_null_ptr
+      end if
+      this%initialized = .false.
+      this%n = 0
+      this%npend = 0
+   end subroutine
+
+   ! ------------------------------------------------------------------- plug-in solvers
+   ! LinearSolver(row_ptr,col_idx,val,rhs,x) of solveFEMSolver_UserDefinedLinearSolver
+   ! (FEMSolverClass.f90:1638-1648): bicgstab_CRS_2 with the reference's defaults on the GPU.
+   subroutine pf_gpu_bicgstab(row_ptr, col_idx, val, rhs, x)
+      real(real64), intent(in) :: val(:), rhs(:)
+      real(real64), intent(inout) :: x(:)
+      integer(int32), intent(in) :: col_idx(:)
+      integer(int64), intent(in) :: row_ptr(:)
+      type(pf_solve_opts) :: o
+      type(pf_stats) :: st
+      integer(c_int) :: code
+      o = default_opts()
+      code = c_pf_bicgstab_crs(ctx(), int(size(row_ptr) - 1, c_int64_t), row_ptr, col_idx, val, rhs, x, o, st)
+      if (code == PF_BREAKDOWN_VV .or. code == PF_BREAKDOWN_C2C3) stop "Bicgstab devide by zero"
+      if (code /= PF_NOT_CONVERGED) call check(code, "pf_gpu_bicgstab")
+   end subroutine
+
+   ! function form (FEMSolverClass.f90:1674-1683)
+   function pf_gpu_bicgstab_func(row_ptr, col_idx, val, rhs, x0) result(x)
+      real(real64), intent(in) :: val(:), rhs(:)
+      real(real64), intent(in) :: x0(:)
+      real(real64), allocatable :: x(:)
+      integer(int32), intent(in) :: col_idx(:)
+      integer(int64), intent(in) :: row_ptr(:)
+      x = x0
+      call pf_gpu_bicgstab(row_ptr, col_idx, val, rhs, x)
+   end function
+
+   ! sub_crs_matvec (LinearSolverClass.f90:3728-3776): new_vector = A old_vector
+   subroutine pf_gpu_spmv(row_ptr, col_idx, val, old_vector, new_vector)
+      real(real64), intent(in) :: val(:), old_vector(:)
+      integer(int32), intent(in) :: col_idx(:)
+      integer(int64), intent(in) :: row_ptr(:)
+      real(real64), allocatable, intent(inout) :: new_vector(:)
+      integer(c_int) :: code
+      if (.not. allocated(new_vector)) allocate (new_vector(size(row_ptr) - 1))
+      code = c_pf_spmv_crs(ctx(), int(size(row_ptr) - 1, c_int64_t), row_ptr, col_idx, val, old_vector, new_vector)
+      call check(code, "pf_gpu_spmv")
+   end subroutine
+
+end module plantfem_gpu

@@ -40,7 +40,7 @@ def test_options_defaults_without_gpu():
     assert o.itrmax == 100000 and o.precision == 64
     # dble(1.0e-10): the REAL(4) literal widened (FEMSolverClass.f90:70-71)
     assert o.er0 == o.relative_er == float.fromhex("0x1.b7cdfe0000000p-34")
-    assert ctypes.sizeof(L.SolveOpts) == 4 * 2 + 8 * 4 + 4 * 8
+    assert ctypes.sizeof(L.SolveOpts) == 4 * 2 + 8 * 4 + 4 * 10
 
 
 def test_no_oracle_in_product():

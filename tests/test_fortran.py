@@ -1,0 +1,58 @@
+"""The Fortran FEMSolver_ binding (plantfem_gpu module): builds, runs, matches the oracle."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+BIN = ROOT / "plantfem_amd" / "lib" / "bin"
+
+
+def test_fortran_binaries_built_and_link():
+    for name in ("cube_static", "plugin_solver"):
+        assert (BIN / name).exists(), name
+    out = subprocess.run([str(BIN / "cube_static"), "--version"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "plantfem_gpu library version" in out.stdout
+
+
+def _oracle_cube(n):
+    from oracle import oracle as O
+    nod, el = O.cube3d(n, n, n)
+    ne = el.shape[0]
+    E = 1.0e7 * (1.0 + 0.5 * np.sin(np.arange(1, ne + 1, dtype=np.float64)))
+    nu = np.full(ne, 0.3)
+    rp, ci = O.zero_matrix(el, nod.shape[0], 3)
+    val = O.assemble(nod, el, E, nu, rp, ci)
+    rhs = O.traction_box(nod, el, [-1e300, 1e300, -1e300, 1e300, 1 - 1e-9, 1 + 1e-9], -1.0e4)
+    fixed = np.zeros(rhs.size, np.uint8)
+    bottom = np.nonzero(nod[:, 2] <= 0)[0]
+    for d in range(3):
+        fixed[3 * bottom + d] = 1
+    x, st, *_ = O.solve(rp, ci, val, rhs, fixed, np.zeros(rhs.size), relative_er=1e-14, er0=1e-30)
+    assert st == 0
+    return x
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,mode", [(8, "crs"), (5, "elem")])
+def test_fortran_cube_static_matches_oracle(tmp_path, n, mode):
+    out = tmp_path / "x.bin"
+    print("fortran cube_static", n, mode, flush=True)
+    res = subprocess.run([str(BIN / "cube_static"), str(n), str(out), mode], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    x = np.fromfile(out, dtype=np.float64)
+    xo = _oracle_cube(n)
+    assert x.size == xo.size
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
+@pytest.mark.gpu
+def test_fortran_plugin_solver():
+    print("fortran plugin_solver", flush=True)
+    res = subprocess.run([str(BIN / "plugin_solver")], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    x = np.array([float(v) for v in res.stdout.split("x=")[1].split("\n")[0].split()])
+    A = np.array([[2, -1, 0, 0], [-1, 2, -1, 0], [0, 2, 3, 1], [0, 0, 1, 2]], float)
+    assert np.allclose(x, np.linalg.solve(A, [1, 2, 3, 4]), rtol=1e-9, atol=1e-9)
