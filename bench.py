@@ -99,6 +99,7 @@ def main(argv=None):
         for kind in (1, 2, 3):
             _, ms = s.spmv(kind, xr, repeat=5, download=False)
             log(f"profile: spmv kind {kind} {ms:.3f} ms")
+        log("profile: importing torch for the calibration copy")
         import torch
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 512 MiB
         dst = torch.empty_like(src)
