@@ -99,13 +99,19 @@ def main(argv=None):
         for kind in (1, 2, 3):
             _, ms = s.spmv(kind, xr, repeat=5, download=False)
             log(f"profile: spmv kind {kind} {ms:.3f} ms")
-        log("profile: importing torch for the calibration copy")
-        import torch
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 512 MiB
-        dst = torch.empty_like(src)
+        # HBM counter calibration: a plain 512 MiB device-to-device copy through the HIP
+        # runtime's own copy kernel (reads and writes exactly 512 MiB per launch)
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so")
+        nb = 512 * 1024 * 1024
+        src, dst = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(src), C.c_size_t(nb)) == 0 and hip.hipMalloc(C.byref(dst), C.c_size_t(nb)) == 0
+        assert hip.hipMemset(src, 1, C.c_size_t(nb)) == 0
         for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+            assert hip.hipMemcpy(dst, src, C.c_size_t(nb), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(src)
+        hip.hipFree(dst)
         log("profile: 3 x 512 MiB device copies done")
         return
 
@@ -173,8 +179,15 @@ def main(argv=None):
     k1_bytes = spmv_bytes(2, n, nnz) + 8.0 * n  # + r0 read for the fused dot
     k1_ms = float(np.mean([x["spmv_p_ms"] for x in stats]))
     k1_gbs = k1_bytes / (k1_ms * 1e-3) / 1e9
+    # measured HBM bytes per K1 launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_box{a.size}.json"
+    if tfile.exists():
+        tr = json.loads(tfile.read_text())
+        if tr.get("n_dof") == n and tr.get("nnz") == nnz:
+            traffic = tr["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_spmv_p (blocked FP64 y=A p fused with r0.y)", "bytes_per_launch": k1_bytes,
                 "avg_launch_ms": round(k1_ms, 4)}
 

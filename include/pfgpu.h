@@ -210,6 +210,61 @@ typedef struct {
 } pf_system_view;
 int pf_system_view_get(pf_system *sys, pf_system_view *view);
 
+/* ---- batched form: many independent meshes in one system --------------------------
+ * The reference solves organ meshes one at a time, one FEMSolver_ per organ
+ * (Soybean%getDisplacement, SoybeanClass.f90:8417-8533; solve() per organ,
+ * FEMSolverClass.f90:1697-1825).  Here the meshes are concatenated into one pf_system
+ * (node ids of mesh m shifted by its first node, element lists concatenated) so mesh setup,
+ * ZeroMatrix, assembly, tractions and fix run once for all of them; the CRS is then
+ * block-diagonal and every mesh's block equals the reference's CRS of that mesh alone.
+ * pf_system_batch_solve runs bicgstab_CRS_2 for every mesh in one persistent kernel. */
+
+/* Traction boxes per mesh: box m (6 doubles, as pf_system_add_traction) and pressure m act
+ * on the facets of mesh m only (nodes node_off[m] .. node_off[m+1]-1, 0-based). */
+int pf_system_add_tractions(pf_system *sys, int32_t nmesh, const int64_t *node_off, const double *boxes,
+                            const double *pressures);
+
+/* Declare the mesh partition: node_off[0] = 0 < node_off[1] < ... < node_off[nmesh] = nn.
+ * The CRS must be block-diagonal over it (checked).  cost (may be NULL) sets the order in
+ * which meshes are taken from the queue, largest first (default: the mesh's nonzeros). */
+int pf_system_batch_set(pf_system *sys, int32_t nmesh, const int64_t *node_off, const double *cost);
+
+/* Launch shape of the persistent kernel. */
+typedef struct {
+  int32_t layout;        /* 1: blocked Sell3 (64-node slices, DOF=3 node blocks); 0: CRS rows */
+  int32_t sub;           /* layout 0: lanes per CRS row (8, 16 or 32) */
+  int32_t block;         /* threads per workgroup: 256, 512 or 1024 */
+  int32_t blocks_per_cu; /* resident workgroups per CU in the grid (0: the occupancy limit) */
+} pf_batch_tune;
+
+/* Result of one mesh. */
+typedef struct {
+  int32_t status;        /* PF_OK, PF_BREAKDOWN_*, PF_NOT_CONVERGED */
+  int32_t iterations;    /* BiCGSTAB iterations of this mesh */
+  int32_t replacements;  /* precision 32: FP64 residual replacements */
+  int32_t workgroup;     /* workgroup that solved it */
+  double rr, rr1;        /* last r.r and r.r after iteration 1 (precision 64) */
+  double true_relres;    /* FP64 ||b - A x|| / ||b|| of this mesh */
+  double start_ms, end_ms; /* device wall clock, from the start of the first mesh */
+} pf_batch_mesh_stats;
+
+/* Solve every mesh (x0 = 0) with the options' precision (64 or 32), stop rule, itrmax and
+ * tolerances, applying each mesh's Dirichlet elimination first as pf_system_solve does.
+ * tune NULL: the shape chosen by pf_system_batch_autotune, else a default.  The queue
+ * reset and the kernel are captured in one hipGraph (opts->use_graph).  mesh_stats (nmesh
+ * entries) and x_out (n) may be NULL.  Returns PF_OK when every mesh converged, else the
+ * status of the first mesh (in mesh order) that did not. */
+int pf_system_batch_solve(pf_system *sys, const pf_solve_opts *opts, const pf_batch_tune *tune, double *x_out,
+                          pf_batch_mesh_stats *mesh_stats, pf_stats *total);
+
+/* Autotuner: times every launch shape (blocked / CRS with 8, 16, 32 lanes per row; 256,
+ * 512, 1024 threads per workgroup) on this batch with `iters` iterations per mesh, keeps
+ * the fastest for later pf_system_batch_solve calls of the same precision and returns it
+ * in *best; up to max_results (shape, ms) pairs go to tried / tried_ms, *ntried = count.
+ * Overwrites the system's x. */
+int pf_system_batch_autotune(pf_system *sys, const pf_solve_opts *opts, int32_t iters, pf_batch_tune *best,
+                             int32_t max_results, pf_batch_tune *tried, double *tried_ms, int32_t *ntried);
+
 #ifdef __cplusplus
 }
 #endif

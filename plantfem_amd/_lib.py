@@ -84,6 +84,29 @@ class SystemView(C.Structure):
     ]
 
 
+class BatchTune(C.Structure):
+    _fields_ = [("layout", C.c_int32), ("sub", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchMeshStats(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("iterations", C.c_int32), ("replacements", C.c_int32), ("workgroup", C.c_int32),
+        ("rr", C.c_double), ("rr1", C.c_double), ("true_relres", C.c_double),
+        ("start_ms", C.c_double), ("end_ms", C.c_double),
+    ]
+
+
+# numpy view of an array of BatchMeshStats
+BATCH_STATS_DTYPE = np.dtype([
+    ("status", np.int32), ("iterations", np.int32), ("replacements", np.int32), ("workgroup", np.int32),
+    ("rr", np.float64), ("rr1", np.float64), ("true_relres", np.float64),
+    ("start_ms", np.float64), ("end_ms", np.float64),
+])
+
+
 P = C.c_void_p
 I32, I64, F64 = C.c_int32, C.c_int64, C.c_double
 PI32, PI64, PF64, PF32 = (C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double),
@@ -123,6 +146,12 @@ SIGNATURES = {
     "pf_system_get_x": (C.c_int, [P, PF64]),
     "pf_system_view_get": (C.c_int, [P, C.POINTER(SystemView)]),
     "pf_system_spmv": (C.c_int, [P, I32, PF64, PF64, I32, PF64]),
+    "pf_system_add_tractions": (C.c_int, [P, I32, PI64, PF64, PF64]),
+    "pf_system_batch_set": (C.c_int, [P, I32, PI64, PF64]),
+    "pf_system_batch_solve": (C.c_int, [P, C.POINTER(SolveOpts), C.POINTER(BatchTune), PF64, P,
+                                        C.POINTER(Stats)]),
+    "pf_system_batch_autotune": (C.c_int, [P, C.POINTER(SolveOpts), I32, C.POINTER(BatchTune), I32,
+                                           C.POINTER(BatchTune), PF64, PI32]),
 }
 
 

@@ -333,14 +333,29 @@ __device__ inline double tri_area(const double *A, const double *B, const double
 
 // one thread per node: sum, in ascending element then facet order, the quarter facet
 // forces of every facet (fully inside the open box) that holds this node
+// MULTI: node ranges node_off[m] .. node_off[m+1]-1 (0-based) each get their own box and
+// pressure (the meshes of a batch); nodes outside every range are left alone
+template <bool MULTI>
 __global__ void k_traction(int64_t nn, int64_t ne, const double *__restrict__ xyz,
                            const int32_t *__restrict__ en, const int64_t *__restrict__ eoff,
                            const int32_t *__restrict__ elist, double bx0, double bx1, double by0,
-                           double by1, double bz0, double bz1, double pressure,
-                           double *__restrict__ rhs) {
+                           double by1, double bz0, double bz1, double pressure_one, int32_t nbox,
+                           const double *__restrict__ boxes, const double *__restrict__ pressures,
+                           const int64_t *__restrict__ node_off, double *__restrict__ rhs) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nn) return;
-  const double box[6] = {bx0, bx1, by0, by1, bz0, bz1};
+  double box[6] = {bx0, bx1, by0, by1, bz0, bz1};
+  double pressure = pressure_one;
+  if (MULTI) {
+    if (i < node_off[0] || i >= node_off[nbox]) return;
+    int32_t lo = 0, hi = nbox;  // largest m with node_off[m] <= i
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (node_off[mid] <= i) lo = mid; else hi = mid;
+    }
+    for (int k = 0; k < 6; ++k) box[k] = boxes[6 * lo + k];
+    pressure = pressures[lo];
+  }
   double f0 = rhs[3 * i], f1 = rhs[3 * i + 1], f2 = rhs[3 * i + 2];
   const int64_t s0 = eoff[i], s1 = eoff[i + 1];
   for (int64_t s = s0; s < s1; ++s) {
@@ -469,9 +484,23 @@ int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double p
     return PF_INVALID;
   }
   if (g.nn == 0) return PF_OK;
-  k_traction<<<pf_blocks(g.nn, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, g.eoff.p,
-                                                            g.elist.p, box[0], box[1], box[2], box[3],
-                                                            box[4], box[5], pressure, rhs);
+  k_traction<false><<<pf_blocks(g.nn, 128), 128, 0, ctx->stream>>>(
+      g.nn, g.ne, g.nodcoord.p, g.elemnod.p, g.eoff.p, g.elist.p, box[0], box[1], box[2], box[3], box[4], box[5],
+      pressure, 0, nullptr, nullptr, nullptr, rhs);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_traction_boxes(pf_ctx *ctx, const NodeGraph &g, int32_t count, const int64_t *node_off_dev,
+                      const double *boxes_dev, const double *pressures_dev, double *rhs) {
+  if (g.nne != 8 || !g.built || !g.has_coords) {
+    pf_set_error("traction needs an 8-node mesh with coordinates and its node->element lists");
+    return PF_INVALID;
+  }
+  if (g.nn == 0 || count <= 0) return PF_OK;
+  k_traction<true><<<pf_blocks(g.nn, 128), 128, 0, ctx->stream>>>(
+      g.nn, g.ne, g.nodcoord.p, g.elemnod.p, g.eoff.p, g.elist.p, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, count,
+      boxes_dev, pressures_dev, node_off_dev, rhs);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

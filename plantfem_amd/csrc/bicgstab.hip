@@ -566,8 +566,12 @@ struct Launch {
   int layout;  // 0 CRS, 1 Sell3
 };
 
-// partial-sum slots of G doubles: K1 (1 dot), K2 (2), K3 (2), init / residual (2)
-constexpr int PART_P1 = 0, PART_P2 = 1, PART_P3 = 3, PART_PI = 5, PART_SLOTS = 7;
+// partial-sum rows of G doubles each: K1 (1 dot), K2 (2), K3 (2), and one region shared by
+// the init, true-residual and replacement kernels (at most 4 dots: k_rel_residual)
+constexpr int PART_P1 = 0, PART_P2 = 1, PART_P3 = 3, PART_PI = 5, PART_PI_ROWS = 4;
+constexpr int PART_SLOTS = PART_PI + PART_PI_ROWS;
+static_assert(PART_P2 >= PART_P1 + 1 && PART_P3 >= PART_P2 + 2 && PART_PI >= PART_P3 + 2,
+              "partial rows overlap");
 
 unsigned fused_grid(pf_ctx *ctx, int64_t work_threads) {
   // enough resident waves to cover HBM latency on every CU; the next kernel re-reads one
@@ -577,6 +581,17 @@ unsigned fused_grid(pf_ctx *ctx, int64_t work_threads) {
   if (b > cap) b = cap;
   if (b < 1) b = 1;
   return (unsigned)b;
+}
+
+// every partial row holds one double per block of the writing kernel: grids above G would
+// write past w->part
+int check_grid(pf_ctx *ctx, const Launch &L) {
+  const unsigned G = (unsigned)ctx->num_cus * 8;
+  if (L.grid > G || L.grid_e > G || L.grid == 0 || L.grid_e == 0) {
+    pf_set_error("internal: launch grid %u/%u outside 1..%u", L.grid, L.grid_e, G);
+    return PF_INVALID;
+  }
+  return PF_OK;
 }
 
 int choose_sub(double avg_row) {
@@ -605,6 +620,8 @@ struct Rel {
 
 template <bool INIT>
 int enqueue_replacement(pf_ctx *ctx, const Rel &R, Scal *sc, double *part) {
+  static_assert(PART_PI_ROWS >= 4, "k_rel_residual writes 4 partial rows");
+  PF_CHECK(check_grid(ctx, R.L));
   hipStream_t st = ctx->stream;
   if (R.L.layout == 1) {
     k_rel_residual<1, 64, INIT><<<R.L.grid, BT, 0, st>>>(R.m, R.b, R.scale, R.zhi, R.r, R.r0, R.p, R.zlo, sc, part);
@@ -620,6 +637,7 @@ int enqueue_replacement(pf_ctx *ctx, const Rel &R, Scal *sc, double *part) {
 template <typename V>
 int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V> &m, const V *b, V *x, V *r, V *r0, V *p, Scal *sc,
                  double *part, bool zerox) {
+  PF_CHECK(check_grid(ctx, L));
   hipStream_t st = ctx->stream;
   double *pi = part + (size_t)PART_PI * ctx->num_cus * 8;
   const unsigned g = zerox ? L.grid_e : L.grid;
@@ -638,6 +656,7 @@ int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V> &m, const V *b, V *x
 template <typename V>
 int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V> &m, V *x, V *r, const V *r0, V *p, V *y, V *v,
                   Scal *sc, double *part, double *hist, int count, const Rel *rel, hipEvent_t *ev = nullptr) {
+  PF_CHECK(check_grid(ctx, L));
   hipStream_t st = ctx->stream;
   const size_t G = (size_t)ctx->num_cus * 8;
   double *p1 = part + PART_P1 * G, *p2 = part + PART_P2 * G, *p3 = part + PART_P3 * G;
@@ -934,12 +953,19 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   float ms = 0.f;
   PF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   res.solve_ms = ms;
+  if (res.status == PF_BREAKDOWN_VV)
+    pf_set_error("BiCGSTAB: v.v == 0 in iteration %d", res.iterations);
+  else if (res.status == PF_BREAKDOWN_C2C3)
+    pf_set_error("BiCGSTAB: c2*omega == 0 or a non-finite scalar after %d iterations", res.iterations);
+  else if (res.status == PF_NOT_CONVERGED)
+    pf_set_error("BiCGSTAB: not converged after %d iterations", res.iterations);
   for (int k = 0; k < 4; ++k) res.t_ms[k] = w->t_count[k] ? w->t_sum[k] / (double)w->t_count[k] : 0.0;
   res.timed = w->t_count[0];
 
   // true residual of the solved system in FP64 (CRS rows, independent of the solve layout)
   {
     PF_CHECK(w->rref.ensure(n));
+    PF_CHECK(check_grid(ctx, Lres));
     PF_DISPATCH_SUB(Lres.sub, (k_residual64<0, S_><<<Lres.grid, BT, 0, ctx->stream>>>(mcrs, b, x, w->rref.p, pres)));
     k_residual_finish<<<1, BT, 0, ctx->stream>>>(pres, (int)Lres.grid, w->sc.p);
     PF_HIP(hipGetLastError());

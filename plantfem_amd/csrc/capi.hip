@@ -363,8 +363,35 @@ struct pf_system {
   DevBuf<double> sx, sy;  // pf_system_spmv operands
   DevBuf<double> E, nu;   // resident materials
   bool has_mat = false;
-  ~pf_system() { pf_solver_work_destroy(work); }
+  uint64_t mat_version = 1;  // bumped with every change of the CRS values or structure
+  uint64_t rhs_version = 1, fix_version = 1;
+  uint64_t bc_mat = 0, bc_rhs = 0, bc_fix = 0;  // versions the elimination was last applied to
+  BatchWork *batch = nullptr;
+  ~pf_system() {
+    pf_solver_work_destroy(work);
+    pf_batch_work_destroy(batch);
+  }
 };
+
+static inline void mark_changed(pf_system *s) {
+  s->matrix_changed = true;
+  ++s->mat_version;
+}
+
+// Dirichlet elimination (FEMSolverClass.f90:1707-1756) of the current fixes on the current
+// CRS and RHS, skipped when none of them changed since it last ran (it is idempotent on the
+// values it already produced, so repeated solves keep their cached layouts)
+static int apply_bc(pf_system *s) {
+  if (!s->has_fix) return PF_OK;
+  if (s->bc_mat == s->mat_version && s->bc_rhs == s->rhs_version && s->bc_fix == s->fix_version) return PF_OK;
+  const bool matrix_part = s->bc_mat != s->mat_version || s->bc_fix != s->fix_version;
+  PF_CHECK(pf_bc_apply(s->ctx, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->rhs.p, s->fixed.p, s->ubar.p));
+  if (matrix_part) mark_changed(s);
+  s->bc_mat = s->mat_version;
+  s->bc_rhs = s->rhs_version;
+  s->bc_fix = s->fix_version;
+  return PF_OK;
+}
 
 namespace {
 int sys_check(pf_system *s, bool need_crs) {
@@ -388,6 +415,8 @@ int sys_alloc_vectors(pf_system *s, bool zero) {
     PF_HIP(hipMemsetAsync(s->x.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
   }
   s->has_fix = false;
+  ++s->rhs_version;
+  ++s->fix_version;
   return PF_OK;
 }
 }  // namespace
@@ -445,7 +474,7 @@ extern "C" int pf_system_zero_matrix(pf_system *s, int32_t dof, int64_t *nnz) {
   PF_CHECK(sys_alloc_vectors(s, true));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->graph_crs = true;
-  s->matrix_changed = true;
+  mark_changed(s);
   if (nnz) *nnz = s->nnz;
   return PF_OK;
 }
@@ -471,7 +500,7 @@ extern "C" int pf_system_set_crs(pf_system *s, int64_t n, int64_t nnz, const int
   if (resize || !s->rhs.p) PF_CHECK(sys_alloc_vectors(s, true));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->graph_crs = false;
-  s->matrix_changed = true;
+  mark_changed(s);
   return PF_OK;
 }
 
@@ -509,7 +538,7 @@ extern "C" int pf_system_stiffness(pf_system *s, const double *E, const double *
   }
   PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
   PF_HIP(hipStreamSynchronize(ctx->stream));
-  s->matrix_changed = true;
+  mark_changed(s);
   return PF_OK;
 }
 
@@ -545,7 +574,7 @@ extern "C" int pf_system_add_element_matrices(pf_system *s, int64_t count, const
                                err.p));
   int32_t herr = 0;
   PF_CHECK(download(ctx, &herr, err.p, 1));
-  s->matrix_changed = true;
+  mark_changed(s);
   if (herr) {
     pf_set_error("setMatrix: fill-in request, no CRS slot for an element entry");
     return PF_INVALID;
@@ -558,6 +587,7 @@ extern "C" int pf_system_set_rhs(pf_system *s, const double *rhs) {
   DeviceGuard guard(s->ctx);
   PF_CHECK(upload(s->ctx, s->rhs, rhs, (size_t)s->n));
   PF_HIP(hipStreamSynchronize(s->ctx->stream));
+  ++s->rhs_version;
   return PF_OK;
 }
 
@@ -578,6 +608,7 @@ extern "C" int pf_system_add_element_vectors(pf_system *s, int64_t count, const 
   PF_CHECK(upload(ctx, dids, elem_ids, (size_t)count));
   PF_CHECK(upload(ctx, dvec, vec, (size_t)(count * nk)));
   if (!s->g.built) PF_CHECK(pf_graph_node_elems(ctx, s->g));
+  ++s->rhs_version;
   return pf_add_element_vecs(ctx, s->g, s->dof, count, dids.p, dvec.p, s->rhs.p);
 }
 
@@ -598,6 +629,7 @@ extern "C" int pf_system_add_traction(pf_system *s, const double *box, double pr
   if (!s->g.built) PF_CHECK(pf_graph_node_elems(ctx, s->g));
   PF_CHECK(pf_traction_box(ctx, s->g, box, pressure, s->rhs.p));
   PF_HIP(hipStreamSynchronize(ctx->stream));
+  ++s->rhs_version;
   return PF_OK;
 }
 
@@ -614,6 +646,7 @@ extern "C" int pf_system_fix(pf_system *s, int64_t nids, const int32_t *ids, con
     PF_HIP(hipMemsetAsync(s->ubar.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
     s->has_fix = true;
   }
+  ++s->fix_version;
   if (nids == 0) return PF_OK;
   DevBuf<int32_t> dids;
   DevBuf<double> dvals;
@@ -630,10 +663,7 @@ extern "C" int pf_system_solve(pf_system *s, const pf_solve_opts *opts, const do
   pf_solve_opts o;
   if (opts) o = *opts; else pf_solve_opts_default(&o);
   PF_CHECK(ensure_work(ctx, &s->work));
-  if (s->has_fix) {
-    PF_CHECK(pf_bc_apply(ctx, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->rhs.p, s->fixed.p, s->ubar.p));
-    s->matrix_changed = true;
-  }
+  PF_CHECK(apply_bc(s));
   if (x0) {
     PF_CHECK(upload(ctx, s->x, x0, (size_t)s->n));
   } else {
@@ -689,4 +719,76 @@ extern "C" int pf_system_spmv(pf_system *s, int32_t kind, const double *x, doubl
   s->matrix_changed = false;
   if (y) PF_CHECK(download(ctx, y, s->sy.p, (size_t)s->n));
   return PF_OK;
+}
+
+// ============================================================================ batch
+extern "C" int pf_system_add_tractions(pf_system *s, int32_t nmesh, const int64_t *node_off, const double *boxes,
+                                       const double *pressures) {
+  PF_CHECK(sys_check(s, true));
+  if (!s->has_mesh || s->g.nne != 8 || s->dof != 3 || nmesh < 0 || (nmesh > 0 && (!node_off || !boxes || !pressures))) {
+    pf_set_error("pf_system_add_tractions: hex8 mesh with a DOF=3 CRS, node offsets, boxes and pressures required");
+    return PF_INVALID;
+  }
+  if (nmesh == 0) return PF_OK;
+  for (int32_t m = 0; m < nmesh; ++m)
+    if (node_off[m] < 0 || node_off[m + 1] < node_off[m] || node_off[m + 1] > s->g.nn) {
+      pf_set_error("pf_system_add_tractions: node offsets must be non-decreasing within 0..%lld", (long long)s->g.nn);
+      return PF_INVALID;
+    }
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  if (!s->g.built) PF_CHECK(pf_graph_node_elems(ctx, s->g));
+  DevBuf<int64_t> doff;
+  DevBuf<double> dbox, dp;
+  PF_CHECK(upload(ctx, doff, node_off, (size_t)nmesh + 1));
+  PF_CHECK(upload(ctx, dbox, boxes, (size_t)nmesh * 6));
+  PF_CHECK(upload(ctx, dp, pressures, (size_t)nmesh));
+  PF_CHECK(pf_traction_boxes(ctx, s->g, nmesh, doff.p, dbox.p, dp.p, s->rhs.p));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  ++s->rhs_version;
+  return PF_OK;
+}
+
+extern "C" int pf_system_batch_set(pf_system *s, int32_t nmesh, const int64_t *node_off, const double *cost) {
+  PF_CHECK(sys_check(s, true));
+  DeviceGuard guard(s->ctx);
+  return pf_batch_configure(s->ctx, &s->batch, s->n, s->row_ptr.p, s->col_idx.p, nmesh, node_off, cost);
+}
+
+static int batch_prepare(pf_system *s) {
+  if (!s->batch) {
+    pf_set_error("batch: call pf_system_batch_set first");
+    return PF_INVALID;
+  }
+  return apply_bc(s);
+}
+
+extern "C" int pf_system_batch_solve(pf_system *s, const pf_solve_opts *opts, const pf_batch_tune *tune,
+                                     double *x_out, pf_batch_mesh_stats *mesh_stats, pf_stats *total) {
+  PF_CHECK(sys_check(s, true));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  pf_solve_opts o;
+  if (opts) o = *opts; else pf_solve_opts_default(&o);
+  PF_CHECK(batch_prepare(s));
+  const int rc = pf_batch_solve_device(ctx, s->batch, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->rhs.p,
+                                       s->x.p, o, tune, s->mat_version, mesh_stats, total);
+  if (rc < 0 || rc == PF_INVALID) return rc;
+  if (s->has_fix) PF_CHECK(pf_bc_set_fixed_x(ctx, s->n, s->fixed.p, s->ubar.p, s->x.p));
+  if (x_out) PF_CHECK(download(ctx, x_out, s->x.p, (size_t)s->n));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return rc;
+}
+
+extern "C" int pf_system_batch_autotune(pf_system *s, const pf_solve_opts *opts, int32_t iters, pf_batch_tune *best,
+                                        int32_t max_results, pf_batch_tune *tried, double *tried_ms,
+                                        int32_t *ntried) {
+  PF_CHECK(sys_check(s, true));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  pf_solve_opts o;
+  if (opts) o = *opts; else pf_solve_opts_default(&o);
+  PF_CHECK(batch_prepare(s));
+  return pf_batch_autotune_device(ctx, s->batch, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->rhs.p, s->x.p, o,
+                                  iters, s->mat_version, best, max_results, tried, tried_ms, ntried);
 }

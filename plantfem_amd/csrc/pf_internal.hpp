@@ -98,6 +98,15 @@ struct Sell3 {
   DevBuf<int32_t> width;                   // nslices
   DevBuf<int32_t> col;                     // nslots*64 padded node ids (-1 = pad)
   DevBuf<V> val;                           // nslots*64*9
+  // slices of a block-diagonal batch never cross a mesh boundary: slice s then covers
+  // nodes snode[s] .. send[s]-1 (device arrays owned by the caller); null = s*64 .. s*64+63
+  const int64_t *snode = nullptr, *send = nullptr;
+};
+
+// slice -> node map of a batched Sell3 (device arrays, nslices entries each)
+struct SliceMap {
+  int64_t nslices = 0;
+  const int64_t *snode = nullptr, *send = nullptr;
 };
 
 // ---------------------------------------------------------------------------------
@@ -126,6 +135,9 @@ int pf_add_element_mats(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count,
 int pf_add_element_vecs(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count, const int32_t *ids,
                         const double *vec, double *rhs);
 int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double pressure, double *rhs);
+// one box and pressure per node range node_off[m] .. node_off[m+1]-1 (device arrays)
+int pf_traction_boxes(pf_ctx *ctx, const NodeGraph &g, int32_t count, const int64_t *node_off_dev,
+                      const double *boxes_dev, const double *pressures_dev, double *rhs);
 
 // bc.hip
 int pf_bc_apply(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
@@ -143,7 +155,8 @@ int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, con
 // scale (may be NULL): values stored as s_i a_ij s_j
 template <typename V>
 int pf_sell3_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
-                   const double *val, Sell3<V> &s, const double *scale = nullptr);
+                   const double *val, Sell3<V> &s, const double *scale = nullptr,
+                   const SliceMap *map = nullptr);
 template <typename V>
 int pf_sell3_values(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const double *val, Sell3<V> &s,
                     const double *scale = nullptr);
@@ -174,6 +187,22 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
 int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                  const double *val, int64_t nnz, bool matrix_changed, int kind, const double *x, double *y,
                  int repeat, double *ms);
+
+// batch.hip: many independent meshes of one block-diagonal system, one persistent kernel
+struct BatchWork;
+void pf_batch_work_destroy(BatchWork *b);
+// node_off (host, nmesh+1): mesh m owns nodes node_off[m] .. node_off[m+1]-1; cost (host,
+// may be NULL) orders the queue, largest first (default: the mesh's nonzeros)
+int pf_batch_configure(pf_ctx *ctx, BatchWork **b, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                       int32_t nmesh, const int64_t *node_off, const double *cost);
+int pf_batch_solve_device(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, const int32_t *col_idx,
+                          const double *val, int64_t nnz, const double *rhs, double *x, const pf_solve_opts &o,
+                          const pf_batch_tune *tune, uint64_t mat_version, pf_batch_mesh_stats *mesh_stats,
+                          pf_stats *total);
+int pf_batch_autotune_device(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, const int32_t *col_idx,
+                             const double *val, int64_t nnz, const double *rhs, double *x, const pf_solve_opts &o,
+                             int32_t iters, uint64_t mat_version, pf_batch_tune *best, int32_t max_results,
+                             pf_batch_tune *tried, double *tried_ms, int32_t *ntried);
 
 // common launch geometry
 static inline unsigned pf_blocks(int64_t work, int threads, int64_t cap = 1 << 20) {

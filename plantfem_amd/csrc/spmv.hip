@@ -58,14 +58,24 @@ __global__ void k_block_check(int64_t nn, const int64_t *__restrict__ row_ptr,
   if (!ok) atomicOr(bad, 1);
 }
 
-__global__ void k_sell3_width(int64_t nn, int64_t nslices, const int64_t *__restrict__ row_ptr,
+// first node and end node of slice s (plain: 64 consecutive nodes; mapped: a batch slice)
+struct SliceNodes {
+  int64_t nn;
+  const int64_t *snode, *send;
+  __device__ int64_t first(int64_t s) const { return snode ? snode[s] : s * 64; }
+  __device__ int64_t end(int64_t s) const {
+    if (snode) return send[s];
+    return s * 64 + 64 < nn ? s * 64 + 64 : nn;
+  }
+};
+
+__global__ void k_sell3_width(SliceNodes sn, int64_t nslices, const int64_t *__restrict__ row_ptr,
                               int32_t *__restrict__ width) {
   const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (s >= nslices) return;
   int32_t w = 0;
-  for (int l = 0; l < 64; ++l) {
-    const int64_t i = s * 64 + l;
-    if (i >= nn) break;
+  const int64_t e = sn.end(s);
+  for (int64_t i = sn.first(s); i < e; ++i) {
     const int32_t dg = (int32_t)((row_ptr[3 * i + 1] - row_ptr[3 * i]) / 3);
     w = dg > w ? dg : w;
   }
@@ -73,7 +83,7 @@ __global__ void k_sell3_width(int64_t nn, int64_t nslices, const int64_t *__rest
 }
 
 // one thread per cell (slot, lane): node column index, or -1 for padding
-__global__ void k_sell3_cols(int64_t nn, int64_t nslices, int64_t ncells, const int64_t *__restrict__ soff,
+__global__ void k_sell3_cols(SliceNodes sn, int64_t nslices, int64_t ncells, const int64_t *__restrict__ soff,
                              const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
                              int32_t *__restrict__ col) {
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ncells;
@@ -85,10 +95,10 @@ __global__ void k_sell3_cols(int64_t nn, int64_t nslices, int64_t ncells, const 
       const int64_t mid = (lo + hi) >> 1;
       if (soff[mid] <= slot) lo = mid; else hi = mid;
     }
-    const int64_t i = lo * 64 + lane;
+    const int64_t i = sn.first(lo) + lane;
     const int64_t t = slot - soff[lo];
     int32_t c = -1;
-    if (i < nn) {
+    if (i < sn.end(lo)) {
       const int64_t rb = row_ptr[3 * i] - 1;
       if (3 * t < row_ptr[3 * i + 1] - 1 - rb) c = (col_idx[rb + 3 * t] - 1) / 3;
     }
@@ -97,7 +107,7 @@ __global__ void k_sell3_cols(int64_t nn, int64_t nslices, int64_t ncells, const 
 }
 
 template <typename V>
-__global__ void k_sell3_vals(int64_t nn, int64_t nslices, int64_t ncells, const int64_t *__restrict__ soff,
+__global__ void k_sell3_vals(SliceNodes sn, int64_t nslices, int64_t ncells, const int64_t *__restrict__ soff,
                              const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
                              const double *__restrict__ val, const double *__restrict__ scale,
                              V *__restrict__ out) {
@@ -110,11 +120,11 @@ __global__ void k_sell3_vals(int64_t nn, int64_t nslices, int64_t ncells, const 
       const int64_t mid = (lo + hi) >> 1;
       if (soff[mid] <= slot) lo = mid; else hi = mid;
     }
-    const int64_t i = lo * 64 + lane;
+    const int64_t i = sn.first(lo) + lane;
     const int64_t t = slot - soff[lo];
     V *o = out + slot * 64 * 9 + lane;
     bool live = false;
-    if (i < nn) {
+    if (i < sn.end(lo)) {
       const int64_t len = row_ptr[3 * i + 1] - row_ptr[3 * i];
       live = 3 * t < len;
     }
@@ -214,22 +224,25 @@ int pf_crs_to_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
 
 template <typename V>
 int pf_sell3_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
-                   const double *val, Sell3<V> &s, const double *scale) {
+                   const double *val, Sell3<V> &s, const double *scale, const SliceMap *map) {
   const int64_t nn = n / 3;
   s.nn = nn;
-  s.nslices = (nn + 63) / 64;
+  s.nslices = map ? map->nslices : (nn + 63) / 64;
+  s.snode = map ? map->snode : nullptr;
+  s.send = map ? map->send : nullptr;
+  const SliceNodes sn{nn, s.snode, s.send};
   DevBuf<int64_t> tmp;
   PF_CHECK(s.width.ensure((size_t)s.nslices + 1));
   PF_CHECK(s.soff.ensure((size_t)s.nslices + 1));
   if (s.nslices > 0)
-    k_sell3_width<<<pf_blocks(s.nslices, 256), 256, 0, ctx->stream>>>(nn, s.nslices, row_ptr, s.width.p);
+    k_sell3_width<<<pf_blocks(s.nslices, 256), 256, 0, ctx->stream>>>(sn, s.nslices, row_ptr, s.width.p);
   PF_CHECK(pf_scan_i32_to_i64(ctx, s.width.p, s.nslices, s.soff.p, tmp));
   PF_CHECK(pf_read_i64(ctx, s.soff.p + s.nslices, &s.nslots));
   const int64_t ncells = s.nslots * 64;
   PF_CHECK(s.col.ensure((size_t)ncells + 1));
   PF_CHECK(s.val.ensure((size_t)ncells * 9 + 1));
   if (ncells > 0)
-    k_sell3_cols<<<pf_blocks(ncells, 256, 65536), 256, 0, ctx->stream>>>(nn, s.nslices, ncells, s.soff.p,
+    k_sell3_cols<<<pf_blocks(ncells, 256, 65536), 256, 0, ctx->stream>>>(sn, s.nslices, ncells, s.soff.p,
                                                                        row_ptr, col_idx, s.col.p);
   PF_HIP(hipGetLastError());
   return pf_sell3_values<V>(ctx, n, row_ptr, val, s, scale);
@@ -240,8 +253,9 @@ int pf_sell3_values(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const double
                     const double *scale) {
   (void)n;
   const int64_t ncells = s.nslots * 64;
+  const SliceNodes sn{s.nn, s.snode, s.send};
   if (ncells > 0)
-    k_sell3_vals<V><<<pf_blocks(ncells, 256, 65536), 256, 0, ctx->stream>>>(s.nn, s.nslices, ncells, s.soff.p,
+    k_sell3_vals<V><<<pf_blocks(ncells, 256, 65536), 256, 0, ctx->stream>>>(sn, s.nslices, ncells, s.soff.p,
                                                                           row_ptr, s.col.p, val, scale, s.val.p);
   PF_HIP(hipGetLastError());
   return PF_OK;
@@ -257,9 +271,9 @@ int pf_spmv_sell3_launch(pf_ctx *ctx, const Sell3<V> &s, const V *x, V *y) {
 }
 
 template int pf_sell3_build<double>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const double *,
-                                    Sell3<double> &, const double *);
+                                    Sell3<double> &, const double *, const SliceMap *);
 template int pf_sell3_build<float>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const double *,
-                                   Sell3<float> &, const double *);
+                                   Sell3<float> &, const double *, const SliceMap *);
 template int pf_sell3_values<double>(pf_ctx *, int64_t, const int64_t *, const double *, Sell3<double> &,
                                      const double *);
 template int pf_sell3_values<float>(pf_ctx *, int64_t, const int64_t *, const double *, Sell3<float> &,

@@ -82,6 +82,10 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
 class FEMDomain:
     """Subset of FEMDomain_ used by the static solve path."""
 
@@ -347,6 +351,54 @@ class FEMSolver:
             L.PF_OK, L.PF_NOT_CONVERGED, L.PF_BREAKDOWN_VV, L.PF_BREAKDOWN_C2C3)
         L.check(code, "solve (Bicgstab devide by zero)" if code in (1, 2) else "solve", ok=ok)
         return x
+
+
+    # ---- batched form: independent meshes concatenated into this system ----------------
+    def addTractions(self, node_off, boxes, pressures):
+        """One traction box and pressure per mesh (nodes node_off[m]..node_off[m+1]-1)."""
+        off = _i64(node_off)
+        bx = _f64(np.asarray(boxes, dtype=np.float64).reshape(-1, 6))
+        pr = _f64(pressures)
+        L.check(L.load().pf_system_add_tractions(self.h, off.size - 1, L.ptr(off, C.c_int64),
+                                                 L.ptr(bx, C.c_double), L.ptr(pr, C.c_double)), "addTractions")
+
+    def setBatch(self, node_off, cost=None):
+        """Declare the mesh partition of a block-diagonal system (node offsets, 0-based)."""
+        off = _i64(node_off)
+        cst = None if cost is None else _f64(cost)
+        self._nmesh = off.size - 1
+        L.check(L.load().pf_system_batch_set(self.h, self._nmesh, L.ptr(off, C.c_int64),
+                                             L.ptr(cst, C.c_double)), "setBatch")
+
+    def autotuneBatch(self, iters: int = 20, precision: int = 64, **opts):
+        """Time every launch shape of the persistent kernel; returns (best, [(shape, ms), ...])."""
+        o = L.default_opts(itrmax=int(self.itrmax), er0=float(self.er0), relative_er=float(self.relative_er),
+                           precision=int(precision), **opts)
+        best = L.BatchTune()
+        tried = (L.BatchTune * 32)()
+        ms = np.zeros(32)
+        nt = C.c_int32(0)
+        L.check(L.load().pf_system_batch_autotune(self.h, C.byref(o), int(iters), C.byref(best), 32, tried,
+                                                  L.ptr(ms, C.c_double), C.byref(nt)), "autotuneBatch")
+        return best.as_dict(), [(tried[i].as_dict(), float(ms[i])) for i in range(min(nt.value, 32))]
+
+    def solveBatch(self, precision: int = 64, tune: dict | None = None, download: bool = True,
+                   raise_on_breakdown: bool = True, **opts):
+        """Solve every mesh of the batch in one persistent kernel; returns (x, per-mesh stats)."""
+        self._flush()
+        o = L.default_opts(itrmax=int(self.itrmax), er0=float(self.er0), relative_er=float(self.relative_er),
+                           debug=int(bool(self.debug)), precision=int(precision), **opts)
+        t = None if tune is None else L.BatchTune(**tune)
+        st = np.zeros(self._nmesh, dtype=L.BATCH_STATS_DTYPE)
+        tot = L.Stats()
+        x = np.empty(self.n) if download else None
+        code = L.load().pf_system_batch_solve(self.h, C.byref(o), None if t is None else C.byref(t),
+                                              L.ptr(x, C.c_double), st.ctypes.data_as(C.c_void_p), C.byref(tot))
+        self.last_stats = tot.as_dict()
+        ok = (L.PF_OK, L.PF_NOT_CONVERGED) if raise_on_breakdown else (
+            L.PF_OK, L.PF_NOT_CONVERGED, L.PF_BREAKDOWN_VV, L.PF_BREAKDOWN_C2C3)
+        L.check(code, "solveBatch", ok=ok)
+        return x, st
 
 
 def bicgstab_crs(row_ptr, col_idx, val, rhs, x0=None, ctx: L.Context | None = None, **opts):
