@@ -265,6 +265,31 @@ int pf_system_batch_solve(pf_system *sys, const pf_solve_opts *opts, const pf_ba
 int pf_system_batch_autotune(pf_system *sys, const pf_solve_opts *opts, int32_t iters, pf_batch_tune *best,
                              int32_t max_results, pf_batch_tune *tried, double *tried_ms, int32_t *ntried);
 
+/* ---- differentiable form: device arrays, many load cases ----------------------------
+ * The reference has no adjoint; these entry points serve a torch.autograd.Function whose
+ * forward is the static solve (StiffnessMatrix(YoungModulus,PoissonRatio) + fix + solve,
+ * FEMDomainClass.f90:11123-11232, FEMSolverClass.f90:1697-1825) for R load cases, and whose
+ * backward is one more solve with K^T = K plus a per-element contraction. */
+
+/* StiffnessMatrix(YoungModulus(:),PoissonRatio(:)) from DEVICE arrays of length ne. */
+int pf_system_stiffness_dev(pf_system *sys, const double *E_dev, const double *nu_dev, int32_t jmode);
+
+/* bicgstab_CRS_2 for nrhs (1..64) right-hand sides of the system's matrix at once, after the
+ * Dirichlet elimination (the fixed rows of every right-hand side are set to 0; nonzero
+ * prescribed values are rejected).  B_dev, X_dev: DEVICE arrays, n x nrhs interleaved
+ * (dof i of case l at i*nrhs + l); x0 = 0.  Every case stops on its own by the options' stop
+ * rule; per_rhs (nrhs entries) and total may be NULL.  Returns PF_OK when every case
+ * converged, else the status of the first case that did not. */
+int pf_system_solve_multi_dev(pf_system *sys, int32_t nrhs, const double *B_dev, double *X_dev,
+                              const pf_solve_opts *opts, pf_stats *per_rhs, pf_stats *total);
+
+/* out_dev[e] = sum_l lam_e(l)^T Ke(E_e, nu_e) u_e(l) for every element (DEVICE arrays, U and
+ * Lam interleaved as above, fixed DOFs excluded).  Ke is linear in E, so this is
+ * d/d(log E_e) of sum_l lam(l)^T K u(l); the adjoint gradient of a loss L(u) is -out with
+ * lam solving K lam = dL/du. */
+int pf_system_elem_sensitivity_dev(pf_system *sys, int32_t nrhs, const double *U_dev, const double *Lam_dev,
+                                   double *out_dev);
+
 #ifdef __cplusplus
 }
 #endif

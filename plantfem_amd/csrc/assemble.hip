@@ -14,6 +14,8 @@
 // deterministic gather instead of the reference's OpenMP reduction: one thread owns one
 // CRS row, walks the row node's incident elements in ascending id and adds their Ke rows
 // into its own row, so every entry is summed in ascending element order with no atomics.
+#include <algorithm>
+
 #include "pf_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -220,6 +222,53 @@ __global__ void __launch_bounds__(128) k_ke_batch(int64_t nn, int64_t ne, const 
   double *out = ke + q * 576;
 #pragma unroll
   for (int c = 0; c < 24; ++c) out[c * 24 + r] = acc[c];
+}
+
+// ---- adjoint sensitivity: out[e] = sum_l lam_e(l)^T Ke(E_e) u_e(l) ----------------------
+// Ke is linear in E, so this is d/d(log E_e) of sum_l lam(l)^T K u(l).  The element matrices
+// of a chunk of elements come from k_ke_batch (same arithmetic as the assembly); here one
+// wavefront per element stages its Ke in LDS and lane l contracts load case l of the
+// interleaved vectors (dof i, case l at i*R + l).  Fixed DOFs (fixed != NULL) are dropped
+// from both sides: the eliminated system's matrix is P K P + (I - P).
+__global__ void __launch_bounds__(64) k_ke_contract(int64_t ne, const int32_t *__restrict__ en, int64_t first,
+                                                    int64_t count, const double *__restrict__ ke, int R,
+                                                    const double *__restrict__ U, const double *__restrict__ Lam,
+                                                    const uint8_t *__restrict__ fixed, double *__restrict__ out) {
+  __shared__ double K[24][25];
+  __shared__ int64_t dofs[24];  // DOF of element row r, -1 when fixed
+  const int lane = threadIdx.x;
+  for (int64_t q = blockIdx.x; q < count; q += gridDim.x) {
+    const int64_t e = first + q;
+    const double *kq = ke + q * 576;  // column-major 24 x 24
+    for (int t = lane; t < 576; t += 64) K[t % 24][t / 24] = kq[t];
+    if (lane < 24) {
+      const int64_t dof = 3 * (int64_t)(en[(int64_t)(lane / 3) * ne + e] - 1) + lane % 3;
+      dofs[lane] = (fixed && fixed[dof]) ? -1 : dof;
+    }
+    __syncthreads();
+    double s = 0.0;
+    for (int l = lane; l < R; l += 64) {
+      double u[24];
+#pragma unroll
+      for (int c = 0; c < 24; ++c) {
+        const int64_t dof = dofs[c];
+        u[c] = dof < 0 ? 0.0 : U[dof * R + l];
+      }
+#pragma unroll 1
+      for (int r = 0; r < 24; ++r) {
+        const int64_t dof = dofs[r];
+        if (dof < 0) continue;
+        double t = 0.0;
+#pragma unroll
+        for (int c = 0; c < 24; ++c) t += K[r][c] * u[c];
+        s += Lam[dof * R + l] * t;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) out[e] = s;
+    __syncthreads();
+  }
 }
 
 // ---- batched setMatrix / setVector: node -> (call index, local node) lists ---------
@@ -488,6 +537,28 @@ int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double p
       g.nn, g.ne, g.nodcoord.p, g.elemnod.p, g.eoff.p, g.elist.p, box[0], box[1], box[2], box[3], box[4], box[5],
       pressure, 0, nullptr, nullptr, nullptr, rhs);
   PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_elem_bilinear(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode, int R,
+                     const double *U, const double *Lam, const uint8_t *fixed, double *out) {
+  if (g.nne != 8 || !g.has_coords) {
+    pf_set_error("element sensitivities need an 8-node mesh with coordinates");
+    return PF_INVALID;
+  }
+  if (g.ne == 0) return PF_OK;
+  const int64_t chunk = std::min<int64_t>(g.ne, 32768);  // 151 MB of element matrices per chunk
+  DevBuf<double> ke;
+  PF_CHECK(ke.ensure((size_t)chunk * 576));
+  for (int64_t first = 0; first < g.ne; first += chunk) {
+    const int64_t count = std::min(chunk, g.ne - first);
+    k_ke_batch<<<pf_blocks(count * 24, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu,
+                                                                   jmode, first, count, ke.p);
+    k_ke_contract<<<(unsigned)std::min<int64_t>(count, 65536), 64, 0, ctx->stream>>>(g.ne, g.elemnod.p, first, count,
+                                                                                     ke.p, R, U, Lam, fixed, out);
+    PF_HIP(hipGetLastError());
+  }
+  PF_HIP(hipStreamSynchronize(ctx->stream));
   return PF_OK;
 }
 

@@ -367,9 +367,13 @@ struct pf_system {
   uint64_t rhs_version = 1, fix_version = 1;
   uint64_t bc_mat = 0, bc_rhs = 0, bc_fix = 0;  // versions the elimination was last applied to
   BatchWork *batch = nullptr;
+  MultiWork *multi = nullptr;
+  int32_t jmode = 0;          // J mode of the last assembly (reused by the sensitivities)
+  bool ubar_nonzero = false;  // some fixed DOF has a nonzero prescribed value
   ~pf_system() {
     pf_solver_work_destroy(work);
     pf_batch_work_destroy(batch);
+    pf_multi_work_destroy(multi);
   }
 };
 
@@ -415,6 +419,7 @@ int sys_alloc_vectors(pf_system *s, bool zero) {
     PF_HIP(hipMemsetAsync(s->x.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
   }
   s->has_fix = false;
+  s->ubar_nonzero = false;
   ++s->rhs_version;
   ++s->fix_version;
   return PF_OK;
@@ -538,6 +543,28 @@ extern "C" int pf_system_stiffness(pf_system *s, const double *E, const double *
   }
   PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
   PF_HIP(hipStreamSynchronize(ctx->stream));
+  s->jmode = jmode;
+  mark_changed(s);
+  return PF_OK;
+}
+
+extern "C" int pf_system_stiffness_dev(pf_system *s, const double *E_dev, const double *nu_dev, int32_t jmode) {
+  PF_CHECK(sys_check(s, false));
+  if (!s->has_mesh || s->g.nne != 8 || !E_dev || !nu_dev) {
+    pf_set_error("pf_system_stiffness_dev: hex8 mesh and device arrays E, nu required");
+    return PF_INVALID;
+  }
+  if (!(s->graph_crs && s->dof == 3)) PF_CHECK(pf_system_zero_matrix(s, 3, nullptr));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_CHECK(s->E.ensure((size_t)s->g.ne));
+  PF_CHECK(s->nu.ensure((size_t)s->g.ne));
+  PF_HIP(hipMemcpyAsync(s->E.p, E_dev, sizeof(double) * s->g.ne, hipMemcpyDeviceToDevice, ctx->stream));
+  PF_HIP(hipMemcpyAsync(s->nu.p, nu_dev, sizeof(double) * s->g.ne, hipMemcpyDeviceToDevice, ctx->stream));
+  s->has_mat = true;
+  PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  s->jmode = jmode;
   mark_changed(s);
   return PF_OK;
 }
@@ -648,6 +675,12 @@ extern "C" int pf_system_fix(pf_system *s, int64_t nids, const int32_t *ids, con
   }
   ++s->fix_version;
   if (nids == 0) return PF_OK;
+  if (values) {
+    for (int64_t q = 0; q < nids; ++q)
+      if (values[q] != 0.0 && ids[q] >= 1 && ids[q] <= s->n) s->ubar_nonzero = true;
+  } else if (fixvalue != 0.0) {
+    s->ubar_nonzero = true;
+  }
   DevBuf<int32_t> dids;
   DevBuf<double> dvals;
   PF_CHECK(upload(ctx, dids, ids, (size_t)nids));
@@ -791,4 +824,40 @@ extern "C" int pf_system_batch_autotune(pf_system *s, const pf_solve_opts *opts,
   PF_CHECK(batch_prepare(s));
   return pf_batch_autotune_device(ctx, s->batch, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->rhs.p, s->x.p, o,
                                   iters, s->mat_version, best, max_results, tried, tried_ms, ntried);
+}
+
+// ============================================================================ multi-RHS / adjoint
+extern "C" int pf_system_solve_multi_dev(pf_system *s, int32_t nrhs, const double *B_dev, double *X_dev,
+                                         const pf_solve_opts *opts, pf_stats *per_rhs, pf_stats *total) {
+  PF_CHECK(sys_check(s, true));
+  if (!B_dev || !X_dev) {
+    pf_set_error("pf_system_solve_multi_dev: device arrays B and X required");
+    return PF_INVALID;
+  }
+  if (s->ubar_nonzero) {
+    pf_set_error("pf_system_solve_multi_dev: nonzero prescribed displacements are not supported (fix with 0)");
+    return PF_INVALID;
+  }
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  pf_solve_opts o;
+  if (opts) o = *opts; else pf_solve_opts_default(&o);
+  PF_CHECK(apply_bc(s));
+  return pf_multi_solve_device(ctx, &s->multi, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->mat_version,
+                               s->has_fix ? s->fixed.p : nullptr, nrhs, B_dev, X_dev, o, per_rhs, total);
+}
+
+extern "C" int pf_system_elem_sensitivity_dev(pf_system *s, int32_t nrhs, const double *U_dev, const double *Lam_dev,
+                                              double *out_dev) {
+  PF_CHECK(sys_check(s, true));
+  if (!s->has_mat || s->dof != 3 || nrhs < 1 || !U_dev || !Lam_dev || !out_dev) {
+    pf_set_error("pf_system_elem_sensitivity_dev: assembled hex8 stiffness, nrhs >= 1 and device arrays required");
+    return PF_INVALID;
+  }
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_CHECK(pf_elem_bilinear(ctx, s->g, s->E.p, s->nu.p, s->jmode, nrhs, U_dev, Lam_dev,
+                            s->has_fix ? s->fixed.p : nullptr, out_dev));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
 }

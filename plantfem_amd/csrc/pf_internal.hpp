@@ -135,6 +135,10 @@ int pf_add_element_mats(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count,
 int pf_add_element_vecs(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count, const int32_t *ids,
                         const double *vec, double *rhs);
 int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double pressure, double *rhs);
+// out[e] = sum_l lam_e(l)^T Ke(E_e, nu_e) u_e(l) over R interleaved cases (device arrays;
+// fixed DOFs, if given, excluded)
+int pf_elem_bilinear(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode, int R,
+                     const double *U, const double *Lam, const uint8_t *fixed, double *out);
 // one box and pressure per node range node_off[m] .. node_off[m+1]-1 (device arrays)
 int pf_traction_boxes(pf_ctx *ctx, const NodeGraph &g, int32_t count, const int64_t *node_off_dev,
                       const double *boxes_dev, const double *pressures_dev, double *rhs);
@@ -203,6 +207,14 @@ int pf_batch_autotune_device(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, 
                              const double *val, int64_t nnz, const double *rhs, double *x, const pf_solve_opts &o,
                              int32_t iters, uint64_t mat_version, pf_batch_tune *best, int32_t max_results,
                              pf_batch_tune *tried, double *tried_ms, int32_t *ntried);
+
+// multi.hip: one matrix, many right-hand sides (interleaved, dof i / case l at i*R + l)
+struct MultiWork;
+void pf_multi_work_destroy(MultiWork *m);
+int pf_multi_solve_device(pf_ctx *ctx, MultiWork **w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                          const double *val, uint64_t mat_version, const uint8_t *fixed, int32_t nrhs,
+                          const double *B_dev, double *X_dev, const pf_solve_opts &o, pf_stats *per_rhs,
+                          pf_stats *total);
 
 // common launch geometry
 static inline unsigned pf_blocks(int64_t work, int threads, int64_t cap = 1 << 20) {

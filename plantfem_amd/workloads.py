@@ -89,3 +89,38 @@ def crs_sizes(nx: int, ny: int, nz: int) -> tuple[int, int]:
     """(n, nnz) of ZeroMatrix(DOF=3) on an nx x ny x nz Cube3D: nnz = 9 prod(3 m_i - 2), m_i = n_i + 1."""
     nn = (nx + 1) * (ny + 1) * (nz + 1)
     return 3 * nn, 9 * (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
+
+
+@dataclass
+class InverseProblem:
+    """Recover per-element Young's moduli from displacements observed on the surface."""
+    nodcoord: np.ndarray
+    elemnod: np.ndarray
+    nu: np.ndarray
+    fixed_dofs: np.ndarray   # 1-based DOF ids (clamped base)
+    E_hidden: np.ndarray     # the field the observations come from
+    F: np.ndarray            # (n, ncases) load cases: random top-face traction fields
+    obs_nodes: np.ndarray    # 0-based observed surface nodes
+    noise: float             # relative standard deviation of the observation noise
+    seed: int
+
+    @property
+    def obs_dofs(self) -> np.ndarray:
+        return np.sort(np.concatenate([3 * self.obs_nodes, 3 * self.obs_nodes + 1, 3 * self.obs_nodes + 2]))
+
+
+def inverse_problem(n: int = 60, ncases: int = 64, nobs: int = 2000, seed: int = 0, sigma: float = 0.5,
+                    noise: float = 0.01) -> InverseProblem:
+    """n^3 unit-cube hex mesh clamped at z = 0; hidden E = 1e7 lognormal(sigma); ncases random
+    top-face traction fields; nobs random surface nodes (not on the clamped face)."""
+    from .autograd import surface_nodes, top_traction_fields  # numpy-only helpers
+    nod, el = cube3d(n, n, n)
+    rng = np.random.default_rng(seed)
+    ne = el.shape[0]
+    E = lognormal_E(ne, sigma, rng)
+    bottom = np.nonzero(nod[:, 2] <= nod[:, 2].min())[0] + 1
+    fixed = np.sort(np.concatenate([3 * bottom - 2, 3 * bottom - 1, 3 * bottom])).astype(np.int32)
+    F = top_traction_fields(nod, el, ncases, rng)
+    surf = surface_nodes(nod)
+    obs = np.sort(rng.choice(surf, size=min(nobs, surf.size), replace=False))
+    return InverseProblem(nod, el, np.full(ne, 0.3), fixed, E, F, obs, noise, seed)

@@ -1,0 +1,158 @@
+"""Differentiable form: multi-RHS solve, adjoint gradient w.r.t. per-element log E.
+
+The gradient is checked against (1) an FP64 adjoint gradient built on the CPU with the
+oracle's bicgstab_CRS_2 and its element matrices, and (2) central finite differences of the
+loss (tight solves on a small mesh, where a single element's effect on the loss is large
+enough for differences to resolve it).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plantfem_amd import workloads as W
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_inverse_problem_workload():
+    p = W.inverse_problem(n=6, ncases=4, nobs=30, seed=2)
+    n = p.nodcoord.shape[0] * 3
+    assert p.F.shape == (n, 4) and p.obs_nodes.size == 30
+    # all force on the top face, downward, totalling the mean pressure times the area
+    zt = p.nodcoord[:, 2].max()
+    top = np.abs(p.nodcoord[:, 2] - zt) < 1e-12
+    Fz = p.F[2::3]
+    assert np.all(Fz[~top] == 0) and np.all(p.F[0::3] == 0) and np.all(p.F[1::3] == 0)
+    assert np.all(Fz[top].sum(axis=0) < 0)
+    # observed nodes lie on the surface and not on the clamped base
+    nod = p.nodcoord[p.obs_nodes]
+    assert np.all(nod[:, 2] > 0)
+    on = np.zeros(len(nod), bool)
+    for d in range(3):
+        on |= (nod[:, d] == 0) | (nod[:, d] == 1)
+    assert on.all()
+
+
+def _system(p, **opts):
+    from plantfem_amd.autograd import ElasticSystem
+    return ElasticSystem(p.nodcoord, p.elemnod, p.fixed_dofs, p.nu, **opts)
+
+
+def _oracle_eliminated(p, E):
+    rp, ci = O.zero_matrix(p.elemnod, p.nodcoord.shape[0], 3)
+    val = O.assemble(p.nodcoord, p.elemnod, E, p.nu, rp, ci)
+    n = 3 * p.nodcoord.shape[0]
+    fixed = np.zeros(n, np.uint8)
+    fixed[p.fixed_dofs - 1] = 1
+    rhs = np.zeros(n)
+    O.apply_dirichlet(rp, ci, val, rhs, fixed, np.zeros(n))
+    return rp, ci, val, fixed
+
+
+@pytest.mark.gpu
+def test_multi_rhs_matches_oracle():
+    p = W.inverse_problem(n=8, ncases=5, nobs=20, seed=1)
+    sysm = _system(p, relative_er=1e-13)
+    E = torch.tensor(p.E_hidden, device="cuda")
+    sysm.assemble(E)
+    U = sysm.solve(torch.tensor(p.F, device="cuda")).cpu().numpy()
+    st = sysm.last_stats
+    say("multi-RHS iterations", [c["iterations"] for c in st["cases"]], "relres", st["total"]["true_relres"])
+    assert st["total"]["status"] == 0 and st["total"]["true_relres"] < 1e-12
+    rp, ci, val, fixed = _oracle_eliminated(p, p.E_hidden)
+    for l in range(p.F.shape[1]):
+        b = np.where(fixed == 1, 0.0, p.F[:, l])
+        xo, status, it, *_ = O.solve(rp, ci, val, b, fixed, np.zeros(b.size), relative_er=1e-14, er0=1e-300)
+        assert status == 0
+        err = np.linalg.norm(U[:, l] - xo) / np.linalg.norm(xo)
+        assert err <= 1e-10, (l, err)
+        assert np.all(U[fixed == 1, l] == 0.0)
+
+
+def _loss_and_grad(p, sysm, logE, D, obs):
+    from plantfem_amd.autograd import static_solve
+    logE = logE.clone().requires_grad_(True)
+    F = torch.tensor(p.F, device="cuda")
+    U = static_solve(logE, F, sysm)
+    r = U[obs] - D
+    L = (r * r).sum()
+    L.backward()
+    return float(L), logE.grad.detach().cpu().numpy()
+
+
+def _observations(p, sysm):
+    E = torch.tensor(p.E_hidden, device="cuda")
+    sysm.assemble(E)
+    U = sysm.solve(torch.tensor(p.F, device="cuda"))
+    obs = torch.tensor(p.obs_dofs, device="cuda")
+    D = U[obs]
+    g = torch.Generator(device="cpu").manual_seed(p.seed)
+    D = D * (1.0 + p.noise * torch.randn(D.shape, generator=g, dtype=torch.float64).to("cuda"))
+    return D, obs
+
+
+@pytest.mark.gpu
+def test_adjoint_gradient_matches_oracle_adjoint():
+    p = W.inverse_problem(n=6, ncases=4, nobs=40, seed=3)
+    sysm = _system(p, relative_er=1e-13)
+    D, obs = _observations(p, sysm)
+    logE0 = torch.full((p.elemnod.shape[0],), float(np.log(1e7)), dtype=torch.float64, device="cuda")
+    L0, g = _loss_and_grad(p, sysm, logE0, D, obs)
+    say("loss", L0, "|g|", np.linalg.norm(g))
+    # CPU FP64 adjoint with the oracle's solver and element matrices
+    E0 = np.exp(logE0.cpu().numpy())
+    rp, ci, val, fixed = _oracle_eliminated(p, E0)
+    Ke = O.hex8_stiffness_all(p.nodcoord, p.elemnod, E0, p.nu)          # (ne, 24, 24)
+    dofs = (3 * (p.elemnod[:, :, None] - 1) + np.arange(3)).reshape(-1, 24)
+    keep = (fixed[dofs] == 0)
+    od = p.obs_dofs
+    Dn = D.cpu().numpy()
+    gref = np.zeros(p.elemnod.shape[0])
+    for l in range(p.F.shape[1]):
+        b = np.where(fixed == 1, 0.0, p.F[:, l])
+        u, st, *_ = O.solve(rp, ci, val, b, fixed, np.zeros(b.size), relative_er=1e-14, er0=1e-300)
+        gu = np.zeros(b.size)
+        gu[od] = 2.0 * (u[od] - Dn[:, l])
+        gu[fixed == 1] = 0.0
+        lam, st2, *_ = O.solve(rp, ci, val, gu, fixed, np.zeros(b.size), relative_er=1e-14, er0=1e-300)
+        ue = np.where(keep, u[dofs], 0.0)
+        le = np.where(keep, lam[dofs], 0.0)
+        gref -= np.einsum("er,erc,ec->e", le, Ke, ue)
+    err = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    say("adjoint vs oracle adjoint", err)
+    assert err <= 1e-8
+
+
+@pytest.mark.gpu
+def test_adjoint_gradient_central_differences():
+    p = W.inverse_problem(n=5, ncases=3, nobs=30, seed=4)
+    sysm = _system(p, relative_er=1e-13, stop_mode=1)
+    D, obs = _observations(p, sysm)
+    rng = np.random.default_rng(0)
+    logE0 = torch.tensor(np.log(1e7) + 0.3 * rng.standard_normal(p.elemnod.shape[0]), device="cuda")
+    L0, g = _loss_and_grad(p, sysm, logE0, D, obs)
+    h = 1e-4
+    els = rng.choice(p.elemnod.shape[0], size=8, replace=False)
+    from plantfem_amd.autograd import static_solve
+    F = torch.tensor(p.F, device="cuda")
+    worst = 0.0
+    for e in els:
+        vals = []
+        for s in (+1, -1):
+            lp = logE0.clone()
+            lp[e] += s * h
+            with torch.no_grad():
+                U = static_solve(lp, F, sysm)
+                vals.append(float(((U[obs] - D) ** 2).sum()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        rel = abs(fd - g[e]) / max(abs(g[e]), 1e-300)
+        worst = max(worst, rel)
+        say("element", int(e), "adjoint", g[e], "fd", fd, "rel", rel)
+    assert worst <= 1e-6
