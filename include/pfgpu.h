@@ -265,6 +265,29 @@ int pf_system_batch_solve(pf_system *sys, const pf_solve_opts *opts, const pf_ba
 int pf_system_batch_autotune(pf_system *sys, const pf_solve_opts *opts, int32_t iters, pf_batch_tune *best,
                              int32_t max_results, pf_batch_tune *tried, double *tried_ms, int32_t *ntried);
 
+/* ---- body force and stress recovery ----------------------------------------------------- */
+
+/* MassVectorFEMDomain(ElementID, Density, DOF=3, Accel) (FEMDomainClass.f90:10993-11073) for
+ * elements first..first+count-1 (1-based): mv[q*24 + 3a + k] = rho sum_gp N_a accel_k detJ. */
+int pf_hex8_mass_vector(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                        double density, const double *accel, int64_t first, int64_t count, double *mv);
+
+/* setVector(MassVector(...)) for every element in ascending order, added to the RHS (the body
+ * force loop of Tutorial/fem/benchmark_bar_in_gravity_field.f90:20-31).  density: per element
+ * (ne values) or NULL for density_all; accel: 3 values. */
+int pf_system_add_body_force(pf_system *sys, const double *density, double density_all, const double *accel);
+
+/* getElementCauchyStress / StressMatrix without GaussPoint (FEMDomainClass.f90:15249-15380,
+ * :11912-12032): the cell average over the 8 Gauss points of D B u_e, per element:
+ * sigma[e*9 + 3j + i] = sigma(i+1, j+1).  disp: node-major displacement (the solver's x);
+ * E, nu: 1 or ne values, as the reference accepts. */
+int pf_element_stress(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                      const double *disp, const double *E, int64_t nE, const double *nu, int64_t nnu,
+                      int32_t jmode, double *sigma);
+/* the same for the system's mesh and its current solution x */
+int pf_system_element_stress(pf_system *sys, const double *E, int64_t nE, const double *nu, int64_t nnu,
+                             int32_t jmode, double *sigma);
+
 /* ---- differentiable form: device arrays, many load cases ----------------------------
  * The reference has no adjoint; these entry points serve a torch.autograd.Function whose
  * forward is the static solve (StiffnessMatrix(YoungModulus,PoissonRatio) + fix + solve,

@@ -26,6 +26,10 @@ _SIG = {
     "orc_set_element_matrix": (I32, [I64, I32, P32, I64, I32, PD, P64, P32, PD]),
     "orc_traction_box": (None, [I64, I64, PD, P32, PD, D, PD]),
     "orc_select_box": (I64, [I64, PD, PD, P32]),
+    "orc_hex8_mass_vector": (None, [PD, D, PD, PD]),
+    "orc_body_force": (None, [I64, I64, PD, P32, PD, PD, PD]),
+    "orc_hex8_cell_stress": (None, [PD, PD, D, D, I32, PD]),
+    "orc_cell_stress_all": (None, [I64, I64, PD, P32, PD, PD, PD, I32, PD]),
     "orc_apply_dirichlet": (None, [I64, P64, P32, PD, PD, PU8, PD]),
     "orc_spmv": (None, [I64, P64, P32, PD, PD, PD, I32]),
     "orc_bicgstab_crs2": (I32, [I64, P64, P32, PD, PD, PD, I32, D, D, I32, P32, PD, PD, PD]),
@@ -128,6 +132,34 @@ def traction_box(nodcoord, elemnod, box, pressure, rhs=None):
                             _p(_fl(elemnod, np.int32), C.c_int32), _p(_c(box, np.float64), C.c_double),
                             float(pressure), _p(out, C.c_double))
     return out
+
+
+def hex8_mass_vector(X, rho, accel):
+    mv = np.empty(24)
+    load().orc_hex8_mass_vector(_p(_c(X, np.float64), C.c_double), float(rho),
+                                _p(_c(accel, np.float64), C.c_double), _p(mv, C.c_double))
+    return mv
+
+
+def body_force(nodcoord, elemnod, rho, accel, rhs=None):
+    nn, ne = nodcoord.shape[0], elemnod.shape[0]
+    out = np.zeros(3 * nn) if rhs is None else np.array(rhs, dtype=np.float64)
+    rho = np.broadcast_to(np.asarray(rho, dtype=np.float64), (ne,)).copy()
+    load().orc_body_force(nn, ne, _p(_fl(nodcoord, np.float64), C.c_double), _p(_fl(elemnod, np.int32), C.c_int32),
+                          _p(rho, C.c_double), _p(_c(accel, np.float64), C.c_double), _p(out, C.c_double))
+    return out
+
+
+def cell_stress(nodcoord, elemnod, disp, E, nu, jmode=0):
+    """(ne, 3, 3) cell-averaged Cauchy stress per element."""
+    nn, ne = nodcoord.shape[0], elemnod.shape[0]
+    E = np.broadcast_to(np.asarray(E, dtype=np.float64), (ne,)).copy()
+    nu = np.broadcast_to(np.asarray(nu, dtype=np.float64), (ne,)).copy()
+    out = np.empty(ne * 9)
+    load().orc_cell_stress_all(nn, ne, _p(_fl(nodcoord, np.float64), C.c_double), _p(_fl(elemnod, np.int32), C.c_int32),
+                               _p(_c(disp, np.float64), C.c_double), _p(E, C.c_double), _p(nu, C.c_double), jmode,
+                               _p(out, C.c_double))
+    return out.reshape(ne, 3, 3).transpose(0, 2, 1)
 
 
 def select_box(nodcoord, box):

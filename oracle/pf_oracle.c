@@ -310,6 +310,143 @@ void orc_hex8_stiffness(const double *X, double E, double nu, int jmode, double 
     }
 }
 
+/* shared by the mass vector and the stress: J, detJ and G = (J^-T or J^-1) dN at one Gauss point */
+static double hex8_gp_G(const double *X, int gp, int jmode, double G[3][8])
+{
+    double dN[3][8], J[3][3], Ji[3][3];
+    hex8_dN(GPT[gp], dN);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int a = 0; a < 8; ++a) s = s + dN[i][a] * X[a * 3 + j];
+            J[i][j] = s;
+        }
+    const double detJ = det3(J);
+    if (G) {
+        inv3(J, Ji);
+        for (int p = 0; p < 3; ++p)
+            for (int a = 0; a < 8; ++a) {
+                double s = 0.0;
+                for (int i = 0; i < 3; ++i)
+                    s = s + (jmode == 0 ? dN[i][a] * Ji[i][p] : Ji[p][i] * dN[i][a]);
+                G[p][a] = s;
+            }
+    }
+    return detJ;
+}
+
+/* Nmat(a) = 0.1250d0*(1.0d0 -+ gzi(1))*(1.0d0 -+ gzi(2))*(1.0d0 -+ gzi(3)) (ShapeFunctionClass.f90:781-788) */
+static double hex8_N(int a, const double *g)
+{
+    static const double S[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                   {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    const double t0 = S[a][0] > 0 ? 1.0 + g[0] : 1.0 - g[0];
+    const double t1 = S[a][1] > 0 ? 1.0 + g[1] : 1.0 - g[1];
+    const double t2 = S[a][2] > 0 ? 1.0 + g[2] : 1.0 - g[2];
+    return ((0.125 * t0) * t1) * t2;
+}
+
+/* MassVector = MassVector + matmul(Nmat, accel)*det per Gauss point, then rho*MassVector
+ * (FEMDomainClass.f90:11032-11071).  matmul(Nmat, accel)(3a+k) has one nonzero term
+ * N_a * accel_k; the zero terms add signed zeros only. */
+void orc_hex8_mass_vector(const double *X, double rho, const double *accel, double *mv)
+{
+    for (int r = 0; r < 24; ++r) mv[r] = 0.0;
+    for (int gp = 0; gp < 8; ++gp) {
+        const double detJ = hex8_gp_G(X, gp, 0, NULL);
+        for (int a = 0; a < 8; ++a) {
+            const double Na = hex8_N(a, GPT[gp]);
+            for (int k = 0; k < 3; ++k) mv[3 * a + k] = mv[3 * a + k] + (Na * accel[k]) * detJ;
+        }
+    }
+    for (int r = 0; r < 24; ++r) mv[r] = rho * mv[r];
+}
+
+void orc_body_force(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                    const double *rho, const double *accel, double *rhs)
+{
+    for (int64_t e = 0; e < ne; ++e) {
+        double X[24], mv[24];
+        for (int a = 0; a < 8; ++a)
+            for (int d = 0; d < 3; ++d)
+                X[a * 3 + d] = nodcoord[(int64_t)d * nn + elemnod[(int64_t)a * ne + e] - 1];
+        orc_hex8_mass_vector(X, rho[e], accel, mv);
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) {
+                const int64_t row = 3 * (int64_t)(elemnod[(int64_t)a * ne + e] - 1) + k;
+                rhs[row] = rhs[row] + mv[3 * a + k];
+            }
+    }
+}
+
+/* per Gauss point Stressvec = matmul(Dmat, matmul(Bmat, ElemDisp)), accumulated into the 3x3
+ * tensor in the reference's order, then divided by the number of Gauss points
+ * (FEMDomainClass.f90:11978-12031) */
+void orc_hex8_cell_stress(const double *X, const double *ue, double E, double nu, int jmode, double *s)
+{
+    const double mu = E / 2.0 / (1.0 + nu);
+    const double lam = nu * E / (1.0 + nu) / (1.0 - 2.0 * nu);
+    double D[6][6];
+    memset(D, 0, sizeof(D));
+    D[0][0] = 2.0 * mu + lam; D[0][1] = lam; D[0][2] = lam;
+    D[1][0] = lam; D[1][1] = 2.0 * mu + lam; D[1][2] = lam;
+    D[2][0] = lam; D[2][1] = lam; D[2][2] = 2.0 * mu + lam;
+    D[3][3] = mu; D[4][4] = mu; D[5][5] = mu;
+    double S[3][3];
+    memset(S, 0, sizeof(S));
+    for (int gp = 0; gp < 8; ++gp) {
+        double G[3][8];
+        hex8_gp_G(X, gp, jmode, G);
+        double B[6][24];
+        memset(B, 0, sizeof(B));
+        for (int q = 0; q < 8; ++q) {
+            for (int p = 0; p < 3; ++p) B[p][3 * q + p] = G[p][q];
+            B[3][3 * q + 0] = G[1][q]; B[3][3 * q + 1] = G[0][q];
+            B[4][3 * q + 1] = G[2][q]; B[4][3 * q + 2] = G[1][q];
+            B[5][3 * q + 0] = G[2][q]; B[5][3 * q + 2] = G[0][q];
+        }
+        double eps[6], sv[6];
+        for (int k = 0; k < 6; ++k) {
+            double t = 0.0;
+            for (int c = 0; c < 24; ++c) t = t + B[k][c] * ue[c];
+            eps[k] = t;
+        }
+        for (int k = 0; k < 6; ++k) {
+            double t = 0.0;
+            for (int m = 0; m < 6; ++m) t = t + D[k][m] * eps[m];
+            sv[k] = t;
+        }
+        S[0][0] = S[0][0] + sv[0];
+        S[1][1] = S[1][1] + sv[1];
+        S[2][2] = S[2][2] + sv[2];
+        S[0][1] = S[0][1] + sv[3];
+        S[1][2] = S[1][2] + sv[4];
+        S[0][2] = S[0][2] + sv[5];
+        S[1][0] = S[1][0] + sv[3];
+        S[2][1] = S[2][1] + sv[4];
+        S[2][0] = S[2][0] + sv[5];
+    }
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) s[3 * j + i] = S[i][j] / 8.0;
+}
+
+void orc_cell_stress_all(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                         const double *disp, const double *E, const double *nu, int jmode, double *sigma)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; ++e) {
+        double X[24], ue[24];
+        for (int a = 0; a < 8; ++a) {
+            const int64_t node = elemnod[(int64_t)a * ne + e] - 1;
+            for (int d = 0; d < 3; ++d) {
+                X[a * 3 + d] = nodcoord[(int64_t)d * nn + node];
+                ue[3 * a + d] = disp[3 * node + d];
+            }
+        }
+        orc_hex8_cell_stress(X, ue, E[e], nu[e], jmode, sigma + e * 9);
+    }
+}
+
 void orc_hex8_stiffness_all(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
                             const double *E, const double *nu, int jmode, double *ke)
 {

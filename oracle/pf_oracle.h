@@ -76,6 +76,20 @@ void orc_traction_box(int64_t nn, int64_t ne, const double *nodcoord, const int3
  * when ids != NULL. */
 int64_t orc_select_box(int64_t nn, const double *nodcoord, const double *box, int32_t *ids);
 
+/* MassVectorFEMDomain(ElementID, Density, DOF=3, Accel) (FEMDomainClass.f90:10993-11073) of one
+ * hex8: mv[3a+k] = rho * sum_gp N_a(gp) accel_k detJ(gp) (unit Gauss weights). */
+void orc_hex8_mass_vector(const double *X, double rho, const double *accel, double *mv);
+/* sum of the element mass vectors in ascending element order added to rhs (the setVector loop
+ * of Tutorial/fem/benchmark_bar_in_gravity_field.f90:20-31); rho per element. */
+void orc_body_force(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                    const double *rho, const double *accel, double *rhs);
+/* StressMatrixFEMDomain without GaussPoint (FEMDomainClass.f90:11912-12032): cell average over
+ * the 8 Gauss points of D B u_e; s[3*j + i] = sigma(i+1, j+1).  disp is the solver's x
+ * (node-major, ArrayClass reshape). */
+void orc_hex8_cell_stress(const double *X, const double *ue, double E, double nu, int jmode, double *s);
+void orc_cell_stress_all(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                         const double *disp, const double *E, const double *nu, int jmode, double *sigma);
+
 /* A14 steps 1-4: Dirichlet elimination of solveFEMSolver (FEMSolverClass.f90:1707-1756). */
 void orc_apply_dirichlet(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
                          double *rhs, const uint8_t *fixed, const double *ubar);

@@ -152,6 +152,10 @@ SIGNATURES = {
                                         C.POINTER(Stats)]),
     "pf_system_batch_autotune": (C.c_int, [P, C.POINTER(SolveOpts), I32, C.POINTER(BatchTune), I32,
                                            C.POINTER(BatchTune), PF64, PI32]),
+    "pf_hex8_mass_vector": (C.c_int, [P, I64, I64, PF64, PI32, F64, PF64, I64, I64, PF64]),
+    "pf_system_add_body_force": (C.c_int, [P, PF64, F64, PF64]),
+    "pf_element_stress": (C.c_int, [P, I64, I64, PF64, PI32, PF64, PF64, I64, PF64, I64, I32, PF64]),
+    "pf_system_element_stress": (C.c_int, [P, PF64, I64, PF64, I64, I32, PF64]),
     "pf_system_stiffness_dev": (C.c_int, [P, P, P, I32]),
     "pf_system_solve_multi_dev": (C.c_int, [P, I32, P, P, C.POINTER(SolveOpts), P, C.POINTER(Stats)]),
     "pf_system_elem_sensitivity_dev": (C.c_int, [P, I32, P, P, P]),

@@ -224,6 +224,203 @@ __global__ void __launch_bounds__(128) k_ke_batch(int64_t nn, int64_t ne, const 
   for (int c = 0; c < 24; ++c) out[c * 24 + r] = acc[c];
 }
 
+// ---- body force and stress recovery --------------------------------------------------
+// J, detJ and G = (J^-T or J^-1) dN/dxi at Gauss point gp, in the operation order of
+// hex8_ke_row (ShapeFunctionClass.f90:1476, FEMDomainClass.f90:12289)
+__device__ inline double hex8_gp_G(const double (&X)[8][3], int gp, int jmode, double (&G)[3][8], bool want_G) {
+  const double gx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? GPV : -GPV;
+  const double gy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? GPV : -GPV;
+  const double gz = (gp >= 4) ? GPV : -GPV;
+  double dN[3][8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) dN_node(b, gx, gy, gz, dN[0][b], dN[1][b], dN[2][b]);
+  double J[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) s = s + dN[i][b] * X[b][j];
+      J[i][j] = s;
+    }
+  const double detJ = det3(J);
+  if (want_G) {
+    double Ji[3][3];
+    inv3(J, Ji);
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        double s = 0.0;
+        if (jmode == 0) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) s = s + dN[i][b] * Ji[i][p];
+        } else {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) s = s + Ji[p][i] * dN[i][b];
+        }
+        G[p][b] = s;
+      }
+  }
+  return detJ;
+}
+
+// N_a = 0.1250d0*(1 -+ xi)*(1 -+ eta)*(1 -+ zeta) at Gauss point gp (ShapeFunctionClass.f90:781-788)
+__device__ inline double hex8_N(int a, int gp) {
+  const double gx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? GPV : -GPV;
+  const double gy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? GPV : -GPV;
+  const double gz = (gp >= 4) ? GPV : -GPV;
+  const double t0 = SX[a] > 0 ? 1.0 + gx : 1.0 - gx;
+  const double t1 = SY[a] > 0 ? 1.0 + gy : 1.0 - gy;
+  const double t2 = SZ[a] > 0 ? 1.0 + gz : 1.0 - gz;
+  return ((0.125 * t0) * t1) * t2;
+}
+
+// MassVectorFEMDomain (FEMDomainClass.f90:10993-11073) entries of local node a:
+// mv_k = rho * sum_gp (N_a accel_k) detJ (the other terms of matmul(Nmat, accel) are zeros)
+__device__ inline void hex8_mass_node(const double (&X)[8][3], int a, double rho, double ax, double ay, double az,
+                                      double &m0, double &m1, double &m2) {
+  double G[3][8];
+  m0 = 0.0;
+  m1 = 0.0;
+  m2 = 0.0;
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    const double detJ = hex8_gp_G(X, gp, 0, G, false);
+    const double Na = hex8_N(a, gp);
+    m0 = m0 + (Na * ax) * detJ;
+    m1 = m1 + (Na * ay) * detJ;
+    m2 = m2 + (Na * az) * detJ;
+  }
+  m0 = rho * m0;
+  m1 = rho * m1;
+  m2 = rho * m2;
+}
+
+// element mass vectors of elements first..first+count-1: one thread per (element, node)
+__global__ void k_mass_vec_batch(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                                 const int32_t *__restrict__ en, double rho, double ax, double ay, double az,
+                                 int64_t first, int64_t count, double *__restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= count * 8) return;
+  const int64_t q = t / 8;
+  const int a = (int)(t - q * 8);
+  int32_t nodes[8];
+  double X[8][3];
+  load_element(nn, ne, xyz, en, first + q, nodes, X);
+  double m0, m1, m2;
+  hex8_mass_node(X, a, rho, ax, ay, az, m0, m1, m2);
+  out[q * 24 + 3 * a] = m0;
+  out[q * 24 + 3 * a + 1] = m1;
+  out[q * 24 + 3 * a + 2] = m2;
+}
+
+// rhs += sum over elements (ascending id) of their mass vectors: one thread per node
+__global__ void k_body_force(int64_t nn, int64_t ne, const double *__restrict__ xyz, const int32_t *__restrict__ en,
+                             const int64_t *__restrict__ eoff, const int32_t *__restrict__ elist,
+                             const double *__restrict__ rho, double rho_all, double ax, double ay, double az,
+                             double *__restrict__ rhs) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  double f0 = rhs[3 * i], f1 = rhs[3 * i + 1], f2 = rhs[3 * i + 2];
+  const int64_t s0 = eoff[i], s1 = eoff[i + 1];
+  for (int64_t s = s0; s < s1; ++s) {
+    const int64_t e = elist[s];
+    if (s > s0 && elist[s - 1] == e) continue;
+    int32_t nodes[8];
+    double X[8][3];
+    load_element(nn, ne, xyz, en, e, nodes, X);
+    const double r = rho ? rho[e] : rho_all;
+    for (int a = 0; a < 8; ++a) {
+      if (nodes[a] != (int32_t)(i + 1)) continue;
+      double m0, m1, m2;
+      hex8_mass_node(X, a, r, ax, ay, az, m0, m1, m2);
+      f0 = f0 + m0;
+      f1 = f1 + m1;
+      f2 = f2 + m2;
+    }
+  }
+  rhs[3 * i] = f0;
+  rhs[3 * i + 1] = f1;
+  rhs[3 * i + 2] = f2;
+}
+
+// StressMatrixFEMDomain without GaussPoint (FEMDomainClass.f90:11912-12032): the cell
+// average over the 8 Gauss points of Stressvec = matmul(Dmat, matmul(Bmat, ElemDisp)),
+// accumulated into the 3x3 tensor in the reference's order.  Sums skip the structural
+// zeros of B and D (adding a signed zero does not change a sum that starts at +0).
+// out[e*9 + 3j + i] = sigma(i+1, j+1); E, nu per element (stride 1) or one value (stride 0).
+__global__ void __launch_bounds__(128) k_cell_stress(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                                                     const int32_t *__restrict__ en, const double *__restrict__ disp,
+                                                     const double *__restrict__ E, int sE,
+                                                     const double *__restrict__ nu, int snu, int jmode,
+                                                     double *__restrict__ out) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int32_t nodes[8];
+  double X[8][3];
+  load_element(nn, ne, xyz, en, e, nodes, X);
+  double u[24];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[3 * a + d] = disp[3 * (int64_t)(nodes[a] - 1) + d];
+  const double Ee = E[sE ? e : 0], ve = nu[snu ? e : 0];
+  const double mu = Ee / 2.0 / (1.0 + ve);
+  const double lam = ve * Ee / (1.0 + ve) / (1.0 - 2.0 * ve);
+  const double dg = 2.0 * mu + lam;
+  double S00 = 0.0, S11 = 0.0, S22 = 0.0, S01 = 0.0, S12 = 0.0, S02 = 0.0, S10 = 0.0, S21 = 0.0, S20 = 0.0;
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    double G[3][8];
+    hex8_gp_G(X, gp, jmode, G, true);
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      e0 = e0 + G[0][q] * u[3 * q];
+      e1 = e1 + G[1][q] * u[3 * q + 1];
+      e2 = e2 + G[2][q] * u[3 * q + 2];
+      e3 = e3 + G[1][q] * u[3 * q];
+      e3 = e3 + G[0][q] * u[3 * q + 1];
+      e4 = e4 + G[2][q] * u[3 * q + 1];
+      e4 = e4 + G[1][q] * u[3 * q + 2];
+      e5 = e5 + G[2][q] * u[3 * q];
+      e5 = e5 + G[0][q] * u[3 * q + 2];
+    }
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    v0 = v0 + dg * e0;
+    v0 = v0 + lam * e1;
+    v0 = v0 + lam * e2;
+    v1 = v1 + lam * e0;
+    v1 = v1 + dg * e1;
+    v1 = v1 + lam * e2;
+    v2 = v2 + lam * e0;
+    v2 = v2 + lam * e1;
+    v2 = v2 + dg * e2;
+    const double v3 = 0.0 + mu * e3, v4 = 0.0 + mu * e4, v5 = 0.0 + mu * e5;
+    S00 = S00 + v0;
+    S11 = S11 + v1;
+    S22 = S22 + v2;
+    S01 = S01 + v3;
+    S12 = S12 + v4;
+    S02 = S02 + v5;
+    S10 = S10 + v3;
+    S21 = S21 + v4;
+    S20 = S20 + v5;
+  }
+  double *o = out + e * 9;
+  o[0] = S00 / 8.0;
+  o[1] = S10 / 8.0;
+  o[2] = S20 / 8.0;
+  o[3] = S01 / 8.0;
+  o[4] = S11 / 8.0;
+  o[5] = S21 / 8.0;
+  o[6] = S02 / 8.0;
+  o[7] = S12 / 8.0;
+  o[8] = S22 / 8.0;
+}
+
 // ---- adjoint sensitivity: out[e] = sum_l lam_e(l)^T Ke(E_e) u_e(l) ----------------------
 // Ke is linear in E, so this is d/d(log E_e) of sum_l lam(l)^T K u(l).  The element matrices
 // of a chunk of elements come from k_ke_batch (same arithmetic as the assembly); here one
@@ -559,6 +756,46 @@ int pf_elem_bilinear(pf_ctx *ctx, const NodeGraph &g, const double *E, const dou
     PF_HIP(hipGetLastError());
   }
   PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int pf_hex8_mass_batch(pf_ctx *ctx, const NodeGraph &g, double rho, const double *accel, int64_t first,
+                       int64_t count, double *out) {
+  if (g.nne != 8 || !g.has_coords) {
+    pf_set_error("mass vectors need an 8-node mesh with coordinates");
+    return PF_INVALID;
+  }
+  if (count <= 0) return PF_OK;
+  k_mass_vec_batch<<<pf_blocks(count * 8, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, rho,
+                                                                      accel[0], accel[1], accel[2], first, count, out);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_body_force(pf_ctx *ctx, const NodeGraph &g, const double *rho_dev, double rho_all, const double *accel,
+                  double *rhs) {
+  if (g.nne != 8 || !g.built || !g.has_coords) {
+    pf_set_error("body force needs an 8-node mesh with coordinates and its node->element lists");
+    return PF_INVALID;
+  }
+  if (g.nn == 0) return PF_OK;
+  k_body_force<<<pf_blocks(g.nn, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, g.eoff.p,
+                                                              g.elist.p, rho_dev, rho_all, accel[0], accel[1],
+                                                              accel[2], rhs);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_cell_stress(pf_ctx *ctx, const NodeGraph &g, const double *disp, const double *E, int sE, const double *nu,
+                   int snu, int jmode, double *out) {
+  if (g.nne != 8 || !g.has_coords) {
+    pf_set_error("stress recovery needs an 8-node mesh with coordinates");
+    return PF_INVALID;
+  }
+  if (g.ne == 0) return PF_OK;
+  k_cell_stress<<<pf_blocks(g.ne, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, disp, E, sE,
+                                                               nu, snu, jmode, out);
+  PF_HIP(hipGetLastError());
   return PF_OK;
 }
 

@@ -220,6 +220,56 @@ extern "C" int pf_hex8_stiffness(pf_ctx *ctx, int64_t nn, int64_t ne, const doub
   return download(ctx, ke, dke.p, (size_t)count * 576);
 }
 
+extern "C" int pf_hex8_mass_vector(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord,
+                                   const int32_t *elemnod, double density, const double *accel, int64_t first,
+                                   int64_t count, double *mv) {
+  PF_CHECK(check_ctx(ctx));
+  if (first < 1 || count < 0 || first - 1 + count > ne || !accel || !mv || !nodcoord) {
+    pf_set_error("pf_hex8_mass_vector: element range %lld..%lld outside 1..%lld or null argument",
+                 (long long)first, (long long)(first + count - 1), (long long)ne);
+    return PF_INVALID;
+  }
+  DeviceGuard guard(ctx);
+  NodeGraph g;
+  PF_CHECK(set_mesh(ctx, g, nn, ne, 8, nodcoord, elemnod));
+  PF_CHECK(pf_graph_check_mesh(ctx, g));
+  DevBuf<double> dmv;
+  PF_CHECK(dmv.ensure((size_t)count * 24 + 1));
+  PF_CHECK(pf_hex8_mass_batch(ctx, g, density, accel, first - 1, count, dmv.p));
+  return download(ctx, mv, dmv.p, (size_t)count * 24);
+}
+
+static int stress_params(int64_t ne, int64_t nE, int64_t nnu, const double *E, const double *nu) {
+  if (!E || !nu || !(nE == 1 || nE == ne) || !(nnu == 1 || nnu == ne)) {
+    pf_set_error("stress: size(E) and size(v) must be 1 or the number of elements");
+    return PF_INVALID;
+  }
+  return PF_OK;
+}
+
+extern "C" int pf_element_stress(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                                 const double *disp, const double *E, int64_t nE, const double *nu, int64_t nnu,
+                                 int32_t jmode, double *sigma) {
+  PF_CHECK(check_ctx(ctx));
+  PF_CHECK(stress_params(ne, nE, nnu, E, nu));
+  if (!disp || !sigma || !nodcoord) {
+    pf_set_error("pf_element_stress: null argument");
+    return PF_INVALID;
+  }
+  DeviceGuard guard(ctx);
+  NodeGraph g;
+  PF_CHECK(set_mesh(ctx, g, nn, ne, 8, nodcoord, elemnod));
+  PF_CHECK(pf_graph_check_mesh(ctx, g));
+  DevBuf<double> du, dE, dnu, ds;
+  PF_CHECK(upload(ctx, du, disp, (size_t)(3 * nn)));
+  PF_CHECK(upload(ctx, dE, E, (size_t)nE));
+  PF_CHECK(upload(ctx, dnu, nu, (size_t)nnu));
+  PF_CHECK(ds.ensure((size_t)ne * 9 + 1));
+  PF_CHECK(pf_cell_stress(ctx, g, du.p, dE.p, nE == ne && ne > 1 ? 1 : 0, dnu.p, nnu == ne && ne > 1 ? 1 : 0,
+                          jmode, ds.p));
+  return download(ctx, sigma, ds.p, (size_t)ne * 9);
+}
+
 extern "C" int pf_hex8_stiffness_crs(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord,
                                      const int32_t *elemnod, const double *E, const double *nu, int32_t jmode,
                                      const int64_t *row_ptr, const int32_t *col_idx, double *val) {
@@ -860,4 +910,42 @@ extern "C" int pf_system_elem_sensitivity_dev(pf_system *s, int32_t nrhs, const 
                             s->has_fix ? s->fixed.p : nullptr, out_dev));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   return PF_OK;
+}
+
+// ============================================================================ body force / stress
+extern "C" int pf_system_add_body_force(pf_system *s, const double *density, double density_all,
+                                        const double *accel) {
+  PF_CHECK(sys_check(s, true));
+  if (!s->has_mesh || s->g.nne != 8 || s->dof != 3 || !accel) {
+    pf_set_error("pf_system_add_body_force: hex8 mesh with a DOF=3 CRS and an acceleration required");
+    return PF_INVALID;
+  }
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  if (!s->g.built) PF_CHECK(pf_graph_node_elems(ctx, s->g));
+  DevBuf<double> drho;
+  if (density) PF_CHECK(upload(ctx, drho, density, (size_t)s->g.ne));
+  PF_CHECK(pf_body_force(ctx, s->g, density ? drho.p : nullptr, density_all, accel, s->rhs.p));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  ++s->rhs_version;
+  return PF_OK;
+}
+
+extern "C" int pf_system_element_stress(pf_system *s, const double *E, int64_t nE, const double *nu, int64_t nnu,
+                                        int32_t jmode, double *sigma) {
+  PF_CHECK(sys_check(s, true));
+  if (!s->has_mesh || s->g.nne != 8 || s->dof != 3 || !sigma) {
+    pf_set_error("pf_system_element_stress: hex8 mesh with a DOF=3 system required");
+    return PF_INVALID;
+  }
+  PF_CHECK(stress_params(s->g.ne, nE, nnu, E, nu));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  DevBuf<double> dE, dnu, ds;
+  PF_CHECK(upload(ctx, dE, E, (size_t)nE));
+  PF_CHECK(upload(ctx, dnu, nu, (size_t)nnu));
+  PF_CHECK(ds.ensure((size_t)s->g.ne * 9 + 1));
+  PF_CHECK(pf_cell_stress(ctx, s->g, s->x.p, dE.p, nE == s->g.ne && s->g.ne > 1 ? 1 : 0, dnu.p,
+                          nnu == s->g.ne && s->g.ne > 1 ? 1 : 0, jmode, ds.p));
+  return download(ctx, sigma, ds.p, (size_t)s->g.ne * 9);
 }

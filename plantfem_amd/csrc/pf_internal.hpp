@@ -135,6 +135,15 @@ int pf_add_element_mats(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count,
 int pf_add_element_vecs(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count, const int32_t *ids,
                         const double *vec, double *rhs);
 int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double pressure, double *rhs);
+// MassVectorFEMDomain of elements first..first+count-1 (0-based first): out[q*24 + 3a + k]
+int pf_hex8_mass_batch(pf_ctx *ctx, const NodeGraph &g, double rho, const double *accel, int64_t first,
+                       int64_t count, double *out);
+// rhs += sum of element mass vectors (rho per element from rho_dev, else rho_all), ascending element order
+int pf_body_force(pf_ctx *ctx, const NodeGraph &g, const double *rho_dev, double rho_all, const double *accel,
+                  double *rhs);
+// cell-averaged Cauchy stress per element, out[e*9 + 3j + i]; sE / snu: 1 per element, 0 one value
+int pf_cell_stress(pf_ctx *ctx, const NodeGraph &g, const double *disp, const double *E, int sE, const double *nu,
+                   int snu, int jmode, double *out);
 // out[e] = sum_l lam_e(l)^T Ke(E_e, nu_e) u_e(l) over R interleaved cases (device arrays;
 // fixed DOFs, if given, excluded)
 int pf_elem_bilinear(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode, int R,

@@ -190,6 +190,98 @@ class FEMDomain:
         return ke.reshape(count, 24, 24).transpose(0, 2, 1)
 
 
+    def resize(self, x=None, y=None, z=None):
+        """resizeFEMDomain(x=, y=, z=) -> resizeMeshobj x_len/y_len/z_len (MeshClass.f90:892-933):
+        rate = new length / (max - min), coordinates = rate * coordinates."""
+        X = np.array(self.nodcoord, dtype=np.float64, order="F")
+        for d, L_new in enumerate((x, y, z)):
+            if L_new is None:
+                continue
+            old = X[:, d].max() - X[:, d].min()
+            X[:, d] = (float(L_new) / old) * X[:, d]
+        self.nodcoord = X
+        return self
+
+    def MassVector(self, ElementID, Density: float = 1.0, DOF: int = 3, Accel=(0.0, 0.0, 0.0)) -> np.ndarray:
+        """MassVectorFEMDomain (FEMDomainClass.f90:10993-11073): rho sum_gp N accel detJ, 24 values
+        per element (one element id) or (count, 24) for an array of ids."""
+        if DOF != 3:
+            raise ValueError("DOF=3 only on this path")
+        lib = L.load()
+        xyz, en = self._flat()
+        ids = np.atleast_1d(np.asarray(ElementID, dtype=np.int64))
+        acc = _f64(Accel)
+        out = np.empty((ids.size, 24))
+        for q, e in enumerate(ids):
+            L.check(lib.pf_hex8_mass_vector(self._ctx().h, self.nn(), self.ne(), L.ptr(xyz, C.c_double),
+                                            L.ptr(en, C.c_int32), float(Density), L.ptr(acc, C.c_double), int(e), 1,
+                                            L.ptr(out[q], C.c_double)), "MassVector")
+        return out[0] if np.ndim(ElementID) == 0 else out
+
+    def StressMatrices(self, displacement, E, v, jmode: int = 0) -> np.ndarray:
+        """Cell-averaged Cauchy stress of every element, (ne, 3, 3) (StressMatrixFEMDomain without
+        GaussPoint, FEMDomainClass.f90:11912-12032)."""
+        lib = L.load()
+        xyz, en = self._flat()
+        Ea, va = _f64(np.atleast_1d(E)), _f64(np.atleast_1d(v))
+        sig = np.empty(self.ne() * 9)
+        L.check(lib.pf_element_stress(self._ctx().h, self.nn(), self.ne(), L.ptr(xyz, C.c_double), L.ptr(en, C.c_int32),
+                                      L.ptr(_f64(displacement), C.c_double), L.ptr(Ea, C.c_double), Ea.size,
+                                      L.ptr(va, C.c_double), va.size, jmode, L.ptr(sig, C.c_double)),
+                "StressMatrices")
+        return sig.reshape(self.ne(), 3, 3).transpose(0, 2, 1)
+
+    def getElementCauchyStress(self, displacement, E, v, i=None, j=None, option=None) -> np.ndarray:
+        """getElementCauchyStressFEMDomain (FEMDomainClass.f90:15249-15380): per element sigma(i,j) or
+        an invariant ("p", "I1", "I2", "I3", "J1", "J2", "J3", "1,1", ...) of the cell stress."""
+        S = self.StressMatrices(displacement, E, v)
+        return stress_option(S, i, j, option)
+
+
+def _trace(S):
+    return (S[:, 0, 0] + S[:, 1, 1]) + S[:, 2, 2]
+
+
+def _det3(S):
+    # det_mat cofactor expansion along column 1 (MathClass.f90:1460-1476)
+    m1 = S[:, 1, 1] * S[:, 2, 2] - S[:, 2, 1] * S[:, 1, 2]
+    m2 = S[:, 0, 1] * S[:, 2, 2] - S[:, 2, 1] * S[:, 0, 2]
+    m3 = S[:, 0, 1] * S[:, 1, 2] - S[:, 1, 1] * S[:, 0, 2]
+    return ((0.0 + S[:, 0, 0] * m1) + (-S[:, 1, 0]) * m2) + S[:, 2, 0] * m3
+
+
+def stress_option(S, i=None, j=None, option=None) -> np.ndarray:
+    """The scalar getElementCauchyStress returns for each cell stress tensor S (ne, 3, 3)."""
+    if i is not None and j is not None:
+        return S[:, i - 1, j - 1].copy()
+    o = option
+    if o in ("p", "P"):
+        return _trace(S) / 3.0
+    if o in ("I", "I1", "i1", "trace", "tr", "TRACE", "TR"):
+        return _trace(S)
+    if o in ("II", "I2", "i2", "ii"):
+        return (_trace(S) * _trace(S) - _trace(S @ S)) * 0.5
+    if o in ("III", "I3", "i3", "iii"):
+        return _det3(S)
+    if o in ("J", "J1", "j1", "j", "JJ", "J2", "j2", "jj", "JJJ", "J3", "j3", "jjj"):
+        dev = S - (_trace(S) / 3.0)[:, None, None] * np.eye(3)
+        if o in ("J", "J1", "j1", "j"):
+            return _trace(dev)
+        if o in ("JJ", "J2", "j2", "jj"):
+            return (_trace(dev) * _trace(dev) - _trace(dev @ dev)) * 0.5
+        return _det3(dev)
+    comp = {"1,1": (0, 0), "(1,1)": (0, 0), "_{1,1}": (0, 0), "2,2": (1, 1), "(2,2)": (1, 1), "_{2,2}": (1, 1),
+            "3,3": (2, 2), "(3,3)": (2, 2), "_{3,3}": (2, 2)}
+    for key, ij in (("1,2", (0, 1)), ("1,3", (0, 2)), ("3,2", (1, 2))):
+        a, b = key.split(",")
+        for k in (f"{a},{b}", f"({a},{b})", f"_{{{a},{b}}}", f"{b},{a}", f"({b},{a})", f"_{{{b},{a}}}"):
+            comp[k] = ij
+    if o in comp:
+        p, q = comp[o]
+        return S[:, p, q].copy()
+    raise ValueError(f"unknown stress option {o!r}")
+
+
 class FEMSolver:
     """FEMSolver_ (FEMSolverClass.f90:13-164) for one domain, state resident on the GPU."""
 
@@ -352,6 +444,28 @@ class FEMSolver:
         L.check(code, "solve (Bicgstab devide by zero)" if code in (1, 2) else "solve", ok=ok)
         return x
 
+
+    def addBodyForce(self, Density=1.0, Accel=(0.0, 0.0, 0.0)):
+        """setVector(MassVector(ElementID=e, Density, DOF=3, Accel)) for every element in
+        ascending order (Density: one value or one per element)."""
+        lib = L.load()
+        acc = _f64(Accel)
+        if np.ndim(Density) == 0:
+            L.check(lib.pf_system_add_body_force(self.h, None, float(Density), L.ptr(acc, C.c_double)), "addBodyForce")
+        else:
+            d = _f64(Density)
+            L.check(lib.pf_system_add_body_force(self.h, L.ptr(d, C.c_double), 0.0, L.ptr(acc, C.c_double)),
+                    "addBodyForce")
+
+    def elementStress(self, E, v, jmode: int = 0) -> np.ndarray:
+        """Cell-averaged Cauchy stress (ne, 3, 3) of the current solution x."""
+        lib = L.load()
+        Ea, va = _f64(np.atleast_1d(E)), _f64(np.atleast_1d(v))
+        ne = self.domain.ne()
+        sig = np.empty(ne * 9)
+        L.check(lib.pf_system_element_stress(self.h, L.ptr(Ea, C.c_double), Ea.size, L.ptr(va, C.c_double), va.size,
+                                             jmode, L.ptr(sig, C.c_double)), "elementStress")
+        return sig.reshape(ne, 3, 3).transpose(0, 2, 1)
 
     # ---- batched form: independent meshes concatenated into this system ----------------
     def addTractions(self, node_off, boxes, pressures):

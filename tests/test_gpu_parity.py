@@ -298,3 +298,65 @@ def test_mixed_precision_full_grid(ctx):
         assert st["true_relres"] < 1e-10 and err <= 1e-6
     else:
         assert st["status"] in (1, 2, 3)
+
+
+# ------------------------------------------------------------------ body force, stress recovery
+def test_mass_vector_and_body_force_bit_exact(ctx):
+    nod, el = permuted_mesh((4, 3, 5), 7, jitter=0.2)
+    d = FEMDomain(nod, el, ctx)
+    rng = np.random.default_rng(1)
+    acc = np.array([0.4, -0.3, -9.8])
+    ids = np.arange(1, el.shape[0] + 1)
+    mv = d.MassVector(ids, Density=2.4, Accel=acc)
+    for q in range(0, el.shape[0], 7):
+        X = nod[el[q] - 1].ravel()
+        assert np.array_equal(mv[q], O.hex8_mass_vector(X, 2.4, acc))
+    rho = 1.0 + rng.random(el.shape[0])
+    s = FEMSolver(ctx)
+    s.setDomain(d)
+    s.setCRS(DOF=3)
+    s.addBodyForce(rho, acc)
+    say("body force")
+    assert np.array_equal(s.getRHS(), O.body_force(nod, el, rho, acc))
+
+
+@pytest.mark.parametrize("jmode", [0, 1])
+def test_cell_stress_bit_exact(ctx, jmode):
+    nod, el = permuted_mesh((5, 4, 3), 11, jitter=0.15)
+    rng = np.random.default_rng(jmode)
+    disp = 1e-3 * rng.standard_normal(3 * nod.shape[0])
+    E = 1e7 * np.exp(0.5 * rng.standard_normal(el.shape[0]))
+    nu = rng.uniform(0.2, 0.4, el.shape[0])
+    S = FEMDomain(nod, el, ctx).StressMatrices(disp, E, nu, jmode=jmode)
+    say("cell stress", jmode)
+    assert np.array_equal(S, O.cell_stress(nod, el, disp, E, nu, jmode))
+    # one value of E and v for every element, as the reference also accepts
+    S1 = FEMDomain(nod, el, ctx).StressMatrices(disp, [2e7], [0.3], jmode=jmode)
+    assert np.array_equal(S1, O.cell_stress(nod, el, disp, np.full(el.shape[0], 2e7),
+                                            np.full(el.shape[0], 0.3), jmode))
+
+
+def test_bar_in_gravity_known_answer_on_gpu(ctx):
+    """Tutorial/fem/benchmark_bar_in_gravity_field.f90 end to end on the device: element
+    matrices + body force, clamped base, solve, trace of the bottom cell stress."""
+    kn = GOLD["bar_in_gravity"]
+    nx, ny, nz = kn["mesh"]
+    d = FEMDomain(ctx=ctx).create("Cube3D", x_num=nx, y_num=ny, z_num=nz)
+    d.resize(z=10.0)
+    ne = d.ne()
+    s = FEMSolver(ctx)
+    s.setDomain(d)
+    s.setCRS(DOF=3)
+    s.setStiffness(np.full(ne, kn["E"]), np.full(ne, kn["nu"]))
+    s.addBodyForce(kn["rho"], [0.0, 0.0, -kn["g"]])
+    bottom = d.select(z_max=d.z_min())
+    for k in (2, 1, 0):
+        s.fix(1, bottom * 3 - k, FixValue=0.0)
+    x = s.solve()
+    say("bar solve", s.last_stats["iterations"], s.last_stats["status"])
+    I1 = d.getElementCauchyStress(x, [kn["E"]], [kn["nu"]], option="I1")
+    I1s = np.trace(s.elementStress([kn["E"]], [kn["nu"]]), axis1=1, axis2=2)
+    say("I1 bottom element", I1[0], I1s[0])
+    assert abs(I1[0] - kn["theory_kPa"]) <= 1e-6 * abs(kn["theory_kPa"])
+    assert abs(I1[0] - kn["simulated_kPa"]) <= 1e-5 * abs(kn["simulated_kPa"])
+    assert abs(I1s[0] - I1[0]) <= 1e-12 * abs(I1[0])

@@ -12,7 +12,7 @@ import scipy.sparse.linalg as spl
 
 from oracle import oracle as O
 from plantfem_amd import workloads as W
-from plantfem_amd.fem import cube3d
+from plantfem_amd.fem import FEMDomain, cube3d, stress_option
 
 GOLD = json.loads((Path(__file__).parent / "golden" / "known_answers.json").read_text())
 SIGNS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
@@ -370,3 +370,101 @@ def test_uniaxial_column_closed_form():
     u = x.reshape(-1, 3)
     want = np.stack([nu * p * nod[:, 0] / E, nu * p * nod[:, 1] / E, -p * nod[:, 2] / E], axis=1)
     assert np.abs(u - want).max() <= 1e-9 * np.abs(want).max()
+
+
+# ------------------------------------------------------------------ body force, stress recovery
+def _brick(lx=2.0, ly=3.0, lz=5.0, off=(1.0, -2.0, 0.5)):
+    X = np.array([[0, 0, 0], [lx, 0, 0], [lx, ly, 0], [0, ly, 0],
+                  [0, 0, lz], [lx, 0, lz], [lx, ly, lz], [0, ly, lz]], float) + np.asarray(off)
+    return X
+
+
+@pytest.mark.parametrize("nu", [0.0, 0.27, 0.45])
+def test_cell_stress_patch_test(nu):
+    """A linear displacement u = A x + b gives the constant stress D sym(A) on a brick (the
+    reference's J^-T is exact there: J is diagonal), and a rigid rotation gives zero stress."""
+    X = _brick()
+    rng = np.random.default_rng(3)
+    A = 1e-3 * rng.standard_normal((3, 3))
+    b = rng.standard_normal(3)
+    ue = (X @ A.T + b).ravel()
+    E = 2.1e7
+    s = np.empty(9)
+    O.load().orc_hex8_cell_stress(O._p(np.ascontiguousarray(X.ravel()), O.C.c_double), O._p(ue, O.C.c_double),
+                                  E, nu, 0, O._p(s, O.C.c_double))
+    S = s.reshape(3, 3).T
+    eps = 0.5 * (A + A.T)
+    mu, lam = E / 2 / (1 + nu), nu * E / (1 + nu) / (1 - 2 * nu)
+    ref = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    assert np.abs(S - ref).max() <= 1e-12 * np.abs(ref).max()
+    W = np.array([[0, -1e-3, 2e-3], [1e-3, 0, -4e-4], [-2e-3, 4e-4, 0]])
+    ur = (X @ W.T + b).ravel()
+    O.load().orc_hex8_cell_stress(O._p(np.ascontiguousarray(X.ravel()), O.C.c_double), O._p(ur, O.C.c_double),
+                                  E, nu, 0, O._p(s, O.C.c_double))
+    assert np.abs(s).max() <= 1e-9 * E * 1e-3
+
+
+def test_mass_vector_closed_form():
+    """rho a V / 8 per node of a brick; on any trilinear hex the 2x2x2 rule with unit weights
+    integrates detJ exactly, so the nodal forces sum to rho a V."""
+    X = _brick()
+    mv = O.hex8_mass_vector(X.ravel(), 2.4, [0.5, 0.0, -9.8]).reshape(8, 3)
+    V = 2.0 * 3.0 * 5.0
+    assert np.allclose(mv, np.tile([2.4 * 0.5 * V / 8, 0.0, -2.4 * 9.8 * V / 8], (8, 1)), rtol=1e-14, atol=0)
+    # distorted hex: total = rho a V with V from the divergence theorem on the 12 triangles
+    rng = np.random.default_rng(5)
+    Xd = X + 0.3 * rng.standard_normal(X.shape)
+    mvd = O.hex8_mass_vector(Xd.ravel(), 1.7, [0.0, 0.0, -9.8]).reshape(8, 3)
+    faces = [(0, 3, 2, 1), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7), (4, 5, 6, 7)]
+    c = Xd.mean(axis=0)
+    V = 0.0
+    for f in faces:  # trilinear faces are bilinear patches: split each about its centroid (exact volume)
+        fc = Xd[list(f)].mean(axis=0)
+        for k in range(4):
+            p, q = Xd[f[k]], Xd[f[(k + 1) % 4]]
+            V += np.dot(np.cross(p - c, q - c), fc - c) / 6.0
+    assert abs(mvd[:, 2].sum() - (-1.7 * 9.8 * V)) <= 1e-9 * 1.7 * 9.8 * abs(V)
+
+
+def test_body_force_is_ordered_scatter_of_mass_vectors():
+    nod, el = cube3d(3, 2, 4)
+    rng = np.random.default_rng(2)
+    nod = np.asfortranarray(nod + 0.05 * rng.standard_normal(nod.shape))
+    rho = 1.0 + rng.random(el.shape[0])
+    acc = np.array([0.3, -0.2, -9.8])
+    rhs = O.body_force(nod, el, rho, acc)
+    ref = np.zeros_like(rhs)
+    for e in range(el.shape[0]):
+        X = nod[el[e] - 1].ravel()
+        mv = O.hex8_mass_vector(X, rho[e], acc)
+        for a in range(8):
+            for k in range(3):
+                r = 3 * (el[e, a] - 1) + k
+                ref[r] = ref[r] + mv[3 * a + k]
+    assert np.array_equal(rhs, ref)
+
+
+def test_bar_in_gravity_known_answer():
+    """Tutorial/fem/benchmark_bar_in_gravity_field.f90: 10x10x100 bar of length 10, E = 2.17e7,
+    nu = 0, rho = 2.4, g = 9.8, clamped base: trace of the bottom element's cell stress
+    = -rho g (L - h/2) = -234.024 kPa (the tutorial prints -234.026)."""
+    kn = GOLD["bar_in_gravity"]
+    nx, ny, nz = kn["mesh"]
+    nod, el = cube3d(nx, ny, nz)
+    d = FEMDomain(nod, el)
+    d.resize(z=10.0)
+    nod = d.nodcoord
+    ne = el.shape[0]
+    E, nu = np.full(ne, kn["E"]), np.full(ne, kn["nu"])
+    rp, ci = O.zero_matrix(el, nod.shape[0], 3)
+    val = O.assemble(nod, el, E, nu, rp, ci)
+    rhs = O.body_force(nod, el, kn["rho"], [0.0, 0.0, -kn["g"]])
+    bottom = np.nonzero(nod[:, 2] <= nod[:, 2].min())[0]
+    fixed = np.zeros(rhs.size, np.uint8)
+    for k in range(3):
+        fixed[3 * bottom + k] = 1
+    x, st, it, *_ = O.solve(rp, ci, val, rhs, fixed, np.zeros(rhs.size))
+    assert st == 0
+    I1 = stress_option(O.cell_stress(nod, el, x, E, nu), option="I1")
+    assert abs(I1[0] - kn["theory_kPa"]) <= 1e-6 * abs(kn["theory_kPa"])
+    assert abs(I1[0] - kn["simulated_kPa"]) <= 1e-5 * abs(kn["simulated_kPa"])
