@@ -349,10 +349,13 @@ def bench_adjoint(a):
     # which cancels the h^2 term, so h can be large enough to keep solve noise small
     h = 1e-3
 
-    def loss_at(lp):
-        with torch.no_grad():
-            U = static_solve(lp, F2, s2)
-            return float(((U[obs2] - D2) ** 2).sum())
+    s2.assemble(torch.exp(logE2))
+    U20 = s2.solve(F2)
+
+    def loss_at(lp):  # perturbed solve from the unperturbed solution, its residual reduced 1e-11
+        s2.assemble(torch.exp(lp))
+        U = s2.solve(F2, x0=U20, stop_mode=0, relative_er=1e-11, er0=1e-300)
+        return float(((U[obs2] - D2) ** 2).sum())
 
     rels = []
     for e in rng.choice(ne2, size=min(a.fd_elems, ne2), replace=False):
@@ -374,7 +377,8 @@ def bench_adjoint(a):
            "loss": Lval, "grad_norm": float(torch.linalg.norm(g)),
            "adjoint_solve_iterations_sum": fwd_it,
            "dof_iterations_per_s": round(2 * n * fwd_it / t_grad, 1),
-           "fd_check": {"mesh": f"{a.fd_size}^3", "elements": len(rels), "h": h, "rule": "central, Richardson (h, 2h)",
+           "fd_check": {"mesh": f"{a.fd_size}^3", "elements": len(rels), "h": h,
+                        "rule": "central, Richardson (h, 2h), perturbed solves warm-started, residual reduced 1e-11",
                         "max_rel_err": max(rels)}}
     print(json.dumps(out), flush=True)
 

@@ -60,7 +60,7 @@ typedef struct {
                             D^-1/2 A D^-1/2 with D = diag(A) (default 1) */
   int32_t inner_max;     /* precision 32: also replace the residual after this many iterations without
                             a replacement (default 256; 0 = never) */
-  int32_t reserved;
+  int32_t use_x0;        /* multi-RHS solve: X holds the initial guess on entry (default 0: x0 = 0) */
 } pf_solve_opts;
 
 typedef struct {

@@ -50,7 +50,7 @@ class SolveOpts(C.Structure):
         ("kernel_timing", C.c_int32),
         ("jacobi", C.c_int32),
         ("inner_max", C.c_int32),
-        ("reserved", C.c_int32),
+        ("use_x0", C.c_int32),
     ]
 
 

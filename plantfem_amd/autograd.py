@@ -57,16 +57,23 @@ class ElasticSystem:
                                                  C.c_void_p(self.nu.data_ptr()), self.jmode), "stiffness")
         self._assembled = E
 
-    def solve(self, F: torch.Tensor) -> torch.Tensor:
-        """U with K U = F for every column of F (n, R), fixed rows of U = 0."""
+    def solve(self, F: torch.Tensor, x0: torch.Tensor | None = None, **opts) -> torch.Tensor:
+        """U with K U = F for every column of F (n, R), fixed rows of U = 0.  x0 (n, R): initial
+        guess; opts override the system's solver options for this call."""
         F = F.detach().to(dtype=torch.float64).contiguous()
         if F.dim() == 1:
             F = F[:, None]
         n, R = F.shape
         if n != self.n:
             raise ValueError("right-hand sides must have n rows")
-        U = torch.empty_like(F)
-        o = L.default_opts(**self.opts)
+        if x0 is None:
+            U = torch.empty_like(F)
+        else:
+            U = x0.detach().to(dtype=torch.float64).reshape(n, R).clone().contiguous()
+        kw = dict(self.opts)
+        kw.update(opts)
+        kw["use_x0"] = 0 if x0 is None else 1
+        o = L.default_opts(**kw)
         per = (L.Stats * R)()
         tot = L.Stats()
         _sync()

@@ -594,9 +594,9 @@ int check_grid(pf_ctx *ctx, const Launch &L) {
   return PF_OK;
 }
 
-int choose_sub(double avg_row) {
+int choose_sub(double avg_row) {  // ~five nonzeros per lane, 4..64 lanes per CRS row
   int sub = 4;
-  while (sub < 64 && sub * 2 <= avg_row) sub *= 2;
+  while (sub < 64 && sub * 1.4142 * 5.0 < avg_row) sub *= 2;
   return sub;
 }
 

@@ -122,7 +122,7 @@ extern "C" void pf_solve_opts_default(pf_solve_opts *o) {
   o->kernel_timing = 0;
   o->jacobi = 1;
   o->inner_max = 256;
-  o->reserved = 0;
+  o->use_x0 = 0;
 }
 
 // =============================================================================== context

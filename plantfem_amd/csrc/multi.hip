@@ -7,9 +7,9 @@
 // once per wavefront (same address in every lane) and applied to R cases, and every gather
 // of x is a contiguous R-wide load.  Each case keeps its own scalars (alpha, omega, beta,
 // r.r, stop state); a case that has stopped leaves its x, r, p untouched while the others
-// continue.  Per iteration: four streaming kernels (the two products with their fused dot
-// products, the x/r update, the p update) and three one-block kernels that reduce the
-// per-block partial sums of every case in a fixed order and advance its scalars.
+// continue.  Per iteration: five streaming kernels (the two products with their fused dot
+// products, e = r - alpha y, the x/r update, the p update) and three one-block kernels that
+// reduce the per-block partial sums of every case in a fixed order and advance its scalars.
 //
 // The matrix is read in a node-block CSR copy of the system's CRS (9 values of a 3x3
 // block contiguous, one int32 node column per block), built once per matrix.
@@ -71,7 +71,10 @@ __device__ inline void node_rows(const Blk &B, int R, XF xf, OP op) {
   const int64_t g0 = (blockIdx.x * (int64_t)MBT + threadIdx.x) / RP;
   const int64_t ng = (int64_t)gridDim.x * (MBT / RP);
   const bool live = l < R;
-  for (int64_t i = g0; i < B.nn; i += ng) {
+  for (int64_t i0 = g0; i0 < B.nn; i0 += ng) {
+    // with 64 cases a wavefront owns one node: make the node (and so every matrix address
+    // below) wave-uniform so the block values come through scalar loads
+    const int64_t i = RP == 64 ? (int64_t)__builtin_amdgcn_readfirstlane((int)i0) : i0;
     const int64_t b0 = B.noff[i], b1 = B.noff[i + 1];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int64_t blk = b0; blk < b1; ++blk) {
@@ -107,19 +110,25 @@ __device__ inline void node_entries(int64_t nn, int R, OP op) {
     for (int d = 0; d < 3; ++d) op(3 * i + d, l);
 }
 
+// one-block reductions: 16 lanes per case (cases 0..63 in a 1024-thread block) sum the G
+// partials of their case in a fixed order and leave the total in the case's lane 0
+constexpr int RBT = 1024, RLANES = 16;
+
+__device__ inline double case_total(const double *__restrict__ part, int64_t row, int G) {
+  const int j = threadIdx.x % RLANES;
+  double t = 0.0;
+  for (int b = j; b < G; b += RLANES) t += part[row * G + b];
+#pragma unroll
+  for (int off = RLANES / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, RLANES);
+  return t;
+}
+
 struct Vec {
   const double *__restrict__ v;
   int R;
   __device__ double operator()(int64_t j, int l) const { return v[j * R + l]; }
 };
 
-struct EVec {
-  const double *__restrict__ r;
-  const double *__restrict__ y;
-  double alpha;  // of the calling thread's case
-  int R;
-  __device__ double operator()(int64_t j, int l) const { return r[j * R + l] - alpha * y[j * R + l]; }
-};
 
 // ---- setup: b~ = b with the fixed rows set to 0 (ubar = 0); x = 0; r = r0 = p = b~ ------
 template <int RP>
@@ -128,7 +137,7 @@ __global__ void __launch_bounds__(MBT) k_m_init(int64_t nn, int R, const double 
                                                 double *__restrict__ x, double *__restrict__ r,
                                                 double *__restrict__ r0, double *__restrict__ p,
                                                 double *__restrict__ part) {
-  double acc[1] = {0.0};
+  double acc[2] = {0.0, 0.0};
   node_entries<RP>(nn, R, [&](int64_t i, int l) {
     const int64_t g = i * R + l;
     const double v = (fixed && fixed[i]) ? 0.0 : b[g];
@@ -139,28 +148,63 @@ __global__ void __launch_bounds__(MBT) k_m_init(int64_t nn, int R, const double 
     p[g] = v;
     acc[0] += v * v;
   });
-  case_partials<RP, 1>(acc, R, part);
+  acc[1] = acc[0];
+  case_partials<RP, 2>(acc, R, part);
 }
 
-__global__ void k_m_init_finish(const double *__restrict__ part, int G, MScal *__restrict__ sc,
-                                MCtl *__restrict__ ctl) {
-  const int l = threadIdx.x;
-  if (l < ctl->nrhs) {
-    double t = 0.0;
-    for (int b = 0; b < G; ++b) t += part[(int64_t)l * G + b];
-    MScal s{};
-    s.c1 = t;
-    s.rr = t;
-    s.bnorm2 = t;
-    s.status = PF_NOT_CONVERGED;
-    s.done = (ctl->itrmax <= 0 || !(t > 0.0)) ? 1 : 0;  // b = 0: x = 0 is the solution
-    if (!(t > 0.0)) s.status = PF_OK;
-    sc[l] = s;
+// with an initial guess: x_fixed = 0, r = b~ - A x, r0 = p = r (b~ already written by k_m_init)
+template <int RP>
+__global__ void __launch_bounds__(MBT) k_m_init_x0(Blk B, int R, const double *__restrict__ bt,
+                                                   const double *__restrict__ x, double *__restrict__ r,
+                                                   double *__restrict__ r0, double *__restrict__ p,
+                                                   double *__restrict__ part) {
+  double acc[2] = {0.0, 0.0};
+  node_rows<RP>(B, R, Vec{x, R}, [&](int64_t i, int l, double ax) {
+    const int64_t g = i * R + l;
+    const double v = bt[g] - ax;
+    r[g] = v;
+    r0[g] = v;
+    p[g] = v;
+    acc[0] += v * v;
+    acc[1] += bt[g] * bt[g];
+  });
+  case_partials<RP, 2>(acc, R, part);
+}
+
+template <int RP>
+__global__ void __launch_bounds__(MBT) k_m_fix_x0(int64_t nn, int R, const double *__restrict__ b,
+                                                  const uint8_t *__restrict__ fixed, double *__restrict__ bt,
+                                                  double *__restrict__ x) {
+  node_entries<RP>(nn, R, [&](int64_t i, int l) {
+    const int64_t g = i * R + l;
+    const bool f = fixed && fixed[i];
+    bt[g] = f ? 0.0 : b[g];
+    if (f) x[g] = 0.0;
+  });
+}
+
+__global__ void __launch_bounds__(RBT) k_m_init_finish(const double *__restrict__ part, int G, MScal *__restrict__ sc,
+                                                       MCtl *__restrict__ ctl) {
+  const int l = threadIdx.x / RLANES;
+  const int R = ctl->nrhs;
+  if (l < R) {
+    const double t = case_total(part, l, G);      // r.r
+    const double bb = case_total(part, R + l, G);  // b.b
+    if (threadIdx.x % RLANES == 0) {
+      MScal s{};
+      s.c1 = t;
+      s.rr = t;
+      s.bnorm2 = bb;
+      s.status = PF_NOT_CONVERGED;
+      s.done = (ctl->itrmax <= 0 || !(t > 0.0)) ? 1 : 0;  // r = 0: x (0, or the guess) solves it
+      if (!(t > 0.0)) s.status = PF_OK;
+      sc[l] = s;
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     int all = 1;
-    for (int k = 0; k < ctl->nrhs; ++k) all &= sc[k].done;
+    for (int k = 0; k < R; ++k) all &= sc[k].done;
     ctl->all_done = all;
   }
 }
@@ -179,43 +223,55 @@ __global__ void __launch_bounds__(MBT) k_m_spmv_p(Blk B, int R, const double *__
   case_partials<RP, 1>(acc, R, part);
 }
 
-__global__ void k_m_alpha(const double *__restrict__ part, int G, MScal *__restrict__ sc, const MCtl *__restrict__ ctl) {
-  const int l = threadIdx.x;
+__global__ void __launch_bounds__(RBT) k_m_alpha(const double *__restrict__ part, int G, MScal *__restrict__ sc,
+                                                 const MCtl *__restrict__ ctl) {
+  const int l = threadIdx.x / RLANES;
   if (ctl->all_done || l >= ctl->nrhs || sc[l].done) return;
-  double t = 0.0;
-  for (int b = 0; b < G; ++b) t += part[(int64_t)l * G + b];
-  sc[l].c2 = t;
-  sc[l].alpha = sc[l].c1 / t;
+  const double t = case_total(part, l, G);
+  if (threadIdx.x % RLANES == 0) {
+    sc[l].c2 = t;
+    sc[l].alpha = sc[l].c1 / t;
+  }
 }
 
 // ---- K2: v = A e, e = r - alpha y ; partials e.v, v.v ---------------------------------
+// e = r - alpha y, stored once so the product below gathers one vector instead of two (with
+// 64 cases the gathers, 27 x 1.5 KB per node, dominate the product's traffic)
 template <int RP>
-__global__ void __launch_bounds__(MBT) k_m_spmv_e(Blk B, int R, const double *__restrict__ r,
-                                                  const double *__restrict__ y, double *__restrict__ v,
-                                                  const MScal *__restrict__ sc, const MCtl *__restrict__ ctl,
-                                                  double *__restrict__ part) {
+__global__ void __launch_bounds__(MBT) k_m_evec(int64_t nn, int R, const double *__restrict__ r,
+                                                const double *__restrict__ y, double *__restrict__ e,
+                                                const MScal *__restrict__ sc, const MCtl *__restrict__ ctl) {
   if (ctl->all_done) return;
   const int lc = threadIdx.x % RP;
-  const EVec ef{r, y, lc < R ? sc[lc].alpha : 0.0, R};
+  const double al = lc < R ? sc[lc].alpha : 0.0;
+  node_entries<RP>(nn, R, [&](int64_t i, int l) {
+    const int64_t g = i * R + l;
+    e[g] = r[g] - al * y[g];
+  });
+}
+
+template <int RP>
+__global__ void __launch_bounds__(MBT) k_m_spmv_e(Blk B, int R, const double *__restrict__ e,
+                                                  double *__restrict__ v, const MCtl *__restrict__ ctl,
+                                                  double *__restrict__ part) {
+  if (ctl->all_done) return;
   double acc[2] = {0.0, 0.0};
-  node_rows<RP>(B, R, ef, [&](int64_t i, int l, double vv) {
+  node_rows<RP>(B, R, Vec{e, R}, [&](int64_t i, int l, double vv) {
     v[i * R + l] = vv;
-    const double e = ef(i, l);
-    acc[0] += e * vv;
+    acc[0] += e[i * R + l] * vv;
     acc[1] += vv * vv;
   });
   case_partials<RP, 2>(acc, R, part);
 }
 
-__global__ void k_m_omega(const double *__restrict__ part, int G, MScal *__restrict__ sc, const MCtl *__restrict__ ctl) {
-  const int l = threadIdx.x;
+__global__ void __launch_bounds__(RBT) k_m_omega(const double *__restrict__ part, int G, MScal *__restrict__ sc,
+                                                 const MCtl *__restrict__ ctl) {
+  const int l = threadIdx.x / RLANES;
   if (ctl->all_done || l >= ctl->nrhs || sc[l].done) return;
   const int R = ctl->nrhs;
-  double ev = 0.0, vv = 0.0;
-  for (int b = 0; b < G; ++b) {
-    ev += part[(int64_t)l * G + b];
-    vv += part[(int64_t)(R + l) * G + b];
-  }
+  const double ev = case_total(part, l, G);
+  const double vv = case_total(part, R + l, G);
+  if (threadIdx.x % RLANES != 0) return;
   if (vv == 0.0 || !isfinite(sc[l].alpha)) {  // FEMSolverClass.f90:2266
     sc[l].status = vv == 0.0 ? PF_BREAKDOWN_VV : PF_BREAKDOWN_C2C3;
     sc[l].itr += vv == 0.0 ? 1 : 0;
@@ -229,7 +285,7 @@ __global__ void k_m_omega(const double *__restrict__ part, int G, MScal *__restr
 template <int RP>
 __global__ void __launch_bounds__(MBT) k_m_update(int64_t nn, int R, double *__restrict__ x, double *__restrict__ r,
                                                   const double *__restrict__ r0, const double *__restrict__ p,
-                                                  const double *__restrict__ y, const double *__restrict__ v,
+                                                  const double *__restrict__ e, const double *__restrict__ v,
                                                   const MScal *__restrict__ sc, const MCtl *__restrict__ ctl,
                                                   double *__restrict__ part) {
   if (ctl->all_done) return;
@@ -240,9 +296,9 @@ __global__ void __launch_bounds__(MBT) k_m_update(int64_t nn, int R, double *__r
   node_entries<RP>(nn, R, [&](int64_t i, int l) {
     if (skip) return;
     const int64_t g = i * R + l;
-    const double e = r[g] - al * y[g];
-    x[g] = x[g] + al * p[g] + om * e;
-    const double rn = e - om * v[g];
+    const double eg = e[g];
+    x[g] = x[g] + al * p[g] + om * eg;
+    const double rn = eg - om * v[g];
     r[g] = rn;
     acc[0] += rn * rn;
     acc[1] += r0[g] * rn;
@@ -250,37 +306,38 @@ __global__ void __launch_bounds__(MBT) k_m_update(int64_t nn, int R, double *__r
   case_partials<RP, 2>(acc, R, part);
 }
 
-__global__ void k_m_beta(const double *__restrict__ part, int G, MScal *__restrict__ sc, MCtl *__restrict__ ctl) {
-  const int l = threadIdx.x;
+__global__ void __launch_bounds__(RBT) k_m_beta(const double *__restrict__ part, int G, MScal *__restrict__ sc,
+                                                MCtl *__restrict__ ctl) {
+  const int l = threadIdx.x / RLANES;
   const int R = ctl->nrhs;
-  if (!ctl->all_done && l < R && !sc[l].done) {
-    double rr = 0.0, r0r = 0.0;
-    for (int b = 0; b < G; ++b) {
-      rr += part[(int64_t)l * G + b];
-      r0r += part[(int64_t)(R + l) * G + b];
+  const bool live = !ctl->all_done && l < R && !sc[l].done;
+  if (live) {
+    const double rr = case_total(part, l, G);
+    const double r0r = case_total(part, R + l, G);
+    if (threadIdx.x % RLANES == 0) {
+      MScal s = sc[l];
+      s.itr += 1;
+      s.rr = rr;
+      if (s.itr == 1) s.rr1 = rr;
+      const bool stop = ctl->stop_mode == 0 ? (sqrt(rr / s.rr1) < ctl->rel || sqrt(rr) < ctl->er0)
+                                            : sqrt(rr) < ctl->rel * sqrt(s.bnorm2);
+      const double c2w = s.c2 * s.omega;
+      const double beta = r0r / c2w;
+      if (stop) {
+        s.status = PF_OK;
+        s.done = 1;
+      } else if (c2w == 0.0 || !isfinite(beta) || !isfinite(rr)) {  // FEMSolverClass.f90:2295
+        s.status = PF_BREAKDOWN_C2C3;
+        s.done = 1;
+      } else if (s.itr >= ctl->itrmax) {
+        s.status = PF_NOT_CONVERGED;
+        s.done = 1;
+      } else {
+        s.c1 = r0r;
+        s.beta = beta;
+      }
+      sc[l] = s;
     }
-    MScal s = sc[l];
-    s.itr += 1;
-    s.rr = rr;
-    if (s.itr == 1) s.rr1 = rr;
-    const bool stop = ctl->stop_mode == 0 ? (sqrt(rr / s.rr1) < ctl->rel || sqrt(rr) < ctl->er0)
-                                          : sqrt(rr) < ctl->rel * sqrt(s.bnorm2);
-    const double c2w = s.c2 * s.omega;
-    const double beta = r0r / c2w;
-    if (stop) {
-      s.status = PF_OK;
-      s.done = 1;
-    } else if (c2w == 0.0 || !isfinite(beta) || !isfinite(rr)) {  // FEMSolverClass.f90:2295
-      s.status = PF_BREAKDOWN_C2C3;
-      s.done = 1;
-    } else if (s.itr >= ctl->itrmax) {
-      s.status = PF_NOT_CONVERGED;
-      s.done = 1;
-    } else {
-      s.c1 = r0r;
-      s.beta = beta;
-    }
-    sc[l] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0 && !ctl->all_done) {
@@ -355,7 +412,7 @@ struct MultiWork {
   DevBuf<double> bval;
   bool have_bcsr = false;
   uint64_t mat_version = ~0ull;
-  DevBuf<double> bt, r, r0, p, y, v, part;
+  DevBuf<double> bt, r, r0, p, y, v, e, part;
   DevBuf<MScal> sc;
   DevBuf<MCtl> ctl;
   MCtl *h_ctl = nullptr;
@@ -400,12 +457,13 @@ int enqueue_chunk(pf_ctx *ctx, MultiWork *w, const MLaunch &L, const Blk &B, dou
   double *p1 = w->part.p, *p2 = w->part.p + (size_t)R * G, *p3 = w->part.p + (size_t)3 * R * G;
   for (int it = 0; it < count; ++it) {
     PF_DISPATCH_RP(L.rp, (k_m_spmv_p<RP_><<<L.grid, MBT, 0, st>>>(B, R, w->p.p, w->y.p, w->r0.p, w->ctl.p, p1)));
-    k_m_alpha<<<1, 64, 0, st>>>(p1, G, w->sc.p, w->ctl.p);
-    PF_DISPATCH_RP(L.rp, (k_m_spmv_e<RP_><<<L.grid, MBT, 0, st>>>(B, R, w->r.p, w->y.p, w->v.p, w->sc.p, w->ctl.p, p2)));
-    k_m_omega<<<1, 64, 0, st>>>(p2, G, w->sc.p, w->ctl.p);
-    PF_DISPATCH_RP(L.rp, (k_m_update<RP_><<<L.grid, MBT, 0, st>>>(B.nn, R, x, w->r.p, w->r0.p, w->p.p, w->y.p, w->v.p,
+    k_m_alpha<<<1, RBT, 0, st>>>(p1, G, w->sc.p, w->ctl.p);
+    PF_DISPATCH_RP(L.rp, (k_m_evec<RP_><<<L.grid, MBT, 0, st>>>(B.nn, R, w->r.p, w->y.p, w->e.p, w->sc.p, w->ctl.p)));
+    PF_DISPATCH_RP(L.rp, (k_m_spmv_e<RP_><<<L.grid, MBT, 0, st>>>(B, R, w->e.p, w->v.p, w->ctl.p, p2)));
+    k_m_omega<<<1, RBT, 0, st>>>(p2, G, w->sc.p, w->ctl.p);
+    PF_DISPATCH_RP(L.rp, (k_m_update<RP_><<<L.grid, MBT, 0, st>>>(B.nn, R, x, w->r.p, w->r0.p, w->p.p, w->e.p, w->v.p,
                                                                   w->sc.p, w->ctl.p, p3)));
-    k_m_beta<<<1, 64, 0, st>>>(p3, G, w->sc.p, w->ctl.p);
+    k_m_beta<<<1, RBT, 0, st>>>(p3, G, w->sc.p, w->ctl.p);
     PF_DISPATCH_RP(L.rp, (k_m_pupdate<RP_><<<L.grid, MBT, 0, st>>>(B.nn, R, w->p.p, w->r.p, w->y.p, w->sc.p, w->ctl.p)));
   }
   PF_HIP(hipGetLastError());
@@ -469,16 +527,18 @@ int pf_multi_solve_device(pf_ctx *ctx, MultiWork **pw, int64_t n, const int64_t 
   PF_CHECK(w->p.ensure(nv));
   PF_CHECK(w->y.ensure(nv));
   PF_CHECK(w->v.ensure(nv));
+  PF_CHECK(w->e.ensure(nv));
   PF_CHECK(w->sc.ensure(64));
   PF_CHECK(w->ctl.ensure(1));
   MLaunch L;
   L.rp = rp_of(R);
   {
-    // enough resident waves for HBM latency, at most 8 blocks per CU (partials are re-read
-    // by the one-block reductions)
+    // at most 4 blocks per CU (16 waves): every block leaves one partial per case that the
+    // one-block reductions sum (measured at 40^3, 64 cases: 8 blocks/CU -> 106 us of
+    // reductions per iteration)
     const int64_t nodes_per_block = MBT / L.rp;
     int64_t g = (nn + nodes_per_block - 1) / nodes_per_block;
-    const int64_t cap = (int64_t)ctx->num_cus * 8;
+    const int64_t cap = (int64_t)ctx->num_cus * 4;
     L.grid = (unsigned)std::max<int64_t>(1, std::min(g, cap));
   }
   PF_CHECK(w->part.ensure((size_t)5 * R * L.grid + 16));
@@ -493,16 +553,22 @@ int pf_multi_solve_device(pf_ctx *ctx, MultiWork **pw, int64_t n, const int64_t 
   *w->h_ctl = hc;
   PF_HIP(hipMemcpyAsync(w->ctl.p, w->h_ctl, sizeof(MCtl), hipMemcpyHostToDevice, ctx->stream));
   PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-  PF_DISPATCH_RP(L.rp, (k_m_init<RP_><<<L.grid, MBT, 0, ctx->stream>>>(nn, R, B_dev, fixed, w->bt.p, X_dev, w->r.p,
-                                                                      w->r0.p, w->p.p, w->part.p)));
-  k_m_init_finish<<<1, 64, 0, ctx->stream>>>(w->part.p, (int)L.grid, w->sc.p, w->ctl.p);
+  if (o.use_x0) {
+    PF_DISPATCH_RP(L.rp, (k_m_fix_x0<RP_><<<L.grid, MBT, 0, ctx->stream>>>(nn, R, B_dev, fixed, w->bt.p, X_dev)));
+    PF_DISPATCH_RP(L.rp, (k_m_init_x0<RP_><<<L.grid, MBT, 0, ctx->stream>>>(B, R, w->bt.p, X_dev, w->r.p, w->r0.p,
+                                                                           w->p.p, w->part.p)));
+  } else {
+    PF_DISPATCH_RP(L.rp, (k_m_init<RP_><<<L.grid, MBT, 0, ctx->stream>>>(nn, R, B_dev, fixed, w->bt.p, X_dev,
+                                                                        w->r.p, w->r0.p, w->p.p, w->part.p)));
+  }
+  k_m_init_finish<<<1, RBT, 0, ctx->stream>>>(w->part.p, (int)L.grid, w->sc.p, w->ctl.p);
   PF_HIP(hipGetLastError());
   int chunk = o.check_every > 0 ? o.check_every : 32;
   const bool use_graph = o.use_graph != 0;
   uint64_t key[12] = {(uint64_t)L.grid, (uint64_t)L.rp, (uint64_t)R, (uint64_t)chunk, (uint64_t)n,
                       (uint64_t)(uintptr_t)X_dev, (uint64_t)(uintptr_t)w->bval.p, (uint64_t)(uintptr_t)w->r.p,
                       (uint64_t)(uintptr_t)w->p.p, (uint64_t)(uintptr_t)w->part.p, (uint64_t)(uintptr_t)w->y.p,
-                      (uint64_t)(uintptr_t)w->v.p ^ (uint64_t)(uintptr_t)w->r0.p};
+                      (uint64_t)(uintptr_t)w->v.p ^ (uint64_t)(uintptr_t)w->r0.p ^ ((uint64_t)(uintptr_t)w->e.p << 1)};
   bool graph_ok = false;
   if (use_graph) {
     if (!w->gexec || std::memcmp(w->gkey, key, sizeof(key)) != 0) {

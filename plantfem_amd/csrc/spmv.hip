@@ -169,9 +169,10 @@ template <typename V>
 int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                        const V *val, const V *x, V *y, double avg_row) {
   if (n <= 0) return PF_OK;
-  // subgroup width ~ average row length, between 4 and 64 lanes
+  // subgroup of ~avg_row/5 lanes (4..64): about five nonzeros per lane (measured at 100^3,
+  // 79 nonzeros per row: 16 lanes 0.82 ms, 64 lanes 1.15 ms)
   int sub = 4;
-  while (sub < 64 && sub * 2 <= avg_row) sub *= 2;
+  while (sub < 64 && sub * 1.4142 * 5.0 < avg_row) sub *= 2;
   const int64_t threads = n * sub;
   const unsigned blocks = pf_blocks(threads, 256, INT32_MAX);
   switch (sub) {

@@ -28,7 +28,7 @@ module plantfem_gpu
       integer(c_int32_t) :: itrmax, precision
       real(c_double) :: er0, relative_er, inner_tol, target_relres
       integer(c_int32_t) :: max_outer, spmv_kind, check_every, use_graph, debug, stop_mode, &
-                            kernel_timing, jacobi, inner_max, reserved
+                            kernel_timing, jacobi, inner_max, use_x0
    end type
 
    type, bind(C) :: pf_stats
