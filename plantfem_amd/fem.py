@@ -238,6 +238,17 @@ class FEMDomain:
         return stress_option(S, i, j, option)
 
 
+def loadMatrix(name: str) -> "CRS":
+    """The three files saveMatrix writes, back into a CRS (any whitespace-separated numbers,
+    Fortran D exponents accepted)."""
+    def nums(path, dtype):
+        with open(path) as f:
+            txt = f.read().replace("D", "E").replace("d", "e")
+        return np.array(txt.split(), dtype=np.float64).astype(dtype)
+    return CRS(nums(name + "_indptr.txt", np.int64), nums(name + "_indices.txt", np.int32),
+               nums(name + "_data.txt", np.float64))
+
+
 def _trace(S):
     return (S[:, 0, 0] + S[:, 1, 1]) + S[:, 2, 2]
 
@@ -444,6 +455,29 @@ class FEMSolver:
         L.check(code, "solve (Bicgstab devide by zero)" if code in (1, 2) else "solve", ok=ok)
         return x
 
+
+    def saveMatrix(self, name: str, CRS_as_dense: bool = False, zero_or_nonzero: bool = False):
+        """saveMatrixFEMSolver (FEMSolverClass.f90:1225-1307): name_data.txt (one value per line),
+        name_indices.txt and name_indptr.txt (one integer and a blank per line, 1-based), or
+        name_dense.csv with CRS_as_dense ("0"/"*" per entry with zero_or_nonzero)."""
+        crs = self.getCRS()
+        if CRS_as_dense:
+            with open(name + "_dense.csv", "w") as f:
+                for i in range(crs.n):
+                    row = np.zeros(crs.n)
+                    k0, k1 = crs.row_ptr[i] - 1, crs.row_ptr[i + 1] - 1
+                    row[crs.col_idx[k0:k1] - 1] = crs.val[k0:k1]
+                    if zero_or_nonzero:
+                        f.write("".join("0" if v == 0.0 else "*" for v in row) + "\n")
+                    else:
+                        f.write(",".join(repr(float(v)) for v in row) + "\n")
+            return
+        with open(name + "_data.txt", "w") as f:
+            f.write("".join(f"  {float(v)!r}\n" for v in crs.val))
+        with open(name + "_indices.txt", "w") as f:
+            f.write("".join(f"{int(c)} \n" for c in crs.col_idx))
+        with open(name + "_indptr.txt", "w") as f:
+            f.write("".join(f"{int(r)} \n" for r in crs.row_ptr))
 
     def addBodyForce(self, Density=1.0, Accel=(0.0, 0.0, 0.0)):
         """setVector(MassVector(ElementID=e, Density, DOF=3, Accel)) for every element in

@@ -259,6 +259,7 @@ module plantfem_gpu
       generic :: setCRS => setCRSFEMSolver, setCRS_CRSobjFEMSolver
       procedure :: setRHS => setRHSFEMSolver
       procedure :: getCRS => getCRSFEMSolver
+      procedure :: saveMatrix => saveMatrixFEMSolver
       procedure :: setMatrix => setMatrixFEMSolver
       procedure :: setVector => setVectorFEMSolver
       procedure :: setStiffness => setStiffnessFEMSolver
@@ -684,6 +685,30 @@ contains
       allocate (this%CRS_RHS(this%n))
       code = c_pf_system_get_rhs(this%sys, this%CRS_RHS)
       call check(code, "getRHS")
+   end subroutine
+
+   ! saveMatrixFEMSolver (FEMSolverClass.f90:1225-1307), CRS form: name_data.txt (one value per
+   ! line, list-directed), name_indices.txt and name_indptr.txt (one integer and a blank per line)
+   subroutine saveMatrixFEMSolver(this, name)
+      class(FEMSolver_), target, intent(inout) :: this
+      character(*), intent(in) :: name
+      integer :: u, i
+      call this%getCRS()
+      open (newunit=u, file=name//"_data.txt", status="replace", action="write")
+      do i = 1, size(this%CRS_val)
+         write (u, *) this%CRS_val(i)
+      end do
+      close (u)
+      open (newunit=u, file=name//"_indices.txt", status="replace", action="write")
+      do i = 1, size(this%CRS_Index_Col)
+         write (u, '(I0,A)') this%CRS_Index_Col(i), " "
+      end do
+      close (u)
+      open (newunit=u, file=name//"_indptr.txt", status="replace", action="write")
+      do i = 1, size(this%CRS_Index_Row)
+         write (u, '(I0,A)') int(this%CRS_Index_Row(i)), " "
+      end do
+      close (u)
    end subroutine
 
    subroutine removeFEMSolver(this)

@@ -360,3 +360,22 @@ def test_bar_in_gravity_known_answer_on_gpu(ctx):
     assert abs(I1[0] - kn["theory_kPa"]) <= 1e-6 * abs(kn["theory_kPa"])
     assert abs(I1[0] - kn["simulated_kPa"]) <= 1e-5 * abs(kn["simulated_kPa"])
     assert abs(I1s[0] - I1[0]) <= 1e-12 * abs(I1[0])
+
+
+def test_save_matrix_round_trip(ctx, tmp_path):
+    """saveMatrix files (FEMSolverClass.f90:1225-1307) read back into the same CRS."""
+    from plantfem_amd.fem import loadMatrix
+    w = W.box(3, sigma=0.5, seed=2)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    name = str(tmp_path / "K")
+    s.saveMatrix(name)
+    crs = s.getCRS()
+    back = loadMatrix(name)
+    assert np.array_equal(back.row_ptr, crs.row_ptr) and np.array_equal(back.col_idx, crs.col_idx)
+    assert np.array_equal(back.val, crs.val)
+    assert open(name + "_indices.txt").readline() == f"{crs.col_idx[0]} \n"
+    s.saveMatrix(name, CRS_as_dense=True, zero_or_nonzero=True)
+    rows = open(name + "_dense.csv").read().split()
+    assert len(rows) == crs.n and rows[0].count("*") == crs.row_ptr[1] - crs.row_ptr[0]
