@@ -48,7 +48,14 @@ typedef struct {
                             replacement (default 0.1) */
   double target_relres;  /* precision 32: stop when ||b-Ax||/||b|| (FP64, at a replacement) < target
                             (default 1e-8) */
-  int32_t max_outer;     /* unused, kept for layout (0) */
+  int32_t preconditioning; /* solve(preconditioning=) (FEMSolverClass.f90:1768-1790), FP64 only:
+                              0 none: bicgstab_CRS_2 (default);
+                              1 "PointJacobi": JacobiPreconditionerCRS on the eliminated system (in
+                                place, as the reference), then bicgstab_CRS_2 on the scaled system;
+                              2 "ILU": bicgstab_CRS_ILU with the ILU(0) factor of the eliminated
+                                matrix, computed at the first such solve and kept (ILU_MATRIX);
+                              3 "PointJacobi" as the reference's branch runs it: the scaling and no
+                                solver call, so x = x0 (zeros) with the fixed values */
   int32_t spmv_kind;     /* 0 auto (blocked when the CRS has DOF=3 node blocks), 1 CRS row kernel, 2 blocked */
   int32_t check_every;   /* iterations enqueued between host polls of the stop flag (default 32) */
   int32_t use_graph;     /* capture the polled chunk in a hipGraph (default 1) */
@@ -126,6 +133,25 @@ int pf_bicgstab_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_
                     const double *val, const double *rhs, double *x, const pf_solve_opts *opts,
                     pf_stats *stats);
 
+/* ---- preconditioners (solveFEMSolver preconditioning=, FEMSolverClass.f90:1768-1790) ---- */
+/* JacobiPreconditionerCRS (FEMSolverClass.f90:2508-2529) in place on host arrays: every row
+ * whose diagonal (last stored entry of the column) is nonzero is multiplied by its
+ * reciprocal, rhs(i) too (rhs may be NULL). */
+int pf_jacobi_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
+                  double *rhs);
+
+/* ILUCRS(fill_in_order=0) (SparseClass.f90:1629-1700): fval (nnz) receives the factor of val
+ * on the same pattern, columns taken in their stored order as the reference does.  Rows may
+ * hold at most 512 entries and no column twice (PF_INVALID otherwise). */
+int pf_ilu0_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                double *fval);
+
+/* ILU_matvecCRS (SparseClass.f90:1795-1845) of a factor fval: y = D^-1 L^-1 v, L the unit
+ * lower triangle of fval and D its diagonal (1 where 0); the reference's backward loop
+ * leaves its vector unchanged, so the strict upper triangle is not applied. */
+int pf_ilu_apply_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *fval,
+                     const double *v, double *y);
+
 /* ---- device-pointer entry points (torch interop) ---------------------------------- */
 /* y = A x on device arrays; 1-based CRS as above. */
 int pf_dev_spmv_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
@@ -188,6 +214,16 @@ int pf_system_fix(pf_system *sys, int64_t nids, const int32_t *ids, const double
 int pf_system_solve(pf_system *sys, const pf_solve_opts *opts, const double *x0, double *x_out,
                     pf_stats *stats);
 int pf_system_get_x(pf_system *sys, double *x);
+
+/* The ILU(0) factor kept by the system (FEMSolver_%ILU_MATRIX, FEMSolverClass.f90:67,
+ * :1777-1784): pf_system_ilu_factor computes it from the eliminated matrix when none is kept
+ * (elimination applied first) and downloads its values (fval may be NULL) and the number of
+ * levels of its schedule (levels may be NULL); pf_system_ilu_apply computes y = M^-1 v with
+ * it (host arrays of length n); pf_system_ilu_reset drops it so the next ILU solve factors
+ * the current matrix again. */
+int pf_system_ilu_factor(pf_system *sys, double *fval, int32_t *levels);
+int pf_system_ilu_apply(pf_system *sys, const double *v, double *y);
+int pf_system_ilu_reset(pf_system *sys);
 
 /* y = A x with the system's current CRS (elimination included once solve() has run).
  * kind 1: CRS row kernel, FP64; 2: blocked kernel, FP64; 3: blocked kernel, FP32 values and

@@ -34,6 +34,10 @@ _SIG = {
     "orc_spmv": (None, [I64, P64, P32, PD, PD, PD, I32]),
     "orc_bicgstab_crs2": (I32, [I64, P64, P32, PD, PD, PD, I32, D, D, I32, P32, PD, PD, PD]),
     "orc_solve": (I32, [I64, P64, P32, PD, PD, PU8, PD, PD, I32, D, D, I32, P32, PD, PD]),
+    "orc_jacobi_scale": (None, [I64, P64, P32, PD, PD]),
+    "orc_ilu0": (None, [I64, P64, P32, PD]),
+    "orc_ilu_apply": (None, [I64, P64, P32, PD, PD, PD]),
+    "orc_bicgstab_ilu": (I32, [I64, P64, P32, PD, PD, PD, PD, I32, D, D, I32, P32, PD, PD]),
 }
 
 
@@ -224,6 +228,52 @@ def solve(row_ptr, col_idx, val, rhs, fixed, ubar, x0=None, itrmax=100000, er0=N
                           _p(val, C.c_double), _p(rhs, C.c_double), _p(_c(fixed, np.uint8), C.c_uint8),
                           _p(_c(ubar, np.float64), C.c_double), _p(x, C.c_double), itrmax, er0, relative_er,
                           nthreads, C.byref(it), C.byref(rr), C.byref(rr1))
+    return x, st, it.value, rr.value, rr1.value
+
+
+def jacobi_scale(row_ptr, col_idx, val, rhs):
+    """JacobiPreconditionerCRS on copies; returns (val, rhs)."""
+    val = np.array(val, dtype=np.float64)
+    rhs = np.array(rhs, dtype=np.float64)
+    load().orc_jacobi_scale(rhs.size, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                            _p(val, C.c_double), _p(rhs, C.c_double))
+    return val, rhs
+
+
+def ilu0(row_ptr, col_idx, val):
+    """ILUCRS(0) on a copy of val: the factor values on the same pattern."""
+    f = np.array(val, dtype=np.float64)
+    load().orc_ilu0(row_ptr.size - 1, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                    _p(f, C.c_double))
+    return f
+
+
+def ilu_apply(row_ptr, col_idx, fval, v):
+    """ILU_matvecCRS of the factor fval applied to v."""
+    n = row_ptr.size - 1
+    y = np.empty(n)
+    load().orc_ilu_apply(n, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                         _p(_c(fval, np.float64), C.c_double), _p(_c(v, np.float64), C.c_double), _p(y, C.c_double))
+    return y
+
+
+def bicgstab_ilu(row_ptr, col_idx, val, b, fval=None, x0=None, itrmax=100000, er0=None, relative_er=None,
+                 nthreads=0):
+    """bicgstab_CRS_ILU; fval = ILU(0) factor (computed here when None).  Returns (x, status, iters, rr, rr1)."""
+    er = float(np.float32(1e-10))
+    er0 = er if er0 is None else er0
+    relative_er = er if relative_er is None else relative_er
+    if fval is None:
+        fval = ilu0(row_ptr, col_idx, val)
+    n = b.size
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = C.c_int(0)
+    rr = C.c_double(0)
+    rr1 = C.c_double(0)
+    st = load().orc_bicgstab_ilu(n, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                                 _p(_c(val, np.float64), C.c_double), _p(_c(fval, np.float64), C.c_double),
+                                 _p(_c(b, np.float64), C.c_double), _p(x, C.c_double), itrmax, er0, relative_er,
+                                 nthreads, C.byref(it), C.byref(rr), C.byref(rr1))
     return x, st, it.value, rr.value, rr1.value
 
 

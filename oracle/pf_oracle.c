@@ -684,3 +684,158 @@ int orc_solve(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double 
             if (fixed[i]) x[i] = ubar[i];
     return st;
 }
+
+/* ---- preconditioned forms (solveFEMSolver preconditioning=, FEMSolverClass.f90:1768-1790) ---- */
+
+/* JacobiPreconditionerCRS (FEMSolverClass.f90:2508-2529): each row with a nonzero diagonal
+ * (getCRSval, :2488-2503: the last stored entry of the column) is multiplied by 1/A_ii, the
+ * right-hand side entry too.  Rows with a zero diagonal are left as they are. */
+void orc_jacobi_scale(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, double *rhs)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double akk = 0.0;
+        for (int64_t j = row_ptr[i] - 1; j < row_ptr[i + 1] - 1; ++j)
+            if (col_idx[j] == i + 1) akk = val[j];
+        if (akk == 0.0) continue;
+        const double inv = 1.0 / akk;
+        rhs[i] = rhs[i] * inv;
+        for (int64_t j = row_ptr[i] - 1; j < row_ptr[i + 1] - 1; ++j) val[j] = val[j] * inv;
+    }
+}
+
+/* CRS get/update/add (SparseClass.f90:1500-1560): the first stored entry of the column; a
+ * column outside [min, max] of the row's columns, or absent, reads 0 and is not written. */
+static int64_t crs_find(const int64_t *row_ptr, const int32_t *col_idx, int64_t row /*0-based*/,
+                        int32_t col /*1-based*/)
+{
+    const int64_t a = row_ptr[row] - 1, b = row_ptr[row + 1] - 1;
+    if (a >= b) return -1;
+    int32_t lo = col_idx[a], hi = col_idx[a];
+    for (int64_t j = a; j < b; ++j) {
+        if (col_idx[j] < lo) lo = col_idx[j];
+        if (col_idx[j] > hi) hi = col_idx[j];
+    }
+    if (col > hi || col < lo) return -1;
+    for (int64_t j = a; j < b; ++j)
+        if (col_idx[j] == col) return j;
+    return -1;
+}
+
+/* diagCRS (SparseClass.f90:1614-1624): first stored diagonal entry, 0 when absent */
+static double crs_diag(const int64_t *row_ptr, const int32_t *col_idx, const double *val, int64_t i)
+{
+    for (int64_t j = row_ptr[i] - 1; j < row_ptr[i + 1] - 1; ++j)
+        if (col_idx[j] == i + 1) return val[j];
+    return 0.0;
+}
+
+/* ILUCRS(fill_in_order=0) (SparseClass.f90:1629-1700), in place on val.  Rows i = 2..n in
+ * order; the columns k < i of row i are taken in their STORED order (the reference notes
+ * that they should be sorted, but ZeroMatrix stores them in first-appearance order):
+ *     A_ik = get(i,k) / diag_vec(k);  update(i,k, A_ik)
+ *     for every stored column j > k of row i:  add(i,j, -A_ik * get(k,j))
+ *     diag_vec(i) = get(i,i)
+ * diag_vec starts as diagCRS of the input. */
+void orc_ilu0(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val)
+{
+    double *dv = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) dv[i] = crs_diag(row_ptr, col_idx, val, i);
+    for (int64_t i = 1; i < n; ++i) {
+        const int64_t a = row_ptr[i] - 1, b = row_ptr[i + 1] - 1;
+        for (int64_t kk = a; kk < b; ++kk) {
+            const int32_t k = col_idx[kk];             /* 1-based */
+            if (k >= i + 1) continue;
+            int64_t q = crs_find(row_ptr, col_idx, i, k);
+            const double aik = (q < 0 ? 0.0 : val[q]) / dv[k - 1];
+            if (q >= 0) val[q] = aik;
+            for (int64_t jj = a; jj < b; ++jj) {
+                const int32_t j = col_idx[jj];
+                if (j < k + 1) continue;
+                const int64_t qk = crs_find(row_ptr, col_idx, k - 1, j);
+                const double g = qk < 0 ? 0.0 : val[qk];
+                const int64_t qi = crs_find(row_ptr, col_idx, i, j);
+                if (qi >= 0) val[qi] = val[qi] + (-(aik * g));
+            }
+            const int64_t qd = crs_find(row_ptr, col_idx, i, i + 1);
+            dv[i] = qd < 0 ? 0.0 : val[qd];
+        }
+    }
+    free(dv);
+}
+
+/* ILU_matvecCRS (SparseClass.f90:1795-1845) of the factor F from orc_ilu0: y = v; forward
+ * substitution with the unit lower triangle (stored columns < i, in stored order); division
+ * by the stored diagonal (1 where it is 0).  The reference's closing backward loop writes
+ * new_vector(i) inside its column loop and then restores the value it held before the loop,
+ * so the upper triangle beyond the diagonal is never applied: y = D_U^-1 L^-1 v. */
+void orc_ilu_apply(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *fval,
+                   const double *v, double *y)
+{
+    for (int64_t i = 0; i < n; ++i) y[i] = v[i];
+    for (int64_t i = 1; i < n; ++i) {
+        double yi = y[i];
+        for (int64_t j = row_ptr[i] - 1; j < row_ptr[i + 1] - 1; ++j)
+            if (col_idx[j] < i + 1) yi = yi - fval[j] * y[col_idx[j] - 1];
+        y[i] = yi;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        double d = crs_diag(row_ptr, col_idx, fval, i);
+        if (d == 0.0) d = 1.0;
+        y[i] = y[i] / d;
+    }
+}
+
+/* bicgstab_CRS_ILU (LinearSolverClass.f90:4104-4229): right-preconditioned BiCGSTAB with
+ * M^-1 = orc_ilu_apply of the factor fval (computed by the caller, as ILU_MATRIX caches it):
+ *     r = b - A x;  return when r.r < er  (before any iteration)
+ *     y = A M^-1 p;  e = r - alp y;  v = A M^-1 e;  x += alp M^-1 p + c3 M^-1 e
+ * and the stop tests of bicgstab_CRS_2 (relative to r.r after iteration 1, then sqrt(rr) < er). */
+int orc_bicgstab_ilu(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                     const double *fval, const double *b, double *x, int itrmax, double er0,
+                     double relative_er, int nthreads, int *iters, double *rr_out, double *rr1_out)
+{
+    double *r = (double *)malloc(sizeof(double) * (size_t)n * 8);
+    double *r0 = r + n, *p = r0 + n, *y = p + n, *e = y + n, *v = e + n, *pd = v + n, *ps = pd + n;
+    double c1, c2, c3, alp, bet, ev, vv, rr = 0.0, re_er0 = 0.0;
+    int status = ORC_NOT_CONVERGED, itr = 0;
+    orc_spmv(n, row_ptr, col_idx, val, x, y, nthreads);
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - y[i];
+    c1 = dotp(n, r, r);
+    rr = c1;
+    if (c1 < er0) {
+        status = ORC_OK;
+        goto done;
+    }
+    for (int64_t i = 0; i < n; ++i) { p[i] = r[i]; r0[i] = r[i]; }
+    for (itr = 1; itr <= itrmax; ++itr) {
+        c1 = dotp(n, r0, r);
+        orc_ilu_apply(n, row_ptr, col_idx, fval, p, pd);
+        orc_spmv(n, row_ptr, col_idx, val, pd, y, nthreads);
+        c2 = dotp(n, r0, y);
+        alp = c1 / c2;
+        for (int64_t i = 0; i < n; ++i) e[i] = r[i] - alp * y[i];
+        orc_ilu_apply(n, row_ptr, col_idx, fval, e, ps);
+        orc_spmv(n, row_ptr, col_idx, val, ps, v, nthreads);
+        ev = dotp(n, e, v);
+        vv = dotp(n, v, v);
+        if (vv == 0.0) { status = ORC_BREAKDOWN_VV; break; }
+        c3 = ev / vv;
+        for (int64_t i = 0; i < n; ++i) x[i] = x[i] + alp * pd[i] + c3 * ps[i];
+        for (int64_t i = 0; i < n; ++i) r[i] = e[i] - c3 * v[i];
+        rr = dotp(n, r, r);
+        if (itr == 1) re_er0 = rr;
+        if (sqrt(rr / re_er0) < relative_er) { status = ORC_OK; break; }
+        if (sqrt(rr) < er0) { status = ORC_OK; break; }
+        c1 = dotp(n, r0, r);
+        bet = c1 / (c2 * c3);
+        if ((c2 * c3) == 0.0) { status = ORC_BREAKDOWN_C2C3; break; }
+        for (int64_t i = 0; i < n; ++i) p[i] = r[i] + bet * (p[i] - c3 * y[i]);
+    }
+    if (itr > itrmax) itr = itrmax;
+done:
+    if (iters) *iters = itr;
+    if (rr_out) *rr_out = rr;
+    if (rr1_out) *rr1_out = re_er0;
+    free(r);
+    return status;
+}

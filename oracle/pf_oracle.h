@@ -111,6 +111,21 @@ int orc_solve(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double 
               const uint8_t *fixed, const double *ubar, double *x, int itrmax, double er0,
               double relative_er, int nthreads, int *iters, double *rr_out, double *rr1_out);
 
+/* JacobiPreconditionerCRS (FEMSolverClass.f90:2508-2529): rows scaled by 1/A_ii in place. */
+void orc_jacobi_scale(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, double *rhs);
+
+/* ILUCRS(0) (SparseClass.f90:1629-1700) in place, columns k < i taken in stored order. */
+void orc_ilu0(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val);
+
+/* ILU_matvecCRS (SparseClass.f90:1795-1845): y = D_U^-1 L^-1 v (its backward loop is a no-op). */
+void orc_ilu_apply(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *fval,
+                   const double *v, double *y);
+
+/* bicgstab_CRS_ILU (LinearSolverClass.f90:4104-4229) with the factor fval from orc_ilu0. */
+int orc_bicgstab_ilu(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                     const double *fval, const double *b, double *x, int itrmax, double er0,
+                     double relative_er, int nthreads, int *iters, double *rr_out, double *rr1_out);
+
 #ifdef __cplusplus
 }
 #endif

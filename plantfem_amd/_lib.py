@@ -41,7 +41,7 @@ class SolveOpts(C.Structure):
         ("relative_er", C.c_double),
         ("inner_tol", C.c_double),
         ("target_relres", C.c_double),
-        ("max_outer", C.c_int32),
+        ("preconditioning", C.c_int32),
         ("spmv_kind", C.c_int32),
         ("check_every", C.c_int32),
         ("use_graph", C.c_int32),
@@ -144,6 +144,12 @@ SIGNATURES = {
     "pf_system_fix": (C.c_int, [P, I64, PI32, PF64, F64]),
     "pf_system_solve": (C.c_int, [P, C.POINTER(SolveOpts), PF64, PF64, C.POINTER(Stats)]),
     "pf_system_get_x": (C.c_int, [P, PF64]),
+    "pf_system_ilu_factor": (C.c_int, [P, PF64, C.POINTER(C.c_int32)]),
+    "pf_system_ilu_apply": (C.c_int, [P, PF64, PF64]),
+    "pf_system_ilu_reset": (C.c_int, [P]),
+    "pf_jacobi_crs": (C.c_int, [P, I64, PI64, PI32, PF64, PF64]),
+    "pf_ilu0_crs": (C.c_int, [P, I64, PI64, PI32, PF64, PF64]),
+    "pf_ilu_apply_crs": (C.c_int, [P, I64, PI64, PI32, PF64, PF64, PF64]),
     "pf_system_view_get": (C.c_int, [P, C.POINTER(SystemView)]),
     "pf_system_spmv": (C.c_int, [P, I32, PF64, PF64, I32, PF64]),
     "pf_system_add_tractions": (C.c_int, [P, I32, PI64, PF64, PF64]),

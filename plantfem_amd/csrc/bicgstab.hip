@@ -383,6 +383,96 @@ __global__ void __launch_bounds__(BT) k_pupdate(int64_t n, V *__restrict__ p, co
   }
 }
 
+// ---- bicgstab_CRS_ILU (LinearSolverClass.f90:4104-4229) ---------------------------------
+// Right-preconditioned: pd = M^-1 p and ps = M^-1 e come from the level-scheduled sweeps
+// (the p update and e = r - alpha y are formed inside them), so one iteration is
+//   S1 sweep (p, pd) ; K1 y = A pd ; KA alpha ; S2 sweep (e, ps) ; K2 v = A ps ;
+//   K3 x += alpha pd + omega ps, r = e - omega v ; K4 stop tests, beta (k_pupdate on no rows)
+// First iteration: beta = omega = 0 and y = 0 make the S1 update p = r + 0 (p - 0) = r.
+__global__ void __launch_bounds__(BT) k_ilu_start(Scal *__restrict__ sc, double *__restrict__ y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) y[i] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->beta = 0.0;
+    sc->omega = 0.0;
+    // "if (c1 < er0) return" on r.r before the loop (LinearSolverClass.f90:4160)
+    const bool stop = sc->stop_mode == 1 ? sqrt(sc->c1) < sc->rel * sqrt(sc->bnorm2) : sc->c1 < sc->er0;
+    if (stop && sc->itrmax > 0) {
+      sc->done = 1;
+      sc->status = PF_OK;
+    }
+  }
+}
+
+// c2 = r0.y, alpha = c1 / c2 (one block)
+__global__ void __launch_bounds__(BT) k_ilu_alpha(const double *__restrict__ pin, int Gin, Scal *__restrict__ sc) {
+  if (sc->done) return;
+  double t[1];
+  read_partials<1>(pin, Gin, t);
+  if (threadIdx.x == 0) {
+    const double alpha = sc->c1 / t[0];
+    sc->c2 = t[0];
+    sc->alpha = alpha;
+    if (sc->guard && !isfinite(alpha)) {
+      sc->status = PF_BREAKDOWN_C2C3;
+      sc->done = 1;
+    }
+  }
+}
+
+// v = A ps -> partials e.v, v.v
+template <int L, int SUB>
+__global__ void __launch_bounds__(BT) k_spmv_ps(Mat<double> m, const double *__restrict__ ps,
+                                                const double *__restrict__ e, double *__restrict__ v,
+                                                const Scal *__restrict__ sc, double *__restrict__ part) {
+  if (sc->done) return;
+  double acc[2] = {0.0, 0.0};
+  auto op = [&](int64_t row, double vv) {
+    v[row] = vv;
+    acc[0] += e[row] * vv;
+    acc[1] += vv * vv;
+  };
+  for_rows<double, L, SUB>(m, pfdev::PlainX<double>{ps}, op);
+  write_partials<2>(acc, part);
+}
+
+// omega ; x += alpha pd + omega ps ; r = e - omega v -> partials r.r, r0.r
+__global__ void __launch_bounds__(BT) k_update_ilu(int64_t n, double *__restrict__ x, double *__restrict__ r,
+                                                   const double *__restrict__ r0, const double *__restrict__ pd,
+                                                   const double *__restrict__ ps, const double *__restrict__ e,
+                                                   const double *__restrict__ v, Scal *__restrict__ sc,
+                                                   const double *__restrict__ pin, int Gin, double *__restrict__ part) {
+  if (sc->done) return;
+  double t[2];
+  read_partials<2>(pin, Gin, t);
+  const bool first = threadIdx.x == 0 && blockIdx.x == 0;
+  if (t[1] == 0.0) {  // "Bicgstab devide by zero" (LinearSolverClass.f90:4196)
+    if (first) {
+      sc->status = PF_BREAKDOWN_VV;
+      sc->done = 1;
+    }
+    return;
+  }
+  const double omega = t[0] / t[1];
+  if (sc->guard && !isfinite(omega)) {
+    if (first) {
+      sc->status = PF_BREAKDOWN_VV;
+      sc->done = 1;
+    }
+    return;
+  }
+  if (first) sc->omega = omega;
+  const double alpha = sc->alpha;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) {
+    x[i] = x[i] + alpha * pd[i] + omega * ps[i];
+    const double rn = e[i] - omega * v[i];
+    r[i] = rn;
+    acc[0] += rn * rn;
+    acc[1] += r0[i] * rn;
+  }
+  write_partials<2>(acc, part);
+}
+
 // ---- refinement (mixed precision) -----------------------------------------------------
 // r64 = b - A64 x64 -> partials |r|^2, |b|^2
 template <int L, int SUB>
@@ -525,6 +615,12 @@ struct SolverWork {
   Sell3<float> s32;
   DevBuf<int32_t> flag;
   DevBuf<double> jscale;  // 1/sqrt|a_ii|
+  // ILU(0) form: the factor is kept across solves, as FEMSolver_%ILU_MATRIX is
+  // (FEMSolverClass.f90:1777-1784), until pf_ilu_reset
+  IluFactor ilu;
+  DevBuf<double> pd, ps, e;
+  hipGraphExec_t gexec_ilu = nullptr;
+  uint64_t gkey_ilu[12] = {};
   bool have_s64 = false, have_s32 = false, have_v32 = false, have_scale = false;
   bool s32_scaled = false, v32_scaled = false;
   int block_ok = -1;  // -1 unknown
@@ -537,6 +633,7 @@ struct SolverWork {
   ~SolverWork() {
     for (int k = 0; k < 2; ++k)
       if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+    if (gexec_ilu) (void)hipGraphExecDestroy(gexec_ilu);
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     if (h_sc) (void)hipHostFree(h_sc);
   }
@@ -815,6 +912,100 @@ Mat<V> make_mat(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const
   return m;
 }
 
+// one chunk of bicgstab_CRS_ILU iterations (the factor in w->ilu)
+int enqueue_iters_ilu(pf_ctx *ctx, SolverWork *w, const Launch &L, const Mat<double> &m, double *x, double *hist,
+                      int count) {
+  PF_CHECK(check_grid(ctx, L));
+  hipStream_t st = ctx->stream;
+  Scal *sc = w->sc.p;
+  const size_t G = (size_t)ctx->num_cus * 8;
+  double *p1 = w->part.p + PART_P1 * G, *p2 = w->part.p + PART_P2 * G, *p3 = w->part.p + PART_P3 * G;
+  const int g1 = (int)L.grid, ge = (int)L.grid_e;
+  IluSweepArgs s1;
+  s1.r = w->r.p;
+  s1.y = w->y.p;
+  s1.pe = w->p.p;
+  s1.beta = &sc->beta;
+  s1.omega = &sc->omega;
+  s1.done = &sc->done;
+  s1.out = w->pd.p;
+  IluSweepArgs s2;
+  s2.r = w->r.p;
+  s2.y = w->y.p;
+  s2.pe = w->e.p;
+  s2.alpha = &sc->alpha;
+  s2.done = &sc->done;
+  s2.out = w->ps.p;
+  for (int it = 0; it < count; ++it) {
+    const int par = it & 1;
+    PF_CHECK(pf_ilu_sweep_enqueue(ctx, w->ilu, 1, s1));
+    if (L.layout == 1)
+      k_spmv_p<double, 1, 64><<<L.grid, BT, 0, st>>>(m, w->pd.p, w->y.p, w->r0.p, sc, p1);
+    else
+      PF_DISPATCH_SUB(L.sub, (k_spmv_p<double, 0, S_><<<L.grid, BT, 0, st>>>(m, w->pd.p, w->y.p, w->r0.p, sc, p1)));
+    k_ilu_alpha<<<1, BT, 0, st>>>(p1, g1, sc);
+    PF_CHECK(pf_ilu_sweep_enqueue(ctx, w->ilu, 2, s2));
+    if (L.layout == 1)
+      k_spmv_ps<1, 64><<<L.grid, BT, 0, st>>>(m, w->ps.p, w->e.p, w->v.p, sc, p2);
+    else
+      PF_DISPATCH_SUB(L.sub, (k_spmv_ps<0, S_><<<L.grid, BT, 0, st>>>(m, w->ps.p, w->e.p, w->v.p, sc, p2)));
+    k_update_ilu<<<L.grid_e, BT, 0, st>>>(m.n, x, w->r.p, w->r0.p, w->pd.p, w->ps.p, w->e.p, w->v.p, sc, p2, g1, p3);
+    k_pupdate<double><<<1, BT, 0, st>>>(0, w->p.p, w->r.p, w->y.p, sc, p3, ge, par, hist, nullptr, nullptr);
+  }
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+// iterations of bicgstab_CRS_ILU to the stop, in hipGraph chunks of 2 (an iteration is
+// 2 * levels + 5 launches, so polls are cheap next to it)
+int run_loop_ilu(pf_ctx *ctx, SolverWork *w, const Launch &L, const Mat<double> &m, double *x,
+                 const pf_solve_opts &o) {
+  Scal *sc = w->sc.p;
+  Scal *hs = w->h_sc;
+  const int chunk = 2;
+  bool use_graph = o.use_graph != 0;
+  const uint64_t key[12] = {(uint64_t)L.grid, (uint64_t)L.grid_e * 131 + (uint64_t)L.sub, (uint64_t)L.layout,
+                            (uint64_t)m.n, (uint64_t)(uintptr_t)m.val, (uint64_t)(uintptr_t)m.sval,
+                            (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)w->pd.p,
+                            (uint64_t)(uintptr_t)w->ilu.lval.p, (uint64_t)(uintptr_t)w->ilu.grp.p,
+                            (uint64_t)w->ilu.lev.size(), (uint64_t)w->ilu.lnnz};
+  if (use_graph && (w->gexec_ilu == nullptr || std::memcmp(w->gkey_ilu, key, sizeof(key)) != 0)) {
+    if (w->gexec_ilu) {
+      PF_HIP(hipGraphExecDestroy(w->gexec_ilu));
+      w->gexec_ilu = nullptr;
+    }
+    hipGraph_t graph;
+    PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_iters_ilu(ctx, w, L, m, x, nullptr, chunk);
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != PF_OK) return rc;
+    if (ce != hipSuccess) {
+      pf_set_error("hipStreamEndCapture: %s", hipGetErrorString(ce));
+      return PF_ERR_HIP;
+    }
+    hipError_t ie = hipGraphInstantiate(&w->gexec_ilu, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      w->gexec_ilu = nullptr;
+      use_graph = false;
+    } else {
+      std::memcpy(w->gkey_ilu, key, sizeof(key));
+    }
+  }
+  for (;;) {
+    if (use_graph) {
+      PF_HIP(hipGraphLaunch(w->gexec_ilu, ctx->stream));
+    } else {
+      PF_CHECK(enqueue_iters_ilu(ctx, w, L, m, x, nullptr, chunk));
+    }
+    PF_HIP(hipMemcpyAsync(hs, sc, sizeof(Scal), hipMemcpyDeviceToHost, ctx->stream));
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    if (o.debug) std::printf("BiCGSTAB ILU >> itr %d  sqrt(rr) %.6e\n", hs->itr_last, std::sqrt(hs->rr));
+    if (hs->done) break;
+  }
+  return PF_OK;
+}
+
 }  // namespace
 
 int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
@@ -871,6 +1062,19 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     w->have_v32 = true;
     w->v32_scaled = scaled;
   }
+  const bool ilu = o.preconditioning == 2;
+  if (ilu) {
+    if (mixed) {
+      pf_set_error("ILU(0) preconditioning runs in FP64 only (precision 64)");
+      return PF_INVALID;
+    }
+    if (w->block_ok < 0) PF_CHECK(check_blocks(ctx, w, n, row_ptr, col_idx));
+    if (!w->ilu.ready || w->ilu.n != n || w->ilu.nnz != nnz)
+      PF_CHECK(pf_ilu_factor(ctx, n, row_ptr, col_idx, val, nnz, w->block_ok == 1 ? 3 : 1, w->ilu));
+    PF_CHECK(w->pd.ensure(n));
+    PF_CHECK(w->ps.ensure(n));
+    PF_CHECK(w->e.ensure(n));
+  }
   PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   PF_HIP(hipEventSynchronize(ctx->ev1));
   float setup_ms = 0.f;
@@ -887,7 +1091,21 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   double *pres = w->part.p + PART_PI * G;
 
   PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-  if (!mixed) {
+  if (ilu) {
+    Mat<double> m64 = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
+    Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), fused_grid(ctx, n), sub, layout};
+    const int guard = o.stop_mode == 1 ? 1 : 0;
+    PF_CHECK(prepare_scalars(ctx, w, 0, o, o.itrmax, o.relative_er, o.stop_mode == 1 ? 1 : 0, 0, guard));
+    PF_CHECK(enqueue_init<double>(ctx, L64, m64, b, x, w->r.p, w->r0.p, w->p.p, w->sc.p, w->part.p, false));
+    k_ilu_start<<<L64.grid_e, BT, 0, ctx->stream>>>(w->sc.p, w->y.p, n);
+    PF_HIP(hipGetLastError());
+    PF_CHECK(run_loop_ilu(ctx, w, L64, m64, x, o));
+    const Scal &h = w->h_sc[0];
+    res.status = h.status;
+    res.iterations = h.itr_last + (h.status == PF_BREAKDOWN_VV ? 1 : 0);
+    res.rr = h.rr;
+    res.rr1 = h.rr1;
+  } else if (!mixed) {
     Mat<double> m64 = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
     Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), fused_grid(ctx, n), sub, layout};
     const int guard = o.stop_mode == 1 ? 1 : 0;  // mode 0 keeps the reference's unguarded arithmetic
@@ -975,6 +1193,21 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     res.true_relres = bn > 0 ? std::sqrt(w->h_sc[0].rnorm2) / bn : std::sqrt(w->h_sc[0].rnorm2);
   }
   return PF_OK;
+}
+
+int pf_work_ilu(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                const double *val, int64_t nnz, bool matrix_changed, IluFactor **out) {
+  note_matrix(w, n, val, matrix_changed);
+  PF_CHECK(w->flag.ensure(1));
+  if (w->block_ok < 0) PF_CHECK(check_blocks(ctx, w, n, row_ptr, col_idx));
+  if (!w->ilu.ready || w->ilu.n != n || w->ilu.nnz != nnz)
+    PF_CHECK(pf_ilu_factor(ctx, n, row_ptr, col_idx, val, nnz, w->block_ok == 1 ? 3 : 1, w->ilu));
+  *out = &w->ilu;
+  return PF_OK;
+}
+
+void pf_work_ilu_reset(SolverWork *w) {
+  if (w) w->ilu.ready = false;
 }
 
 int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,

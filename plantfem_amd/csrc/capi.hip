@@ -113,7 +113,7 @@ extern "C" void pf_solve_opts_default(pf_solve_opts *o) {
   o->relative_er = (double)1.0e-10f;
   o->inner_tol = 0.1;
   o->target_relres = 1e-8;
-  o->max_outer = 0;
+  o->preconditioning = 0;
   o->spmv_kind = 0;
   o->check_every = 32;
   o->use_graph = 1;
@@ -363,7 +363,16 @@ extern "C" int pf_bicgstab_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, c
   PF_CHECK(upload(ctx, dv, val, (size_t)nnz));
   PF_CHECK(upload(ctx, db, rhs, (size_t)n));
   PF_CHECK(upload(ctx, dx, x, (size_t)n));
+  if (o.preconditioning < 0 || o.preconditioning > 3) {
+    pf_set_error("preconditioning must be 0 (none), 1 (PointJacobi), 2 (ILU) or 3 (PointJacobi, no solve)");
+    return PF_INVALID;
+  }
+  if (o.preconditioning == 1 || o.preconditioning == 3) PF_CHECK(pf_jacobi_rows(ctx, n, rp.p, ci.p, dv.p, db.p));
   SolveResult res;
+  if (o.preconditioning == 3) {
+    fill_stats(stats, res);
+    return PF_OK;
+  }
   PF_CHECK(pf_bicgstab_device(ctx, ctx->work, n, rp.p, ci.p, dv.p, nnz, db.p, dx.p, o, true, res));
   fill_stats(stats, res);
   PF_CHECK(download(ctx, x, dx.p, (size_t)n));
@@ -371,6 +380,89 @@ extern "C" int pf_bicgstab_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, c
   pf_solver_work_destroy(ctx->work);
   ctx->work = nullptr;
   return res.status;
+}
+
+extern "C" int pf_jacobi_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
+                             double *rhs) {
+  PF_CHECK(check_ctx(ctx));
+  if (n < 0 || !row_ptr || !val) return PF_INVALID;
+  const int64_t nnz = row_ptr[n] - 1;
+  PF_CHECK(validate_crs(n, nnz, row_ptr, col_idx));
+  DeviceGuard guard(ctx);
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> ci;
+  DevBuf<double> dv, db;
+  PF_CHECK(upload(ctx, rp, row_ptr, (size_t)n + 1));
+  PF_CHECK(upload(ctx, ci, col_idx, (size_t)nnz));
+  PF_CHECK(upload(ctx, dv, val, (size_t)nnz));
+  if (rhs) PF_CHECK(upload(ctx, db, rhs, (size_t)n));
+  PF_CHECK(pf_jacobi_rows(ctx, n, rp.p, ci.p, dv.p, rhs ? db.p : nullptr));
+  PF_CHECK(download(ctx, val, dv.p, (size_t)nnz));
+  if (rhs) PF_CHECK(download(ctx, rhs, db.p, (size_t)n));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+namespace {
+// node-block structure of a CRS on the device (ILU groups of three rows when present)
+int ilu_group(pf_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, int *G) {
+  DevBuf<int32_t> flag;
+  PF_CHECK(flag.ensure(1));
+  int ok = 0;
+  PF_CHECK(pf_block_structure_check(ctx, n, rp, ci, n / 3, flag.p, &ok));
+  *G = ok ? 3 : 1;
+  return PF_OK;
+}
+}  // namespace
+
+extern "C" int pf_ilu0_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                           double *fval) {
+  PF_CHECK(check_ctx(ctx));
+  if (n < 0 || !row_ptr || !val || !fval) return PF_INVALID;
+  const int64_t nnz = row_ptr[n] - 1;
+  PF_CHECK(validate_crs(n, nnz, row_ptr, col_idx));
+  DeviceGuard guard(ctx);
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> ci;
+  DevBuf<double> dv;
+  PF_CHECK(upload(ctx, rp, row_ptr, (size_t)n + 1));
+  PF_CHECK(upload(ctx, ci, col_idx, (size_t)nnz));
+  PF_CHECK(upload(ctx, dv, val, (size_t)nnz));
+  int G = 1;
+  PF_CHECK(ilu_group(ctx, n, rp.p, ci.p, &G));
+  IluFactor F;
+  PF_CHECK(pf_ilu_factor(ctx, n, rp.p, ci.p, dv.p, nnz, G, F));
+  PF_CHECK(download(ctx, fval, F.f.p, (size_t)nnz));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+extern "C" int pf_ilu_apply_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                                const double *fval, const double *v, double *y) {
+  PF_CHECK(check_ctx(ctx));
+  if (n < 0 || !row_ptr || !fval || !v || !y) return PF_INVALID;
+  const int64_t nnz = row_ptr[n] - 1;
+  PF_CHECK(validate_crs(n, nnz, row_ptr, col_idx));
+  DeviceGuard guard(ctx);
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> ci;
+  DevBuf<double> df, dvv, dy;
+  PF_CHECK(upload(ctx, rp, row_ptr, (size_t)n + 1));
+  PF_CHECK(upload(ctx, ci, col_idx, (size_t)nnz));
+  PF_CHECK(upload(ctx, df, fval, (size_t)nnz));
+  PF_CHECK(upload(ctx, dvv, v, (size_t)n));
+  PF_CHECK(dy.ensure((size_t)n + 1));
+  int G = 1;
+  PF_CHECK(ilu_group(ctx, n, rp.p, ci.p, &G));
+  IluFactor F;
+  PF_CHECK(pf_ilu_factor(ctx, n, rp.p, ci.p, df.p, nnz, G, F, false));
+  IluSweepArgs a;
+  a.v = dvv.p;
+  a.out = dy.p;
+  PF_CHECK(pf_ilu_sweep_enqueue(ctx, F, 0, a));
+  PF_CHECK(download(ctx, y, dy.p, (size_t)n));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
 }
 
 extern "C" int pf_dev_spmv_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
@@ -746,21 +838,79 @@ extern "C" int pf_system_solve(pf_system *s, const pf_solve_opts *opts, const do
   pf_solve_opts o;
   if (opts) o = *opts; else pf_solve_opts_default(&o);
   PF_CHECK(ensure_work(ctx, &s->work));
+  if (o.preconditioning < 0 || o.preconditioning > 3) {
+    pf_set_error("preconditioning must be 0 (none), 1 (PointJacobi), 2 (ILU) or 3 (PointJacobi, no solve)");
+    return PF_INVALID;
+  }
   PF_CHECK(apply_bc(s));
+  if (o.preconditioning == 1 || o.preconditioning == 3) {
+    // JacobiPreconditionerCRS on CRS_val / CRS_RHS after the elimination
+    // (FEMSolverClass.f90:1769-1775); the elimination stays valid (the fixed rows hold
+    // a unit diagonal), re-running it is a no-op
+    PF_CHECK(pf_jacobi_rows(ctx, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->rhs.p));
+    mark_changed(s);
+    ++s->rhs_version;
+  }
   if (x0) {
     PF_CHECK(upload(ctx, s->x, x0, (size_t)s->n));
   } else {
     PF_HIP(hipMemsetAsync(s->x.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
   }
   SolveResult res;
-  PF_CHECK(pf_bicgstab_device(ctx, s->work, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->rhs.p, s->x.p,
-                              o, s->matrix_changed, res));
-  s->matrix_changed = false;
+  if (o.preconditioning == 3) {
+    // the reference's PointJacobi branch calls no solver: x = x0 (or zeros), then the fixed values
+    res.status = PF_OK;
+  } else {
+    PF_CHECK(pf_bicgstab_device(ctx, s->work, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->rhs.p,
+                                s->x.p, o, s->matrix_changed, res));
+    s->matrix_changed = false;
+  }
   if (s->has_fix) PF_CHECK(pf_bc_set_fixed_x(ctx, s->n, s->fixed.p, s->ubar.p, s->x.p));
   fill_stats(stats, res);
   if (x_out) PF_CHECK(download(ctx, x_out, s->x.p, (size_t)s->n));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   return res.status;
+}
+
+extern "C" int pf_system_ilu_factor(pf_system *s, double *fval, int32_t *levels) {
+  PF_CHECK(sys_check(s, true));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_CHECK(ensure_work(ctx, &s->work));
+  PF_CHECK(apply_bc(s));
+  IluFactor *F = nullptr;
+  PF_CHECK(pf_work_ilu(ctx, s->work, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->matrix_changed, &F));
+  s->matrix_changed = false;
+  if (levels) *levels = (int32_t)F->lev.size() - 1;
+  if (fval) PF_CHECK(download(ctx, fval, F->f.p, (size_t)s->nnz));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+extern "C" int pf_system_ilu_apply(pf_system *s, const double *v, double *y) {
+  PF_CHECK(sys_check(s, true));
+  if (!v || !y) return PF_INVALID;
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_CHECK(pf_system_ilu_factor(s, nullptr, nullptr));
+  IluFactor *F = nullptr;
+  PF_CHECK(pf_work_ilu(ctx, s->work, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, false, &F));
+  PF_CHECK(s->sx.ensure((size_t)s->n + 1));
+  PF_CHECK(s->sy.ensure((size_t)s->n + 1));
+  PF_CHECK(upload(ctx, s->sx, v, (size_t)s->n));
+  IluSweepArgs a;
+  a.v = s->sx.p;
+  a.out = s->sy.p;
+  PF_CHECK(pf_ilu_sweep_enqueue(ctx, *F, 0, a));
+  PF_CHECK(download(ctx, y, s->sy.p, (size_t)s->n));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+extern "C" int pf_system_ilu_reset(pf_system *s) {
+  PF_CHECK(sys_check(s, false));
+  pf_work_ilu_reset(s->work);
+  return PF_OK;
 }
 
 extern "C" int pf_system_get_x(pf_system *s, double *x) {

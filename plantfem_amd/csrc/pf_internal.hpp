@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/pfgpu.h"
 
@@ -180,6 +181,42 @@ int pf_crs_to_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
 template <typename V>
 int pf_spmv_sell3_launch(pf_ctx *ctx, const Sell3<V> &s, const V *x, V *y);
 
+// ilu.hip: ILU(0) factor of ILUCRS (SparseClass.f90:1629-1700) and its application
+// ILU_matvecCRS (SparseClass.f90:1795-1845), level-scheduled
+struct IluFactor {
+  int64_t n = 0, nnz = 0, ngroups = 0, lnnz = 0;
+  int G = 1;                  // rows per group: 3 = one node's rows (DOF=3 node blocks), else 1
+  std::vector<int64_t> lev;   // host: level l holds level-order positions lev[l] .. lev[l+1]-1
+  DevBuf<double> f;           // factor values on the CRS pattern (the reference's Pmat%val)
+  DevBuf<double> dv;          // diagonal of the factor per row (0 when not stored)
+  DevBuf<int32_t> grp;        // group ids in level order
+  DevBuf<int64_t> lptr;       // per row in level order: its entries in columns of earlier groups
+  DevBuf<int32_t> lcol;       //   0-based columns
+  DevBuf<double> lval;        //   factor values
+  DevBuf<double> wv;          // G = 3: within-node lower entries (1,0), (2,0), (2,1) per position
+  DevBuf<double> dd;          // per row in level order: the divisor of the apply (0 -> 1)
+  DevBuf<double> t;           // n: L^-1 src before the division
+  DevBuf<int32_t> err;
+  bool ready = false;
+};
+// inputs of one sweep y = D^-1 L^-1 src (mode 0: src = v; 1: src = p = r + beta (p - omega y),
+// stored into pe; 2: src = e = r - alpha y, stored into pe); scalars are device pointers
+struct IluSweepArgs {
+  const double *v = nullptr, *r = nullptr, *y = nullptr;
+  double *pe = nullptr;
+  const double *alpha = nullptr, *beta = nullptr, *omega = nullptr;
+  const int32_t *done = nullptr;  // nonzero: the sweep returns at entry
+  double *out = nullptr;
+};
+// factor = true: F.f = ILU(0) of val; false: val already is a factor (schedule and sweep
+// structures only).  G = 3 only for a CRS with the DOF=3 node-block structure.
+int pf_ilu_factor(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                  int64_t nnz, int G, IluFactor &F, bool factor = true);
+int pf_ilu_sweep_enqueue(pf_ctx *ctx, IluFactor &F, int mode, const IluSweepArgs &a);
+// JacobiPreconditionerCRS in place (rhs may be NULL)
+int pf_jacobi_rows(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
+                   double *rhs);
+
 // bicgstab.hip
 struct SolveResult {
   int status = 0, iterations = 0, outer = 0, kind = 0;
@@ -200,6 +237,11 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
 int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                  const double *val, int64_t nnz, bool matrix_changed, int kind, const double *x, double *y,
                  int repeat, double *ms);
+
+// the ILU(0) factor kept in w (computed from val when none is kept or the sizes changed)
+int pf_work_ilu(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                const double *val, int64_t nnz, bool matrix_changed, IluFactor **out);
+void pf_work_ilu_reset(SolverWork *w);
 
 // batch.hip: many independent meshes of one block-diagonal system, one persistent kernel
 struct BatchWork;

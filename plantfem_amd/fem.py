@@ -440,9 +440,24 @@ class FEMSolver:
 
     # (7-2)
     def solve(self, x0=None, precision: int = 64, raise_on_breakdown: bool = True, download: bool = True,
-              **opts):
-        """solve() (FEMSolverClass.f90:1697-1825).  download=False keeps x on the device (returns None)."""
+              preconditioning: str | None = None, jacobi_solve: bool = True, **opts):
+        """solve([preconditioning], [x0]) (FEMSolverClass.f90:1697-1825).  download=False keeps x
+        on the device (returns None).
+
+        preconditioning (FEMSolverClass.f90:1768-1790): None -> bicgstab_CRS_2; "ILU" or
+        "incompleteLU" -> bicgstab_CRS_ILU with the ILU(0) factor kept from the first such solve
+        (iluReset() drops it); "PointJacobi" -> JacobiPreconditionerCRS on the eliminated system,
+        then bicgstab_CRS_2 on the scaled system.  The reference's PointJacobi branch calls no
+        solver after the scaling (its x is x0 with the fixed values); jacobi_solve=False does
+        exactly that."""
         self._flush()
+        if preconditioning is not None:
+            if preconditioning == "PointJacobi":
+                opts["preconditioning"] = 1 if jacobi_solve else 3
+            elif preconditioning in ("ILU", "incompleteLU"):
+                opts["preconditioning"] = 2
+            else:  # the reference prints "invalid preconditioning" and stops
+                raise ValueError(f"FEMSolver :: invalid preconditioning {preconditioning!r}")
         o = L.default_opts(itrmax=int(self.itrmax), er0=float(self.er0), relative_er=float(self.relative_er),
                            debug=int(bool(self.debug)), precision=int(precision), **opts)
         st = L.Stats()
@@ -455,6 +470,26 @@ class FEMSolver:
         L.check(code, "solve (Bicgstab devide by zero)" if code in (1, 2) else "solve", ok=ok)
         return x
 
+
+    def iluFactor(self):
+        """(factor values on the CRS pattern, number of schedule levels) of the ILU(0) factor the
+        solver keeps (computed now from the eliminated matrix when none is kept)."""
+        self._flush()
+        f = np.empty(self.nnz)
+        lv = C.c_int32(0)
+        L.check(L.load().pf_system_ilu_factor(self.h, L.ptr(f, C.c_double), C.byref(lv)), "iluFactor")
+        return f, int(lv.value)
+
+    def iluApply(self, v) -> np.ndarray:
+        """ILU_matvec of the kept factor: D^-1 L^-1 v."""
+        self._flush()
+        y = np.empty(self.n)
+        L.check(L.load().pf_system_ilu_apply(self.h, L.ptr(_f64(v), C.c_double), L.ptr(y, C.c_double)), "iluApply")
+        return y
+
+    def iluReset(self):
+        """Drop the kept ILU(0) factor; the next ILU solve factors the current matrix."""
+        L.check(L.load().pf_system_ilu_reset(self.h), "iluReset")
 
     def saveMatrix(self, name: str, CRS_as_dense: bool = False, zero_or_nonzero: bool = False):
         """saveMatrixFEMSolver (FEMSolverClass.f90:1225-1307): name_data.txt (one value per line),
@@ -559,12 +594,49 @@ def bicgstab_crs(row_ptr, col_idx, val, rhs, x0=None, ctx: L.Context | None = No
     va = _f64(val)
     b = _f64(rhs)
     x = np.zeros(b.size) if x0 is None else np.array(x0, dtype=np.float64)
+    pc = opts.pop("preconditioning", None)
+    if isinstance(pc, str):
+        pc = {"PointJacobi": 1, "ILU": 2, "incompleteLU": 2}[pc]
+    if pc is not None:
+        opts["preconditioning"] = int(pc)
     o = L.default_opts(**opts)
     st = L.Stats()
     code = L.load().pf_bicgstab_crs(ctx.h, b.size, L.ptr(rp, C.c_int64), L.ptr(ci, C.c_int32), L.ptr(va, C.c_double),
                                     L.ptr(b, C.c_double), L.ptr(x, C.c_double), C.byref(o), C.byref(st))
     L.check(code, "bicgstab_crs", ok=(L.PF_OK, L.PF_NOT_CONVERGED, L.PF_BREAKDOWN_VV, L.PF_BREAKDOWN_C2C3))
     return x, st.as_dict()
+
+
+def jacobi_preconditioner(row_ptr, col_idx, val, rhs=None, ctx: L.Context | None = None):
+    """JacobiPreconditionerCRS (FEMSolverClass.f90:2508-2529) on copies: returns (val, rhs)."""
+    ctx = ctx or L.default_context()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    va = np.array(val, dtype=np.float64)
+    b = None if rhs is None else np.array(rhs, dtype=np.float64)
+    L.check(L.load().pf_jacobi_crs(ctx.h, rp.size - 1, L.ptr(rp, C.c_int64), L.ptr(_i32(col_idx), C.c_int32),
+                                   L.ptr(va, C.c_double), L.ptr(b, C.c_double)), "jacobi_preconditioner")
+    return va, b
+
+
+def ilu0(row_ptr, col_idx, val, ctx: L.Context | None = None) -> np.ndarray:
+    """ILUCRS(fill_in_order=0) (SparseClass.f90:1629-1700): the factor values on the same pattern."""
+    ctx = ctx or L.default_context()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    f = np.empty(int(rp[-1] - 1))
+    L.check(L.load().pf_ilu0_crs(ctx.h, rp.size - 1, L.ptr(rp, C.c_int64), L.ptr(_i32(col_idx), C.c_int32),
+                                 L.ptr(_f64(val), C.c_double), L.ptr(f, C.c_double)), "ilu0")
+    return f
+
+
+def ilu_apply(row_ptr, col_idx, fval, v, ctx: L.Context | None = None) -> np.ndarray:
+    """ILU_matvecCRS (SparseClass.f90:1795-1845) of the factor fval: D^-1 L^-1 v."""
+    ctx = ctx or L.default_context()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    y = np.empty(rp.size - 1)
+    L.check(L.load().pf_ilu_apply_crs(ctx.h, rp.size - 1, L.ptr(rp, C.c_int64), L.ptr(_i32(col_idx), C.c_int32),
+                                      L.ptr(_f64(fval), C.c_double), L.ptr(_f64(v), C.c_double),
+                                      L.ptr(y, C.c_double)), "ilu_apply")
+    return y
 
 
 def spmv_crs(row_ptr, col_idx, val, x, ctx: L.Context | None = None) -> np.ndarray:

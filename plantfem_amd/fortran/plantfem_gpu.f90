@@ -27,7 +27,7 @@ module plantfem_gpu
    type, bind(C) :: pf_solve_opts
       integer(c_int32_t) :: itrmax, precision
       real(c_double) :: er0, relative_er, inner_tol, target_relres
-      integer(c_int32_t) :: max_outer, spmv_kind, check_every, use_graph, debug, stop_mode, &
+      integer(c_int32_t) :: preconditioning, spmv_kind, check_every, use_graph, debug, stop_mode, &
                             kernel_timing, jacobi, inner_max, use_x0
    end type
 
@@ -645,9 +645,14 @@ contains
       call check(code, "fix")
    end subroutine
 
-   ! solve() (FEMSolverClass.f90:1697-1825): elimination, bicgstab_CRS_2, x_fixed = ubar
-   function solveFEMSolver(this, x0, precision) result(x)
+   ! solve(algorithm, preconditioning, x0) (FEMSolverClass.f90:1697-1825): elimination, then
+   ! bicgstab_CRS_2, or with preconditioning="PointJacobi" the Jacobi-scaled system (the
+   ! reference calls no solver after the scaling; "PointJacobi-noSolve" does the same), or with
+   ! "ILU"/"incompleteLU" bicgstab_CRS_ILU; x_fixed = ubar.  algorithm is accepted for the
+   ! interface; every algorithm runs BiCGSTAB here.
+   function solveFEMSolver(this, algorithm, preconditioning, x0, precision) result(x)
       class(FEMSolver_), intent(inout) :: this
+      character(*), optional, intent(in) :: algorithm, preconditioning
       real(real64), optional, target, intent(in) :: x0(:)
       integer(int32), optional, intent(in) :: precision
       real(real64), allocatable :: x(:)
@@ -662,6 +667,20 @@ contains
       o%relative_er = this%relative_er
       o%debug = merge(1, 0, this%debug)
       if (present(precision)) o%precision = precision
+      if (present(algorithm)) continue
+      if (present(preconditioning)) then
+         select case (preconditioning)
+         case ("PointJacobi")
+            o%preconditioning = 1
+         case ("ILU", "incompleteLU")
+            o%preconditioning = 2
+         case ("PointJacobi-noSolve")
+            o%preconditioning = 3
+         case default
+            print *, "[Warning!] :: FEMSolver :: invalid preconditioning"
+            stop
+         end select
+      end if
       allocate (xl(this%n))
       px0 = c_null_ptr
       if (present(x0)) px0 = c_loc(x0)
