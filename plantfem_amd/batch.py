@@ -1,12 +1,15 @@
 """Batched static solves of many independent organ meshes on one GPU.
 
-The reference solves each organ with its own FEMSolver_ (Soybean%getDisplacement,
-SoybeanClass.f90:8417-8533: per organ setDomain / setCRS / element matrices / fix / solve).
-Here the meshes are concatenated into one block-diagonal system: node ids of mesh m are
-shifted by its first node, element lists and material arrays are concatenated, so graph
-build, element assembly, tractions and fixes run once for the whole batch, and every mesh's
-CRS block is the reference's CRS of that mesh alone.  FEMSolver.solveBatch then solves all
-meshes in one persistent-kernel launch.
+Every organ mesh is solved on its own, as one single-domain FEMSolver_ per organ would
+(setDomain / setCRS(DOF=3) / StiffnessMatrix / traction / fix / solve, FEMSolverClass.f90:
+329-694, :1697-1825).  (The reference's whole-plant Soybean%getDisplacement,
+SoybeanClass.f90:8417-8533, instead couples the organs into one multi-domain system with
+overset penalty terms, setEbOM; that coupling is not part of this form.)  Here the meshes are
+concatenated into one block-diagonal system: node ids of mesh m are shifted by its first
+node, element lists and material arrays are concatenated, so graph build, element assembly,
+tractions and fixes run once for the whole batch, and every mesh's CRS block is the
+single-domain CRS of that mesh alone.  FEMSolver.solveBatch then solves all meshes in one
+persistent-kernel launch.
 
 organ_batch(count, seed) makes the synthetic plant-organ workload: leaf slabs of n x m x 2
 hex8 elements (n, m in [20, 120]) clamped along one edge with a pressure on the upper face,

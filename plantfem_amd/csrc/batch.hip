@@ -4,8 +4,10 @@
 // element of mesh m references nodes of mesh m only), assembled by the ordinary
 // ZeroMatrix / StiffnessMatrix / traction / fix kernels, so the CRS block of each mesh is
 // exactly the one the reference builds for that mesh alone (column ids shifted by the
-// mesh's first DOF).  The reference solves such organ meshes one after another, one
-// FEMSolver_ per organ (Soybean%getDisplacement, SoybeanClass.f90:8417-8533).
+// mesh's first DOF).  It stands for solving such meshes one after another with one
+// single-domain FEMSolver_ each; the reference's whole-plant Soybean%getDisplacement
+// (SoybeanClass.f90:8417-8533) couples organs through overset penalties instead, which
+// this form does not do.
 //
 // Here every mesh runs bicgstab_CRS_2 (FEMSolverClass.f90:2199-2298) inside ONE launch of a
 // persistent kernel: a workgroup takes the next mesh from a queue (largest cost first),

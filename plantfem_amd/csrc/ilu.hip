@@ -44,6 +44,21 @@ __device__ inline int lds_find(const int32_t *cols, int len, int32_t c) {
   return best == MAXR ? -1 : best;
 }
 
+// column -> position map of one row in LDS (open addressing, HT slots, load <= 1/2)
+constexpr int HT = 2 * MAXR;
+__device__ inline uint32_t hslot(int32_t c) { return ((uint32_t)c * 0x9E3779B1u) >> (32 - 10); }
+static_assert(HT == 1024, "hslot takes 10 bits");
+
+__device__ inline int hfind(const int32_t *hkey, const int16_t *hpos, int32_t c) {
+  uint32_t h = hslot(c);
+  for (;;) {
+    const int32_t k = hkey[h];
+    if (k == c) return hpos[h];
+    if (k == -1) return -1;
+    h = (h + 1) & (HT - 1);
+  }
+}
+
 // err bits: 1 row longer than MAXR, 2 a row stores a column twice
 template <int G>
 __global__ void __launch_bounds__(FT) k_ilu_level(const int64_t *__restrict__ row_ptr,
@@ -56,6 +71,8 @@ __global__ void __launch_bounds__(FT) k_ilu_level(const int64_t *__restrict__ ro
   __shared__ double vk[MAXR];
   __shared__ int32_t lenv[G];
   __shared__ double dloc[G];
+  __shared__ int32_t hkey[HT];
+  __shared__ int16_t hpos[HT];
   const int lane = threadIdx.x;
   const int64_t g = grp[p0 + blockIdx.x];
   for (int a = 0; a < G; ++a) {
@@ -79,6 +96,15 @@ __global__ void __launch_bounds__(FT) k_ilu_level(const int64_t *__restrict__ ro
       if (lane == 0) atomicOr(err, 2);
       return;
     }
+    for (int h = lane; h < HT; h += FT) hkey[h] = -1;
+    __syncthreads();
+    for (int q = lane; q < len; q += FT) {  // distinct columns: every insert claims a new slot
+      const int32_t c = ci[a][q];
+      uint32_t h = hslot(c);
+      while (atomicCAS(&hkey[h], -1, c) != -1) h = (h + 1) & (HT - 1);
+      hpos[h] = (int16_t)q;
+    }
+    __syncthreads();
     for (int kk = 0; kk < len; ++kk) {
       const int32_t k = ci[a][kk];  // 1-based; the same value in every lane
       if ((int64_t)k >= i + 1) continue;
@@ -111,16 +137,28 @@ __global__ void __launch_bounds__(FT) k_ilu_level(const int64_t *__restrict__ ro
       }
       __syncthreads();
       if (lane == 0) vi[a][kk] = aik;  // update(i,k,A_ik); column k itself is not touched below
-      for (int jj = lane; jj < len; jj += FT) {
-        const int32_t j = ci[a][jj];
-        if (j < k + 1) continue;
-        double gv = 0.0;  // get(k,j)
-        for (int q = 0; q < lenk; ++q)
-          if (cp[q] == j) {
-            gv = vp[q];
-            break;
-          }
-        vi[a][jj] = __dadd_rn(vi[a][jj], -__dmul_rn(aik, gv));
+      if (isfinite(aik)) {
+        // add(i,j, -A_ik*get(k,j)) for the stored j > k of row i: a column row k does not
+        // store adds -(A_ik*0) = -0, which leaves every value as it is, so only the entries of
+        // row k are visited, each finding its column in row i through the hash
+        for (int m = lane; m < lenk; m += FT) {
+          const int32_t j = cp[m];
+          if (j < k + 1) continue;
+          const int q = hfind(hkey, hpos, j);
+          if (q >= 0) vi[a][q] = __dadd_rn(vi[a][q], -__dmul_rn(aik, vp[m]));
+        }
+      } else {  // -(inf*0) = NaN reaches every stored j > k, as in the reference
+        for (int jj = lane; jj < len; jj += FT) {
+          const int32_t j = ci[a][jj];
+          if (j < k + 1) continue;
+          double gv = 0.0;  // get(k,j)
+          for (int q = 0; q < lenk; ++q)
+            if (cp[q] == j) {
+              gv = vp[q];
+              break;
+            }
+          vi[a][jj] = __dadd_rn(vi[a][jj], -__dmul_rn(aik, gv));
+        }
       }
       __syncthreads();
     }
@@ -220,10 +258,11 @@ __global__ void __launch_bounds__(ST) k_ilu_sweep(int64_t p0, int64_t p1, const 
                                                   const double *__restrict__ lval, const double *__restrict__ wv,
                                                   const double *__restrict__ dd, IluSweepArgs sa,
                                                   double *__restrict__ t) {
-  if (sa.done && *sa.done) return;
   const int64_t pos = p0 + (blockIdx.x * (int64_t)ST + threadIdx.x) / SSUB;
   const int sub = threadIdx.x % SSUB;
   if (pos >= p1) return;  // whole subgroups leave together
+  // the stop flag is read with the other loads and tested before anything is stored
+  const int32_t done = sa.done ? *sa.done : 0;
   const int64_t g = grp[pos];
   double s[G];
 #pragma unroll
@@ -233,7 +272,7 @@ __global__ void __launch_bounds__(ST) k_ilu_sweep(int64_t p0, int64_t p1, const 
     for (int64_t q = lptr[pos * G + a] + sub; q < q1; q += SSUB) acc += lval[q] * t[lcol[q]];
     s[a] = pfdev::subgroup_sum<double, SSUB>(acc);
   }
-  if (sub != 0) return;
+  if (sub != 0 || done) return;
   double tv[G];
 #pragma unroll
   for (int a = 0; a < G; ++a) {
