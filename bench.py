@@ -20,6 +20,8 @@ Two further workloads print their own line (not the default):
                                          meshes/s, DOF-iterations/s, per-mesh latency
     python bench.py --workload adjoint   60^3 inverse problem, 64 load cases, 2000 observed surface
                                          nodes: forward + adjoint gradient time, gradient checks
+    python bench.py --workload precond   60^3 box with row-sorted columns: unpreconditioned,
+                                         PointJacobi and ILU(0) BiCGSTAB (solve(preconditioning=))
 """
 from __future__ import annotations
 
@@ -65,7 +67,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-graph", action="store_true", help="direct kernel launches only (for profilers)")
-    ap.add_argument("--workload", choices=("box", "batch", "adjoint"), default="box")
+    ap.add_argument("--workload", choices=("box", "batch", "adjoint", "precond"), default="box")
+    ap.add_argument("--precond-size", type=int, default=60)
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-itrmax", type=int, default=3000,
                     help="iteration budget per mesh of the batched solve")
@@ -88,6 +91,8 @@ def main(argv=None):
         return bench_batch(a)
     if a.workload == "adjoint":
         return bench_adjoint(a)
+    if a.workload == "precond":
+        return bench_precond(a)
 
     from plantfem_amd import _lib as L
     from plantfem_amd import workloads as W
@@ -137,13 +142,13 @@ def main(argv=None):
         log("profile: 3 x 512 MiB device copies done")
         return
 
-    def step(precision=64, timing=0):
+    def step(precision=64, timing=0, target=1e-8):
         s.setCRS(DOF=3)                       # ZeroMatrix graph build (device)
         s.setStiffness()                      # element stiffness assembly (device)
         s.addTraction(w.traction_box, w.pressure)
         s.fix(1, w.fixed_dofs, FixValue=0.0)
-        s.relative_er = 1e-8
-        s.solve(precision=precision, stop_mode=1, download=False, kernel_timing=timing, target_relres=1e-8,
+        s.relative_er = target
+        s.solve(precision=precision, stop_mode=1, download=False, kernel_timing=timing, target_relres=target,
                 raise_on_breakdown=False, use_graph=0 if a.no_graph else 1)
         return dict(s.last_stats)
 
@@ -172,6 +177,9 @@ def main(argv=None):
     t_graph = time.perf_counter() - t
 
     # ---- mixed precision on the same matrix --------------------------------------------
+    # FP32 BiCGSTAB on D^-1 A, restarted at every FP64 true-residual replacement; the step
+    # is the same as the FP64 one (graph build, assembly, BCs, solve to relres 1e-8)
+    x64 = s.getX()
     mixed_times, st32 = [], None
     for i in range(max(1, a.steps)):
         ctx.synchronize()
@@ -183,9 +191,22 @@ def main(argv=None):
             f"inner {st32['iterations']}, relres {st32['true_relres']:.2e}")
     t_mixed = float(np.median(mixed_times))
     x32 = s.getX()
-    step(64, 0)
-    x64 = s.getX()
-    mixed_err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+    # the same to relres 1e-10, and a tight FP64 reference (relres ~1e-11) both are checked against
+    ctx.synchronize()
+    t = time.perf_counter()
+    st32t = step(32, 0, target=1e-10)
+    ctx.synchronize()
+    t_mixed_tight = time.perf_counter() - t
+    x32t = s.getX()
+    st_ref = step(64, 0, target=1e-12)
+    x_ref = s.getX()
+
+    def rel(u):
+        return float(np.linalg.norm(u - x_ref) / np.linalg.norm(x_ref))
+
+    mixed_err = rel(x32)
+    log(f"mixed: {t_mixed:.3f} s to 1e-8 (err {mixed_err:.2e}), {t_mixed_tight:.3f} s to 1e-10 "
+        f"(err {rel(x32t):.2e}); FP64 err {rel(x64):.2e}; reference relres {st_ref['true_relres']:.2e}")
 
     # ---- SpMV kernels on the eliminated matrix ----------------------------------------
     spmv = {}
@@ -241,11 +262,21 @@ def main(argv=None):
         "iterations": iters, "solve_ms": round(st64["solve_ms"], 2), "true_relres": st64["true_relres"],
         "dof_iterations_per_s": round(n * iters / (st64["solve_ms"] * 1e-3), 1),
         "solve_ms_graph_mode": round(st_graph["solve_ms"], 2), "step_s_graph_mode": round(t_graph, 4),
-        "mixed": {"status": st32["status"], "time_to_solve_s": round(t_mixed, 4), "solve_ms": round(st32["solve_ms"], 2),
+        "mixed": {"scheme": "FP32 BiCGSTAB on D^-1 A (row Jacobi), FP64 true residual and solution, restart at "
+                            "every replacement (inner tolerance 0.05, at most 300 inner iterations)",
+                  "status": st32["status"], "time_to_solve_s": round(t_mixed, 4), "solve_ms": round(st32["solve_ms"], 2),
                   "outer": st32["outer_iterations"], "inner_iterations": st32["iterations"],
-                  "true_relres": st32["true_relres"], "rel_err_vs_fp64": mixed_err,
+                  "true_relres": st32["true_relres"], "rel_err_vs_fp64_reference": mixed_err,
                   "converged": st32["status"] == 0,
-                  "speedup_vs_fp64": round(t_step / t_mixed, 3) if st32["status"] == 0 else None},
+                  "speedup_vs_fp64": round(t_step / t_mixed, 3) if st32["status"] == 0 else None,
+                  "to_relres_1e-10": {"status": st32t["status"], "time_to_solve_s": round(t_mixed_tight, 4),
+                                      "inner_iterations": st32t["iterations"], "outer": st32t["outer_iterations"],
+                                      "true_relres": st32t["true_relres"], "rel_err_vs_fp64_reference": rel(x32t),
+                                      "speedup_vs_fp64": round(t_step / t_mixed_tight, 3)
+                                      if st32t["status"] == 0 else None},
+                  "fp64_rel_err_vs_reference": rel(x64),
+                  "reference": f"FP64 bicgstab_CRS_2 on the same system to true relres "
+                               f"{st_ref['true_relres']:.2e} ({st_ref['iterations']} iterations)"},
         "spmv": spmv, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -294,6 +325,86 @@ def bench_batch(a):
         log(f"batch precision {prec}: {json.dumps(leg)}")
         out["fp64" if prec == 64 else "mixed"] = leg
     out["value"] = out["fp64"]["meshes_per_s_converged"]
+    print(json.dumps(out), flush=True)
+
+
+def bench_precond(a):
+    """solve(preconditioning=) on a box whose CRS rows hold sorted columns (the order the
+    multi-domain setCRS produces; ILU(0) in the reference's stored-order form needs it)."""
+    from plantfem_amd import _lib as L
+    from plantfem_amd import workloads as W
+    from plantfem_amd.fem import CRS, FEMDomain, FEMSolver
+    ctx = L.default_context()
+    w = W.box(a.precond_size, sigma=1.0, seed=a.seed, nu_range=(0.25, 0.35))
+    t = time.perf_counter()
+    s0 = FEMSolver(ctx)
+    s0.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s0.setCRS(DOF=3)
+    s0.setStiffness(w.E, w.nu)
+    s0.addTraction(w.traction_box, w.pressure)
+    crs, rhs = s0.getCRS(), s0.getRHS()
+    s0.close()
+    rows = np.repeat(np.arange(crs.n, dtype=np.int64), np.diff(crs.row_ptr))
+    o = np.lexsort((crs.col_idx, rows))
+    sorted_crs = CRS(crs.row_ptr, np.ascontiguousarray(crs.col_idx[o]), np.ascontiguousarray(crs.val[o]))
+    del rows, o
+    log(f"precond: {a.precond_size}^3 box, n={crs.n} nnz={crs.nnz}, assembled and row-sorted in "
+        f"{time.perf_counter() - t:.1f} s")
+
+    def fresh():
+        s = FEMSolver(ctx)
+        s.setCRS(CRS=sorted_crs)
+        s.setRHS(rhs)
+        s.fix(1, w.fixed_dofs.astype(np.int32), FixValue=0.0)
+        s.relative_er = 1e-8
+        return s
+
+    out = {"metric": "time-to-solve @relres 1e-8 with solve(preconditioning=)", "unit": "s", "n_gpus": 1,
+           "higher_is_better": False, "data": "synthetic", "dtype": "f64",
+           "config": {"workload": f"hex box {a.precond_size}^3, E=1e7*lognormal(1.0), nu~U[0.25,0.35], clamped "
+                                  "bottom, 1e4 Pa top traction; CRS rows sorted by column",
+                      "n_dof": int(crs.n), "nnz": int(crs.nnz), "stop": "||b - A x|| < 1e-8 ||b||"}}
+    legs = {}
+    g = 0 if a.no_graph else 1
+    for name, pc in (("none", None), ("PointJacobi", "PointJacobi"), ("ILU", "ILU")):
+        s = fresh()
+        s.solve(preconditioning=pc, stop_mode=1, raise_on_breakdown=False, use_graph=g)  # warm (ILU: factor kept)
+        f_levels = s.iluFactor()[1] if pc == "ILU" else None
+        setup = s.last_stats["setup_ms"]
+        s.solve(preconditioning=pc, stop_mode=1, raise_on_breakdown=False, use_graph=g)
+        st = s.last_stats
+        leg = {"status": st["status"], "iterations": st["iterations"], "solve_ms": round(st["solve_ms"], 2),
+               "ms_per_iteration": round(st["solve_ms"] / max(1, st["iterations"]), 4),
+               "true_relres": st["true_relres"], "setup_ms_first_solve": round(setup, 2)}
+        if f_levels is not None:
+            leg["levels"] = f_levels
+            leg["sweep_launches_per_iteration"] = 2 * f_levels
+        legs[name] = leg
+        log(f"precond {name}: {json.dumps(leg)}")
+        if pc == "ILU":
+            fval, _ = s.iluFactor()
+            ilu_iters = st["iterations"]
+        s.close()
+    out["legs"] = legs
+    out["value"] = legs["ILU"]["solve_ms"] / 1e3
+    if not a.no_cpu:
+        # oracle bicgstab_CRS_ILU on the same eliminated system with the (bitwise identical)
+        # factor from the device, a bounded number of iterations, extrapolated
+        from oracle import oracle as O
+        s = fresh()
+        s.itrmax = 1
+        s.solve(stop_mode=1, raise_on_breakdown=False)  # elimination on the device
+        e = s.getCRS()
+        b = s.getRHS()
+        s.close()
+        k = 3
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        t = time.perf_counter()
+        O.bicgstab_ilu(e.row_ptr, e.col_idx, e.val, b, fval=fval, itrmax=k, nthreads=threads)
+        dt = (time.perf_counter() - t) / k
+        out["cpu_baseline"] = {"value": round(dt * ilu_iters, 2), "unit": "s", "cores": threads, "kind": "port",
+                               "sample": f"{k} bicgstab_CRS_ILU iterations (oracle: OpenMP row SpMV, sequential "
+                                         f"ILU_matvec sweeps) timed, x {ilu_iters} GPU iterations; factor excluded"}
     print(json.dumps(out), flush=True)
 
 

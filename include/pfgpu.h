@@ -45,7 +45,7 @@ typedef struct {
   double relative_er;    /* stop sqrt(r.r / r1.r1) < relative_er (r1 = residual after iteration 1) */
   double inner_tol;      /* precision 32: replace the FP32 recursive residual by the FP64 true residual
                             once its norm fell below inner_tol times its largest value since the last
-                            replacement (default 0.1) */
+                            replacement (default 0.05) */
   double target_relres;  /* precision 32: stop when ||b-Ax||/||b|| (FP64, at a replacement) < target
                             (default 1e-8) */
   int32_t preconditioning; /* solve(preconditioning=) (FEMSolverClass.f90:1768-1790), FP64 only:
@@ -63,10 +63,13 @@ typedef struct {
   int32_t stop_mode;     /* 0: the reference's tests (default); 1: sqrt(r.r) < relative_er * ||b|| */
   int32_t kernel_timing; /* 1: bracket every kernel of the iteration loop with hipEvents (no hipGraph)
                             and report the average device time of each in pf_stats */
-  int32_t jacobi;        /* precision 32: solve the FP32 inner system symmetrically scaled,
-                            D^-1/2 A D^-1/2 with D = diag(A) (default 1) */
+  int32_t jacobi;        /* precision 32, scaling of the FP32 inner system (D = diag(A)):
+                            2 (default): rows, D^-1 A, and every replacement restarts the inner
+                              BiCGSTAB from the FP64 residual (iterative refinement);
+                            1: symmetric, D^-1/2 A D^-1/2, reliable updates without restarts;
+                            0: none */
   int32_t inner_max;     /* precision 32: also replace the residual after this many iterations without
-                            a replacement (default 256; 0 = never) */
+                            a replacement (default 300; 0 = never) */
   int32_t use_x0;        /* multi-RHS solve: X holds the initial guess on entry (default 0: x0 = 0) */
 } pf_solve_opts;
 

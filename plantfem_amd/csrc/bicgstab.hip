@@ -52,7 +52,8 @@ struct Scal {
   int32_t itr[2];      // completed iterations, slot by parity (written by K4)
   int32_t itr_best[2]; // iteration of rr_best
   int32_t itr_last;    // host-facing copy of the latest count
-  int32_t itrmax, done, status, stop_mode, hist_cap, guard, stall, pad;
+  int32_t itrmax, done, status, stop_mode, hist_cap, guard, stall;
+  int32_t restart;     // reliable updates: also restart the recurrence (r0 = p = r) at a replacement
 };
 
 template <typename V>
@@ -517,22 +518,29 @@ __global__ void k_rel_setz(int64_t n, const double *__restrict__ x, const double
     zhi[i] = x[i] / scale[i];
 }
 
-// r' = S (b - A S z_hi) in FP64, stored as FP32 r (INIT: also r0 = p = r, z_lo = 0)
+// r' = S (b - A C z_hi) in FP64 (S row scale, C column scale), stored as FP32 r (INIT: also
+// r0 = p = r, z_lo = 0; a restarting replacement also sets r0 = p = r)
 // partials: |r'|^2, |b - A x|^2, r0.r', |b|^2
 template <int L, int SUB, bool INIT>
 __global__ void __launch_bounds__(BT) k_rel_residual(Mat<double> m, const double *__restrict__ b,
-                                                     const double *__restrict__ scale, const double *__restrict__ zhi,
+                                                     const double *__restrict__ scale,
+                                                     const double *__restrict__ cscale, const double *__restrict__ zhi,
                                                      float *__restrict__ r, float *__restrict__ r0,
                                                      float *__restrict__ p, float *__restrict__ zlo,
                                                      const Scal *__restrict__ sc, double *__restrict__ part) {
   if (!INIT && !sc->pending) return;
+  const bool restart = !INIT && sc->restart;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   auto op = [&](int64_t row, double ax) {
     const double bu = b[row] - ax;
     const double rp = scale[row] * bu;
     const float rf = (float)rp;
     r[row] = rf;
-    if (INIT) {
+    if (restart) {
+      r0[row] = rf;
+      p[row] = rf;
+      acc[2] += (double)rf * (double)rf;
+    } else if (INIT) {
       r0[row] = rf;
       p[row] = rf;
       zlo[row] = 0.0f;
@@ -544,7 +552,7 @@ __global__ void __launch_bounds__(BT) k_rel_residual(Mat<double> m, const double
     acc[0] += (double)rf * (double)rf;
     acc[1] += bu * bu;
   };
-  for_rows<double, L, SUB>(m, ScaledX{scale, zhi}, op);
+  for_rows<double, L, SUB>(m, ScaledX{cscale, zhi}, op);
   write_partials<4>(acc, part);
 }
 
@@ -614,7 +622,8 @@ struct SolverWork {
   Sell3<double> s64;
   Sell3<float> s32;
   DevBuf<int32_t> flag;
-  DevBuf<double> jscale;  // 1/sqrt|a_ii|
+  DevBuf<double> jscale;  // FP32 system row scale: 1/sqrt|a_ii| (symmetric) or 1/a_ii (row)
+  DevBuf<double> jones;   // column scale of the row-scaled form
   // ILU(0) form: the factor is kept across solves, as FEMSolver_%ILU_MATRIX is
   // (FEMSolverClass.f90:1777-1784), until pf_ilu_reset
   IluFactor ilu;
@@ -622,7 +631,8 @@ struct SolverWork {
   hipGraphExec_t gexec_ilu = nullptr;
   uint64_t gkey_ilu[12] = {};
   bool have_s64 = false, have_s32 = false, have_v32 = false, have_scale = false;
-  bool s32_scaled = false, v32_scaled = false;
+  int scale_mode = 0;            // what jscale holds: 1 symmetric, 2 row
+  int s32_mode = 0, v32_mode = 0;  // scaling of the FP32 copies: 0 none, 1 symmetric, 2 row
   int block_ok = -1;  // -1 unknown
   const void *key_val = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -710,7 +720,7 @@ int choose_sub(double avg_row) {  // ~five nonzeros per lane, 4..64 lanes per CR
 struct Rel {
   Mat<double> m;
   Launch L;
-  const double *b, *scale;
+  const double *b, *scale, *cscale;  // row and column scale of the FP32 system
   double *zhi;
   float *r, *r0, *p, *zlo;
 };
@@ -721,10 +731,11 @@ int enqueue_replacement(pf_ctx *ctx, const Rel &R, Scal *sc, double *part) {
   PF_CHECK(check_grid(ctx, R.L));
   hipStream_t st = ctx->stream;
   if (R.L.layout == 1) {
-    k_rel_residual<1, 64, INIT><<<R.L.grid, BT, 0, st>>>(R.m, R.b, R.scale, R.zhi, R.r, R.r0, R.p, R.zlo, sc, part);
+    k_rel_residual<1, 64, INIT><<<R.L.grid, BT, 0, st>>>(R.m, R.b, R.scale, R.cscale, R.zhi, R.r, R.r0, R.p,
+                                                         R.zlo, sc, part);
   } else {
-    PF_DISPATCH_SUB(R.L.sub, (k_rel_residual<0, S_, INIT><<<R.L.grid, BT, 0, st>>>(R.m, R.b, R.scale, R.zhi, R.r,
-                                                                                    R.r0, R.p, R.zlo, sc, part)));
+    PF_DISPATCH_SUB(R.L.sub, (k_rel_residual<0, S_, INIT><<<R.L.grid, BT, 0, st>>>(
+                                 R.m, R.b, R.scale, R.cscale, R.zhi, R.r, R.r0, R.p, R.zlo, sc, part)));
   }
   k_rel_finish<INIT><<<1, BT, 0, st>>>(part, (int)R.L.grid, sc);
   PF_HIP(hipGetLastError());
@@ -1039,28 +1050,38 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     }
   }
   const bool mixed = o.precision == 32;
-  const bool scaled = mixed && o.jacobi != 0;
-  if (scaled && !w->have_scale) {
+  // FP32 system: 0 A, 1 S A S (S = diag^-1/2, unknown scaled), 2 D^-1 A (rows only, restarted)
+  const int smode = mixed ? (o.jacobi == 2 ? 2 : (o.jacobi != 0 ? 1 : 0)) : 0;
+  if (smode != 0 && (!w->have_scale || w->scale_mode != smode)) {
     PF_CHECK(w->jscale.ensure(n));
-    PF_CHECK(pf_jacobi_scale(ctx, n, row_ptr, col_idx, val, w->jscale.p));
+    if (smode == 1) {
+      PF_CHECK(pf_jacobi_scale(ctx, n, row_ptr, col_idx, val, w->jscale.p));
+    } else {
+      PF_CHECK(pf_jacobi_row_scale(ctx, n, row_ptr, col_idx, val, w->jscale.p));
+      PF_CHECK(w->jones.ensure(n));
+      k_ones<<<fused_grid(ctx, n), BT, 0, ctx->stream>>>(n, w->jones.p);
+      PF_HIP(hipGetLastError());
+    }
     w->have_scale = true;
+    w->scale_mode = smode;
   }
-  const double *scale = scaled ? w->jscale.p : nullptr;
+  const double *scale = smode ? w->jscale.p : nullptr;
+  const double *cscale = smode == 2 ? w->jones.p : scale;
   if (layout == 1) {
     if (!w->have_s64) {  // FP64 products (mixed: the residual replacements)
       PF_CHECK(pf_sell3_build<double>(ctx, n, row_ptr, col_idx, val, w->s64));
       w->have_s64 = true;
     }
-    if (mixed && (!w->have_s32 || w->s32_scaled != scaled)) {
-      PF_CHECK(pf_sell3_build<float>(ctx, n, row_ptr, col_idx, val, w->s32, scale));
+    if (mixed && (!w->have_s32 || w->s32_mode != smode)) {
+      PF_CHECK(pf_sell3_build<float>(ctx, n, row_ptr, col_idx, val, w->s32, scale, nullptr, cscale));
       w->have_s32 = true;
-      w->s32_scaled = scaled;
+      w->s32_mode = smode;
     }
-  } else if (mixed && (!w->have_v32 || w->v32_scaled != scaled)) {
+  } else if (mixed && (!w->have_v32 || w->v32_mode != smode)) {
     PF_CHECK(w->val32.ensure((size_t)nnz + 1));
-    PF_CHECK(pf_crs_to_f32(ctx, n, row_ptr, col_idx, val, scale, w->val32.p));
+    PF_CHECK(pf_crs_to_f32(ctx, n, row_ptr, col_idx, val, scale, w->val32.p, cscale));
     w->have_v32 = true;
-    w->v32_scaled = scaled;
+    w->v32_mode = smode;
   }
   const bool ilu = o.preconditioning == 2;
   if (ilu) {
@@ -1119,7 +1140,8 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     res.rr = h.rr;
     res.rr1 = h.rr1;
   } else {
-    // FP32 BiCGSTAB on S A S (S = Jacobi scaling or identity) with FP64 reliable updates
+    // FP32 BiCGSTAB on S A C (S = C = Jacobi scaling or identity; or S = D^-1, C = I with a
+    // restart at every replacement) with FP64 reliable updates
     PF_CHECK(w->zhi.ensure(n));
     PF_CHECK(w->x32.ensure(n));
     PF_CHECK(w->r32.ensure(n));
@@ -1128,11 +1150,11 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     PF_CHECK(w->y32.ensure(n));
     PF_CHECK(w->v32.ensure(n));
     const unsigned ge = fused_grid(ctx, n);
-    if (!scale) {  // unscaled: S = I
+    if (!scale) {  // unscaled: S = C = I
       PF_CHECK(w->jscale.ensure(n));
       k_ones<<<ge, BT, 0, ctx->stream>>>(n, w->jscale.p);
       PF_HIP(hipGetLastError());
-      scale = w->jscale.p;
+      scale = cscale = w->jscale.p;
       w->have_scale = false;
     }
     Mat<float> m32 = make_mat<float>(n, row_ptr, col_idx, w->have_v32 ? w->val32.p : nullptr,
@@ -1143,6 +1165,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     R.L = Launch{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), ge, sub, layout};
     R.b = b;
     R.scale = scale;
+    R.cscale = cscale;
     R.zhi = w->zhi.p;
     R.r = w->r32.p;
     R.r0 = w->r032.p;
@@ -1151,13 +1174,14 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     PF_CHECK(prepare_scalars(ctx, w, 1, o, o.itrmax, 0.0, 2, 0, 1, o.inner_max > 0 ? o.inner_max : 0));
     w->h_sc[1].target = o.target_relres;
     w->h_sc[1].delta = o.inner_tol > 0.0 && o.inner_tol < 1.0 ? o.inner_tol : 0.1;
+    w->h_sc[1].restart = smode == 2 ? 1 : 0;
     PF_HIP(hipMemcpyAsync(w->sc.p + 1, w->h_sc + 1, sizeof(Scal), hipMemcpyHostToDevice, ctx->stream));
-    k_rel_setz<<<ge, BT, 0, ctx->stream>>>(n, x, scale, w->zhi.p);
+    k_rel_setz<<<ge, BT, 0, ctx->stream>>>(n, x, cscale, w->zhi.p);
     PF_HIP(hipGetLastError());
     PF_CHECK(enqueue_replacement<true>(ctx, R, w->sc.p + 1, w->part.p + PART_PI * G));
     PF_CHECK(run_loop<float>(ctx, w, 1, L32, m32, w->x32.p, w->r32.p, w->r032.p, w->p32.p, w->y32.p, w->v32.p,
                              nullptr, o, &R));
-    k_rel_final<<<ge, BT, 0, ctx->stream>>>(n, w->zhi.p, w->x32.p, scale, x);
+    k_rel_final<<<ge, BT, 0, ctx->stream>>>(n, w->zhi.p, w->x32.p, cscale, x);
     PF_HIP(hipGetLastError());
     const Scal &hi = w->h_sc[1];
     res.status = hi.status;
@@ -1231,10 +1255,10 @@ int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, 
       PF_CHECK(pf_sell3_build<double>(ctx, n, row_ptr, col_idx, val, w->s64));
       w->have_s64 = true;
     }
-    if (kind == 3 && (!w->have_s32 || w->s32_scaled)) {
+    if (kind == 3 && (!w->have_s32 || w->s32_mode != 0)) {
       PF_CHECK(pf_sell3_build<float>(ctx, n, row_ptr, col_idx, val, w->s32));
       w->have_s32 = true;
-      w->s32_scaled = false;
+      w->s32_mode = 0;
     }
   }
   if (repeat < 1) repeat = 1;

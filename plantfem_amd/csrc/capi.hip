@@ -111,7 +111,7 @@ extern "C" void pf_solve_opts_default(pf_solve_opts *o) {
   o->precision = 64;
   o->er0 = (double)1.0e-10f;  // dble(1.0e-10): REAL(4) literal (FEMSolverClass.f90:70-71)
   o->relative_er = (double)1.0e-10f;
-  o->inner_tol = 0.1;
+  o->inner_tol = 0.05;
   o->target_relres = 1e-8;
   o->preconditioning = 0;
   o->spmv_kind = 0;
@@ -120,8 +120,8 @@ extern "C" void pf_solve_opts_default(pf_solve_opts *o) {
   o->debug = 0;
   o->stop_mode = 0;
   o->kernel_timing = 0;
-  o->jacobi = 1;
-  o->inner_max = 256;
+  o->jacobi = 2;
+  o->inner_max = 300;
   o->use_x0 = 0;
 }
 

@@ -166,18 +166,20 @@ int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int
                        const V *val, const V *x, V *y, double avg_row);
 int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                              int64_t nn, int32_t *ok_dev, int *ok_host);
-// scale (may be NULL): values stored as s_i a_ij s_j
+// scale (may be NULL): values stored as s_i a_ij c_j, c = cscale (NULL: c = s, the symmetric form)
 template <typename V>
 int pf_sell3_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                    const double *val, Sell3<V> &s, const double *scale = nullptr,
-                   const SliceMap *map = nullptr);
+                   const SliceMap *map = nullptr, const double *cscale = nullptr);
 template <typename V>
 int pf_sell3_values(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const double *val, Sell3<V> &s,
-                    const double *scale = nullptr);
+                    const double *scale = nullptr, const double *cscale = nullptr);
 int pf_jacobi_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
                     double *scale);
+int pf_jacobi_row_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                        double *scale);  // 1/a_ii (1 where 0)
 int pf_crs_to_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
-                  const double *scale, float *out);
+                  const double *scale, float *out, const double *cscale = nullptr);
 template <typename V>
 int pf_spmv_sell3_launch(pf_ctx *ctx, const Sell3<V> &s, const V *x, V *y);
 

@@ -110,7 +110,7 @@ template <typename V>
 __global__ void k_sell3_vals(SliceNodes sn, int64_t nslices, int64_t ncells, const int64_t *__restrict__ soff,
                              const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
                              const double *__restrict__ val, const double *__restrict__ scale,
-                             V *__restrict__ out) {
+                             const double *__restrict__ cscale, V *__restrict__ out) {
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ncells;
        g += (int64_t)gridDim.x * blockDim.x) {
     const int64_t slot = g >> 6;
@@ -131,7 +131,7 @@ __global__ void k_sell3_vals(SliceNodes sn, int64_t nslices, int64_t ncells, con
     for (int d = 0; d < 3; ++d)
       for (int k = 0; k < 3; ++k) {
         double a = live ? val[row_ptr[3 * i + d] - 1 + 3 * t + k] : 0.0;
-        if (live && scale) a = (a * scale[3 * i + d]) * scale[3 * (int64_t)col[g] + k];
+        if (live && scale) a = (a * scale[3 * i + d]) * cscale[3 * (int64_t)col[g] + k];
         o[(3 * d + k) * 64] = (V)a;
       }
   }
@@ -151,14 +151,30 @@ __global__ void k_jacobi_scale(int64_t n, const int64_t *__restrict__ row_ptr, c
   scale[i] = d != 0.0 ? 1.0 / sqrt(fabs(d)) : 1.0;
 }
 
-// FP32 copy of CRS values, optionally scaled s_i a_ij s_j (one thread per row)
+// d_i = 1/a_ii (1 where the diagonal is missing or zero)
+__global__ void k_jacobi_row_scale(int64_t n, const int64_t *__restrict__ row_ptr,
+                                   const int32_t *__restrict__ col_idx, const double *__restrict__ val,
+                                   double *__restrict__ scale) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int64_t k = row_ptr[i] - 1; k < row_ptr[i + 1] - 1; ++k)
+    if (col_idx[k] == i + 1) {
+      d = val[k];
+      break;
+    }
+  scale[i] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// FP32 copy of CRS values, optionally scaled s_i a_ij c_j (one thread per row)
 __global__ void k_crs_f32(int64_t n, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
-                          const double *__restrict__ val, const double *__restrict__ scale, float *__restrict__ out) {
+                          const double *__restrict__ val, const double *__restrict__ scale,
+                          const double *__restrict__ cscale, float *__restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int64_t k = row_ptr[i] - 1; k < row_ptr[i + 1] - 1; ++k) {
     double a = val[k];
-    if (scale) a = (a * scale[i]) * scale[col_idx[k] - 1];
+    if (scale) a = (a * scale[i]) * cscale[col_idx[k] - 1];
     out[k] = (float)a;
   }
 }
@@ -215,17 +231,27 @@ int pf_jacobi_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_
   return PF_OK;
 }
 
-int pf_crs_to_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
-                  const double *scale, float *out) {
+int pf_jacobi_row_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                        double *scale) {
   if (n <= 0) return PF_OK;
-  k_crs_f32<<<pf_blocks(n, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, scale, out);
+  k_jacobi_row_scale<<<pf_blocks(n, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, scale);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_crs_to_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                  const double *scale, float *out, const double *cscale) {
+  if (n <= 0) return PF_OK;
+  k_crs_f32<<<pf_blocks(n, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, scale,
+                                                                   cscale ? cscale : scale, out);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
 
 template <typename V>
 int pf_sell3_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
-                   const double *val, Sell3<V> &s, const double *scale, const SliceMap *map) {
+                   const double *val, Sell3<V> &s, const double *scale, const SliceMap *map,
+                   const double *cscale) {
   const int64_t nn = n / 3;
   s.nn = nn;
   s.nslices = map ? map->nslices : (nn + 63) / 64;
@@ -246,18 +272,19 @@ int pf_sell3_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t
     k_sell3_cols<<<pf_blocks(ncells, 256, 65536), 256, 0, ctx->stream>>>(sn, s.nslices, ncells, s.soff.p,
                                                                        row_ptr, col_idx, s.col.p);
   PF_HIP(hipGetLastError());
-  return pf_sell3_values<V>(ctx, n, row_ptr, val, s, scale);
+  return pf_sell3_values<V>(ctx, n, row_ptr, val, s, scale, cscale);
 }
 
 template <typename V>
 int pf_sell3_values(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const double *val, Sell3<V> &s,
-                    const double *scale) {
+                    const double *scale, const double *cscale) {
   (void)n;
   const int64_t ncells = s.nslots * 64;
   const SliceNodes sn{s.nn, s.snode, s.send};
   if (ncells > 0)
     k_sell3_vals<V><<<pf_blocks(ncells, 256, 65536), 256, 0, ctx->stream>>>(sn, s.nslices, ncells, s.soff.p,
-                                                                          row_ptr, s.col.p, val, scale, s.val.p);
+                                                                          row_ptr, s.col.p, val, scale,
+                                                                          cscale ? cscale : scale, s.val.p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -272,12 +299,12 @@ int pf_spmv_sell3_launch(pf_ctx *ctx, const Sell3<V> &s, const V *x, V *y) {
 }
 
 template int pf_sell3_build<double>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const double *,
-                                    Sell3<double> &, const double *, const SliceMap *);
+                                    Sell3<double> &, const double *, const SliceMap *, const double *);
 template int pf_sell3_build<float>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const double *,
-                                   Sell3<float> &, const double *, const SliceMap *);
+                                   Sell3<float> &, const double *, const SliceMap *, const double *);
 template int pf_sell3_values<double>(pf_ctx *, int64_t, const int64_t *, const double *, Sell3<double> &,
-                                     const double *);
+                                     const double *, const double *);
 template int pf_sell3_values<float>(pf_ctx *, int64_t, const int64_t *, const double *, Sell3<float> &,
-                                    const double *);
+                                    const double *, const double *);
 template int pf_spmv_sell3_launch<double>(pf_ctx *, const Sell3<double> &, const double *, double *);
 template int pf_spmv_sell3_launch<float>(pf_ctx *, const Sell3<float> &, const float *, float *);
