@@ -299,13 +299,18 @@ def bench_batch(a):
                                   "nu 0.3; clamped at one end/edge; pressure 1e2..1e4 Pa",
                       "n_dof": int(ndof.sum()), "nnz": int(s.nnz), "itrmax_per_mesh": a.batch_itrmax,
                       "stop": "||b - A x|| / ||b|| < 1e-8 per mesh"}}
-    for prec in (64, 32):
+    for leg_name, prec, pc in (("fp64", 64, None), ("mixed", 32, None), ("fp64_pointjacobi", 64, "PointJacobi")):
+        if pc is not None:  # the scaling is destructive: a fresh system
+            s.close()
+            s = B.build_solver(u)
         best, table = s.autotuneBatch(iters=40, precision=prec)
-        log(f"autotune precision {prec}: best {best}")
+        log(f"autotune {leg_name}: best {best}")
         s.itrmax = a.batch_itrmax
         s.relative_er = 1e-8
-        s.solveBatch(precision=prec, stop_mode=1, raise_on_breakdown=False, target_relres=1e-8)  # warm
-        x, st = s.solveBatch(precision=prec, stop_mode=1, raise_on_breakdown=False, target_relres=1e-8)
+        s.solveBatch(precision=prec, stop_mode=1, raise_on_breakdown=False, target_relres=1e-8,
+                     preconditioning=pc)  # warm
+        x, st = s.solveBatch(precision=prec, stop_mode=1, raise_on_breakdown=False, target_relres=1e-8,
+                             preconditioning=pc)
         tot = s.last_stats
         ms = tot["solve_ms"]
         conv = st["status"] == 0
@@ -322,8 +327,8 @@ def bench_batch(a):
                "max_true_relres_converged": float(st["true_relres"][conv].max()) if conv.any() else None,
                "autotune_ms_per_40_iterations": {f"{t['layout']}/{t['sub']}/{t['block']}": round(v, 2)
                                                  for t, v in table}}
-        log(f"batch precision {prec}: {json.dumps(leg)}")
-        out["fp64" if prec == 64 else "mixed"] = leg
+        log(f"batch {leg_name}: {json.dumps(leg)}")
+        out[leg_name] = leg
     out["value"] = out["fp64"]["meshes_per_s_converged"]
     print(json.dumps(out), flush=True)
 

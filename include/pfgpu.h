@@ -290,9 +290,11 @@ typedef struct {
 /* Solve every mesh (x0 = 0) with the options' precision (64 or 32), stop rule, itrmax and
  * tolerances, applying each mesh's Dirichlet elimination first as pf_system_solve does.
  * tune NULL: the shape chosen by pf_system_batch_autotune, else a default.  The queue
- * reset and the kernel are captured in one hipGraph (opts->use_graph).  mesh_stats (nmesh
- * entries) and x_out (n) may be NULL.  Returns PF_OK when every mesh converged, else the
- * status of the first mesh (in mesh order) that did not. */
+ * reset and the kernel are captured in one hipGraph (opts->use_graph).  opts->preconditioning
+ * 1 applies JacobiPreconditionerCRS to the eliminated system first (in place, row-local, so
+ * each mesh is scaled alone); 0 none; others are rejected.  mesh_stats (nmesh entries) and
+ * x_out (n) may be NULL.  Returns PF_OK when every mesh converged, else the status of the
+ * first mesh (in mesh order) that did not. */
 int pf_system_batch_solve(pf_system *sys, const pf_solve_opts *opts, const pf_batch_tune *tune, double *x_out,
                           pf_batch_mesh_stats *mesh_stats, pf_stats *total);
 

@@ -22,9 +22,10 @@
 // CRS rows reduced by SUB-lane subgroups ("subwarp width"); BTB threads per workgroup, i.e.
 // BTB/64 slices or BTB/SUB rows per pass of the workgroup ("rows per block").
 //
-// precision 32: the FP32 recurrence on S A S (S = diag(A)^-1/2 with opts.jacobi, else the
-// identity) with FP64 reliable updates (FP64 solution accumulation, FP64 true-residual
-// replacement, FP64 dot products), the per-mesh form of the mixed solver in bicgstab.hip.
+// precision 32: the FP32 recurrence on D^-1 A restarted at every FP64 replacement (opts.jacobi
+// 2, the default), or on S A S (S = diag(A)^-1/2, jacobi 1) or A (jacobi 0) with reliable
+// updates (FP64 solution accumulation, FP64 true-residual replacement, FP64 dot products):
+// the per-mesh form of the mixed solver in bicgstab.hip.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -65,7 +66,9 @@ struct BArgs {
   double *r, *r0, *p, *y, *v;
   float *r32, *r032, *p32, *y32, *v32, *zlo;
   double *zhi;
-  const double *scale;
+  const double *scale;   // FP32 system row scale
+  const double *cscale;  // FP32 system column scale (the unknown is x = C z)
+  int32_t restart;       // replacements restart the FP32 recurrence (r0 = p = r)
   pf_batch_mesh_stats *stat;
   int64_t *t_raw;  // per mesh: start, end wall-clock ticks
 };
@@ -309,12 +312,16 @@ __device__ MeshResult solve32(const BArgs &a, const BMesh &bm, double (*sh)[BTB 
         a.zlo[i] = 0.0f;
       }
       __syncthreads();
-      // r = S (b - A S z_hi) in FP64 -> FP32 ; |r|^2, |b - A x|^2, r0.r
+      // r = S (b - A C z_hi) in FP64 -> FP32 ; |r|^2, |b - A x|^2, r0.r (restart: r0 = p = r)
       double q4[3] = {0.0, 0.0, 0.0};
-      mesh_rows<double, L, SUB, BTB>(a.A, bm, ScaledX{a.scale, a.zhi}, [&](int64_t row, double ax) {
+      mesh_rows<double, L, SUB, BTB>(a.A, bm, ScaledX{a.cscale, a.zhi}, [&](int64_t row, double ax) {
         const double bu = a.b[row] - ax;
         const float rf = (float)(a.scale[row] * bu);
         a.r32[row] = rf;
+        if (a.restart) {
+          a.r032[row] = rf;
+          a.p32[row] = rf;
+        }
         q4[0] += (double)rf * (double)rf;
         q4[1] += bu * bu;
         q4[2] += (double)a.r032[row] * (double)rf;
@@ -335,8 +342,8 @@ __device__ MeshResult solve32(const BArgs &a, const BMesh &bm, double (*sh)[BTB 
       __syncthreads();
     }
   }
-  // x = S (z_hi + z_lo)
-  for (int64_t i = d0 + threadIdx.x; i < d1; i += BTB) a.x[i] = a.scale[i] * (a.zhi[i] + (double)a.zlo[i]);
+  // x = C (z_hi + z_lo)
+  for (int64_t i = d0 + threadIdx.x; i < d1; i += BTB) a.x[i] = a.cscale[i] * (a.zhi[i] + (double)a.zlo[i]);
   res.iterations = itr;
   res.replacements = nupd;
   res.rr = rr;
@@ -437,12 +444,12 @@ struct BatchWork {
   Sell3<double> s64;
   Sell3<float> s32;
   DevBuf<float> val32;
-  DevBuf<double> scale;
+  DevBuf<double> scale, ones;
   DevBuf<double> r, r0, p, y, v, zhi;
   DevBuf<float> r32, r032, p32, y32, v32, zlo;
   DevBuf<pf_batch_mesh_stats> stat;
   bool block_ok = false, have_s64 = false, have_s32 = false, have_v32 = false, have_scale = false;
-  bool scaled = false;  // the FP32 copies hold S A S (S = diag(A)^-1/2) rather than A
+  int scaled = 0;  // the FP32 copies hold 0: A, 1: S A S (S = diag(A)^-1/2), 2: D^-1 A
   uint64_t mat_version = ~0ull;
   hipGraphExec_t gexec = nullptr;
   uint64_t gkey[12] = {};
@@ -625,7 +632,7 @@ __global__ void k_fill_ones(int64_t n, double *__restrict__ a) {
 }
 
 int prepare(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
-            int64_t nnz, bool mixed, int layout, bool jacobi) {
+            int64_t nnz, bool mixed, int layout, int jacobi) {
   const int64_t n = b->n;
   SliceMap map;
   map.nslices = b->nslices;
@@ -633,8 +640,13 @@ int prepare(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, const int32_t *co
   map.send = b->send.p;
   if (mixed && (!b->have_scale || b->scaled != jacobi)) {
     PF_CHECK(b->scale.ensure(n));
-    if (jacobi) {
+    PF_CHECK(b->ones.ensure(n));
+    k_fill_ones<<<pf_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(n, b->ones.p);
+    PF_HIP(hipGetLastError());
+    if (jacobi == 1) {
       PF_CHECK(pf_jacobi_scale(ctx, n, row_ptr, col_idx, val, b->scale.p));
+    } else if (jacobi == 2) {
+      PF_CHECK(pf_jacobi_row_scale(ctx, n, row_ptr, col_idx, val, b->scale.p));
     } else {
       k_fill_ones<<<pf_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(n, b->scale.p);
       PF_HIP(hipGetLastError());
@@ -653,7 +665,8 @@ int prepare(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, const int32_t *co
     b->have_s64 = true;
   }
   if (mixed && layout == 1 && !b->have_s32) {
-    PF_CHECK(pf_sell3_build<float>(ctx, n, row_ptr, col_idx, val, b->s32, b->scale.p, &map));
+    PF_CHECK(pf_sell3_build<float>(ctx, n, row_ptr, col_idx, val, b->s32, b->scale.p, &map,
+                                   b->scaled == 2 ? b->ones.p : b->scale.p));
     const int64_t nc = b->s32.nslots * 64;
     if (nc > 0)
       k_bsell_pad<<<pf_blocks(nc, 256, 65536), 256, 0, ctx->stream>>>(b->nslices, b->s32.soff.p, b->snode.p,
@@ -663,7 +676,8 @@ int prepare(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, const int32_t *co
   }
   if (mixed && layout == 0 && !b->have_v32) {
     PF_CHECK(b->val32.ensure((size_t)nnz + 1));
-    PF_CHECK(pf_crs_to_f32(ctx, n, row_ptr, col_idx, val, b->scale.p, b->val32.p));
+    PF_CHECK(pf_crs_to_f32(ctx, n, row_ptr, col_idx, val, b->scale.p, b->val32.p,
+                           b->scaled == 2 ? b->ones.p : b->scale.p));
     b->have_v32 = true;
   }
   PF_CHECK(b->r.ensure(n));
@@ -718,6 +732,8 @@ BArgs make_args(BatchWork *b, const int64_t *row_ptr, const int32_t *col_idx, co
     a.zlo = b->zlo.p;
     a.zhi = b->zhi.p;
     a.scale = b->scale.p;
+    a.cscale = b->scaled == 2 ? b->ones.p : b->scale.p;
+    a.restart = b->scaled == 2 ? 1 : 0;
   }
   a.stat = b->stat.p;
   a.t_raw = b->t_raw.p;
@@ -824,7 +840,7 @@ int pf_batch_solve_device(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, con
     t.blocks_per_cu = 0;
   }
   PF_CHECK(valid_tune(t, b->block_ok));
-  PF_CHECK(prepare(ctx, b, row_ptr, col_idx, val, nnz, mixed, t.layout, o.jacobi != 0));
+  PF_CHECK(prepare(ctx, b, row_ptr, col_idx, val, nnz, mixed, t.layout, o.jacobi == 2 ? 2 : (o.jacobi != 0 ? 1 : 0)));
   const BArgs a = make_args(b, row_ptr, col_idx, val, rhs, x, o, mixed);
   float ms = 0.f;
   PF_CHECK(run_batch(ctx, b, a, t, mixed, o.use_graph != 0, &ms));

@@ -1003,7 +1003,16 @@ extern "C" int pf_system_batch_solve(pf_system *s, const pf_solve_opts *opts, co
   DeviceGuard guard(ctx);
   pf_solve_opts o;
   if (opts) o = *opts; else pf_solve_opts_default(&o);
+  if (o.preconditioning != 0 && o.preconditioning != 1) {
+    pf_set_error("batch: preconditioning must be 0 (none) or 1 (PointJacobi)");
+    return PF_INVALID;
+  }
   PF_CHECK(batch_prepare(s));
+  if (o.preconditioning == 1) {  // JacobiPreconditionerCRS is row-local: the union's = every mesh's
+    PF_CHECK(pf_jacobi_rows(ctx, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->rhs.p));
+    mark_changed(s);
+    ++s->rhs_version;
+  }
   const int rc = pf_batch_solve_device(ctx, s->batch, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz, s->rhs.p,
                                        s->x.p, o, tune, s->mat_version, mesh_stats, total);
   if (rc < 0 || rc == PF_INVALID) return rc;

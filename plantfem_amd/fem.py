@@ -566,9 +566,15 @@ class FEMSolver:
         return best.as_dict(), [(tried[i].as_dict(), float(ms[i])) for i in range(min(nt.value, 32))]
 
     def solveBatch(self, precision: int = 64, tune: dict | None = None, download: bool = True,
-                   raise_on_breakdown: bool = True, **opts):
-        """Solve every mesh of the batch in one persistent kernel; returns (x, per-mesh stats)."""
+                   raise_on_breakdown: bool = True, preconditioning: str | None = None, **opts):
+        """Solve every mesh of the batch in one persistent kernel; returns (x, per-mesh stats).
+        preconditioning="PointJacobi" scales the eliminated rows first (JacobiPreconditionerCRS,
+        in place) and solves the scaled systems."""
         self._flush()
+        if preconditioning is not None:
+            if preconditioning != "PointJacobi":
+                raise ValueError(f"solveBatch :: invalid preconditioning {preconditioning!r}")
+            opts["preconditioning"] = 1
         o = L.default_opts(itrmax=int(self.itrmax), er0=float(self.er0), relative_er=float(self.relative_er),
                            debug=int(bool(self.debug)), precision=int(precision), **opts)
         t = None if tune is None else L.BatchTune(**tune)

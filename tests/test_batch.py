@@ -134,9 +134,10 @@ def test_batch_fp64_matches_oracle(built, tune):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("layout", [1, 0])
-@pytest.mark.parametrize("jacobi", [1, 0])
+@pytest.mark.parametrize("jacobi", [2, 1, 0])
 def test_batch_mixed_blocks_match_oracle(ctx, layout, jacobi):
-    """Mixed precision on compact blocks, where the FP32 recurrence converges."""
+    """Mixed precision on compact blocks, where the FP32 recurrence converges in every scaling
+    (2: rows with restarts, the default; 1: symmetric; 0: none)."""
     rng = np.random.default_rng(7)
     orgs = [B.block(k, rng) for k in (4, 6, 9, 5, 8)]
     u = B.union(orgs)
@@ -165,6 +166,34 @@ def test_batch_mixed_reports_failure_honestly(built):
             assert st["true_relres"][m] < 1e-10
         else:
             assert st["status"][m] in (1, 2, 3)
+
+
+@pytest.mark.gpu
+def test_batch_point_jacobi_matches_oracle(ctx):
+    """solveBatch(preconditioning="PointJacobi"): every mesh's system is scaled by its rows'
+    diagonal (the oracle's JacobiPreconditionerCRS of that mesh alone) and solved."""
+    rng = np.random.default_rng(11)
+    orgs = [B.block(k, rng) for k in (4, 7, 5)] + [B.leaf(6, 5, rng)]
+    u = B.union(orgs)
+    s = B.build_solver(u, ctx)
+    s.itrmax = 100000
+    s.relative_er = 1e-14
+    s.er0 = 1e-30
+    x, st = s.solveBatch(preconditioning="PointJacobi")
+    say("pointjacobi iterations", st["iterations"].tolist(), "relres", st["true_relres"].max())
+    assert np.all(st["status"] == 0)
+    crs = s.getCRS()
+    for m, o in enumerate(orgs):
+        rp, ci, val, rhs, fixed = oracle_system(o)
+        ub = np.zeros(o.ndof)
+        v2, b2 = O.apply_dirichlet(rp, ci, val, rhs, fixed, ub)
+        vj, bj = O.jacobi_scale(rp, ci, v2, b2)
+        d = u.dof_range(m)
+        k0, k1 = crs.row_ptr[d.start] - 1, crs.row_ptr[d.stop] - 1
+        assert np.array_equal(crs.val[k0:k1], vj)  # the mesh's scaled block, bitwise
+        xo, sto, *_ = O.bicgstab(rp, ci, vj, bj, relative_er=1e-14, er0=1e-30)
+        xo[fixed.astype(bool)] = ub[fixed.astype(bool)]
+        assert np.linalg.norm(x[d] - xo) / np.linalg.norm(xo) <= 1e-10
 
 
 @pytest.mark.gpu
