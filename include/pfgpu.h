@@ -116,6 +116,12 @@ int pf_hex8_stiffness(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoor
                       const int32_t *elemnod, const double *E, const double *nu, int32_t jmode,
                       int64_t first, int64_t count, double *ke);
 
+/* MassMatrixFEMDomain(ElementID, Density, DOF=3) (FEMDomainClass.f90:10874-10945) for elements
+ * first..first+count-1 (1-based): me receives count 24x24 column-major matrices; density has
+ * 1 or ne entries (per element). */
+int pf_hex8_mass_matrix(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                        const double *density, int64_t ndensity, int64_t first, int64_t count, double *me);
+
 /* StiffnessMatrix_as_CRS_FEMDomain(YoungModulus(:),PoissonRatio(:))
  * (FEMDomainClass.f90:11123-11232): val (nnz) of the ZeroMatrix pattern given by
  * row_ptr/col_idx, summed in ascending element order. */
@@ -217,6 +223,34 @@ int pf_system_fix(pf_system *sys, int64_t nids, const int32_t *ids, const double
 int pf_system_solve(pf_system *sys, const pf_solve_opts *opts, const double *x0, double *x_out,
                     pf_stats *stats);
 int pf_system_get_x(pf_system *sys, double *x);
+
+/* MassMatrix_as_CRS_FEMDomain(Density(:), DOF=3) (FEMDomainClass.f90:10364-10467) assembled
+ * into the CRS values on the device (overwrites val, element contributions in ascending element
+ * order); density: one value per element (host array), or NULL to use density_all. */
+int pf_system_mass_matrix(pf_system *sys, const double *density, double density_all);
+
+/* zerosFEMSolver (FEMSolverClass.f90:1310-1322): CRS values and RHS set to zero. */
+int pf_system_zeros(pf_system *sys);
+
+/* keepThisMatrixAs(As) (FEMSolverClass.f90:1544-1604): a named copy of the current CRS values
+ * (the eigen solver reads "A" = stiffness and "B" = mass). */
+int pf_system_keep_matrix(pf_system *sys, const char *name);
+
+/* fix_eig(IDs) (FEMSolverClass.f90:1607-1628): 1-based DOF ids whose amplitudes are zero in
+ * the eigen problem (merged with earlier calls, unique). */
+int pf_system_fix_eig(pf_system *sys, int64_t nids, const int32_t *ids);
+
+/* eigFEMSolver (FEMSolverClass.f90:1384-1541): A u = lambda B u with the kept "A" and "B", the
+ * fix_eig DOFs removed.  nev <= 0: every eigenpair of the reduced problem, as the reference's
+ * default LAPACK DSPGVD path (dense on the device, at most 16384 free DOFs); nev > 0: the nev
+ * smallest by LOBPCG (its use_LOBPCG option; tol on max_i |A x_i - l_i B x_i| / (max_j |l_j| |B x_i|),
+ * default 1e-8 for tol <= 0; maxit default 10000).  eigen_values (ascending) and eigen_vectors
+ * (n x count column-major, B-orthonormal, zero at the removed DOFs) are host arrays and may be
+ * NULL; *count receives the number of pairs (pf_system_eig_count gives it for nev <= 0).
+ * stats: status, iterations (LOBPCG), true_relres = the worst relative residual, solve_ms. */
+int pf_system_eig_count(pf_system *sys, int32_t *count);
+int pf_system_eig(pf_system *sys, int32_t nev, double tol, int32_t maxit, double *eigen_values,
+                  double *eigen_vectors, int32_t *count, pf_stats *stats);
 
 /* The ILU(0) factor kept by the system (FEMSolver_%ILU_MATRIX, FEMSolverClass.f90:67,
  * :1777-1784): pf_system_ilu_factor computes it from the eliminated matrix when none is kept

@@ -35,6 +35,8 @@ _SIG = {
     "orc_bicgstab_crs2": (I32, [I64, P64, P32, PD, PD, PD, I32, D, D, I32, P32, PD, PD, PD]),
     "orc_solve": (I32, [I64, P64, P32, PD, PD, PU8, PD, PD, I32, D, D, I32, P32, PD, PD]),
     "orc_jacobi_scale": (None, [I64, P64, P32, PD, PD]),
+    "orc_hex8_mass_matrix": (None, [PD, D, PD]),
+    "orc_mass_matrix_crs": (I32, [I64, I64, PD, P32, PD, P64, P32, PD]),
     "orc_ilu0": (None, [I64, P64, P32, PD]),
     "orc_ilu_apply": (None, [I64, P64, P32, PD, PD, PD]),
     "orc_bicgstab_ilu": (I32, [I64, P64, P32, PD, PD, PD, PD, I32, D, D, I32, P32, PD, PD]),
@@ -229,6 +231,28 @@ def solve(row_ptr, col_idx, val, rhs, fixed, ubar, x0=None, itrmax=100000, er0=N
                           _p(_c(ubar, np.float64), C.c_double), _p(x, C.c_double), itrmax, er0, relative_er,
                           nthreads, C.byref(it), C.byref(rr), C.byref(rr1))
     return x, st, it.value, rr.value, rr1.value
+
+
+def hex8_mass_matrix(X, rho):
+    """MassMatrixFEMDomain of one hex8 (X: 8x3 node coordinates): 24x24."""
+    me = np.zeros(576)
+    load().orc_hex8_mass_matrix(_p(_c(X, np.float64).ravel(), C.c_double), float(rho), _p(me, C.c_double))
+    return me.reshape(24, 24, order="F")
+
+
+def mass_matrix_crs(nodcoord, elemnod, rho, row_ptr, col_idx):
+    """MassMatrix_as_CRS (rho: one value or one per element) on the given ZeroMatrix pattern."""
+    xyz = _fl(nodcoord, np.float64)
+    en = _fl(elemnod, np.int32)
+    nn, ne = np.asarray(nodcoord).shape[0], np.asarray(elemnod).shape[0]
+    r = np.broadcast_to(np.asarray(rho, dtype=np.float64), (ne,)).copy()
+    val = np.zeros(int(row_ptr[-1] - 1))
+    st = load().orc_mass_matrix_crs(nn, ne, _p(xyz, C.c_double), _p(en, C.c_int32), _p(r, C.c_double),
+                                    _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                                    _p(val, C.c_double))
+    if st != 0:
+        raise ValueError("mass matrix entry outside the pattern")
+    return val
 
 
 def jacobi_scale(row_ptr, col_idx, val, rhs):

@@ -839,3 +839,56 @@ done:
     free(r);
     return status;
 }
+
+/* MassMatrixFEMDomain(ElementID, Density, DOF=3) (FEMDomainClass.f90:10874-10945): per Gauss
+ * point MassMatrix += matmul(Nmat, transpose(Nmat)) * det(J), then rho * MassMatrix.  Entry
+ * (3a+d, 3b+k) of matmul(Nmat, Nmat^T) is N_a N_b when d == k and 0 otherwise (the other terms
+ * are products with zero).  me is 24x24 column-major: me[c*24 + r]. */
+void orc_hex8_mass_matrix(const double *X, double rho, double *me)
+{
+    double acc[8][8];
+    for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+    for (int gp = 0; gp < 8; ++gp) {
+        const double detJ = hex8_gp_G(X, gp, 0, NULL);
+        double N[8];
+        for (int a = 0; a < 8; ++a) N[a] = hex8_N(a, GPT[gp]);
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) acc[a][b] = acc[a][b] + (N[a] * N[b]) * detJ;
+    }
+    for (int c = 0; c < 24; ++c)
+        for (int r = 0; r < 24; ++r)
+            me[c * 24 + r] = (r % 3 == c % 3) ? rho * acc[r / 3][c / 3] : rho * 0.0;
+}
+
+/* MassMatrix_as_CRS_FEMDomain(Density(:), DOF=3) (FEMDomainClass.f90:10364-10467): the
+ * element mass matrices added into the ZeroMatrix pattern in ascending element order (the
+ * serial branch; the OpenMP branch adds the same terms without an order) */
+int orc_mass_matrix_crs(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                        const double *rho, const int64_t *row_ptr, const int32_t *col_idx, double *val)
+{
+    const int64_t n = 3 * nn;
+    for (int64_t k = 0; k < row_ptr[n] - 1; ++k) val[k] = 0.0;
+    for (int64_t e = 0; e < ne; ++e) {
+        double X[24], me[576];
+        int32_t nodes[8];
+        for (int a = 0; a < 8; ++a) {
+            nodes[a] = elemnod[(int64_t)a * ne + e];
+            for (int d = 0; d < 3; ++d) X[a * 3 + d] = nodcoord[(int64_t)d * nn + nodes[a] - 1];
+        }
+        orc_hex8_mass_matrix(X, rho[e], me);
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b)
+                for (int d1 = 0; d1 < 3; ++d1)
+                    for (int d2 = 0; d2 < 3; ++d2) {
+                        const int64_t p1 = 3 * (int64_t)(nodes[a] - 1) + d1;
+                        const int32_t p2 = 3 * (nodes[b] - 1) + d2 + 1;
+                        int64_t q = row_ptr[p1] - 1;
+                        for (; q < row_ptr[p1 + 1] - 1; ++q)
+                            if (col_idx[q] == p2) break;
+                        if (q >= row_ptr[p1 + 1] - 1) return ORC_INVALID;
+                        val[q] = val[q] + me[(3 * b + d2) * 24 + 3 * a + d1];
+                    }
+    }
+    return ORC_OK;
+}

@@ -126,6 +126,14 @@ int orc_bicgstab_ilu(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, 
                      const double *fval, const double *b, double *x, int itrmax, double er0,
                      double relative_er, int nthreads, int *iters, double *rr_out, double *rr1_out);
 
+/* MassMatrixFEMDomain (FEMDomainClass.f90:10874-10945), DOF = 3: 24x24 column-major. */
+void orc_hex8_mass_matrix(const double *X, double rho, double *me);
+
+/* MassMatrix_as_CRS_FEMDomain (FEMDomainClass.f90:10364-10467) on the ZeroMatrix pattern,
+ * ascending element order; rho per element. */
+int orc_mass_matrix_crs(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                        const double *rho, const int64_t *row_ptr, const int32_t *col_idx, double *val);
+
 #ifdef __cplusplus
 }
 #endif

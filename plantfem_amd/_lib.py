@@ -160,6 +160,14 @@ SIGNATURES = {
                                            C.POINTER(BatchTune), PF64, PI32]),
     "pf_hex8_mass_vector": (C.c_int, [P, I64, I64, PF64, PI32, F64, PF64, I64, I64, PF64]),
     "pf_system_add_body_force": (C.c_int, [P, PF64, F64, PF64]),
+    "pf_hex8_mass_matrix": (C.c_int, [P, I64, I64, PF64, PI32, PF64, I64, I64, I64, PF64]),
+    "pf_system_mass_matrix": (C.c_int, [P, PF64, F64]),
+    "pf_system_zeros": (C.c_int, [P]),
+    "pf_system_keep_matrix": (C.c_int, [P, C.c_char_p]),
+    "pf_system_fix_eig": (C.c_int, [P, I64, PI32]),
+    "pf_system_eig_count": (C.c_int, [P, C.POINTER(C.c_int32)]),
+    "pf_system_eig": (C.c_int, [P, C.c_int32, F64, C.c_int32, PF64, PF64, C.POINTER(C.c_int32),
+                                C.POINTER(Stats)]),
     "pf_element_stress": (C.c_int, [P, I64, I64, PF64, PI32, PF64, PF64, I64, PF64, I64, I32, PF64]),
     "pf_system_element_stress": (C.c_int, [P, PF64, I64, PF64, I64, I32, PF64]),
     "pf_system_stiffness_dev": (C.c_int, [P, P, P, I32]),

@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FC", "/opt/rocm/llvm/bin/flang")
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["scan.hip", "graph.hip", "assemble.hip", "bc.hip", "spmv.hip", "ilu.hip", "bicgstab.hip", "batch.hip", "multi.hip", "capi.hip"]
+HIP_SOURCES = ["scan.hip", "graph.hip", "assemble.hip", "bc.hip", "spmv.hip", "ilu.hip", "bicgstab.hip", "batch.hip", "multi.hip", "eig.hip", "capi.hip"]
 # files whose arithmetic must follow the reference's operation order exactly
 NO_CONTRACT = {"assemble.hip", "bc.hip", "ilu.hip"}
 
@@ -70,7 +70,8 @@ def build_hip(jobs: int = 8) -> Path:
         list(ex.map(_run, cmds))
     out = LIB / "libpfgpu.so"
     tmp = LIB / "libpfgpu.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
+          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, out)
     return out
 

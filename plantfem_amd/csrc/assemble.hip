@@ -316,6 +316,89 @@ __global__ void k_mass_vec_batch(int64_t nn, int64_t ne, const double *__restric
   out[q * 24 + 3 * a + 2] = m2;
 }
 
+// MassMatrixFEMDomain (FEMDomainClass.f90:10874-10945), DOF = 3, row block of local node a:
+// m_b = sum_gp (N_a N_b) detJ for the 8 local nodes b; the matrix entry (3a+d, 3b+k) is
+// rho * m_b when d == k and rho * 0 otherwise (products with zeros in matmul(Nmat, Nmat^T))
+__device__ inline void hex8_mass_row(const double (&X)[8][3], int a, double (&m)[8]) {
+  double G[3][8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) m[b] = 0.0;
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    const double detJ = hex8_gp_G(X, gp, 0, G, false);
+    const double Na = hex8_N(a, gp);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) m[b] = m[b] + (Na * hex8_N(b, gp)) * detJ;
+  }
+}
+
+// dense element mass matrices of elements first..first+count-1 (24x24 column-major): one
+// thread per (element, local node a) writes the three rows of a
+__global__ void k_mass_mat_batch(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                                 const int32_t *__restrict__ en, const double *__restrict__ rho, int srho,
+                                 int64_t first, int64_t count, double *__restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= count * 8) return;
+  const int64_t q = t / 8;
+  const int a = (int)(t - q * 8);
+  const int64_t e = first + q;
+  int32_t nodes[8];
+  double X[8][3];
+  load_element(nn, ne, xyz, en, e, nodes, X);
+  double m[8];
+  hex8_mass_row(X, a, m);
+  const double r = rho[srho ? e : 0];
+  double *o = out + q * 576;
+  for (int b = 0; b < 8; ++b)
+    for (int k = 0; k < 3; ++k)
+      for (int d = 0; d < 3; ++d) o[(3 * b + k) * 24 + 3 * a + d] = d == k ? r * m[b] : r * 0.0;
+}
+
+// MassMatrix_as_CRS_FEMDomain (FEMDomainClass.f90:10364-10467): one thread per CRS row, the
+// element contributions gathered in ascending element order
+__global__ void __launch_bounds__(128) k_mass_rows(
+    int64_t nn, int64_t ne, const double *__restrict__ xyz, const int32_t *__restrict__ en,
+    const double *__restrict__ rho, int srho, const int64_t *__restrict__ eoff, const int32_t *__restrict__ elist,
+    const int64_t *__restrict__ noff, const int32_t *__restrict__ ncol, const int64_t *__restrict__ row_ptr,
+    double *__restrict__ val) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= 3 * nn) return;
+  const int64_t i = r / 3;
+  const int d = (int)(r - 3 * i);
+  const int64_t base = row_ptr[r] - 1;
+  const int64_t len = row_ptr[r + 1] - 1 - base;
+  for (int64_t k = 0; k < len; ++k) val[base + k] = 0.0;
+  const int64_t c0 = noff[i];
+  const int32_t deg = (int32_t)(noff[i + 1] - c0);
+  const int64_t s0 = eoff[i], s1 = eoff[i + 1];
+  for (int64_t s = s0; s < s1; ++s) {
+    const int64_t e = elist[s];
+    if (s > s0 && elist[s - 1] == e) continue;  // node listed twice in one element
+    int32_t nodes[8];
+    double X[8][3];
+    load_element(nn, ne, xyz, en, e, nodes, X);
+    const double re = rho[srho ? e : 0];
+    for (int a = 0; a < 8; ++a) {
+      if (nodes[a] != (int32_t)(i + 1)) continue;
+      double m[8];
+      hex8_mass_row(X, a, m);
+      for (int b = 0; b < 8; ++b) {
+        int32_t p = 0;
+        const int32_t target = nodes[b] - 1;
+        for (int32_t q = 0; q < deg; ++q)
+          if (ncol[c0 + q] == target) {
+            p = q;
+            break;
+          }
+        double *dst = val + base + 3 * (int64_t)p;
+        dst[0] = dst[0] + (d == 0 ? re * m[b] : re * 0.0);
+        dst[1] = dst[1] + (d == 1 ? re * m[b] : re * 0.0);
+        dst[2] = dst[2] + (d == 2 ? re * m[b] : re * 0.0);
+      }
+    }
+  }
+}
+
 // rhs += sum over elements (ascending id) of their mass vectors: one thread per node
 __global__ void k_body_force(int64_t nn, int64_t ne, const double *__restrict__ xyz, const int32_t *__restrict__ en,
                              const int64_t *__restrict__ eoff, const int32_t *__restrict__ elist,
@@ -768,6 +851,32 @@ int pf_hex8_mass_batch(pf_ctx *ctx, const NodeGraph &g, double rho, const double
   if (count <= 0) return PF_OK;
   k_mass_vec_batch<<<pf_blocks(count * 8, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, rho,
                                                                       accel[0], accel[1], accel[2], first, count, out);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_hex8_mass_mat(pf_ctx *ctx, const NodeGraph &g, const double *rho, int srho, int64_t first, int64_t count,
+                     double *out) {
+  if (g.nne != 8 || !g.has_coords) {
+    pf_set_error("mass matrices need an 8-node mesh with coordinates");
+    return PF_INVALID;
+  }
+  if (count <= 0) return PF_OK;
+  k_mass_mat_batch<<<pf_blocks(count * 8, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, rho,
+                                                                      srho, first, count, out);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_assemble_mass(pf_ctx *ctx, const NodeGraph &g, const double *rho, int srho, const int64_t *row_ptr,
+                     double *val) {
+  if (g.nne != 8 || !g.cols_built || !g.has_coords) {
+    pf_set_error("mass assembly needs an 8-node mesh with coordinates and its graph built");
+    return PF_INVALID;
+  }
+  if (g.nn == 0) return PF_OK;
+  k_mass_rows<<<pf_blocks(3 * g.nn, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, rho, srho,
+                                                                g.eoff.p, g.elist.p, g.noff.p, g.ncol.p, row_ptr, val);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -1,8 +1,11 @@
 // capi.hip -- the extern "C" entry points declared in include/pfgpu.h.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <string>
 #include <vector>
 
 #include "pf_internal.hpp"
@@ -237,6 +240,27 @@ extern "C" int pf_hex8_mass_vector(pf_ctx *ctx, int64_t nn, int64_t ne, const do
   PF_CHECK(dmv.ensure((size_t)count * 24 + 1));
   PF_CHECK(pf_hex8_mass_batch(ctx, g, density, accel, first - 1, count, dmv.p));
   return download(ctx, mv, dmv.p, (size_t)count * 24);
+}
+
+extern "C" int pf_hex8_mass_matrix(pf_ctx *ctx, int64_t nn, int64_t ne, const double *nodcoord,
+                                   const int32_t *elemnod, const double *density, int64_t ndensity, int64_t first,
+                                   int64_t count, double *me) {
+  PF_CHECK(check_ctx(ctx));
+  if (first < 1 || count < 0 || first - 1 + count > ne || !me || !nodcoord || !density ||
+      !(ndensity == 1 || ndensity == ne)) {
+    pf_set_error("pf_hex8_mass_matrix: element range %lld..%lld outside 1..%lld, a null argument, or a density "
+                 "count other than 1 or ne", (long long)first, (long long)(first + count - 1), (long long)ne);
+    return PF_INVALID;
+  }
+  DeviceGuard guard(ctx);
+  NodeGraph g;
+  PF_CHECK(set_mesh(ctx, g, nn, ne, 8, nodcoord, elemnod));
+  PF_CHECK(pf_graph_check_mesh(ctx, g));
+  DevBuf<double> drho, dme;
+  PF_CHECK(upload(ctx, drho, density, (size_t)ndensity));
+  PF_CHECK(dme.ensure((size_t)count * 576 + 1));
+  PF_CHECK(pf_hex8_mass_mat(ctx, g, drho.p, ndensity == ne ? 1 : 0, first - 1, count, dme.p));
+  return download(ctx, me, dme.p, (size_t)count * 576);
 }
 
 static int stress_params(int64_t ne, int64_t nE, int64_t nnu, const double *E, const double *nu) {
@@ -512,6 +536,14 @@ struct pf_system {
   MultiWork *multi = nullptr;
   int32_t jmode = 0;          // J mode of the last assembly (reused by the sensitivities)
   bool ubar_nonzero = false;  // some fixed DOF has a nonzero prescribed value
+  // keepThisMatrixAs: named copies of the CRS values (same pattern) and the fix_eig list
+  struct Kept {
+    std::string name;
+    int64_t nnz = 0;
+    DevBuf<double> val;
+  };
+  std::vector<std::unique_ptr<Kept>> kept;
+  std::vector<int32_t> fix_eig;
   ~pf_system() {
     pf_solver_work_destroy(work);
     pf_batch_work_destroy(batch);
@@ -911,6 +943,104 @@ extern "C" int pf_system_ilu_reset(pf_system *s) {
   PF_CHECK(sys_check(s, false));
   pf_work_ilu_reset(s->work);
   return PF_OK;
+}
+
+extern "C" int pf_system_mass_matrix(pf_system *s, const double *density, double density_all) {
+  PF_CHECK(sys_check(s, false));
+  if (!s->has_mesh || s->g.nne != 8) {
+    pf_set_error("pf_system_mass_matrix: hex8 mesh required");
+    return PF_INVALID;
+  }
+  if (!(s->graph_crs && s->dof == 3)) PF_CHECK(pf_system_zero_matrix(s, 3, nullptr));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  DevBuf<double> drho;
+  if (density) PF_CHECK(upload(ctx, drho, density, (size_t)s->g.ne));
+  else PF_CHECK(upload(ctx, drho, &density_all, 1));
+  PF_CHECK(pf_assemble_mass(ctx, s->g, drho.p, density ? 1 : 0, s->row_ptr.p, s->val.p));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  mark_changed(s);
+  return PF_OK;
+}
+
+extern "C" int pf_system_zeros(pf_system *s) {
+  PF_CHECK(sys_check(s, true));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_HIP(hipMemsetAsync(s->val.p, 0, sizeof(double) * (s->nnz + 1), ctx->stream));
+  if (s->rhs.p) PF_HIP(hipMemsetAsync(s->rhs.p, 0, sizeof(double) * (s->n + 1), ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  mark_changed(s);
+  ++s->rhs_version;
+  return PF_OK;
+}
+
+extern "C" int pf_system_keep_matrix(pf_system *s, const char *name) {
+  PF_CHECK(sys_check(s, true));
+  if (!name) return PF_INVALID;
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  pf_system::Kept *k = nullptr;
+  for (auto &e : s->kept)
+    if (e->name == name) k = e.get();
+  if (!k) {
+    s->kept.emplace_back(new pf_system::Kept());
+    k = s->kept.back().get();
+    k->name = name;
+  }
+  PF_CHECK(k->val.ensure((size_t)s->nnz + 1));
+  PF_HIP(hipMemcpyAsync(k->val.p, s->val.p, sizeof(double) * s->nnz, hipMemcpyDeviceToDevice, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  k->nnz = s->nnz;
+  return PF_OK;
+}
+
+extern "C" int pf_system_fix_eig(pf_system *s, int64_t nids, const int32_t *ids) {
+  PF_CHECK(sys_check(s, false));
+  if (nids < 0 || (nids > 0 && !ids)) return PF_INVALID;
+  s->fix_eig.insert(s->fix_eig.end(), ids, ids + nids);
+  std::sort(s->fix_eig.begin(), s->fix_eig.end());
+  s->fix_eig.erase(std::unique(s->fix_eig.begin(), s->fix_eig.end()), s->fix_eig.end());
+  return PF_OK;
+}
+
+extern "C" int pf_system_eig_count(pf_system *s, int32_t *count) {
+  PF_CHECK(sys_check(s, true));
+  if (!count) return PF_INVALID;
+  int64_t nfix = 0;
+  for (int32_t id : s->fix_eig)
+    if (id >= 1 && id <= s->n) ++nfix;
+  *count = (int32_t)(s->n - nfix);
+  return PF_OK;
+}
+
+extern "C" int pf_system_eig(pf_system *s, int32_t nev, double tol, int32_t maxit, double *eigen_values,
+                             double *eigen_vectors, int32_t *count, pf_stats *stats) {
+  PF_CHECK(sys_check(s, true));
+  pf_system::Kept *A = nullptr, *B = nullptr;
+  for (auto &e : s->kept) {
+    if (e->name == "A") A = e.get();
+    if (e->name == "B") B = e.get();
+  }
+  if (!A || !B || A->nnz != s->nnz || B->nnz != s->nnz) {
+    pf_set_error("eig: keep the stiffness as \"A\" and the mass as \"B\" on the current CRS pattern first");
+    return PF_INVALID;
+  }
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  int32_t cnt = 0;
+  EigResult r;
+  PF_CHECK(pf_eig_device(ctx, s->n, s->row_ptr.p, s->col_idx.p, A->val.p, B->val.p, s->nnz, s->fix_eig, nev,
+                         tol > 0.0 ? tol : 1e-8, maxit > 0 ? maxit : 10000, eigen_values, eigen_vectors, &cnt, r));
+  if (count) *count = cnt;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->status = r.status;
+    stats->iterations = r.iterations;
+    stats->true_relres = r.max_res;
+    stats->solve_ms = r.ms;
+  }
+  return r.status;
 }
 
 extern "C" int pf_system_get_x(pf_system *s, double *x) {

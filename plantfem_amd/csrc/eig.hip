@@ -1,0 +1,337 @@
+// eig.hip -- eigFEMSolver (FEMSolverClass.f90:1384-1541) on the device: A u = lambda B u for
+// the matrices kept as "A" (stiffness) and "B" (mass) by keepThisMatrixAs, with the DOFs given
+// to fix_eig removed from both (reduce_crs_matrix) and put back as zeros in the eigenvectors.
+//
+//  * every eigenpair (the reference's default: LAPACK DSPGVD on the packed upper triangles of
+//    the reduced matrices): the reduced matrices are scattered into dense column-major arrays
+//    on the device and rocSOLVER's dsygvd (itype 1, eigenvectors, upper) solves them; the
+//    eigenvectors come back B-orthonormal, eigenvalues ascending, as DSPGVD returns them.
+//  * the nev lowest pairs (the reference's use_LOBPCG option): block LOBPCG in the full DOF
+//    space with the fixed rows held at zero, the CRS SpMV for A and B, a Jacobi (diag A)
+//    preconditioner, Rayleigh-Ritz on [X W P] with rocBLAS Gram products and rocSOLVER dsygvd
+//    on the (up to 3 nev)^2 projected pencil; W and P are scaled to unit B-norm first.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pf_internal.hpp"
+
+#define PF_RB(call)                                                                 \
+  do {                                                                              \
+    rocblas_status s_ = (call);                                                     \
+    if (s_ != rocblas_status_success) {                                             \
+      pf_set_error("%s:%d %s -> rocblas status %d", __FILE__, __LINE__, #call, (int)s_); \
+      return PF_ERR_HIP;                                                            \
+    }                                                                               \
+  } while (0)
+
+namespace {
+
+// dense reduced matrix, column-major nf x nf: D[red(j)*nf + red(i)] = a_ij for free i, j
+__global__ void k_dense_scatter(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                const double *__restrict__ val, const int32_t *__restrict__ red, int64_t nf,
+                                double *__restrict__ D) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t ir = red[i];
+  if (ir < 0) return;
+  for (int64_t k = rp[i] - 1; k < rp[i + 1] - 1; ++k) {
+    const int32_t jr = red[ci[k] - 1];
+    if (jr >= 0) D[(int64_t)jr * nf + ir] += val[k];
+  }
+}
+
+// out (n x cols) = reduced vectors Z (nf x cols) put back at the free DOFs, 0 at the fixed ones
+__global__ void k_expand(int64_t n, int64_t nf, int64_t cols, const int32_t *__restrict__ red,
+                         const double *__restrict__ Z, double *__restrict__ out) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * cols;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / n, i = t - c * n;
+    const int32_t ir = red[i];
+    out[t] = ir >= 0 ? Z[c * nf + ir] : 0.0;
+  }
+}
+
+// deterministic start block: uniform(-0.5, 0.5) from a hash of (row, column), 0 at fixed rows
+__global__ void k_init_block(int64_t n, int64_t m, const int32_t *__restrict__ red, double *__restrict__ X) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * m; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n;
+    uint64_t h = (uint64_t)t * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    X[t] = red[i] >= 0 ? (double)(h >> 11) * (1.0 / 9007199254740992.0) - 0.5 : 0.0;
+  }
+}
+
+// W = dinv (AX - lambda BX), 0 at fixed rows; R (residual) kept for the norms
+__global__ void k_residual_block(int64_t n, int64_t m, const double *__restrict__ AX, const double *__restrict__ BX,
+                                 const double *__restrict__ lam, const double *__restrict__ dinv,
+                                 const int32_t *__restrict__ red, double *__restrict__ R, double *__restrict__ W) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * m; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / n, i = t - c * n;
+    const double r = red[i] >= 0 ? AX[t] - lam[c] * BX[t] : 0.0;
+    R[t] = r;
+    W[t] = dinv[i] * r;
+  }
+}
+
+__global__ void k_dinv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                       const double *__restrict__ val, const int32_t *__restrict__ red, double *__restrict__ dinv) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int64_t k = rp[i] - 1; k < rp[i + 1] - 1; ++k)
+    if (ci[k] == i + 1) {
+      d = val[k];
+      break;
+    }
+  dinv[i] = (red[i] >= 0 && d != 0.0) ? 1.0 / fabs(d) : 0.0;
+}
+
+// columns c of S, AS, BS (n rows) scaled by s[c]
+__global__ void k_scale_cols(int64_t n, int64_t c0, int64_t cols, const double *__restrict__ s, double *__restrict__ S,
+                             double *__restrict__ AS, double *__restrict__ BS) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * cols; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / n;
+    const int64_t g = c0 * n + t;
+    const double f = s[c];
+    S[g] *= f;
+    AS[g] *= f;
+    BS[g] *= f;
+  }
+}
+
+__global__ void k_copy(int64_t count, const double *__restrict__ a, double *__restrict__ b) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+    b[t] = a[t];
+}
+
+struct Handle {
+  rocblas_handle h = nullptr;
+  ~Handle() {
+    if (h) (void)rocblas_destroy_handle(h);
+  }
+};
+
+unsigned grid_for(int64_t work) { return pf_blocks(work, 256, 65536); }
+
+}  // namespace
+
+int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *A,
+                  const double *B, int64_t nnz, const std::vector<int32_t> &fixed_ids, int32_t nev, double tol,
+                  int32_t maxit, double *values, double *vectors, int32_t *count, EigResult &res) {
+  res = EigResult();
+  hipStream_t st = ctx->stream;
+  // reduced index of every DOF (-1: removed)
+  std::vector<int32_t> h_red((size_t)n, 0);
+  for (int32_t id : fixed_ids)
+    if (id >= 1 && id <= n) h_red[id - 1] = -1;
+  int64_t nf = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (h_red[i] >= 0) h_red[i] = (int32_t)nf++;
+  DevBuf<int32_t> red;
+  PF_CHECK(red.ensure((size_t)n + 1));
+  PF_HIP(hipMemcpyAsync(red.p, h_red.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  Handle H;
+  PF_RB(rocblas_create_handle(&H.h));
+  PF_RB(rocblas_set_stream(H.h, st));
+  PF_HIP(hipEventRecord(ctx->ev0, st));
+  const double avg_row = n > 0 ? (double)nnz / (double)n : 1.0;
+
+  if (nev <= 0) {  // ---------------------------------------------------- every eigenpair
+    if (nf > 16384) {
+      pf_set_error("eig: %lld free DOFs is too many for the dense solve (at most 16384); ask for nev pairs",
+                   (long long)nf);
+      return PF_INVALID;
+    }
+    *count = (int32_t)nf;
+    if (nf == 0) return PF_OK;
+    DevBuf<double> Da, Db, W, E;
+    DevBuf<rocblas_int> info;
+    PF_CHECK(Da.ensure((size_t)(nf * nf)));
+    PF_CHECK(Db.ensure((size_t)(nf * nf)));
+    PF_CHECK(W.ensure((size_t)nf));
+    PF_CHECK(E.ensure((size_t)nf));
+    PF_CHECK(info.ensure(1));
+    PF_HIP(hipMemsetAsync(Da.p, 0, sizeof(double) * nf * nf, st));
+    PF_HIP(hipMemsetAsync(Db.p, 0, sizeof(double) * nf * nf, st));
+    k_dense_scatter<<<pf_blocks(n, 256, INT32_MAX), 256, 0, st>>>(n, row_ptr, col_idx, A, red.p, nf, Da.p);
+    k_dense_scatter<<<pf_blocks(n, 256, INT32_MAX), 256, 0, st>>>(n, row_ptr, col_idx, B, red.p, nf, Db.p);
+    PF_HIP(hipGetLastError());
+    PF_RB(rocsolver_dsygvd(H.h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)nf, Da.p,
+                           (rocblas_int)nf, Db.p, (rocblas_int)nf, W.p, E.p, info.p));
+    rocblas_int hinfo = 0;
+    PF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+    if (values) PF_HIP(hipMemcpyAsync(values, W.p, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+    if (vectors) {
+      DevBuf<double> out;
+      PF_CHECK(out.ensure((size_t)(n * nf)));
+      k_expand<<<grid_for(n * nf), 256, 0, st>>>(n, nf, nf, red.p, Da.p, out.p);
+      PF_HIP(hipGetLastError());
+      PF_HIP(hipMemcpyAsync(vectors, out.p, sizeof(double) * n * nf, hipMemcpyDeviceToHost, st));
+      PF_HIP(hipStreamSynchronize(st));
+    }
+    PF_HIP(hipStreamSynchronize(st));
+    if (hinfo != 0) {
+      pf_set_error("eig: dsygvd info %d (B not positive definite on the free DOFs, or no convergence)", (int)hinfo);
+      res.status = PF_BREAKDOWN_C2C3;
+      return res.status;
+    }
+    res.status = PF_OK;
+    res.nconv = (int32_t)nf;
+    PF_HIP(hipEventRecord(ctx->ev1, st));
+    PF_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    PF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    res.ms = ms;
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------------------- LOBPCG
+  const int64_t m = nev;
+  if (m > nf || 3 * m > 3000) {
+    pf_set_error("eig: nev = %d must be at most the free DOF count (%lld) and 1000", nev, (long long)nf);
+    return PF_INVALID;
+  }
+  *count = nev;
+  const int64_t K = 3 * m;
+  DevBuf<double> S, AS, BS, T, R, dinv, GA, GB, th, E, sc;
+  DevBuf<rocblas_int> info;
+  PF_CHECK(S.ensure((size_t)(n * K)));
+  PF_CHECK(AS.ensure((size_t)(n * K)));
+  PF_CHECK(BS.ensure((size_t)(n * K)));
+  PF_CHECK(T.ensure((size_t)(n * m * 3)));
+  PF_CHECK(R.ensure((size_t)(n * m)));
+  PF_CHECK(dinv.ensure((size_t)n));
+  PF_CHECK(GA.ensure((size_t)(K * K)));
+  PF_CHECK(GB.ensure((size_t)(K * K)));
+  PF_CHECK(th.ensure((size_t)K));
+  PF_CHECK(E.ensure((size_t)K));
+  PF_CHECK(sc.ensure((size_t)K));
+  PF_CHECK(info.ensure(1));
+  double *X = S.p, *Wb = S.p + n * m, *P = S.p + 2 * n * m;
+  double *AX = AS.p, *AW = AS.p + n * m, *AP = AS.p + 2 * n * m;
+  double *BX = BS.p, *BW = BS.p + n * m, *BP = BS.p + 2 * n * m;
+  k_dinv<<<pf_blocks(n, 256, INT32_MAX), 256, 0, st>>>(n, row_ptr, col_idx, A, red.p, dinv.p);
+  k_init_block<<<grid_for(n * m), 256, 0, st>>>(n, m, red.p, X);
+  PF_HIP(hipGetLastError());
+  auto spmv_block = [&](const double *M, const double *in, double *out, int64_t cols) -> int {
+    for (int64_t c = 0; c < cols; ++c)
+      PF_CHECK(pf_spmv_crs_launch<double>(ctx, n, row_ptr, col_idx, M, in + c * n, out + c * n, avg_row));
+    return PF_OK;
+  };
+  // Rayleigh-Ritz on the first k columns of S; keeps the lowest m in X (and, with k > m, the
+  // W/P part of the combination in P).  Returns 1 when dsygvd fails.
+  std::vector<double> h_th((size_t)K);
+  auto rayleigh_ritz = [&](int64_t k, int *failed) -> int {
+    const double one = 1.0, zero = 0.0;
+    PF_RB(rocblas_dgemm(H.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)k, (rocblas_int)k,
+                        (rocblas_int)n, &one, S.p, (rocblas_int)n, AS.p, (rocblas_int)n, &zero, GA.p, (rocblas_int)k));
+    PF_RB(rocblas_dgemm(H.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)k, (rocblas_int)k,
+                        (rocblas_int)n, &one, S.p, (rocblas_int)n, BS.p, (rocblas_int)n, &zero, GB.p, (rocblas_int)k));
+    PF_RB(rocsolver_dsygvd(H.h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)k, GA.p,
+                           (rocblas_int)k, GB.p, (rocblas_int)k, th.p, E.p, info.p));
+    rocblas_int hinfo = 0;
+    PF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+    PF_HIP(hipMemcpyAsync(h_th.data(), th.p, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+    PF_HIP(hipStreamSynchronize(st));
+    *failed = hinfo != 0;
+    if (*failed) return PF_OK;
+    // T = [S C | AS C | BS C] for the lowest m vectors C = GA[:, 0:m]
+    double *Tx = T.p, *Ta = T.p + n * m, *Tb = T.p + 2 * n * m;
+    PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
+                        (rocblas_int)k, &one, S.p, (rocblas_int)n, GA.p, (rocblas_int)k, &zero, Tx, (rocblas_int)n));
+    PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
+                        (rocblas_int)k, &one, AS.p, (rocblas_int)n, GA.p, (rocblas_int)k, &zero, Ta, (rocblas_int)n));
+    PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
+                        (rocblas_int)k, &one, BS.p, (rocblas_int)n, GA.p, (rocblas_int)k, &zero, Tb, (rocblas_int)n));
+    if (k > m) {  // P = [W P] C[m:k, :] (written into the P slot after X is replaced)
+      PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
+                          (rocblas_int)(k - m), &one, S.p + n * m, (rocblas_int)n, GA.p + m, (rocblas_int)k, &zero, R.p,
+                          (rocblas_int)n));
+      k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, R.p, P);
+      PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
+                          (rocblas_int)(k - m), &one, AS.p + n * m, (rocblas_int)n, GA.p + m, (rocblas_int)k, &zero,
+                          R.p, (rocblas_int)n));
+      k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, R.p, AP);
+      PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
+                          (rocblas_int)(k - m), &one, BS.p + n * m, (rocblas_int)n, GA.p + m, (rocblas_int)k, &zero,
+                          R.p, (rocblas_int)n));
+      k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, R.p, BP);
+    }
+    k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, Tx, X);
+    k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, Ta, AX);
+    k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, Tb, BX);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+  };
+
+  PF_CHECK(spmv_block(A, X, AX, m));
+  PF_CHECK(spmv_block(B, X, BX, m));
+  int failed = 0;
+  PF_CHECK(rayleigh_ritz(m, &failed));
+  if (failed) {
+    pf_set_error("eig: B is not positive definite on the start block");
+    res.status = PF_BREAKDOWN_C2C3;
+    return res.status;
+  }
+  bool have_p = false;
+  std::vector<double> rn((size_t)m), bn((size_t)m), wn((size_t)(2 * m));
+  int it = 0;
+  res.status = PF_NOT_CONVERGED;
+  for (it = 1; it <= maxit; ++it) {
+    PF_HIP(hipMemcpyAsync(th.p, h_th.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
+    k_residual_block<<<grid_for(n * m), 256, 0, st>>>(n, m, AX, BX, th.p, dinv.p, red.p, R.p, Wb);
+    PF_HIP(hipGetLastError());
+    for (int64_t c = 0; c < m; ++c) {
+      PF_RB(rocblas_dnrm2(H.h, (rocblas_int)n, R.p + c * n, 1, &rn[c]));
+      PF_RB(rocblas_dnrm2(H.h, (rocblas_int)n, BX + c * n, 1, &bn[c]));
+    }
+    // |A x - l B x| relative to the block's largest |l| |B x| (rigid-body modes have l = 0)
+    double lmax = 0.0, worst = 0.0;
+    for (int64_t c = 0; c < m; ++c) lmax = std::max(lmax, std::fabs(h_th[c]));
+    for (int64_t c = 0; c < m; ++c) worst = std::max(worst, rn[c] / std::max(lmax * bn[c], 1e-300));
+    res.max_res = worst;
+    if (worst < tol) {
+      res.status = PF_OK;
+      break;
+    }
+    PF_CHECK(spmv_block(A, Wb, AW, m));
+    PF_CHECK(spmv_block(B, Wb, BW, m));
+    // unit B-norm columns of W (and P)
+    const int64_t ncols = have_p ? 2 * m : m;
+    for (int64_t c = 0; c < ncols; ++c)
+      PF_RB(rocblas_ddot(H.h, (rocblas_int)n, S.p + (m + c) * n, 1, BS.p + (m + c) * n, 1, &wn[c]));
+    std::vector<double> f((size_t)ncols);
+    for (int64_t c = 0; c < ncols; ++c) f[c] = wn[c] > 0.0 ? 1.0 / std::sqrt(wn[c]) : 0.0;
+    PF_HIP(hipMemcpyAsync(sc.p, f.data(), sizeof(double) * ncols, hipMemcpyHostToDevice, st));
+    k_scale_cols<<<grid_for(n * ncols), 256, 0, st>>>(n, m, ncols, sc.p, S.p, AS.p, BS.p);
+    PF_HIP(hipGetLastError());
+    PF_CHECK(rayleigh_ritz(m + ncols, &failed));
+    if (failed && have_p) PF_CHECK(rayleigh_ritz(2 * m, &failed));  // projected B lost definiteness: drop P
+    if (failed) {
+      pf_set_error("eig: LOBPCG Rayleigh-Ritz failed (projected B not positive definite) at iteration %d", it);
+      res.status = PF_BREAKDOWN_C2C3;
+      break;
+    }
+    have_p = true;
+  }
+  res.iterations = std::min(it, maxit);
+  res.nconv = res.status == PF_OK ? nev : 0;
+  if (values) std::memcpy(values, h_th.data(), sizeof(double) * m);
+  if (vectors) PF_HIP(hipMemcpyAsync(vectors, X, sizeof(double) * n * m, hipMemcpyDeviceToHost, st));
+  PF_HIP(hipEventRecord(ctx->ev1, st));
+  PF_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  PF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  res.ms = ms;
+  if (res.status == PF_NOT_CONVERGED)
+    pf_set_error("eig: LOBPCG not converged after %d iterations (worst relative residual %.3e)", res.iterations,
+                 res.max_res);
+  return PF_OK;
+}

@@ -142,6 +142,12 @@ int pf_hex8_mass_batch(pf_ctx *ctx, const NodeGraph &g, double rho, const double
 // rhs += sum of element mass vectors (rho per element from rho_dev, else rho_all), ascending element order
 int pf_body_force(pf_ctx *ctx, const NodeGraph &g, const double *rho_dev, double rho_all, const double *accel,
                   double *rhs);
+// MassMatrixFEMDomain (DOF=3) of elements first..first+count-1 (24x24 column-major each) and
+// MassMatrix_as_CRS into the DOF=3 CRS values; rho per element (srho = 1) or one value
+int pf_hex8_mass_mat(pf_ctx *ctx, const NodeGraph &g, const double *rho, int srho, int64_t first, int64_t count,
+                     double *out);
+int pf_assemble_mass(pf_ctx *ctx, const NodeGraph &g, const double *rho, int srho, const int64_t *row_ptr,
+                     double *val);
 // cell-averaged Cauchy stress per element, out[e*9 + 3j + i]; sE / snu: 1 per element, 0 one value
 int pf_cell_stress(pf_ctx *ctx, const NodeGraph &g, const double *disp, const double *E, int sE, const double *nu,
                    int snu, int jmode, double *out);
@@ -244,6 +250,16 @@ int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, 
 int pf_work_ilu(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                 const double *val, int64_t nnz, bool matrix_changed, IluFactor **out);
 void pf_work_ilu_reset(SolverWork *w);
+
+// eig.hip: eigFEMSolver; fixed_ids 1-based (removed DOFs), nev <= 0: every pair (dense),
+// else the nev lowest (LOBPCG).  values (count), vectors (n x count, column-major) on the host.
+struct EigResult {
+  int status = 0, iterations = 0, nconv = 0;
+  double max_res = 0.0, ms = 0.0;
+};
+int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *A,
+                  const double *B, int64_t nnz, const std::vector<int32_t> &fixed_ids, int32_t nev, double tol,
+                  int32_t maxit, double *values, double *vectors, int32_t *count, EigResult &res);
 
 // batch.hip: many independent meshes of one block-diagonal system, one persistent kernel
 struct BatchWork;

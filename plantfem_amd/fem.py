@@ -218,6 +218,23 @@ class FEMDomain:
                                             L.ptr(out[q], C.c_double)), "MassVector")
         return out[0] if np.ndim(ElementID) == 0 else out
 
+    def MassMatrix(self, ElementID, Density=1.0, DOF: int = 3) -> np.ndarray:
+        """MassMatrixFEMDomain (FEMDomainClass.f90:10874-10945): rho sum_gp N N^T det J, a 24x24
+        matrix (one element id) or (count, 24, 24) for an array of ids; Density one value or one
+        per element of the mesh."""
+        if DOF != 3:
+            raise ValueError("DOF=3 only on this path")
+        xyz, en = self._flat()
+        ids = np.atleast_1d(np.asarray(ElementID, dtype=np.int64))
+        rho = _f64(np.atleast_1d(Density))
+        out = np.empty((ids.size, 576))
+        for q, e in enumerate(ids):
+            L.check(L.load().pf_hex8_mass_matrix(self._ctx().h, self.nn(), self.ne(), L.ptr(xyz, C.c_double),
+                                                 L.ptr(en, C.c_int32), L.ptr(rho, C.c_double), rho.size, int(e), 1,
+                                                 L.ptr(out[q], C.c_double)), "MassMatrix")
+        M = out.reshape(ids.size, 24, 24).transpose(0, 2, 1)  # column-major storage
+        return M[0] if np.ndim(ElementID) == 0 else M
+
     def StressMatrices(self, displacement, E, v, jmode: int = 0) -> np.ndarray:
         """Cell-averaged Cauchy stress of every element, (ne, 3, 3) (StressMatrixFEMDomain without
         GaussPoint, FEMDomainClass.f90:11912-12032)."""
@@ -470,6 +487,59 @@ class FEMSolver:
         L.check(code, "solve (Bicgstab devide by zero)" if code in (1, 2) else "solve", ok=ok)
         return x
 
+
+    def setMassMatrix(self, Density=1.0):
+        """MassMatrix_as_CRS(Density(:), DOF=3) assembled into this solver's CRS values
+        (FEMDomainClass.f90:10364-10467); Density one value or one per element."""
+        self._flush()
+        if np.ndim(Density) == 0:
+            L.check(L.load().pf_system_mass_matrix(self.h, None, float(Density)), "setMassMatrix")
+        else:
+            L.check(L.load().pf_system_mass_matrix(self.h, L.ptr(_f64(Density), C.c_double), 0.0), "setMassMatrix")
+        if self.n == 0:
+            self.DOF = 3
+            self.n = 3 * self.domain.nn()
+            nnz = C.c_int64(0)
+            L.check(L.load().pf_system_sizes(self.h, None, C.byref(nnz), None, None), "sizes")
+            self.nnz = nnz.value
+
+    def zeros(self):
+        """zerosFEMSolver: CRS values and RHS set to 0."""
+        self._flush()
+        L.check(L.load().pf_system_zeros(self.h), "zeros")
+
+    def keepThisMatrixAs(self, As: str):
+        """keepThisMatrixAs: keep a named copy of the CRS values ("A" stiffness, "B" mass for eig)."""
+        self._flush()
+        L.check(L.load().pf_system_keep_matrix(self.h, As.encode()), "keepThisMatrixAs")
+
+    def fix_eig(self, IDs):
+        """fix_eig: DOFs (1-based) held at zero amplitude in eig."""
+        ids = _i32(IDs)
+        L.check(L.load().pf_system_fix_eig(self.h, ids.size, L.ptr(ids, C.c_int32)), "fix_eig")
+
+    def eig(self, num_eigen: int | None = None, tol: float = 1e-8, maxit: int = 10000):
+        """eigFEMSolver: (eigen_value, eigen_vectors) of A u = lambda B u (kept "A", "B"), the
+        fix_eig DOFs removed.  num_eigen None: every pair (dense, as the reference's LAPACK
+        path); an integer: the num_eigen smallest by LOBPCG.  eigen_vectors is (n, count),
+        B-orthonormal, zero at the removed DOFs."""
+        lib = L.load()
+        if num_eigen is None:
+            c = C.c_int32(0)
+            L.check(lib.pf_system_eig_count(self.h, C.byref(c)), "eig")
+            cnt, nev = c.value, 0
+        else:
+            cnt, nev = int(num_eigen), int(num_eigen)
+        vals = np.empty(max(cnt, 1))
+        vecs = np.empty((max(cnt, 1), self.n))  # row l = eigenvector l (column-major n x count)
+        got = C.c_int32(0)
+        st = L.Stats()
+        code = lib.pf_system_eig(self.h, nev, float(tol), int(maxit), L.ptr(vals, C.c_double),
+                                 L.ptr(vecs, C.c_double), C.byref(got), C.byref(st))
+        self.last_stats = st.as_dict()
+        L.check(code, "eig", ok=(L.PF_OK, L.PF_NOT_CONVERGED))
+        k = got.value
+        return vals[:k].copy(), np.ascontiguousarray(vecs[:k].T)
 
     def iluFactor(self):
         """(factor values on the CRS pattern, number of schedule levels) of the ILU(0) factor the
