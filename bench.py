@@ -22,6 +22,8 @@ Two further workloads print their own line (not the default):
                                          nodes: forward + adjoint gradient time, gradient checks
     python bench.py --workload precond   60^3 box with row-sorted columns: unpreconditioned,
                                          PointJacobi and ILU(0) BiCGSTAB (solve(preconditioning=))
+    python bench.py --workload modal     mass matrix + eig: the 10 lowest modes of a 20x20x100 bar
+                                         (LOBPCG) and every mode of a 10^3 box (dense)
 """
 from __future__ import annotations
 
@@ -67,7 +69,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-graph", action="store_true", help="direct kernel launches only (for profilers)")
-    ap.add_argument("--workload", choices=("box", "batch", "adjoint", "precond"), default="box")
+    ap.add_argument("--workload", choices=("box", "batch", "adjoint", "precond", "modal"), default="box")
     ap.add_argument("--precond-size", type=int, default=60)
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-itrmax", type=int, default=3000,
@@ -93,6 +95,8 @@ def main(argv=None):
         return bench_adjoint(a)
     if a.workload == "precond":
         return bench_precond(a)
+    if a.workload == "modal":
+        return bench_modal(a)
 
     from plantfem_amd import _lib as L
     from plantfem_amd import workloads as W
@@ -410,6 +414,77 @@ def bench_precond(a):
         out["cpu_baseline"] = {"value": round(dt * ilu_iters, 2), "unit": "s", "cores": threads, "kind": "port",
                                "sample": f"{k} bicgstab_CRS_ILU iterations (oracle: OpenMP row SpMV, sequential "
                                          f"ILU_matvec sweeps) timed, x {ilu_iters} GPU iterations; factor excluded"}
+    print(json.dumps(out), flush=True)
+
+
+def bench_modal(a):
+    """Modal analysis as Tutorial/fem/ModalAnalysis_FixBoundary.f90 runs it: mass matrix kept as
+    "B", stiffness as "A", top and bottom faces removed, eig."""
+    from plantfem_amd import _lib as L
+    from plantfem_amd.fem import FEMDomain, FEMSolver, cube3d
+    ctx = L.default_context()
+
+    def system(dims):
+        nod, el = cube3d(*dims, 1.0 * dims[0], 1.0 * dims[1], 1.0 * dims[2])
+        ne = el.shape[0]
+        s = FEMSolver(ctx)
+        s.setDomain(FEMDomain(nod, el, ctx))
+        t = time.perf_counter()
+        s.setMassMatrix(np.full(ne, 2.7))
+        s.keepThisMatrixAs("B")
+        s.zeros()
+        s.setStiffness(np.full(ne, 70000.0), np.full(ne, 0.3))
+        s.keepThisMatrixAs("A")
+        ctx.synchronize()
+        t_asm = time.perf_counter() - t
+        z = nod[:, 2]
+        nodes = np.nonzero((z == z.max()) | (z == z.min()))[0] + 1
+        s.fix_eig(np.concatenate([3 * nodes - 2, 3 * nodes - 1, 3 * nodes]).astype(np.int32))
+        return s, t_asm
+
+    out = {"metric": "modal analysis: time to the lowest modes (LOBPCG) and to every mode (dense)", "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "data": "synthetic", "dtype": "f64",
+           "config": {"workload": "Cube3D bar 20x20x100 (unit elements), E 7e4, nu 0.3, rho 2.7, top and bottom "
+                                  "faces fixed, 10 lowest modes; and a 10^3 box, every mode"}}
+    s, t_asm = system((20, 20, 100))
+    s.eig(num_eigen=10)  # warm
+    t = time.perf_counter()
+    lam, V = s.eig(num_eigen=10)
+    t_lobpcg = time.perf_counter() - t
+    st = s.last_stats
+    out["lobpcg"] = {"n_dof": int(s.n), "assembly_s": round(t_asm, 4), "eig_s": round(t_lobpcg, 4),
+                     "iterations": st["iterations"], "status": st["status"], "worst_rel_residual": st["true_relres"],
+                     "eigenvalues": [float(v) for v in lam]}
+    log(f"modal lobpcg: {json.dumps(out['lobpcg'])}")
+    s.close()
+    s, _ = system((10, 10, 10))
+    s.eig()
+    t = time.perf_counter()
+    lam, V = s.eig()
+    t_dense = time.perf_counter() - t
+    out["dense"] = {"n_free": int(lam.size), "eig_s": round(t_dense, 4), "lowest": [float(v) for v in lam[:5]]}
+    log(f"modal dense: {json.dumps(out['dense'])}")
+    if not a.no_cpu:  # the reference's path: LAPACK (dsygvd via scipy) on the host, same pencil
+        import scipy.linalg as sl
+        crs = s.getCRS()  # "A" is the current CRS
+        n = crs.n
+        Ad = np.zeros((n, n))
+        rows = np.repeat(np.arange(n), np.diff(crs.row_ptr))
+        Ad[rows, crs.col_idx - 1] = crs.val
+        s.setMassMatrix(2.7)
+        mb = s.getCRS()
+        Bd = np.zeros((n, n))
+        Bd[rows, mb.col_idx - 1] = mb.val
+        keep = np.abs(V).sum(axis=1) > 0  # the free DOFs (eigenvectors vanish at the removed ones)
+        t = time.perf_counter()
+        ref = sl.eigh(Ad[np.ix_(keep, keep)], Bd[np.ix_(keep, keep)], eigvals_only=False)
+        t_cpu = time.perf_counter() - t
+        out["dense"]["max_rel_eigenvalue_diff_vs_lapack"] = float(np.max(np.abs(ref[0] - lam) / np.abs(ref[0])))
+        out["cpu_baseline"] = {"value": round(t_cpu, 4), "unit": "s", "cores": os.cpu_count(), "kind": "port",
+                               "sample": f"LAPACK dsygvd (scipy.linalg.eigh) on the same {int(keep.sum())}-DOF "
+                                         "reduced pencil, all pairs with vectors (the reference's default eig "
+                                         "calls LAPACK DSPGVD)"}
+    out["value"] = out["lobpcg"]["eig_s"]
     print(json.dumps(out), flush=True)
 
 

@@ -243,7 +243,7 @@ int pf_system_fix_eig(pf_system *sys, int64_t nids, const int32_t *ids);
 /* eigFEMSolver (FEMSolverClass.f90:1384-1541): A u = lambda B u with the kept "A" and "B", the
  * fix_eig DOFs removed.  nev <= 0: every eigenpair of the reduced problem, as the reference's
  * default LAPACK DSPGVD path (dense on the device, at most 16384 free DOFs); nev > 0: the nev
- * smallest by LOBPCG (its use_LOBPCG option; tol on max_i |A x_i - l_i B x_i| / (max_j |l_j| |B x_i|),
+ * smallest by LOBPCG (its use_LOBPCG option; nev <= 32; tol on max_i |A x_i - l_i B x_i| / (max_j |l_j| |B x_i|),
  * default 1e-8 for tol <= 0; maxit default 10000).  eigen_values (ascending) and eigen_vectors
  * (n x count column-major, B-orthonormal, zero at the removed DOFs) are host arrays and may be
  * NULL; *count receives the number of pairs (pf_system_eig_count gives it for nev <= 0).

@@ -8,8 +8,10 @@
 //    eigenvectors come back B-orthonormal, eigenvalues ascending, as DSPGVD returns them.
 //  * the nev lowest pairs (the reference's use_LOBPCG option): block LOBPCG in the full DOF
 //    space with the fixed rows held at zero, the CRS SpMV for A and B, a Jacobi (diag A)
-//    preconditioner, Rayleigh-Ritz on [X W P] with rocBLAS Gram products and rocSOLVER dsygvd
-//    on the (up to 3 nev)^2 projected pencil; W and P are scaled to unit B-norm first.
+//    preconditioner, Rayleigh-Ritz on [X W P] with rocBLAS Gram products; the projected
+//    pencil (at most 3 nev square, nev <= 64) is solved on the host by Cholesky of the B-Gram
+//    matrix and cyclic Jacobi rotations (a few microseconds; a device solver for matrices
+//    this small costs milliseconds of launches); W and P are scaled to unit B-norm first.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -111,6 +113,156 @@ __global__ void k_copy(int64_t count, const double *__restrict__ a, double *__re
     b[t] = a[t];
 }
 
+// out[c] = sum_i X[i, c] Y[i, c], one block per column
+__global__ void __launch_bounds__(1024) k_coldots(int64_t n, const double *__restrict__ X,
+                                                  const double *__restrict__ Y, double *__restrict__ out) {
+  __shared__ double sh[16];
+  const int64_t c = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += X[c * n + i] * Y[c * n + i];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += sh[w];
+    out[c] = t;
+  }
+}
+
+// partial Gram matrices G = S^T T over row tiles (S, T column-major n x k, k <= GK): 32 rows
+// of both in LDS per step, thread p accumulates pairs p, p + 1024, ...; part[block][j*k + i]
+constexpr int GK = 96, GR = 32, GPER = (GK * GK + 1023) / 1024;
+__global__ void __launch_bounds__(1024) k_gram(int64_t n, int k, const double *__restrict__ S,
+                                               const double *__restrict__ T, double *__restrict__ part) {
+  __shared__ double sS[GR][GK];
+  __shared__ double sT[GR][GK];
+  const int npair = k * k;
+  double acc[GPER];
+#pragma unroll
+  for (int q = 0; q < GPER; ++q) acc[q] = 0.0;
+  for (int64_t r0 = (int64_t)blockIdx.x * GR; r0 < n; r0 += (int64_t)gridDim.x * GR) {
+    for (int e = threadIdx.x; e < GR * k; e += 1024) {
+      const int rr = e % GR, cc = e / GR;
+      const int64_t r = r0 + rr;
+      sS[rr][cc] = r < n ? S[(int64_t)cc * n + r] : 0.0;
+      sT[rr][cc] = r < n ? T[(int64_t)cc * n + r] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < GPER; ++q) {
+      const int p = threadIdx.x + q * 1024;
+      if (p < npair) {
+        const int i = p % k, j = p / k;
+        double t = 0.0;
+#pragma unroll 8
+        for (int rr = 0; rr < GR; ++rr) t += sS[rr][i] * sT[rr][j];
+        acc[q] += t;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < GPER; ++q) {
+    const int p = threadIdx.x + q * 1024;
+    if (p < npair) part[(int64_t)blockIdx.x * npair + p] = acc[q];
+  }
+}
+
+__global__ void k_gram_sum(int npair, int nblocks, const double *__restrict__ part, double *__restrict__ G) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npair) return;
+  double t = 0.0;
+  for (int b = 0; b < nblocks; ++b) t += part[(int64_t)b * npair + p];
+  G[p] = t;
+}
+
+// symmetric-definite pencil GA z = theta GB z (k x k, column-major, upper parts used) on the
+// host: GB = L L^T, C = L^-1 GA L^-T, cyclic Jacobi on C, z = L^-T y; ascending theta,
+// Z column-major k x k.  Returns false when GB is not positive definite.
+bool small_pencil(int k, const std::vector<double> &GA, const std::vector<double> &GB, std::vector<double> &theta,
+                  std::vector<double> &Z) {
+  std::vector<double> Lm((size_t)k * k, 0.0), C((size_t)k * k), V((size_t)k * k, 0.0);
+  auto a = [&](int i, int j) { return GA[(size_t)std::max(i, j) * k + std::min(i, j)]; };  // upper of column-major
+  auto b = [&](int i, int j) { return GB[(size_t)std::max(i, j) * k + std::min(i, j)]; };
+  for (int j = 0; j < k; ++j) {  // Cholesky, L row-major Lm[i*k + j]
+    double d = b(j, j);
+    for (int q = 0; q < j; ++q) d -= Lm[(size_t)j * k + q] * Lm[(size_t)j * k + q];
+    if (!(d > 0.0)) return false;
+    const double l = std::sqrt(d);
+    Lm[(size_t)j * k + j] = l;
+    for (int i = j + 1; i < k; ++i) {
+      double t = b(i, j);
+      for (int q = 0; q < j; ++q) t -= Lm[(size_t)i * k + q] * Lm[(size_t)j * k + q];
+      Lm[(size_t)i * k + j] = t / l;
+    }
+  }
+  // T = L^-1 GA (forward solves on the columns), C = T L^-T
+  std::vector<double> T((size_t)k * k);
+  for (int c = 0; c < k; ++c)
+    for (int i = 0; i < k; ++i) {
+      double t = a(i, c);
+      for (int q = 0; q < i; ++q) t -= Lm[(size_t)i * k + q] * T[(size_t)q * k + c];
+      T[(size_t)i * k + c] = t / Lm[(size_t)i * k + i];
+    }
+  for (int r = 0; r < k; ++r)
+    for (int j = 0; j < k; ++j) {
+      double t = T[(size_t)r * k + j];
+      for (int q = 0; q < j; ++q) t -= C[(size_t)r * k + q] * Lm[(size_t)j * k + q];
+      C[(size_t)r * k + j] = t / Lm[(size_t)j * k + j];
+    }
+  for (int i = 0; i < k; ++i)
+    for (int j = i + 1; j < k; ++j) C[(size_t)i * k + j] = C[(size_t)j * k + i] = 0.5 * (C[(size_t)i * k + j] + C[(size_t)j * k + i]);
+  for (int i = 0; i < k; ++i) V[(size_t)i * k + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        tot += C[(size_t)i * k + j] * C[(size_t)i * k + j];
+        if (i != j) off += C[(size_t)i * k + j] * C[(size_t)i * k + j];
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < k - 1; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        const double apq = C[(size_t)p * k + q];
+        if (apq == 0.0) continue;
+        const double tau = (C[(size_t)q * k + q] - C[(size_t)p * k + p]) / (2.0 * apq);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+        for (int r = 0; r < k; ++r) {  // columns p, q
+          const double xp = C[(size_t)r * k + p], xq = C[(size_t)r * k + q];
+          C[(size_t)r * k + p] = cs * xp - sn * xq;
+          C[(size_t)r * k + q] = sn * xp + cs * xq;
+        }
+        for (int r = 0; r < k; ++r) {  // rows p, q
+          const double xp = C[(size_t)p * k + r], xq = C[(size_t)q * k + r];
+          C[(size_t)p * k + r] = cs * xp - sn * xq;
+          C[(size_t)q * k + r] = sn * xp + cs * xq;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double vp = V[(size_t)r * k + p], vq = V[(size_t)r * k + q];
+          V[(size_t)r * k + p] = cs * vp - sn * vq;
+          V[(size_t)r * k + q] = sn * vp + cs * vq;
+        }
+      }
+  }
+  std::vector<int> ord(k);
+  for (int i = 0; i < k; ++i) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return C[(size_t)x * k + x] < C[(size_t)y * k + y]; });
+  theta.assign(k, 0.0);
+  Z.assign((size_t)k * k, 0.0);
+  for (int c = 0; c < k; ++c) {
+    const int o = ord[c];
+    theta[c] = C[(size_t)o * k + o];
+    for (int i = k - 1; i >= 0; --i) {  // z = L^-T y (back substitution)
+      double t = V[(size_t)i * k + o];
+      for (int q = i + 1; q < k; ++q) t -= Lm[(size_t)q * k + i] * Z[(size_t)c * k + q];
+      Z[(size_t)c * k + i] = t / Lm[(size_t)i * k + i];
+    }
+  }
+  return true;
+}
+
 struct Handle {
   rocblas_handle h = nullptr;
   ~Handle() {
@@ -194,8 +346,8 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
 
   // ---------------------------------------------------------------------------- LOBPCG
   const int64_t m = nev;
-  if (m > nf || 3 * m > 3000) {
-    pf_set_error("eig: nev = %d must be at most the free DOF count (%lld) and 1000", nev, (long long)nf);
+  if (m > nf || 3 * m > GK) {
+    pf_set_error("eig: nev = %d must be at most the free DOF count (%lld) and %d", nev, (long long)nf, GK / 3);
     return PF_INVALID;
   }
   *count = nev;
@@ -214,6 +366,9 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   PF_CHECK(E.ensure((size_t)K));
   PF_CHECK(sc.ensure((size_t)K));
   PF_CHECK(info.ensure(1));
+  const unsigned gram_blocks = (unsigned)std::min<int64_t>((n + GR - 1) / GR, (int64_t)ctx->num_cus * 2);
+  DevBuf<double> gpart;
+  PF_CHECK(gpart.ensure((size_t)gram_blocks * K * K + 1));
   double *X = S.p, *Wb = S.p + n * m, *P = S.p + 2 * n * m;
   double *AX = AS.p, *AW = AS.p + n * m, *AP = AS.p + 2 * n * m;
   double *BX = BS.p, *BW = BS.p + n * m, *BP = BS.p + 2 * n * m;
@@ -230,18 +385,21 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   std::vector<double> h_th((size_t)K);
   auto rayleigh_ritz = [&](int64_t k, int *failed) -> int {
     const double one = 1.0, zero = 0.0;
-    PF_RB(rocblas_dgemm(H.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)k, (rocblas_int)k,
-                        (rocblas_int)n, &one, S.p, (rocblas_int)n, AS.p, (rocblas_int)n, &zero, GA.p, (rocblas_int)k));
-    PF_RB(rocblas_dgemm(H.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)k, (rocblas_int)k,
-                        (rocblas_int)n, &one, S.p, (rocblas_int)n, BS.p, (rocblas_int)n, &zero, GB.p, (rocblas_int)k));
-    PF_RB(rocsolver_dsygvd(H.h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)k, GA.p,
-                           (rocblas_int)k, GB.p, (rocblas_int)k, th.p, E.p, info.p));
-    rocblas_int hinfo = 0;
-    PF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, st));
-    PF_HIP(hipMemcpyAsync(h_th.data(), th.p, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+    // Gram matrices S^T (A S) and S^T (B S) (a transposed tall-skinny product, which the
+    // library GEMM runs on a handful of workgroups: 21 ms for k = 30, n = 134k; this: one pass)
+    k_gram<<<gram_blocks, 1024, 0, st>>>(n, (int)k, S.p, AS.p, gpart.p);
+    k_gram_sum<<<(unsigned)((k * k + 255) / 256), 256, 0, st>>>((int)(k * k), (int)gram_blocks, gpart.p, GA.p);
+    k_gram<<<gram_blocks, 1024, 0, st>>>(n, (int)k, S.p, BS.p, gpart.p);
+    k_gram_sum<<<(unsigned)((k * k + 255) / 256), 256, 0, st>>>((int)(k * k), (int)gram_blocks, gpart.p, GB.p);
+    PF_HIP(hipGetLastError());
+    std::vector<double> hA((size_t)k * k), hB((size_t)k * k), hth, hZ;
+    PF_HIP(hipMemcpyAsync(hA.data(), GA.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, st));
+    PF_HIP(hipMemcpyAsync(hB.data(), GB.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, st));
     PF_HIP(hipStreamSynchronize(st));
-    *failed = hinfo != 0;
+    *failed = !small_pencil((int)k, hA, hB, hth, hZ);
     if (*failed) return PF_OK;
+    for (int64_t c = 0; c < k; ++c) h_th[c] = hth[c];
+    PF_HIP(hipMemcpyAsync(GA.p, hZ.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, st));
     // T = [S C | AS C | BS C] for the lowest m vectors C = GA[:, 0:m]
     double *Tx = T.p, *Ta = T.p + n * m, *Tb = T.p + 2 * n * m;
     PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
@@ -288,32 +446,61 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
     PF_HIP(hipMemcpyAsync(th.p, h_th.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
     k_residual_block<<<grid_for(n * m), 256, 0, st>>>(n, m, AX, BX, th.p, dinv.p, red.p, R.p, Wb);
     PF_HIP(hipGetLastError());
+    k_coldots<<<(unsigned)m, 1024, 0, st>>>(n, R.p, R.p, sc.p);
+    k_coldots<<<(unsigned)m, 1024, 0, st>>>(n, BX, BX, sc.p + m);
+    PF_HIP(hipGetLastError());
+    std::vector<double> nrm((size_t)(2 * m));
+    PF_HIP(hipMemcpyAsync(nrm.data(), sc.p, sizeof(double) * 2 * m, hipMemcpyDeviceToHost, st));
+    PF_HIP(hipStreamSynchronize(st));
     for (int64_t c = 0; c < m; ++c) {
-      PF_RB(rocblas_dnrm2(H.h, (rocblas_int)n, R.p + c * n, 1, &rn[c]));
-      PF_RB(rocblas_dnrm2(H.h, (rocblas_int)n, BX + c * n, 1, &bn[c]));
+      rn[c] = std::sqrt(nrm[c]);
+      bn[c] = std::sqrt(nrm[m + c]);
     }
-    // |A x - l B x| relative to the block's largest |l| |B x| (rigid-body modes have l = 0)
+    // |A x - l B x| relative to the block's largest |l| |B x| (rigid-body modes have l = 0);
+    // pairs below tol are soft-locked: their W and P columns leave the search space, which
+    // keeps the projected B definite once some pairs have converged
     double lmax = 0.0, worst = 0.0;
     for (int64_t c = 0; c < m; ++c) lmax = std::max(lmax, std::fabs(h_th[c]));
-    for (int64_t c = 0; c < m; ++c) worst = std::max(worst, rn[c] / std::max(lmax * bn[c], 1e-300));
+    std::vector<int64_t> act;
+    for (int64_t c = 0; c < m; ++c) {
+      const double rc = rn[c] / std::max(lmax * bn[c], 1e-300);
+      worst = std::max(worst, rc);
+      if (!(rc < tol)) act.push_back(c);
+    }
     res.max_res = worst;
-    if (worst < tol) {
+    if (act.empty()) {
       res.status = PF_OK;
       break;
     }
-    PF_CHECK(spmv_block(A, Wb, AW, m));
-    PF_CHECK(spmv_block(B, Wb, BW, m));
+    const int64_t na = (int64_t)act.size();
+    // compact [X | W_act | P_act]: every copy moves a column to a lower index, in order
+    for (int64_t q = 0; q < na; ++q)
+      if (act[q] != q) k_copy<<<grid_for(n), 256, 0, st>>>(n, Wb + act[q] * n, Wb + q * n);
+    if (have_p)
+      for (int64_t q = 0; q < na; ++q) {
+        const int64_t src = 2 * m + act[q], dst = m + na + q;
+        if (src != dst) {
+          k_copy<<<grid_for(n), 256, 0, st>>>(n, S.p + src * n, S.p + dst * n);
+          k_copy<<<grid_for(n), 256, 0, st>>>(n, AS.p + src * n, AS.p + dst * n);
+          k_copy<<<grid_for(n), 256, 0, st>>>(n, BS.p + src * n, BS.p + dst * n);
+        }
+      }
+    PF_HIP(hipGetLastError());
+    PF_CHECK(spmv_block(A, Wb, AW, na));
+    PF_CHECK(spmv_block(B, Wb, BW, na));
     // unit B-norm columns of W (and P)
-    const int64_t ncols = have_p ? 2 * m : m;
-    for (int64_t c = 0; c < ncols; ++c)
-      PF_RB(rocblas_ddot(H.h, (rocblas_int)n, S.p + (m + c) * n, 1, BS.p + (m + c) * n, 1, &wn[c]));
+    const int64_t ncols = have_p ? 2 * na : na;
+    k_coldots<<<(unsigned)ncols, 1024, 0, st>>>(n, S.p + m * n, BS.p + m * n, sc.p);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipMemcpyAsync(wn.data(), sc.p, sizeof(double) * ncols, hipMemcpyDeviceToHost, st));
+    PF_HIP(hipStreamSynchronize(st));
     std::vector<double> f((size_t)ncols);
     for (int64_t c = 0; c < ncols; ++c) f[c] = wn[c] > 0.0 ? 1.0 / std::sqrt(wn[c]) : 0.0;
     PF_HIP(hipMemcpyAsync(sc.p, f.data(), sizeof(double) * ncols, hipMemcpyHostToDevice, st));
     k_scale_cols<<<grid_for(n * ncols), 256, 0, st>>>(n, m, ncols, sc.p, S.p, AS.p, BS.p);
     PF_HIP(hipGetLastError());
     PF_CHECK(rayleigh_ritz(m + ncols, &failed));
-    if (failed && have_p) PF_CHECK(rayleigh_ritz(2 * m, &failed));  // projected B lost definiteness: drop P
+    if (failed && have_p) PF_CHECK(rayleigh_ritz(m + na, &failed));  // projected B lost definiteness: drop P
     if (failed) {
       pf_set_error("eig: LOBPCG Rayleigh-Ritz failed (projected B not positive definite) at iteration %d", it);
       res.status = PF_BREAKDOWN_C2C3;

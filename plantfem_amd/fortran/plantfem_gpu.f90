@@ -195,6 +195,45 @@ module plantfem_gpu
          type(c_ptr), value :: x0, x_out
          type(pf_stats), intent(out) :: stats
       end function
+      integer(c_int) function c_pf_hex8_mass_matrix(ctx, nn, ne, nodcoord, elemnod, density, ndensity, first, &
+                                                    count, me) bind(C, name="pf_hex8_mass_matrix")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: nn, ne, ndensity, first, count
+         real(c_double), intent(in) :: nodcoord(*), density(*)
+         integer(c_int32_t), intent(in) :: elemnod(*)
+         real(c_double), intent(out) :: me(*)
+      end function
+      integer(c_int) function c_pf_system_zeros(sys) bind(C, name="pf_system_zeros")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: sys
+      end function
+      integer(c_int) function c_pf_system_keep_matrix(sys, name) bind(C, name="pf_system_keep_matrix")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: sys
+         character(kind=c_char), intent(in) :: name(*)
+      end function
+      integer(c_int) function c_pf_system_fix_eig(sys, nids, ids) bind(C, name="pf_system_fix_eig")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t
+         type(c_ptr), value :: sys
+         integer(c_int64_t), value :: nids
+         integer(c_int32_t), intent(in) :: ids(*)
+      end function
+      integer(c_int) function c_pf_system_eig_count(sys, count) bind(C, name="pf_system_eig_count")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: sys
+         integer(c_int32_t), intent(out) :: count
+      end function
+      integer(c_int) function c_pf_system_eig(sys, nev, tol, maxit, values, vectors, count, stats) &
+         bind(C, name="pf_system_eig")
+         import :: c_int, c_ptr, c_int32_t, c_double, pf_stats
+         type(c_ptr), value :: sys
+         integer(c_int32_t), value :: nev, maxit
+         real(c_double), value :: tol
+         real(c_double), intent(out) :: values(*), vectors(*)
+         integer(c_int32_t), intent(out) :: count
+         type(pf_stats), intent(out) :: stats
+      end function
       function c_strlen(s) bind(C, name="strlen")
          import :: c_ptr, c_size_t
          type(c_ptr), value :: s
@@ -230,6 +269,7 @@ module plantfem_gpu
       procedure, pass :: StiffnessMatrixElementFEMDomain
       procedure, pass :: StiffnessMatrixCRSFEMDomain
       generic :: StiffnessMatrix => StiffnessMatrixElementFEMDomain, StiffnessMatrixCRSFEMDomain
+      procedure :: MassMatrix => MassMatrixFEMDomain
    end type
 
    type :: FEMSolver_
@@ -268,6 +308,10 @@ module plantfem_gpu
       procedure :: solve => solveFEMSolver
       procedure :: flush => flushFEMSolver
       procedure :: remove => removeFEMSolver
+      procedure :: zeros => zerosFEMSolver
+      procedure :: keepThisMatrixAs => keepThisMatrixAsFEMSolver
+      procedure :: fix_eig => fix_eigFEMSolver
+      procedure :: eig => eigFEMSolver
    end type
 
    type(c_ptr), save :: g_ctx = c_null_ptr
@@ -482,6 +526,30 @@ contains
       call check(code, "StiffnessMatrix(ElementID,E,v)")
    end function
 
+   ! MassMatrixFEMDomain(ElementID, Density, DOF) (FEMDomainClass.f90:10874-10945), DOF = 3
+   function MassMatrixFEMDomain(this, ElementID, Density, DOF) result(MassMatrix)
+      class(FEMDomain_), intent(in) :: this
+      integer(int32), intent(in) :: ElementID
+      real(real64), optional, intent(in) :: Density
+      integer(int32), optional, intent(in) :: DOF
+      real(real64), allocatable :: MassMatrix(:, :)
+      real(real64) :: rho(1)
+      integer(c_int) :: code
+      if (present(DOF)) then
+         if (DOF /= 3) then
+            print *, "plantfem_gpu :: MassMatrix :: DOF=3 only"
+            error stop
+         end if
+      end if
+      rho(1) = 1.0d0
+      if (present(Density)) rho(1) = Density
+      allocate (MassMatrix(24, 24))
+      code = c_pf_hex8_mass_matrix(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), this%mesh%NodCoord, &
+                                   this%mesh%ElemNod, rho, 1_c_int64_t, int(ElementID, c_int64_t), 1_c_int64_t, &
+                                   MassMatrix)
+      call check(code, "MassMatrix(ElementID,Density,DOF)")
+   end function
+
    function StiffnessMatrixCRSFEMDomain(this, YoungModulus, PoissonRatio) result(StiffnessMatrix)
       class(FEMDomain_), intent(in) :: this
       real(real64), intent(in) :: YoungModulus(:), PoissonRatio(:)
@@ -690,6 +758,53 @@ contains
       this%CRS_x = xl
       call move_alloc(xl, x)
    end function
+
+   ! zerosFEMSolver (FEMSolverClass.f90:1310-1322)
+   subroutine zerosFEMSolver(this)
+      class(FEMSolver_), intent(inout) :: this
+      call this%flush()
+      call check(c_pf_system_zeros(this%sys), "zeros")
+   end subroutine
+
+   ! keepThisMatrixAsFEMSolver (FEMSolverClass.f90:1544-1604)
+   subroutine keepThisMatrixAsFEMSolver(this, As)
+      class(FEMSolver_), intent(inout) :: this
+      character(*), intent(in) :: As
+      call this%flush()
+      call check(c_pf_system_keep_matrix(this%sys, trim(As)//c_null_char), "keepThisMatrixAs")
+   end subroutine
+
+   ! fix_eigFEMSolver (FEMSolverClass.f90:1607-1628)
+   subroutine fix_eigFEMSolver(this, IDs)
+      class(FEMSolver_), intent(inout) :: this
+      integer(int32), intent(in) :: IDs(:)
+      call check(c_pf_system_fix_eig(this%sys, int(size(IDs), c_int64_t), IDs), "fix_eig")
+   end subroutine
+
+   ! eigFEMSolver (FEMSolverClass.f90:1384-1541): every eigenpair of the kept "A", "B" with the
+   ! fix_eig DOFs removed, or the num_eigen smallest (LOBPCG); eigen_vectors(n, count)
+   subroutine eigFEMSolver(this, num_eigen, eigen_value, eigen_vectors)
+      class(FEMSolver_), intent(inout) :: this
+      integer(int32), optional, intent(in) :: num_eigen
+      real(real64), allocatable, intent(inout) :: eigen_value(:)
+      real(real64), allocatable, intent(inout) :: eigen_vectors(:, :)
+      integer(c_int32_t) :: cnt, nev, got
+      integer(c_int) :: code
+      nev = 0
+      if (present(num_eigen)) nev = num_eigen
+      if (nev > 0) then
+         cnt = nev
+      else
+         call check(c_pf_system_eig_count(this%sys, cnt), "eig")
+      end if
+      if (allocated(eigen_value)) deallocate (eigen_value)
+      if (allocated(eigen_vectors)) deallocate (eigen_vectors)
+      allocate (eigen_value(max(cnt, 1)), eigen_vectors(this%n, max(cnt, 1)))
+      code = c_pf_system_eig(this%sys, nev, 1.0d-8, 10000_c_int32_t, eigen_value, eigen_vectors, got, this%stats)
+      if (code /= PF_NOT_CONVERGED) call check(code, "eig")
+      eigen_value = eigen_value(1:got)
+      eigen_vectors = eigen_vectors(:, 1:got)
+   end subroutine
 
    ! getCRS: host copies of the device CRS and RHS into the public CRS_* components
    subroutine getCRSFEMSolver(this)
