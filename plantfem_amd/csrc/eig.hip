@@ -366,7 +366,8 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   PF_CHECK(E.ensure((size_t)K));
   PF_CHECK(sc.ensure((size_t)K));
   PF_CHECK(info.ensure(1));
-  const unsigned gram_blocks = (unsigned)std::min<int64_t>((n + GR - 1) / GR, (int64_t)ctx->num_cus * 2);
+  // few partial Gram matrices: their sum (k_gram_sum, one thread per entry) reads them all
+  const unsigned gram_blocks = (unsigned)std::min<int64_t>((n + GR - 1) / GR, (int64_t)std::max(ctx->num_cus / 2, 1));
   DevBuf<double> gpart;
   PF_CHECK(gpart.ensure((size_t)gram_blocks * K * K + 1));
   double *X = S.p, *Wb = S.p + n * m, *P = S.p + 2 * n * m;
