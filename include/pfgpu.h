@@ -232,6 +232,38 @@ int pf_system_mass_matrix(pf_system *sys, const double *density, double density_
 /* zerosFEMSolver (FEMSolverClass.f90:1310-1322): CRS values and RHS set to zero. */
 int pf_system_zeros(pf_system *sys);
 
+/* ---- several domains coupled by overset penalties (Soybean%getDisplacement) -------- */
+/* FEMDomain%overset(FEMDomain, DomainID, algorithm=FEMDomain_Overset_GPP, MyDomainID)
+ * (FEMDomainClass.f90:14606-14700): the Gauss points of mesh A (the own domain) that lie in
+ * elements of mesh B (Mesh%nearestElementID, MeshClass.f90:8611-8866).  Meshes as
+ * pf_system_set_mesh (hex8, nodcoord nn x 3 and elemnod ne x 8 column-major, 1-based).
+ * Connects come in the reference's order (own elements ascending among those whose node
+ * range meets B's node box, Gauss points 1..8): own element, Gauss point, element of B (all
+ * 1-based) and the Gauss point's position (3 doubles).  At most cap are written; *count is
+ * the number found (call again with a larger cap when it exceeds cap). */
+int pf_overset_gpp(pf_ctx *ctx, int64_t nnA, int64_t neA, const double *nodcoordA, const int32_t *elemnodA,
+                   int64_t nnB, int64_t neB, const double *nodcoordB, const int32_t *elemnodB, int64_t cap,
+                   int64_t *count, int32_t *elem, int32_t *gauss_point, int32_t *pair_elem, double *position);
+
+/* setCRS(DOF) with several domains (FEMSolverClass.f90:466-676) on the mesh given to
+ * pf_system_set_mesh, which is the domains' meshes concatenated (node and element ids of
+ * domain d offset by the sizes of domains 1..d-1; CRS_ID_Starts_From(d) = 1 + dof * nodes
+ * before d).  The pattern is the union of the ZeroMatrix blocks and of every node pair of
+ * the ncon connects (elem, pair_elem: element ids of the concatenated mesh, 1-based), each
+ * row's columns ascending; values zero.  any_overset = 0 (no domain was overset) gives the
+ * reference's empty CRS (every row empty).  pf_system_stiffness then assembles into this
+ * pattern.  Returns nnz. */
+int pf_system_multi_crs(pf_system *sys, int32_t dof, int32_t any_overset, int64_t ncon, const int32_t *elem,
+                        const int32_t *pair_elem, int64_t *nnz);
+
+/* setEbOM(penalty, DOF) (FEMSolverClass.f90:1891-1985): for each connect in order, adds
+ * penalty * ConnectMatrix (FEMDomainClass.f90:13014-13100) over the 16 nodes of the own and
+ * the paired element: A = (Bc^T Bc) detJ, Bc = [N_own(gp) I, -N_pair(xi) I], detJ of the own
+ * Gauss point, xi the local coordinate of the position in the paired element.  Element ids of
+ * the concatenated mesh (1-based), Gauss points 1..8, positions 3 per connect.  DOF = 3. */
+int pf_system_ebom(pf_system *sys, int64_t ncon, const int32_t *elem, const int32_t *gauss_point,
+                   const int32_t *pair_elem, const double *position, double penalty, int32_t dof);
+
 /* keepThisMatrixAs(As) (FEMSolverClass.f90:1544-1604): a named copy of the current CRS values
  * (the eigen solver reads "A" = stiffness and "B" = mass). */
 int pf_system_keep_matrix(pf_system *sys, const char *name);

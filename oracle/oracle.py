@@ -40,6 +40,14 @@ _SIG = {
     "orc_ilu0": (None, [I64, P64, P32, PD]),
     "orc_ilu_apply": (None, [I64, P64, P32, PD, PD, PD]),
     "orc_bicgstab_ilu": (I32, [I64, P64, P32, PD, PD, PD, PD, I32, D, D, I32, P32, PD, PD]),
+    "orc_gauss_point_position": (None, [PD, I32, PD]),
+    "orc_inside_hex8": (I32, [PD, PD]),
+    "orc_nearest_element": (I64, [I64, I64, PD, P32, PD]),
+    "orc_overset_gpp": (I64, [I64, I64, PD, P32, I64, I64, PD, P32, P32, P32, P32, PD, I64]),
+    "orc_local_coordinate": (None, [PD, PD, PD]),
+    "orc_connect_coeffs": (None, [PD, I32, PD, PD, PD, PD]),
+    "orc_multi_crs": (I64, [I64, I64, P32, I32, I64, P32, P64, P32]),
+    "orc_ebom": (I32, [I64, P64, P32, PD, I64, P32, PD, PD, D]),
 }
 
 
@@ -309,3 +317,100 @@ def build_system(w):
     fixed = np.zeros(3 * w.nn, dtype=np.uint8)
     fixed[w.fixed_dofs - 1] = 1
     return rp, ci, val, rhs, fixed, np.zeros(3 * w.nn)
+
+
+# ---- multi-domain assembly with overset penalties ----------------------------------------
+def gauss_point_position(X, gp):
+    """GlobalPositionOfGaussPoint of Gauss point gp (1-based) of an element with nodes X (8,3)."""
+    out = np.empty(3)
+    load().orc_gauss_point_position(_p(_c(X, np.float64).ravel(), C.c_double), int(gp) - 1, _p(out, C.c_double))
+    return out
+
+
+def inside_hex8(X, p):
+    return bool(load().orc_inside_hex8(_p(_c(X, np.float64).ravel(), C.c_double),
+                                       _p(_c(p, np.float64), C.c_double)))
+
+
+def nearest_element(nodcoord, elemnod, p):
+    """Mesh%nearestElementID: 1-based element id or -1."""
+    xyz = _fl(nodcoord, np.float64)
+    en = _fl(elemnod, np.int32)
+    e = load().orc_nearest_element(nodcoord.shape[0], elemnod.shape[0], _p(xyz, C.c_double), _p(en, C.c_int32),
+                                   _p(_c(p, np.float64), C.c_double))
+    return int(e) + 1 if e >= 0 else -1
+
+
+def overset_gpp(nodA, enA, nodB, enB):
+    """FEMDomain%overset(GPP) of mesh A onto mesh B: (elem, gp, pair_elem, position), 1-based ids."""
+    xa, ea, xb, eb = _fl(nodA, np.float64), _fl(enA, np.int32), _fl(nodB, np.float64), _fl(enB, np.int32)
+    args = (nodA.shape[0], enA.shape[0], _p(xa, C.c_double), _p(ea, C.c_int32), nodB.shape[0], enB.shape[0],
+            _p(xb, C.c_double), _p(eb, C.c_int32))
+    cap = enA.shape[0] * 8
+    me, gp, pe = (np.empty(cap, np.int32) for _ in range(3))
+    pos = np.empty(cap * 3)
+    k = load().orc_overset_gpp(*args, _p(me, C.c_int32), _p(gp, C.c_int32), _p(pe, C.c_int32),
+                               _p(pos, C.c_double), cap)
+    return me[:k].copy(), gp[:k].copy(), pe[:k].copy(), pos[:3 * k].reshape(k, 3).copy()
+
+
+def local_coordinate(X, p):
+    xi = np.empty(3)
+    load().orc_local_coordinate(_p(_c(X, np.float64).ravel(), C.c_double), _p(_c(p, np.float64), C.c_double),
+                                _p(xi, C.c_double))
+    return xi
+
+
+def _elem_X(nodcoord, elemnod, e):
+    return np.asarray(nodcoord, dtype=np.float64)[np.asarray(elemnod)[e - 1] - 1]
+
+
+def connect_coeffs(nodcoord, elemnod, elem, gp, pair_elem, position):
+    """Per connect: c16 (16) = [N_own(gp), -N_pair(xi)] and detJ of the own Gauss point."""
+    k = len(elem)
+    c16 = np.empty((k, 16))
+    dj = np.empty(k)
+    for c in range(k):
+        d = np.empty(1)
+        load().orc_connect_coeffs(_p(_c(_elem_X(nodcoord, elemnod, elem[c]), np.float64).ravel(), C.c_double),
+                                  int(gp[c]),
+                                  _p(_c(_elem_X(nodcoord, elemnod, pair_elem[c]), np.float64).ravel(), C.c_double),
+                                  _p(_c(position[c], np.float64), C.c_double), _p(c16[c], C.c_double),
+                                  _p(d, C.c_double))
+        dj[c] = d[0]
+    return c16, dj
+
+
+def connect_nodes(elemnod, elem, pair_elem):
+    en = np.asarray(elemnod, dtype=np.int32)
+    return np.concatenate([en[np.asarray(elem) - 1], en[np.asarray(pair_elem) - 1]], axis=1).astype(np.int32)
+
+
+def multi_crs(nn, elemnod, any_overset, elem, pair_elem):
+    """Multi-domain setCRS(DOF=3) pattern of the concatenated mesh: (row_ptr, col_idx)."""
+    en = _fl(elemnod, np.int32)
+    nodes = _c(connect_nodes(elemnod, elem, pair_elem), np.int32).ravel() if len(elem) else np.zeros(1, np.int32)
+    ne = elemnod.shape[0]
+    rp = np.empty(3 * nn + 1, np.int64)
+    nnz = load().orc_multi_crs(nn, ne, _p(en, C.c_int32), int(bool(any_overset)), len(elem),
+                               _p(nodes, C.c_int32), _p(rp, C.c_int64), None)
+    ci = np.empty(max(nnz, 1), np.int32)
+    load().orc_multi_crs(nn, ne, _p(en, C.c_int32), int(bool(any_overset)), len(elem), _p(nodes, C.c_int32),
+                         _p(rp, C.c_int64), _p(ci, C.c_int32))
+    return rp, ci[:nnz].copy()
+
+
+def ebom(nodcoord, elemnod, row_ptr, col_idx, val, elem, gp, pair_elem, position, penalty):
+    """setEbOM(penalty, DOF=3) added into a copy of val."""
+    val = np.array(val, dtype=np.float64)
+    k = len(elem)
+    if k == 0:
+        return val
+    c16, dj = connect_coeffs(nodcoord, elemnod, elem, gp, pair_elem, position)
+    nodes = _c(connect_nodes(elemnod, elem, pair_elem), np.int32).ravel()
+    st = load().orc_ebom(row_ptr.size - 1, _p(_c(row_ptr, np.int64), C.c_int64),
+                         _p(_c(col_idx, np.int32), C.c_int32), _p(val, C.c_double), k, _p(nodes, C.c_int32),
+                         _p(_c(c16, np.float64).ravel(), C.c_double), _p(dj, C.c_double), float(penalty))
+    if st != 0:
+        raise ValueError("connect entry outside the pattern")
+    return val

@@ -892,3 +892,265 @@ int orc_mass_matrix_crs(int64_t nn, int64_t ne, const double *nodcoord, const in
     }
     return ORC_OK;
 }
+
+/* ---- multi-domain assembly with overset penalty coupling ---------------------------------
+ * FEMDomain%overset (GPP, FEMDomainClass.f90:14606-14700), Mesh%nearestElementID
+ * (MeshClass.f90:8611-8665), InsideOfElement (hex8 branch, :8672-8866), getLocalCoordinate
+ * (FEMDomainClass.f90:10046-10100), ConnectMatrix (GPP, :13014-13100), the multi-domain setCRS
+ * (FEMSolverClass.f90:466-676) and setEbOM (:1891-1985). */
+
+static void elem_X(int64_t nn, int64_t ne, const double *xyz, const int32_t *en, int64_t e, double *X)
+{
+    for (int a = 0; a < 8; ++a)
+        for (int d = 0; d < 3; ++d) X[a * 3 + d] = xyz[(int64_t)d * nn + en[(int64_t)a * ne + e] - 1];
+}
+
+/* getGlobalPositionOfGaussPoint: matmul(transpose(ElemCoord), Nmat), a summed in order */
+void orc_gauss_point_position(const double *X, int gp, double *pos)
+{
+    for (int d = 0; d < 3; ++d) {
+        double s = 0.0;
+        for (int a = 0; a < 8; ++a) s = s + X[a * 3 + d] * hex8_N(a, GPT[gp]);
+        pos[d] = s;
+    }
+}
+
+/* InsideOfElementMesh, hex8: outside when (p1 x p2) . (x - X_n0) > 0 for any of six faces */
+int orc_inside_hex8(const double *X, const double *p)
+{
+    static const int T[6][3] = {{1, 4, 2}, {1, 2, 5}, {1, 5, 4}, {3, 7, 2}, {7, 8, 6}, {3, 4, 7}};
+    for (int t = 0; t < 6; ++t) {
+        const double *x0 = X + 3 * (T[t][0] - 1), *x1 = X + 3 * (T[t][1] - 1), *x2 = X + 3 * (T[t][2] - 1);
+        const double a0 = x1[0] - x0[0], a1 = x1[1] - x0[1], a2 = x1[2] - x0[2];
+        const double b0 = x2[0] - x0[0], b1 = x2[1] - x0[1], b2 = x2[2] - x0[2];
+        const double o0 = p[0] - x0[0], o1 = p[1] - x0[1], o2 = p[2] - x0[2];
+        const double n0 = a1 * b2 - a2 * b1, n1 = a2 * b0 - a0 * b2, n2 = a0 * b1 - a1 * b0;
+        if ((n0 * o0 + n1 * o1) + n2 * o2 > 0.0) return 0;
+    }
+    return 1;
+}
+
+/* NearestElementIDMesh: -1 outside the nodes' bounding box (inclusive); else the nearest node
+ * (first strict minimum of the squared distance, node 1 first) and the first element (ascending
+ * id) containing that node whose InsideOfElement holds; -1 when none.  0-based result. */
+int64_t orc_nearest_element(int64_t nn, int64_t ne, const double *xyz, const int32_t *en, const double *p)
+{
+    for (int d = 0; d < 3; ++d) {
+        double lo = xyz[(int64_t)d * nn], hi = lo;
+        for (int64_t i = 1; i < nn; ++i) {
+            const double v = xyz[(int64_t)d * nn + i];
+            if (v < lo) lo = v;
+            if (v > hi) hi = v;
+        }
+        if (!(lo <= p[d] && p[d] <= hi)) return -1;
+    }
+    int64_t best = 0;
+    double dx = p[0] - xyz[0], dy = p[1] - xyz[nn], dz = p[2] - xyz[2 * nn];
+    double cur = (dx * dx + dy * dy) + dz * dz;
+    for (int64_t i = 0; i < nn; ++i) {
+        dx = p[0] - xyz[i];
+        dy = p[1] - xyz[nn + i];
+        dz = p[2] - xyz[2 * nn + i];
+        const double t = (dx * dx + dy * dy) + dz * dz;
+        if (t < cur) {
+            best = i;
+            cur = t;
+        }
+    }
+    for (int64_t e = 0; e < ne; ++e) {
+        int has = 0;
+        for (int a = 0; a < 8; ++a) has |= en[(int64_t)a * ne + e] == best + 1;
+        if (!has) continue;
+        double X[24];
+        elem_X(nn, ne, xyz, en, e, X);
+        if (orc_inside_hex8(X, p)) return e;
+    }
+    return -1;
+}
+
+/* overset(FEMDomain, algorithm=GPP) of mesh A onto mesh B: for every element of A whose node
+ * range meets B's bounding box (getElementList xmin..zmax: max >= min_B and min <= max_B per
+ * axis), ascending, and every Gauss point, the element of B containing the point.  Writes
+ * connects (1-based element ids, 1-based Gauss point) up to cap; returns their number. */
+int64_t orc_overset_gpp(int64_t nnA, int64_t neA, const double *xyzA, const int32_t *enA, int64_t nnB, int64_t neB,
+                        const double *xyzB, const int32_t *enB, int32_t *my_elem, int32_t *gp_out,
+                        int32_t *pair_elem, double *pos_out, int64_t cap)
+{
+    double bmin[3], bmax[3];
+    for (int d = 0; d < 3; ++d) {
+        bmin[d] = bmax[d] = xyzB[(int64_t)d * nnB];
+        for (int64_t i = 1; i < nnB; ++i) {
+            const double v = xyzB[(int64_t)d * nnB + i];
+            if (v < bmin[d]) bmin[d] = v;
+            if (v > bmax[d]) bmax[d] = v;
+        }
+    }
+    int64_t count = 0;
+    for (int64_t e = 0; e < neA; ++e) {
+        double X[24];
+        elem_X(nnA, neA, xyzA, enA, e, X);
+        int hit = 1;
+        for (int d = 0; d < 3; ++d) {
+            double lo = X[d], hi = X[d];
+            for (int a = 1; a < 8; ++a) {
+                if (X[a * 3 + d] < lo) lo = X[a * 3 + d];
+                if (X[a * 3 + d] > hi) hi = X[a * 3 + d];
+            }
+            hit &= (hi >= bmin[d]) && (lo <= bmax[d]);
+        }
+        if (!hit) continue;
+        for (int g = 0; g < 8; ++g) {
+            double p[3];
+            orc_gauss_point_position(X, g, p);
+            const int64_t q = orc_nearest_element(nnB, neB, xyzB, enB, p);
+            if (q < 0) continue;
+            if (count < cap) {
+                my_elem[count] = (int32_t)(e + 1);
+                gp_out[count] = g + 1;
+                pair_elem[count] = (int32_t)(q + 1);
+                for (int d = 0; d < 3; ++d) pos_out[count * 3 + d] = p[d];
+            }
+            ++count;
+        }
+    }
+    return count;
+}
+
+/* getLocalCoordinate (hex8): xi = inverse(Jbar) (x - center), Jbar = 0.125 sum_gp J(gp),
+ * center = 0.125 sum_a X_a; inverse as MathClass inverse (cofactors / det_mat) */
+void orc_local_coordinate(const double *X, const double *p, double *xi)
+{
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int g = 0; g < 8; ++g) {
+        double dN[3][8];
+        hex8_dN(GPT[g], dN);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int a = 0; a < 8; ++a) s = s + dN[i][a] * X[a * 3 + j];
+                J[i][j] = J[i][j] + s;
+            }
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) J[i][j] = 0.125 * J[i][j];
+    double c[3] = {0, 0, 0};
+    for (int a = 0; a < 8; ++a)
+        for (int d = 0; d < 3; ++d) c[d] = c[d] + X[a * 3 + d];
+    double x[3];
+    for (int d = 0; d < 3; ++d) x[d] = p[d] - 0.125 * c[d];
+    const double det = det3(J), r = 1.0 / det;
+    double Ji[3][3];
+    Ji[0][0] = r * (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+    Ji[0][1] = r * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+    Ji[0][2] = r * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+    Ji[1][0] = r * (J[1][2] * J[2][0] - J[1][0] * J[2][2]);
+    Ji[1][1] = r * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+    Ji[1][2] = r * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
+    Ji[2][0] = r * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    Ji[2][1] = r * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+    Ji[2][2] = r * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+    for (int i = 0; i < 3; ++i) xi[i] = (Ji[i][0] * x[0] + Ji[i][1] * x[1]) + Ji[i][2] * x[2];
+}
+
+/* ConnectMatrix (GPP) as coefficients: c = [N_mine(gp) (8), -N_pair(xi) (8)] and det J of the
+ * Gauss point; the 48x48 matrix is c_P c_Q det J on equal DOF components, 0 elsewhere */
+void orc_connect_coeffs(const double *Xmy, int gp /*1-based*/, const double *Xpair, const double *pos, double *c16,
+                        double *detJ)
+{
+    *detJ = hex8_gp_G(Xmy, gp - 1, 0, NULL);
+    /* Bc entries start from zero: 0 + N_own, 0 - N_pair */
+    for (int a = 0; a < 8; ++a) c16[a] = 0.0 + hex8_N(a, GPT[gp - 1]);
+    double xi[3];
+    orc_local_coordinate(Xpair, pos, xi);
+    for (int a = 0; a < 8; ++a) c16[8 + a] = 0.0 - hex8_N(a, xi);
+}
+
+/* multi-domain setCRS pattern: the union of the domains' ZeroMatrix(DOF=3) rows (the union
+ * mesh is block diagonal) and every node pair of every connect, columns ascending; all rows
+ * empty when no overset was set up (any_overset = 0).  nodes16: per connect the 16 union node
+ * ids (1-based).  Returns nnz (arrays written when col_idx != NULL, row_ptr n+1). */
+int64_t orc_multi_crs(int64_t nn, int64_t ne, const int32_t *en, int any_overset, int64_t ncon, const int32_t *nodes16,
+                      int64_t *row_ptr, int32_t *col_idx)
+{
+    const int64_t n = 3 * nn;
+    if (!any_overset) {
+        for (int64_t i = 0; i <= n; ++i) row_ptr[i] = 1;
+        return 0;
+    }
+    /* node neighbour sets as sorted arrays (small: dense per-node bitmap of the union would
+     * be nn^2, so a simple growing list with linear dedupe) */
+    int64_t *cnt = (int64_t *)calloc((size_t)nn, sizeof(int64_t));
+    int64_t *capn = (int64_t *)calloc((size_t)nn, sizeof(int64_t));
+    int32_t **lst = (int32_t **)calloc((size_t)nn, sizeof(int32_t *));
+#define ADD_PAIR(I, J)                                                                    \
+    do {                                                                                  \
+        int64_t i_ = (I), j_ = (J);                                                       \
+        int f_ = 0;                                                                       \
+        for (int64_t q_ = 0; q_ < cnt[i_]; ++q_)                                          \
+            if (lst[i_][q_] == (int32_t)j_) { f_ = 1; break; }                            \
+        if (!f_) {                                                                        \
+            if (cnt[i_] == capn[i_]) {                                                    \
+                capn[i_] = capn[i_] ? 2 * capn[i_] : 32;                                  \
+                lst[i_] = (int32_t *)realloc(lst[i_], sizeof(int32_t) * (size_t)capn[i_]); \
+            }                                                                             \
+            lst[i_][cnt[i_]++] = (int32_t)j_;                                             \
+        }                                                                                 \
+    } while (0)
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) ADD_PAIR(en[(int64_t)a * ne + e] - 1, en[(int64_t)b * ne + e] - 1);
+    for (int64_t c = 0; c < ncon; ++c)
+        for (int j = 0; j < 16; ++j)
+            for (int k = 0; k < 16; ++k) ADD_PAIR(nodes16[c * 16 + j] - 1, nodes16[c * 16 + k] - 1);
+#undef ADD_PAIR
+    int64_t nnz = 0;
+    row_ptr[0] = 1;
+    for (int64_t i = 0; i < nn; ++i) {
+        /* ascending node ids (insertion sort) */
+        for (int64_t q = 1; q < cnt[i]; ++q) {
+            int32_t v = lst[i][q];
+            int64_t r = q - 1;
+            while (r >= 0 && lst[i][r] > v) { lst[i][r + 1] = lst[i][r]; --r; }
+            lst[i][r + 1] = v;
+        }
+        for (int d = 0; d < 3; ++d) {
+            if (col_idx)
+                for (int64_t q = 0; q < cnt[i]; ++q)
+                    for (int dd = 0; dd < 3; ++dd) col_idx[nnz + 3 * q + dd] = 3 * lst[i][q] + dd + 1;
+            nnz += 3 * cnt[i];
+            row_ptr[3 * i + d + 1] = nnz + 1;
+        }
+    }
+    for (int64_t i = 0; i < nn; ++i) free(lst[i]);
+    free(lst);
+    free(cnt);
+    free(capn);
+    return nnz;
+}
+
+/* setEbOM: connects in order; for connect nodes j, k and DOF components m, n the entry
+ * (3(node_j-1)+m, 3(node_k-1)+n) gets penalty * ((Bc^T Bc)(3j+m, 3k+n) * detJ), the matmul sum
+ * over the three rows of Bc taken from zero (so +0 off the matching components) */
+int orc_ebom(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, int64_t ncon,
+             const int32_t *nodes16, const double *c16, const double *detJ, double penalty)
+{
+    (void)n;
+    for (int64_t c = 0; c < ncon; ++c)
+        for (int j = 0; j < 16; ++j)
+            for (int k = 0; k < 16; ++k)
+                for (int m = 0; m < 3; ++m)
+                    for (int nd = 0; nd < 3; ++nd) {
+                        double bt = 0.0;
+                        for (int l = 0; l < 3; ++l) {
+                            const double bp = l == m ? c16[c * 16 + j] : 0.0, bq = l == nd ? c16[c * 16 + k] : 0.0;
+                            bt = bt + bp * bq;
+                        }
+                        const int64_t row = 3 * (int64_t)(nodes16[c * 16 + j] - 1) + m;
+                        const int32_t col = 3 * (nodes16[c * 16 + k] - 1) + nd + 1;
+                        int64_t q = row_ptr[row] - 1;
+                        while (q < row_ptr[row + 1] - 1 && col_idx[q] != col) ++q;
+                        if (q >= row_ptr[row + 1] - 1) return ORC_INVALID;
+                        val[q] = val[q] + penalty * (bt * detJ[c]);
+                    }
+    return ORC_OK;
+}

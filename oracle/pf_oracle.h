@@ -134,6 +134,21 @@ void orc_hex8_mass_matrix(const double *X, double rho, double *me);
 int orc_mass_matrix_crs(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
                         const double *rho, const int64_t *row_ptr, const int32_t *col_idx, double *val);
 
+/* multi-domain assembly with overset penalties (FEMDomainClass.f90:14606-14700, :13014-13100;
+ * MeshClass.f90:8611-8866; FEMSolverClass.f90:466-676, :1891-1985) */
+void orc_gauss_point_position(const double *X, int gp, double *pos);
+int orc_inside_hex8(const double *X, const double *p);
+int64_t orc_nearest_element(int64_t nn, int64_t ne, const double *xyz, const int32_t *en, const double *p);
+int64_t orc_overset_gpp(int64_t nnA, int64_t neA, const double *xyzA, const int32_t *enA, int64_t nnB, int64_t neB,
+                        const double *xyzB, const int32_t *enB, int32_t *my_elem, int32_t *gp_out,
+                        int32_t *pair_elem, double *pos_out, int64_t cap);
+void orc_local_coordinate(const double *X, const double *p, double *xi);
+void orc_connect_coeffs(const double *Xmy, int gp, const double *Xpair, const double *pos, double *c16, double *detJ);
+int64_t orc_multi_crs(int64_t nn, int64_t ne, const int32_t *en, int any_overset, int64_t ncon, const int32_t *nodes16,
+                      int64_t *row_ptr, int32_t *col_idx);
+int orc_ebom(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, int64_t ncon,
+             const int32_t *nodes16, const double *c16, const double *detJ, double penalty);
+
 #ifdef __cplusplus
 }
 #endif

@@ -921,3 +921,429 @@ int pf_traction_boxes(pf_ctx *ctx, const NodeGraph &g, int32_t count, const int6
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
+
+// ---- overset penalty coupling (Gauss-point projection) ---------------------------------
+// FEMDomain%overset(algorithm=GPP) (FEMDomainClass.f90:14606-14700) pairs every Gauss point
+// of the own mesh with the element of the other mesh that contains it:
+//   Mesh%nearestElementID (MeshClass.f90:8611-8665): inside the other mesh's node bounding
+//   box (inclusive), take the nearest node (first strict minimum of the squared distance),
+//   then the first of its elements (ascending id) that passes InsideOfElement's six plane
+//   tests (:8746-8866).
+// setEbOM (FEMSolverClass.f90:1891-1985) adds penalty * ConnectMatrix for each connect in
+// order (FEMDomainClass.f90:13014-13100): Bc = [N_own(gp) I, -N_pair(xi) I] (3 x 48) and
+// A = (Bc^T Bc) detJ, xi the local coordinate of the Gauss point in the paired element
+// (getLocalCoordinate: inverse of the Gauss-point-averaged J applied to x - node centroid).
+namespace {
+
+__device__ inline double hex8_N_xi(int a, double x, double y, double z) {
+  const double t0 = SX[a] > 0 ? 1.0 + x : 1.0 - x;
+  const double t1 = SY[a] > 0 ? 1.0 + y : 1.0 - y;
+  const double t2 = SZ[a] > 0 ? 1.0 + z : 1.0 - z;
+  return ((0.125 * t0) * t1) * t2;
+}
+
+// InsideOfElementMesh, hex8: outside when (p1 x p2) . (x - X0) > 0 for any face triple
+__device__ inline bool inside_hex8(const double (&X)[8][3], const double *p) {
+  constexpr int T[6][3] = {{0, 3, 1}, {0, 1, 4}, {0, 4, 3}, {2, 6, 1}, {6, 7, 5}, {2, 3, 6}};
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    const double *x0 = X[T[t][0]], *x1 = X[T[t][1]], *x2 = X[T[t][2]];
+    const double a0 = x1[0] - x0[0], a1 = x1[1] - x0[1], a2 = x1[2] - x0[2];
+    const double b0 = x2[0] - x0[0], b1 = x2[1] - x0[1], b2 = x2[2] - x0[2];
+    const double o0 = p[0] - x0[0], o1 = p[1] - x0[1], o2 = p[2] - x0[2];
+    const double n0 = a1 * b2 - a2 * b1, n1 = a2 * b0 - a0 * b2, n2 = a0 * b1 - a1 * b0;
+    if ((n0 * o0 + n1 * o1) + n2 * o2 > 0.0) return false;
+  }
+  return true;
+}
+
+// node bounding box of one mesh: bb = xmin, ymin, zmin, xmax, ymax, zmax (one block)
+__global__ void __launch_bounds__(256) k_node_bbox(int64_t nn, const double *__restrict__ xyz, double *bb) {
+  __shared__ double smin[3][256], smax[3][256];
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) lo[d] = hi[d] = xyz[(int64_t)d * nn];
+  for (int64_t i = threadIdx.x; i < nn; i += 256)
+    for (int d = 0; d < 3; ++d) {
+      const double v = xyz[(int64_t)d * nn + i];
+      lo[d] = fmin(lo[d], v);
+      hi[d] = fmax(hi[d], v);
+    }
+  for (int d = 0; d < 3; ++d) {
+    smin[d][threadIdx.x] = lo[d];
+    smax[d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int d = 0; d < 3; ++d) {
+        smin[d][threadIdx.x] = fmin(smin[d][threadIdx.x], smin[d][threadIdx.x + s]);
+        smax[d][threadIdx.x] = fmax(smax[d][threadIdx.x], smax[d][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) {
+    bb[threadIdx.x] = smin[threadIdx.x][0];
+    bb[3 + threadIdx.x] = smax[threadIdx.x][0];
+  }
+}
+
+// one wavefront per (own element, Gauss point) w = 8e + gp: pair[w] = containing element of
+// the other mesh (0-based) or -1, pos[3w..] = the Gauss point's global position
+__global__ void __launch_bounds__(256) k_overset_gpp(int64_t nnA, int64_t neA, const double *__restrict__ xyzA,
+                                                     const int32_t *__restrict__ enA, int64_t nnB, int64_t neB,
+                                                     const double *__restrict__ xyzB,
+                                                     const int32_t *__restrict__ enB,
+                                                     const int64_t *__restrict__ eoffB,
+                                                     const int32_t *__restrict__ elistB,
+                                                     const double *__restrict__ bb, int32_t *__restrict__ pair,
+                                                     double *__restrict__ pos) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  if (w >= neA * 8) return;  // uniform per wavefront
+  const int64_t e = w >> 3;
+  const int gp = (int)(w & 7);
+  int32_t nodes[8];
+  double X[8][3];
+  load_element(nnA, neA, xyzA, enA, e, nodes, X);
+  // getElementList(xmin..zmax of the other mesh): element node range meets the box
+  bool hit = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double lo = X[0][d], hi = X[0][d];
+#pragma unroll
+    for (int a = 1; a < 8; ++a) {
+      lo = fmin(lo, X[a][d]);
+      hi = fmax(hi, X[a][d]);
+    }
+    hit = hit && (hi >= bb[d]) && (lo <= bb[3 + d]);
+  }
+  double p[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) s = s + X[a][d] * hex8_N(a, gp);
+    p[d] = s;
+  }
+  hit = hit && (bb[0] <= p[0] && p[0] <= bb[3]) && (bb[1] <= p[1] && p[1] <= bb[4]) &&
+        (bb[2] <= p[2] && p[2] <= bb[5]);
+  if (lane == 0) {
+    pos[3 * w] = p[0];
+    pos[3 * w + 1] = p[1];
+    pos[3 * w + 2] = p[2];
+  }
+  if (!hit) {
+    if (lane == 0) pair[w] = -1;
+    return;
+  }
+  // nearest node: each lane keeps its first strict minimum, ties go to the lower id
+  double best = INFINITY;
+  int32_t bi = INT32_MAX;
+  for (int64_t i = lane; i < nnB; i += 64) {
+    const double dx = p[0] - xyzB[i], dy = p[1] - xyzB[nnB + i], dz = p[2] - xyzB[2 * nnB + i];
+    const double t = (dx * dx + dy * dy) + dz * dz;
+    if (t < best) {
+      best = t;
+      bi = (int32_t)i;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ob = __shfl_xor(best, off, 64);
+    const int32_t oi = __shfl_xor(bi, off, 64);
+    if (ob < best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  int32_t found = -1;
+  if (bi != INT32_MAX) {
+    const int64_t s0 = eoffB[bi], s1 = eoffB[bi + 1];
+    for (int64_t base = s0; base < s1 && found < 0; base += 64) {
+      bool in = false;
+      int32_t cand = -1;
+      if (base + lane < s1) {
+        cand = elistB[base + lane];
+        int32_t nb[8];
+        double Y[8][3];
+        load_element(nnB, neB, xyzB, enB, cand, nb, Y);
+        in = inside_hex8(Y, p);
+      }
+      const unsigned long long m = __ballot(in);
+      if (m) found = __shfl(cand, __ffsll((long long)m) - 1, 64);
+    }
+  }
+  if (lane == 0) pair[w] = found;
+}
+
+__global__ void k_overset_flags(int64_t m, const int32_t *__restrict__ pair, int32_t *__restrict__ flag) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= m) return;
+  flag[w] = pair[w] >= 0 ? 1 : 0;
+}
+
+// connects in (own element, Gauss point) order, ids 1-based
+__global__ void k_overset_scatter(int64_t m, const int32_t *__restrict__ pair, const double *__restrict__ pos,
+                                  const int64_t *__restrict__ off, int32_t *__restrict__ elem,
+                                  int32_t *__restrict__ gpo, int32_t *__restrict__ pe, double *__restrict__ po) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= m || pair[w] < 0) return;
+  const int64_t k = off[w];
+  elem[k] = (int32_t)(w >> 3) + 1;
+  gpo[k] = (int32_t)(w & 7) + 1;
+  pe[k] = pair[w] + 1;
+  po[3 * k] = pos[3 * w];
+  po[3 * k + 1] = pos[3 * w + 1];
+  po[3 * k + 2] = pos[3 * w + 2];
+}
+
+// per connect: c16 = [0 + N_own(gp), 0 - N_pair(xi)], detJ of the own Gauss point, and the
+// 16 node ids (0-based) of the own and the paired element
+__global__ void __launch_bounds__(128) k_connect_coeffs(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                                                        const int32_t *__restrict__ en, int64_t ncon,
+                                                        const int32_t *__restrict__ my,
+                                                        const int32_t *__restrict__ gpid,
+                                                        const int32_t *__restrict__ pr,
+                                                        const double *__restrict__ pos, double *__restrict__ c16,
+                                                        double *__restrict__ detj, int32_t *__restrict__ nodes16) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= ncon) return;
+  int32_t na[8], nb[8];
+  double X[8][3], Y[8][3];
+  load_element(nn, ne, xyz, en, my[c], na, X);
+  load_element(nn, ne, xyz, en, pr[c], nb, Y);
+  const int gp = gpid[c] - 1;
+  double G[3][8];
+  detj[c] = hex8_gp_G(X, gp, 0, G, false);
+  // local coordinate in the paired element
+  double Jb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 1
+  for (int g = 0; g < 8; ++g) {
+    const double gx = (g == 1 || g == 2 || g == 5 || g == 6) ? GPV : -GPV;
+    const double gy = (g == 2 || g == 3 || g == 6 || g == 7) ? GPV : -GPV;
+    const double gz = (g >= 4) ? GPV : -GPV;
+    double dN[3][8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) dN_node(b, gx, gy, gz, dN[0][b], dN[1][b], dN[2][b]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) s = s + dN[i][b] * Y[b][j];
+        Jb[i][j] = Jb[i][j] + s;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Jb[i][j] = 0.125 * Jb[i][j];
+  double x[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) s = s + Y[b][d];
+    x[d] = pos[3 * c + d] - 0.125 * s;
+  }
+  double Ji[3][3];
+  inv3(Jb, Ji);
+  double xi[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) xi[i] = (Ji[i][0] * x[0] + Ji[i][1] * x[1]) + Ji[i][2] * x[2];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    c16[c * 16 + a] = 0.0 + hex8_N(a, gp);
+    c16[c * 16 + 8 + a] = 0.0 - hex8_N_xi(a, xi[0], xi[1], xi[2]);
+    nodes16[c * 16 + a] = na[a] - 1;
+    nodes16[c * 16 + 8 + a] = nb[a] - 1;
+  }
+}
+
+// node -> (connect, slot) lists: count / fill / sort (t = 16c + slot ascends with c)
+__global__ void k_con_count(int64_t total, const int32_t *__restrict__ nodes16, int32_t *__restrict__ cnt) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  atomicAdd(&cnt[nodes16[t]], 1);
+}
+
+__global__ void k_con_fill(int64_t total, const int32_t *__restrict__ nodes16, const int64_t *__restrict__ off,
+                           int32_t *__restrict__ fill, int64_t *__restrict__ list) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int32_t node = nodes16[t];
+  list[off[node] + atomicAdd(&fill[node], 1)] = t;
+}
+
+__device__ inline int64_t find_col(const int32_t *__restrict__ col_idx, int64_t rb, int64_t re, int32_t col,
+                                   bool sorted) {
+  if (sorted) {
+    int64_t lo = rb, hi = re;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (col_idx[mid] < col) lo = mid + 1;
+      else hi = mid;
+    }
+    return (lo < re && col_idx[lo] == col) ? lo : -1;
+  }
+  for (int64_t k = rb; k < re; ++k)
+    if (col_idx[k] == col) return k;
+  return -1;
+}
+
+// one thread per row r = 3i + m: the connects touching node i in order; per connect the 48
+// entries of its row of penalty * A (addMatrixValue, FEMSolverClass.f90:923-960)
+__global__ void __launch_bounds__(128) k_ebom_rows(int64_t nn, const int64_t *__restrict__ off,
+                                                   const int64_t *__restrict__ list,
+                                                   const int32_t *__restrict__ nodes16,
+                                                   const double *__restrict__ c16, const double *__restrict__ detj,
+                                                   double penalty, const int64_t *__restrict__ row_ptr,
+                                                   const int32_t *__restrict__ col_idx, bool sorted,
+                                                   double *__restrict__ val, int32_t *__restrict__ err) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= 3 * nn) return;
+  const int64_t i = r / 3;
+  const int m = (int)(r - 3 * i);
+  const int64_t rb = row_ptr[r] - 1, re = row_ptr[r + 1] - 1;
+  for (int64_t s = off[i]; s < off[i + 1]; ++s) {
+    const int64_t c = list[s] >> 4;
+    const int j = (int)(list[s] & 15);
+    const double cj = c16[c * 16 + j], dj = detj[c];
+    for (int k = 0; k < 16; ++k) {
+      const double ck = c16[c * 16 + k];
+      const int32_t nk = nodes16[c * 16 + k];
+      for (int n = 0; n < 3; ++n) {
+        // (Bc^T Bc)(3j+m, 3k+n) = sum_l Bc(l,3j+m) Bc(l,3k+n), Bc(l, 3q+d) = c_q [l == d]
+        double s3 = 0.0;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) s3 = s3 + (l == m ? cj : 0.0) * (l == n ? ck : 0.0);
+        const double v = penalty * (s3 * dj);
+        const int64_t id = find_col(col_idx, rb, re, 3 * nk + n + 1, sorted);
+        if (id < 0) {
+          atomicOr(err, 1);
+          continue;
+        }
+        val[id] = val[id] + v;
+      }
+    }
+  }
+}
+
+// hex8 stiffness into any CRS holding the mesh's node blocks: one thread per row, incident
+// elements in ascending id, every column located by search
+__global__ void __launch_bounds__(128) k_assemble_rows_search(
+    int64_t nn, int64_t ne, const double *__restrict__ xyz, const int32_t *__restrict__ en,
+    const int64_t *__restrict__ eoff, const int32_t *__restrict__ elist, const double *__restrict__ E,
+    const double *__restrict__ nu, int jmode, const int64_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ col_idx, bool sorted, double *__restrict__ val, int32_t *__restrict__ err) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= 3 * nn) return;
+  const int64_t i = r / 3;
+  const int d = (int)(r - 3 * i);
+  const int64_t rb = row_ptr[r] - 1, re = row_ptr[r + 1] - 1;
+  for (int64_t s = eoff[i]; s < eoff[i + 1]; ++s) {
+    const int64_t e = elist[s];
+    int32_t nodes[8];
+    double X[8][3];
+    load_element(nn, ne, xyz, en, e, nodes, X);
+    int a = 0;
+    while (a < 8 && nodes[a] - 1 != i) ++a;
+    double acc[24];
+    hex8_ke_row(X, E[e], nu[e], jmode, a, d, acc);
+    for (int b = 0; b < 8; ++b)
+      for (int dd = 0; dd < 3; ++dd) {
+        const int64_t id = find_col(col_idx, rb, re, 3 * (nodes[b] - 1) + dd + 1, sorted);
+        if (id < 0) {
+          atomicOr(err, 1);
+          continue;
+        }
+        val[id] = val[id] + acc[3 * b + dd];
+      }
+  }
+}
+
+}  // namespace
+
+int pf_overset_search(pf_ctx *ctx, const NodeGraph &A, const NodeGraph &B, DevBuf<int32_t> &elem,
+                      DevBuf<int32_t> &gp, DevBuf<int32_t> &pair_elem, DevBuf<double> &position, int64_t *count) {
+  *count = 0;
+  if (A.nne != 8 || B.nne != 8 || !A.has_coords || !B.has_coords || !B.built) {
+    pf_set_error("overset: two hex8 meshes with coordinates (and the node->element lists of the other)");
+    return PF_INVALID;
+  }
+  if (A.ne == 0 || B.nn == 0 || B.ne == 0) return PF_OK;
+  const int64_t m = A.ne * 8;
+  DevBuf<double> bb, pos;
+  DevBuf<int32_t> pair, flag;
+  DevBuf<int64_t> off, tmp;
+  PF_CHECK(bb.ensure(6));
+  PF_CHECK(pos.ensure((size_t)m * 3));
+  PF_CHECK(pair.ensure((size_t)m));
+  PF_CHECK(flag.ensure((size_t)m + 1));
+  PF_CHECK(off.ensure((size_t)m + 1));
+  k_node_bbox<<<1, 256, 0, ctx->stream>>>(B.nn, B.nodcoord.p, bb.p);
+  k_overset_gpp<<<pf_blocks(m * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+      A.nn, A.ne, A.nodcoord.p, A.elemnod.p, B.nn, B.ne, B.nodcoord.p, B.elemnod.p, B.eoff.p, B.elist.p, bb.p,
+      pair.p, pos.p);
+  k_overset_flags<<<pf_blocks(m, 256, INT32_MAX), 256, 0, ctx->stream>>>(m, pair.p, flag.p);
+  PF_HIP(hipGetLastError());
+  PF_CHECK(pf_scan_i32_to_i64(ctx, flag.p, m, off.p, tmp));
+  int64_t k = 0;
+  PF_CHECK(pf_read_i64(ctx, off.p + m, &k));
+  PF_CHECK(elem.ensure((size_t)k + 1));
+  PF_CHECK(gp.ensure((size_t)k + 1));
+  PF_CHECK(pair_elem.ensure((size_t)k + 1));
+  PF_CHECK(position.ensure((size_t)k * 3 + 1));
+  if (k > 0)
+    k_overset_scatter<<<pf_blocks(m, 256, INT32_MAX), 256, 0, ctx->stream>>>(m, pair.p, pos.p, off.p, elem.p, gp.p,
+                                                                            pair_elem.p, position.p);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  *count = k;
+  return PF_OK;
+}
+
+int pf_ebom_assemble(pf_ctx *ctx, const NodeGraph &g, int64_t ncon, const int32_t *my, const int32_t *gp,
+                     const int32_t *pr, const double *pos, double penalty, const int64_t *row_ptr,
+                     const int32_t *col_idx, bool sorted, double *val, int32_t *err) {
+  if (g.nne != 8 || !g.has_coords) {
+    pf_set_error("setEbOM needs the hex8 mesh with coordinates");
+    return PF_INVALID;
+  }
+  if (ncon <= 0) return PF_OK;
+  DevBuf<double> c16, detj;
+  DevBuf<int32_t> nodes16, cnt;
+  DevBuf<int64_t> off, list, tmp;
+  const int64_t total = ncon * 16;
+  PF_CHECK(c16.ensure((size_t)total));
+  PF_CHECK(detj.ensure((size_t)ncon));
+  PF_CHECK(nodes16.ensure((size_t)total));
+  PF_CHECK(cnt.ensure((size_t)g.nn + 1));
+  PF_CHECK(off.ensure((size_t)g.nn + 1));
+  PF_CHECK(list.ensure((size_t)total));
+  k_connect_coeffs<<<pf_blocks(ncon, 128, INT32_MAX), 128, 0, ctx->stream>>>(
+      g.nn, g.ne, g.nodcoord.p, g.elemnod.p, ncon, my, gp, pr, pos, c16.p, detj.p, nodes16.p);
+  PF_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (g.nn + 1), ctx->stream));
+  k_con_count<<<pf_blocks(total, 256, INT32_MAX), 256, 0, ctx->stream>>>(total, nodes16.p, cnt.p);
+  PF_CHECK(pf_scan_i32_to_i64(ctx, cnt.p, g.nn, off.p, tmp));
+  PF_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (g.nn + 1), ctx->stream));
+  k_con_fill<<<pf_blocks(total, 256, INT32_MAX), 256, 0, ctx->stream>>>(total, nodes16.p, off.p, cnt.p, list.p);
+  k_batch_sort<<<pf_blocks(g.nn, 256, INT32_MAX), 256, 0, ctx->stream>>>(g.nn, off.p, list.p);
+  k_ebom_rows<<<pf_blocks(3 * g.nn, 128, INT32_MAX), 128, 0, ctx->stream>>>(
+      g.nn, off.p, list.p, nodes16.p, c16.p, detj.p, penalty, row_ptr, col_idx, sorted, val, err);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int pf_assemble_hex8_search(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
+                            const int64_t *row_ptr, const int32_t *col_idx, bool sorted, double *val,
+                            int32_t *err) {
+  if (g.nn == 0) return PF_OK;
+  k_assemble_rows_search<<<pf_blocks(3 * g.nn, 128, INT32_MAX), 128, 0, ctx->stream>>>(
+      g.nn, g.ne, g.nodcoord.p, g.elemnod.p, g.eoff.p, g.elist.p, E, nu, jmode, row_ptr, col_idx, sorted, val,
+      err);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}

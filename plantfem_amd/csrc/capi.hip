@@ -519,6 +519,7 @@ struct pf_system {
   int dof = 0;
   int64_t n = 0, nnz = 0;
   bool graph_crs = false;  // CRS is ZeroMatrix(dof) of the mesh
+  bool multi_crs = false;  // CRS is the multi-domain pattern (node blocks, columns ascending)
   DevBuf<int64_t> row_ptr;
   DevBuf<int32_t> col_idx;
   DevBuf<double> val, rhs, x, ubar;
@@ -626,6 +627,7 @@ extern "C" int pf_system_set_mesh(pf_system *s, int64_t nn, int64_t ne, int32_t 
   s->has_mesh = true;
   s->has_mat = false;
   s->graph_crs = false;
+  s->multi_crs = false;
   return PF_OK;
 }
 
@@ -653,6 +655,7 @@ extern "C" int pf_system_zero_matrix(pf_system *s, int32_t dof, int64_t *nnz) {
   PF_CHECK(sys_alloc_vectors(s, true));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->graph_crs = true;
+  s->multi_crs = false;
   mark_changed(s);
   if (nnz) *nnz = s->nnz;
   return PF_OK;
@@ -679,6 +682,7 @@ extern "C" int pf_system_set_crs(pf_system *s, int64_t n, int64_t nnz, const int
   if (resize || !s->rhs.p) PF_CHECK(sys_alloc_vectors(s, true));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->graph_crs = false;
+  s->multi_crs = false;
   mark_changed(s);
   return PF_OK;
 }
@@ -701,13 +705,35 @@ extern "C" int pf_system_get_crs(pf_system *s, int64_t *row_ptr, int32_t *col_id
   return PF_OK;
 }
 
+// stiffness on the multi-domain pattern: values cleared, every entry summed over the
+// incident elements of the concatenated mesh in ascending id (the order of setMatrix calls
+// looping over domains and their elements)
+static int multi_stiffness(pf_system *s, int32_t jmode) {
+  pf_ctx *ctx = s->ctx;
+  if (!s->g.built) PF_CHECK(pf_graph_node_elems(ctx, s->g));
+  DevBuf<int32_t> err;
+  PF_CHECK(err.ensure(1));
+  PF_HIP(hipMemsetAsync(err.p, 0, sizeof(int32_t), ctx->stream));
+  PF_HIP(hipMemsetAsync(s->val.p, 0, sizeof(double) * (s->nnz + 1), ctx->stream));
+  PF_CHECK(pf_assemble_hex8_search(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->col_idx.p, true, s->val.p,
+                                   err.p));
+  int32_t herr = 0;
+  PF_CHECK(download(ctx, &herr, err.p, 1));
+  if (herr) {
+    pf_set_error("stiffness: no CRS slot for an element entry (the multi-domain CRS is empty or stale)");
+    return PF_INVALID;
+  }
+  return PF_OK;
+}
+
 extern "C" int pf_system_stiffness(pf_system *s, const double *E, const double *nu, int32_t jmode) {
   PF_CHECK(sys_check(s, false));
   if (!s->has_mesh || s->g.nne != 8 || (!E != !nu) || (!E && !s->has_mat)) {
     pf_set_error("pf_system_stiffness: hex8 mesh and E, nu (or resident ones) required");
     return PF_INVALID;
   }
-  if (!(s->graph_crs && s->dof == 3)) PF_CHECK(pf_system_zero_matrix(s, 3, nullptr));
+  if (!(s->graph_crs && s->dof == 3) && !(s->multi_crs && s->dof == 3))
+    PF_CHECK(pf_system_zero_matrix(s, 3, nullptr));
   pf_ctx *ctx = s->ctx;
   DeviceGuard guard(ctx);
   if (E) {
@@ -715,7 +741,8 @@ extern "C" int pf_system_stiffness(pf_system *s, const double *E, const double *
     PF_CHECK(upload(ctx, s->nu, nu, (size_t)s->g.ne));
     s->has_mat = true;
   }
-  PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
+  if (s->multi_crs) PF_CHECK(multi_stiffness(s, jmode));
+  else PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->jmode = jmode;
   mark_changed(s);
@@ -728,7 +755,8 @@ extern "C" int pf_system_stiffness_dev(pf_system *s, const double *E_dev, const 
     pf_set_error("pf_system_stiffness_dev: hex8 mesh and device arrays E, nu required");
     return PF_INVALID;
   }
-  if (!(s->graph_crs && s->dof == 3)) PF_CHECK(pf_system_zero_matrix(s, 3, nullptr));
+  if (!(s->graph_crs && s->dof == 3) && !(s->multi_crs && s->dof == 3))
+    PF_CHECK(pf_system_zero_matrix(s, 3, nullptr));
   pf_ctx *ctx = s->ctx;
   DeviceGuard guard(ctx);
   PF_CHECK(s->E.ensure((size_t)s->g.ne));
@@ -736,7 +764,8 @@ extern "C" int pf_system_stiffness_dev(pf_system *s, const double *E_dev, const 
   PF_HIP(hipMemcpyAsync(s->E.p, E_dev, sizeof(double) * s->g.ne, hipMemcpyDeviceToDevice, ctx->stream));
   PF_HIP(hipMemcpyAsync(s->nu.p, nu_dev, sizeof(double) * s->g.ne, hipMemcpyDeviceToDevice, ctx->stream));
   s->has_mat = true;
-  PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
+  if (s->multi_crs) PF_CHECK(multi_stiffness(s, jmode));
+  else PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->jmode = jmode;
   mark_changed(s);
@@ -1237,4 +1266,152 @@ extern "C" int pf_system_element_stress(pf_system *s, const double *E, int64_t n
   PF_CHECK(pf_cell_stress(ctx, s->g, s->x.p, dE.p, nE == s->g.ne && s->g.ne > 1 ? 1 : 0, dnu.p,
                           nnu == s->g.ne && s->g.ne > 1 ? 1 : 0, jmode, ds.p));
   return download(ctx, sigma, ds.p, (size_t)s->g.ne * 9);
+}
+
+// ====================================================== multi-domain assembly with overset
+static int check_hex8_mesh_args(int64_t nn, int64_t ne, const double *xyz, const int32_t *en) {
+  if (nn < 0 || ne < 0 || (nn > 0 && !xyz) || (ne > 0 && !en) || nn >= (int64_t)INT32_MAX ||
+      ne * 8 >= (int64_t)INT32_MAX) {
+    pf_set_error("invalid hex8 mesh arguments");
+    return PF_INVALID;
+  }
+  return PF_OK;
+}
+
+extern "C" int pf_overset_gpp(pf_ctx *ctx, int64_t nnA, int64_t neA, const double *nodcoordA,
+                              const int32_t *elemnodA, int64_t nnB, int64_t neB, const double *nodcoordB,
+                              const int32_t *elemnodB, int64_t cap, int64_t *count, int32_t *elem,
+                              int32_t *gauss_point, int32_t *pair_elem, double *position) {
+  PF_CHECK(check_ctx(ctx));
+  PF_CHECK(check_hex8_mesh_args(nnA, neA, nodcoordA, elemnodA));
+  PF_CHECK(check_hex8_mesh_args(nnB, neB, nodcoordB, elemnodB));
+  if (!count || cap < 0 || (cap > 0 && (!elem || !gauss_point || !pair_elem || !position))) {
+    pf_set_error("pf_overset_gpp: count and (for cap > 0) the four output arrays required");
+    return PF_INVALID;
+  }
+  DeviceGuard guard(ctx);
+  NodeGraph A, B;
+  PF_CHECK(set_mesh(ctx, A, nnA, neA, 8, nodcoordA, elemnodA));
+  PF_CHECK(pf_graph_check_mesh(ctx, A));
+  PF_CHECK(set_mesh(ctx, B, nnB, neB, 8, nodcoordB, elemnodB));
+  PF_CHECK(pf_graph_node_elems(ctx, B));
+  DevBuf<int32_t> e, g, p;
+  DevBuf<double> x;
+  int64_t k = 0;
+  PF_CHECK(pf_overset_search(ctx, A, B, e, g, p, x, &k));
+  *count = k;
+  const int64_t m = std::min(k, cap);
+  if (m > 0) {
+    PF_CHECK(download(ctx, elem, e.p, (size_t)m));
+    PF_CHECK(download(ctx, gauss_point, g.p, (size_t)m));
+    PF_CHECK(download(ctx, pair_elem, p.p, (size_t)m));
+    PF_CHECK(download(ctx, position, x.p, (size_t)m * 3));
+  }
+  return PF_OK;
+}
+
+static int check_connects(const pf_system *s, int64_t ncon, const int32_t *elem, const int32_t *pair_elem) {
+  for (int64_t c = 0; c < ncon; ++c)
+    if (elem[c] < 1 || elem[c] > s->g.ne || pair_elem[c] < 1 || pair_elem[c] > s->g.ne) {
+      pf_set_error("connect %lld: element id outside 1..%lld", (long long)c + 1, (long long)s->g.ne);
+      return PF_INVALID;
+    }
+  return PF_OK;
+}
+
+extern "C" int pf_system_multi_crs(pf_system *s, int32_t dof, int32_t any_overset, int64_t ncon,
+                                   const int32_t *elem, const int32_t *pair_elem, int64_t *nnz) {
+  PF_CHECK(sys_check(s, false));
+  if (!s->has_mesh || s->g.nne != 8 || dof < 1 || ncon < 0 || (ncon > 0 && (!elem || !pair_elem))) {
+    pf_set_error("pf_system_multi_crs: hex8 mesh, dof >= 1 and the connect arrays required");
+    return PF_INVALID;
+  }
+  PF_CHECK(check_connects(s, ncon, elem, pair_elem));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  s->dof = dof;
+  s->n = (int64_t)dof * s->g.nn;
+  PF_CHECK(s->row_ptr.ensure((size_t)s->n + 1));
+  if (!any_overset) {  // no domain holds overset connects: every row stays empty
+    std::vector<int64_t> ones((size_t)s->n + 1, 1);
+    PF_CHECK(upload(ctx, s->row_ptr, ones.data(), ones.size()));
+    s->nnz = 0;
+    PF_CHECK(s->col_idx.ensure(1));
+    PF_CHECK(s->val.ensure(1));
+  } else {
+    DevBuf<int32_t> dmy, dpr;
+    if (ncon > 0) {
+      std::vector<int32_t> a((size_t)ncon), b((size_t)ncon);
+      for (int64_t c = 0; c < ncon; ++c) {
+        a[c] = elem[c] - 1;
+        b[c] = pair_elem[c] - 1;
+      }
+      PF_CHECK(upload(ctx, dmy, a.data(), a.size()));
+      PF_CHECK(upload(ctx, dpr, b.data(), b.size()));
+    }
+    NodeGraph merged;
+    PF_CHECK(pf_graph_merged_cols(ctx, s->g, ncon, dmy.p, dpr.p, merged));
+    s->nnz = nnz_of(merged, dof);
+    if (s->nnz >= (int64_t)1 << 40) {
+      pf_set_error("CRS too large");
+      return PF_INVALID;
+    }
+    PF_CHECK(s->col_idx.ensure((size_t)s->nnz + 1));
+    PF_CHECK(s->val.ensure((size_t)s->nnz + 1));
+    PF_CHECK(pf_graph_expand(ctx, merged, dof, s->row_ptr.p, s->col_idx.p));
+  }
+  PF_HIP(hipMemsetAsync(s->val.p, 0, sizeof(double) * (s->nnz + 1), ctx->stream));
+  PF_CHECK(sys_alloc_vectors(s, true));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  s->graph_crs = false;
+  s->multi_crs = true;
+  mark_changed(s);
+  if (nnz) *nnz = s->nnz;
+  return PF_OK;
+}
+
+extern "C" int pf_system_ebom(pf_system *s, int64_t ncon, const int32_t *elem, const int32_t *gauss_point,
+                              const int32_t *pair_elem, const double *position, double penalty, int32_t dof) {
+  PF_CHECK(sys_check(s, true));
+  if (!s->has_mesh || s->g.nne != 8 || !s->g.has_coords || ncon < 0 ||
+      (ncon > 0 && (!elem || !gauss_point || !pair_elem || !position))) {
+    pf_set_error("pf_system_ebom: hex8 mesh with coordinates and the connect arrays required");
+    return PF_INVALID;
+  }
+  if (dof != 3 || s->dof != 3) {
+    pf_set_error("pf_system_ebom: DOF = 3 (3-D hex8 displacement) is the supported form");
+    return PF_INVALID;
+  }
+  PF_CHECK(check_connects(s, ncon, elem, pair_elem));
+  for (int64_t c = 0; c < ncon; ++c)
+    if (gauss_point[c] < 1 || gauss_point[c] > 8) {
+      pf_set_error("connect %lld: Gauss point id outside 1..8", (long long)c + 1);
+      return PF_INVALID;
+    }
+  if (ncon == 0) return PF_OK;
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  std::vector<int32_t> a((size_t)ncon), b((size_t)ncon);
+  for (int64_t c = 0; c < ncon; ++c) {
+    a[c] = elem[c] - 1;
+    b[c] = pair_elem[c] - 1;
+  }
+  DevBuf<int32_t> dmy, dgp, dpr, err;
+  DevBuf<double> dpos;
+  PF_CHECK(upload(ctx, dmy, a.data(), a.size()));
+  PF_CHECK(upload(ctx, dpr, b.data(), b.size()));
+  PF_CHECK(upload(ctx, dgp, gauss_point, (size_t)ncon));
+  PF_CHECK(upload(ctx, dpos, position, (size_t)ncon * 3));
+  PF_CHECK(err.ensure(1));
+  PF_HIP(hipMemsetAsync(err.p, 0, sizeof(int32_t), ctx->stream));
+  PF_CHECK(pf_ebom_assemble(ctx, s->g, ncon, dmy.p, dgp.p, dpr.p, dpos.p, penalty, s->row_ptr.p, s->col_idx.p,
+                            s->multi_crs, s->val.p, err.p));
+  int32_t herr = 0;
+  PF_CHECK(download(ctx, &herr, err.p, 1));
+  mark_changed(s);
+  if (herr) {
+    pf_set_error("setEbOM: no CRS slot for a connect entry (setCRS before the overset?)");
+    return PF_INVALID;
+  }
+  return PF_OK;
 }

@@ -232,3 +232,95 @@ int pf_graph_expand(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t *row_ptr, 
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
+
+// ---- multi-domain pattern (setCRS with several domains, FEMSolverClass.f90:466-676) -------
+// The reference concatenates the domains' ZeroMatrix patterns (block diagonal), adds every
+// node pair of every overset connect (the 16 nodes of the own and the paired element) to a
+// COO that drops repeats, and heap-sorts each row.  Here the (row node, column node) pairs of
+// both sources become 64-bit keys, one radix sort and one unique pass leave the node
+// neighbour lists sorted and distinct, and the usual DOF expansion writes the CRS.
+#include <hipcub/hipcub.hpp>
+
+namespace {
+
+__global__ void k_keys_graph(int64_t nn, const int64_t *__restrict__ noff, const int32_t *__restrict__ ncol,
+                             uint64_t *__restrict__ keys) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  for (int64_t q = noff[i]; q < noff[i + 1]; ++q) keys[q] = ((uint64_t)i << 32) | (uint32_t)ncol[q];
+}
+
+// connect c couples element my[c] and element pair[c] (0-based) of one mesh: 16 x 16 pairs
+__global__ void k_keys_connect(int64_t ncon, int64_t ne, const int32_t *__restrict__ en,
+                               const int32_t *__restrict__ my, const int32_t *__restrict__ pair,
+                               uint64_t *__restrict__ keys) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ncon * 256) return;
+  const int64_t c = t >> 8;
+  const int j = (int)((t >> 4) & 15), k = (int)(t & 15);
+  const int32_t rj = j < 8 ? en[(int64_t)j * ne + my[c]] : en[(int64_t)(j - 8) * ne + pair[c]];
+  const int32_t ck = k < 8 ? en[(int64_t)k * ne + my[c]] : en[(int64_t)(k - 8) * ne + pair[c]];
+  keys[t] = ((uint64_t)(uint32_t)(rj - 1) << 32) | (uint32_t)(ck - 1);
+}
+
+__global__ void k_keys_count(int64_t u, const uint64_t *__restrict__ keys, int32_t *__restrict__ deg) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= u) return;
+  atomicAdd(&deg[keys[t] >> 32], 1);
+}
+
+__global__ void k_keys_cols(int64_t u, const uint64_t *__restrict__ keys, int32_t *__restrict__ ncol) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= u) return;
+  ncol[t] = (int32_t)(keys[t] & 0xffffffffu);
+}
+
+}  // namespace
+
+int pf_graph_merged_cols(pf_ctx *ctx, NodeGraph &g, int64_t ncon, const int32_t *my, const int32_t *pair,
+                         NodeGraph &out) {
+  if (g.nne != 8) {
+    pf_set_error("overset connects need an 8-node mesh");
+    return PF_INVALID;
+  }
+  if (!g.cols_built) PF_CHECK(pf_graph_node_cols(ctx, g));
+  const int64_t total = g.nblocks + ncon * 256;
+  out.nn = g.nn;
+  out.ne = g.ne;
+  out.nne = g.nne;
+  PF_CHECK(out.deg.ensure((size_t)g.nn + 1));
+  PF_CHECK(out.noff.ensure((size_t)g.nn + 1));
+  DevBuf<uint64_t> ka, kb;
+  DevBuf<int64_t> tmp, nsel;
+  DevBuf<unsigned char> cub;
+  PF_CHECK(ka.ensure((size_t)total + 1));
+  PF_CHECK(kb.ensure((size_t)total + 1));
+  PF_CHECK(nsel.ensure(1));
+  if (g.nn > 0) k_keys_graph<<<pf_blocks(g.nn, 256, INT32_MAX), 256, 0, ctx->stream>>>(g.nn, g.noff.p, g.ncol.p, ka.p);
+  if (ncon > 0)
+    k_keys_connect<<<pf_blocks(ncon * 256, 256, INT32_MAX), 256, 0, ctx->stream>>>(ncon, g.ne, g.elemnod.p, my,
+                                                                                  pair, ka.p + g.nblocks);
+  PF_HIP(hipGetLastError());
+  int end_bit = 33;
+  while (end_bit < 64 && ((int64_t)1 << (end_bit - 32)) < g.nn) ++end_bit;
+  size_t b1 = 0, b2 = 0;
+  PF_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b1, ka.p, kb.p, total, 0, end_bit, ctx->stream));
+  PF_HIP(hipcub::DeviceSelect::Unique(nullptr, b2, kb.p, ka.p, nsel.p, total, ctx->stream));
+  PF_CHECK(cub.ensure(std::max(b1, b2) + 1));
+  PF_HIP(hipcub::DeviceRadixSort::SortKeys(cub.p, b1, ka.p, kb.p, total, 0, end_bit, ctx->stream));
+  PF_HIP(hipcub::DeviceSelect::Unique(cub.p, b2, kb.p, ka.p, nsel.p, total, ctx->stream));
+  int64_t u = 0;
+  PF_CHECK(pf_read_i64(ctx, nsel.p, &u));
+  PF_HIP(hipMemsetAsync(out.deg.p, 0, sizeof(int32_t) * (g.nn + 1), ctx->stream));
+  PF_CHECK(out.ncol.ensure((size_t)u + 1));
+  if (u > 0) {
+    k_keys_count<<<pf_blocks(u, 256, INT32_MAX), 256, 0, ctx->stream>>>(u, ka.p, out.deg.p);
+    k_keys_cols<<<pf_blocks(u, 256, INT32_MAX), 256, 0, ctx->stream>>>(u, ka.p, out.ncol.p);
+  }
+  PF_HIP(hipGetLastError());
+  PF_CHECK(pf_scan_i32_to_i64(ctx, out.deg.p, g.nn, out.noff.p, tmp));
+  PF_HIP(hipStreamSynchronize(ctx->stream));  // scratch buffers are freed on return
+  out.nblocks = u;
+  out.cols_built = true;
+  return PF_OK;
+}

@@ -123,6 +123,11 @@ int pf_graph_node_elems(pf_ctx *ctx, NodeGraph &g);                 // eoff/elis
 int pf_graph_node_cols(pf_ctx *ctx, NodeGraph &g);                  // deg/noff/ncol
 int pf_graph_expand(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t *row_ptr, int32_t *col_idx);
 int pf_graph_check_mesh(pf_ctx *ctx, const NodeGraph &g);           // node ids in range
+// node neighbour lists of the multi-domain setCRS: g's element pairs and the 16 x 16 node
+// pairs of every overset connect (my/pair: 0-based element ids of g, device), sorted and
+// distinct; fills out.deg/noff/ncol/nblocks for pf_graph_expand
+int pf_graph_merged_cols(pf_ctx *ctx, NodeGraph &g, int64_t ncon, const int32_t *my, const int32_t *pair,
+                         NodeGraph &out);
 
 // assemble.hip
 int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
@@ -136,6 +141,20 @@ int pf_add_element_mats(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count,
 int pf_add_element_vecs(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t count, const int32_t *ids,
                         const double *vec, double *rhs);
 int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double pressure, double *rhs);
+// overset GPP: the Gauss points of A's elements located in elements of B, in (element,
+// Gauss point) order: 1-based own element, Gauss point, element of B, and the position.
+// B needs its node->element lists (pf_graph_node_elems).
+int pf_overset_search(pf_ctx *ctx, const NodeGraph &A, const NodeGraph &B, DevBuf<int32_t> &elem,
+                      DevBuf<int32_t> &gp, DevBuf<int32_t> &pair_elem, DevBuf<double> &position, int64_t *count);
+// setEbOM: penalty * ConnectMatrix of ncon connects (0-based own/paired elements of g, 1-based
+// Gauss point ids, positions) added row by row in connect order; err set for a missing slot
+int pf_ebom_assemble(pf_ctx *ctx, const NodeGraph &g, int64_t ncon, const int32_t *my, const int32_t *gp,
+                     const int32_t *pr, const double *pos, double penalty, const int64_t *row_ptr,
+                     const int32_t *col_idx, bool sorted, double *val, int32_t *err);
+// hex8 stiffness added into a CRS of any column order (sorted: binary search of the rows)
+int pf_assemble_hex8_search(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
+                            const int64_t *row_ptr, const int32_t *col_idx, bool sorted, double *val,
+                            int32_t *err);
 // MassVectorFEMDomain of elements first..first+count-1 (0-based first): out[q*24 + 3a + k]
 int pf_hex8_mass_batch(pf_ctx *ctx, const NodeGraph &g, double rho, const double *accel, int64_t first,
                        int64_t count, double *out);

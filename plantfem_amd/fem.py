@@ -86,6 +86,36 @@ def _i64(a):
     return np.ascontiguousarray(a, dtype=np.int64)
 
 
+FEMDomain_Overset_GPP = 1  # FEMDomainClass.f90:40-41
+FEMDomain_Overset_P2P = 2
+
+
+class OversetConnect:
+    """The OversetConnect_ list of one domain as arrays (GPP): ElementID, GaussPointID (1-based),
+    the paired element (1-based, in the paired domain), position (k, 3) and DomainIDs12 as
+    (own, paired) per connect; all connects are active."""
+
+    def __init__(self):
+        self.ElementID = np.zeros(0, np.int32)
+        self.GaussPointID = np.zeros(0, np.int32)
+        self.PairElementID = np.zeros(0, np.int32)
+        self.position = np.zeros((0, 3))
+        self.DomainIDs12 = np.zeros((0, 2), np.int32)
+
+    @property
+    def n(self) -> int:
+        return int(self.ElementID.size)
+
+    def append(self, elem, gp, pair, pos, my_id: int, pair_id: int):
+        k = len(elem)
+        self.ElementID = np.concatenate([self.ElementID, np.asarray(elem, np.int32)])
+        self.GaussPointID = np.concatenate([self.GaussPointID, np.asarray(gp, np.int32)])
+        self.PairElementID = np.concatenate([self.PairElementID, np.asarray(pair, np.int32)])
+        self.position = np.concatenate([self.position, np.asarray(pos, np.float64).reshape(k, 3)])
+        ids = np.tile(np.array([my_id, pair_id], np.int32), (k, 1))
+        self.DomainIDs12 = np.concatenate([self.DomainIDs12, ids])
+
+
 class FEMDomain:
     """Subset of FEMDomain_ used by the static solve path."""
 
@@ -93,6 +123,40 @@ class FEMDomain:
         self.ctx = ctx
         self.nodcoord = None if nodcoord is None else np.asfortranarray(nodcoord, dtype=np.float64)
         self.elemnod = None if elemnod is None else np.asfortranarray(elemnod, dtype=np.int32)
+        # OversetConnect (FEMDomainClass.f90:14606-14700): None until the first overset call
+        self.OversetConnect: OversetConnect | None = None
+
+    def overset(self, FEMDomain: "FEMDomain", DomainID: int, algorithm=None, MyDomainID: int | None = None):
+        """overset(FEMDomain, DomainID, algorithm=FEMDomain_Overset_GPP, MyDomainID)
+        (FEMDomainClass.f90:14606-14700): every Gauss point of this mesh that lies in an element
+        of FEMDomain becomes one connect (own element, Gauss point, paired element, position,
+        domain ids [MyDomainID (default 1), DomainID]), appended in the reference's order.  The
+        search runs on the GPU (pf_overset_gpp)."""
+        alg = FEMDomain_Overset_GPP if algorithm is None else algorithm
+        if alg in ("P2P", FEMDomain_Overset_P2P):
+            raise NotImplementedError("overset: the point-to-point (P2P) projection is not built; use GPP")
+        if alg not in ("GPP", FEMDomain_Overset_GPP):
+            raise ValueError(f"overset: unknown algorithm {algorithm!r}")
+        my_id = 1 if MyDomainID is None else int(MyDomainID)
+        if self.OversetConnect is None:
+            self.OversetConnect = OversetConnect()
+        xa, ea = self._flat()
+        xb, eb = FEMDomain._flat()
+        lib = L.load()
+        cap = 8 * self.ne()
+        elem, gp, pe = (np.empty(max(cap, 1), np.int32) for _ in range(3))
+        pos = np.empty(max(cap, 1) * 3)
+        count = C.c_int64(0)
+        L.check(lib.pf_overset_gpp(self._ctx().h, self.nn(), self.ne(), L.ptr(xa, C.c_double), L.ptr(ea, C.c_int32),
+                                   FEMDomain.nn(), FEMDomain.ne(), L.ptr(xb, C.c_double), L.ptr(eb, C.c_int32),
+                                   cap, C.byref(count), L.ptr(elem, C.c_int32), L.ptr(gp, C.c_int32),
+                                   L.ptr(pe, C.c_int32), L.ptr(pos, C.c_double)), "overset")
+        k = count.value
+        self.OversetConnect.append(elem[:k], gp[:k], pe[:k], pos[:3 * k].reshape(k, 3), my_id, int(DomainID))
+        return self
+
+    def numOversetElements(self) -> int:
+        return 0 if self.OversetConnect is None else self.OversetConnect.n
 
     def _ctx(self):
         return self.ctx or L.default_context()
@@ -344,18 +408,78 @@ class FEMSolver:
 
     # (1)-(2)
     def init(self, NumDomain: int = 1):
-        if NumDomain != 1:
-            raise NotImplementedError("multi-domain assembly is outside the single-domain static path")
+        """init(NumDomain) (FEMSolverClass.f90:167-210).  With NumDomain >= 2 the domains are
+        assembled as one system (setDomain per slot, then the multi-domain setCRS)."""
+        if NumDomain < 1:
+            raise ValueError("init: NumDomain >= 1")
+        self.FEMDomains: list = [None] * int(NumDomain)
+        self.CRS_ID_Starts_From = None
         return self
 
-    def setDomain(self, FEMDomain: FEMDomain, DomainID: int = 1):
-        if DomainID != 1:
-            raise NotImplementedError("only DomainID=1")
-        self.domain = FEMDomain
-        xyz, en = FEMDomain._flat()
-        L.check(L.load().pf_system_set_mesh(self.h, FEMDomain.nn(), FEMDomain.ne(), FEMDomain.nne(),
-                                            L.ptr(xyz, C.c_double), L.ptr(en, C.c_int32)), "setDomain")
+    @property
+    def _multi(self) -> bool:
+        return len(getattr(self, "FEMDomains", [None])) >= 2
+
+    def setDomain(self, FEMDomain: FEMDomain | None = None, DomainID: int = 1, FEMDomains=None, DomainIDs=None,
+                  FEMDomainPointers=None):
+        """setDomain (FEMSolverClass.f90:329-419): one domain into slot DomainID, a list into
+        DomainIDs (or slots 1..n for FEMDomainPointers)."""
+        if FEMDomainPointers is not None:
+            FEMDomains, DomainIDs = list(FEMDomainPointers), list(range(1, len(FEMDomainPointers) + 1))
+        if FEMDomains is not None:
+            if DomainIDs is None:
+                DomainIDs = list(range(1, len(FEMDomains) + 1))
+            for d, i in zip(FEMDomains, DomainIDs):
+                self.setDomain(FEMDomain=d, DomainID=int(i))
+            return self
+        if not self._multi:
+            if DomainID != 1:
+                raise ValueError("setDomain: DomainID outside 1..NumDomain (call init(NumDomain) first)")
+            self.domain = FEMDomain
+            xyz, en = FEMDomain._flat()
+            L.check(L.load().pf_system_set_mesh(self.h, FEMDomain.nn(), FEMDomain.ne(), FEMDomain.nne(),
+                                                L.ptr(xyz, C.c_double), L.ptr(en, C.c_int32)), "setDomain")
+            return self
+        if not 1 <= DomainID <= len(self.FEMDomains):
+            raise ValueError("setDomain: invalid domain ID")  # "invalid domain IDs" stop
+        self.FEMDomains[DomainID - 1] = FEMDomain
         return self
+
+    def _union(self):
+        """The domains concatenated: node / element offsets per slot and the joint mesh."""
+        doms = self.FEMDomains
+        if any(d is None or d.nn() == 0 for d in doms):
+            raise ValueError("setCRS: every domain slot needs a non-empty mesh")
+        nn = np.array([d.nn() for d in doms], np.int64)
+        ne = np.array([d.ne() for d in doms], np.int64)
+        self._node_off = np.concatenate([[0], np.cumsum(nn)])
+        self._elem_off = np.concatenate([[0], np.cumsum(ne)])
+        xyz = np.concatenate([d.nodcoord for d in doms], axis=0)
+        en = np.concatenate([d.elemnod + int(o) for d, o in zip(doms, self._node_off[:-1])], axis=0)
+        self.domain = FEMDomain(xyz, en, self.ctx)
+        return self.domain
+
+    def _connects(self):
+        """Every active connect of every domain in domain order, as element ids of the joint mesh:
+        (own, Gauss point, paired, positions); any_overset = some domain holds OversetConnect."""
+        own, gp, pr, pos = [], [], [], []
+        any_overset = False
+        for d in self.FEMDomains:
+            oc = d.OversetConnect
+            if oc is None:
+                continue
+            any_overset = True
+            if oc.n == 0:
+                continue
+            own.append(oc.ElementID + self._elem_off[oc.DomainIDs12[:, 0] - 1].astype(np.int32))
+            pr.append(oc.PairElementID + self._elem_off[oc.DomainIDs12[:, 1] - 1].astype(np.int32))
+            gp.append(oc.GaussPointID)
+            pos.append(oc.position)
+        if not own:
+            z = np.zeros(0, np.int32)
+            return any_overset, z, z, z, np.zeros((0, 3))
+        return (any_overset, _i32(np.concatenate(own)), _i32(np.concatenate(gp)), _i32(np.concatenate(pr)),
+                _f64(np.concatenate(pos)))
 
     # (3)
     def setCRS(self, DOF: int | None = None, CRS: CRS | None = None):
@@ -366,10 +490,34 @@ class FEMSolver:
             self.n, self.nnz, self.DOF = CRS.n, CRS.nnz, 0
             return self
         nnz = C.c_int64(0)
-        L.check(lib.pf_system_zero_matrix(self.h, int(DOF), C.byref(nnz)), "setCRS")
+        if self._multi:
+            # multi-domain setCRS (FEMSolverClass.f90:466-676): joint mesh, block pattern
+            # merged with the overset node pairs, columns ascending
+            u = self._union()
+            xyz, en = u._flat()
+            L.check(lib.pf_system_set_mesh(self.h, u.nn(), u.ne(), u.nne(), L.ptr(xyz, C.c_double),
+                                           L.ptr(en, C.c_int32)), "setCRS")
+            any_ov, own, _, pr, _ = self._connects()
+            L.check(lib.pf_system_multi_crs(self.h, int(DOF), int(any_ov), own.size, L.ptr(own, C.c_int32),
+                                            L.ptr(pr, C.c_int32), C.byref(nnz)), "setCRS")
+            self.CRS_ID_Starts_From = (1 + int(DOF) * self._node_off[:-1]).astype(np.int64)
+        else:
+            L.check(lib.pf_system_zero_matrix(self.h, int(DOF), C.byref(nnz)), "setCRS")
         self.DOF = int(DOF)
         self.n = self.DOF * self.domain.nn()
         self.nnz = nnz.value
+        return self
+
+    def setEbOM(self, penalty: float, DOF: int = 3):
+        """setEbOM(penalty, DOF) (FEMSolverClass.f90:1891-1985): penalty * ConnectMatrix of every
+        connect of every domain, in order, added on the GPU."""
+        self._flush()
+        if not self._multi:
+            return self  # one domain: no connects to couple
+        _, own, gp, pr, pos = self._connects()
+        L.check(L.load().pf_system_ebom(self.h, own.size, L.ptr(own, C.c_int32), L.ptr(gp, C.c_int32),
+                                        L.ptr(pr, C.c_int32), L.ptr(pos, C.c_double), float(penalty), int(DOF)),
+                "setEbOM")
         return self
 
     def setRHS(self, RHS):
@@ -407,6 +555,8 @@ class FEMSolver:
     def setMatrix(self, DomainID: int, ElementID: int, DOF: int, Matrix):
         if self.n == 0:
             self.setCRS(DOF=DOF)
+        if self._multi:  # setValue multi-domain branch: ids offset by CRS_ID_Starts_From
+            ElementID = int(ElementID) + int(self._elem_off[int(DomainID) - 1])
         self._pending_ids.append(int(ElementID))
         self._pending_mats.append(np.asfortranarray(Matrix, dtype=np.float64).ravel(order="F"))
         if len(self._pending_ids) >= 65536:
@@ -424,6 +574,8 @@ class FEMSolver:
     def setVector(self, DomainID: int, ElementID, DOF: int, Vector):
         self._flush()
         ids = np.atleast_1d(np.asarray(ElementID, dtype=np.int32))
+        if self._multi:
+            ids = (ids + int(self._elem_off[int(DomainID) - 1])).astype(np.int32)
         vec = np.ascontiguousarray(np.asarray(Vector, dtype=np.float64).reshape(ids.size, -1))
         L.check(L.load().pf_system_add_element_vectors(self.h, ids.size, L.ptr(ids, C.c_int32),
                                                        L.ptr(vec, C.c_double)), "setVector")
@@ -432,6 +584,12 @@ class FEMSolver:
         """Device assembly of StiffnessMatrix(YoungModulus(:),PoissonRatio(:)) into this solver's CRS.
         Without arguments the materials of the previous call (resident on the device) are reused."""
         self._flush()
+        if self._multi and isinstance(YoungModulus, (list, tuple)):
+            YoungModulus = np.concatenate([np.broadcast_to(np.asarray(e, np.float64), (d.ne(),))
+                                           for e, d in zip(YoungModulus, self.FEMDomains)])
+        if self._multi and isinstance(PoissonRatio, (list, tuple)):
+            PoissonRatio = np.concatenate([np.broadcast_to(np.asarray(e, np.float64), (d.ne(),))
+                                           for e, d in zip(PoissonRatio, self.FEMDomains)])
         E = None if YoungModulus is None else _f64(YoungModulus)
         v = None if PoissonRatio is None else _f64(PoissonRatio)
         L.check(L.load().pf_system_stiffness(self.h, L.ptr(E, C.c_double), L.ptr(v, C.c_double), jmode),
@@ -450,6 +608,12 @@ class FEMSolver:
 
     # (5)
     def fix(self, DomainID: int = 1, IDs=None, FixValue: float | None = None, FixValues=None):
+        """fix(IDs, FixValue|FixValues) (FEMSolverClass.f90:1073-1136): IDs are DOF ids of the
+        whole system.  With several domains the reference's DomainID >= 2 form marks IDs but
+        stores the values at IDs + (nodes of the earlier domains); that form is refused here."""
+        if self._multi and DomainID is not None and int(DomainID) >= 2:
+            raise NotImplementedError("fix(DomainID>=2): pass DOF ids of the whole system (offset by "
+                                      "CRS_ID_Starts_From - 1) without DomainID")
         ids = _i32(IDs)
         vals = None if FixValues is None else _f64(FixValues)
         L.check(L.load().pf_system_fix(self.h, ids.size, L.ptr(ids, C.c_int32), L.ptr(vals, C.c_double),
