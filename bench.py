@@ -24,6 +24,10 @@ Two further workloads print their own line (not the default):
                                          PointJacobi and ILU(0) BiCGSTAB (solve(preconditioning=))
     python bench.py --workload modal     mass matrix + eig: the 10 lowest modes of a 20x20x100 bar
                                          (LOBPCG) and every mode of a 10^3 box (dense)
+    python bench.py --workload overset   a plant of 17 organ meshes (stem, 8 petioles, 8 leaf blades)
+                                         coupled by overset penalties as Soybean%getDisplacement
+                                         assembles it: overset search, multi-domain setCRS,
+                                         stiffness, setEbOM, gravity, solve
 """
 from __future__ import annotations
 
@@ -69,7 +73,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-graph", action="store_true", help="direct kernel launches only (for profilers)")
-    ap.add_argument("--workload", choices=("box", "batch", "adjoint", "precond", "modal"), default="box")
+    ap.add_argument("--workload", choices=("box", "batch", "adjoint", "precond", "modal", "overset"), default="box")
+    ap.add_argument("--plant-scale", type=int, default=2, help="refinement of the overset plant (1: 23k DOF)")
     ap.add_argument("--precond-size", type=int, default=60)
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-itrmax", type=int, default=3000,
@@ -95,6 +100,8 @@ def main(argv=None):
         return bench_adjoint(a)
     if a.workload == "precond":
         return bench_precond(a)
+    if a.workload == "overset":
+        return bench_overset(a)
     if a.workload == "modal":
         return bench_modal(a)
 
@@ -485,6 +492,128 @@ def bench_modal(a):
                                          "reduced pencil, all pairs with vectors (the reference's default eig "
                                          "calls LAPACK DSPGVD)"}
     out["value"] = out["lobpcg"]["eig_s"]
+    print(json.dumps(out), flush=True)
+
+
+def bench_overset(a):
+    """Soybean%getDisplacement's sequence (SoybeanClass.f90:8340-8533) on a synthetic plant: overset
+    (GPP) of every petiole onto the stem and every leaf blade onto its petiole, init(NumDomain),
+    setDomain, multi-domain setCRS(DOF=3), per-element stiffness (E 1000, nu 0.3: the Soybean
+    defaults), setEbOM(penalty 1e7, the default), gravity (rho 1), base of the stem fixed,
+    solve (BiCGSTAB, FP64, relres 1e-8)."""
+    from oracle import oracle as O
+    from plantfem_amd import _lib as L
+    from plantfem_amd import fem
+    from plantfem_amd import workloads as W
+    ctx = L.default_context()
+    P = W.plant_scaled(a.plant_scale)
+    E0, nu0, pen, rho = 1000.0, 0.3, 1.0e7, 1.0
+    stem_xyz = P.meshes[0][0]
+    base = np.nonzero(stem_xyz[:, 2] <= stem_xyz[:, 2].min())[0]
+    fix_ids = (3 * base[:, None] + np.arange(1, 4)).ravel().astype(np.int32)
+
+    def run(keep=False):
+        doms = [fem.FEMDomain(n, e, ctx) for n, e in P.meshes]
+        ctx.synchronize()
+        t = [time.perf_counter()]
+        for my, yours in P.pairs:
+            doms[my - 1].overset(doms[yours - 1], DomainID=yours, MyDomainID=my)
+        t.append(time.perf_counter())
+        s = fem.FEMSolver(ctx)
+        s.init(NumDomain=len(doms))
+        s.setDomain(FEMDomainPointers=doms)
+        s.setCRS(DOF=3)
+        ctx.synchronize()
+        t.append(time.perf_counter())
+        s.setStiffness([E0] * len(doms), [nu0] * len(doms))
+        ctx.synchronize()
+        t.append(time.perf_counter())
+        s.setEbOM(penalty=pen, DOF=3)
+        s.addBodyForce(Density=rho, Accel=(0.0, 0.0, -9.8))
+        s.fix(IDs=fix_ids, FixValue=0.0)
+        ctx.synchronize()
+        t.append(time.perf_counter())
+        x = s.solve(use_graph=0 if a.no_graph else 1)
+        t.append(time.perf_counter())
+        phases = dict(zip(("overset", "setCRS", "stiffness", "ebom_load_fix", "solve"), np.diff(t)))
+        return s, doms, x, phases, t[-1] - t[0]
+
+    for _ in range(max(a.warmup, 1)):
+        s, doms, x, _, _ = run()
+        s.close()
+    times, phases = [], []
+    for _ in range(a.steps):
+        s, doms, x, ph, tt = run()
+        times.append(tt)
+        phases.append(ph)
+        st = dict(s.last_stats)
+        if len(times) < a.steps:
+            s.close()
+    k = int(np.argmin(times))
+    out = {"metric": "overset multi-domain static solve: time from organ meshes to displacement", "unit": "s",
+           "value": round(float(np.median(times)), 4), "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "higher_is_better": False, "data": "synthetic", "dtype": "f64",
+           "config": {"workload": f"plant x{a.plant_scale}: stem + 8 petioles + 8 leaf blades (17 Cube3D domains, "
+                                  f"{P.meta['elements']} hex8, {P.meta['dof']} DOF), 16 overset pairs (GPP), "
+                                  "E 1000, nu 0.3, penalty 1e7, gravity, stem base fixed, relres 1e-8"},
+           "phases_s": {k2: round(float(v), 4) for k2, v in phases[k].items()},
+           "solve": {"iterations": st["iterations"], "status": st["status"], "true_relres": st["true_relres"]}}
+    ncon = sum(d.numOversetElements() for d in doms)
+    crs = s.getCRS()
+    out["system"] = {"connects": int(ncon), "nnz": int(crs.nnz), "n": int(crs.n)}
+    log(f"overset: {json.dumps(out)}")
+    # parity of the assembled system with the oracle (pattern and values bitwise)
+    nn = [m[0].shape[0] for m in P.meshes]
+    ne = [m[1].shape[0] for m in P.meshes]
+    noff, eoff = np.concatenate([[0], np.cumsum(nn)]), np.concatenate([[0], np.cumsum(ne)])
+    xyz = np.asfortranarray(np.concatenate([m[0] for m in P.meshes]))
+    en = np.asfortranarray(np.concatenate([m[1] + noff[i] for i, m in enumerate(P.meshes)]).astype(np.int32))
+    t0 = time.perf_counter()
+    own, gps, prs, poss = [], [], [], []
+    for my, yours in P.pairs:
+        me, gp, pe, pos = O.overset_gpp(*P.meshes[my - 1], *P.meshes[yours - 1])
+        own.append(me + eoff[my - 1])
+        gps.append(gp)
+        prs.append(pe + eoff[yours - 1])
+        poss.append(pos)
+    t_ov = time.perf_counter() - t0
+    own, gps, prs, poss = (np.concatenate(v) for v in (own, gps, prs, poss))
+    t0 = time.perf_counter()
+    rp, ci = O.multi_crs(xyz.shape[0], en, True, own, prs)
+    t_crs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    val = O.assemble(xyz, en, np.full(en.shape[0], E0), np.full(en.shape[0], nu0), rp, ci)
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    val = O.ebom(xyz, en, rp, ci, val, own, gps, prs, poss, pen)
+    t_ebom = time.perf_counter() - t0
+    s2 = fem.FEMSolver(ctx)
+    s2.init(NumDomain=len(doms))
+    s2.setDomain(FEMDomainPointers=doms)
+    s2.setCRS(DOF=3)
+    s2.setStiffness([E0] * len(doms), [nu0] * len(doms))
+    s2.setEbOM(penalty=pen, DOF=3)
+    c2 = s2.getCRS()
+    out["parity"] = {"connects_equal": bool(own.size == ncon),
+                     "pattern_bitwise": bool(np.array_equal(c2.row_ptr, rp) and np.array_equal(c2.col_idx, ci)),
+                     "values_bitwise": bool(np.array_equal(c2.val, val))}
+    s2.close()
+    if not a.no_cpu:
+        b = O.body_force(xyz, en, rho, [0.0, 0.0, -9.8])
+        fixed = np.zeros(3 * xyz.shape[0], np.uint8)
+        fixed[fix_ids - 1] = 1
+        v2, b2 = O.apply_dirichlet(rp, ci, val, b, fixed, np.zeros(b.size))
+        nth = os.cpu_count() or 1
+        t0 = time.perf_counter()
+        O.bicgstab(rp, ci, v2, b2, itrmax=a.cpu_iters * 10, er0=1e-300, relative_er=1e-300, nthreads=nth)
+        per_it = (time.perf_counter() - t0) / (a.cpu_iters * 10)
+        cpu_total = t_ov + t_crs + t_asm + t_ebom + per_it * st["iterations"]
+        out["cpu_baseline"] = {"value": round(cpu_total, 3), "unit": "s", "cores": nth, "kind": "port",
+                               "sample": f"oracle overset ({t_ov:.2f} s), setCRS ({t_crs:.2f} s), stiffness "
+                                         f"({t_asm:.2f} s), setEbOM ({t_ebom:.2f} s) in full; BiCGSTAB timed over "
+                                         f"{a.cpu_iters * 10} iterations ({per_it * 1e3:.2f} ms each) and scaled "
+                                         f"to the GPU's {st['iterations']}"}
+    s.close()
     print(json.dumps(out), flush=True)
 
 

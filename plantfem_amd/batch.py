@@ -4,7 +4,8 @@ Every organ mesh is solved on its own, as one single-domain FEMSolver_ per organ
 (setDomain / setCRS(DOF=3) / StiffnessMatrix / traction / fix / solve, FEMSolverClass.f90:
 329-694, :1697-1825).  (The reference's whole-plant Soybean%getDisplacement,
 SoybeanClass.f90:8417-8533, instead couples the organs into one multi-domain system with
-overset penalty terms, setEbOM; that coupling is not part of this form.)  Here the meshes are
+overset penalty terms, setEbOM; that coupling is FEMDomain.overset + FEMSolver.setEbOM, not
+this form.)  Here the meshes are
 concatenated into one block-diagonal system: node ids of mesh m are shifted by its first
 node, element lists and material arrays are concatenated, so graph build, element assembly,
 tractions and fixes run once for the whole batch, and every mesh's CRS block is the

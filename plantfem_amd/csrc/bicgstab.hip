@@ -1048,6 +1048,17 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
       pf_set_error("blocked SpMV requested but the CRS has no DOF=3 node-block structure");
       return PF_INVALID;
     }
+    if (layout == 1 && o.spmv_kind == 0) {
+      if (!w->have_s64) {
+        PF_CHECK(pf_sell3_build<double>(ctx, n, row_ptr, col_idx, val, w->s64));
+        w->have_s64 = true;
+      }
+      // the sliced copy pads every 64-node slice to its widest row: when row lengths vary a
+      // lot (the rows coupled by overset penalties) it outgrows the CRS, which then streams
+      // fewer bytes (matrices below a million entries stay on the sliced copy: launch
+      // latency, not bandwidth, sets their time)
+      if (nnz >= ((int64_t)1 << 20) && (double)w->s64.nslots * 64.0 * 9.0 > 1.5 * (double)nnz) layout = 0;
+    }
   }
   const bool mixed = o.precision == 32;
   // FP32 system: 0 A, 1 S A S (S = diag^-1/2, unknown scaled), 2 D^-1 A (rows only, restarted)

@@ -9,6 +9,9 @@
 !   FEMDomain_%select(x_min,...,z_max)                              FEMDomainClass.f90:12854-12926
 !   FEMSolver_%init/setDomain/setCRS/setRHS/setMatrix/setVector/fix/solve
 !                                                                   FEMSolverClass.f90:84-163
+!   FEMDomain_%overset(FEMDomain,DomainID,algorithm,MyDomainID)     FEMDomainClass.f90:14606-14700
+!   FEMSolver_%init(NumDomain)/setDomain(DomainID)/setCRS/setEbOM   FEMSolverClass.f90:466-676,
+!                                                                   :1891-1985 (several domains)
 ! plus pf_gpu_bicgstab, a subroutine with the exact LinearSolver interface of
 ! solveFEMSolver_UserDefinedLinearSolver (FEMSolverClass.f90:1638-1648), so the reference's own
 ! FEMSolver_ can hand its CRS system to the GPU: x = solver%solve(LinearSolver=pf_gpu_bicgstab).
@@ -19,7 +22,8 @@ module plantfem_gpu
    private
 
    public :: Mesh_, CRS_, FEMDomain_, FEMSolver_, pf_solve_opts, pf_stats
-   public :: pf_gpu_bicgstab, pf_gpu_bicgstab_func, pf_gpu_spmv, pf_gpu_version
+   public :: pf_gpu_bicgstab, pf_gpu_bicgstab_func, pf_gpu_spmv, pf_gpu_version, FEMDomainp_
+   integer(int32), parameter, public :: FEMDomain_Overset_GPP = 1, FEMDomain_Overset_P2P = 2
 
    integer(c_int), parameter, public :: PF_OK = 0, PF_BREAKDOWN_VV = 1, PF_BREAKDOWN_C2C3 = 2, &
                                         PF_NOT_CONVERGED = 3, PF_INVALID = 4
@@ -234,6 +238,36 @@ module plantfem_gpu
          integer(c_int32_t), intent(out) :: count
          type(pf_stats), intent(out) :: stats
       end function
+      integer(c_int) function c_pf_overset_gpp(ctx, nnA, neA, xyzA, enA, nnB, neB, xyzB, enB, cap, count, &
+                                                elem, gp, pair, pos) bind(C, name="pf_overset_gpp")
+         import :: c_int, c_int64_t, c_int32_t, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: nnA, neA, nnB, neB, cap
+         real(c_double), intent(in) :: xyzA(*), xyzB(*)
+         integer(c_int32_t), intent(in) :: enA(*), enB(*)
+         integer(c_int64_t), intent(out) :: count
+         integer(c_int32_t), intent(out) :: elem(*), gp(*), pair(*)
+         real(c_double), intent(out) :: pos(*)
+      end function
+      integer(c_int) function c_pf_system_multi_crs(sys, dof, any_overset, ncon, elem, pair, nnz) &
+         bind(C, name="pf_system_multi_crs")
+         import :: c_int, c_int64_t, c_int32_t, c_ptr
+         type(c_ptr), value :: sys
+         integer(c_int32_t), value :: dof, any_overset
+         integer(c_int64_t), value :: ncon
+         integer(c_int32_t), intent(in) :: elem(*), pair(*)
+         integer(c_int64_t), intent(out) :: nnz
+      end function
+      integer(c_int) function c_pf_system_ebom(sys, ncon, elem, gp, pair, pos, penalty, dof) &
+         bind(C, name="pf_system_ebom")
+         import :: c_int, c_int64_t, c_int32_t, c_double, c_ptr
+         type(c_ptr), value :: sys
+         integer(c_int64_t), value :: ncon
+         integer(c_int32_t), intent(in) :: elem(*), gp(*), pair(*)
+         real(c_double), intent(in) :: pos(*)
+         real(c_double), value :: penalty
+         integer(c_int32_t), value :: dof
+      end function
       function c_strlen(s) bind(C, name="strlen")
          import :: c_ptr, c_size_t
          type(c_ptr), value :: s
@@ -254,8 +288,16 @@ module plantfem_gpu
 
    type :: FEMDomain_
       type(Mesh_) :: mesh
+      ! OversetConnect_ list (GPP) as arrays: own element, Gauss point, paired element (ids in
+      ! the paired domain), position (3, k), DomainIDs12 (2, k); allocated by the first overset
+      integer(int32) :: num_oversetconnect = 0
+      integer(int32), allocatable :: ov_elem(:), ov_gp(:), ov_pair(:), ov_domains(:, :)
+      real(real64), allocatable :: ov_position(:, :)
    contains
       procedure :: create => createFEMDomain
+      procedure :: move => moveFEMDomain
+      procedure :: overset => oversetFEMDomain
+      procedure :: numOversetElements => numOversetElementsFEMDomain
       procedure :: nn => nnFEMDomain
       procedure :: ne => neFEMDomain
       procedure :: nne => nneFEMDomain
@@ -272,9 +314,18 @@ module plantfem_gpu
       procedure :: MassMatrix => MassMatrixFEMDomain
    end type
 
+   type :: FEMDomainp_
+      type(FEMDomain_), pointer :: femdomainp => null()
+   end type
+
    type :: FEMSolver_
       type(c_ptr) :: sys = c_null_ptr
       type(FEMDomain_), pointer :: domain => null()
+      ! several domains: slots, the concatenated mesh, element offsets, CRS_ID_Starts_From
+      type(FEMDomainp_), allocatable :: FEMDomains(:)
+      type(FEMDomain_), pointer :: joint => null()
+      integer(int32), allocatable :: elem_off(:)
+      integer(int64), allocatable :: CRS_ID_Starts_From(:)
       logical :: initialized = .false.
       logical :: debug = .false.
       integer(int32) :: itrmax = 100000
@@ -312,6 +363,7 @@ module plantfem_gpu
       procedure :: keepThisMatrixAs => keepThisMatrixAsFEMSolver
       procedure :: fix_eig => fix_eigFEMSolver
       procedure :: eig => eigFEMSolver
+      procedure :: setEbOM => setEbOMFEMSolver
    end type
 
    type(c_ptr), save :: g_ctx = c_null_ptr
@@ -562,14 +614,76 @@ contains
       call check(code, "StiffnessMatrix(YoungModulus,PoissonRatio)")
    end function
 
+   subroutine moveFEMDomain(this, x, y, z)
+      class(FEMDomain_), intent(inout) :: this
+      real(real64), optional, intent(in) :: x, y, z
+      if (present(x)) this%mesh%NodCoord(:, 1) = this%mesh%NodCoord(:, 1) + x
+      if (present(y)) this%mesh%NodCoord(:, 2) = this%mesh%NodCoord(:, 2) + y
+      if (present(z)) this%mesh%NodCoord(:, 3) = this%mesh%NodCoord(:, 3) + z
+   end subroutine
+
+   ! overset(FEMDomain, DomainID, algorithm, MyDomainID) (FEMDomainClass.f90:14606-14700): the
+   ! Gauss points of this mesh inside elements of FEMDomain become connects, searched on the GPU
+   subroutine oversetFEMDomain(this, FEMDomain, DomainID, algorithm, MyDomainID)
+      class(FEMDomain_), intent(inout) :: this
+      type(FEMDomain_), intent(in) :: FEMDomain
+      integer(int32), intent(in) :: DomainID, algorithm
+      integer(int32), optional, intent(in) :: MyDomainID
+      integer(c_int64_t) :: cap, k
+      integer(int32), allocatable :: e(:), g(:), p(:), ebuf(:), gbuf(:), pbuf(:), dbuf(:, :)
+      real(real64), allocatable :: x(:), xbuf(:, :)
+      integer(int32) :: my, n0
+      integer(c_int) :: code
+      if (algorithm /= FEMDomain_Overset_GPP) stop "plantfem_gpu: overset supports algorithm=FEMDomain_Overset_GPP"
+      my = 1
+      if (present(MyDomainID)) my = MyDomainID
+      cap = 8_c_int64_t*this%ne()
+      allocate (e(max(cap, 1_c_int64_t)), g(max(cap, 1_c_int64_t)), p(max(cap, 1_c_int64_t)), &
+                x(3*max(cap, 1_c_int64_t)))
+      code = c_pf_overset_gpp(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), this%mesh%NodCoord, &
+                              this%mesh%ElemNod, int(FEMDomain%nn(), c_int64_t), int(FEMDomain%ne(), c_int64_t), &
+                              FEMDomain%mesh%NodCoord, FEMDomain%mesh%ElemNod, cap, k, e, g, p, x)
+      call check(code, "overset")
+      n0 = this%num_oversetconnect
+      allocate (ebuf(n0 + k), gbuf(n0 + k), pbuf(n0 + k), dbuf(2, n0 + k), xbuf(3, n0 + k))
+      if (n0 > 0) then
+         ebuf(1:n0) = this%ov_elem
+         gbuf(1:n0) = this%ov_gp
+         pbuf(1:n0) = this%ov_pair
+         dbuf(:, 1:n0) = this%ov_domains
+         xbuf(:, 1:n0) = this%ov_position
+      end if
+      ebuf(n0 + 1:) = e(1:k)
+      gbuf(n0 + 1:) = g(1:k)
+      pbuf(n0 + 1:) = p(1:k)
+      dbuf(1, n0 + 1:) = my
+      dbuf(2, n0 + 1:) = DomainID
+      xbuf(:, n0 + 1:) = reshape(x(1:3*k), [3, int(k)])
+      call move_alloc(ebuf, this%ov_elem)
+      call move_alloc(gbuf, this%ov_gp)
+      call move_alloc(pbuf, this%ov_pair)
+      call move_alloc(dbuf, this%ov_domains)
+      call move_alloc(xbuf, this%ov_position)
+      this%num_oversetconnect = n0 + int(k)
+   end subroutine
+
+   integer(int32) function numOversetElementsFEMDomain(this)
+      class(FEMDomain_), intent(in) :: this
+      numOversetElementsFEMDomain = this%num_oversetconnect
+   end function
+
    ! ------------------------------------------------------------------------ FEMSolver_
+   ! init(NumDomain) (FEMSolverClass.f90:167-210)
    subroutine initFEMSolver(this, NumDomain)
       class(FEMSolver_), intent(inout) :: this
       integer(int32), optional, intent(in) :: NumDomain
       integer(c_int) :: code
-      if (present(NumDomain)) then
-         if (NumDomain /= 1) stop "plantfem_gpu: only the single-domain static path is supported"
-      end if
+      integer(int32) :: nd
+      nd = 1
+      if (present(NumDomain)) nd = NumDomain
+      if (nd < 1) stop "plantfem_gpu: init(NumDomain >= 1)"
+      if (allocated(this%FEMDomains)) deallocate (this%FEMDomains)
+      allocate (this%FEMDomains(nd))
       if (.not. c_associated(this%sys)) then
          code = c_pf_system_create(ctx(), this%sys)
          call check(code, "init")
@@ -577,19 +691,121 @@ contains
       this%initialized = .true.
    end subroutine
 
+   ! setDomain(FEMDomain, DomainID) (FEMSolverClass.f90:329-419)
    subroutine setDomainFEMSolver(this, FEMDomain, DomainID)
       class(FEMSolver_), intent(inout) :: this
       type(FEMDomain_), target, intent(in) :: FEMDomain
       integer(int32), optional, intent(in) :: DomainID
       integer(c_int) :: code
+      integer(int32) :: id
       if (.not. this%initialized) call this%init(NumDomain=1)
-      if (present(DomainID)) then
-         if (DomainID /= 1) stop "plantfem_gpu: only DomainID=1"
-      end if
+      id = 1
+      if (present(DomainID)) id = DomainID
+      if (id < 1 .or. id > size(this%FEMDomains)) stop "ERROR :: setFEMDomains >> invalid domain IDs"
+      this%FEMDomains(id)%femdomainp => FEMDomain
+      if (size(this%FEMDomains) >= 2) return  ! joined at setCRS
       this%domain => FEMDomain
       code = c_pf_system_set_mesh(this%sys, int(FEMDomain%nn(), c_int64_t), int(FEMDomain%ne(), c_int64_t), &
                                   int(FEMDomain%nne(), c_int32_t), FEMDomain%mesh%NodCoord, FEMDomain%mesh%ElemNod)
       call check(code, "setDomain")
+   end subroutine
+
+   ! multi-domain setCRS (FEMSolverClass.f90:466-676): the domains' meshes concatenated, the
+   ! ZeroMatrix blocks merged with the node pairs of every overset connect, columns ascending
+   subroutine setCRSMultiFEMSolver(this, DOF)
+      class(FEMSolver_), intent(inout) :: this
+      integer(int32), intent(in) :: DOF
+      integer(int32) :: nd, d, nn, ne, k, c
+      integer(int32), allocatable :: node_off(:), own(:), pair(:)
+      integer(c_int64_t) :: nnz
+      integer(c_int32_t) :: any_overset
+      integer(c_int) :: code
+      type(FEMDomain_), pointer :: f
+      nd = size(this%FEMDomains)
+      allocate (node_off(nd + 1))
+      if (allocated(this%elem_off)) deallocate (this%elem_off)
+      allocate (this%elem_off(nd + 1))
+      node_off(1) = 0
+      this%elem_off(1) = 0
+      do d = 1, nd
+         if (.not. associated(this%FEMDomains(d)%femdomainp)) stop "plantfem_gpu: setCRS with an empty domain slot"
+         node_off(d + 1) = node_off(d) + this%FEMDomains(d)%femdomainp%nn()
+         this%elem_off(d + 1) = this%elem_off(d) + this%FEMDomains(d)%femdomainp%ne()
+      end do
+      if (associated(this%joint)) deallocate (this%joint)
+      allocate (this%joint)
+      nn = node_off(nd + 1)
+      ne = this%elem_off(nd + 1)
+      allocate (this%joint%mesh%NodCoord(nn, 3), this%joint%mesh%ElemNod(ne, 8))
+      any_overset = 0
+      k = 0
+      do d = 1, nd
+         f => this%FEMDomains(d)%femdomainp
+         this%joint%mesh%NodCoord(node_off(d) + 1:node_off(d + 1), :) = f%mesh%NodCoord(:, 1:3)
+         this%joint%mesh%ElemNod(this%elem_off(d) + 1:this%elem_off(d + 1), :) = f%mesh%ElemNod(:, 1:8) + node_off(d)
+         if (allocated(f%ov_elem)) any_overset = 1
+         k = k + f%num_oversetconnect
+      end do
+      allocate (own(max(k, 1)), pair(max(k, 1)))
+      call connects_of(this, own, pair)
+      this%domain => this%joint
+      code = c_pf_system_set_mesh(this%sys, int(nn, c_int64_t), int(ne, c_int64_t), 8_c_int32_t, &
+                                  this%joint%mesh%NodCoord, this%joint%mesh%ElemNod)
+      call check(code, "setCRS (joint mesh)")
+      code = c_pf_system_multi_crs(this%sys, int(DOF, c_int32_t), any_overset, int(k, c_int64_t), own, pair, nnz)
+      call check(code, "setCRS")
+      if (allocated(this%CRS_ID_Starts_From)) deallocate (this%CRS_ID_Starts_From)
+      allocate (this%CRS_ID_Starts_From(nd))
+      do d = 1, nd
+         this%CRS_ID_Starts_From(d) = 1_int64 + int(DOF, int64)*node_off(d)
+      end do
+      this%DOF = DOF
+      this%n = int(DOF, int64)*nn
+      this%nnz = nnz
+      this%npend = 0
+      c = 0
+   end subroutine
+
+   ! every connect of every domain in domain order, as element ids of the joint mesh
+   subroutine connects_of(this, own, pair, gp, pos)
+      class(FEMSolver_), intent(in) :: this
+      integer(int32), intent(out) :: own(:), pair(:)
+      integer(int32), optional, intent(out) :: gp(:)
+      real(real64), optional, intent(out) :: pos(:, :)
+      integer(int32) :: d, i, k
+      type(FEMDomain_), pointer :: f
+      k = 0
+      do d = 1, size(this%FEMDomains)
+         f => this%FEMDomains(d)%femdomainp
+         do i = 1, f%num_oversetconnect
+            k = k + 1
+            own(k) = f%ov_elem(i) + this%elem_off(f%ov_domains(1, i))
+            pair(k) = f%ov_pair(i) + this%elem_off(f%ov_domains(2, i))
+            if (present(gp)) gp(k) = f%ov_gp(i)
+            if (present(pos)) pos(:, k) = f%ov_position(:, i)
+         end do
+      end do
+   end subroutine
+
+   ! setEbOM(penalty, DOF) (FEMSolverClass.f90:1891-1985)
+   subroutine setEbOMFEMSolver(this, penalty, DOF)
+      class(FEMSolver_), intent(inout) :: this
+      real(real64), intent(in) :: penalty
+      integer(int32), intent(in) :: DOF
+      integer(int32) :: d, k
+      integer(int32), allocatable :: own(:), pair(:), gp(:)
+      real(real64), allocatable :: pos(:, :)
+      integer(c_int) :: code
+      call this%flush()
+      if (size(this%FEMDomains) < 2) return
+      k = 0
+      do d = 1, size(this%FEMDomains)
+         k = k + this%FEMDomains(d)%femdomainp%num_oversetconnect
+      end do
+      allocate (own(max(k, 1)), pair(max(k, 1)), gp(max(k, 1)), pos(3, max(k, 1)))
+      call connects_of(this, own, pair, gp, pos)
+      code = c_pf_system_ebom(this%sys, int(k, c_int64_t), own, gp, pair, pos, penalty, int(DOF, c_int32_t))
+      call check(code, "setEbOM")
    end subroutine
 
    ! setCRS(DOF) single-domain branch (FEMSolverClass.f90:679-694)
@@ -598,6 +814,12 @@ contains
       integer(int32), intent(in) :: DOF
       integer(c_int64_t) :: nnz
       integer(c_int) :: code
+      if (allocated(this%FEMDomains)) then
+         if (size(this%FEMDomains) >= 2) then
+            call setCRSMultiFEMSolver(this, DOF)
+            return
+         end if
+      end if
       code = c_pf_system_zero_matrix(this%sys, int(DOF, c_int32_t), nnz)
       call check(code, "setCRS")
       this%DOF = DOF
@@ -643,8 +865,8 @@ contains
       real(real64), intent(in) :: Matrix(:, :)
       integer(int64), parameter :: batch = 8192
       integer(int64) :: m
-      if (DomainID /= 1) stop "plantfem_gpu: only DomainID=1"
       if (this%n == 0) call this%setCRS(DOF=DOF)
+      if (DomainID < 1 .or. DomainID > size(this%FEMDomains)) stop "plantfem_gpu: setMatrix with an invalid DomainID"
       m = size(Matrix, kind=int64)
       if (.not. allocated(this%pend_ids) .or. this%mat_len /= m) then
          call this%flush()
@@ -654,6 +876,7 @@ contains
       end if
       this%npend = this%npend + 1
       this%pend_ids(this%npend) = ElementID
+      if (allocated(this%elem_off)) this%pend_ids(this%npend) = ElementID + this%elem_off(DomainID)
       this%pend_mat((this%npend - 1)*m + 1:this%npend*m) = reshape(Matrix, [m])
       if (this%npend == batch) call this%flush()
    end subroutine
@@ -664,9 +887,10 @@ contains
       real(real64), intent(in) :: Vector(:)
       integer(c_int) :: code
       integer(int32) :: ids(1)
-      if (DomainID /= 1) stop "plantfem_gpu: only DomainID=1"
+      if (DomainID < 1 .or. DomainID > size(this%FEMDomains)) stop "plantfem_gpu: setVector with an invalid DomainID"
       call this%flush()
       ids(1) = ElementID
+      if (allocated(this%elem_off)) ids(1) = ElementID + this%elem_off(DomainID)
       code = c_pf_system_add_element_vectors(this%sys, 1_c_int64_t, ids, Vector)
       call check(code, "setVector")
    end subroutine
@@ -703,7 +927,8 @@ contains
       real(real64) :: fv
       type(c_ptr) :: pv
       if (present(DomainID)) then
-         if (DomainID /= 1) stop "plantfem_gpu: only DomainID=1"
+         ! the reference's DomainID >= 2 form marks IDs but stores values at IDs + earlier nodes
+         if (DomainID /= 1) stop "plantfem_gpu: fix(DomainID>=2) is not mirrored; pass DOF ids of the whole system"
       end if
       fv = 0.0d0
       if (present(FixValue)) fv = FixValue

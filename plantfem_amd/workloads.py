@@ -124,3 +124,64 @@ def inverse_problem(n: int = 60, ncases: int = 64, nobs: int = 2000, seed: int =
     surf = surface_nodes(nod)
     obs = np.sort(rng.choice(surf, size=min(nobs, surf.size), replace=False))
     return InverseProblem(nod, el, np.full(ne, 0.3), fixed, E, F, obs, noise, seed)
+
+
+@dataclass
+class Plant:
+    """Organ meshes of a synthetic plant and the overset pairs between them, in the form
+    Soybean%getDisplacement consumes (SoybeanClass.f90:8340-8533): domain 1 is the stem,
+    then petioles (overset onto the stem) and leaf blades (overset onto their petiole)."""
+    meshes: list            # [(nodcoord (nn,3), elemnod (ne,8))], domain order
+    pairs: list             # [(MyDomainID, DomainID)] in overset call order
+    meta: dict = field(default_factory=dict)
+
+
+def _shifted(xn, yn, zn, lx, ly, lz, origin):
+    nod, el = cube3d(xn, yn, zn, lx, ly, lz)
+    return np.asfortranarray(nod + np.asarray(origin, dtype=np.float64)), el
+
+
+def plant(stem=(6, 6, 30), h=0.01, petioles: int = 8, petiole_len: int = 8, petiole_cells: int = 6,
+          petiole_width: float = 0.03, leaf_cells: int = 8, leaf_layers: int = 3,
+          leaf_thickness: float = 0.02) -> Plant:
+    """A stem of stem[0] x stem[1] x stem[2] cells of size h standing on z = 0; petioles of
+    petiole_len cells of length h and a square section petiole_width wide (petiole_cells cells
+    across) leave it at rising heights towards +x, -x, +y, -y in turn, each ending under a
+    square leaf blade (leaf_cells x leaf_cells cells of size h, leaf_layers through the
+    thickness).  Small offsets keep every mesh's Gauss points off the faces of the mesh it
+    overlaps."""
+    sx, sy, sz = stem
+    meshes = [_shifted(sx, sy, sz, sx * h, sy * h, sz * h, (0.0, 0.0, 0.0))]
+    pairs = []
+    cx, cy = 0.5 * sx * h, 0.5 * sy * h
+    L, t, lw = petiole_len * h, petiole_width, leaf_cells * h
+    for k in range(petioles):
+        z = (0.35 + 0.55 * k / max(petioles, 1)) * sz * h
+        d = k % 4
+        if d == 0:
+            org, dims = (cx + 0.0013, cy - t / 2 + 0.0007, z), (petiole_len, petiole_cells, petiole_cells, L, t, t)
+            tip = np.array([org[0] + L, cy, z + t / 2])
+        elif d == 1:
+            org, dims = (cx - L - 0.0011, cy - t / 2 + 0.0009, z), (petiole_len, petiole_cells, petiole_cells, L, t, t)
+            tip = np.array([org[0], cy, z + t / 2])
+        elif d == 2:
+            org, dims = (cx - t / 2 + 0.0006, cy + 0.0017, z), (petiole_cells, petiole_len, petiole_cells, t, L, t)
+            tip = np.array([cx, org[1] + L, z + t / 2])
+        else:
+            org, dims = (cx - t / 2 + 0.0004, cy - L - 0.0012, z), (petiole_cells, petiole_len, petiole_cells, t, L, t)
+            tip = np.array([cx, org[1], z + t / 2])
+        meshes.append(_shifted(*dims, org))
+        pairs.append((len(meshes), 1))
+        lorg = tip - np.array([lw / 2 + 0.0008, lw / 2 + 0.0011, 0.0013])
+        meshes.append(_shifted(leaf_cells, leaf_cells, leaf_layers, lw, lw, leaf_thickness, tuple(lorg)))
+        pairs.append((len(meshes), len(meshes) - 1))
+    nn = sum(m[0].shape[0] for m in meshes)
+    ne = sum(m[1].shape[0] for m in meshes)
+    return Plant(meshes, pairs, {"domains": len(meshes), "nodes": nn, "elements": ne, "dof": 3 * nn})
+
+
+def plant_scaled(scale: int = 1) -> Plant:
+    """plant() refined scale times in every direction (same geometry, scale^3 more cells)."""
+    s = int(scale)
+    return plant(stem=(6 * s, 6 * s, 30 * s), h=0.01 / s, petiole_len=8 * s, petiole_cells=6 * s,
+                 leaf_cells=8 * s, leaf_layers=3 * s)

@@ -10,7 +10,7 @@ BIN = ROOT / "plantfem_amd" / "lib" / "bin"
 
 
 def test_fortran_binaries_built_and_link():
-    for name in ("cube_static", "plugin_solver", "modal_bar"):
+    for name in ("cube_static", "plugin_solver", "modal_bar", "overset_organs"):
         assert (BIN / name).exists(), name
     out = subprocess.run([str(BIN / "cube_static"), "--version"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0 and "plantfem_gpu library version" in out.stdout
@@ -94,3 +94,38 @@ def test_fortran_modal_bar_matches_scipy(dims, nev):
         ref = ref[:nev]
     assert lam.size == ref.size
     np.testing.assert_allclose(lam, ref, rtol=1e-7 if nev else 1e-9, atol=0)
+
+
+@pytest.mark.gpu
+def test_fortran_overset_organs_matches_oracle(tmp_path):
+    # two overlapping organs through overset / init(NumDomain=2) / setEbOM, against the oracle's
+    # restatement of the same sequence
+    from oracle import oracle as O
+    from test_overset import box, joint
+    out = tmp_path / "x.bin"
+    print("fortran overset_organs", flush=True)
+    res = subprocess.run([str(BIN / "overset_organs"), str(out)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = np.fromfile(out, dtype=np.int64, count=2)
+    x = np.fromfile(out, dtype=np.float64)[2:]
+    A = box(3, 3, 4, 1.0, 1.0, 2.0, (0.0, 0.0, 0.0))
+    B = box(3, 3, 3, 1.0, 1.0, 1.2, (0.05, 0.07, 1.43))
+    me, gp, pe, pos = O.overset_gpp(*A, *B)
+    xyz, en = joint(A, B)
+    pair = pe + A[1].shape[0]
+    rp, ci = O.multi_crs(xyz.shape[0], en, True, me, pair)
+    assert raw[0] == me.size and raw[1] == ci.size
+    ne = en.shape[0]
+    val = O.assemble(xyz, en, np.full(ne, 1.0e4), np.full(ne, 0.3), rp, ci)
+    val = O.ebom(xyz, en, rp, ci, val, me, gp, pair, pos, 1.0e6)
+    n = 3 * xyz.shape[0]
+    rhs = np.zeros(n)
+    top = A[0].shape[0] + np.nonzero(B[0][:, 2] > B[0][:, 2].max() - 1e-9)[0]
+    rhs[3 * top] = 0.1
+    fixed = np.zeros(n, np.uint8)
+    base = np.nonzero(A[0][:, 2] <= A[0][:, 2].min())[0]
+    for d in range(3):
+        fixed[3 * base + d] = 1
+    xo, st, *_ = O.solve(rp, ci, val, rhs, fixed, np.zeros(n), relative_er=1e-12, er0=1e-30)
+    assert st == 0 and x.size == n
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
