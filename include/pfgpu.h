@@ -245,22 +245,32 @@ int pf_overset_gpp(pf_ctx *ctx, int64_t nnA, int64_t neA, const double *nodcoord
                    int64_t nnB, int64_t neB, const double *nodcoordB, const int32_t *elemnodB, int64_t cap,
                    int64_t *count, int32_t *elem, int32_t *gauss_point, int32_t *pair_elem, double *position);
 
+/* FEMDomain%overset(..., algorithm=FEMDomain_Overset_P2P) (FEMDomainClass.f90:14701-14760):
+ * every node of mesh A that lies in an element of mesh B, in node order: own node id, element
+ * of B (1-based) and the node's position.  As pf_overset_gpp otherwise. */
+int pf_overset_p2p(pf_ctx *ctx, int64_t nnA, int64_t neA, const double *nodcoordA, const int32_t *elemnodA,
+                   int64_t nnB, int64_t neB, const double *nodcoordB, const int32_t *elemnodB, int64_t cap,
+                   int64_t *count, int32_t *node, int32_t *pair_elem, double *position);
+
 /* setCRS(DOF) with several domains (FEMSolverClass.f90:466-676) on the mesh given to
  * pf_system_set_mesh, which is the domains' meshes concatenated (node and element ids of
  * domain d offset by the sizes of domains 1..d-1; CRS_ID_Starts_From(d) = 1 + dof * nodes
  * before d).  The pattern is the union of the ZeroMatrix blocks and of every node pair of
- * the ncon connects (elem, pair_elem: element ids of the concatenated mesh, 1-based), each
- * row's columns ascending; values zero.  any_overset = 0 (no domain was overset) gives the
- * reference's empty CRS (every row empty).  pf_system_stiffness then assembles into this
- * pattern.  Returns nnz. */
+ * the ncon connects (elem, pair_elem: element ids of the concatenated mesh, 1-based; a
+ * connect with gauss_point 0 is a P2P one whose elem is the own NODE id and which couples 9
+ * nodes; gauss_point may be NULL when every connect is GPP), each row's columns ascending;
+ * values zero.  any_overset = 0 (no domain was overset) gives the reference's empty CRS (every
+ * row empty).  pf_system_stiffness then assembles into this pattern.  Returns nnz. */
 int pf_system_multi_crs(pf_system *sys, int32_t dof, int32_t any_overset, int64_t ncon, const int32_t *elem,
-                        const int32_t *pair_elem, int64_t *nnz);
+                        const int32_t *gauss_point, const int32_t *pair_elem, int64_t *nnz);
 
 /* setEbOM(penalty, DOF) (FEMSolverClass.f90:1891-1985): for each connect in order, adds
  * penalty * ConnectMatrix (FEMDomainClass.f90:13014-13100) over the 16 nodes of the own and
  * the paired element: A = (Bc^T Bc) detJ, Bc = [N_own(gp) I, -N_pair(xi) I], detJ of the own
  * Gauss point, xi the local coordinate of the position in the paired element.  Element ids of
- * the concatenated mesh (1-based), Gauss points 1..8, positions 3 per connect.  DOF = 3. */
+ * the concatenated mesh (1-based), Gauss points 1..8, positions 3 per connect.  Gauss point 0
+ * marks a P2P connect (elem = own node id): A = Bc^T Bc with Bc = [I, -N_pair(xi) I] over the
+ * node and the 8 paired nodes, no det J (FEMDomainClass.f90:13100-13127).  DOF = 3. */
 int pf_system_ebom(pf_system *sys, int64_t ncon, const int32_t *elem, const int32_t *gauss_point,
                    const int32_t *pair_elem, const double *position, double penalty, int32_t dof);
 

@@ -45,6 +45,8 @@ _SIG = {
     "orc_nearest_element": (I64, [I64, I64, PD, P32, PD]),
     "orc_overset_gpp": (I64, [I64, I64, PD, P32, I64, I64, PD, P32, P32, P32, P32, PD, I64]),
     "orc_local_coordinate": (None, [PD, PD, PD]),
+    "orc_overset_p2p": (I64, [I64, PD, I64, I64, PD, P32, P32, P32, PD, I64]),
+    "orc_connect_coeffs_p2p": (None, [PD, PD, PD]),
     "orc_connect_coeffs": (None, [PD, I32, PD, PD, PD, PD]),
     "orc_multi_crs": (I64, [I64, I64, P32, I32, I64, P32, P64, P32]),
     "orc_ebom": (I32, [I64, P64, P32, PD, I64, P32, PD, PD, D]),
@@ -354,6 +356,17 @@ def overset_gpp(nodA, enA, nodB, enB):
     return me[:k].copy(), gp[:k].copy(), pe[:k].copy(), pos[:3 * k].reshape(k, 3).copy()
 
 
+def overset_p2p(nodA, enA, nodB, enB):
+    """FEMDomain%overset(P2P) of mesh A onto mesh B: (node, pair_elem, position), 1-based ids."""
+    xa, xb, eb = _fl(nodA, np.float64), _fl(nodB, np.float64), _fl(enB, np.int32)
+    cap = nodA.shape[0]
+    nd, pe = np.empty(cap, np.int32), np.empty(cap, np.int32)
+    pos = np.empty(cap * 3)
+    k = load().orc_overset_p2p(nodA.shape[0], _p(xa, C.c_double), nodB.shape[0], enB.shape[0], _p(xb, C.c_double),
+                               _p(eb, C.c_int32), _p(nd, C.c_int32), _p(pe, C.c_int32), _p(pos, C.c_double), cap)
+    return nd[:k].copy(), pe[:k].copy(), pos[:3 * k].reshape(k, 3).copy()
+
+
 def local_coordinate(X, p):
     xi = np.empty(3)
     load().orc_local_coordinate(_p(_c(X, np.float64).ravel(), C.c_double), _p(_c(p, np.float64), C.c_double),
@@ -366,12 +379,21 @@ def _elem_X(nodcoord, elemnod, e):
 
 
 def connect_coeffs(nodcoord, elemnod, elem, gp, pair_elem, position):
-    """Per connect: c16 (16) = [N_own(gp), -N_pair(xi)] and detJ of the own Gauss point."""
+    """Per connect: c16 (16) = [N_own(gp), -N_pair(xi)] and detJ of the own Gauss point; a P2P
+    connect (gp 0, elem = own node) gets [1, -N_pair(xi), 0 ...] and detJ 1."""
     k = len(elem)
-    c16 = np.empty((k, 16))
+    c16 = np.zeros((k, 16))
     dj = np.empty(k)
     for c in range(k):
         d = np.empty(1)
+        if gp[c] == 0:
+            c9 = np.empty(9)
+            load().orc_connect_coeffs_p2p(
+                _p(_c(_elem_X(nodcoord, elemnod, pair_elem[c]), np.float64).ravel(), C.c_double),
+                _p(_c(position[c], np.float64), C.c_double), _p(c9, C.c_double))
+            c16[c, :9] = c9
+            dj[c] = 1.0
+            continue
         load().orc_connect_coeffs(_p(_c(_elem_X(nodcoord, elemnod, elem[c]), np.float64).ravel(), C.c_double),
                                   int(gp[c]),
                                   _p(_c(_elem_X(nodcoord, elemnod, pair_elem[c]), np.float64).ravel(), C.c_double),
@@ -381,15 +403,24 @@ def connect_coeffs(nodcoord, elemnod, elem, gp, pair_elem, position):
     return c16, dj
 
 
-def connect_nodes(elemnod, elem, pair_elem):
+def connect_nodes(elemnod, elem, pair_elem, gp=None):
+    """16 node ids per connect (1-based); P2P connects (gp 0): own node, 8 paired nodes, 0s."""
     en = np.asarray(elemnod, dtype=np.int32)
-    return np.concatenate([en[np.asarray(elem) - 1], en[np.asarray(pair_elem) - 1]], axis=1).astype(np.int32)
+    elem, pair = np.asarray(elem), np.asarray(pair_elem)
+    p2p = np.zeros(elem.size, bool) if gp is None else np.asarray(gp) == 0
+    g = ~p2p
+    out = np.zeros((elem.size, 16), np.int32)
+    out[g, :8] = en[elem[g] - 1]
+    out[g, 8:] = en[pair[g] - 1]
+    out[p2p, 0] = elem[p2p]
+    out[p2p, 1:9] = en[pair[p2p] - 1]
+    return out
 
 
-def multi_crs(nn, elemnod, any_overset, elem, pair_elem):
+def multi_crs(nn, elemnod, any_overset, elem, pair_elem, gp=None):
     """Multi-domain setCRS(DOF=3) pattern of the concatenated mesh: (row_ptr, col_idx)."""
     en = _fl(elemnod, np.int32)
-    nodes = _c(connect_nodes(elemnod, elem, pair_elem), np.int32).ravel() if len(elem) else np.zeros(1, np.int32)
+    nodes = _c(connect_nodes(elemnod, elem, pair_elem, gp), np.int32).ravel() if len(elem) else np.zeros(1, np.int32)
     ne = elemnod.shape[0]
     rp = np.empty(3 * nn + 1, np.int64)
     nnz = load().orc_multi_crs(nn, ne, _p(en, C.c_int32), int(bool(any_overset)), len(elem),
@@ -407,7 +438,7 @@ def ebom(nodcoord, elemnod, row_ptr, col_idx, val, elem, gp, pair_elem, position
     if k == 0:
         return val
     c16, dj = connect_coeffs(nodcoord, elemnod, elem, gp, pair_elem, position)
-    nodes = _c(connect_nodes(elemnod, elem, pair_elem), np.int32).ravel()
+    nodes = _c(connect_nodes(elemnod, elem, pair_elem, gp), np.int32).ravel()
     st = load().orc_ebom(row_ptr.size - 1, _p(_c(row_ptr, np.int64), C.c_int64),
                          _p(_c(col_idx, np.int32), C.c_int32), _p(val, C.c_double), k, _p(nodes, C.c_int32),
                          _p(_c(c16, np.float64).ravel(), C.c_double), _p(dj, C.c_double), float(penalty))

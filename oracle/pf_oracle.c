@@ -1016,6 +1016,37 @@ int64_t orc_overset_gpp(int64_t nnA, int64_t neA, const double *xyzA, const int3
     return count;
 }
 
+/* overset(FEMDomain, algorithm=P2P) (FEMDomainClass.f90:14701-14760): every node of A whose
+ * position lies in an element of B (nearestElementID), in node order: node (1-based), element
+ * of B (1-based), position.  Returns the count. */
+int64_t orc_overset_p2p(int64_t nnA, const double *xyzA, int64_t nnB, int64_t neB, const double *xyzB,
+                        const int32_t *enB, int32_t *node_out, int32_t *pair_elem, double *pos_out, int64_t cap)
+{
+    int64_t count = 0;
+    for (int64_t i = 0; i < nnA; ++i) {
+        const double p[3] = {xyzA[i], xyzA[nnA + i], xyzA[2 * nnA + i]};
+        const int64_t q = orc_nearest_element(nnB, neB, xyzB, enB, p);
+        if (q < 0) continue;
+        if (count < cap) {
+            node_out[count] = (int32_t)(i + 1);
+            pair_elem[count] = (int32_t)(q + 1);
+            for (int d = 0; d < 3; ++d) pos_out[count * 3 + d] = p[d];
+        }
+        ++count;
+    }
+    return count;
+}
+
+/* ConnectMatrix without a shape function (P2P, FEMDomainClass.f90:13100-13127): Bc = [I,
+ * -N_pair(xi) I] over the node and the 8 paired nodes; c9 = [1, 0 - N_pair(xi)] */
+void orc_connect_coeffs_p2p(const double *Xpair, const double *pos, double *c9)
+{
+    double xi[3];
+    orc_local_coordinate(Xpair, pos, xi);
+    c9[0] = 1.0;
+    for (int a = 0; a < 8; ++a) c9[1 + a] = 0.0 - hex8_N(a, xi);
+}
+
 /* getLocalCoordinate (hex8): xi = inverse(Jbar) (x - center), Jbar = 0.125 sum_gp J(gp),
  * center = 0.125 sum_a X_a; inverse as MathClass inverse (cofactors / det_mat) */
 void orc_local_coordinate(const double *X, const double *p, double *xi)
@@ -1068,7 +1099,7 @@ void orc_connect_coeffs(const double *Xmy, int gp /*1-based*/, const double *Xpa
 /* multi-domain setCRS pattern: the union of the domains' ZeroMatrix(DOF=3) rows (the union
  * mesh is block diagonal) and every node pair of every connect, columns ascending; all rows
  * empty when no overset was set up (any_overset = 0).  nodes16: per connect the 16 union node
- * ids (1-based).  Returns nnz (arrays written when col_idx != NULL, row_ptr n+1). */
+ * ids (1-based; 0 marks an unused slot of a 9-node P2P connect).  Returns nnz (arrays written when col_idx != NULL, row_ptr n+1). */
 int64_t orc_multi_crs(int64_t nn, int64_t ne, const int32_t *en, int any_overset, int64_t ncon, const int32_t *nodes16,
                       int64_t *row_ptr, int32_t *col_idx)
 {
@@ -1101,7 +1132,9 @@ int64_t orc_multi_crs(int64_t nn, int64_t ne, const int32_t *en, int any_overset
             for (int b = 0; b < 8; ++b) ADD_PAIR(en[(int64_t)a * ne + e] - 1, en[(int64_t)b * ne + e] - 1);
     for (int64_t c = 0; c < ncon; ++c)
         for (int j = 0; j < 16; ++j)
-            for (int k = 0; k < 16; ++k) ADD_PAIR(nodes16[c * 16 + j] - 1, nodes16[c * 16 + k] - 1);
+            for (int k = 0; k < 16; ++k)
+                if (nodes16[c * 16 + j] > 0 && nodes16[c * 16 + k] > 0)
+                    ADD_PAIR(nodes16[c * 16 + j] - 1, nodes16[c * 16 + k] - 1);
 #undef ADD_PAIR
     int64_t nnz = 0;
     row_ptr[0] = 1;
@@ -1140,6 +1173,7 @@ int orc_ebom(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *
             for (int k = 0; k < 16; ++k)
                 for (int m = 0; m < 3; ++m)
                     for (int nd = 0; nd < 3; ++nd) {
+                        if (nodes16[c * 16 + j] <= 0 || nodes16[c * 16 + k] <= 0) continue; /* P2P: 9 nodes */
                         double bt = 0.0;
                         for (int l = 0; l < 3; ++l) {
                             const double bp = l == m ? c16[c * 16 + j] : 0.0, bq = l == nd ? c16[c * 16 + k] : 0.0;

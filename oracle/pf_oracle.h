@@ -142,6 +142,9 @@ int64_t orc_nearest_element(int64_t nn, int64_t ne, const double *xyz, const int
 int64_t orc_overset_gpp(int64_t nnA, int64_t neA, const double *xyzA, const int32_t *enA, int64_t nnB, int64_t neB,
                         const double *xyzB, const int32_t *enB, int32_t *my_elem, int32_t *gp_out,
                         int32_t *pair_elem, double *pos_out, int64_t cap);
+int64_t orc_overset_p2p(int64_t nnA, const double *xyzA, int64_t nnB, int64_t neB, const double *xyzB,
+                        const int32_t *enB, int32_t *node_out, int32_t *pair_elem, double *pos_out, int64_t cap);
+void orc_connect_coeffs_p2p(const double *Xpair, const double *pos, double *c9);
 void orc_local_coordinate(const double *X, const double *p, double *xi);
 void orc_connect_coeffs(const double *Xmy, int gp, const double *Xpair, const double *pos, double *c16, double *detJ);
 int64_t orc_multi_crs(int64_t nn, int64_t ne, const int32_t *en, int any_overset, int64_t ncon, const int32_t *nodes16,

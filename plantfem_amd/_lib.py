@@ -165,7 +165,8 @@ SIGNATURES = {
     "pf_system_zeros": (C.c_int, [P]),
     "pf_system_keep_matrix": (C.c_int, [P, C.c_char_p]),
     "pf_overset_gpp": (C.c_int, [P, I64, I64, PF64, PI32, I64, I64, PF64, PI32, I64, PI64, PI32, PI32, PI32, PF64]),
-    "pf_system_multi_crs": (C.c_int, [P, I32, I32, I64, PI32, PI32, PI64]),
+    "pf_overset_p2p": (C.c_int, [P, I64, I64, PF64, PI32, I64, I64, PF64, PI32, I64, PI64, PI32, PI32, PF64]),
+    "pf_system_multi_crs": (C.c_int, [P, I32, I32, I64, PI32, PI32, PI32, PI64]),
     "pf_system_ebom": (C.c_int, [P, I64, PI32, PI32, PI32, PF64, F64, I32]),
     "pf_system_fix_eig": (C.c_int, [P, I64, PI32]),
     "pf_system_eig_count": (C.c_int, [P, C.POINTER(C.c_int32)]),

@@ -987,8 +987,10 @@ __global__ void __launch_bounds__(256) k_node_bbox(int64_t nn, const double *__r
   }
 }
 
-// one wavefront per (own element, Gauss point) w = 8e + gp: pair[w] = containing element of
-// the other mesh (0-based) or -1, pos[3w..] = the Gauss point's global position
+// GPP (P2P = false): one wavefront per (own element, Gauss point) w = 8e + gp; P2P: one per own
+// node w (the point is the node itself, FEMDomainClass.f90:14701-14760, no element filter).
+// pair[w] = containing element of the other mesh (0-based) or -1, pos[3w..] = the point.
+template <bool P2P>
 __global__ void __launch_bounds__(256) k_overset_gpp(int64_t nnA, int64_t neA, const double *__restrict__ xyzA,
                                                      const int32_t *__restrict__ enA, int64_t nnB, int64_t neB,
                                                      const double *__restrict__ xyzB,
@@ -999,31 +1001,37 @@ __global__ void __launch_bounds__(256) k_overset_gpp(int64_t nnA, int64_t neA, c
                                                      double *__restrict__ pos) {
   const int lane = threadIdx.x & 63;
   const int64_t w = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  if (w >= neA * 8) return;  // uniform per wavefront
-  const int64_t e = w >> 3;
-  const int gp = (int)(w & 7);
-  int32_t nodes[8];
-  double X[8][3];
-  load_element(nnA, neA, xyzA, enA, e, nodes, X);
-  // getElementList(xmin..zmax of the other mesh): element node range meets the box
+  if (w >= (P2P ? nnA : neA * 8)) return;  // uniform per wavefront
   bool hit = true;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double lo = X[0][d], hi = X[0][d];
-#pragma unroll
-    for (int a = 1; a < 8; ++a) {
-      lo = fmin(lo, X[a][d]);
-      hi = fmax(hi, X[a][d]);
-    }
-    hit = hit && (hi >= bb[d]) && (lo <= bb[3 + d]);
-  }
   double p[3];
+  if (P2P) {
+    p[0] = xyzA[w];
+    p[1] = xyzA[nnA + w];
+    p[2] = xyzA[2 * nnA + w];
+  } else {
+    const int64_t e = w >> 3;
+    const int gp = (int)(w & 7);
+    int32_t nodes[8];
+    double X[8][3];
+    load_element(nnA, neA, xyzA, enA, e, nodes, X);
+    // getElementList(xmin..zmax of the other mesh): element node range meets the box
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double s = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      double lo = X[0][d], hi = X[0][d];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) s = s + X[a][d] * hex8_N(a, gp);
-    p[d] = s;
+      for (int a = 1; a < 8; ++a) {
+        lo = fmin(lo, X[a][d]);
+        hi = fmax(hi, X[a][d]);
+      }
+      hit = hit && (hi >= bb[d]) && (lo <= bb[3 + d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 8; ++a) s = s + X[a][d] * hex8_N(a, gp);
+      p[d] = s;
+    }
   }
   hit = hit && (bb[0] <= p[0] && p[0] <= bb[3]) && (bb[1] <= p[1] && p[1] <= bb[4]) &&
         (bb[2] <= p[2] && p[2] <= bb[5]);
@@ -1082,15 +1090,16 @@ __global__ void k_overset_flags(int64_t m, const int32_t *__restrict__ pair, int
   flag[w] = pair[w] >= 0 ? 1 : 0;
 }
 
-// connects in (own element, Gauss point) order, ids 1-based
-__global__ void k_overset_scatter(int64_t m, const int32_t *__restrict__ pair, const double *__restrict__ pos,
-                                  const int64_t *__restrict__ off, int32_t *__restrict__ elem,
-                                  int32_t *__restrict__ gpo, int32_t *__restrict__ pe, double *__restrict__ po) {
+// connects in (own element, Gauss point) order, ids 1-based; P2P: (own node, 0)
+__global__ void k_overset_scatter(int64_t m, bool p2p, const int32_t *__restrict__ pair,
+                                  const double *__restrict__ pos, const int64_t *__restrict__ off,
+                                  int32_t *__restrict__ elem, int32_t *__restrict__ gpo, int32_t *__restrict__ pe,
+                                  double *__restrict__ po) {
   const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (w >= m || pair[w] < 0) return;
   const int64_t k = off[w];
-  elem[k] = (int32_t)(w >> 3) + 1;
-  gpo[k] = (int32_t)(w & 7) + 1;
+  elem[k] = p2p ? (int32_t)w + 1 : (int32_t)(w >> 3) + 1;
+  gpo[k] = p2p ? 0 : (int32_t)(w & 7) + 1;
   pe[k] = pair[w] + 1;
   po[3 * k] = pos[3 * w];
   po[3 * k + 1] = pos[3 * w + 1];
@@ -1108,13 +1117,14 @@ __global__ void __launch_bounds__(128) k_connect_coeffs(int64_t nn, int64_t ne, 
                                                         double *__restrict__ detj, int32_t *__restrict__ nodes16) {
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= ncon) return;
+  const bool p2p = gpid[c] == 0;  // P2P: my[c] is the own node (0-based)
   int32_t na[8], nb[8];
   double X[8][3], Y[8][3];
-  load_element(nn, ne, xyz, en, my[c], na, X);
+  if (!p2p) load_element(nn, ne, xyz, en, my[c], na, X);
   load_element(nn, ne, xyz, en, pr[c], nb, Y);
-  const int gp = gpid[c] - 1;
+  const int gp = p2p ? 0 : gpid[c] - 1;
   double G[3][8];
-  detj[c] = hex8_gp_G(X, gp, 0, G, false);
+  detj[c] = p2p ? 1.0 : hex8_gp_G(X, gp, 0, G, false);
   // local coordinate in the paired element
   double Jb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll 1
@@ -1152,6 +1162,21 @@ __global__ void __launch_bounds__(128) k_connect_coeffs(int64_t nn, int64_t ne, 
   double xi[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) xi[i] = (Ji[i][0] * x[0] + Ji[i][1] * x[1]) + Ji[i][2] * x[2];
+  if (p2p) {  // Bc = [I, -N_pair(xi) I] over the own node and the 8 paired nodes, no det J
+    c16[c * 16] = 1.0;
+    nodes16[c * 16] = my[c];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      c16[c * 16 + 1 + a] = 0.0 - hex8_N_xi(a, xi[0], xi[1], xi[2]);
+      nodes16[c * 16 + 1 + a] = nb[a] - 1;
+    }
+#pragma unroll
+    for (int a = 9; a < 16; ++a) {
+      c16[c * 16 + a] = 0.0;
+      nodes16[c * 16 + a] = -1;  // unused slot
+    }
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     c16[c * 16 + a] = 0.0 + hex8_N(a, gp);
@@ -1164,16 +1189,23 @@ __global__ void __launch_bounds__(128) k_connect_coeffs(int64_t nn, int64_t ne, 
 // node -> (connect, slot) lists: count / fill / sort (t = 16c + slot ascends with c)
 __global__ void k_con_count(int64_t total, const int32_t *__restrict__ nodes16, int32_t *__restrict__ cnt) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= total) return;
+  if (t >= total || nodes16[t] < 0) return;
   atomicAdd(&cnt[nodes16[t]], 1);
 }
 
-__global__ void k_con_fill(int64_t total, const int32_t *__restrict__ nodes16, const int64_t *__restrict__ off,
-                           int32_t *__restrict__ fill, int64_t *__restrict__ list) {
+// (node << 32 | 16c + slot) keys: sorted, they give every node its (connect, slot) list in
+// connect order
+__global__ void k_con_keys(int64_t total, const int32_t *__restrict__ nodes16, uint64_t *__restrict__ keys) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int32_t node = nodes16[t];
-  list[off[node] + atomicAdd(&fill[node], 1)] = t;
+  // unused slots sort behind every node's list (the node counts exclude them)
+  keys[t] = nodes16[t] < 0 ? ~(uint64_t)0 : ((uint64_t)(uint32_t)nodes16[t] << 32) | (uint64_t)(uint32_t)t;
+}
+
+__global__ void k_con_list(int64_t total, const uint64_t *__restrict__ keys, int64_t *__restrict__ list) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  list[t] = (int64_t)(keys[t] & 0xffffffffu);
 }
 
 __device__ inline int64_t find_col(const int32_t *__restrict__ col_idx, int64_t rb, int64_t re, int32_t col,
@@ -1192,42 +1224,61 @@ __device__ inline int64_t find_col(const int32_t *__restrict__ col_idx, int64_t 
   return -1;
 }
 
-// one thread per row r = 3i + m: the connects touching node i in order; per connect the 48
-// entries of its row of penalty * A (addMatrixValue, FEMSolverClass.f90:923-960)
-__global__ void __launch_bounds__(128) k_ebom_rows(int64_t nn, const int64_t *__restrict__ off,
-                                                   const int64_t *__restrict__ list,
-                                                   const int32_t *__restrict__ nodes16,
-                                                   const double *__restrict__ c16, const double *__restrict__ detj,
-                                                   double penalty, const int64_t *__restrict__ row_ptr,
-                                                   const int32_t *__restrict__ col_idx, bool sorted,
-                                                   double *__restrict__ val, int32_t *__restrict__ err) {
-  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// rows longer than this are updated by one lane straight in global memory
+constexpr int EB_MAXROW = 2048;
+
+// one wavefront per row r = 3i + m: the connects touching node i in connect order; within a
+// connect lane 3k + n adds entry (3j + m, 3k + n) of penalty * A (addMatrixValue,
+// FEMSolverClass.f90:923-960) to the row held in LDS.  The 48 columns of one connect are
+// distinct, and the barrier between connects keeps every entry's sum in connect order.
+__global__ void __launch_bounds__(64) k_ebom_rows(int64_t nn, const int64_t *__restrict__ off,
+                                                  const int64_t *__restrict__ list,
+                                                  const int32_t *__restrict__ nodes16,
+                                                  const double *__restrict__ c16, const double *__restrict__ detj,
+                                                  double penalty, const int64_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ col_idx, bool sorted,
+                                                  double *__restrict__ val, int32_t *__restrict__ err) {
+  __shared__ double sv[EB_MAXROW];
+  const int lane = threadIdx.x;
+  const int64_t r = blockIdx.x;
   if (r >= 3 * nn) return;
   const int64_t i = r / 3;
   const int m = (int)(r - 3 * i);
+  const int64_t s0 = off[i], s1 = off[i + 1];
+  if (s0 == s1) return;
   const int64_t rb = row_ptr[r] - 1, re = row_ptr[r + 1] - 1;
-  for (int64_t s = off[i]; s < off[i + 1]; ++s) {
+  const int64_t len = re - rb;
+  const bool in_lds = len <= EB_MAXROW;
+  if (!in_lds && lane != 0) return;
+  if (in_lds) {
+    for (int64_t q = lane; q < len; q += 64) sv[q] = val[rb + q];
+    __syncthreads();
+  }
+  for (int64_t s = s0; s < s1; ++s) {
     const int64_t c = list[s] >> 4;
     const int j = (int)(list[s] & 15);
     const double cj = c16[c * 16 + j], dj = detj[c];
-    for (int k = 0; k < 16; ++k) {
+    for (int kn = in_lds ? lane : 0; kn < 48; kn += in_lds ? 64 : 1) {
+      const int k = kn / 3, n = kn - 3 * (kn / 3);
+      if (nodes16[c * 16 + k] < 0) continue;  // P2P connects use 9 of the 16 slots
       const double ck = c16[c * 16 + k];
-      const int32_t nk = nodes16[c * 16 + k];
-      for (int n = 0; n < 3; ++n) {
-        // (Bc^T Bc)(3j+m, 3k+n) = sum_l Bc(l,3j+m) Bc(l,3k+n), Bc(l, 3q+d) = c_q [l == d]
-        double s3 = 0.0;
+      // (Bc^T Bc)(3j+m, 3k+n) = sum_l Bc(l,3j+m) Bc(l,3k+n), Bc(l, 3q+d) = c_q [l == d]
+      double s3 = 0.0;
 #pragma unroll
-        for (int l = 0; l < 3; ++l) s3 = s3 + (l == m ? cj : 0.0) * (l == n ? ck : 0.0);
-        const double v = penalty * (s3 * dj);
-        const int64_t id = find_col(col_idx, rb, re, 3 * nk + n + 1, sorted);
-        if (id < 0) {
-          atomicOr(err, 1);
-          continue;
-        }
-        val[id] = val[id] + v;
+      for (int l = 0; l < 3; ++l) s3 = s3 + (l == m ? cj : 0.0) * (l == n ? ck : 0.0);
+      const double v = penalty * (s3 * dj);
+      const int64_t id = find_col(col_idx, rb, re, 3 * nodes16[c * 16 + k] + n + 1, sorted);
+      if (id < 0) {
+        atomicOr(err, 1);
+        continue;
       }
+      if (in_lds) sv[id - rb] = sv[id - rb] + v;
+      else val[id] = val[id] + v;
     }
+    if (in_lds) __syncthreads();
   }
+  if (in_lds)
+    for (int64_t q = lane; q < len; q += 64) val[rb + q] = sv[q];
 }
 
 // hex8 stiffness into any CRS holding the mesh's node blocks: one thread per row, incident
@@ -1265,15 +1316,15 @@ __global__ void __launch_bounds__(128) k_assemble_rows_search(
 
 }  // namespace
 
-int pf_overset_search(pf_ctx *ctx, const NodeGraph &A, const NodeGraph &B, DevBuf<int32_t> &elem,
+int pf_overset_search(pf_ctx *ctx, const NodeGraph &A, const NodeGraph &B, bool p2p, DevBuf<int32_t> &elem,
                       DevBuf<int32_t> &gp, DevBuf<int32_t> &pair_elem, DevBuf<double> &position, int64_t *count) {
   *count = 0;
   if (A.nne != 8 || B.nne != 8 || !A.has_coords || !B.has_coords || !B.built) {
     pf_set_error("overset: two hex8 meshes with coordinates (and the node->element lists of the other)");
     return PF_INVALID;
   }
-  if (A.ne == 0 || B.nn == 0 || B.ne == 0) return PF_OK;
-  const int64_t m = A.ne * 8;
+  if ((p2p ? A.nn : A.ne) == 0 || B.nn == 0 || B.ne == 0) return PF_OK;
+  const int64_t m = p2p ? A.nn : A.ne * 8;
   DevBuf<double> bb, pos;
   DevBuf<int32_t> pair, flag;
   DevBuf<int64_t> off, tmp;
@@ -1283,9 +1334,14 @@ int pf_overset_search(pf_ctx *ctx, const NodeGraph &A, const NodeGraph &B, DevBu
   PF_CHECK(flag.ensure((size_t)m + 1));
   PF_CHECK(off.ensure((size_t)m + 1));
   k_node_bbox<<<1, 256, 0, ctx->stream>>>(B.nn, B.nodcoord.p, bb.p);
-  k_overset_gpp<<<pf_blocks(m * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
-      A.nn, A.ne, A.nodcoord.p, A.elemnod.p, B.nn, B.ne, B.nodcoord.p, B.elemnod.p, B.eoff.p, B.elist.p, bb.p,
-      pair.p, pos.p);
+  if (p2p)
+    k_overset_gpp<true><<<pf_blocks(m * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+        A.nn, A.ne, A.nodcoord.p, A.elemnod.p, B.nn, B.ne, B.nodcoord.p, B.elemnod.p, B.eoff.p, B.elist.p, bb.p,
+        pair.p, pos.p);
+  else
+    k_overset_gpp<false><<<pf_blocks(m * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+        A.nn, A.ne, A.nodcoord.p, A.elemnod.p, B.nn, B.ne, B.nodcoord.p, B.elemnod.p, B.eoff.p, B.elist.p, bb.p,
+        pair.p, pos.p);
   k_overset_flags<<<pf_blocks(m, 256, INT32_MAX), 256, 0, ctx->stream>>>(m, pair.p, flag.p);
   PF_HIP(hipGetLastError());
   PF_CHECK(pf_scan_i32_to_i64(ctx, flag.p, m, off.p, tmp));
@@ -1296,8 +1352,8 @@ int pf_overset_search(pf_ctx *ctx, const NodeGraph &A, const NodeGraph &B, DevBu
   PF_CHECK(pair_elem.ensure((size_t)k + 1));
   PF_CHECK(position.ensure((size_t)k * 3 + 1));
   if (k > 0)
-    k_overset_scatter<<<pf_blocks(m, 256, INT32_MAX), 256, 0, ctx->stream>>>(m, pair.p, pos.p, off.p, elem.p, gp.p,
-                                                                            pair_elem.p, position.p);
+    k_overset_scatter<<<pf_blocks(m, 256, INT32_MAX), 256, 0, ctx->stream>>>(m, p2p, pair.p, pos.p, off.p, elem.p,
+                                                                            gp.p, pair_elem.p, position.p);
   PF_HIP(hipGetLastError());
   PF_HIP(hipStreamSynchronize(ctx->stream));
   *count = k;
@@ -1312,6 +1368,10 @@ int pf_ebom_assemble(pf_ctx *ctx, const NodeGraph &g, int64_t ncon, const int32_
     return PF_INVALID;
   }
   if (ncon <= 0) return PF_OK;
+  if (ncon >= ((int64_t)1 << 28)) {
+    pf_set_error("setEbOM: at most 2^28 connects per call");
+    return PF_INVALID;
+  }
   DevBuf<double> c16, detj;
   DevBuf<int32_t> nodes16, cnt;
   DevBuf<int64_t> off, list, tmp;
@@ -1327,11 +1387,17 @@ int pf_ebom_assemble(pf_ctx *ctx, const NodeGraph &g, int64_t ncon, const int32_
   PF_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (g.nn + 1), ctx->stream));
   k_con_count<<<pf_blocks(total, 256, INT32_MAX), 256, 0, ctx->stream>>>(total, nodes16.p, cnt.p);
   PF_CHECK(pf_scan_i32_to_i64(ctx, cnt.p, g.nn, off.p, tmp));
-  PF_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (g.nn + 1), ctx->stream));
-  k_con_fill<<<pf_blocks(total, 256, INT32_MAX), 256, 0, ctx->stream>>>(total, nodes16.p, off.p, cnt.p, list.p);
-  k_batch_sort<<<pf_blocks(g.nn, 256, INT32_MAX), 256, 0, ctx->stream>>>(g.nn, off.p, list.p);
-  k_ebom_rows<<<pf_blocks(3 * g.nn, 128, INT32_MAX), 128, 0, ctx->stream>>>(
-      g.nn, off.p, list.p, nodes16.p, c16.p, detj.p, penalty, row_ptr, col_idx, sorted, val, err);
+  DevBuf<uint64_t> ka, kb;
+  PF_CHECK(ka.ensure((size_t)total));
+  PF_CHECK(kb.ensure((size_t)total));
+  k_con_keys<<<pf_blocks(total, 256, INT32_MAX), 256, 0, ctx->stream>>>(total, nodes16.p, ka.p);
+  PF_HIP(hipGetLastError());
+  const int end_bit = 64;  // the unused-slot keys set every bit
+  bool in_b = false;
+  PF_CHECK(pf_sort_u64(ctx, ka.p, kb.p, total, end_bit, &in_b));
+  k_con_list<<<pf_blocks(total, 256, INT32_MAX), 256, 0, ctx->stream>>>(total, in_b ? kb.p : ka.p, list.p);
+  k_ebom_rows<<<(unsigned)(3 * g.nn), 64, 0, ctx->stream>>>(g.nn, off.p, list.p, nodes16.p, c16.p, detj.p, penalty,
+                                                          row_ptr, col_idx, sorted, val, err);
   PF_HIP(hipGetLastError());
   PF_HIP(hipStreamSynchronize(ctx->stream));
   return PF_OK;

@@ -1278,15 +1278,15 @@ static int check_hex8_mesh_args(int64_t nn, int64_t ne, const double *xyz, const
   return PF_OK;
 }
 
-extern "C" int pf_overset_gpp(pf_ctx *ctx, int64_t nnA, int64_t neA, const double *nodcoordA,
-                              const int32_t *elemnodA, int64_t nnB, int64_t neB, const double *nodcoordB,
-                              const int32_t *elemnodB, int64_t cap, int64_t *count, int32_t *elem,
-                              int32_t *gauss_point, int32_t *pair_elem, double *position) {
+static int overset_common(pf_ctx *ctx, bool p2p, int64_t nnA, int64_t neA, const double *nodcoordA,
+                          const int32_t *elemnodA, int64_t nnB, int64_t neB, const double *nodcoordB,
+                          const int32_t *elemnodB, int64_t cap, int64_t *count, int32_t *own, int32_t *gauss_point,
+                          int32_t *pair_elem, double *position) {
   PF_CHECK(check_ctx(ctx));
   PF_CHECK(check_hex8_mesh_args(nnA, neA, nodcoordA, elemnodA));
   PF_CHECK(check_hex8_mesh_args(nnB, neB, nodcoordB, elemnodB));
-  if (!count || cap < 0 || (cap > 0 && (!elem || !gauss_point || !pair_elem || !position))) {
-    pf_set_error("pf_overset_gpp: count and (for cap > 0) the four output arrays required");
+  if (!count || cap < 0 || (cap > 0 && (!own || (!p2p && !gauss_point) || !pair_elem || !position))) {
+    pf_set_error("overset: count and (for cap > 0) the output arrays required");
     return PF_INVALID;
   }
   DeviceGuard guard(ctx);
@@ -1298,35 +1298,59 @@ extern "C" int pf_overset_gpp(pf_ctx *ctx, int64_t nnA, int64_t neA, const doubl
   DevBuf<int32_t> e, g, p;
   DevBuf<double> x;
   int64_t k = 0;
-  PF_CHECK(pf_overset_search(ctx, A, B, e, g, p, x, &k));
+  PF_CHECK(pf_overset_search(ctx, A, B, p2p, e, g, p, x, &k));
   *count = k;
   const int64_t m = std::min(k, cap);
   if (m > 0) {
-    PF_CHECK(download(ctx, elem, e.p, (size_t)m));
-    PF_CHECK(download(ctx, gauss_point, g.p, (size_t)m));
+    PF_CHECK(download(ctx, own, e.p, (size_t)m));
+    if (!p2p) PF_CHECK(download(ctx, gauss_point, g.p, (size_t)m));
     PF_CHECK(download(ctx, pair_elem, p.p, (size_t)m));
     PF_CHECK(download(ctx, position, x.p, (size_t)m * 3));
   }
   return PF_OK;
 }
 
-static int check_connects(const pf_system *s, int64_t ncon, const int32_t *elem, const int32_t *pair_elem) {
-  for (int64_t c = 0; c < ncon; ++c)
-    if (elem[c] < 1 || elem[c] > s->g.ne || pair_elem[c] < 1 || pair_elem[c] > s->g.ne) {
-      pf_set_error("connect %lld: element id outside 1..%lld", (long long)c + 1, (long long)s->g.ne);
+extern "C" int pf_overset_gpp(pf_ctx *ctx, int64_t nnA, int64_t neA, const double *nodcoordA,
+                              const int32_t *elemnodA, int64_t nnB, int64_t neB, const double *nodcoordB,
+                              const int32_t *elemnodB, int64_t cap, int64_t *count, int32_t *elem,
+                              int32_t *gauss_point, int32_t *pair_elem, double *position) {
+  return overset_common(ctx, false, nnA, neA, nodcoordA, elemnodA, nnB, neB, nodcoordB, elemnodB, cap, count, elem,
+                        gauss_point, pair_elem, position);
+}
+
+extern "C" int pf_overset_p2p(pf_ctx *ctx, int64_t nnA, int64_t neA, const double *nodcoordA,
+                              const int32_t *elemnodA, int64_t nnB, int64_t neB, const double *nodcoordB,
+                              const int32_t *elemnodB, int64_t cap, int64_t *count, int32_t *node,
+                              int32_t *pair_elem, double *position) {
+  return overset_common(ctx, true, nnA, neA, nodcoordA, elemnodA, nnB, neB, nodcoordB, elemnodB, cap, count, node,
+                        nullptr, pair_elem, position);
+}
+
+// own: element id (GPP) or node id (P2P, Gauss point id 0) of the joint mesh, 1-based
+static int check_connects(const pf_system *s, int64_t ncon, const int32_t *elem, const int32_t *gauss_point,
+                          const int32_t *pair_elem) {
+  for (int64_t c = 0; c < ncon; ++c) {
+    const bool p2p = gauss_point && gauss_point[c] == 0;
+    const int64_t lim = p2p ? s->g.nn : s->g.ne;
+    if (elem[c] < 1 || elem[c] > lim || pair_elem[c] < 1 || pair_elem[c] > s->g.ne ||
+        (gauss_point && (gauss_point[c] < 0 || gauss_point[c] > 8))) {
+      pf_set_error("connect %lld: own %s id, paired element id or Gauss point id out of range", (long long)c + 1,
+                   p2p ? "node" : "element");
       return PF_INVALID;
     }
+  }
   return PF_OK;
 }
 
 extern "C" int pf_system_multi_crs(pf_system *s, int32_t dof, int32_t any_overset, int64_t ncon,
-                                   const int32_t *elem, const int32_t *pair_elem, int64_t *nnz) {
+                                   const int32_t *elem, const int32_t *gauss_point, const int32_t *pair_elem,
+                                   int64_t *nnz) {
   PF_CHECK(sys_check(s, false));
   if (!s->has_mesh || s->g.nne != 8 || dof < 1 || ncon < 0 || (ncon > 0 && (!elem || !pair_elem))) {
     pf_set_error("pf_system_multi_crs: hex8 mesh, dof >= 1 and the connect arrays required");
     return PF_INVALID;
   }
-  PF_CHECK(check_connects(s, ncon, elem, pair_elem));
+  PF_CHECK(check_connects(s, ncon, elem, gauss_point, pair_elem));
   pf_ctx *ctx = s->ctx;
   DeviceGuard guard(ctx);
   s->dof = dof;
@@ -1339,7 +1363,7 @@ extern "C" int pf_system_multi_crs(pf_system *s, int32_t dof, int32_t any_overse
     PF_CHECK(s->col_idx.ensure(1));
     PF_CHECK(s->val.ensure(1));
   } else {
-    DevBuf<int32_t> dmy, dpr;
+    DevBuf<int32_t> dmy, dpr, dgp;
     if (ncon > 0) {
       std::vector<int32_t> a((size_t)ncon), b((size_t)ncon);
       for (int64_t c = 0; c < ncon; ++c) {
@@ -1348,9 +1372,10 @@ extern "C" int pf_system_multi_crs(pf_system *s, int32_t dof, int32_t any_overse
       }
       PF_CHECK(upload(ctx, dmy, a.data(), a.size()));
       PF_CHECK(upload(ctx, dpr, b.data(), b.size()));
+      if (gauss_point) PF_CHECK(upload(ctx, dgp, gauss_point, (size_t)ncon));
     }
     NodeGraph merged;
-    PF_CHECK(pf_graph_merged_cols(ctx, s->g, ncon, dmy.p, dpr.p, merged));
+    PF_CHECK(pf_graph_merged_cols(ctx, s->g, ncon, dmy.p, dpr.p, gauss_point ? dgp.p : nullptr, merged));
     s->nnz = nnz_of(merged, dof);
     if (s->nnz >= (int64_t)1 << 40) {
       pf_set_error("CRS too large");
@@ -1382,12 +1407,7 @@ extern "C" int pf_system_ebom(pf_system *s, int64_t ncon, const int32_t *elem, c
     pf_set_error("pf_system_ebom: DOF = 3 (3-D hex8 displacement) is the supported form");
     return PF_INVALID;
   }
-  PF_CHECK(check_connects(s, ncon, elem, pair_elem));
-  for (int64_t c = 0; c < ncon; ++c)
-    if (gauss_point[c] < 1 || gauss_point[c] > 8) {
-      pf_set_error("connect %lld: Gauss point id outside 1..8", (long long)c + 1);
-      return PF_INVALID;
-    }
+  PF_CHECK(check_connects(s, ncon, elem, gauss_point, pair_elem));
   if (ncon == 0) return PF_OK;
   pf_ctx *ctx = s->ctx;
   DeviceGuard guard(ctx);

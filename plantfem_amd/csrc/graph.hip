@@ -250,16 +250,25 @@ __global__ void k_keys_graph(int64_t nn, const int64_t *__restrict__ noff, const
   for (int64_t q = noff[i]; q < noff[i + 1]; ++q) keys[q] = ((uint64_t)i << 32) | (uint32_t)ncol[q];
 }
 
+// 1-based node of slot j of connect c: GPP slots 0-7 own element, 8-15 paired element; P2P
+// (gp == 0) slot 0 the own node my[c] (0-based), 1-8 the paired element, 9-15 repeat slot 0
+__device__ inline int32_t connect_node(int64_t ne, const int32_t *__restrict__ en, int32_t my, int32_t pair,
+                                       bool p2p, int j) {
+  if (p2p) return j >= 1 && j <= 8 ? en[(int64_t)(j - 1) * ne + pair] : my + 1;
+  return j < 8 ? en[(int64_t)j * ne + my] : en[(int64_t)(j - 8) * ne + pair];
+}
+
 // connect c couples element my[c] and element pair[c] (0-based) of one mesh: 16 x 16 pairs
 __global__ void k_keys_connect(int64_t ncon, int64_t ne, const int32_t *__restrict__ en,
                                const int32_t *__restrict__ my, const int32_t *__restrict__ pair,
-                               uint64_t *__restrict__ keys) {
+                               const int32_t *__restrict__ gp, uint64_t *__restrict__ keys) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ncon * 256) return;
   const int64_t c = t >> 8;
   const int j = (int)((t >> 4) & 15), k = (int)(t & 15);
-  const int32_t rj = j < 8 ? en[(int64_t)j * ne + my[c]] : en[(int64_t)(j - 8) * ne + pair[c]];
-  const int32_t ck = k < 8 ? en[(int64_t)k * ne + my[c]] : en[(int64_t)(k - 8) * ne + pair[c]];
+  const bool p2p = gp && gp[c] == 0;
+  const int32_t rj = connect_node(ne, en, my[c], pair[c], p2p, j);
+  const int32_t ck = connect_node(ne, en, my[c], pair[c], p2p, k);
   keys[t] = ((uint64_t)(uint32_t)(rj - 1) << 32) | (uint32_t)(ck - 1);
 }
 
@@ -278,7 +287,7 @@ __global__ void k_keys_cols(int64_t u, const uint64_t *__restrict__ keys, int32_
 }  // namespace
 
 int pf_graph_merged_cols(pf_ctx *ctx, NodeGraph &g, int64_t ncon, const int32_t *my, const int32_t *pair,
-                         NodeGraph &out) {
+                         const int32_t *gp, NodeGraph &out) {
   if (g.nne != 8) {
     pf_set_error("overset connects need an 8-node mesh");
     return PF_INVALID;
@@ -299,7 +308,7 @@ int pf_graph_merged_cols(pf_ctx *ctx, NodeGraph &g, int64_t ncon, const int32_t 
   if (g.nn > 0) k_keys_graph<<<pf_blocks(g.nn, 256, INT32_MAX), 256, 0, ctx->stream>>>(g.nn, g.noff.p, g.ncol.p, ka.p);
   if (ncon > 0)
     k_keys_connect<<<pf_blocks(ncon * 256, 256, INT32_MAX), 256, 0, ctx->stream>>>(ncon, g.ne, g.elemnod.p, my,
-                                                                                  pair, ka.p + g.nblocks);
+                                                                                  pair, gp, ka.p + g.nblocks);
   PF_HIP(hipGetLastError());
   int end_bit = 33;
   while (end_bit < 64 && ((int64_t)1 << (end_bit - 32)) < g.nn) ++end_bit;
@@ -322,5 +331,18 @@ int pf_graph_merged_cols(pf_ctx *ctx, NodeGraph &g, int64_t ncon, const int32_t 
   PF_HIP(hipStreamSynchronize(ctx->stream));  // scratch buffers are freed on return
   out.nblocks = u;
   out.cols_built = true;
+  return PF_OK;
+}
+
+int pf_sort_u64(pf_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int end_bit, bool *in_tmp) {
+  *in_tmp = false;
+  if (n <= 1) return PF_OK;
+  size_t bytes = 0;
+  PF_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys, tmp, n, 0, end_bit, ctx->stream));
+  DevBuf<unsigned char> cub;
+  PF_CHECK(cub.ensure(bytes + 1));
+  PF_HIP(hipcub::DeviceRadixSort::SortKeys(cub.p, bytes, keys, tmp, n, 0, end_bit, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));  // cub scratch is freed on return
+  *in_tmp = true;
   return PF_OK;
 }

@@ -123,11 +123,15 @@ int pf_graph_node_elems(pf_ctx *ctx, NodeGraph &g);                 // eoff/elis
 int pf_graph_node_cols(pf_ctx *ctx, NodeGraph &g);                  // deg/noff/ncol
 int pf_graph_expand(pf_ctx *ctx, const NodeGraph &g, int dof, int64_t *row_ptr, int32_t *col_idx);
 int pf_graph_check_mesh(pf_ctx *ctx, const NodeGraph &g);           // node ids in range
-// node neighbour lists of the multi-domain setCRS: g's element pairs and the 16 x 16 node
-// pairs of every overset connect (my/pair: 0-based element ids of g, device), sorted and
+// node neighbour lists of the multi-domain setCRS: g's element pairs and the node pairs of
+// every overset connect (my/pair: 0-based element ids of g, device; gp: Gauss point ids, 0 =
+// P2P, where my is the own node and the connect has 9 nodes; NULL = all GPP), sorted and
 // distinct; fills out.deg/noff/ncol/nblocks for pf_graph_expand
 int pf_graph_merged_cols(pf_ctx *ctx, NodeGraph &g, int64_t ncon, const int32_t *my, const int32_t *pair,
-                         NodeGraph &out);
+                         const int32_t *gp, NodeGraph &out);
+// ascending radix sort of n 64-bit keys over bits [0, end_bit): the result lands in tmp
+// (*in_tmp = true) unless n <= 1
+int pf_sort_u64(pf_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int end_bit, bool *in_tmp);
 
 // assemble.hip
 int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
@@ -144,10 +148,12 @@ int pf_traction_box(pf_ctx *ctx, const NodeGraph &g, const double *box, double p
 // overset GPP: the Gauss points of A's elements located in elements of B, in (element,
 // Gauss point) order: 1-based own element, Gauss point, element of B, and the position.
 // B needs its node->element lists (pf_graph_node_elems).
-int pf_overset_search(pf_ctx *ctx, const NodeGraph &A, const NodeGraph &B, DevBuf<int32_t> &elem,
+// p2p: the points are A's nodes instead (elem = own node, gp = 0), as overset(P2P)
+int pf_overset_search(pf_ctx *ctx, const NodeGraph &A, const NodeGraph &B, bool p2p, DevBuf<int32_t> &elem,
                       DevBuf<int32_t> &gp, DevBuf<int32_t> &pair_elem, DevBuf<double> &position, int64_t *count);
-// setEbOM: penalty * ConnectMatrix of ncon connects (0-based own/paired elements of g, 1-based
-// Gauss point ids, positions) added row by row in connect order; err set for a missing slot
+// setEbOM: penalty * ConnectMatrix of ncon connects (0-based own element, or own node when the
+// Gauss point id is 0 (P2P); 0-based paired element; positions) added row by row in connect
+// order; err set for a missing slot
 int pf_ebom_assemble(pf_ctx *ctx, const NodeGraph &g, int64_t ncon, const int32_t *my, const int32_t *gp,
                      const int32_t *pr, const double *pos, double penalty, const int64_t *row_ptr,
                      const int32_t *col_idx, bool sorted, double *val, int32_t *err);

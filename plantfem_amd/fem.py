@@ -53,6 +53,76 @@ def cube3d(x_num: int = 10, y_num: int = 10, z_num: int = 10, x_len: float = 1.0
     return nod, elem
 
 
+PF_GLYCINE_MAX = 1  # MeshClass.f90:6
+
+
+def leaf3d(x_num: int = 10, y_num: int = 10, z_num: int = 10, x_len: float = 1.0, y_len: float = 1.0,
+           z_len: float = 1.0, shaperatio: float = 0.3, species=None, SoyWidthRatio: float = 0.5):
+    """Mesh%create(meshtype="Leaf3D") (MeshClass.f90:6017-6276): the rectangular3D mesh (Cube3D
+    numbering; x = width, y = thickness, z = length) centred in x and y, then shaped along z.
+
+    Default (MeshClass.f90:6246-6272): half-width ratio 1/10 up to z = zl/20 (where the
+    thickness grows to width/10), rising linearly to 1 at z = shaperatio*zl, falling linearly to
+    1/10 at the tip.  species=PF_GLYCINE_MAX (:6033-6094): the soybean outline (circular arc to
+    the widest point alpha = SoyWidthRatio*width on one side, (1-SoyWidthRatio)*width on the
+    other, an arc of radius 1.2 r to the tip, each blended half-and-half with a straight line),
+    a fat base (x/10, thickness x5 below z = length/25) and the curl y -= 0.1 theta (theta - 1).
+    The soybean branch decides each node's side from the x extent of the nodes as already
+    moved, so it runs node by node in id order."""
+    nod, el = cube3d(x_num, y_num, z_num, x_len, y_len, z_len)
+    x, y, z = nod[:, 0].copy(), nod[:, 1].copy(), nod[:, 2].copy()
+    x = x - (x.max() - x.min()) * 0.50
+    y = y - (y.max() - y.min()) * 0.50
+    if species is None:
+        thickness_ = y.max() - y.min()
+        width = x.max() - x.min()
+        thickness_ratio = (width * 1.0 / 10.0) / thickness_
+        zl = z.max() - z.min()
+        s = float(shaperatio)
+        z20 = 1.0 / 20.0 * zl
+        base = z <= z20
+        rise = (~base) & (z <= zl * s)
+        ratio = np.where(base, 1.0 / 10.0,
+                         np.where(rise, 1.0 / 10.0 + 0.90 / (zl * s - z20) * (z - z20),
+                                  1.0 - 0.90 / (zl - s * zl) * (z - s * zl)))
+        y = np.where(base, y * thickness_ratio, y)
+        x = x * ratio
+    elif species == PF_GLYCINE_MAX:
+        lin_curve_ratio = 0.50
+        length = z.max() - z.min()
+        width = x.max() - x.min()
+        sw = float(SoyWidthRatio)
+        if sw >= 1.0 or sw <= 0.0:
+            raise ValueError(f"mesh%create(leaf3d, PF_SOYBEAN) >> invalid SoyWidthRatio {SoyWidthRatio}")
+        thickness_ratio = 5.0
+        xmax, xmin = x.max(), x.min()
+        for i in range(x.size):
+            xx = z[i]
+            alpha = sw * width if x[i] <= (xmax + xmin) * 0.50 else (1.0 - sw) * width
+            r = (alpha * alpha + (length - alpha) * (length - alpha)) / (2 * alpha) * 1.20
+            if xx <= 1.0 / 25.0 * length:
+                x[i] = x[i] * 1.0 / 10.0
+                y[i] = y[i] * thickness_ratio
+            else:
+                if xx < alpha:
+                    yy = np.sqrt(alpha * alpha - (xx - alpha) * (xx - alpha))
+                    yy_ = xx
+                else:
+                    yy_ = alpha + (-alpha) / (length - alpha) * (xx - alpha)
+                    yy = alpha - r + np.sqrt(r * r - (xx - alpha) * (xx - alpha))
+                yy = abs(lin_curve_ratio * yy + (1.0 - lin_curve_ratio) * yy_)
+                x[i] = x[i] * (yy / alpha)
+            # the reference re-reads maxval/minval of the moved coordinates for the next node
+            xmax, xmin = x.max(), x.min()
+        theta = np.abs(x / (width / 2.0))
+        y = y - 0.10 * theta * (theta - 1.0)
+    else:
+        raise ValueError(f"Mesh%create => No such species as {species}")
+    out = np.empty_like(nod, order="F")
+    out[:, 0], out[:, 1], out[:, 2] = x, y, z
+    return out, el
+
+
 @dataclass
 class CRS:
     """CRS_ (SparseClass.f90:128-131): 1-based int64 row_ptr, int32 col_idx, float64 val."""
@@ -91,11 +161,13 @@ FEMDomain_Overset_P2P = 2
 
 
 class OversetConnect:
-    """The OversetConnect_ list of one domain as arrays (GPP): ElementID, GaussPointID (1-based),
+    """The OversetConnect_ list of one domain as arrays: ElementID (own element, GPP; 0 for P2P),
+    NodeID (own node, P2P; 0 for GPP), GaussPointID (1-based; 0 marks P2P as in the reference),
     the paired element (1-based, in the paired domain), position (k, 3) and DomainIDs12 as
     (own, paired) per connect; all connects are active."""
 
     def __init__(self):
+        self.NodeID = np.zeros(0, np.int32)
         self.ElementID = np.zeros(0, np.int32)
         self.GaussPointID = np.zeros(0, np.int32)
         self.PairElementID = np.zeros(0, np.int32)
@@ -106,8 +178,11 @@ class OversetConnect:
     def n(self) -> int:
         return int(self.ElementID.size)
 
-    def append(self, elem, gp, pair, pos, my_id: int, pair_id: int):
-        k = len(elem)
+    def append(self, elem, gp, pair, pos, my_id: int, pair_id: int, node=None):
+        k = len(pair)
+        elem = np.zeros(k, np.int32) if elem is None else elem
+        node = np.zeros(k, np.int32) if node is None else node
+        self.NodeID = np.concatenate([self.NodeID, np.asarray(node, np.int32)])
         self.ElementID = np.concatenate([self.ElementID, np.asarray(elem, np.int32)])
         self.GaussPointID = np.concatenate([self.GaussPointID, np.asarray(gp, np.int32)])
         self.PairElementID = np.concatenate([self.PairElementID, np.asarray(pair, np.int32)])
@@ -133,9 +208,8 @@ class FEMDomain:
         domain ids [MyDomainID (default 1), DomainID]), appended in the reference's order.  The
         search runs on the GPU (pf_overset_gpp)."""
         alg = FEMDomain_Overset_GPP if algorithm is None else algorithm
-        if alg in ("P2P", FEMDomain_Overset_P2P):
-            raise NotImplementedError("overset: the point-to-point (P2P) projection is not built; use GPP")
-        if alg not in ("GPP", FEMDomain_Overset_GPP):
+        p2p = alg in ("P2P", FEMDomain_Overset_P2P)
+        if not p2p and alg not in ("GPP", FEMDomain_Overset_GPP):
             raise ValueError(f"overset: unknown algorithm {algorithm!r}")
         my_id = 1 if MyDomainID is None else int(MyDomainID)
         if self.OversetConnect is None:
@@ -143,16 +217,23 @@ class FEMDomain:
         xa, ea = self._flat()
         xb, eb = FEMDomain._flat()
         lib = L.load()
-        cap = 8 * self.ne()
+        cap = self.nn() if p2p else 8 * self.ne()
         elem, gp, pe = (np.empty(max(cap, 1), np.int32) for _ in range(3))
         pos = np.empty(max(cap, 1) * 3)
         count = C.c_int64(0)
-        L.check(lib.pf_overset_gpp(self._ctx().h, self.nn(), self.ne(), L.ptr(xa, C.c_double), L.ptr(ea, C.c_int32),
-                                   FEMDomain.nn(), FEMDomain.ne(), L.ptr(xb, C.c_double), L.ptr(eb, C.c_int32),
-                                   cap, C.byref(count), L.ptr(elem, C.c_int32), L.ptr(gp, C.c_int32),
-                                   L.ptr(pe, C.c_int32), L.ptr(pos, C.c_double)), "overset")
-        k = count.value
-        self.OversetConnect.append(elem[:k], gp[:k], pe[:k], pos[:3 * k].reshape(k, 3), my_id, int(DomainID))
+        meshes = (self._ctx().h, self.nn(), self.ne(), L.ptr(xa, C.c_double), L.ptr(ea, C.c_int32),
+                  FEMDomain.nn(), FEMDomain.ne(), L.ptr(xb, C.c_double), L.ptr(eb, C.c_int32), cap, C.byref(count))
+        if p2p:
+            L.check(lib.pf_overset_p2p(*meshes, L.ptr(elem, C.c_int32), L.ptr(pe, C.c_int32),
+                                       L.ptr(pos, C.c_double)), "overset")
+            k = count.value
+            self.OversetConnect.append(None, np.zeros(k, np.int32), pe[:k], pos[:3 * k].reshape(k, 3), my_id,
+                                       int(DomainID), node=elem[:k])
+        else:
+            L.check(lib.pf_overset_gpp(*meshes, L.ptr(elem, C.c_int32), L.ptr(gp, C.c_int32), L.ptr(pe, C.c_int32),
+                                       L.ptr(pos, C.c_double)), "overset")
+            k = count.value
+            self.OversetConnect.append(elem[:k], gp[:k], pe[:k], pos[:3 * k].reshape(k, 3), my_id, int(DomainID))
         return self
 
     def numOversetElements(self) -> int:
@@ -162,11 +243,23 @@ class FEMDomain:
         return self.ctx or L.default_context()
 
     def create(self, meshtype: str = "Cube3D", x_num=None, y_num=None, z_num=None, x_len=None, y_len=None,
-               z_len=None):
-        if meshtype not in ("Cube3D", "Cube", "cube3D"):
-            raise ValueError(f"meshtype {meshtype!r}: only Cube3D is on this path")
-        self.nodcoord, self.elemnod = cube3d(x_num or 10, y_num or 10, z_num or 10, x_len or 1.0, y_len or 1.0,
-                                             z_len or 1.0)
+               z_len=None, shaperatio: float = 0.3, species=None, SoyWidthRatio: float = 0.5):
+        """create(meshtype, x_num, y_num, z_num, x_len, y_len, z_len) (FEMDomainClass.f90:7215-7310):
+        "Cube3D" / "Cube" (the same numbering) or "Leaf3D" (organ mesh of LeafClass, with
+        shaperatio or species=PF_GLYCINE_MAX and SoyWidthRatio)."""
+        args = (x_num or 10, y_num or 10, z_num or 10, x_len or 1.0, y_len or 1.0, z_len or 1.0)
+        if meshtype in ("Cube3D", "Cube", "cube3D", "rectangular3D"):
+            self.nodcoord, self.elemnod = cube3d(*args)
+        elif meshtype == "Leaf3D":
+            self.nodcoord, self.elemnod = leaf3d(*args, shaperatio=shaperatio, species=species,
+                                                 SoyWidthRatio=SoyWidthRatio)
+        else:
+            raise ValueError(f"meshtype {meshtype!r}: Cube3D, Cube and Leaf3D are on this path")
+        return self
+
+    def move(self, x: float = 0.0, y: float = 0.0, z: float = 0.0):
+        """move(x, y, z): translate every node."""
+        self.nodcoord = np.asfortranarray(self.nodcoord + np.array([x, y, z], dtype=np.float64))
         return self
 
     # sizes (FEMDomainClass.f90:10117-10173)
@@ -471,7 +564,9 @@ class FEMSolver:
             any_overset = True
             if oc.n == 0:
                 continue
-            own.append(oc.ElementID + self._elem_off[oc.DomainIDs12[:, 0] - 1].astype(np.int32))
+            p2p = oc.GaussPointID == 0  # own node of the joint mesh instead of the own element
+            own.append(np.where(p2p, oc.NodeID + self._node_off[oc.DomainIDs12[:, 0] - 1],
+                                oc.ElementID + self._elem_off[oc.DomainIDs12[:, 0] - 1]).astype(np.int32))
             pr.append(oc.PairElementID + self._elem_off[oc.DomainIDs12[:, 1] - 1].astype(np.int32))
             gp.append(oc.GaussPointID)
             pos.append(oc.position)
@@ -497,9 +592,9 @@ class FEMSolver:
             xyz, en = u._flat()
             L.check(lib.pf_system_set_mesh(self.h, u.nn(), u.ne(), u.nne(), L.ptr(xyz, C.c_double),
                                            L.ptr(en, C.c_int32)), "setCRS")
-            any_ov, own, _, pr, _ = self._connects()
+            any_ov, own, gp, pr, _ = self._connects()
             L.check(lib.pf_system_multi_crs(self.h, int(DOF), int(any_ov), own.size, L.ptr(own, C.c_int32),
-                                            L.ptr(pr, C.c_int32), C.byref(nnz)), "setCRS")
+                                            L.ptr(gp, C.c_int32), L.ptr(pr, C.c_int32), C.byref(nnz)), "setCRS")
             self.CRS_ID_Starts_From = (1 + int(DOF) * self._node_off[:-1]).astype(np.int64)
         else:
             L.check(lib.pf_system_zero_matrix(self.h, int(DOF), C.byref(nnz)), "setCRS")

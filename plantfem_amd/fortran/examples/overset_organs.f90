@@ -2,7 +2,8 @@
 ! Soybean%getDisplacement drives FEMSolver_ (SoybeanClass.f90:8340-8533): overset, init with
 ! several domains, setDomain, setCRS, per-domain setMatrix, setEbOM, fix, solve.
 !
-!   overset_organs out.bin    writes the connect count, nnz and x (real64 stream)
+!   overset_organs out.bin [P2P]   writes the connect count, nnz and x (real64 stream);
+!                                  P2P couples the lower organ's nodes instead of its Gauss points
 program overset_organs
    use plantfem_gpu
    use iso_fortran_env, only: int32, int64, real64
@@ -12,14 +13,18 @@ program overset_organs
    integer(int32), allocatable :: FixBoundary(:)
    real(real64), allocatable :: x(:), rhs(:)
    integer(int32) :: d, e, i, u, nn1
-   character(len=256) :: outfile
+   character(len=256) :: outfile, alg
+   integer(int32) :: algorithm
 
    call get_command_argument(1, outfile)
+   call get_command_argument(2, alg)
+   algorithm = FEMDomain_Overset_GPP
+   if (trim(alg) == "P2P") algorithm = FEMDomain_Overset_P2P
    call organ(1)%create("Cube3D", x_num=3, y_num=3, z_num=4, x_len=1.0d0, y_len=1.0d0, z_len=2.0d0)
    call organ(2)%create("Cube3D", x_num=3, y_num=3, z_num=3, x_len=1.0d0, y_len=1.0d0, z_len=1.2d0)
    call organ(2)%move(x=0.05d0, y=0.07d0, z=1.43d0)
 
-   call organ(1)%overset(FEMDomain=organ(2), DomainID=2, MyDomainID=1, algorithm=FEMDomain_Overset_GPP)
+   call organ(1)%overset(FEMDomain=organ(2), DomainID=2, MyDomainID=1, algorithm=algorithm)
 
    call solver%init(NumDomain=2)
    call solver%setDomain(FEMDomain=organ(1), DomainID=1)

@@ -249,13 +249,24 @@ module plantfem_gpu
          integer(c_int32_t), intent(out) :: elem(*), gp(*), pair(*)
          real(c_double), intent(out) :: pos(*)
       end function
-      integer(c_int) function c_pf_system_multi_crs(sys, dof, any_overset, ncon, elem, pair, nnz) &
+      integer(c_int) function c_pf_overset_p2p(ctx, nnA, neA, xyzA, enA, nnB, neB, xyzB, enB, cap, count, &
+                                                node, pair, pos) bind(C, name="pf_overset_p2p")
+         import :: c_int, c_int64_t, c_int32_t, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: nnA, neA, nnB, neB, cap
+         real(c_double), intent(in) :: xyzA(*), xyzB(*)
+         integer(c_int32_t), intent(in) :: enA(*), enB(*)
+         integer(c_int64_t), intent(out) :: count
+         integer(c_int32_t), intent(out) :: node(*), pair(*)
+         real(c_double), intent(out) :: pos(*)
+      end function
+      integer(c_int) function c_pf_system_multi_crs(sys, dof, any_overset, ncon, elem, gp, pair, nnz) &
          bind(C, name="pf_system_multi_crs")
          import :: c_int, c_int64_t, c_int32_t, c_ptr
          type(c_ptr), value :: sys
          integer(c_int32_t), value :: dof, any_overset
          integer(c_int64_t), value :: ncon
-         integer(c_int32_t), intent(in) :: elem(*), pair(*)
+         integer(c_int32_t), intent(in) :: elem(*), gp(*), pair(*)
          integer(c_int64_t), intent(out) :: nnz
       end function
       integer(c_int) function c_pf_system_ebom(sys, ncon, elem, gp, pair, pos, penalty, dof) &
@@ -288,8 +299,9 @@ module plantfem_gpu
 
    type :: FEMDomain_
       type(Mesh_) :: mesh
-      ! OversetConnect_ list (GPP) as arrays: own element, Gauss point, paired element (ids in
-      ! the paired domain), position (3, k), DomainIDs12 (2, k); allocated by the first overset
+      ! OversetConnect_ list as arrays: own element (GPP) or own node (P2P), Gauss point (0 for
+      ! P2P), paired element (ids in the paired domain), position (3, k), DomainIDs12 (2, k);
+      ! allocated by the first overset
       integer(int32) :: num_oversetconnect = 0
       integer(int32), allocatable :: ov_elem(:), ov_gp(:), ov_pair(:), ov_domains(:, :)
       real(real64), allocatable :: ov_position(:, :)
@@ -324,7 +336,7 @@ module plantfem_gpu
       ! several domains: slots, the concatenated mesh, element offsets, CRS_ID_Starts_From
       type(FEMDomainp_), allocatable :: FEMDomains(:)
       type(FEMDomain_), pointer :: joint => null()
-      integer(int32), allocatable :: elem_off(:)
+      integer(int32), allocatable :: elem_off(:), node_off(:)
       integer(int64), allocatable :: CRS_ID_Starts_From(:)
       logical :: initialized = .false.
       logical :: debug = .false.
@@ -634,15 +646,27 @@ contains
       real(real64), allocatable :: x(:), xbuf(:, :)
       integer(int32) :: my, n0
       integer(c_int) :: code
-      if (algorithm /= FEMDomain_Overset_GPP) stop "plantfem_gpu: overset supports algorithm=FEMDomain_Overset_GPP"
+      if (algorithm /= FEMDomain_Overset_GPP .and. algorithm /= FEMDomain_Overset_P2P) &
+         stop "plantfem_gpu: overset algorithm is FEMDomain_Overset_GPP or FEMDomain_Overset_P2P"
       my = 1
       if (present(MyDomainID)) my = MyDomainID
-      cap = 8_c_int64_t*this%ne()
+      if (algorithm == FEMDomain_Overset_P2P) then
+         cap = int(this%nn(), c_int64_t)
+      else
+         cap = 8_c_int64_t*this%ne()
+      end if
       allocate (e(max(cap, 1_c_int64_t)), g(max(cap, 1_c_int64_t)), p(max(cap, 1_c_int64_t)), &
                 x(3*max(cap, 1_c_int64_t)))
-      code = c_pf_overset_gpp(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), this%mesh%NodCoord, &
-                              this%mesh%ElemNod, int(FEMDomain%nn(), c_int64_t), int(FEMDomain%ne(), c_int64_t), &
-                              FEMDomain%mesh%NodCoord, FEMDomain%mesh%ElemNod, cap, k, e, g, p, x)
+      if (algorithm == FEMDomain_Overset_P2P) then  ! own node ids, Gauss point 0
+         code = c_pf_overset_p2p(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), this%mesh%NodCoord, &
+                                 this%mesh%ElemNod, int(FEMDomain%nn(), c_int64_t), int(FEMDomain%ne(), c_int64_t), &
+                                 FEMDomain%mesh%NodCoord, FEMDomain%mesh%ElemNod, cap, k, e, p, x)
+         g = 0
+      else
+         code = c_pf_overset_gpp(ctx(), int(this%nn(), c_int64_t), int(this%ne(), c_int64_t), this%mesh%NodCoord, &
+                                 this%mesh%ElemNod, int(FEMDomain%nn(), c_int64_t), int(FEMDomain%ne(), c_int64_t), &
+                                 FEMDomain%mesh%NodCoord, FEMDomain%mesh%ElemNod, cap, k, e, g, p, x)
+      end if
       call check(code, "overset")
       n0 = this%num_oversetconnect
       allocate (ebuf(n0 + k), gbuf(n0 + k), pbuf(n0 + k), dbuf(2, n0 + k), xbuf(3, n0 + k))
@@ -716,48 +740,50 @@ contains
       class(FEMSolver_), intent(inout) :: this
       integer(int32), intent(in) :: DOF
       integer(int32) :: nd, d, nn, ne, k, c
-      integer(int32), allocatable :: node_off(:), own(:), pair(:)
+      integer(int32), allocatable :: own(:), pair(:), gp(:)
       integer(c_int64_t) :: nnz
       integer(c_int32_t) :: any_overset
       integer(c_int) :: code
       type(FEMDomain_), pointer :: f
       nd = size(this%FEMDomains)
-      allocate (node_off(nd + 1))
+      if (allocated(this%node_off)) deallocate (this%node_off)
+      allocate (this%node_off(nd + 1))
       if (allocated(this%elem_off)) deallocate (this%elem_off)
       allocate (this%elem_off(nd + 1))
-      node_off(1) = 0
+      this%node_off(1) = 0
       this%elem_off(1) = 0
       do d = 1, nd
          if (.not. associated(this%FEMDomains(d)%femdomainp)) stop "plantfem_gpu: setCRS with an empty domain slot"
-         node_off(d + 1) = node_off(d) + this%FEMDomains(d)%femdomainp%nn()
+         this%node_off(d + 1) = this%node_off(d) + this%FEMDomains(d)%femdomainp%nn()
          this%elem_off(d + 1) = this%elem_off(d) + this%FEMDomains(d)%femdomainp%ne()
       end do
       if (associated(this%joint)) deallocate (this%joint)
       allocate (this%joint)
-      nn = node_off(nd + 1)
+      nn = this%node_off(nd + 1)
       ne = this%elem_off(nd + 1)
       allocate (this%joint%mesh%NodCoord(nn, 3), this%joint%mesh%ElemNod(ne, 8))
       any_overset = 0
       k = 0
       do d = 1, nd
          f => this%FEMDomains(d)%femdomainp
-         this%joint%mesh%NodCoord(node_off(d) + 1:node_off(d + 1), :) = f%mesh%NodCoord(:, 1:3)
-         this%joint%mesh%ElemNod(this%elem_off(d) + 1:this%elem_off(d + 1), :) = f%mesh%ElemNod(:, 1:8) + node_off(d)
+         this%joint%mesh%NodCoord(this%node_off(d) + 1:this%node_off(d + 1), :) = f%mesh%NodCoord(:, 1:3)
+         this%joint%mesh%ElemNod(this%elem_off(d) + 1:this%elem_off(d + 1), :) = f%mesh%ElemNod(:, 1:8) + &
+            this%node_off(d)
          if (allocated(f%ov_elem)) any_overset = 1
          k = k + f%num_oversetconnect
       end do
-      allocate (own(max(k, 1)), pair(max(k, 1)))
-      call connects_of(this, own, pair)
+      allocate (own(max(k, 1)), pair(max(k, 1)), gp(max(k, 1)))
+      call connects_of(this, own, pair, gp)
       this%domain => this%joint
       code = c_pf_system_set_mesh(this%sys, int(nn, c_int64_t), int(ne, c_int64_t), 8_c_int32_t, &
                                   this%joint%mesh%NodCoord, this%joint%mesh%ElemNod)
       call check(code, "setCRS (joint mesh)")
-      code = c_pf_system_multi_crs(this%sys, int(DOF, c_int32_t), any_overset, int(k, c_int64_t), own, pair, nnz)
+      code = c_pf_system_multi_crs(this%sys, int(DOF, c_int32_t), any_overset, int(k, c_int64_t), own, gp, pair, nnz)
       call check(code, "setCRS")
       if (allocated(this%CRS_ID_Starts_From)) deallocate (this%CRS_ID_Starts_From)
       allocate (this%CRS_ID_Starts_From(nd))
       do d = 1, nd
-         this%CRS_ID_Starts_From(d) = 1_int64 + int(DOF, int64)*node_off(d)
+         this%CRS_ID_Starts_From(d) = 1_int64 + int(DOF, int64)*this%node_off(d)
       end do
       this%DOF = DOF
       this%n = int(DOF, int64)*nn
@@ -779,7 +805,11 @@ contains
          f => this%FEMDomains(d)%femdomainp
          do i = 1, f%num_oversetconnect
             k = k + 1
-            own(k) = f%ov_elem(i) + this%elem_off(f%ov_domains(1, i))
+            if (f%ov_gp(i) == 0) then  ! P2P: own node of the joint mesh
+               own(k) = f%ov_elem(i) + this%node_off(f%ov_domains(1, i))
+            else
+               own(k) = f%ov_elem(i) + this%elem_off(f%ov_domains(1, i))
+            end if
             pair(k) = f%ov_pair(i) + this%elem_off(f%ov_domains(2, i))
             if (present(gp)) gp(k) = f%ov_gp(i)
             if (present(pos)) pos(:, k) = f%ov_position(:, i)

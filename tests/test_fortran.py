@@ -97,23 +97,28 @@ def test_fortran_modal_bar_matches_scipy(dims, nev):
 
 
 @pytest.mark.gpu
-def test_fortran_overset_organs_matches_oracle(tmp_path):
+@pytest.mark.parametrize("alg", ["GPP", "P2P"])
+def test_fortran_overset_organs_matches_oracle(tmp_path, alg):
     # two overlapping organs through overset / init(NumDomain=2) / setEbOM, against the oracle's
     # restatement of the same sequence
     from oracle import oracle as O
     from test_overset import box, joint
     out = tmp_path / "x.bin"
-    print("fortran overset_organs", flush=True)
-    res = subprocess.run([str(BIN / "overset_organs"), str(out)], capture_output=True, text=True, timeout=120)
+    print("fortran overset_organs", alg, flush=True)
+    res = subprocess.run([str(BIN / "overset_organs"), str(out), alg], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     raw = np.fromfile(out, dtype=np.int64, count=2)
     x = np.fromfile(out, dtype=np.float64)[2:]
     A = box(3, 3, 4, 1.0, 1.0, 2.0, (0.0, 0.0, 0.0))
     B = box(3, 3, 3, 1.0, 1.0, 1.2, (0.05, 0.07, 1.43))
-    me, gp, pe, pos = O.overset_gpp(*A, *B)
+    if alg == "GPP":
+        me, gp, pe, pos = O.overset_gpp(*A, *B)
+    else:
+        me, pe, pos = O.overset_p2p(*A, *B)
+        gp = np.zeros(me.size, np.int32)
     xyz, en = joint(A, B)
     pair = pe + A[1].shape[0]
-    rp, ci = O.multi_crs(xyz.shape[0], en, True, me, pair)
+    rp, ci = O.multi_crs(xyz.shape[0], en, True, me, pair, gp)
     assert raw[0] == me.size and raw[1] == ci.size
     ne = en.shape[0]
     val = O.assemble(xyz, en, np.full(ne, 1.0e4), np.full(ne, 0.3), rp, ci)

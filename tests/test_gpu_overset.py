@@ -151,3 +151,69 @@ def test_gpu_overset_solve_matches_oracle(ctx):
     assert st == 0 and s.last_stats["status"] == 0
     assert np.linalg.norm(x - x_o) / np.linalg.norm(x_o) < 1e-9
     assert s.last_stats["true_relres"] < 1e-10
+
+
+def test_gpu_overset_leaf3d_on_petiole_bitwise(ctx):
+    # a soybean Leaf3D blade (curved, tapered, thickened elements) whose base sits in a box
+    # petiole: the inside tests and local coordinates run on non-brick elements
+    leaf = fem.FEMDomain(ctx=ctx).create("Leaf3D", x_num=8, y_num=2, z_num=24, x_len=0.1, y_len=0.01, z_len=0.3,
+                                         species=fem.PF_GLYCINE_MAX)
+    pet = box(4, 4, 10, 0.043, 0.061, 0.12, (-0.0213, -0.0297, -0.0911))
+    dp = domain(pet, ctx)
+    leaf.overset(dp, DomainID=1, MyDomainID=2)
+    me, gp, pe, pos = O.overset_gpp(leaf.nodcoord, leaf.elemnod, *pet)
+    oc = leaf.OversetConnect
+    assert oc.n == me.size > 50
+    assert np.array_equal(oc.ElementID, me) and np.array_equal(oc.PairElementID, pe)
+    assert np.array_equal(oc.GaussPointID, gp) and np.array_equal(oc.position, pos)
+    s = fem.FEMSolver(ctx)
+    s.init(NumDomain=2)
+    s.setDomain(FEMDomains=[dp, leaf], DomainIDs=[1, 2])
+    s.setCRS(DOF=3)
+    s.setStiffness([2000.0, 1000.0], [0.3, 0.3])
+    s.setEbOM(penalty=1.0e7, DOF=3)
+    crs = s.getCRS()
+    xyz, en = joint(pet, (leaf.nodcoord, leaf.elemnod))
+    own = me + pet[1].shape[0]
+    rp, ci = O.multi_crs(xyz.shape[0], en, True, own, pe)
+    assert np.array_equal(crs.row_ptr, rp) and np.array_equal(crs.col_idx, ci)
+    E = np.concatenate([np.full(pet[1].shape[0], 2000.0), np.full(leaf.ne(), 1000.0)])
+    val = O.assemble(xyz, en, E, np.full(en.shape[0], 0.3), rp, ci)
+    val = O.ebom(xyz, en, rp, ci, val, own, gp, pe, pos, 1.0e7)
+    assert np.array_equal(crs.val, val)
+
+
+def test_gpu_overset_p2p_and_mixed_bitwise(ctx):
+    # P2P connects (nodes of A in B) next to GPP ones (C onto B) in one system
+    A, B = overlapping_pair()
+    Cm = box(3, 3, 3, 1.5, 1.5, 1.5, (3.11, 1.43, 2.02))
+    da, db, dc = domain(A, ctx), domain(B, ctx), domain(Cm, ctx)
+    da.overset(db, DomainID=2, algorithm=fem.FEMDomain_Overset_P2P, MyDomainID=1)
+    dc.overset(db, DomainID=2, algorithm="GPP", MyDomainID=3)
+    nd, pe, pos = O.overset_p2p(*A, *B)
+    oc = da.OversetConnect
+    assert oc.n == nd.size > 5 and np.all(oc.GaussPointID == 0)
+    assert np.array_equal(oc.NodeID, nd) and np.array_equal(oc.PairElementID, pe) and np.array_equal(oc.position, pos)
+    me, gp, pe2, pos2 = O.overset_gpp(*Cm, *B)
+    assert np.array_equal(dc.OversetConnect.ElementID, me) and me.size > 0
+    s = fem.FEMSolver(ctx)
+    s.init(NumDomain=3)
+    s.setDomain(FEMDomains=[da, db, dc], DomainIDs=[1, 2, 3])
+    s.setCRS(DOF=3)
+    s.setStiffness([1e3, 2e3, 3e3], [0.3, 0.3, 0.3])
+    s.setEbOM(penalty=1.0e6, DOF=3)
+    crs = s.getCRS()
+    xyz = np.asfortranarray(np.concatenate([A[0], B[0], Cm[0]]))
+    na, nb = A[0].shape[0], B[0].shape[0]
+    ea, eb = A[1].shape[0], B[1].shape[0]
+    en = np.asfortranarray(np.concatenate([A[1], B[1] + na, Cm[1] + na + nb]).astype(np.int32))
+    own = np.concatenate([nd, me + ea + eb]).astype(np.int32)
+    gps = np.concatenate([np.zeros(nd.size, np.int32), gp]).astype(np.int32)
+    prs = np.concatenate([pe + ea, pe2 + ea]).astype(np.int32)
+    poss = np.concatenate([pos, pos2])
+    rp, ci = O.multi_crs(xyz.shape[0], en, True, own, prs, gps)
+    assert np.array_equal(crs.row_ptr, rp) and np.array_equal(crs.col_idx, ci)
+    E = np.concatenate([np.full(ea, 1e3), np.full(eb, 2e3), np.full(Cm[1].shape[0], 3e3)])
+    val = O.assemble(xyz, en, E, np.full(en.shape[0], 0.3), rp, ci)
+    val = O.ebom(xyz, en, rp, ci, val, own, gps, prs, poss, 1.0e6)
+    assert np.array_equal(crs.val, val)

@@ -1,0 +1,72 @@
+"""Leaf3D organ meshes (Mesh%create(meshtype="Leaf3D"), MeshClass.f90:6017-6276), host side.
+
+Pins: the outline's closed-form half-widths at the base, the widest point and the tip; the
+base thickening to width/10 (default) or x5 (soybean); unchanged z and Cube3D connectivity;
+positive Jacobians at every Gauss point of the shaped elements."""
+import numpy as np
+import pytest
+
+from plantfem_amd.fem import PF_GLYCINE_MAX, FEMDomain, cube3d, leaf3d
+
+CORNERS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                    [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=np.float64)
+
+
+def min_detj(nod, el):
+    g = 1.0 / np.sqrt(3.0)
+    out = np.inf
+    for gp in CORNERS * g:
+        dN = np.stack([CORNERS[:, k] * 0.125 * np.prod(1 + np.delete(CORNERS, k, 1) * np.delete(gp, k), axis=1)
+                       for k in range(3)])
+        J = np.einsum("ka,eaj->ekj", dN, nod[el - 1])
+        out = min(out, np.linalg.det(J).min())
+    return out
+
+
+def half_width(nod, zval):
+    on = np.isclose(nod[:, 2], zval, rtol=0, atol=1e-12)
+    return np.abs(nod[on, 0]).max(), nod[on, 1].max() - nod[on, 1].min()
+
+
+def test_leaf3d_default_outline():
+    w, t, L, s = 0.2, 0.01, 1.0, 0.3
+    nod, el = leaf3d(8, 2, 20, w, t, L, shaperatio=s)
+    n0, e0 = cube3d(8, 2, 20, w, t, L)
+    assert np.array_equal(el, e0) and np.array_equal(nod[:, 2], n0[:, 2])
+    hw_base, th_base = half_width(nod, 0.0)
+    assert np.isclose(hw_base, 0.1 * w / 2, rtol=1e-12)
+    assert np.isclose(th_base, w / 10, rtol=1e-12)            # base thickened to width/10
+    hw_top, th_top = half_width(nod, L)
+    assert np.isclose(hw_top, 0.1 * w / 2, rtol=1e-12) and np.isclose(th_top, t, rtol=1e-12)
+    hw_mid, _ = half_width(nod, n0[:, 2][np.argmin(np.abs(n0[:, 2] - s * L))])
+    assert np.isclose(hw_mid, w / 2, rtol=1e-9)               # full width at z = shaperatio * length
+    assert min_detj(nod, el) > 0
+
+
+def test_leaf3d_soybean_outline():
+    w, t, L = 0.1, 0.01, 0.3
+    nod, el = leaf3d(8, 2, 30, w, t, L, species=PF_GLYCINE_MAX, SoyWidthRatio=0.5)
+    n0, _ = cube3d(8, 2, 30, w, t, L)
+    x0 = n0[:, 0] - w / 2
+    base = n0[:, 2] <= L / 25
+    assert np.allclose(nod[base, 0], x0[base] / 10, rtol=1e-14, atol=0)
+    # thickness x5 at the base, then the curl y -= 0.1 theta (theta - 1), theta = |x| / (w/2)
+    theta = np.abs(nod[:, 0] / (w / 2))
+    y_uncurled = nod[:, 1] + 0.10 * theta * (theta - 1.0)
+    assert np.allclose(y_uncurled[base], 5.0 * (n0[base, 1] - t / 2), rtol=1e-12, atol=1e-15)
+    assert np.allclose(y_uncurled[~base], n0[~base, 1] - t / 2, rtol=1e-12, atol=1e-15)
+    # widest at z = alpha = 0.5 w (arc top), narrowing to the tip
+    hw_alpha, _ = half_width(nod, n0[:, 2][np.argmin(np.abs(n0[:, 2] - 0.5 * w))])
+    assert np.isclose(hw_alpha, w / 2, rtol=1e-6)
+    assert half_width(nod, L)[0] < 0.05 * w
+    assert min_detj(nod, el) > 0
+    with pytest.raises(ValueError):
+        leaf3d(4, 2, 4, species=PF_GLYCINE_MAX, SoyWidthRatio=1.0)
+
+
+def test_femdomain_create_leaf3d_and_move():
+    d = FEMDomain().create("Leaf3D", x_num=6, y_num=2, z_num=10, x_len=0.1, y_len=0.01, z_len=0.2)
+    nod, _ = leaf3d(6, 2, 10, 0.1, 0.01, 0.2)
+    assert np.array_equal(d.nodcoord, nod)
+    d.move(x=1.0, z=-0.5)
+    assert np.allclose(d.nodcoord, nod + [1.0, 0.0, -0.5], rtol=0, atol=1e-15)

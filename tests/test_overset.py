@@ -221,3 +221,54 @@ def test_overset_system_solve():
         Y = xyz[en[pair[c] - 1] - 1]
         ub = N(O.local_coordinate(Y, pos[c])) @ ux[en[pair[c] - 1] - 1]
         assert np.abs(ua - ub).max() < 2e-2 * np.abs(ux).max()
+
+
+def test_overset_p2p_matches_brute_force():
+    (xa, ea), (xb, eb) = overlapping_pair()
+    nd, pe, pos = O.overset_p2p(xa, ea, xb, eb)
+    lo, hi = xb.min(axis=0), xb.max(axis=0)
+    exp = []
+    for i in range(xa.shape[0]):
+        p = xa[i]
+        if np.all((lo <= p) & (p <= hi)):
+            cand = containing_elements(xb, eb, p)
+            assert cand.size == 1
+            exp.append((i + 1, cand[0]))
+    assert len(exp) > 5
+    assert np.array_equal(nd, [t[0] for t in exp]) and np.array_equal(pe, [t[1] for t in exp])
+    assert np.array_equal(pos, xa[nd - 1])
+
+
+def test_ebom_p2p_dense_and_null_space():
+    A, B = overlapping_pair()
+    nd, pe, pos = O.overset_p2p(*A, *B)
+    xyz, en = joint(A, B)
+    pair = pe + A[1].shape[0]
+    gp = np.zeros(nd.size, np.int32)
+    rp, ci = O.multi_crs(xyz.shape[0], en, True, nd, pair, gp)
+    pen = 1.0e5
+    val = O.ebom(xyz, en, rp, ci, np.zeros(ci.size), nd, gp, pair, pos, pen)
+    n = 3 * xyz.shape[0]
+    dense = np.zeros((n, n))
+    for c in range(nd.size):
+        Y = xyz[en[pair[c] - 1] - 1]
+        h = Y.max(axis=0) - Y.min(axis=0)
+        xi = 2.0 * (pos[c] - Y.mean(axis=0)) / h
+        Bc = np.kron(np.concatenate([[1.0], -N(xi)])[None, :], np.eye(3))   # 3 x 27, no det J
+        dofs = (3 * (np.concatenate([[nd[c]], en[pair[c] - 1]]) - 1)[:, None] + np.arange(3)).ravel()
+        dense[np.ix_(dofs, dofs)] += pen * (Bc.T @ Bc)
+    got = np.zeros((n, n))
+    for i in range(n):
+        got[i, ci[rp[i] - 1:rp[i + 1] - 1] - 1] = val[rp[i] - 1:rp[i + 1] - 1]
+    np.testing.assert_allclose(got, dense, rtol=1e-12, atol=1e-12 * pen)
+    for d in range(3):
+        u = np.zeros(n)
+        u[d::3] = 1.0
+        assert np.abs(got @ u).max() < 1e-9 * pen
+    Gm = np.array([[0.1, -0.2, 0.3], [0.05, 0.2, -0.1], [-0.3, 0.0, 0.15]])
+    assert np.abs(got @ (xyz @ Gm.T).ravel()).max() < 1e-8 * pen
+    # a P2P connect couples 9 nodes: its rows gain the paired element's node blocks only
+    rp0, ci0 = O.multi_crs(xyz.shape[0], en, True, nd[:0], pair[:0], gp[:0])
+    row = 3 * (nd[0] - 1)
+    added = set(ci[rp[row] - 1:rp[row + 1] - 1]) - set(ci0[rp0[row] - 1:rp0[row + 1] - 1])
+    assert added == {3 * (m - 1) + d + 1 for m in en[pair[0] - 1] for d in range(3)}
