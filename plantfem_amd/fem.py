@@ -15,6 +15,8 @@ from __future__ import annotations
 import ctypes as C
 from dataclasses import dataclass
 
+import math
+
 import numpy as np
 
 from . import _lib as L
@@ -123,6 +125,127 @@ def leaf3d(x_num: int = 10, y_num: int = 10, z_num: int = 10, x_len: float = 1.0
     return out, el
 
 
+def _rect2d(xn: int, yn: int, lx: float, ly: float):
+    """rectangular2D (MeshClass.f90:6745-6808): x fastest, lx/dble(xn)*dble(i-1); quads."""
+    i = np.arange(xn + 1, dtype=np.float64)
+    j = np.arange(yn + 1, dtype=np.float64)
+    Y, X = np.meshgrid(ly / float(yn) * j, lx / float(xn) * i, indexing="ij")
+    nod = np.stack([X.ravel(), Y.ravel()], axis=1)
+    ii, jj = np.meshgrid(np.arange(1, xn + 1), np.arange(1, yn + 1), indexing="xy")
+    ii, jj = ii.ravel(), jj.ravel()
+    el = np.stack([ii + (jj - 1) * (xn + 1), ii + 1 + (jj - 1) * (xn + 1), xn + 2 + ii + (jj - 1) * (xn + 1),
+                   xn + 1 + ii + (jj - 1) * (xn + 1)], axis=1).astype(np.int32)
+    return nod, el
+
+
+def circle2d(x_num: int = 10, y_num: int = 10, x_len: float | None = None, y_len: float | None = None):
+    """Circle2D (MeshClass.f90:6524-6743): a (2m)^2 quad square on [-1,1]^2 rounded towards a
+    disc (node scale alpha(1/R) with R = sqrt(1 + (minor/major)^2), times sqrt(2)*0.95, then
+    (2m-1)/(2m)/sqrt(2)), surrounded by 2m rings of 8m nodes at radii 1 + i/(2m); the square's
+    boundary is tied to ring 1 and rings 1..m+2 by 8m quads per layer (the outer rings stay
+    unused), m = max(x_num, y_num)//2 + 1; resized to x_len, y_len over all nodes.  2-D nodes
+    (nn, 2) and quads (ne, 4), 1-based."""
+    xn = x_num // 2 + 1
+    yn = y_num // 2 + 1
+    xn = yn = max(xn, yn)
+    nod, el = _rect2d(2 * xn, 2 * yn, 2.0, 2.0)
+    nod = nod - 1.0
+    x, y = nod[:, 0].copy(), nod[:, 1].copy()
+    s2 = math.sqrt(2.0)
+    theta = (1.0 - 0.80) / (1.0 - 1.0 / s2)
+    for k in range(x.size):
+        xx, yy = x[k], y[k]
+        RR = 1.00
+        if xx != 0.0:
+            if abs(yy / xx) > 1.0:
+                if yy != 0.0:
+                    RR = math.sqrt((xx / yy) * (xx / yy) + 1)
+            else:
+                RR = math.sqrt((yy / xx) * (yy / xx) + 1)
+        alpha = theta * ((1 / RR) - 1.0 / s2) + 0.790
+        x[k] = (xx * alpha) * s2 * 0.95
+        y[k] = (yy * alpha) * s2 * 0.95
+    c = float(2 * xn - 1) / float(2 * xn)
+    x = c * x / s2
+    y = c * y / s2
+    m = 2 * xn
+    sq = (m + 1) * (m + 1)
+    ring = np.empty((m * 4 * m, 2))
+    for i in range(1, m + 1):
+        r = 1.0 + float(i) * (1.0 / float(m))
+        for j in range(1, 4 * m + 1):
+            a = 2.0 * math.pi / 4.0 / float(m) * float(j - 1)
+            ring[(i - 1) * 4 * m + j - 1] = (r * math.cos(a), r * math.sin(a))
+    e1 = []
+    j = 0
+    for i in range(1, xn + 1):
+        j += 1
+        e1.append([(m + 1) * (xn + i), sq + j, sq + j + 1, (m + 1) * (xn + i + 1)])
+    for i in range(1, m + 1):
+        j += 1
+        e1.append([sq - i + 1, sq + j, sq + j + 1, sq - i])
+    for i in range(1, m + 1):
+        j += 1
+        e1.append([sq - (m + 1) + 1 - (i - 1) * (m + 1), sq + j, sq + j + 1, sq - (m + 1) + 1 - i * (m + 1)])
+    for i in range(1, m + 1):
+        j += 1
+        e1.append([i, sq + j, sq + j + 1, i + 1])
+    for i in range(1, xn + 1):
+        j += 1
+        e1.append([(m + 1) * i, sq + j, sq + j + 1, (m + 1) * (i + 1)])
+    e1[j - 1][2] = sq + 1
+    for i in range(1, xn + 2):
+        ini = j + 1
+        for k in range(1, 8 * xn):
+            j += 1
+            e1.append([sq + j - 8 * xn, sq + j, sq + j + 1, sq + j + 1 - 8 * xn])
+        j += 1
+        e1.append([sq + j - 8 * xn, sq + j, sq + ini, sq + ini - 8 * xn])
+    nod = np.concatenate([np.stack([x, y], axis=1), ring])
+    el = np.concatenate([el, np.array(e1, dtype=np.int32)]).astype(np.int32)
+    if x_len is not None:
+        nod[:, 0] = x_len / (nod[:, 0].max() - nod[:, 0].min()) * nod[:, 0]
+    if y_len is not None:
+        nod[:, 1] = y_len / (nod[:, 1].max() - nod[:, 1].min()) * nod[:, 1]
+    return nod, el
+
+
+def clean_mesh(nod, el):
+    """cleanMesh (MeshClass.f90:8571-8605): drop nodes no element uses, renumber the rest."""
+    used = np.zeros(nod.shape[0], bool)
+    used[el.ravel() - 1] = True
+    removed_before = np.cumsum(~used)  # removed nodes with id <= k
+    return nod[used].copy(), (el - removed_before[el - 1]).astype(np.int32)
+
+
+def convert_2d_to_3d(nod, el, thickness: float = 1.0, division: int = 1):
+    """Convert2Dto3D (MeshClass.f90:6992-7090): layer j of the 2-D nodes at z = Tn/dble(N)*dble(j-1),
+    element e of layer j = (2-D nodes + n(j-1), 2-D nodes + n j)."""
+    n = nod.shape[0]
+    z = float(thickness) / float(division) * np.arange(division + 1, dtype=np.float64)
+    out = np.empty((n * (division + 1), 3), order="F")
+    out[:, 0] = np.tile(nod[:, 0], division + 1)
+    out[:, 1] = np.tile(nod[:, 1], division + 1)
+    out[:, 2] = np.repeat(z, n)
+    layers = np.arange(division)[:, None, None] * n
+    e3 = np.concatenate([(el[None] + layers).reshape(-1, el.shape[1]), (el[None] + layers + n).reshape(-1, el.shape[1])],
+                        axis=1)
+    return out, np.asfortranarray(e3.astype(np.int32))
+
+
+def cylinder3d(x_num: int = 10, y_num: int = 10, z_num: int = 10, x_len: float = 1.0, y_len: float = 1.0,
+               z_len: float = 1.0):
+    """Mesh%create(meshtype="Cylinder") (MeshClass.f90:6506-6522), the stem mesh of StemClass:
+    Circle2D on a unit disc, clean, Convert2Dto3D(thickness=z_len, division=z_num), resized to
+    x_len, y_len over the remaining nodes."""
+    nod, el = circle2d(x_num, y_num, 1.0, 1.0)
+    nod, el = clean_mesh(nod, el)
+    nod, el = convert_2d_to_3d(nod, el, z_len, z_num)
+    nod[:, 0] = x_len / (nod[:, 0].max() - nod[:, 0].min()) * nod[:, 0]
+    nod[:, 1] = y_len / (nod[:, 1].max() - nod[:, 1].min()) * nod[:, 1]
+    return np.asfortranarray(nod), el
+
+
 @dataclass
 class CRS:
     """CRS_ (SparseClass.f90:128-131): 1-based int64 row_ptr, int32 col_idx, float64 val."""
@@ -214,6 +337,9 @@ class FEMDomain:
         my_id = 1 if MyDomainID is None else int(MyDomainID)
         if self.OversetConnect is None:
             self.OversetConnect = OversetConnect()
+            self._overset_p2p = p2p
+        elif self._overset_p2p != p2p:  # FEMDomainClass.f90:14656-14662, :14717-14723
+            raise ValueError("overset: P2P & GPP cannot be mixed on one domain")
         xa, ea = self._flat()
         xb, eb = FEMDomain._flat()
         lib = L.load()
@@ -245,16 +371,18 @@ class FEMDomain:
     def create(self, meshtype: str = "Cube3D", x_num=None, y_num=None, z_num=None, x_len=None, y_len=None,
                z_len=None, shaperatio: float = 0.3, species=None, SoyWidthRatio: float = 0.5):
         """create(meshtype, x_num, y_num, z_num, x_len, y_len, z_len) (FEMDomainClass.f90:7215-7310):
-        "Cube3D" / "Cube" (the same numbering) or "Leaf3D" (organ mesh of LeafClass, with
-        shaperatio or species=PF_GLYCINE_MAX and SoyWidthRatio)."""
+        "Cube3D" / "Cube" (the same numbering), "Cylinder" (StemClass's round stem) or "Leaf3D"
+        (organ mesh of LeafClass, with shaperatio or species=PF_GLYCINE_MAX and SoyWidthRatio)."""
         args = (x_num or 10, y_num or 10, z_num or 10, x_len or 1.0, y_len or 1.0, z_len or 1.0)
         if meshtype in ("Cube3D", "Cube", "cube3D", "rectangular3D"):
             self.nodcoord, self.elemnod = cube3d(*args)
+        elif meshtype in ("Cylinder", "Cylinder3D"):
+            self.nodcoord, self.elemnod = cylinder3d(*args)
         elif meshtype == "Leaf3D":
             self.nodcoord, self.elemnod = leaf3d(*args, shaperatio=shaperatio, species=species,
                                                  SoyWidthRatio=SoyWidthRatio)
         else:
-            raise ValueError(f"meshtype {meshtype!r}: Cube3D, Cube and Leaf3D are on this path")
+            raise ValueError(f"meshtype {meshtype!r}: Cube3D, Cube, Cylinder and Leaf3D are on this path")
         return self
 
     def move(self, x: float = 0.0, y: float = 0.0, z: float = 0.0):

@@ -302,7 +302,7 @@ module plantfem_gpu
       ! OversetConnect_ list as arrays: own element (GPP) or own node (P2P), Gauss point (0 for
       ! P2P), paired element (ids in the paired domain), position (3, k), DomainIDs12 (2, k);
       ! allocated by the first overset
-      integer(int32) :: num_oversetconnect = 0
+      integer(int32) :: num_oversetconnect = 0, overset_kind = 0  ! 0 none, 1 GPP, 2 P2P
       integer(int32), allocatable :: ov_elem(:), ov_gp(:), ov_pair(:), ov_domains(:, :)
       real(real64), allocatable :: ov_position(:, :)
    contains
@@ -648,6 +648,9 @@ contains
       integer(c_int) :: code
       if (algorithm /= FEMDomain_Overset_GPP .and. algorithm /= FEMDomain_Overset_P2P) &
          stop "plantfem_gpu: overset algorithm is FEMDomain_Overset_GPP or FEMDomain_Overset_P2P"
+      if (this%overset_kind /= 0 .and. this%overset_kind /= algorithm) &
+         stop "[ERROR] FEMDomainClass > oversetFEMDomain >> P2P & GPP cannot be mixed."
+      this%overset_kind = algorithm
       my = 1
       if (present(MyDomainID)) my = MyDomainID
       if (algorithm == FEMDomain_Overset_P2P) then

@@ -217,3 +217,41 @@ def test_gpu_overset_p2p_and_mixed_bitwise(ctx):
     val = O.assemble(xyz, en, E, np.full(en.shape[0], 0.3), rp, ci)
     val = O.ebom(xyz, en, rp, ci, val, own, gps, prs, poss, 1.0e6)
     assert np.array_equal(crs.val, val)
+
+
+def test_gpu_overset_cylinder_stem_bitwise(ctx):
+    # a round stem (Cylinder mesh: the rounded square core and the ring layers) with one box
+    # petiole overset into it by GPP and another by P2P
+    stem = fem.FEMDomain(ctx=ctx).create("Cylinder", x_num=8, y_num=8, z_num=12, x_len=0.08, y_len=0.08, z_len=0.6)
+    pet = box(10, 3, 3, 0.2, 0.03, 0.03, (0.0113, -0.0141, 0.3017))
+    pet2 = box(10, 3, 3, 0.2, 0.03, 0.03, (-0.2113, -0.0137, 0.1519))
+    dp, dq = domain(pet, ctx), domain(pet2, ctx)
+    dp.overset(stem, DomainID=1, MyDomainID=2)
+    dq.overset(stem, DomainID=1, MyDomainID=3, algorithm=fem.FEMDomain_Overset_P2P)
+    with pytest.raises(ValueError):
+        dq.overset(stem, DomainID=1, MyDomainID=3)  # the reference stops: P2P & GPP cannot be mixed
+    me, gp, pe, pos = O.overset_gpp(*pet, stem.nodcoord, stem.elemnod)
+    nd, pe2, pos2 = O.overset_p2p(*pet2, stem.nodcoord, stem.elemnod)
+    assert me.size > 20 and nd.size > 2
+    assert np.array_equal(dp.OversetConnect.ElementID, me) and np.array_equal(dp.OversetConnect.PairElementID, pe)
+    assert np.array_equal(dq.OversetConnect.NodeID, nd) and np.array_equal(dq.OversetConnect.PairElementID, pe2)
+    assert np.array_equal(dp.OversetConnect.position, pos) and np.array_equal(dq.OversetConnect.position, pos2)
+    s = fem.FEMSolver(ctx)
+    s.init(NumDomain=3)
+    s.setDomain(FEMDomains=[stem, dp, dq], DomainIDs=[1, 2, 3])
+    s.setCRS(DOF=3)
+    s.setStiffness([1000.0, 1000.0, 1000.0], [0.3, 0.3, 0.3])
+    s.setEbOM(penalty=1.0e7, DOF=3)
+    ns, es = stem.nn(), stem.ne()
+    npt, ept = pet[0].shape[0], pet[1].shape[0]
+    xyz = np.asfortranarray(np.concatenate([stem.nodcoord, pet[0], pet2[0]]))
+    en = np.asfortranarray(np.concatenate([stem.elemnod, pet[1] + ns, pet2[1] + ns + npt]).astype(np.int32))
+    own = np.concatenate([me + es, nd + ns + npt]).astype(np.int32)
+    gps = np.concatenate([gp, np.zeros(nd.size, np.int32)]).astype(np.int32)
+    prs = np.concatenate([pe, pe2]).astype(np.int32)
+    rp, ci = O.multi_crs(xyz.shape[0], en, True, own, prs, gps)
+    crs = s.getCRS()
+    assert np.array_equal(crs.row_ptr, rp) and np.array_equal(crs.col_idx, ci)
+    val = O.assemble(xyz, en, np.full(en.shape[0], 1000.0), np.full(en.shape[0], 0.3), rp, ci)
+    val = O.ebom(xyz, en, rp, ci, val, own, gps, prs, np.concatenate([pos, pos2]), 1.0e7)
+    assert np.array_equal(crs.val, val)

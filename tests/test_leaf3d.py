@@ -70,3 +70,40 @@ def test_femdomain_create_leaf3d_and_move():
     assert np.array_equal(d.nodcoord, nod)
     d.move(x=1.0, z=-0.5)
     assert np.allclose(d.nodcoord, nod + [1.0, 0.0, -0.5], rtol=0, atol=1e-15)
+
+
+def test_circle2d_tiles_the_outer_polygon():
+    # Circle2D (MeshClass.f90:6524-6743): quads counter-clockwise, conforming, and together
+    # exactly the polygon of the outermost used ring (no gaps, no overlaps)
+    from plantfem_amd.fem import circle2d, clean_mesh
+    for xnum in (4, 10):
+        nod, el = clean_mesh(*circle2d(xnum, xnum))
+        P = nod[el - 1]                                      # (ne, 4, 2)
+        area = 0.5 * np.sum(P[:, :, 0] * np.roll(P[:, :, 1], -1, axis=1) - np.roll(P[:, :, 0], -1, axis=1) * P[:, :, 1],
+                            axis=1)
+        assert area.min() > 0
+        edges = {}
+        for q in el:
+            for a in range(4):
+                e = tuple(sorted((q[a], q[(a + 1) % 4])))
+                edges[e] = edges.get(e, 0) + 1
+        assert set(edges.values()) <= {1, 2}
+        bnd = [e for e, c in edges.items() if c == 1]
+        m = 2 * (xnum // 2 + 1)
+        assert len(bnd) == 4 * m                              # the outer ring's 8m/2... 4m-gon edges
+        r = np.hypot(nod[:, 0], nod[:, 1])
+        R = r.max()
+        k = len(bnd)
+        assert np.isclose(area.sum(), 0.5 * k * R * R * np.sin(2 * np.pi / k), rtol=1e-12)
+
+
+def test_cylinder_stem_mesh():
+    from plantfem_amd.fem import cylinder3d
+    nod, el = cylinder3d(10, 10, 6, 0.08, 0.08, 0.6)
+    assert np.isclose(nod[:, 0].max() - nod[:, 0].min(), 0.08, rtol=1e-14)
+    assert np.isclose(nod[:, 1].max() - nod[:, 1].min(), 0.08, rtol=1e-14)
+    assert np.isclose(nod[:, 2].max(), 0.6, rtol=1e-14) and nod[:, 2].min() == 0.0
+    assert el.min() == 1 and el.max() == nod.shape[0]                  # cleaned: every node used
+    assert min_detj(nod, el) > 0
+    d = FEMDomain().create("Cylinder", x_num=10, y_num=10, z_num=6, x_len=0.08, y_len=0.08, z_len=0.6)
+    assert np.array_equal(d.nodcoord, nod) and np.array_equal(d.elemnod, el)
