@@ -75,6 +75,8 @@ def main(argv=None):
     ap.add_argument("--no-graph", action="store_true", help="direct kernel launches only (for profilers)")
     ap.add_argument("--workload", choices=("box", "batch", "adjoint", "precond", "modal", "overset"), default="box")
     ap.add_argument("--plant-scale", type=int, default=2, help="refinement of the overset plant (1: 23k DOF)")
+    ap.add_argument("--plant-kind", choices=("boxes", "organs"), default="boxes",
+                    help="overset plant of Cube3D boxes, or of Cylinder stem / box petioles / Leaf3D blades")
     ap.add_argument("--precond-size", type=int, default=60)
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-itrmax", type=int, default=3000,
@@ -506,7 +508,7 @@ def bench_overset(a):
     from plantfem_amd import fem
     from plantfem_amd import workloads as W
     ctx = L.default_context()
-    P = W.plant_scaled(a.plant_scale)
+    P = W.plant_organs(a.plant_scale) if a.plant_kind == "organs" else W.plant_scaled(a.plant_scale)
     E0, nu0, pen, rho = 1000.0, 0.3, 1.0e7, 1.0
     stem_xyz = P.meshes[0][0]
     base = np.nonzero(stem_xyz[:, 2] <= stem_xyz[:, 2].min())[0]
@@ -553,14 +555,27 @@ def bench_overset(a):
     out = {"metric": "overset multi-domain static solve: time from organ meshes to displacement", "unit": "s",
            "value": round(float(np.median(times)), 4), "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "higher_is_better": False, "data": "synthetic", "dtype": "f64",
-           "config": {"workload": f"plant x{a.plant_scale}: stem + 8 petioles + 8 leaf blades (17 Cube3D domains, "
-                                  f"{P.meta['elements']} hex8, {P.meta['dof']} DOF), 16 overset pairs (GPP), "
+           "config": {"workload": f"plant x{a.plant_scale}: stem + 8 petioles + 8 leaf blades (17 domains: "
+                                  + ("Cube3D boxes" if a.plant_kind == "boxes" else P.meta["organs"])
+                                  + f", {P.meta['elements']} hex8, {P.meta['dof']} DOF), 16 overset pairs (GPP), "
                                   "E 1000, nu 0.3, penalty 1e7, gravity, stem base fixed, relres 1e-8"},
            "phases_s": {k2: round(float(v), 4) for k2, v in phases[k].items()},
            "solve": {"iterations": st["iterations"], "status": st["status"], "true_relres": st["true_relres"]}}
     ncon = sum(d.numOversetElements() for d in doms)
     crs = s.getCRS()
     out["system"] = {"connects": int(ncon), "nnz": int(crs.nnz), "n": int(crs.n)}
+    # roofline of the solve's dominant kernel (y = A p fused with r0.y), per-launch hipEvents
+    # over 2000 iterations of the same system
+    s.itrmax = 2000
+    s.solve(kernel_timing=1, download=False, raise_on_breakdown=False, use_graph=0)
+    stt = s.last_stats
+    kind = int(stt["spmv_kind_used"])
+    k1_bytes = spmv_bytes(kind, int(crs.n), int(crs.nnz)) + 8.0 * crs.n
+    k1_gbs = k1_bytes / (stt["spmv_p_ms"] * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                       "kernel": f"k_spmv_p ({'CRS' if kind == 1 else 'blocked'} FP64 y=A p fused with r0.y)",
+                       "bytes_per_launch": k1_bytes, "avg_launch_ms": round(stt["spmv_p_ms"], 4)}
     log(f"overset: {json.dumps(out)}")
     # parity of the assembled system with the oracle (pattern and values bitwise)
     nn = [m[0].shape[0] for m in P.meshes]

@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .fem import cube3d
+from .fem import PF_GLYCINE_MAX, cube3d, cylinder3d, leaf3d
 
 TOP_TRACTION = 1.0e4  # Pa, downward
 
@@ -185,3 +185,48 @@ def plant_scaled(scale: int = 1) -> Plant:
     s = int(scale)
     return plant(stem=(6 * s, 6 * s, 30 * s), h=0.01 / s, petiole_len=8 * s, petiole_cells=6 * s,
                  leaf_cells=8 * s, leaf_layers=3 * s)
+
+
+# leaf axes (width a, thickness b, length c) -> global axes for a blade pointing along +x, -x,
+# +y, -y: proper rotations (determinant +1), so element orientation is kept
+_LEAF_ROT = [np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]], float), np.array([[0, 0, -1], [-1, 0, 0], [0, 1, 0]], float),
+             np.array([[-1, 0, 0], [0, 0, 1], [0, 1, 0]], float), np.array([[1, 0, 0], [0, 0, -1], [0, 1, 0]], float)]
+
+
+def plant_organs(scale: int = 1, petioles: int = 8) -> Plant:
+    """A plant of the reference's organ meshes: a Cylinder stem (StemClass), box petioles and
+    soybean Leaf3D blades (LeafClass) laid flat at the petiole tips, pointing outwards; the
+    petioles are overset onto the stem and the blades onto their petiole."""
+    s = int(scale)
+    sd, sl = 0.08, 0.6
+    stem = cylinder3d(8 * s, 8 * s, 30 * s, sd, sd, sl)
+    meshes = [(np.asfortranarray(stem[0]), stem[1])]
+    pairs = []
+    L, t = 0.10, 0.03
+    for k in range(petioles):
+        z = (0.35 + 0.55 * k / max(petioles, 1)) * sl
+        d = k % 4
+        off = 0.2 * sd  # petiole base inside the stem
+        if d == 0:
+            org, dims = (off + 0.0013, -t / 2 + 0.0007, z), (10 * s, 3 * s, 3 * s, L, t, t)
+            tip = np.array([off + L, 0.0, z + t / 2])
+        elif d == 1:
+            org, dims = (-off - L - 0.0011, -t / 2 + 0.0009, z), (10 * s, 3 * s, 3 * s, L, t, t)
+            tip = np.array([-off - L, 0.0, z + t / 2])
+        elif d == 2:
+            org, dims = (-t / 2 + 0.0006, off + 0.0017, z), (3 * s, 10 * s, 3 * s, t, L, t)
+            tip = np.array([0.0, off + L, z + t / 2])
+        else:
+            org, dims = (-t / 2 + 0.0004, -off - L - 0.0012, z), (3 * s, 10 * s, 3 * s, t, L, t)
+            tip = np.array([0.0, -off - L, z + t / 2])
+        meshes.append(_shifted(*dims, org))
+        pairs.append((len(meshes), 1))
+        nod, el = leaf3d(8 * s, 2 * s, 16 * s, 0.12, 0.01, 0.2, species=PF_GLYCINE_MAX)
+        # the blade's base (length 0 .. 0.02) reaches back over the petiole's last 2 cm
+        g = nod @ _LEAF_ROT[d].T + tip - 0.02 * _LEAF_ROT[d][:, 2] + np.array([0.0011, 0.0007, 0.0003])
+        meshes.append((np.asfortranarray(g), el))
+        pairs.append((len(meshes), len(meshes) - 1))
+    nn = sum(m[0].shape[0] for m in meshes)
+    ne = sum(m[1].shape[0] for m in meshes)
+    return Plant(meshes, pairs, {"domains": len(meshes), "nodes": nn, "elements": ne, "dof": 3 * nn,
+                                 "organs": "Cylinder stem, box petioles, soybean Leaf3D blades"})
