@@ -742,7 +742,7 @@ contains
    subroutine setCRSMultiFEMSolver(this, DOF)
       class(FEMSolver_), intent(inout) :: this
       integer(int32), intent(in) :: DOF
-      integer(int32) :: nd, d, nn, ne, k, c
+      integer(int32) :: nd, d, nn, ne, k
       integer(int32), allocatable :: own(:), pair(:), gp(:)
       integer(c_int64_t) :: nnz
       integer(c_int32_t) :: any_overset
@@ -792,7 +792,6 @@ contains
       this%n = int(DOF, int64)*nn
       this%nnz = nnz
       this%npend = 0
-      c = 0
    end subroutine
 
    ! every connect of every domain in domain order, as element ids of the joint mesh
@@ -1113,6 +1112,13 @@ contains
       this%initialized = .false.
       this%n = 0
       this%npend = 0
+      ! several domains: the joined mesh and the offsets go with the system
+      if (associated(this%joint)) deallocate (this%joint)
+      if (allocated(this%FEMDomains)) deallocate (this%FEMDomains)
+      if (allocated(this%elem_off)) deallocate (this%elem_off)
+      if (allocated(this%node_off)) deallocate (this%node_off)
+      if (allocated(this%CRS_ID_Starts_From)) deallocate (this%CRS_ID_Starts_From)
+      this%domain => null()
    end subroutine
 
    ! ------------------------------------------------------------------- plug-in solvers
