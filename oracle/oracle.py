@@ -49,6 +49,7 @@ _SIG = {
     "orc_connect_coeffs_p2p": (None, [PD, PD, PD]),
     "orc_connect_coeffs": (None, [PD, I32, PD, PD, PD, PD]),
     "orc_multi_crs": (I64, [I64, I64, P32, I32, I64, P32, P64, P32]),
+    "orc_connect_coeffs_all": (None, [I64, I64, PD, P32, I64, P32, P32, P32, PD, PD, PD]),
     "orc_ebom": (I32, [I64, P64, P32, PD, I64, P32, PD, PD, D]),
 }
 
@@ -374,32 +375,21 @@ def local_coordinate(X, p):
     return xi
 
 
-def _elem_X(nodcoord, elemnod, e):
-    return np.asarray(nodcoord, dtype=np.float64)[np.asarray(elemnod)[e - 1] - 1]
-
-
 def connect_coeffs(nodcoord, elemnod, elem, gp, pair_elem, position):
     """Per connect: c16 (16) = [N_own(gp), -N_pair(xi)] and detJ of the own Gauss point; a P2P
     connect (gp 0, elem = own node) gets [1, -N_pair(xi), 0 ...] and detJ 1."""
     k = len(elem)
     c16 = np.zeros((k, 16))
     dj = np.empty(k)
-    for c in range(k):
-        d = np.empty(1)
-        if gp[c] == 0:
-            c9 = np.empty(9)
-            load().orc_connect_coeffs_p2p(
-                _p(_c(_elem_X(nodcoord, elemnod, pair_elem[c]), np.float64).ravel(), C.c_double),
-                _p(_c(position[c], np.float64), C.c_double), _p(c9, C.c_double))
-            c16[c, :9] = c9
-            dj[c] = 1.0
-            continue
-        load().orc_connect_coeffs(_p(_c(_elem_X(nodcoord, elemnod, elem[c]), np.float64).ravel(), C.c_double),
-                                  int(gp[c]),
-                                  _p(_c(_elem_X(nodcoord, elemnod, pair_elem[c]), np.float64).ravel(), C.c_double),
-                                  _p(_c(position[c], np.float64), C.c_double), _p(c16[c], C.c_double),
-                                  _p(d, C.c_double))
-        dj[c] = d[0]
+    if k == 0:
+        return c16, dj
+    xyz = _fl(nodcoord, np.float64)
+    en = _fl(elemnod, np.int32)
+    load().orc_connect_coeffs_all(np.asarray(nodcoord).shape[0], np.asarray(elemnod).shape[0], _p(xyz, C.c_double),
+                                  _p(en, C.c_int32), k, _p(_c(elem, np.int32), C.c_int32),
+                                  _p(_c(gp, np.int32), C.c_int32), _p(_c(pair_elem, np.int32), C.c_int32),
+                                  _p(_c(position, np.float64).ravel(), C.c_double), _p(c16, C.c_double),
+                                  _p(dj, C.c_double))
     return c16, dj
 
 

@@ -1096,6 +1096,28 @@ void orc_connect_coeffs(const double *Xmy, int gp /*1-based*/, const double *Xpa
     for (int a = 0; a < 8; ++a) c16[8 + a] = 0.0 - hex8_N(a, xi);
 }
 
+/* connect coefficients of ncon connects of one (joint) mesh: elem/pair 1-based element ids
+ * (elem = the own node for gp == 0, P2P), gp 1-based Gauss point or 0; c16 16 per connect
+ * (P2P: [1, -N_pair(xi), 0...]), detJ (1 for P2P) */
+void orc_connect_coeffs_all(int64_t nn, int64_t ne, const double *xyz, const int32_t *en, int64_t ncon,
+                            const int32_t *elem, const int32_t *gp, const int32_t *pair, const double *pos,
+                            double *c16, double *detJ)
+{
+    for (int64_t c = 0; c < ncon; ++c) {
+        double Xp[24];
+        elem_X(nn, ne, xyz, en, pair[c] - 1, Xp);
+        for (int a = 0; a < 16; ++a) c16[c * 16 + a] = 0.0;
+        if (gp[c] == 0) {
+            orc_connect_coeffs_p2p(Xp, pos + 3 * c, c16 + 16 * c);
+            detJ[c] = 1.0;
+        } else {
+            double Xm[24];
+            elem_X(nn, ne, xyz, en, elem[c] - 1, Xm);
+            orc_connect_coeffs(Xm, gp[c], Xp, pos + 3 * c, c16 + 16 * c, detJ + c);
+        }
+    }
+}
+
 /* multi-domain setCRS pattern: the union of the domains' ZeroMatrix(DOF=3) rows (the union
  * mesh is block diagonal) and every node pair of every connect, columns ascending; all rows
  * empty when no overset was set up (any_overset = 0).  nodes16: per connect the 16 union node

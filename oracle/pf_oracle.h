@@ -147,6 +147,9 @@ int64_t orc_overset_p2p(int64_t nnA, const double *xyzA, int64_t nnB, int64_t ne
 void orc_connect_coeffs_p2p(const double *Xpair, const double *pos, double *c9);
 void orc_local_coordinate(const double *X, const double *p, double *xi);
 void orc_connect_coeffs(const double *Xmy, int gp, const double *Xpair, const double *pos, double *c16, double *detJ);
+void orc_connect_coeffs_all(int64_t nn, int64_t ne, const double *xyz, const int32_t *en, int64_t ncon,
+                            const int32_t *elem, const int32_t *gp, const int32_t *pair, const double *pos,
+                            double *c16, double *detJ);
 int64_t orc_multi_crs(int64_t nn, int64_t ne, const int32_t *en, int any_overset, int64_t ncon, const int32_t *nodes16,
                       int64_t *row_ptr, int32_t *col_idx);
 int orc_ebom(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, int64_t ncon,
