@@ -35,7 +35,7 @@ def graph_2x1x1():
 def main():
     data = {
         "graph_1x1x1_node1": {
-            "source": "SURVEY of FEMDomainClass.f90:10693-10696 on Cube3D 1x1x1 (hex local order)",
+            "source": "FEMDomainClass.f90:10693-10696 applied by hand to Cube3D 1x1x1 (hex local order)",
             "row": [1, 2, 4, 3, 5, 6, 8, 7]},
         "graph_2x1x1_dof1": graph_2x1x1(),
         "cube3d_2x1x1_elemnod": {"source": "MeshClass.f90:6796-6806 (2-D) + :7065-7070 (layers)",
