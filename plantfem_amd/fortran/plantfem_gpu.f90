@@ -2,7 +2,8 @@
 ! path, calling the gfx950 library libpfgpu.so through iso_c_binding (include/pfgpu.h).
 !
 ! The type-bound names and argument meanings follow the reference:
-!   FEMDomain_%create("Cube3D",x_num,y_num,z_num,x_len,y_len,z_len)  FEMDomainClass.f90:7215-7310
+!   FEMDomain_%create("Cube3D"|"Leaf3D"|"Cylinder",x_num,...,z_len)  FEMDomainClass.f90:7215-7310,
+!                                                                   MeshClass.f90:6017-6276, :6506-6743
 !   FEMDomain_%ZeroMatrix(DOF)                                      FEMDomainClass.f90:10581-10840
 !   FEMDomain_%StiffnessMatrix(ElementID,E,v)                       FEMDomainClass.f90:11457-11515
 !   FEMDomain_%StiffnessMatrix(YoungModulus,PoissonRatio)           FEMDomainClass.f90:11123-11232
@@ -431,16 +432,14 @@ contains
 
    ! ------------------------------------------------------------------------ FEMDomain_
    ! Cube3D numbering (MeshClass.f90:6745-6808, :6992-7090): x fastest, lx/dble(xn)*dble(i-1)
-   subroutine createFEMDomain(this, meshtype, x_num, y_num, z_num, x_len, y_len, z_len)
+   subroutine createFEMDomain(this, meshtype, x_num, y_num, z_num, x_len, y_len, z_len, shaperatio, species, &
+                              SoyWidthRatio)
       class(FEMDomain_), intent(inout) :: this
       character(*), intent(in) :: meshtype
-      integer(int32), optional, intent(in) :: x_num, y_num, z_num
-      real(real64), optional, intent(in) :: x_len, y_len, z_len
-      integer(int32) :: xn, yn, zn, i, j, k, n, e, a
-      integer(int64) :: nn2, ne2
+      integer(int32), optional, intent(in) :: x_num, y_num, z_num, species
+      real(real64), optional, intent(in) :: x_len, y_len, z_len, shaperatio, SoyWidthRatio
+      integer(int32) :: xn, yn, zn
       real(real64) :: lx, ly, lz
-      integer(int32) :: q(4)
-      if (meshtype /= "Cube3D" .and. meshtype /= "Cube") stop "plantfem_gpu: only Cube3D is on this path"
       xn = 10; yn = 10; zn = 10; lx = 1.0d0; ly = 1.0d0; lz = 1.0d0
       if (present(x_num)) xn = x_num
       if (present(y_num)) yn = y_num
@@ -448,19 +447,39 @@ contains
       if (present(x_len)) lx = x_len
       if (present(y_len)) ly = y_len
       if (present(z_len)) lz = z_len
+      select case (meshtype)
+      case ("Cube3D", "Cube", "cube3D", "rectangular3D")
+         call cube3d_mesh(this%mesh, xn, yn, zn, lx, ly, lz)
+      case ("Leaf3D")
+         call cube3d_mesh(this%mesh, xn, yn, zn, lx, ly, lz)
+         call leaf3d_shape(this%mesh, shaperatio, species, SoyWidthRatio)
+      case ("Cylinder", "Cylinder3D")
+         call cylinder3d_mesh(this%mesh, xn, yn, zn, lx, ly, lz)
+      case default
+         stop "plantfem_gpu: meshtype is Cube3D, Cube, Cylinder or Leaf3D on this path"
+      end select
+   end subroutine
+
+   subroutine cube3d_mesh(mesh, xn, yn, zn, lx, ly, lz)
+      type(Mesh_), intent(inout) :: mesh
+      integer(int32), intent(in) :: xn, yn, zn
+      real(real64), intent(in) :: lx, ly, lz
+      integer(int32) :: i, j, k, n, e, a
+      integer(int64) :: nn2, ne2
+      integer(int32) :: q(4)
       nn2 = int(xn + 1, int64)*(yn + 1)
       ne2 = int(xn, int64)*yn
-      if (allocated(this%mesh%NodCoord)) deallocate (this%mesh%NodCoord)
-      if (allocated(this%mesh%ElemNod)) deallocate (this%mesh%ElemNod)
-      allocate (this%mesh%NodCoord(nn2*(zn + 1), 3), this%mesh%ElemNod(ne2*zn, 8))
+      if (allocated(mesh%NodCoord)) deallocate (mesh%NodCoord)
+      if (allocated(mesh%ElemNod)) deallocate (mesh%ElemNod)
+      allocate (mesh%NodCoord(nn2*(zn + 1), 3), mesh%ElemNod(ne2*zn, 8))
       n = 0
       do k = 1, zn + 1
          do j = 1, yn + 1
             do i = 1, xn + 1
                n = n + 1
-               this%mesh%NodCoord(n, 1) = lx/dble(xn)*dble(i - 1)
-               this%mesh%NodCoord(n, 2) = ly/dble(yn)*dble(j - 1)
-               this%mesh%NodCoord(n, 3) = lz/dble(zn)*dble(k - 1)
+               mesh%NodCoord(n, 1) = lx/dble(xn)*dble(i - 1)
+               mesh%NodCoord(n, 2) = ly/dble(yn)*dble(j - 1)
+               mesh%NodCoord(n, 3) = lz/dble(zn)*dble(k - 1)
             end do
          end do
       end do
@@ -472,12 +491,218 @@ contains
                q = [i + (j - 1)*(xn + 1), i + 1 + (j - 1)*(xn + 1), xn + 2 + i + (j - 1)*(xn + 1), &
                     xn + 1 + i + (j - 1)*(xn + 1)]
                do a = 1, 4
-                  this%mesh%ElemNod(e, a) = q(a) + int(nn2, int32)*(k - 1)
-                  this%mesh%ElemNod(e, a + 4) = q(a) + int(nn2, int32)*k
+                  mesh%ElemNod(e, a) = q(a) + int(nn2, int32)*(k - 1)
+                  mesh%ElemNod(e, a + 4) = q(a) + int(nn2, int32)*k
                end do
             end do
          end do
       end do
+   end subroutine
+
+   ! Leaf3D (MeshClass.f90:6017-6276): the rectangular3D mesh (x width, y thickness, z length)
+   ! centred in x and y, then the default outline (shaperatio) or the soybean one (species 1)
+   subroutine leaf3d_shape(mesh, shaperatio, species, SoyWidthRatio)
+      type(Mesh_), intent(inout) :: mesh
+      real(real64), optional, intent(in) :: shaperatio, SoyWidthRatio
+      integer(int32), optional, intent(in) :: species
+      real(real64) :: s, thickness_, width, thickness_ratio, zl, z20, zc, ratio, length, sw, alpha, r, xx, yy, yy_, &
+                      xmax, xmin, theta
+      real(real64), parameter :: lin_curve_ratio = 0.50d0
+      integer(int32) :: i, sp
+      mesh%NodCoord(:, 1) = mesh%NodCoord(:, 1) - (maxval(mesh%NodCoord(:, 1)) - minval(mesh%NodCoord(:, 1)))*0.50d0
+      mesh%NodCoord(:, 2) = mesh%NodCoord(:, 2) - (maxval(mesh%NodCoord(:, 2)) - minval(mesh%NodCoord(:, 2)))*0.50d0
+      sp = 0
+      if (present(species)) sp = species
+      if (sp == 0) then
+         s = 0.30d0
+         if (present(shaperatio)) s = shaperatio
+         thickness_ = maxval(mesh%NodCoord(:, 2)) - minval(mesh%NodCoord(:, 2))
+         width = maxval(mesh%NodCoord(:, 1)) - minval(mesh%NodCoord(:, 1))
+         thickness_ratio = (width*1.0d0/10.0d0)/thickness_
+         zl = maxval(mesh%NodCoord(:, 3)) - minval(mesh%NodCoord(:, 3))
+         z20 = 1.0d0/20.0d0*zl
+         do i = 1, size(mesh%NodCoord, 1)
+            zc = mesh%NodCoord(i, 3)
+            if (zc <= z20) then
+               ratio = 1.0d0/10.0d0
+               mesh%NodCoord(i, 2) = mesh%NodCoord(i, 2)*thickness_ratio
+            elseif (zc <= zl*s) then
+               ratio = 1.0d0/10.0d0 + 0.90d0/(zl*s - z20)*(zc - z20)
+            else
+               ratio = 1.0d0 - 0.90d0/(zl - s*zl)*(zc - s*zl)
+            end if
+            mesh%NodCoord(i, 1) = mesh%NodCoord(i, 1)*ratio
+         end do
+      elseif (sp == 1) then  ! PF_GLYCINE_MAX
+         sw = 0.50d0
+         if (present(SoyWidthRatio)) sw = SoyWidthRatio
+         if (sw >= 1.0d0 .or. sw <= 0.0d0) stop "ERROR  >> mesh%create(leaf3d, PF_SOYBEAN) >> invalid SoyWidthRatio"
+         length = maxval(mesh%NodCoord(:, 3)) - minval(mesh%NodCoord(:, 3))
+         width = maxval(mesh%NodCoord(:, 1)) - minval(mesh%NodCoord(:, 1))
+         do i = 1, size(mesh%NodCoord, 1)
+            xmax = maxval(mesh%NodCoord(:, 1))
+            xmin = minval(mesh%NodCoord(:, 1))
+            xx = mesh%NodCoord(i, 3)
+            if (mesh%NodCoord(i, 1) <= (xmax + xmin)*0.50d0) then
+               alpha = sw*width
+            else
+               alpha = (1.0d0 - sw)*width
+            end if
+            r = (alpha*alpha + (length - alpha)*(length - alpha))/(2*alpha)*1.20d0
+            if (xx <= 1.0d0/25.0d0*length) then
+               mesh%NodCoord(i, 1) = mesh%NodCoord(i, 1)*1.0d0/10.0d0
+               mesh%NodCoord(i, 2) = mesh%NodCoord(i, 2)*5.0d0
+               cycle
+            elseif (xx < alpha) then
+               yy = sqrt(alpha*alpha - (xx - alpha)*(xx - alpha))
+               yy_ = xx
+            else
+               yy_ = alpha + (-alpha)/(length - alpha)*(xx - alpha)
+               yy = alpha - r + sqrt(r*r - (xx - alpha)*(xx - alpha))
+            end if
+            yy = abs(lin_curve_ratio*yy + (1.0d0 - lin_curve_ratio)*yy_)
+            mesh%NodCoord(i, 1) = mesh%NodCoord(i, 1)*(yy/alpha)
+         end do
+         do i = 1, size(mesh%NodCoord, 1)
+            theta = abs(mesh%NodCoord(i, 1)/(width/2.0d0))
+            mesh%NodCoord(i, 2) = mesh%NodCoord(i, 2) - 0.10d0*theta*(theta - 1.0d0)
+         end do
+      else
+         stop "[ERROR] Mesh%create => No such species"
+      end if
+   end subroutine
+
+   ! Cylinder (MeshClass.f90:6506-6522): Circle2D (:6524-6743) on a unit disc, cleanMesh,
+   ! Convert2Dto3D(thickness=lz, division=zn), resized to lx, ly
+   subroutine cylinder3d_mesh(mesh, x_num, y_num, zn, lx, ly, lz)
+      type(Mesh_), intent(inout) :: mesh
+      integer(int32), intent(in) :: x_num, y_num, zn
+      real(real64), intent(in) :: lx, ly, lz
+      real(real64), allocatable :: c2(:, :), c3(:, :)
+      integer(int32), allocatable :: q2(:, :), used(:), newid(:)
+      real(real64) :: xx, yy, RR, theta, alpha, rr_, ang, cf
+      real(real64), parameter :: pi = 3.14159265358979323846d0
+      integer(int32) :: xn, m, sq, nsq, nring, i, j, k, n, ini, nn2, ne2, e, a
+      xn = max(x_num/2 + 1, y_num/2 + 1)
+      m = 2*xn
+      nsq = (m + 1)*(m + 1)
+      nring = m*4*m
+      sq = nsq
+      allocate (c2(nsq + nring, 2), q2(m*m + 8*xn*(xn + 2), 4))
+      ! rectangular2D (2m x 2m, side 2) shifted to [-1, 1]^2, then rounded
+      n = 0
+      do j = 1, m + 1
+         do i = 1, m + 1
+            n = n + 1
+            c2(n, 1) = 2.0d0/dble(m)*dble(i - 1) - 1.0d0
+            c2(n, 2) = 2.0d0/dble(m)*dble(j - 1) - 1.0d0
+         end do
+      end do
+      e = 0
+      do j = 1, m
+         do i = 1, m
+            e = e + 1
+            q2(e, :) = [i + (j - 1)*(m + 1), i + 1 + (j - 1)*(m + 1), m + 2 + i + (j - 1)*(m + 1), &
+                        m + 1 + i + (j - 1)*(m + 1)]
+         end do
+      end do
+      theta = (1.0d0 - 0.80d0)/(1.0d0 - 1.0d0/sqrt(2.0d0))
+      do i = 1, nsq
+         xx = c2(i, 1)
+         yy = c2(i, 2)
+         RR = 1.00d0
+         if (xx /= 0.0d0) then
+            if (abs(yy/xx) > 1.0d0) then
+               if (yy /= 0.0d0) RR = sqrt((xx/yy)*(xx/yy) + 1)
+            else
+               RR = sqrt((yy/xx)*(yy/xx) + 1)
+            end if
+         end if
+         alpha = theta*((1/RR) - 1.0d0/sqrt(2.0d0)) + 0.790d0
+         c2(i, 1) = (xx*alpha)*(sqrt(2.0d0))*0.95d0
+         c2(i, 2) = (yy*alpha)*(sqrt(2.0d0))*0.95d0
+      end do
+      cf = dble(2*xn - 1)/dble(2*xn)
+      c2(1:nsq, 1) = cf*c2(1:nsq, 1)/sqrt(2.0d0)
+      c2(1:nsq, 2) = cf*c2(1:nsq, 2)/sqrt(2.0d0)
+      do i = 1, m
+         rr_ = 1.0d0 + dble(i)*(1.0d0/dble(m))
+         do j = 1, 4*m
+            ang = 2.0d0*pi/4.0d0/dble(m)*dble(j - 1)
+            c2(nsq + (i - 1)*4*m + j, 1) = rr_*cos(ang)
+            c2(nsq + (i - 1)*4*m + j, 2) = rr_*sin(ang)
+         end do
+      end do
+      ! the square's boundary tied to ring 1, then ring layers 1..xn+1
+      j = 0
+      do i = 1, xn
+         j = j + 1
+         q2(m*m + j, :) = [(m + 1)*(xn + i), sq + j, sq + j + 1, (m + 1)*(xn + i + 1)]
+      end do
+      do i = 1, m
+         j = j + 1
+         q2(m*m + j, :) = [sq - i + 1, sq + j, sq + j + 1, sq - i]
+      end do
+      do i = 1, m
+         j = j + 1
+         q2(m*m + j, :) = [sq - (m + 1) + 1 - (i - 1)*(m + 1), sq + j, sq + j + 1, sq - (m + 1) + 1 - i*(m + 1)]
+      end do
+      do i = 1, m
+         j = j + 1
+         q2(m*m + j, :) = [i, sq + j, sq + j + 1, i + 1]
+      end do
+      do i = 1, xn
+         j = j + 1
+         q2(m*m + j, :) = [(m + 1)*i, sq + j, sq + j + 1, (m + 1)*(i + 1)]
+      end do
+      q2(m*m + j, 3) = sq + 1
+      do i = 1, xn + 1
+         ini = j + 1
+         do k = 1, 8*xn - 1
+            j = j + 1
+            q2(m*m + j, :) = [sq + j - 8*xn, sq + j, sq + j + 1, sq + j + 1 - 8*xn]
+         end do
+         j = j + 1
+         q2(m*m + j, :) = [sq + j - 8*xn, sq + j, sq + ini, sq + ini - 8*xn]
+      end do
+      ! Circle2D's resize to a unit disc (over every node, the unused outer rings included)
+      c2(:, 1) = 1.0d0/(maxval(c2(:, 1)) - minval(c2(:, 1)))*c2(:, 1)
+      c2(:, 2) = 1.0d0/(maxval(c2(:, 2)) - minval(c2(:, 2)))*c2(:, 2)
+      ! cleanMesh: drop the unused nodes
+      allocate (used(size(c2, 1)), newid(size(c2, 1)))
+      used = 0
+      do e = 1, size(q2, 1)
+         do a = 1, 4
+            used(q2(e, a)) = 1
+         end do
+      end do
+      nn2 = 0
+      do i = 1, size(c2, 1)
+         if (used(i) == 1) nn2 = nn2 + 1
+         newid(i) = nn2
+      end do
+      ne2 = size(q2, 1)
+      ! Convert2Dto3D and the final resize
+      if (allocated(mesh%NodCoord)) deallocate (mesh%NodCoord)
+      if (allocated(mesh%ElemNod)) deallocate (mesh%ElemNod)
+      allocate (c3(nn2, 2), mesh%NodCoord(nn2*(zn + 1), 3), mesh%ElemNod(ne2*zn, 8))
+      do i = 1, size(c2, 1)
+         if (used(i) == 1) c3(newid(i), :) = c2(i, :)
+      end do
+      do k = 1, zn + 1
+         mesh%NodCoord(nn2*(k - 1) + 1:nn2*k, 1:2) = c3
+         mesh%NodCoord(nn2*(k - 1) + 1:nn2*k, 3) = lz/dble(zn)*dble(k - 1)
+      end do
+      do k = 1, zn
+         do e = 1, ne2
+            do a = 1, 4
+               mesh%ElemNod(ne2*(k - 1) + e, a) = newid(q2(e, a)) + nn2*(k - 1)
+               mesh%ElemNod(ne2*(k - 1) + e, a + 4) = newid(q2(e, a)) + nn2*k
+            end do
+         end do
+      end do
+      mesh%NodCoord(:, 1) = lx/(maxval(mesh%NodCoord(:, 1)) - minval(mesh%NodCoord(:, 1)))*mesh%NodCoord(:, 1)
+      mesh%NodCoord(:, 2) = ly/(maxval(mesh%NodCoord(:, 2)) - minval(mesh%NodCoord(:, 2)))*mesh%NodCoord(:, 2)
    end subroutine
 
    integer(int32) function nnFEMDomain(this)

@@ -10,7 +10,7 @@ BIN = ROOT / "plantfem_amd" / "lib" / "bin"
 
 
 def test_fortran_binaries_built_and_link():
-    for name in ("cube_static", "plugin_solver", "modal_bar", "overset_organs"):
+    for name in ("cube_static", "plugin_solver", "modal_bar", "overset_organs", "organ_meshes"):
         assert (BIN / name).exists(), name
     out = subprocess.run([str(BIN / "cube_static"), "--version"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0 and "plantfem_gpu library version" in out.stdout
@@ -134,3 +134,26 @@ def test_fortran_overset_organs_matches_oracle(tmp_path, alg):
     xo, st, *_ = O.solve(rp, ci, val, rhs, fixed, np.zeros(n), relative_er=1e-12, er0=1e-30)
     assert st == 0 and x.size == n
     assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_fortran_organ_meshes_match_python(tmp_path):
+    # FEMDomain_%create("Leaf3D" | "Cylinder") of the Fortran module and plantfem_amd.fem's
+    # generators give the same meshes, bit for bit (host only, no device needed)
+    from plantfem_amd.fem import PF_GLYCINE_MAX, cylinder3d, leaf3d
+    out = tmp_path / "m.bin"
+    res = subprocess.run([str(BIN / "organ_meshes"), str(out)], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = out.read_bytes()
+    off = 0
+    refs = [leaf3d(8, 2, 20, 0.2, 0.01, 1.0, shaperatio=0.3),
+            leaf3d(8, 2, 30, 0.1, 0.01, 0.3, species=PF_GLYCINE_MAX, SoyWidthRatio=0.5),
+            cylinder3d(10, 10, 6, 0.08, 0.08, 0.6)]
+    for nod, el in refs:
+        nn, ne = np.frombuffer(raw, np.int64, 2, off)
+        off += 16
+        x = np.frombuffer(raw, np.float64, nn * 3, off).reshape(3, nn).T
+        off += 24 * nn
+        e = np.frombuffer(raw, np.int32, ne * 8, off).reshape(8, ne).T
+        off += 32 * ne
+        assert np.array_equal(x, nod) and np.array_equal(e, el)
+    assert off == len(raw)
