@@ -243,6 +243,50 @@ __global__ void __launch_bounds__(BT) k_spmv_e(Mat<V> m, const V *__restrict__ r
   write_partials<2>(acc, part);
 }
 
+// K2 in two parts for the CRS layout, where every nonzero gathers its own x entry and the
+// fused form would gather r and y for each: c2, alpha and e = r - alpha y as a stream, then
+// v = A e with plain gathers.  e and v come out bit-identical to the fused kernel's.
+template <typename V>
+__global__ void __launch_bounds__(BT) k_evec(int64_t n, const V *__restrict__ r, const V *__restrict__ y,
+                                             V *__restrict__ e, Scal *__restrict__ sc,
+                                             const double *__restrict__ pin, int Gin) {
+  if (sc->done) return;
+  double t[1];
+  read_partials<1>(pin, Gin, t);
+  const double c2 = t[0];
+  const double alpha_d = sc->c1 / c2;
+  const bool first = threadIdx.x == 0 && blockIdx.x == 0;
+  if (first) {
+    sc->c2 = c2;
+    sc->alpha = alpha_d;
+  }
+  if (sc->guard && !isfinite(alpha_d)) {  // r0.Ap lost: stop before x is touched
+    if (first) {
+      sc->status = PF_BREAKDOWN_C2C3;
+      sc->done = 1;
+    }
+    return;
+  }
+  const EFromRY<V> ef{r, y, (V)alpha_d};
+  for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) e[i] = ef(i);
+}
+
+template <typename V, int L, int SUB>
+__global__ void __launch_bounds__(BT) k_spmv_ev(Mat<V> m, const V *__restrict__ e, V *__restrict__ v,
+                                                Scal *__restrict__ sc, double *__restrict__ part) {
+  if (sc->done) return;
+  const pfdev::PlainX<V> ef{e};
+  double acc[2] = {0.0, 0.0};
+  auto op = [&](int64_t row, V vv) {
+    v[row] = vv;
+    const V ee = e[row];
+    acc[0] += (double)ee * (double)vv;
+    acc[1] += (double)vv * (double)vv;
+  };
+  for_rows<V, L, SUB>(m, ef, op);
+  write_partials<2>(acc, part);
+}
+
 // ---- K3: omega ; x += alpha p + omega e ; r = e - omega v -> partials r.r, r0.r --------
 template <typename V>
 __global__ void __launch_bounds__(BT) k_update(int64_t n, V *__restrict__ x, V *__restrict__ r,
@@ -636,7 +680,7 @@ struct SolverWork {
   int block_ok = -1;  // -1 unknown
   const void *key_val = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  uint64_t gkey[2][20] = {};  // everything a captured chunk bakes in
+  uint64_t gkey[2][21] = {};  // everything a captured chunk bakes in
   std::vector<hipEvent_t> evs;  // kernel_timing: 5 per iteration of a chunk
   double t_sum[4] = {0, 0, 0, 0};
   int64_t t_count[4] = {0, 0, 0, 0};
@@ -763,7 +807,8 @@ int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V> &m, const V *b, V *x
 
 template <typename V>
 int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V> &m, V *x, V *r, const V *r0, V *p, V *y, V *v,
-                  Scal *sc, double *part, double *hist, int count, const Rel *rel, hipEvent_t *ev = nullptr) {
+                  Scal *sc, double *part, double *hist, int count, const Rel *rel, V *ebuf,
+                  hipEvent_t *ev = nullptr) {
   PF_CHECK(check_grid(ctx, L));
   hipStream_t st = ctx->stream;
   const size_t G = (size_t)ctx->num_cus * 8;
@@ -779,7 +824,12 @@ int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V> &m, V *x, V *r, con
     } else {
       PF_DISPATCH_SUB(L.sub, (k_spmv_p<V, 0, S_><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, p1)));
       if (ev) PF_HIP(hipEventRecord(ev[5 * it + 1], st));
-      PF_DISPATCH_SUB(L.sub, (k_spmv_e<V, 0, S_><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, p1, g1, p2)));
+      if (ebuf) {
+        k_evec<V><<<L.grid_e, BT, 0, st>>>(m.n, r, y, ebuf, sc, p1, g1);
+        PF_DISPATCH_SUB(L.sub, (k_spmv_ev<V, 0, S_><<<L.grid, BT, 0, st>>>(m, ebuf, v, sc, p2)));
+      } else {
+        PF_DISPATCH_SUB(L.sub, (k_spmv_e<V, 0, S_><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, p1, g1, p2)));
+      }
     }
     if (ev) PF_HIP(hipEventRecord(ev[5 * it + 2], st));
     k_update<V><<<L.grid_e, BT, 0, st>>>(m.n, x, r, r0, p, y, v, sc, p2, g1, p3);
@@ -796,7 +846,7 @@ int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V> &m, V *x, V *r, con
 // Run the iteration loop of one BiCGSTAB (scalars slot `slot`) to its stop.
 template <typename V>
 int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V> &m, V *x, V *r, const V *r0,
-             V *p, V *y, V *v, double *hist, const pf_solve_opts &o, const Rel *rel = nullptr) {
+             V *p, V *y, V *v, double *hist, const pf_solve_opts &o, const Rel *rel = nullptr, V *ebuf = nullptr) {
   Scal *sc = w->sc.p + slot;
   Scal *hs = w->h_sc + slot;
   int chunk = o.check_every > 0 ? o.check_every : 32;
@@ -810,7 +860,8 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V>
       w->evs.push_back(e);
     }
   }
-  uint64_t key[20] = {(uint64_t)L.grid, (uint64_t)L.grid_e * 131 + (uint64_t)L.sub, (uint64_t)L.layout,
+  uint64_t key[21] = {(uint64_t)(uintptr_t)ebuf, (uint64_t)L.grid, (uint64_t)L.grid_e * 131 + (uint64_t)L.sub,
+                      (uint64_t)L.layout,
                       (uint64_t)chunk, (uint64_t)m.n, (uint64_t)(uintptr_t)m.val, (uint64_t)(uintptr_t)m.sval,
                       (uint64_t)(uintptr_t)m.col, (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)r,
                       (uint64_t)(uintptr_t)r0, (uint64_t)(uintptr_t)p, (uint64_t)(uintptr_t)y,
@@ -824,7 +875,7 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V>
     }
     hipGraph_t graph;
     PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel);
+    int rc = enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel, ebuf);
     hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
     if (rc != PF_OK) return rc;
     if (ce != hipSuccess) {
@@ -845,7 +896,7 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V>
     if (use_graph) {
       PF_HIP(hipGraphLaunch(w->gexec[slot], ctx->stream));
     } else {
-      PF_CHECK(enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel,
+      PF_CHECK(enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel, ebuf,
                                 timing ? w->evs.data() : nullptr));
     }
     PF_HIP(hipMemcpyAsync(hs, sc, sizeof(Scal), hipMemcpyDeviceToHost, ctx->stream));
@@ -1143,7 +1194,13 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     const int guard = o.stop_mode == 1 ? 1 : 0;  // mode 0 keeps the reference's unguarded arithmetic
     PF_CHECK(prepare_scalars(ctx, w, 0, o, o.itrmax, o.relative_er, o.stop_mode == 1 ? 1 : 0, 0, guard));
     PF_CHECK(enqueue_init<double>(ctx, L64, m64, b, x, w->r.p, w->r0.p, w->p.p, w->sc.p, w->part.p, false));
-    PF_CHECK(run_loop<double>(ctx, w, 0, L64, m64, x, w->r.p, w->r0.p, w->p.p, w->y.p, w->v.p, nullptr, o));
+    double *ebuf = nullptr;
+    if (layout == 0) {  // CRS: e as a stream, then the product gathers e alone
+      PF_CHECK(w->e.ensure(n));
+      ebuf = w->e.p;
+    }
+    PF_CHECK(run_loop<double>(ctx, w, 0, L64, m64, x, w->r.p, w->r0.p, w->p.p, w->y.p, w->v.p, nullptr, o, nullptr,
+                              ebuf));
     const Scal &h = w->h_sc[0];
     res.status = h.status;
     // the reference's loop counter: a v.v == 0 stop happens inside iteration itr+1
