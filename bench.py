@@ -71,7 +71,9 @@ def main(argv=None):
     ap.add_argument("--size", type=int, default=100)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--cpu-iters", type=int, default=20, help="oracle iterations timed for the CPU baseline")
+    ap.add_argument("--mixed-repeats", type=int, default=5)
+    ap.add_argument("--no-c3", action="store_true", help="skip the config 3 (sigma 2.0, 1e4 contrast) leg")
     ap.add_argument("--no-graph", action="store_true", help="direct kernel launches only (for profilers)")
     ap.add_argument("--workload", choices=("box", "batch", "adjoint", "precond", "modal", "overset"), default="box")
     ap.add_argument("--plant-scale", type=int, default=2, help="refinement of the overset plant (1: 23k DOF)")
@@ -155,14 +157,14 @@ def main(argv=None):
         log("profile: 3 x 512 MiB device copies done")
         return
 
-    def step(precision=64, timing=0, target=1e-8):
+    def step(precision=64, timing=0, target=1e-8, stop_mode=1):
         s.setCRS(DOF=3)                       # ZeroMatrix graph build (device)
         s.setStiffness()                      # element stiffness assembly (device)
         s.addTraction(w.traction_box, w.pressure)
         s.fix(1, w.fixed_dofs, FixValue=0.0)
         s.relative_er = target
-        s.solve(precision=precision, stop_mode=1, download=False, kernel_timing=timing, target_relres=target,
-                raise_on_breakdown=False, use_graph=0 if a.no_graph else 1)
+        s.solve(precision=precision, stop_mode=stop_mode, download=False, kernel_timing=timing,
+                target_relres=target, raise_on_breakdown=False, use_graph=0 if a.no_graph else 1)
         return dict(s.last_stats)
 
     # ---- FP64 time to solve (timed steps carry per-kernel hipEvent timing) -------------
@@ -188,13 +190,17 @@ def main(argv=None):
     st_graph = step(64, 0)
     ctx.synchronize()
     t_graph = time.perf_counter() - t
+    # the same solve stopped by the reference's own rule (bicgstab_CRS_2, er0 = relative_er =
+    # dble(1.0e-10)), for its iteration count
+    st_rc = step(64, 0, target=L.ER0_DEFAULT, stop_mode=0)
+    log(f"reference criterion: {st_rc['iterations']} it, relres {st_rc['true_relres']:.2e}")
 
     # ---- mixed precision on the same matrix --------------------------------------------
     # FP32 BiCGSTAB on D^-1 A, restarted at every FP64 true-residual replacement; the step
     # is the same as the FP64 one (graph build, assembly, BCs, solve to relres 1e-8)
     x64 = s.getX()
     mixed_times, st32 = [], None
-    for i in range(max(1, a.steps)):
+    for i in range(max(1, min(a.steps, a.mixed_repeats))):
         ctx.synchronize()
         t = time.perf_counter()
         st32 = step(32, 0)
@@ -204,24 +210,36 @@ def main(argv=None):
             f"inner {st32['iterations']}, relres {st32['true_relres']:.2e}")
     t_mixed = float(np.median(mixed_times))
     x32 = s.getX()
-    # the same to relres 1e-10, and a tight FP64 reference (relres ~1e-11) both are checked against
     ctx.synchronize()
     t = time.perf_counter()
     st32t = step(32, 0, target=1e-10)
     ctx.synchronize()
     t_mixed_tight = time.perf_counter() - t
     x32t = s.getX()
-    st_ref = step(64, 0, target=1e-12)
+    # reference: FP64 BiCGSTAB on the PointJacobi-scaled system to a scaled relres of 1e-13; its
+    # residual on the unscaled system is certified on the CPU (oracle elimination + product)
+    s.setCRS(DOF=3)
+    s.setStiffness()
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    crs2_pre, rhs2_pre = (s.getCRS(), s.getRHS()) if not a.no_cpu else (None, None)
+    s.relative_er = 1e-13
+    s.itrmax = 100000
+    s.solve(preconditioning="PointJacobi", stop_mode=1, download=False, raise_on_breakdown=False)
+    st_ref = dict(s.last_stats)
     x_ref = s.getX()
+    s.itrmax = 40000
 
-    def rel(u):
-        return float(np.linalg.norm(u - x_ref) / np.linalg.norm(x_ref))
+    def rel(u, ref=x_ref):
+        return float(np.linalg.norm(u - ref) / np.linalg.norm(ref))
 
     mixed_err = rel(x32)
     log(f"mixed: {t_mixed:.3f} s to 1e-8 (err {mixed_err:.2e}), {t_mixed_tight:.3f} s to 1e-10 "
-        f"(err {rel(x32t):.2e}); FP64 err {rel(x64):.2e}; reference relres {st_ref['true_relres']:.2e}")
+        f"(err {rel(x32t):.2e}); FP64 err {rel(x64):.2e}; reference: {st_ref['iterations']} PointJacobi "
+        f"iterations, scaled relres {st_ref['true_relres']:.2e}")
 
     # ---- SpMV kernels on the eliminated matrix ----------------------------------------
+    step(64, 0)  # the unscaled eliminated system again
     spmv = {}
     xr = np.random.default_rng(1).standard_normal(n)
     for kind, name in ((1, "crs_f64"), (2, "blocked_f64"), (3, "blocked_f32")):
@@ -246,23 +264,18 @@ def main(argv=None):
                 "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_spmv_p (blocked FP64 y=A p fused with r0.y)", "bytes_per_launch": k1_bytes,
                 "avg_launch_ms": round(k1_ms, 4)}
+    s.close()
 
-    # ---- CPU baseline: oracle BiCGSTAB iterations on the host ---------------------------
+    # ---- config 3: mixed precision where FP32 robustness is at stake -------------------
+    c3 = None if a.no_c3 else bench_c3(a, ctx)
+
+    # ---- CPU baseline (the oracle on the host cores) and the certification of the references
     cpu = None
     if not a.no_cpu:
-        from oracle import oracle as O
-        crs = s.getCRS()   # eliminated system of the last solve
-        rhs = s.getRHS()
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-        O.bicgstab(crs.row_ptr, crs.col_idx, crs.val, rhs, itrmax=1, nthreads=threads)  # warm
-        t = time.perf_counter()
-        O.bicgstab(crs.row_ptr, crs.col_idx, crs.val, rhs, itrmax=a.cpu_iters, nthreads=threads)
-        t_it = (time.perf_counter() - t) / a.cpu_iters
-        cpu = {"value": round(t_it * st64["iterations"], 2), "unit": "s", "cores": threads, "kind": "port",
-               "sample": f"{a.cpu_iters} bicgstab_CRS_2 iterations (OpenMP row SpMV with per-nonzero atomics, "
-                         f"serial dots) of the same eliminated {n}-DOF system, {t_it:.3f} s/iteration, "
-                         f"extrapolated to the GPU's {st64['iterations']} iterations"}
-        log(f"cpu: {t_it:.3f} s/iteration on {threads} threads")
+        cpu = cpu_leg(a, w, st64["iterations"], crs2_pre, rhs2_pre, x_ref, c3)
+        del crs2_pre, rhs2_pre
+    if c3 is not None:
+        c3.pop("_pre", None)
 
     iters = st64["iterations"]
     line = {
@@ -274,6 +287,10 @@ def main(argv=None):
                    "n_dof": n, "nnz": nnz, "seed": a.seed},
         "iterations": iters, "solve_ms": round(st64["solve_ms"], 2), "true_relres": st64["true_relres"],
         "dof_iterations_per_s": round(n * iters / (st64["solve_ms"] * 1e-3), 1),
+        "reference_criterion": {"stop": "bicgstab_CRS_2: sqrt(r.r / r1.r1) < dble(1.0e-10) or sqrt(r.r) < "
+                                        "dble(1.0e-10) (FEMSolverClass.f90:2275-2289)",
+                                "iterations": st_rc["iterations"], "true_relres": st_rc["true_relres"],
+                                "solve_ms": round(st_rc["solve_ms"], 2)},
         "solve_ms_graph_mode": round(st_graph["solve_ms"], 2), "step_s_graph_mode": round(t_graph, 4),
         "mixed": {"scheme": "FP32 BiCGSTAB on D^-1 A (row Jacobi), FP64 true residual and solution, restart at "
                             "every replacement (inner tolerance 0.05, at most 300 inner iterations)",
@@ -288,11 +305,201 @@ def main(argv=None):
                                       "speedup_vs_fp64": round(t_step / t_mixed_tight, 3)
                                       if st32t["status"] == 0 else None},
                   "fp64_rel_err_vs_reference": rel(x64),
-                  "reference": f"FP64 bicgstab_CRS_2 on the same system to true relres "
-                               f"{st_ref['true_relres']:.2e} ({st_ref['iterations']} iterations)"},
+                  "reference": f"FP64 BiCGSTAB on the PointJacobi-scaled system, {st_ref['iterations']} iterations "
+                               f"to scaled relres {st_ref['true_relres']:.2e}; unscaled residual certified in "
+                               f"cpu_baseline.certified"},
+        "config3": c3,
         "spmv": spmv, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_c3(a, ctx):
+    """Config 3: the 100^3 box with E = 1e7 lognormal(2.0) clipped to a 1e4 contrast.  Plain FP64,
+    FP64 on the PointJacobi-scaled system D^-1 A (the scaling the mixed form uses) swept over
+    scaled stop targets, and the mixed form swept over FP64 relres targets, each to the first
+    target whose error against a tight FP64 reference is <= 1e-8."""
+    from plantfem_amd import workloads as W
+    from plantfem_amd.fem import FEMDomain, FEMSolver
+    w3 = W.config(3, seed=a.seed)
+    log(f"config 3: E contrast {w3.E.max() / w3.E.min():.3g}")
+
+    def build(s=None):
+        if s is None:
+            s = FEMSolver(ctx)
+            s.setDomain(FEMDomain(w3.nodcoord, w3.elemnod, ctx))
+            s.setCRS(DOF=3)
+            s.setStiffness(w3.E, w3.nu)
+        else:
+            s.setCRS(DOF=3)
+            s.setStiffness()
+        s.addTraction(w3.traction_box, w3.pressure)
+        s.fix(1, w3.fixed_dofs, FixValue=0.0)
+        return s
+
+    def timed(s, **kw):
+        ctx.synchronize()
+        t = time.perf_counter()
+        s.solve(download=False, raise_on_breakdown=False, stop_mode=1, **kw)
+        ctx.synchronize()
+        return time.perf_counter() - t, dict(s.last_stats)
+
+    sa = build()                     # unscaled system (plain FP64, mixed, residual checks)
+    crs3_pre, rhs3_pre = (sa.getCRS(), sa.getRHS()) if not a.no_cpu else (None, None)
+    sp = build()                     # PointJacobi-scaled solves (the scaling is in place)
+    sp.itrmax = 100000
+    sp.relative_er = 1e-13
+    t, st = timed(sp, preconditioning="PointJacobi")
+    x_ref = sp.getX()
+    ref = {"iterations": st["iterations"], "scaled_relres": st["true_relres"], "status": st["status"]}
+    log(f"c3 reference: {json.dumps(ref)}")
+
+    def unscaled_relres(x):
+        y, _ = sa.spmv(2, x)
+        b = sa.getRHS()
+        return float(np.linalg.norm(b - y) / np.linalg.norm(b))
+
+    def err(x):
+        return float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
+
+    out = {"workload": "hex box 100^3, E=1e7*lognormal(2.0) clipped to a 1e4 contrast, nu~U[0.25,0.35], clamped "
+                       "bottom, 1e4 Pa top traction", "e_contrast": float(w3.E.max() / w3.E.min()),
+           "error_bar": 1e-8, "reference": ref}
+    # plain FP64 (bicgstab_CRS_2 recurrence, unscaled) to relres 1e-8
+    sa.itrmax = 1
+    sa.relative_er = 1e-8
+    sa.solve(stop_mode=1, download=False, raise_on_breakdown=False)  # elimination + layouts
+    sa.itrmax = 100000
+    tp, st = timed(sa)
+    xp = sa.getX()
+    out["fp64_plain"] = {"relres_target": 1e-8, "s": round(tp, 4), "iterations": st["iterations"],
+                         "status": st["status"], "true_relres": st["true_relres"], "rel_err": err(xp)}
+    log(f"c3 fp64 plain: {json.dumps(out['fp64_plain'])}")
+    # FP64 on D^-1 A: the same scaling as the mixed form, swept
+    legs = []
+    for tgt in (1e-8, 1e-9, 1e-10, 1e-11, 1e-12):
+        build(sp)
+        sp.relative_er = tgt
+        tj, st = timed(sp, preconditioning="PointJacobi")
+        x = sp.getX()
+        leg = {"scaled_relres_target": tgt, "s": round(tj, 4), "iterations": st["iterations"],
+               "status": st["status"], "true_relres": unscaled_relres(x), "rel_err": err(x)}
+        legs.append(leg)
+        log(f"c3 fp64 PointJacobi: {json.dumps(leg)}")
+        if leg["rel_err"] <= 1e-8 and leg["true_relres"] <= 1e-8:
+            break
+    out["fp64_pointjacobi"] = legs
+    # mixed: FP32 inner BiCGSTAB on D^-1 A, FP64 refinement to the relres target
+    mlegs = []
+    for tgt in (1e-8, 1e-9, 1e-10, 1e-11, 1e-12):
+        times, st = [], None
+        for _ in range(2):
+            tm, st = timed(sa, precision=32, target_relres=tgt)
+            times.append(tm)
+        x = sa.getX()
+        leg = {"relres_target": tgt, "s": round(min(times), 4), "status": st["status"],
+               "inner_iterations": st["iterations"], "outer": st["outer_iterations"],
+               "true_relres": st["true_relres"], "rel_err": err(x)}
+        mlegs.append(leg)
+        log(f"c3 mixed: {json.dumps(leg)}")
+        if leg["status"] == 0 and leg["rel_err"] <= 1e-8:
+            break
+    out["mixed"] = mlegs
+    ok_m = [m for m in mlegs if m["status"] == 0 and m["rel_err"] <= 1e-8]
+    ok_j = [j for j in legs if j["status"] == 0 and j["rel_err"] <= 1e-8 and j["true_relres"] <= 1e-8]
+    out["error_bar_met"] = bool(ok_m)
+    if ok_m:
+        m = ok_m[0]
+        out["mixed_at_error_bar"] = {"relres_target": m["relres_target"], "s": m["s"]}
+        if ok_j:
+            out["fp32_speedup_at_equal_scaling"] = round(ok_j[0]["s"] / m["s"], 3)
+        if out["fp64_plain"]["status"] == 0:
+            out["speedup_vs_plain_fp64_at_1e-8"] = round(out["fp64_plain"]["s"] / m["s"], 3)
+    m8 = mlegs[0]
+    if m8["status"] == 0 and legs[0]["status"] == 0:
+        out["fp32_speedup_at_relres_1e-8_equal_scaling"] = round(legs[0]["s"] / m8["s"], 3)
+    sa.close()
+    sp.close()
+    out["_pre"] = (crs3_pre, rhs3_pre, w3, x_ref)
+    return out
+
+
+def cpu_leg(a, w, gpu_iters, crs2_pre, rhs2_pre, x_ref2, c3):
+    """The oracle on the host cores: config 1 end to end, config 2 per phase (graph, assembly on
+    a 100 x 100 x 10 slab scaled by 10, bicgstab_CRS_2 iterations on the same eliminated system),
+    a 1-thread figure; and the residual certification of the FP64 references (oracle elimination
+    and sub_crs_matvec on the GPU-assembled values, which the GPU tests pin bitwise)."""
+    from oracle import oracle as O
+    from plantfem_amd import workloads as W
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        model = "unknown"
+    out = {"unit": "s", "cores": threads, "kind": "port",
+           "host": {"nproc": os.cpu_count(), "cpu_model": model, "threads_used": threads}}
+
+    def certify(crs, rhs, fixed_dofs, x):
+        fixed = np.zeros(rhs.size, np.uint8)
+        fixed[fixed_dofs - 1] = 1
+        v2, r2 = O.apply_dirichlet(crs.row_ptr, crs.col_idx, crs.val, rhs, fixed, np.zeros(rhs.size))
+        return float(np.linalg.norm(r2 - O.spmv(crs.row_ptr, crs.col_idx, v2, x, nthreads=threads))
+                     / np.linalg.norm(r2)), v2, r2
+
+    cert = {}
+    relres2, v2, r2 = certify(crs2_pre, rhs2_pre, w.fixed_dofs, x_ref2)
+    cert["config2_reference_relres"] = relres2
+    log(f"cpu: config 2 reference certified, relres {relres2:.2e}")
+    # config 2 per phase
+    t = time.perf_counter()
+    O.zero_matrix(w.elemnod, w.nn, 3)
+    t_graph = time.perf_counter() - t
+    slab = W.box(a.size, sigma=1.0, seed=a.seed, nu_range=(0.25, 0.35), nz=max(1, a.size // 10))
+    srp, sci = O.zero_matrix(slab.elemnod, slab.nn, 3)
+    t = time.perf_counter()
+    O.assemble(slab.nodcoord, slab.elemnod, slab.E, slab.nu, srp, sci)
+    t_asm = (time.perf_counter() - t) * w.ne / slab.ne
+    del srp, sci
+    O.bicgstab(crs2_pre.row_ptr, crs2_pre.col_idx, v2, r2, itrmax=1, nthreads=threads)  # warm
+    k = a.cpu_iters
+    t = time.perf_counter()
+    O.bicgstab(crs2_pre.row_ptr, crs2_pre.col_idx, v2, r2, itrmax=k, er0=0.0, relative_er=0.0, nthreads=threads)
+    t_it = (time.perf_counter() - t) / k
+    k1 = max(2, k // 10)
+    t = time.perf_counter()
+    O.bicgstab(crs2_pre.row_ptr, crs2_pre.col_idx, v2, r2, itrmax=k1, er0=0.0, relative_er=0.0, nthreads=1)
+    t_it1 = (time.perf_counter() - t) / k1
+    del v2, r2
+    out["value"] = round(t_graph + t_asm + t_it * gpu_iters, 2)
+    out["sample"] = (f"config 2 ({w.ndof} DOF) per phase on {threads} threads: ZeroMatrix graph {t_graph:.2f} s; "
+                     f"stiffness assembly (serial ascending-element scatter) {t_asm:.1f} s, timed on a "
+                     f"{a.size}x{a.size}x{slab.ne // (a.size * a.size)} slab and scaled by the element count; "
+                     f"{k} bicgstab_CRS_2 iterations (OpenMP row SpMV with per-nonzero atomics) "
+                     f"{t_it:.3f} s/iteration, times the GPU's {gpu_iters} iterations")
+    out["phases"] = {"graph_s": round(t_graph, 3), "assembly_s": round(t_asm, 2),
+                     "solve_s_per_iteration": round(t_it, 4), "solve_iterations_timed": k,
+                     "solve_s_extrapolated": round(t_it * gpu_iters, 1)}
+    out["one_thread"] = {"solve_s_per_iteration": round(t_it1, 4), "iterations_timed": k1,
+                         "solve_s_extrapolated": round(t_it1 * gpu_iters, 1)}
+    log(f"cpu: config 2 graph {t_graph:.2f} s, assembly {t_asm:.1f} s, {t_it:.3f} s/iteration "
+        f"({threads} threads), {t_it1:.3f} s/iteration (1 thread)")
+    # config 1 end to end, not extrapolated: graph, assembly, traction, elimination, solve to relres 1e-8
+    w1 = W.config(1, seed=a.seed)
+    t = time.perf_counter()
+    rp, ci, val, rhs, fixed, ub = O.build_system(w1)
+    v1, r1 = O.apply_dirichlet(rp, ci, val, rhs, fixed, ub)
+    x1, st1, it1, *_ = O.bicgstab(rp, ci, v1, r1, er0=1e-8 * np.linalg.norm(r1), relative_er=0.0, nthreads=threads)
+    t_c1 = time.perf_counter() - t
+    out["config1_whole_path"] = {"s": round(t_c1, 3), "iterations": it1, "status": st1, "threads": threads,
+                                 "stop": "||b - A x|| < 1e-8 ||b|| (recurrence residual)"}
+    log(f"cpu: config 1 whole path {t_c1:.2f} s ({it1} iterations)")
+    if c3 is not None and "_pre" in c3:
+        crs3, rhs3, w3, x3 = c3["_pre"]
+        cert["config3_reference_relres"], _, _ = certify(crs3, rhs3, w3.fixed_dofs, x3)
+        log(f"cpu: config 3 reference certified, relres {cert['config3_reference_relres']:.2e}")
+    out["certified"] = {**cert, "how": "oracle Dirichlet elimination and sub_crs_matvec on the GPU-assembled "
+                                       "CRS (bitwise equal to the oracle's assembly), ||b - A x_ref|| / ||b||"}
+    return out
 
 
 def bench_batch(a):

@@ -17,6 +17,7 @@
  */
 #ifndef PFGPU_H
 #define PFGPU_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -60,7 +61,11 @@ typedef struct {
   int32_t check_every;   /* iterations enqueued between host polls of the stop flag (default 32) */
   int32_t use_graph;     /* capture the polled chunk in a hipGraph (default 1) */
   int32_t debug;         /* print sqrt(r.r) per poll */
-  int32_t stop_mode;     /* 0: the reference's tests (default); 1: sqrt(r.r) < relative_er * ||b|| */
+  int32_t stop_mode;     /* 0: bicgstab_CRS_2's tests (default, FEMSolverClass.f90:2275-2289);
+                            1: sqrt(r.r) < relative_er * ||b||;
+                            2: bicgstab_CRS's tests (LinearSolverClass.f90:1493-1569) with er = er0:
+                               return before the loop when r0.r0 < er (:1527), stop when
+                               r.r / r0.r0 < er, no square root (:1561); FP64, no ILU */
   int32_t kernel_timing; /* 1: bracket every kernel of the iteration loop with hipEvents (no hipGraph)
                             and report the average device time of each in pf_stats */
   int32_t jacobi;        /* precision 32, scaling of the FP32 inner system (D = diag(A)):
@@ -167,6 +172,24 @@ int pf_dev_spmv_crs(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_
                     const double *val, const double *x, double *y);
 int pf_dev_spmv_crs_f32(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                         const float *val, const float *x, float *y);
+
+/* crs_spmv_real64 / crs_spmv_real32 (SparseClass-c.c:38-66, bound in SparseClass.f90:24-50):
+ * new_vector += A old_vector on host arrays (the product is ADDED, as the reference's C
+ * kernels do; matmul_c_CRS zeroes new_vector first, SparseClass.f90:727-742).  1-based
+ * row_ptr (n+1) and col_idx; col_size = size(col_idx) >= row_ptr(n+1)-1.  FP32 values and
+ * vectors accumulate in FP32 like crs_spmv_real32. */
+int pf_crs_spmv_real64(pf_ctx *ctx, const double *val, const int64_t *row_ptr, const int32_t *col_idx,
+                       const double *old_vector, double *new_vector, int64_t n, int64_t col_size);
+int pf_crs_spmv_real32(pf_ctx *ctx, const float *val, const int64_t *row_ptr, const int32_t *col_idx,
+                       const float *old_vector, float *new_vector, int64_t n, int64_t col_size);
+/* The same under the reference's own C names and signatures, so a program linked against this
+ * library in place of SparseClass-c.o runs matmul_c_CRS / matmul_c_real32_CRS on the GPU
+ * (device 0, a context created on first use).  These return nothing, like the reference's:
+ * an invalid CRS or a HIP error prints a message and aborts the process. */
+void crs_spmv_real64(double val[], long row_ptr[], int col_idx[], double old_vector[], double new_vector[],
+                     size_t n, size_t col_size);
+void crs_spmv_real32(float val[], long row_ptr[], int col_idx[], float old_vector[], float new_vector[],
+                     size_t n, size_t col_size);
 
 /* ---- FEMSolver_ state on the device (FEMSolverClass.f90:13-164) ------------------ */
 int pf_system_create(pf_ctx *ctx, pf_system **out);

@@ -34,6 +34,10 @@ _SIG = {
     "orc_spmv": (None, [I64, P64, P32, PD, PD, PD, I32]),
     "orc_bicgstab_crs2": (I32, [I64, P64, P32, PD, PD, PD, I32, D, D, I32, P32, PD, PD, PD]),
     "orc_solve": (I32, [I64, P64, P32, PD, PD, PU8, PD, PD, I32, D, D, I32, P32, PD, PD]),
+    "orc_bicgstab_crs": (I32, [I64, P64, P32, PD, PD, PD, I32, D, I32, P32, PD, PD, PD]),
+    "orc_crs_spmv_real64": (None, [PD, P64, P32, PD, PD, I64]),
+    "orc_crs_spmv_real32": (None, [C.POINTER(C.c_float), P64, P32, C.POINTER(C.c_float), C.POINTER(C.c_float), I64]),
+    "orc_assemble_stiffness_rows": (I32, [I64, I64, PD, P32, PD, PD, I32, PU8, P64, P32, PD]),
     "orc_jacobi_scale": (None, [I64, P64, P32, PD, PD]),
     "orc_hex8_mass_matrix": (None, [PD, D, PD]),
     "orc_mass_matrix_crs": (I32, [I64, I64, PD, P32, PD, P64, P32, PD]),
@@ -223,6 +227,50 @@ def bicgstab(row_ptr, col_idx, val, b, x0=None, itrmax=100000, er0=None, relativ
                                   C.byref(rr1), _p(h, C.c_double))
     out = (x, st, it.value, rr.value, rr1.value)
     return out + (h[:it.value],) if hist else out
+
+
+def bicgstab_crs(row_ptr, col_idx, val, b, er, x0=None, itrmax=100000, nthreads=0, hist=False):
+    """bicgstab_CRS (LinearSolver_ form); returns (x, status, iters, rr, init_rr[, hist])."""
+    n = b.size
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = C.c_int(0)
+    rr = C.c_double(0)
+    rr0 = C.c_double(0)
+    h = np.zeros(max(itrmax, 1)) if hist else None
+    st = load().orc_bicgstab_crs(n, _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+                                 _p(_c(val, np.float64), C.c_double), _p(_c(b, np.float64), C.c_double),
+                                 _p(x, C.c_double), itrmax, float(er), nthreads, C.byref(it), C.byref(rr),
+                                 C.byref(rr0), _p(h, C.c_double))
+    out = (x, st, it.value, rr.value, rr0.value)
+    return out + (h[:it.value],) if hist else out
+
+
+def crs_spmv(row_ptr, col_idx, val, old_vector, new_vector):
+    """crs_spmv_real64 / crs_spmv_real32 (by the dtype of val): new_vector + A old_vector."""
+    f32 = np.asarray(val).dtype == np.float32
+    dt = np.float32 if f32 else np.float64
+    ct = C.c_float if f32 else C.c_double
+    y = np.array(new_vector, dtype=dt)
+    fn = load().orc_crs_spmv_real32 if f32 else load().orc_crs_spmv_real64
+    fn(_p(_c(val, dt), ct), _p(_c(row_ptr, np.int64), C.c_int64), _p(_c(col_idx, np.int32), C.c_int32),
+       _p(_c(old_vector, dt), ct), _p(y, ct), y.size)
+    return y
+
+
+def assemble_rows(nodcoord, elemnod, E, nu, row_ptr, col_idx, nodes, jmode=0):
+    """Stiffness CRS values of the DOF rows of the given 0-based nodes (others zero)."""
+    nn = nodcoord.shape[0]
+    mark = np.zeros(nn, np.uint8)
+    mark[np.asarray(nodes)] = 1
+    val = np.empty(col_idx.size)
+    rc = load().orc_assemble_stiffness_rows(nn, elemnod.shape[0], _p(_fl(nodcoord, np.float64), C.c_double),
+                                            _p(_fl(elemnod, np.int32), C.c_int32),
+                                            _p(_c(E, np.float64), C.c_double), _p(_c(nu, np.float64), C.c_double),
+                                            jmode, _p(mark, C.c_uint8), _p(_c(row_ptr, np.int64), C.c_int64),
+                                            _p(_c(col_idx, np.int32), C.c_int32), _p(val, C.c_double))
+    if rc:
+        raise ValueError("assembly: missing CRS slot")
+    return val
 
 
 def solve(row_ptr, col_idx, val, rhs, fixed, ubar, x0=None, itrmax=100000, er0=None, relative_er=None, nthreads=0):

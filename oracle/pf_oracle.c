@@ -671,6 +671,111 @@ int orc_bicgstab_crs2(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
     return status;
 }
 
+/* A17: bicgstab_CRS (LinearSolverClass.f90:1493-1569), statement by statement */
+int orc_bicgstab_crs(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                     const double *b, double *x, int itrmax, double er, int nthreads, int *iters,
+                     double *rr_out, double *init_rr_out, double *hist)
+{
+    const double er0 = er;                                    /* :1504 */
+    double *r = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1) * 6);
+    double *r0 = r + n, *p = r0 + n, *y = p + n, *e = y + n, *v = e + n;
+    double c1, c2, c3, alp, bet, ev, vv, rr = 0.0, init_rr;
+    int status = ORC_NOT_CONVERGED, itr = 0;
+    orc_spmv(n, row_ptr, col_idx, val, x, y, nthreads);       /* ax = crs_matvec(x) (:1516-1517) */
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - y[i];       /* :1518 */
+    c1 = dotp(n, r, r);                                       /* :1522 */
+    init_rr = c1;                                             /* :1524 */
+    if (c1 < er0) {                                           /* :1527: return before the loop */
+        status = ORC_OK;
+        goto done;
+    }
+    for (int64_t i = 0; i < n; ++i) { p[i] = r[i]; r0[i] = r[i]; }
+    for (itr = 1; itr <= itrmax; ++itr) {
+        c1 = dotp(n, r0, r);
+        orc_spmv(n, row_ptr, col_idx, val, p, y, nthreads);
+        c2 = dotp(n, r0, y);
+        alp = c1 / c2;
+        for (int64_t i = 0; i < n; ++i) e[i] = r[i] - alp * y[i];
+        orc_spmv(n, row_ptr, col_idx, val, e, v, nthreads);
+        ev = dotp(n, e, v);
+        vv = dotp(n, v, v);
+        if (vv == 0.0) { status = ORC_BREAKDOWN_VV; break; }  /* :1549 */
+        c3 = ev / vv;
+        for (int64_t i = 0; i < n; ++i) x[i] = x[i] + alp * p[i] + c3 * e[i];
+        for (int64_t i = 0; i < n; ++i) r[i] = e[i] - c3 * v[i];
+        rr = dotp(n, r, r);
+        if (hist) hist[itr - 1] = sqrt(rr);
+        if (rr / init_rr < er0) { status = ORC_OK; break; }   /* :1561, no square root */
+        c1 = dotp(n, r0, r);
+        bet = c1 / (c2 * c3);
+        if ((c2 * c3) == 0.0) { status = ORC_BREAKDOWN_C2C3; break; }  /* :1564 */
+        for (int64_t i = 0; i < n; ++i) p[i] = r[i] + bet * (p[i] - c3 * y[i]);
+    }
+    if (itr > itrmax) itr = itrmax;
+done:
+    if (iters) *iters = itr;
+    if (rr_out) *rr_out = rr;
+    if (init_rr_out) *init_rr_out = init_rr;
+    free(r);
+    return status;
+}
+
+/* A19: crs_spmv_real64 / crs_spmv_real32 (SparseClass-c.c:38-66) without the OpenMP split
+ * (each row is summed by one thread there too, so the order within a row is the same) */
+void orc_crs_spmv_real64(const double *val, const int64_t *row_ptr, const int32_t *col_idx,
+                         const double *old_vector, double *new_vector, int64_t n)
+{
+    for (int64_t row = 0; row < n; ++row)
+        for (int64_t col = row_ptr[row]; col < row_ptr[row + 1]; ++col)
+            new_vector[row] = new_vector[row] + val[col - 1] * old_vector[col_idx[col - 1] - 1];
+}
+
+void orc_crs_spmv_real32(const float *val, const int64_t *row_ptr, const int32_t *col_idx,
+                         const float *old_vector, float *new_vector, int64_t n)
+{
+    for (int64_t row = 0; row < n; ++row)
+        for (int64_t col = row_ptr[row]; col < row_ptr[row + 1]; ++col)
+            new_vector[row] = new_vector[row] + val[col - 1] * old_vector[col_idx[col - 1] - 1];
+}
+
+/* orc_assemble_stiffness (A9) on the rows of the marked nodes only: the element loop, the
+ * Ke evaluation and the row searches are the full assembly's, so every entry of a marked row
+ * receives the same additions in the same order */
+int orc_assemble_stiffness_rows(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                                const double *E, const double *nu, int jmode, const uint8_t *row_node,
+                                const int64_t *row_ptr, const int32_t *col_idx, double *val)
+{
+    memset(val, 0, sizeof(double) * (size_t)(row_ptr[3 * nn] - 1));
+    double ke[576];
+    for (int64_t e = 0; e < ne; ++e) {
+        int touches = 0;
+        for (int a = 0; a < 8; ++a) touches |= row_node[elemnod[(int64_t)a * ne + e] - 1];
+        if (!touches) continue;
+        double X[24];
+        for (int a = 0; a < 8; ++a)
+            for (int d = 0; d < 3; ++d)
+                X[a * 3 + d] = nodcoord[(int64_t)d * nn + elemnod[(int64_t)a * ne + e] - 1];
+        orc_hex8_stiffness(X, E[e], nu[e], jmode, ke);
+        for (int i = 0; i < 8; ++i) {
+            const int64_t node = elemnod[(int64_t)i * ne + e] - 1;
+            if (!row_node[node]) continue;
+            for (int j = 0; j < 3; ++j) {
+                const int64_t row = 3 * node + j;
+                for (int ii = 0; ii < 8; ++ii)
+                    for (int jj = 0; jj < 3; ++jj) {
+                        const int64_t col = 3 * (int64_t)(elemnod[(int64_t)ii * ne + e] - 1) + jj + 1;
+                        int64_t k = row_ptr[row] - 1, kend = row_ptr[row + 1] - 1, id = -1;
+                        for (; k < kend; ++k)
+                            if (col_idx[k] == col) { id = k; break; }
+                        if (id < 0) return ORC_INVALID;
+                        val[id] = val[id] + ke[(int64_t)(3 * ii + jj) * 24 + (3 * i + j)];
+                    }
+            }
+        }
+    }
+    return ORC_OK;
+}
+
 /* A14 whole: elimination, BiCGSTAB from x0 (x on entry), then x_fixed = ubar (:1814-1823) */
 int orc_solve(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, double *rhs,
               const uint8_t *fixed, const double *ubar, double *x, int itrmax, double er0,

@@ -106,6 +106,29 @@ int orc_bicgstab_crs2(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                       const double *b, double *x, int itrmax, double er0, double relative_er,
                       int nthreads, int *iters, double *rr_out, double *rr1_out, double *hist);
 
+/* A17: bicgstab_CRS (LinearSolverClass.f90:1493-1569), the LinearSolver_ form: the same
+ * recurrence as bicgstab_CRS_2, but er0 = er, init_rr = r.r of the initial residual, a return
+ * before the loop when init_rr < er (:1527) and the exit rr/init_rr < er with no square root
+ * (:1561).  *iters = 0 for the early return.  hist as above. */
+int orc_bicgstab_crs(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                     const double *b, double *x, int itrmax, double er, int nthreads, int *iters,
+                     double *rr_out, double *init_rr_out, double *hist);
+
+/* A19: crs_spmv_real64 / crs_spmv_real32 (SparseClass-c.c:38-66): new_vector(row) =
+ * new_vector(row) + val*old_vector(col), entries of each row in stored order, serial. */
+void orc_crs_spmv_real64(const double *val, const int64_t *row_ptr, const int32_t *col_idx,
+                         const double *old_vector, double *new_vector, int64_t n);
+void orc_crs_spmv_real32(const float *val, const int64_t *row_ptr, const int32_t *col_idx,
+                         const float *old_vector, float *new_vector, int64_t n);
+
+/* orc_assemble_stiffness restricted to the DOF rows of the nodes with row_node[node] != 0:
+ * the same ascending element loop and row searches, so each computed entry equals the full
+ * assembly's bitwise; val (nnz) is zeroed here and entries of other rows stay zero.  For
+ * spot checks of large meshes (only elements touching a marked node are evaluated). */
+int orc_assemble_stiffness_rows(int64_t nn, int64_t ne, const double *nodcoord, const int32_t *elemnod,
+                                const double *E, const double *nu, int jmode, const uint8_t *row_node,
+                                const int64_t *row_ptr, const int32_t *col_idx, double *val);
+
 /* Whole solveFEMSolver (FEMSolverClass.f90:1697-1825) on a copy: elimination, BiCGSTAB, x_fixed = ubar. */
 int orc_solve(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, double *rhs,
               const uint8_t *fixed, const double *ubar, double *x, int itrmax, double er0,

@@ -128,6 +128,10 @@ SIGNATURES = {
     "pf_bicgstab_crs": (C.c_int, [P, I64, PI64, PI32, PF64, PF64, PF64, C.POINTER(SolveOpts), C.POINTER(Stats)]),
     "pf_dev_spmv_crs": (C.c_int, [P, I64, P, P, P, P, P]),
     "pf_dev_spmv_crs_f32": (C.c_int, [P, I64, P, P, P, P, P]),
+    "pf_crs_spmv_real64": (C.c_int, [P, PF64, PI64, PI32, PF64, PF64, I64, I64]),
+    "pf_crs_spmv_real32": (C.c_int, [P, PF32, PI64, PI32, PF32, PF32, I64, I64]),
+    "crs_spmv_real64": (None, [PF64, PI64, PI32, PF64, PF64, C.c_size_t, C.c_size_t]),
+    "crs_spmv_real32": (None, [PF32, PI64, PI32, PF32, PF32, C.c_size_t, C.c_size_t]),
     "pf_system_create": (C.c_int, [P, C.POINTER(P)]),
     "pf_system_destroy": (C.c_int, [P]),
     "pf_system_set_mesh": (C.c_int, [P, I64, I64, I32, PF64, PI32]),

@@ -68,7 +68,7 @@ struct BArgs {
   double *zhi;
   const double *scale;   // FP32 system row scale
   const double *cscale;  // FP32 system column scale (the unknown is x = C z)
-  int32_t restart;       // replacements restart the FP32 recurrence (r0 = p = r)
+  int64_t restart;       // replacements restart the FP32 recurrence (r0 = p = r)
   pf_batch_mesh_stats *stat;
   int64_t *t_raw;  // per mesh: start, end wall-clock ticks
 };
@@ -452,7 +452,15 @@ struct BatchWork {
   int scaled = 0;  // the FP32 copies hold 0: A, 1: S A S (S = diag(A)^-1/2), 2: D^-1 A
   uint64_t mat_version = ~0ull;
   hipGraphExec_t gexec = nullptr;
-  uint64_t gkey[12] = {};
+  // what the captured launch bakes in: the kernel argument block (no padding: every member
+  // after the four int32 is 8 bytes wide), the launch shape and the grid
+  struct GraphKey {
+    BArgs a;
+    pf_batch_tune t;
+    uint32_t grid;
+    int32_t mixed;
+  } gkey;
+  bool gkey_valid = false;
   pf_batch_tune tuned[2] = {};  // [0] FP64, [1] mixed
   bool have_tuned[2] = {false, false};
   ~BatchWork() {
@@ -491,7 +499,7 @@ int pf_batch_configure(pf_ctx *ctx, BatchWork **pb, int64_t n, const int64_t *ro
   b->node_off.assign(node_off, node_off + nmesh + 1);
   b->have_s64 = b->have_s32 = b->have_v32 = b->have_scale = false;
   b->mat_version = ~0ull;
-  std::memset(b->gkey, 0, sizeof(b->gkey));
+  b->gkey_valid = false;
   // slices of 64 nodes that never cross a mesh
   std::vector<int64_t> sn, se;
   b->hmesh.resize(nmesh);
@@ -755,42 +763,31 @@ int run_batch(pf_ctx *ctx, BatchWork *b, const BArgs &a, const pf_batch_tune &t,
               float *ms) {
   const unsigned grid = batch_grid(ctx, b, t, mixed);
   const Launcher ln{ctx, &a, grid};
-  uint64_t key[12] = {(uint64_t)t.layout, (uint64_t)t.sub, (uint64_t)t.block, (uint64_t)grid, (uint64_t)mixed,
-                      (uint64_t)a.itrmax, (uint64_t)(uintptr_t)a.b, (uint64_t)(uintptr_t)a.x,
-                      (uint64_t)(uintptr_t)a.A.val, (uint64_t)(uintptr_t)a.A.sval,
-                      (uint64_t)(uintptr_t)a.A32.sval ^ (uint64_t)(uintptr_t)a.A32.val,
-                      (uint64_t)a.stop_mode * 1000003ull ^ (uint64_t)a.inner_max};
-  // scalars baked into the kernel argument block: tolerance bits
-  uint64_t tb[4];
-  std::memcpy(&tb[0], &a.er0, 8);
-  std::memcpy(&tb[1], &a.rel, 8);
-  std::memcpy(&tb[2], &a.target, 8);
-  std::memcpy(&tb[3], &a.delta, 8);
-  key[11] ^= (tb[0] * 3) ^ (tb[1] * 5) ^ (tb[2] * 7) ^ (tb[3] * 11);
+  BatchWork::GraphKey key;
+  std::memset(&key, 0, sizeof(key));
+  std::memcpy(&key.a, &a, sizeof(BArgs));
+  key.t = t;
+  key.grid = grid;
+  key.mixed = mixed ? 1 : 0;
   PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   bool launched = false;
-  if (use_graph) {
-    if (!b->gexec || std::memcmp(b->gkey, key, sizeof(key)) != 0) {
+  if (use_graph && pf_graphs_allowed()) {
+    if (!b->gexec || !b->gkey_valid || std::memcmp(&b->gkey, &key, sizeof(key)) != 0) {
       if (b->gexec) {
         PF_HIP(hipGraphExecDestroy(b->gexec));
         b->gexec = nullptr;
       }
-      hipGraph_t graph;
-      PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      k_queue_reset<<<1, 64, 0, ctx->stream>>>(b->queue.p);
-      const int rc = mixed ? dispatch<32>(t, ln) : dispatch<64>(t, ln);
-      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-      if (rc != PF_OK) return rc;
-      if (ce != hipSuccess) {
-        pf_set_error("hipStreamEndCapture: %s", hipGetErrorString(ce));
-        return PF_ERR_HIP;
-      }
-      hipError_t ie = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ie != hipSuccess) {
-        b->gexec = nullptr;
-      } else {
-        std::memcpy(b->gkey, key, sizeof(key));
+      b->gkey_valid = false;
+      PF_CHECK(pf_capture_graph(
+          ctx,
+          [&] {
+            k_queue_reset<<<1, 64, 0, ctx->stream>>>(b->queue.p);
+            return mixed ? dispatch<32>(t, ln) : dispatch<64>(t, ln);
+          },
+          &b->gexec));
+      if (b->gexec) {
+        std::memcpy(&b->gkey, &key, sizeof(key));
+        b->gkey_valid = true;
       }
     }
     if (b->gexec) {
@@ -823,10 +820,14 @@ int pf_batch_solve_device(pf_ctx *ctx, BatchWork *b, const int64_t *row_ptr, con
     pf_set_error("batch: precision must be 64 or 32");
     return PF_INVALID;
   }
+  if (o.stop_mode != 0 && o.stop_mode != 1) {
+    pf_set_error("batch: stop_mode must be 0 (bicgstab_CRS_2) or 1 (|r| < relative_er |b|)");
+    return PF_INVALID;
+  }
   if (mat_version != b->mat_version) {
     b->have_s64 = b->have_s32 = b->have_v32 = b->have_scale = false;
     b->mat_version = mat_version;
-    std::memset(b->gkey, 0, sizeof(b->gkey));
+    b->gkey_valid = false;
   }
   pf_batch_tune t;
   if (tune) {

@@ -37,10 +37,13 @@
 namespace {
 
 constexpr int BT = 256;  // threads per block of every fused kernel (4 wavefronts)
+constexpr int KEY_WORDS = 40;      // words of the key of a captured iteration chunk
+constexpr int ILU_KEY_WORDS = 32;  // the same for the ILU chunk
 
 struct Scal {
   double c1, c2, alpha, omega, beta;
   double rr, rr1, bnorm2, er0, rel;
+  double rr0;          // r.r of the initial residual (stop rule of bicgstab_CRS)
   double rnorm2;       // refinement: |b - A x|^2
   double rr_best[2];   // stagnation watch: smallest r.r so far (slot by parity)
   double maxr[2];      // reliable updates: largest |r| since the last replacement (slot by parity)
@@ -175,6 +178,7 @@ __global__ void __launch_bounds__(BT) k_init_finish(const double *__restrict__ p
   if (threadIdx.x == 0) {
     sc->c1 = t[0];
     sc->rr = t[0];
+    sc->rr0 = t[0];
     sc->bnorm2 = t[1];
     sc->itr[0] = sc->itr[1] = 0;
     sc->itr_best[0] = sc->itr_best[1] = 0;
@@ -182,6 +186,11 @@ __global__ void __launch_bounds__(BT) k_init_finish(const double *__restrict__ p
     sc->itr_last = 0;
     sc->done = sc->itrmax <= 0 ? 1 : 0;
     sc->status = PF_NOT_CONVERGED;
+    // bicgstab_CRS returns before its loop when r.r < er (LinearSolverClass.f90:1527)
+    if (sc->stop_mode == 3 && t[0] < sc->er0) {
+      sc->done = 1;
+      sc->status = PF_OK;
+    }
   }
 }
 
@@ -354,6 +363,8 @@ __global__ void __launch_bounds__(BT) k_pupdate(int64_t n, V *__restrict__ p, co
     stop = (sqrt(rr / rr1) < sc->rel) || (sqrt(rr) < sc->er0);
   } else if (sc->stop_mode == 1) {
     stop = sqrt(rr) < sc->rel * sqrt(sc->bnorm2);
+  } else if (sc->stop_mode == 3) {  // bicgstab_CRS: rr / init_rr < er, no square root (:1561)
+    stop = rr / sc->rr0 < sc->er0;
   } else {
     // reliable updates: convergence is judged on the FP64 true residual after a
     // replacement; here only decide whether one is due (|r| fell by delta since its max)
@@ -673,14 +684,14 @@ struct SolverWork {
   IluFactor ilu;
   DevBuf<double> pd, ps, e;
   hipGraphExec_t gexec_ilu = nullptr;
-  uint64_t gkey_ilu[12] = {};
+  uint64_t gkey_ilu[ILU_KEY_WORDS] = {};
   bool have_s64 = false, have_s32 = false, have_v32 = false, have_scale = false;
   int scale_mode = 0;            // what jscale holds: 1 symmetric, 2 row
   int s32_mode = 0, v32_mode = 0;  // scaling of the FP32 copies: 0 none, 1 symmetric, 2 row
   int block_ok = -1;  // -1 unknown
   const void *key_val = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  uint64_t gkey[2][21] = {};  // everything a captured chunk bakes in
+  uint64_t gkey[2][KEY_WORDS] = {};  // everything a captured chunk bakes in
   std::vector<hipEvent_t> evs;  // kernel_timing: 5 per iteration of a chunk
   double t_sum[4] = {0, 0, 0, 0};
   int64_t t_count[4] = {0, 0, 0, 0};
@@ -852,7 +863,7 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V>
   int chunk = o.check_every > 0 ? o.check_every : 32;
   chunk += chunk & 1;  // even: the parity of the iteration counter slots carries across chunks
   const bool timing = o.kernel_timing != 0;
-  bool use_graph = o.use_graph != 0 && !timing;
+  bool use_graph = o.use_graph != 0 && !timing && pf_graphs_allowed();
   if (timing) {
     while ((int)w->evs.size() < 5 * chunk) {
       hipEvent_t e;
@@ -860,36 +871,35 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V>
       w->evs.push_back(e);
     }
   }
-  uint64_t key[21] = {(uint64_t)(uintptr_t)ebuf, (uint64_t)L.grid, (uint64_t)L.grid_e * 131 + (uint64_t)L.sub,
-                      (uint64_t)L.layout,
-                      (uint64_t)chunk, (uint64_t)m.n, (uint64_t)(uintptr_t)m.val, (uint64_t)(uintptr_t)m.sval,
-                      (uint64_t)(uintptr_t)m.col, (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)r,
-                      (uint64_t)(uintptr_t)r0, (uint64_t)(uintptr_t)p, (uint64_t)(uintptr_t)y,
-                      (uint64_t)(uintptr_t)v, (uint64_t)(uintptr_t)w->part.p, (uint64_t)(uintptr_t)hist,
-                      rel ? (uint64_t)(uintptr_t)rel->m.sval : 0, rel ? (uint64_t)(uintptr_t)rel->m.val : 0,
-                      rel ? (uint64_t)(uintptr_t)rel->zhi : 0, rel ? (uint64_t)(uintptr_t)rel->b : 0};
+  // everything the captured chunk bakes in: a replay is valid only while all of it is
+  const uint64_t key[KEY_WORDS] = {
+      (uint64_t)(uintptr_t)ebuf, (uint64_t)L.grid, (uint64_t)L.grid_e * 131 + (uint64_t)L.sub, (uint64_t)L.layout,
+      (uint64_t)chunk, (uint64_t)m.n, (uint64_t)(uintptr_t)m.row_ptr, (uint64_t)(uintptr_t)m.col_idx,
+      (uint64_t)(uintptr_t)m.val, (uint64_t)m.nn, (uint64_t)m.nslices, (uint64_t)(uintptr_t)m.soff,
+      (uint64_t)(uintptr_t)m.width, (uint64_t)(uintptr_t)m.col, (uint64_t)(uintptr_t)m.sval,
+      (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)r, (uint64_t)(uintptr_t)r0, (uint64_t)(uintptr_t)p,
+      (uint64_t)(uintptr_t)y, (uint64_t)(uintptr_t)v, (uint64_t)(uintptr_t)sc, (uint64_t)(uintptr_t)w->part.p,
+      (uint64_t)(uintptr_t)hist,
+      rel ? (uint64_t)(uintptr_t)rel->m.row_ptr : 0, rel ? (uint64_t)(uintptr_t)rel->m.col_idx : 0,
+      rel ? (uint64_t)(uintptr_t)rel->m.val : 0, rel ? (uint64_t)(uintptr_t)rel->m.soff : 0,
+      rel ? (uint64_t)(uintptr_t)rel->m.width : 0, rel ? (uint64_t)(uintptr_t)rel->m.col : 0,
+      rel ? (uint64_t)(uintptr_t)rel->m.sval : 0, rel ? (uint64_t)rel->L.grid * 131 + rel->L.sub : 0,
+      rel ? (uint64_t)(uintptr_t)rel->b : 0, rel ? (uint64_t)(uintptr_t)rel->scale : 0,
+      rel ? (uint64_t)(uintptr_t)rel->cscale : 0, rel ? (uint64_t)(uintptr_t)rel->zhi : 0,
+      rel ? (uint64_t)(uintptr_t)rel->r : 0, rel ? (uint64_t)(uintptr_t)rel->r0 : 0,
+      rel ? (uint64_t)(uintptr_t)rel->p : 0, rel ? (uint64_t)(uintptr_t)rel->zlo : 0};
   if (use_graph && (w->gexec[slot] == nullptr || std::memcmp(w->gkey[slot], key, sizeof(key)) != 0)) {
     if (w->gexec[slot]) {
       PF_HIP(hipGraphExecDestroy(w->gexec[slot]));
       w->gexec[slot] = nullptr;
     }
-    hipGraph_t graph;
-    PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel, ebuf);
-    hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-    if (rc != PF_OK) return rc;
-    if (ce != hipSuccess) {
-      pf_set_error("hipStreamEndCapture: %s", hipGetErrorString(ce));
-      return PF_ERR_HIP;
-    }
-    hipError_t ie = hipGraphInstantiate(&w->gexec[slot], graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ie != hipSuccess) {
-      w->gexec[slot] = nullptr;
-      use_graph = false;  // direct launches of the same kernels
-    } else {
+    PF_CHECK(pf_capture_graph(
+        ctx, [&] { return enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel, ebuf); },
+        &w->gexec[slot]));
+    if (w->gexec[slot])
       std::memcpy(w->gkey[slot], key, sizeof(key));
-    }
+    else
+      use_graph = false;  // direct launches of the same kernels
   }
   for (;;) {
     const int itr_before = hs->itr_last;
@@ -1025,34 +1035,32 @@ int run_loop_ilu(pf_ctx *ctx, SolverWork *w, const Launch &L, const Mat<double> 
   Scal *sc = w->sc.p;
   Scal *hs = w->h_sc;
   const int chunk = 2;
-  bool use_graph = o.use_graph != 0;
-  const uint64_t key[12] = {(uint64_t)L.grid, (uint64_t)L.grid_e * 131 + (uint64_t)L.sub, (uint64_t)L.layout,
-                            (uint64_t)m.n, (uint64_t)(uintptr_t)m.val, (uint64_t)(uintptr_t)m.sval,
-                            (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)w->pd.p,
-                            (uint64_t)(uintptr_t)w->ilu.lval.p, (uint64_t)(uintptr_t)w->ilu.grp.p,
-                            (uint64_t)w->ilu.lev.size(), (uint64_t)w->ilu.lnnz};
+  bool use_graph = o.use_graph != 0 && pf_graphs_allowed();
+  const IluFactor &F = w->ilu;
+  uint64_t lev_hash = 1469598103934665603ull ^ (uint64_t)F.G;  // level bounds are launch arguments
+  for (int64_t v : F.lev) lev_hash = (lev_hash ^ (uint64_t)v) * 1099511628211ull;
+  const uint64_t key[ILU_KEY_WORDS] = {
+      (uint64_t)L.grid, (uint64_t)L.grid_e * 131 + (uint64_t)L.sub, (uint64_t)L.layout, (uint64_t)m.n,
+      (uint64_t)(uintptr_t)m.row_ptr, (uint64_t)(uintptr_t)m.col_idx, (uint64_t)(uintptr_t)m.val,
+      (uint64_t)(uintptr_t)m.soff, (uint64_t)(uintptr_t)m.width, (uint64_t)(uintptr_t)m.col,
+      (uint64_t)(uintptr_t)m.sval, (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)w->sc.p,
+      (uint64_t)(uintptr_t)w->part.p, (uint64_t)(uintptr_t)w->r.p, (uint64_t)(uintptr_t)w->r0.p,
+      (uint64_t)(uintptr_t)w->p.p, (uint64_t)(uintptr_t)w->y.p, (uint64_t)(uintptr_t)w->v.p,
+      (uint64_t)(uintptr_t)w->e.p, (uint64_t)(uintptr_t)w->pd.p, (uint64_t)(uintptr_t)w->ps.p,
+      (uint64_t)(uintptr_t)F.grp.p, (uint64_t)(uintptr_t)F.lptr.p, (uint64_t)(uintptr_t)F.lcol.p,
+      (uint64_t)(uintptr_t)F.lval.p, (uint64_t)(uintptr_t)F.wv.p, (uint64_t)(uintptr_t)F.dd.p,
+      (uint64_t)(uintptr_t)F.t.p, (uint64_t)F.lev.size(), (uint64_t)F.lnnz, lev_hash};
   if (use_graph && (w->gexec_ilu == nullptr || std::memcmp(w->gkey_ilu, key, sizeof(key)) != 0)) {
     if (w->gexec_ilu) {
       PF_HIP(hipGraphExecDestroy(w->gexec_ilu));
       w->gexec_ilu = nullptr;
     }
-    hipGraph_t graph;
-    PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_iters_ilu(ctx, w, L, m, x, nullptr, chunk);
-    hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-    if (rc != PF_OK) return rc;
-    if (ce != hipSuccess) {
-      pf_set_error("hipStreamEndCapture: %s", hipGetErrorString(ce));
-      return PF_ERR_HIP;
-    }
-    hipError_t ie = hipGraphInstantiate(&w->gexec_ilu, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ie != hipSuccess) {
-      w->gexec_ilu = nullptr;
-      use_graph = false;
-    } else {
+    PF_CHECK(pf_capture_graph(ctx, [&] { return enqueue_iters_ilu(ctx, w, L, m, x, nullptr, chunk); },
+                              &w->gexec_ilu));
+    if (w->gexec_ilu)
       std::memcpy(w->gkey_ilu, key, sizeof(key));
-    }
+    else
+      use_graph = false;
   }
   for (;;) {
     if (use_graph) {
@@ -1077,6 +1085,14 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   for (int k = 0; k < 4; ++k) {
     w->t_sum[k] = 0.0;
     w->t_count[k] = 0;
+  }
+  if (o.stop_mode < 0 || o.stop_mode > 2) {
+    pf_set_error("stop_mode must be 0 (bicgstab_CRS_2), 1 (|r| < relative_er |b|) or 2 (bicgstab_CRS)");
+    return PF_INVALID;
+  }
+  if (o.stop_mode == 2 && (o.precision == 32 || o.preconditioning == 2)) {
+    pf_set_error("stop_mode 2 (bicgstab_CRS) applies to the unpreconditioned FP64 solver");
+    return PF_INVALID;
   }
   if (n <= 0) return PF_OK;
   note_matrix(w, n, val, matrix_changed);
@@ -1191,8 +1207,10 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   } else if (!mixed) {
     Mat<double> m64 = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
     Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), fused_grid(ctx, n), sub, layout};
-    const int guard = o.stop_mode == 1 ? 1 : 0;  // mode 0 keeps the reference's unguarded arithmetic
-    PF_CHECK(prepare_scalars(ctx, w, 0, o, o.itrmax, o.relative_er, o.stop_mode == 1 ? 1 : 0, 0, guard));
+    const int guard = o.stop_mode == 1 ? 1 : 0;  // modes 0 and 2 keep the reference's unguarded arithmetic
+    // kernel stop rules: 0 bicgstab_CRS_2, 1 |r| < relative_er |b|, 3 bicgstab_CRS (er = er0)
+    const int kmode = o.stop_mode == 2 ? 3 : o.stop_mode;
+    PF_CHECK(prepare_scalars(ctx, w, 0, o, o.itrmax, o.relative_er, kmode, 0, guard));
     PF_CHECK(enqueue_init<double>(ctx, L64, m64, b, x, w->r.p, w->r0.p, w->p.p, w->sc.p, w->part.p, false));
     double *ebuf = nullptr;
     if (layout == 0) {  // CRS: e as a stream, then the product gathers e alone

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -17,6 +18,17 @@ void pf_set_error(const char *fmt, ...) {
   va_start(ap, fmt);
   std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
+}
+
+bool pf_graphs_allowed() {
+  static const bool allowed = [] {
+    if (std::getenv("PF_NO_GRAPH")) return false;
+    if (std::getenv("ROCPROFILER_LIBRARY_CTOR")) return false;
+    const char *pre = std::getenv("LD_PRELOAD");
+    if (pre && std::strstr(pre, "rocprofiler")) return false;
+    return true;
+  }();
+  return allowed;
 }
 
 namespace {
@@ -509,6 +521,91 @@ extern "C" int pf_dev_spmv_crs_f32(pf_ctx *ctx, int64_t n, const int64_t *row_pt
   PF_CHECK(pf_read_i64(ctx, row_ptr + n, &end));
   PF_CHECK(pf_spmv_crs_launch<float>(ctx, n, row_ptr, col_idx, val, x, y, (double)(end - 1) / n));
   return PF_OK;
+}
+
+namespace {
+// new_vector += A old_vector on host arrays, as crs_spmv_real64/real32 (SparseClass-c.c:38-66)
+template <typename V>
+int crs_spmv_acc_host(pf_ctx *ctx, const V *val, const int64_t *row_ptr, const int32_t *col_idx,
+                      const V *old_vector, V *new_vector, int64_t n, int64_t col_size) {
+  PF_CHECK(check_ctx(ctx));
+  if (n < 0 || col_size < 0 || !row_ptr) {
+    pf_set_error("crs_spmv: invalid sizes or null row_ptr");
+    return PF_INVALID;
+  }
+  if (n == 0) return PF_OK;
+  if (!old_vector || !new_vector) {
+    pf_set_error("crs_spmv: null vector");
+    return PF_INVALID;
+  }
+  const int64_t nnz = row_ptr[n] - 1;
+  PF_CHECK(validate_crs(n, nnz, row_ptr, col_idx));
+  if (nnz > 0 && !val) {
+    pf_set_error("crs_spmv: null val");
+    return PF_INVALID;
+  }
+  if (nnz > col_size) {
+    pf_set_error("crs_spmv: row_ptr(n+1)-1 = %lld entries but col_size = %lld", (long long)nnz,
+                 (long long)col_size);
+    return PF_INVALID;
+  }
+  DeviceGuard guard(ctx);
+  DevBuf<int64_t> rp;
+  DevBuf<int32_t> ci;
+  DevBuf<V> dv, dx, dy;
+  PF_CHECK(upload(ctx, rp, row_ptr, (size_t)n + 1));
+  PF_CHECK(upload(ctx, ci, col_idx, (size_t)nnz));
+  PF_CHECK(upload(ctx, dv, val, (size_t)nnz));
+  PF_CHECK(upload(ctx, dx, old_vector, (size_t)n));
+  PF_CHECK(upload(ctx, dy, (const V *)new_vector, (size_t)n));
+  PF_CHECK(pf_spmv_crs_launch<V>(ctx, n, rp.p, ci.p, dv.p, dx.p, dy.p, (double)nnz / (double)n, true));
+  return download(ctx, new_vector, dy.p, (size_t)n);
+}
+
+// the context of the reference-named exports: device 0, created on first use
+pf_ctx *reference_ctx() {
+  static pf_ctx *ctx = [] {
+    pf_ctx *c = nullptr;
+    if (pf_ctx_create(0, &c) != PF_OK) c = nullptr;
+    return c;
+  }();
+  return ctx;
+}
+
+template <typename V>
+void crs_spmv_reference_abi(const char *name, const V *val, const long *row_ptr, const int *col_idx,
+                            const V *old_vector, V *new_vector, size_t n, size_t col_size) {
+  static_assert(sizeof(long) == sizeof(int64_t) && sizeof(int) == sizeof(int32_t), "LP64 ABI expected");
+  pf_ctx *ctx = reference_ctx();
+  const int rc = ctx ? crs_spmv_acc_host<V>(ctx, val, (const int64_t *)row_ptr, (const int32_t *)col_idx,
+                                            old_vector, new_vector, (int64_t)n, (int64_t)col_size)
+                     : PF_ERR_HIP;
+  if (rc != PF_OK) {
+    // the reference's signature returns nothing: a failed product must not pass silently
+    std::fprintf(stderr, "%s: %s\n", name, ctx ? pf_last_error() : "no HIP device");
+    std::abort();
+  }
+}
+}  // namespace
+
+extern "C" int pf_crs_spmv_real64(pf_ctx *ctx, const double *val, const int64_t *row_ptr, const int32_t *col_idx,
+                                  const double *old_vector, double *new_vector, int64_t n, int64_t col_size) {
+  return crs_spmv_acc_host<double>(ctx, val, row_ptr, col_idx, old_vector, new_vector, n, col_size);
+}
+
+extern "C" int pf_crs_spmv_real32(pf_ctx *ctx, const float *val, const int64_t *row_ptr, const int32_t *col_idx,
+                                  const float *old_vector, float *new_vector, int64_t n, int64_t col_size) {
+  return crs_spmv_acc_host<float>(ctx, val, row_ptr, col_idx, old_vector, new_vector, n, col_size);
+}
+
+extern "C" void crs_spmv_real64(double val[], long row_ptr[], int col_idx[], double old_vector[],
+                                double new_vector[], size_t n, size_t col_size) {
+  crs_spmv_reference_abi<double>("crs_spmv_real64", val, row_ptr, col_idx, old_vector, new_vector, n, col_size);
+}
+
+extern "C" void crs_spmv_real32(float val[], long row_ptr[], int col_idx[], float old_vector[],
+                                float new_vector[], size_t n, size_t col_size) {
+  crs_spmv_reference_abi<float>("crs_spmv_real32", val, row_ptr, col_idx, old_vector, new_vector, n, col_size);
 }
 
 // ============================================================================ pf_system

@@ -22,6 +22,7 @@
 namespace {
 
 constexpr int MBT = 256;  // threads per block of the streaming kernels
+constexpr int MKEY_WORDS = 20;  // words of the key of a captured iteration chunk
 
 struct MScal {
   double c1, c2, alpha, omega, beta, rr, rr1, bnorm2;
@@ -417,7 +418,7 @@ struct MultiWork {
   DevBuf<MCtl> ctl;
   MCtl *h_ctl = nullptr;
   hipGraphExec_t gexec = nullptr;
-  uint64_t gkey[12] = {};
+  uint64_t gkey[MKEY_WORDS] = {};
   ~MultiWork() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (h_ctl) (void)hipHostFree(h_ctl);
@@ -484,6 +485,10 @@ int pf_multi_solve_device(pf_ctx *ctx, MultiWork **pw, int64_t n, const int64_t 
     pf_set_error("multi-RHS solve needs a DOF=3 system");
     return PF_INVALID;
   }
+  if (o.stop_mode != 0 && o.stop_mode != 1) {
+    pf_set_error("multi-RHS solve: stop_mode must be 0 (bicgstab_CRS_2) or 1 (|r| < relative_er |b|)");
+    return PF_INVALID;
+  }
   if (!*pw) {
     *pw = new MultiWork();
     hipError_t e = hipHostMalloc((void **)&(*pw)->h_ctl, sizeof(MCtl), hipHostMallocDefault);
@@ -546,7 +551,7 @@ int pf_multi_solve_device(pf_ctx *ctx, MultiWork **pw, int64_t n, const int64_t 
   MCtl hc;
   std::memset(&hc, 0, sizeof(hc));
   hc.itrmax = o.itrmax;
-  hc.stop_mode = o.stop_mode == 1 ? 1 : 0;
+  hc.stop_mode = o.stop_mode == 1 ? 1 : 0;  // 0 or 1 (checked at entry)
   hc.nrhs = R;
   hc.er0 = o.er0;
   hc.rel = o.relative_er;
@@ -564,11 +569,15 @@ int pf_multi_solve_device(pf_ctx *ctx, MultiWork **pw, int64_t n, const int64_t 
   k_m_init_finish<<<1, RBT, 0, ctx->stream>>>(w->part.p, (int)L.grid, w->sc.p, w->ctl.p);
   PF_HIP(hipGetLastError());
   int chunk = o.check_every > 0 ? o.check_every : 32;
-  const bool use_graph = o.use_graph != 0;
-  uint64_t key[12] = {(uint64_t)L.grid, (uint64_t)L.rp, (uint64_t)R, (uint64_t)chunk, (uint64_t)n,
-                      (uint64_t)(uintptr_t)X_dev, (uint64_t)(uintptr_t)w->bval.p, (uint64_t)(uintptr_t)w->r.p,
-                      (uint64_t)(uintptr_t)w->p.p, (uint64_t)(uintptr_t)w->part.p, (uint64_t)(uintptr_t)w->y.p,
-                      (uint64_t)(uintptr_t)w->v.p ^ (uint64_t)(uintptr_t)w->r0.p ^ ((uint64_t)(uintptr_t)w->e.p << 1)};
+  const bool use_graph = o.use_graph != 0 && pf_graphs_allowed();
+  // everything the captured chunk bakes in
+  const uint64_t key[MKEY_WORDS] = {
+      (uint64_t)L.grid, (uint64_t)L.rp, (uint64_t)R, (uint64_t)chunk, (uint64_t)n, (uint64_t)(uintptr_t)X_dev,
+      (uint64_t)B.nn, (uint64_t)(uintptr_t)B.noff, (uint64_t)(uintptr_t)B.bcol, (uint64_t)(uintptr_t)B.bval,
+      (uint64_t)(uintptr_t)w->r.p, (uint64_t)(uintptr_t)w->r0.p, (uint64_t)(uintptr_t)w->p.p,
+      (uint64_t)(uintptr_t)w->y.p, (uint64_t)(uintptr_t)w->v.p, (uint64_t)(uintptr_t)w->e.p,
+      (uint64_t)(uintptr_t)w->part.p, (uint64_t)(uintptr_t)w->sc.p, (uint64_t)(uintptr_t)w->ctl.p,
+      (uint64_t)(uintptr_t)w->bt.p};
   bool graph_ok = false;
   if (use_graph) {
     if (!w->gexec || std::memcmp(w->gkey, key, sizeof(key)) != 0) {
@@ -576,19 +585,8 @@ int pf_multi_solve_device(pf_ctx *ctx, MultiWork **pw, int64_t n, const int64_t 
         PF_HIP(hipGraphExecDestroy(w->gexec));
         w->gexec = nullptr;
       }
-      hipGraph_t graph;
-      PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_chunk(ctx, w, L, B, X_dev, chunk);
-      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-      if (rc != PF_OK) return rc;
-      if (ce != hipSuccess) {
-        pf_set_error("hipStreamEndCapture: %s", hipGetErrorString(ce));
-        return PF_ERR_HIP;
-      }
-      hipError_t ie = hipGraphInstantiate(&w->gexec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ie == hipSuccess) std::memcpy(w->gkey, key, sizeof(key));
-      else w->gexec = nullptr;
+      PF_CHECK(pf_capture_graph(ctx, [&] { return enqueue_chunk(ctx, w, L, B, X_dev, chunk); }, &w->gexec));
+      if (w->gexec) std::memcpy(w->gkey, key, sizeof(key));
     }
     graph_ok = w->gexec != nullptr;
   }

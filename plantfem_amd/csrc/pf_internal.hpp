@@ -192,9 +192,10 @@ int pf_bc_fix(pf_ctx *ctx, int64_t n, int64_t nids, const int32_t *ids, const do
 int pf_bc_set_fixed_x(pf_ctx *ctx, int64_t n, const uint8_t *fixed, const double *ubar, double *x);
 
 // spmv.hip
+// y = A x (accumulate: y += A x) with a subgroup of lanes per CRS row
 template <typename V>
 int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
-                       const V *val, const V *x, V *y, double avg_row);
+                       const V *val, const V *x, V *y, double avg_row, bool accumulate = false);
 int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                              int64_t nn, int32_t *ok_dev, int *ok_host);
 // scale (may be NULL): values stored as s_i a_ij c_j, c = cscale (NULL: c = s, the symmetric form)
@@ -309,6 +310,34 @@ int pf_multi_solve_device(pf_ctx *ctx, MultiWork **w, int64_t n, const int64_t *
                           const double *val, uint64_t mat_version, const uint8_t *fixed, int32_t nrhs,
                           const double *B_dev, double *X_dev, const pf_solve_opts &o, pf_stats *per_rhs,
                           pf_stats *total);
+
+// hipGraphs: pf_graphs_allowed() is false when the process runs under a profiler that
+// intercepts the HSA queues (rocprofv3 sets ROCPROFILER_LIBRARY_CTOR and preloads its tool
+// library) or when PF_NO_GRAPH is set; callers then launch the same kernels directly.
+bool pf_graphs_allowed();
+
+// Capture enqueue() (which returns a PF_* status) on ctx->stream into *exec.  The captured
+// graph is destroyed on every path.  Returns the enqueue's error, or PF_ERR_HIP when the
+// capture itself failed; *exec stays null when instantiation fails (the caller then
+// launches directly).
+template <typename F>
+int pf_capture_graph(pf_ctx *ctx, F &&enqueue, hipGraphExec_t *exec) {
+  *exec = nullptr;
+  hipGraph_t graph = nullptr;
+  PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue();
+  const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc != PF_OK || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != PF_OK) return rc;
+    pf_set_error("hipStreamEndCapture: %s", hipGetErrorString(ce));
+    return PF_ERR_HIP;
+  }
+  const hipError_t ie = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) *exec = nullptr;
+  return PF_OK;
+}
 
 // common launch geometry
 static inline unsigned pf_blocks(int64_t work, int threads, int64_t cap = 1 << 20) {

@@ -3,7 +3,9 @@
 
 namespace {
 
-template <typename V, int SUB>
+// y = A x, or y = y + A x (ACC: the reference's C kernels crs_spmv_real64/real32,
+// SparseClass-c.c:38-66, add into new_vector)
+template <typename V, int SUB, bool ACC>
 __global__ void __launch_bounds__(256) k_spmv_crs(int64_t n, const int64_t *__restrict__ row_ptr,
                                                   const int32_t *__restrict__ col_idx,
                                                   const V *__restrict__ val, const V *__restrict__ x,
@@ -13,7 +15,7 @@ __global__ void __launch_bounds__(256) k_spmv_crs(int64_t n, const int64_t *__re
   const int sub = (int)(g % SUB);
   if (row >= n) return;  // whole subgroups leave together (SUB divides 64)
   V s = pfdev::crs_row<V, V, SUB>(row_ptr, col_idx, val, row, sub, pfdev::PlainX<V>{x});
-  if (sub == 0) y[row] = s;
+  if (sub == 0) y[row] = ACC ? y[row] + s : s;
 }
 
 template <typename V>
@@ -183,7 +185,7 @@ __global__ void k_crs_f32(int64_t n, const int64_t *__restrict__ row_ptr, const 
 
 template <typename V>
 int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
-                       const V *val, const V *x, V *y, double avg_row) {
+                       const V *val, const V *x, V *y, double avg_row, bool accumulate) {
   if (n <= 0) return PF_OK;
   // subgroup of ~avg_row/5 lanes (4..64): about five nonzeros per lane (measured at 100^3,
   // 79 nonzeros per row: 16 lanes 0.82 ms, 64 lanes 1.15 ms)
@@ -191,20 +193,29 @@ int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int
   while (sub < 64 && sub * 1.4142 * 5.0 < avg_row) sub *= 2;
   const int64_t threads = n * sub;
   const unsigned blocks = pf_blocks(threads, 256, INT32_MAX);
+#define PF_SPMV_CRS_CASE(S)                                                                  \
+  case S:                                                                                   \
+    if (accumulate)                                                                         \
+      k_spmv_crs<V, S, true><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); \
+    else                                                                                    \
+      k_spmv_crs<V, S, false><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); \
+    break;
   switch (sub) {
-    case 4: k_spmv_crs<V, 4><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
-    case 8: k_spmv_crs<V, 8><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
-    case 16: k_spmv_crs<V, 16><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
-    case 32: k_spmv_crs<V, 32><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
-    default: k_spmv_crs<V, 64><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, x, y); break;
+    PF_SPMV_CRS_CASE(4)
+    PF_SPMV_CRS_CASE(8)
+    PF_SPMV_CRS_CASE(16)
+    PF_SPMV_CRS_CASE(32)
+    default:
+    PF_SPMV_CRS_CASE(64)
   }
+#undef PF_SPMV_CRS_CASE
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
 template int pf_spmv_crs_launch<double>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const double *,
-                                        const double *, double *, double);
+                                        const double *, double *, double, bool);
 template int pf_spmv_crs_launch<float>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const float *,
-                                       const float *, float *, double);
+                                       const float *, float *, double, bool);
 
 int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                              int64_t nn, int32_t *ok_dev, int *ok_host) {

@@ -12,7 +12,8 @@ def declared_functions():
     names = set()
     for h in (ROOT / "include").glob("*.h"):
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
-        for m in re.finditer(r"\b(pf_[a-z0-9_]+)\s*\(", text):
+        # pf_* entry points and the exports under the reference's own C names
+        for m in re.finditer(r"\b((?:pf_|crs_spmv_)[a-z0-9_]+)\s*\(", text):
             names.add(m.group(1))
     return sorted(names)
 
@@ -20,6 +21,7 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     assert "pf_bicgstab_crs" in names and "pf_crs_graph" in names and len(names) >= 25
+    assert {"crs_spmv_real64", "crs_spmv_real32", "pf_crs_spmv_real64", "pf_crs_spmv_real32"} <= set(names)
 
 
 def test_library_exports_every_declared_symbol():

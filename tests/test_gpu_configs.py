@@ -1,0 +1,139 @@
+"""The benchmark configurations against the oracle on the GPU.
+
+Config 2 (100^3 box, E = 1e7 lognormal(1.0), nu ~ U[0.25, 0.35]) at its full size: the CRS
+graph bit for bit against the oracle's ZeroMatrix (FEMDomainClass.f90:10665-10838), the
+assembled values of 1,000 random node rows bit for bit against the oracle's element matrices
+scattered in ascending element order (FEMDomainClass.f90:11123-11232), the traction RHS bit
+for bit, and the FP64 solution's residual recomputed on the CPU with the oracle's elimination
+(FEMSolverClass.f90:1707-1756) and sub_crs_matvec (LinearSolverClass.f90:3728-3776).
+
+Configs 2 and 3 at a reduced size the oracle solves tightly (20^3, the same generators): the
+plain FP64 path and the mixed-precision path against the oracle's bicgstab_CRS_2
+(FEMSolverClass.f90:2199-2298) converged to r-relative 1e-14, which agrees with a sparse
+direct solve of the oracle's system to ~1e-13 (tests/test_oracle.py).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plantfem_amd import workloads as W
+from plantfem_amd.fem import FEMDomain, FEMSolver
+
+pytestmark = pytest.mark.gpu
+
+REDUCED = 20  # mesh of the reduced-size checks (27,783 DOF)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def solver_for(w, ctx):
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    return s
+
+
+def test_c2_full_size_graph_values_and_residual(ctx):
+    w = W.config(2)
+    n = w.ndof
+    say("C2 100^3: n", n)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    crs = s.getCRS()  # assembled, before the elimination
+    rhs_gpu = s.getRHS()
+    assert crs.nnz == W.crs_sizes(100, 100, 100)[1]
+    say("C2 oracle graph")
+    rp, ci = O.zero_matrix(w.elemnod, w.nn, 3)
+    assert np.array_equal(crs.row_ptr, rp)
+    assert np.array_equal(crs.col_idx, ci)
+    del ci
+    # values of 1,000 random node rows (3,000 DOF rows) against the oracle's ascending-element scatter
+    nodes = np.sort(np.random.default_rng(20).choice(w.nn, 1000, replace=False))
+    # include the corners and a node of every face class
+    nodes = np.unique(np.concatenate([nodes, [0, 100, 101 * 101 - 1, w.nn - 1, 101 * 50 + 50]]))
+    say("C2 oracle rows", nodes.size)
+    want = O.assemble_rows(w.nodcoord, w.elemnod, w.E, w.nu, rp, crs.col_idx, nodes)
+    for i in nodes:
+        a, b = rp[3 * i] - 1, rp[3 * i + 3] - 1
+        assert np.array_equal(crs.val[a:b], want[a:b]), int(i)
+    del want
+    rhs = O.traction_box(w.nodcoord, w.elemnod, w.traction_box, w.pressure)
+    assert np.array_equal(rhs_gpu, rhs)
+    # FP64 solve, then its residual on the CPU with the oracle's elimination and product
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    s.itrmax = 40000
+    s.relative_er = 1e-9
+    say("C2 FP64 solve")
+    x = s.solve(stop_mode=1)
+    st = s.last_stats
+    say("C2 iterations", st["iterations"], "gpu relres", st["true_relres"])
+    assert st["status"] == 0
+    fixed = np.zeros(n, np.uint8)
+    fixed[w.fixed_dofs - 1] = 1
+    v2, r2 = O.apply_dirichlet(rp, crs.col_idx, crs.val, rhs, fixed, np.zeros(n))
+    res = np.linalg.norm(r2 - O.spmv(rp, crs.col_idx, v2, x)) / np.linalg.norm(r2)
+    say("C2 oracle relres", res)
+    assert res <= 1e-8
+    assert abs(res - st["true_relres"]) <= 1e-3 * res
+    assert np.all(x[w.fixed_dofs - 1] == 0.0)
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def reduced():
+    """Oracle systems and tight solutions of the config 2 and 3 generators at REDUCED^3."""
+    out = {}
+    for k, w in ((2, W.box(REDUCED, sigma=1.0, seed=0, nu_range=(0.25, 0.35))),
+                 (3, W.box(REDUCED, sigma=2.0, seed=0, nu_range=(0.25, 0.35), clip_ratio=1.0e4))):
+        rp, ci, val, rhs, fixed, ub = O.build_system(w)
+        say(f"oracle solve, config {k} generator at {REDUCED}^3")
+        x, st, it, *_ = O.solve(rp, ci, val, rhs, fixed, ub, relative_er=1e-14, er0=1e-30)
+        assert st == 0
+        out[k] = (w, x, it)
+    return out
+
+
+@pytest.mark.parametrize("kind", [2, 1])
+def test_c2_generator_fp64_matches_oracle(ctx, reduced, kind):
+    w, xo, ito = reduced[2]
+    s = solver_for(w, ctx)
+    s.relative_er = 1e-14
+    s.er0 = 1e-30
+    x = s.solve(spmv_kind=kind)
+    st = s.last_stats
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    say("C2 generator kind", kind, "iters gpu", st["iterations"], "oracle", ito, "err", err)
+    assert st["status"] == 0 and st["spmv_kind_used"] == kind
+    assert err <= 1e-10
+
+
+def test_c3_generator_matches_oracle(ctx, reduced):
+    """The mixed solve of the config 3 system (sigma 2.0, clipped to a 1e4 contrast) to relres
+    1e-10, and the FP64 solve, against the oracle's tight solution."""
+    w, xo, ito = reduced[3]
+    assert w.E.max() / w.E.min() <= 1.0e4 * (1 + 1e-12)
+    s = solver_for(w, ctx)
+    s.relative_er = 1e-14
+    s.er0 = 1e-30
+    x64 = s.solve()
+    st64 = s.last_stats
+    err64 = np.linalg.norm(x64 - xo) / np.linalg.norm(xo)
+    say("C3 generator FP64 iters", st64["iterations"], "oracle", ito, "err", err64)
+    assert st64["status"] == 0 and err64 <= 1e-10
+    x32 = s.solve(precision=32, target_relres=1e-10)
+    st = s.last_stats
+    err = np.linalg.norm(x32 - xo) / np.linalg.norm(xo)
+    say("C3 generator mixed: replacements", st["outer_iterations"], "FP32 iterations", st["iterations"],
+        "relres", st["true_relres"], "err", err)
+    assert st["status"] == 0 and st["true_relres"] < 1e-10
+    assert err <= 1e-8
+    s.close()

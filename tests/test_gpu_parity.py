@@ -161,15 +161,165 @@ def test_traction_and_dirichlet_bit_exact(ctx):
 
 
 # ------------------------------------------------------------------------------ SpMV
-@pytest.mark.parametrize("kind", [1, 2])
+def fp32_bound(rp, ci, val, x, nnz_row):
+    """|A||x| per row times 2 gamma_k (k entries per row, unit roundoff 2^-24): the error bound of
+    any summation order of the row's FP32 products (Higham, Accuracy and Stability, 3.1)."""
+    u = 2.0 ** -24
+    g = 2 * nnz_row * u / (1 - nnz_row * u)
+    return g * O.spmv(rp, ci, np.abs(np.asarray(val, np.float64)), np.abs(np.asarray(x, np.float64)))
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
 def test_spmv_matches_oracle(ctx, kind):
+    """The system's products by layout: 1 CRS rows and 2 blocked, FP64, against sub_crs_matvec;
+    3 blocked FP32 (values and x rounded to FP32, FP32 sums) against the FP32 restatement of
+    crs_spmv_real32 within the FP32 summation bound."""
     w = W.box(12, sigma=1.0, seed=8)
     rp, ci, val, rhs, fixed, ub = O.build_system(w)
     x = np.random.default_rng(3).standard_normal(rhs.size)
-    say("spmv crs")
-    y = spmv_crs(rp, ci, val, x, ctx)
-    ref = O.spmv(rp, ci, val, x)
-    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    say("spmv kind", kind)
+    y, _ = s.spmv(kind, x)
+    if kind in (1, 2):
+        ref = O.spmv(rp, ci, val, x)
+        assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+    else:
+        ref = O.crs_spmv(rp, ci, val.astype(np.float32), x.astype(np.float32), np.zeros(rhs.size, np.float32))
+        x32 = x.astype(np.float32)
+        bound = fp32_bound(rp, ci, val.astype(np.float32), x32, int(np.diff(rp).max()))
+        assert np.all(np.abs(y - ref.astype(np.float64)) <= bound)
+    s.close()
+
+
+def test_dev_spmv_crs_f32(ctx):
+    """pf_dev_spmv_crs_f32 on device arrays against the FP32 restatement (y = A x)."""
+    import torch
+    w = W.box(9, sigma=1.0, seed=4)
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    n = rhs.size
+    rng = np.random.default_rng(5)
+    v32 = val.astype(np.float32)
+    x32 = rng.standard_normal(n).astype(np.float32)
+    dev = torch.device("cuda", ctx.device)
+    t_rp, t_ci = torch.from_numpy(rp).to(dev), torch.from_numpy(ci).to(dev)
+    t_v, t_x = torch.from_numpy(v32).to(dev), torch.from_numpy(x32).to(dev)
+    t_y = torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    say("dev spmv f32")
+    L.check(L.load().pf_dev_spmv_crs_f32(ctx.h, n, t_rp.data_ptr(), t_ci.data_ptr(), t_v.data_ptr(),
+                                         t_x.data_ptr(), t_y.data_ptr()), "pf_dev_spmv_crs_f32")
+    ctx.synchronize()
+    y = t_y.cpu().numpy().astype(np.float64)
+    ref = O.crs_spmv(rp, ci, v32, x32, np.zeros(n, np.float32)).astype(np.float64)
+    assert np.all(np.abs(y - ref) <= fp32_bound(rp, ci, v32, x32, int(np.diff(rp).max())))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_crs_spmv_accumulates(ctx, prec):
+    """crs_spmv_real64/real32 (SparseClass-c.c:38-66) add A x into new_vector; the pf_ form and
+    the export under the reference's C name against the oracle's restatement."""
+    dt, ct = (np.float64, C.c_double) if prec == 64 else (np.float32, C.c_float)
+    w = W.box(7, sigma=0.8, seed=6)
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    n = rhs.size
+    rng = np.random.default_rng(7)
+    v = val.astype(dt)
+    x = rng.standard_normal(n).astype(dt)
+    y0 = (1e7 * rng.standard_normal(n)).astype(dt)
+    ref = O.crs_spmv(rp, ci, v, x, y0).astype(np.float64)
+    lib = L.load()
+    pf = lib.pf_crs_spmv_real64 if prec == 64 else lib.pf_crs_spmv_real32
+    ref_name = lib.crs_spmv_real64 if prec == 64 else lib.crs_spmv_real32
+    say("crs_spmv accumulate", prec)
+    y1 = y0.copy()
+    L.check(pf(ctx.h, L.ptr(v, ct), L.ptr(rp, C.c_int64), L.ptr(ci, C.c_int32), L.ptr(x, ct), L.ptr(y1, ct),
+               n, ci.size), "pf_crs_spmv")
+    y2 = y0.copy()
+    ref_name(L.ptr(v, ct), L.ptr(rp, C.c_int64), L.ptr(ci, C.c_int32), L.ptr(x, ct), L.ptr(y2, ct), n, ci.size)
+    if prec == 64:
+        tol = 1e-13 * np.abs(ref).max()
+        assert np.abs(y1 - ref).max() <= tol and np.abs(y2 - ref).max() <= tol
+    else:
+        # FP32 bound of the row sums plus the rounding of the final add into y0
+        bound = fp32_bound(rp, ci, v, x, int(np.diff(rp).max()) + 1) + 2.0 ** -23 * np.abs(ref)
+        for y in (y1, y2):
+            assert np.all(np.abs(y.astype(np.float64) - ref) <= bound + 2.0 ** -24 * np.abs(y0))
+    # an invalid CRS is rejected by the pf_ form, and the input vector is left alone
+    bad = ci.copy()
+    bad[3] = n + 5
+    y3 = y0.copy()
+    assert pf(ctx.h, L.ptr(v, ct), L.ptr(rp, C.c_int64), L.ptr(bad, C.c_int32), L.ptr(x, ct), L.ptr(y3, ct),
+              n, ci.size) == L.PF_INVALID
+    assert np.array_equal(y3, y0)
+
+
+def test_bicgstab_crs_stop_rule(ctx):
+    """stop_mode 2: bicgstab_CRS (LinearSolverClass.f90:1493-1569) -- the loop ends at the first
+    iteration with r.r / r0.r0 < er (no square root); it stops where the oracle does."""
+    w = W.box(6, sigma=0.8, seed=9)
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    v2, r2 = O.apply_dirichlet(rp, ci, val, rhs, fixed, ub)
+    n = r2.size
+    x0 = 1e-6 * np.random.default_rng(2).standard_normal(n)
+    rr0 = float(np.dot(r2 - O.spmv(rp, ci, v2, x0), r2 - O.spmv(rp, ci, v2, x0)))
+    for er in (1e-8, 1e-16):
+        say("bicgstab_CRS er", er)
+        x, st = bicgstab_crs(rp, ci, v2, r2, x0=x0, ctx=ctx, stop_mode=2, er0=er)
+        xo, sto, ito, rro, rr0o, h = O.bicgstab_crs(rp, ci, v2, r2, er, x0=x0, hist=True)
+        say("iters gpu", st["iterations"], "oracle", ito)
+        assert st["status"] == 0 and sto == 0
+        assert abs(st["iterations"] - ito) <= 1
+        assert st["rr"] / rr0 < er  # the rule itself, at the GPU's stop
+        k = st["iterations"]
+        if k >= 2 and k - 2 < h.size:  # and not one iteration earlier (oracle's recurrence)
+            assert h[k - 2] ** 2 / rr0o >= er * (1 - 1e-6)
+        assert abs(rr0o - rr0) <= 1e-12 * rr0
+        # when r.r / r0.r0 sits at the threshold the two may stop one iteration apart: compare
+        # with the oracle's iterate after the GPU's number of iterations
+        xk, *_ = O.bicgstab(rp, ci, v2, r2, x0=x0, itrmax=k, er0=0.0, relative_er=0.0)
+        assert np.linalg.norm(x - xk) / np.linalg.norm(xk) <= 1e-6
+        if k == ito:
+            assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+    # the return before the loop when r0.r0 < er (:1527): x is left as it came
+    say("bicgstab_CRS early return")
+    b = np.full(n, 1e-9)
+    x, st = bicgstab_crs(rp, ci, v2, b, x0=np.zeros(n), ctx=ctx, stop_mode=2, er0=1e-10)
+    assert st["status"] == 0 and st["iterations"] == 0 and np.all(x == 0.0)
+    xo, sto, ito, *_ = O.bicgstab_crs(rp, ci, v2, b, 1e-10)
+    assert sto == 0 and ito == 0
+    # the LinearSolver_ tutorial system (Tutorial/std/convertCOOtoCRS_LinearSolver.f90, n = 100,
+    # er0 = dble(1.0e-12)): nonsymmetric, so only the rule at the stop is compared
+    import scipy.sparse as sp
+    m = 100
+    A = np.zeros((m, m))
+    A[0, 0], A[0, 1] = 1.0, 0.5
+    for i in range(1, m - 1):
+        A[i, i - 1] = A[i, i] = 0.5
+        A[i, i + 1] = 0.5 if i != m - 2 else 0.0
+    A[m - 1, m - 1] = 1.0
+    bt = np.zeros(m)
+    bt[m - 2], bt[m - 1] = -5.0, 10.0
+    M = sp.csr_matrix(A)
+    trp, tci = (M.indptr + 1).astype(np.int64), (M.indices + 1).astype(np.int32)
+    er = float(np.float32(1e-12))
+    x, st = bicgstab_crs(trp, tci, M.data, bt, ctx=ctx, stop_mode=2, er0=er, itrmax=10000)
+    xo, sto, ito, *_ = O.bicgstab_crs(trp, tci, M.data, bt, er, itrmax=10000)
+    say("tutorial system: iters gpu", st["iterations"], "oracle", ito)
+    assert st["status"] == sto == 0 and st["rr"] / float(bt @ bt) < er
+
+
+def test_breakdown_alpha_guarded(ctx):
+    """r0.(A p) = 0 on the first iteration: with stop_mode 1 the solver reports
+    PF_BREAKDOWN_C2C3 before x is touched (A = [[0, 1], [-1, 0]], b = (1, 0))."""
+    rp = np.array([1, 2, 3], np.int64)
+    ci = np.array([2, 1], np.int32)
+    val = np.array([1.0, -1.0])
+    say("alpha breakdown")
+    x, st = bicgstab_crs(rp, ci, val, np.array([1.0, 0.0]), ctx=ctx, stop_mode=1)
+    assert st["status"] == L.PF_BREAKDOWN_C2C3
+    assert np.array_equal(x, np.zeros(2))
 
 
 # ---------------------------------------------------------------------------- solver

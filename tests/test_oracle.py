@@ -338,6 +338,112 @@ def test_bicgstab_first_iteration_closed_form():
     assert np.allclose(x, x1, rtol=1e-15) and np.isclose(rr1, r1 @ r1, rtol=1e-14)
 
 
+def _eliminated(w):
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    v2, r2 = O.apply_dirichlet(rp, ci, val, rhs, fixed, ub)
+    return rp, ci, v2, r2
+
+
+def test_bicgstab_crs_stop_rule_from_history():
+    """bicgstab_CRS (LinearSolverClass.f90:1493-1569) runs bicgstab_CRS_2's recurrence and stops
+    at the first iteration with r.r / r0.r0 < er (r0 = b - A x0, no square root).  The history of
+    bicgstab_CRS_2 run past that point fixes the iteration; the square-root test and an r1-relative
+    test stop elsewhere, so either mistake fails."""
+    rp, ci, v2, r2 = _eliminated(W.box(5, sigma=0.7, seed=3))
+    x0 = 1e-7 * np.random.default_rng(1).standard_normal(r2.size)
+    r0 = r2 - O.spmv(rp, ci, v2, x0)
+    rr0 = float(r0 @ r0)
+    _, _, _, _, rr1, h = O.bicgstab(rp, ci, v2, r2, x0=x0, itrmax=400, er0=0.0, relative_er=0.0, hist=True)
+    for er in (1e-6, 1e-12):
+        k = int(np.argmax(h ** 2 / rr0 < er)) + 1
+        assert h[k - 1] ** 2 / rr0 < er and (k == 1 or h[k - 2] ** 2 / rr0 >= er)
+        x, st, it, rr, init_rr = O.bicgstab_crs(rp, ci, v2, r2, er, x0=x0)
+        assert st == 0 and it == k and init_rr == pytest.approx(rr0, rel=1e-13)
+        # the same iterate as bicgstab_CRS_2 stopped after k iterations
+        xk, *_ = O.bicgstab(rp, ci, v2, r2, x0=x0, itrmax=k, er0=0.0, relative_er=0.0)
+        assert np.array_equal(x, xk)
+        assert k != int(np.argmax(h / np.sqrt(rr0) < er)) + 1      # with a square root
+        assert k != int(np.argmax(h ** 2 / rr1 < er)) + 1          # relative to r1
+    # return before the loop: x unchanged, no iteration
+    x0 = np.random.default_rng(2).standard_normal(r2.size)
+    b = O.spmv(rp, ci, v2, x0) + 1e-9
+    x, st, it, rr, init_rr = O.bicgstab_crs(rp, ci, v2, b, 1e-10, x0=x0)
+    assert st == 0 and it == 0 and np.array_equal(x, x0) and init_rr < 1e-10
+
+
+def _py_crs_spmv(rp, ci, val, x, y, dt):
+    """crs_spmv_real64/real32 (SparseClass-c.c:38-66) as a plain loop in the given precision."""
+    y = np.array(y, dtype=dt)
+    for row in range(rp.size - 1):
+        for col in range(rp[row], rp[row + 1]):
+            y[row] = dt(y[row] + dt(val[col - 1] * x[ci[col - 1] - 1]))
+    return y
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_crs_spmv_accumulate_restatement(dt):
+    """new_vector + A old_vector, row entries in stored order: bitwise equal to a plain loop in
+    the same precision, and equal to y + A x (dense) to rounding."""
+    rng = np.random.default_rng(4)
+    A = sp.random(40, 40, density=0.25, random_state=2, format="csr")
+    A.data = rng.standard_normal(A.data.size)
+    perm = rng.permutation(A.indices.size)  # unsorted columns within rows
+    rp = (A.indptr + 1).astype(np.int64)
+    ci = (A.indices + 1).astype(np.int32)
+    for i in range(40):  # shuffle each row's stored order
+        a, b = A.indptr[i], A.indptr[i + 1]
+        q = a + np.argsort(perm[a:b])
+        ci[a:b], A.data[a:b] = ci[q].copy(), A.data[q].copy()
+    val = A.data.astype(dt)
+    x = rng.standard_normal(40).astype(dt)
+    y0 = rng.standard_normal(40).astype(dt)
+    y = O.crs_spmv(rp, ci, val, x, y0)
+    assert y.dtype == dt and np.array_equal(y, _py_crs_spmv(rp, ci, val, x, y0, dt))
+    dense = np.zeros((40, 40))
+    for i in range(40):
+        for k in range(rp[i] - 1, rp[i + 1] - 1):
+            dense[i, ci[k] - 1] += float(val[k])
+    tol = 1e-13 if dt == np.float64 else 1e-5
+    assert np.allclose(y.astype(np.float64), y0.astype(np.float64) + dense @ x.astype(np.float64), rtol=tol, atol=tol)
+
+
+def test_assemble_rows_equals_full_assembly():
+    """The row-restricted assembly used for large-mesh spot checks equals the full
+    ascending-element assembly bitwise on the marked rows and is zero elsewhere."""
+    rng = np.random.default_rng(8)
+    nod, el = O.cube3d(5, 4, 3)
+    nod = np.asfortranarray(nod + 0.04 * rng.standard_normal(nod.shape) / 5)
+    perm = rng.permutation(el.shape[0])
+    el = np.asfortranarray(el[perm])
+    E = 1e7 * np.exp(rng.standard_normal(el.shape[0]))
+    nu = rng.uniform(0.25, 0.35, el.shape[0])
+    rp, ci = O.zero_matrix(el, nod.shape[0], 3)
+    full = O.assemble(nod, el, E, nu, rp, ci)
+    nodes = rng.choice(nod.shape[0], 17, replace=False)
+    part = O.assemble_rows(nod, el, E, nu, rp, ci, nodes)
+    mask = np.zeros(ci.size, bool)
+    for i in nodes:
+        mask[rp[3 * i] - 1:rp[3 * i + 3] - 1] = True
+    assert np.array_equal(part[mask], full[mask]) and np.all(part[~mask] == 0.0)
+
+
+@pytest.mark.parametrize("cfg", [2, 3])
+def test_config_generators_tight_oracle_equals_direct(cfg):
+    """The generators of configs 2 and 3 (sigma 1.0, and sigma 2.0 clipped to a 1e4 contrast) at
+    12^3: the oracle's bicgstab_CRS_2 to r1-relative 1e-14 agrees with a sparse direct solve of
+    the same eliminated system, so it serves as the FP64 reference of the GPU tests."""
+    if cfg == 2:
+        w = W.box(12, sigma=1.0, seed=0, nu_range=(0.25, 0.35))
+    else:
+        w = W.box(12, sigma=2.0, seed=0, nu_range=(0.25, 0.35), clip_ratio=1.0e4)
+        assert w.E.max() / w.E.min() <= 1.0e4 * (1 + 1e-12) and w.E.max() / w.E.min() > 1e3
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    x, st, it, *_ = O.solve(rp, ci, val, rhs, fixed, ub, relative_er=1e-14, er0=1e-30)
+    v2, r2 = O.apply_dirichlet(rp, ci, val, rhs, fixed, ub)
+    xd = spl.spsolve(sp.csr_matrix((v2, ci - 1, rp - 1), shape=(r2.size, r2.size)).tocsc(), r2)
+    assert st == 0 and np.linalg.norm(x - xd) <= 1e-11 * np.linalg.norm(xd)
+
+
 def test_c1_matches_direct_solve():
     """Config-1 path (mesh, graph, Ke, traction, elimination, BiCGSTAB) against a direct
     sparse factorisation of the same eliminated system."""
