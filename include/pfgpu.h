@@ -191,6 +191,22 @@ void crs_spmv_real64(double val[], long row_ptr[], int col_idx[], double old_vec
 void crs_spmv_real32(float val[], long row_ptr[], int col_idx[], float old_vector[], float new_vector[],
                      size_t n, size_t col_size);
 
+/* ---- allocation-free solve ---------------------------------------------------------
+ * No counterpart in the reference: pf_bicgstab_crs (bicgstab_CRS_2, FEMSolverClass.f90:
+ * 2199-2298) on DEVICE arrays for callers that cannot allocate device memory (code under
+ * hipStream capture, tight serving loops, harnesses that forbid hipMalloc).  The work vectors,
+ * partial sums and scalars live in a static device array of each GPU (about 30 MB, held until
+ * the process ends), so at most PF_NOALLOC_MAX_ROWS rows; larger systems are rejected with
+ * PF_INVALID.  Makes no device or host allocation and captures no hipGraph.  FP64 on the CRS
+ * layout; opts as pf_bicgstab_crs (precision 64, preconditioning 0, stop_mode 0/1/2); x holds x0
+ * on entry.  The CRS is checked on the device first (nondecreasing row_ptr from 1, columns in
+ * 1..n); col_idx and val must hold row_ptr(n+1)-1 entries.  Calls on one GPU share its scratch
+ * array: calls that overlap in time on one GPU race. */
+#define PF_NOALLOC_MAX_ROWS 524288
+int pf_dev_bicgstab_crs_noalloc(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                                const double *val, const double *rhs, double *x, const pf_solve_opts *opts,
+                                pf_stats *stats);
+
 /* ---- FEMSolver_ state on the device (FEMSolverClass.f90:13-164) ------------------ */
 int pf_system_create(pf_ctx *ctx, pf_system **out);
 int pf_system_destroy(pf_system *sys);

@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -19,6 +20,7 @@ PF_NOT_CONVERGED = 3
 PF_INVALID = 4
 PF_ERR_HIP = -1
 PF_ERR_NOMEM = -2
+PF_NOALLOC_MAX_ROWS = 524288  # include/pfgpu.h
 
 STATUS_NAMES = {
     PF_OK: "ok",
@@ -130,6 +132,7 @@ SIGNATURES = {
     "pf_dev_spmv_crs_f32": (C.c_int, [P, I64, P, P, P, P, P]),
     "pf_crs_spmv_real64": (C.c_int, [P, PF64, PI64, PI32, PF64, PF64, I64, I64]),
     "pf_crs_spmv_real32": (C.c_int, [P, PF32, PI64, PI32, PF32, PF32, I64, I64]),
+    "pf_dev_bicgstab_crs_noalloc": (C.c_int, [P, I64, P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(Stats)]),
     "crs_spmv_real64": (None, [PF64, PI64, PI32, PF64, PF64, C.c_size_t, C.c_size_t]),
     "crs_spmv_real32": (None, [PF32, PI64, PI32, PF32, PF32, C.c_size_t, C.c_size_t]),
     "pf_system_create": (C.c_int, [P, C.POINTER(P)]),
@@ -242,6 +245,12 @@ class Context:
     """One device and one HIP stream (pf_ctx)."""
 
     def __init__(self, device: int = 0):
+        # torch's wheel bundles its own HIP/HSA runtime beside the one this library links; in one
+        # process torch's must come up first or torch later finds no device, so start it now
+        # when torch is in use
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_available():
+            torch.cuda.init()
         lib = load()
         h = P()
         check(lib.pf_ctx_create(int(device), C.byref(h)), "pf_ctx_create")

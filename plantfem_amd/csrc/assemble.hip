@@ -266,6 +266,131 @@ __device__ inline double hex8_gp_G(const double (&X)[8][3], int gp, int jmode, d
   return detJ;
 }
 
+// Ke of one element per wavefront with the geometry evaluated once per element: lanes 0..7
+// evaluate J, det J and G = (J^-T or J^-1) dN/dxi at Gauss point `lane` (hex8_gp_G, the same
+// operations as hex8_ke_row) into LDS; lane 2r+h (r < 24) then forms row r of Ke over the
+// columns of nodes 4h..4h+3 with hex8_ke_row's products, sums and Gauss point order, so every
+// entry is bitwise the one hex8_ke_row gives.  COLMAJOR: ke[q*576 + c*24 + r] (the reference's
+// Fortran layout), else row-major ke[q*576 + r*24 + c].
+template <bool COLMAJOR>
+__global__ void __launch_bounds__(256) k_ke_elem(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                                                 const int32_t *__restrict__ en, const double *__restrict__ E,
+                                                 const double *__restrict__ nu, int jmode, int64_t first,
+                                                 int64_t count, double *__restrict__ ke) {
+  __shared__ double sG[4][8][25];  // per wavefront: 8 Gauss points x (G(3,8), det J)
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const bool live = q < count;
+  const int64_t e = first + (live ? q : 0);
+  if (live && lane < 8) {
+    int32_t nodes[8];
+    double X[8][3];
+    load_element(nn, ne, xyz, en, e, nodes, X);
+    double G[3][8];
+    const double dj = hex8_gp_G(X, lane, jmode, G, true);
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) sG[w][lane][p * 8 + b] = G[p][b];
+    sG[w][lane][24] = dj;
+  }
+  __syncthreads();
+  if (!live || lane >= 48) return;
+  const int r = lane >> 1, h = lane & 1;
+  const int a = r / 3, d = r - 3 * (r / 3);
+  const double Ee = E[e], ve = nu[e];
+  const double mu = Ee / 2.0 / (1.0 + ve);
+  const double lam = ve * Ee / (1.0 + ve) / (1.0 - 2.0 * ve);
+  const double dg = 2.0 * mu + lam;
+  double acc[12];
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    const double *g = sG[w][gp];
+    const double detJ = g[24];
+    const double ga0 = g[a], ga1 = g[8 + a], ga2 = g[16 + a];
+    const double gd = (d == 0) ? ga0 : ((d == 1) ? ga1 : ga2);
+    const double T0 = gd * ((d == 0) ? dg : lam);
+    const double T1 = gd * ((d == 1) ? dg : lam);
+    const double T2 = gd * ((d == 2) ? dg : lam);
+    const double T3 = (d == 0) ? ga1 * mu : ((d == 1) ? ga0 * mu : 0.0);
+    const double T4 = (d == 0) ? 0.0 : ((d == 1) ? ga2 * mu : ga1 * mu);
+    const double T5 = (d == 0) ? ga2 * mu : ((d == 1) ? 0.0 : ga0 * mu);
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb) {
+      const int b = 4 * h + bb;
+      const double G0 = g[b], G1 = g[8 + b], G2 = g[16 + b];
+      const double k0 = (T0 * G0 + T3 * G1) + T5 * G2;
+      const double k1 = (T1 * G1 + T3 * G0) + T4 * G2;
+      const double k2 = (T2 * G2 + T4 * G1) + T5 * G0;
+      if (gp == 0) {
+        acc[3 * bb + 0] = k0 * detJ;
+        acc[3 * bb + 1] = k1 * detJ;
+        acc[3 * bb + 2] = k2 * detJ;
+      } else {
+        acc[3 * bb + 0] = acc[3 * bb + 0] + k0 * detJ;
+        acc[3 * bb + 1] = acc[3 * bb + 1] + k1 * detJ;
+        acc[3 * bb + 2] = acc[3 * bb + 2] + k2 * detJ;
+      }
+    }
+  }
+  double *out = ke + q * 576;
+#pragma unroll
+  for (int t = 0; t < 12; ++t) {
+    const int c = 12 * h + t;
+    out[COLMAJOR ? c * 24 + r : r * 24 + c] = acc[t];
+  }
+}
+
+// CRS values of a hex8 / DOF=3 ZeroMatrix pattern (rows = node neighbour lists ncol expanded
+// by DOF, FEMDomainClass.f90:10780-10836) from row-major element matrices: one
+// SUB-lane subgroup per CRS row r = 3i + d, lanes over the row's entries; entry (r, 3j+d')
+// sums Ke_e(3a+d, 3b+d') over the row node's incident elements e in ascending id, local nodes
+// a (= i) and b (= j) ascending -- the order of the reference's element loop -- starting from
+// 0.0, and is written once.
+template <int SUB>
+__global__ void __launch_bounds__(256) k_gather_rows(int64_t nn, int64_t ne, const int32_t *__restrict__ en,
+                                                     const int64_t *__restrict__ eoff,
+                                                     const int32_t *__restrict__ elist,
+                                                     const int64_t *__restrict__ noff,
+                                                     const int32_t *__restrict__ ncol,
+                                                     const int64_t *__restrict__ row_ptr,
+                                                     const double *__restrict__ ke, double *__restrict__ val) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t r = g / SUB;
+  const int lane = (int)(g % SUB);
+  if (r >= 3 * nn) return;  // whole subgroups
+  const int64_t i = r / 3;
+  const int d = (int)(r - 3 * i);
+  const int64_t base = row_ptr[r] - 1;
+  const int64_t len = row_ptr[r + 1] - 1 - base;
+  const int64_t s0 = eoff[i], s1 = eoff[i + 1];
+  for (int64_t q0 = 0; q0 < len; q0 += SUB) {
+    const int64_t q = q0 + lane;
+    const bool on = q < len;
+    // the row is the node's neighbour list expanded by DOF: entry q is DOF q % 3 of neighbour q / 3
+    const int32_t j1 = on ? ncol[noff[i] + q / 3] + 1 : 0;
+    const int dc = (int)(q % 3);
+    double v = 0.0;
+    for (int64_t s = s0; s < s1; ++s) {
+      const int64_t e = elist[s];
+      if (s > s0 && elist[s - 1] == e) continue;  // node listed twice in one element
+      int32_t nodes[8];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) nodes[b] = en[(int64_t)b * ne + e];
+      const double *ke_e = ke + e * 576;
+#pragma unroll 1
+      for (int a = 0; a < 8; ++a) {
+        if (nodes[a] != (int32_t)(i + 1)) continue;
+        const double *kr = ke_e + (3 * a + d) * 24;
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+          if (nodes[b] == j1) v = v + kr[3 * b + dc];
+      }
+    }
+    if (on) val[base + q] = v;
+  }
+}
+
 // N_a = 0.1250d0*(1 -+ xi)*(1 -+ eta)*(1 -+ zeta) at Gauss point gp (ShapeFunctionClass.f90:781-788)
 __device__ inline double hex8_N(int a, int gp) {
   const double gx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? GPV : -GPV;
@@ -740,9 +865,18 @@ int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const dou
     return PF_INVALID;
   }
   if (g.nn == 0) return PF_OK;
-  k_assemble_rows<<<pf_blocks(3 * g.nn, 128), 128, 0, ctx->stream>>>(
-      g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p,
-      row_ptr, val);
+  // element matrices once per element (row-major scratch of 4.6 KB per element), then one
+  // gather per CRS row; without room for the scratch, the one-pass row kernel
+  if (g.ne > 0 && g.kebuf.ensure((size_t)g.ne * 576) == PF_OK) {
+    k_ke_elem<false><<<pf_blocks(g.ne * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+        g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, 0, g.ne, g.kebuf.p);
+    k_gather_rows<16><<<pf_blocks(3 * g.nn * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+        g.nn, g.ne, g.elemnod.p, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p, row_ptr, g.kebuf.p, val);
+  } else {
+    k_assemble_rows<<<pf_blocks(3 * g.nn, 128), 128, 0, ctx->stream>>>(
+        g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p,
+        row_ptr, val);
+  }
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -754,8 +888,9 @@ int pf_hex8_ke_batch(pf_ctx *ctx, const NodeGraph &g, const double *E, const dou
     return PF_INVALID;
   }
   if (count <= 0) return PF_OK;
-  k_ke_batch<<<pf_blocks(count * 24, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p,
-                                                                  E, nu, jmode, first, count, ke);
+  k_ke_elem<true><<<pf_blocks(count * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p,
+                                                                                 g.elemnod.p, E, nu, jmode, first,
+                                                                                 count, ke);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -832,8 +967,8 @@ int pf_elem_bilinear(pf_ctx *ctx, const NodeGraph &g, const double *E, const dou
   PF_CHECK(ke.ensure((size_t)chunk * 576));
   for (int64_t first = 0; first < g.ne; first += chunk) {
     const int64_t count = std::min(chunk, g.ne - first);
-    k_ke_batch<<<pf_blocks(count * 24, 128), 128, 0, ctx->stream>>>(g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu,
-                                                                   jmode, first, count, ke.p);
+    k_ke_elem<true><<<pf_blocks(count * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+        g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, first, count, ke.p);
     k_ke_contract<<<(unsigned)std::min<int64_t>(count, 65536), 64, 0, ctx->stream>>>(g.ne, g.elemnod.p, first, count,
                                                                                      ke.p, R, U, Lam, fixed, out);
     PF_HIP(hipGetLastError());

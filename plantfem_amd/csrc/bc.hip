@@ -30,27 +30,45 @@ __global__ void k_fix_write(int64_t nids, const int32_t *__restrict__ ids, int64
   ubar[id - 1] = values ? values[t] : fixvalue;
 }
 
-// one thread per row: (1) b_i -= A_ij*ubar_j over fixed columns in CRS order,
-// (2) b_i = ubar_i on fixed rows, (3) zero fixed rows and columns, (4) diagonal 1 on fixed rows
-__global__ void k_bc_rows(int64_t n, const int64_t *__restrict__ row_ptr,
-                          const int32_t *__restrict__ col_idx, double *__restrict__ val,
-                          double *__restrict__ rhs, const uint8_t *__restrict__ fixed,
-                          const double *__restrict__ ubar) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// one SUB-lane subgroup per row, coalesced over the row's columns:
+// (1) b_i -= A_ij*ubar_j over the fixed columns in CRS order, (2) b_i = ubar_i on fixed rows,
+// (3) zero fixed rows and columns, (4) diagonal 1 on fixed rows.
+// Step (1) is a sequential sum in the reference's order, so it runs on the subgroup's first
+// lane, and only for the rows that hold a fixed column (a layer of rows next to each fixed
+// set); every other row keeps b.  A fixed row's b is replaced by ubar whatever (1) gave.
+template <int SUB>
+__global__ void __launch_bounds__(256) k_bc_rows(int64_t n, const int64_t *__restrict__ row_ptr,
+                                                 const int32_t *__restrict__ col_idx, double *__restrict__ val,
+                                                 double *__restrict__ rhs, const uint8_t *__restrict__ fixed,
+                                                 const double *__restrict__ ubar) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = g / SUB;
+  const int lane = (int)(g % SUB);
+  if (i >= n) return;  // whole subgroups leave together (SUB divides 64)
   const int64_t k0 = row_ptr[i] - 1, k1 = row_ptr[i + 1] - 1;
   const bool fr = fixed[i] != 0;
-  double b = rhs[i];
-  for (int64_t k = k0; k < k1; ++k) {
-    const int64_t c = col_idx[k] - 1;
-    if (fixed[c]) b = b - val[k] * ubar[c];
+  int anyf = 0;
+  for (int64_t k = k0 + lane; k < k1; k += SUB) anyf |= fixed[col_idx[k] - 1];
+#pragma unroll
+  for (int off = SUB / 2; off > 0; off >>= 1) anyf |= __shfl_xor(anyf, off, SUB);
+  if (lane == 0) {
+    if (fr) {
+      rhs[i] = ubar[i];
+    } else if (anyf) {
+      double b = rhs[i];
+      for (int64_t k = k0; k < k1; ++k) {
+        const int64_t c = col_idx[k] - 1;
+        if (fixed[c]) b = b - val[k] * ubar[c];
+      }
+      rhs[i] = b;
+    }
   }
-  if (fr) b = ubar[i];
-  rhs[i] = b;
-  for (int64_t k = k0; k < k1; ++k) {
-    const int64_t c = col_idx[k] - 1;
-    if (fr || fixed[c]) val[k] = (fr && c == i) ? 1.0 : 0.0;
-  }
+  // the first lane's reads of val above complete before these stores (one wavefront, in order)
+  if (fr || anyf)
+    for (int64_t k = k0 + lane; k < k1; k += SUB) {
+      const int64_t c = col_idx[k] - 1;
+      if (fr || fixed[c]) val[k] = (fr && c == i) ? 1.0 : 0.0;
+    }
 }
 
 __global__ void k_set_fixed_x(int64_t n, const uint8_t *__restrict__ fixed,
@@ -80,9 +98,19 @@ int pf_bc_fix(pf_ctx *ctx, int64_t n, int64_t nids, const int32_t *ids, const do
 }
 
 int pf_bc_apply(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
-                double *rhs, const uint8_t *fixed, const double *ubar) {
+                double *rhs, const uint8_t *fixed, const double *ubar, double avg_row) {
   if (n <= 0) return PF_OK;
-  k_bc_rows<<<pf_blocks(n, 256), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs, fixed, ubar);
+  // about five entries per lane, as the CRS row product
+  int sub = 4;
+  while (sub < 64 && sub * 1.4142 * 5.0 < avg_row) sub *= 2;
+  const unsigned blocks = pf_blocks(n * sub, 256, INT32_MAX);
+  switch (sub) {
+    case 4: k_bc_rows<4><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs, fixed, ubar); break;
+    case 8: k_bc_rows<8><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs, fixed, ubar); break;
+    case 16: k_bc_rows<16><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs, fixed, ubar); break;
+    case 32: k_bc_rows<32><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs, fixed, ubar); break;
+    default: k_bc_rows<64><<<blocks, 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs, fixed, ubar); break;
+  }
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

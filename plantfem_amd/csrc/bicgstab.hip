@@ -1305,6 +1305,89 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   return PF_OK;
 }
 
+size_t pf_bicgstab_workspace_doubles(pf_ctx *ctx, int64_t n) {
+  const size_t G = (size_t)ctx->num_cus * 8;
+  const size_t scal = (sizeof(Scal) + sizeof(double) - 1) / sizeof(double);
+  return 7 * (size_t)n + PART_SLOTS * G + 16 + 8 + scal;
+}
+
+int pf_bicgstab_crs_workspace(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                              const double *val, int64_t nnz, const double *b, double *x, const pf_solve_opts &o,
+                              double *ws, size_t ws_doubles, SolveResult &res) {
+  res = SolveResult();
+  if (o.precision != 64 || o.preconditioning != 0 || o.kernel_timing != 0) {
+    pf_set_error("workspace solver: precision 64, no preconditioning and no kernel timing only");
+    return PF_INVALID;
+  }
+  if (o.stop_mode < 0 || o.stop_mode > 2) {
+    pf_set_error("stop_mode must be 0 (bicgstab_CRS_2), 1 (|r| < relative_er |b|) or 2 (bicgstab_CRS)");
+    return PF_INVALID;
+  }
+  if (n <= 0) return PF_OK;
+  const size_t G = (size_t)ctx->num_cus * 8;
+  if (pf_bicgstab_workspace_doubles(ctx, n) > ws_doubles) {
+    pf_set_error("workspace solver: %zu doubles of workspace needed, %zu available",
+                 pf_bicgstab_workspace_doubles(ctx, n), ws_doubles);
+    return PF_INVALID;
+  }
+  // workspace: r, r0, p, y, v, e, rref (n each), the partial rows, the scalars (64-byte aligned)
+  double *r = ws, *r0 = r + n, *p = r0 + n, *y = p + n, *v = y + n, *e = v + n, *rref = e + n;
+  double *part = rref + n;
+  double *sc_d = part + PART_SLOTS * G + 16;
+  sc_d += (8 - ((uintptr_t)sc_d / sizeof(double)) % 8) % 8;
+  Scal *sc = reinterpret_cast<Scal *>(sc_d);
+
+  const int sub = choose_sub((double)nnz / (double)n);
+  const Mat<double> m = make_mat<double>(n, row_ptr, col_idx, val, nullptr);
+  const Launch L{fused_grid(ctx, n * (int64_t)sub), fused_grid(ctx, n), sub, 0};
+  Scal h;
+  std::memset(&h, 0, sizeof(h));
+  h.er0 = o.er0;
+  h.rel = o.relative_er;
+  h.itrmax = o.itrmax;
+  h.stop_mode = o.stop_mode == 2 ? 3 : o.stop_mode;
+  h.guard = o.stop_mode == 1 ? 1 : 0;
+  PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  PF_HIP(hipMemcpyAsync(sc, &h, sizeof(Scal), hipMemcpyHostToDevice, ctx->stream));
+  PF_CHECK(enqueue_init<double>(ctx, L, m, b, x, r, r0, p, sc, part, false));
+  int chunk = o.check_every > 0 ? o.check_every : 32;
+  chunk += chunk & 1;  // even: the iteration counter slots alternate by parity
+  for (;;) {
+    PF_CHECK(enqueue_iters<double>(ctx, L, m, x, r, r0, p, y, v, sc, part, nullptr, chunk, nullptr, e));
+    PF_HIP(hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, ctx->stream));
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    if (o.debug) std::printf("BiCGSTAB >> itr %d  sqrt(rr) %.6e\n", h.itr_last, std::sqrt(h.rr));
+    if (h.done) break;
+  }
+  PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  PF_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  PF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  res.solve_ms = ms;
+  res.kind = 1;
+  res.status = h.status;
+  res.iterations = h.itr_last + (h.status == PF_BREAKDOWN_VV ? 1 : 0);
+  res.rr = h.rr;
+  res.rr1 = h.rr1;
+  if (res.status == PF_BREAKDOWN_VV)
+    pf_set_error("BiCGSTAB: v.v == 0 in iteration %d", res.iterations);
+  else if (res.status == PF_BREAKDOWN_C2C3)
+    pf_set_error("BiCGSTAB: c2*omega == 0 or a non-finite scalar after %d iterations", res.iterations);
+  else if (res.status == PF_NOT_CONVERGED)
+    pf_set_error("BiCGSTAB: not converged after %d iterations", res.iterations);
+  // true residual in FP64
+  double *pres = part + (size_t)PART_PI * G;
+  PF_CHECK(check_grid(ctx, L));
+  PF_DISPATCH_SUB(L.sub, (k_residual64<0, S_><<<L.grid, BT, 0, ctx->stream>>>(m, b, x, rref, pres)));
+  k_residual_finish<<<1, BT, 0, ctx->stream>>>(pres, (int)L.grid, sc);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  const double bn = std::sqrt(h.bnorm2);
+  res.true_relres = bn > 0 ? std::sqrt(h.rnorm2) / bn : std::sqrt(h.rnorm2);
+  return PF_OK;
+}
+
 int pf_work_ilu(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                 const double *val, int64_t nnz, bool matrix_changed, IluFactor **out) {
   note_matrix(w, n, val, matrix_changed);

@@ -162,6 +162,12 @@ extern "C" int pf_ctx_create(int device, pf_ctx **out) {
     delete c;
     return PF_ERR_HIP;
   }
+  // the device address of the allocation-free solver's scratch array, once per device
+  const int rc = pf_noalloc_setup(device);
+  if (rc != PF_OK) {
+    delete c;
+    return rc;
+  }
   *out = c;
   return PF_OK;
 }
@@ -661,7 +667,8 @@ static int apply_bc(pf_system *s) {
   if (!s->has_fix) return PF_OK;
   if (s->bc_mat == s->mat_version && s->bc_rhs == s->rhs_version && s->bc_fix == s->fix_version) return PF_OK;
   const bool matrix_part = s->bc_mat != s->mat_version || s->bc_fix != s->fix_version;
-  PF_CHECK(pf_bc_apply(s->ctx, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->rhs.p, s->fixed.p, s->ubar.p));
+  PF_CHECK(pf_bc_apply(s->ctx, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->rhs.p, s->fixed.p, s->ubar.p,
+                       s->n > 0 ? (double)s->nnz / (double)s->n : 1.0));
   if (matrix_part) mark_changed(s);
   s->bc_mat = s->mat_version;
   s->bc_rhs = s->rhs_version;

@@ -62,15 +62,38 @@ __global__ void k_sort_incident(int64_t nn, const int64_t *__restrict__ eoff,
 
 // Candidate j of node i is local node (j % nne) of the (j / nne)-th incident element.
 // Pass 0 counts first occurrences; pass 1 writes them (0-based node ids) at noff[i].
+// One wavefront per node: lane j holds candidate j (coalesced over the node's element list),
+// is a first occurrence when no lower lane holds the same node (64 shuffles), and the first
+// occurrences are compacted in lane order by a ballot -- the order of the serial scan.  A
+// node with more than 64 candidates is scanned serially by lane 0.
 template <int PASS>
-__global__ void k_node_neighbours(int64_t nn, int64_t ne, int nne, const int32_t *__restrict__ en,
-                                  const int64_t *__restrict__ eoff,
-                                  const int32_t *__restrict__ elist, int32_t *__restrict__ deg,
-                                  const int64_t *__restrict__ noff, int32_t *__restrict__ ncol) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nn) return;
+__global__ void __launch_bounds__(256) k_node_neighbours(int64_t nn, int64_t ne, int nne,
+                                                         const int32_t *__restrict__ en,
+                                                         const int64_t *__restrict__ eoff,
+                                                         const int32_t *__restrict__ elist,
+                                                         int32_t *__restrict__ deg, const int64_t *__restrict__ noff,
+                                                         int32_t *__restrict__ ncol) {
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (i >= nn) return;  // whole wavefronts
   const int64_t s0 = eoff[i];
   const int64_t m = (eoff[i + 1] - s0) * nne;
+  if (m <= 64) {
+    const int32_t cj = lane < m ? en[(int64_t)(lane % nne) * ne + elist[s0 + lane / nne]] : 0;
+    bool first = lane < m;
+    for (int k = 0; k < 63; ++k) {
+      const int32_t v = __shfl(cj, k, 64);
+      if (k < lane && v == cj) first = false;
+    }
+    const uint64_t mask = __ballot(first);
+    if (PASS == 0) {
+      if (lane == 0) deg[i] = (int32_t)__popcll(mask);
+    } else if (first) {
+      ncol[noff[i] + __popcll(mask & ((1ull << lane) - 1ull))] = cj - 1;
+    }
+    return;
+  }
+  if (lane != 0) return;
   int32_t count = 0;
   int32_t *out = nullptr;
   if (PASS == 1) out = ncol + noff[i];
@@ -192,7 +215,7 @@ int pf_graph_node_cols(pf_ctx *ctx, NodeGraph &g) {
   PF_CHECK(g.noff.ensure((size_t)g.nn + 1));
   PF_CHECK(mx.ensure(1));
   if (g.nn > 0)
-    k_node_neighbours<0><<<pf_blocks(g.nn, 128), 128, 0, ctx->stream>>>(
+    k_node_neighbours<0><<<pf_blocks(g.nn * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
         g.nn, g.ne, g.nne, g.elemnod.p, g.eoff.p, g.elist.p, g.deg.p, nullptr, nullptr);
   PF_CHECK(pf_scan_i32_to_i64(ctx, g.deg.p, g.nn, g.noff.p, tmp));
   int64_t nb = 0;
@@ -200,7 +223,7 @@ int pf_graph_node_cols(pf_ctx *ctx, NodeGraph &g) {
   PF_CHECK(g.ncol.ensure((size_t)nb + 1));
   PF_HIP(hipMemsetAsync(mx.p, 0, sizeof(int32_t), ctx->stream));
   if (g.nn > 0) {
-    k_node_neighbours<1><<<pf_blocks(g.nn, 128), 128, 0, ctx->stream>>>(
+    k_node_neighbours<1><<<pf_blocks(g.nn * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
         g.nn, g.ne, g.nne, g.elemnod.p, g.eoff.p, g.elist.p, g.deg.p, g.noff.p, g.ncol.p);
     k_max_deg<<<pf_blocks(g.nn, 256), 256, 0, ctx->stream>>>(g.nn, g.deg.p, mx.p);
   }

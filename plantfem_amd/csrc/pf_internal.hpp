@@ -87,6 +87,7 @@ struct NodeGraph {
   bool has_coords = false;   // nodcoord uploaded
   bool built = false;        // eoff/elist valid
   bool cols_built = false;   // deg/noff/ncol valid
+  mutable DevBuf<double> kebuf;  // element matrices of the two-pass stiffness assembly (ne x 576)
 };
 
 // blocked sliced-ELL copy of a DOF=3 CRS: 64 nodes per slice (one wavefront), slot t of
@@ -186,7 +187,7 @@ int pf_traction_boxes(pf_ctx *ctx, const NodeGraph &g, int32_t count, const int6
 
 // bc.hip
 int pf_bc_apply(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
-                double *rhs, const uint8_t *fixed, const double *ubar);
+                double *rhs, const uint8_t *fixed, const double *ubar, double avg_row);
 int pf_bc_fix(pf_ctx *ctx, int64_t n, int64_t nids, const int32_t *ids, const double *values,
               double fixvalue, uint8_t *fixed, double *ubar);
 int pf_bc_set_fixed_x(pf_ctx *ctx, int64_t n, const uint8_t *fixed, const double *ubar, double *x);
@@ -266,6 +267,17 @@ void pf_solver_work_destroy(SolverWork *w);
 int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr,
                        const int32_t *col_idx, const double *val, int64_t nnz, const double *b,
                        double *x, const pf_solve_opts &o, bool matrix_changed, SolveResult &res);
+// bicgstab_CRS_2 (or the stop rules of stop_mode 1 and 2) in FP64 on the CRS layout with every
+// work vector, partial sum and scalar in the caller's device workspace ws (ws_doubles doubles,
+// at least pf_bicgstab_workspace_doubles(ctx, n)): no device allocation, no hipGraph; host polls
+// go to a stack copy.  Every vector the kernels read is written first within the call.
+size_t pf_bicgstab_workspace_doubles(pf_ctx *ctx, int64_t n);
+// noalloc.hip: look up the device address of the allocation-free solver's scratch array on
+// `device` (current), once per device
+int pf_noalloc_setup(int device);
+int pf_bicgstab_crs_workspace(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                              const double *val, int64_t nnz, const double *b, double *x, const pf_solve_opts &o,
+                              double *ws, size_t ws_doubles, SolveResult &res);
 // y = A x on device vectors with layout kind (1 CRS FP64, 2 blocked FP64, 3 blocked FP32);
 // `repeat` timed launches, *ms = average device ms per launch
 int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,

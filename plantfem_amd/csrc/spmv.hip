@@ -38,10 +38,14 @@ __global__ void __launch_bounds__(256) k_spmv_sell3(int64_t nn, int64_t nslices,
 }
 
 // rows 3i..3i+2 share one node-column list and each node column c appears as the aligned
-// triple 3c+1, 3c+2, 3c+3 (1-based) -- the layout ZeroMatrix(DOF=3) produces
-__global__ void k_block_check(int64_t nn, const int64_t *__restrict__ row_ptr,
-                              const int32_t *__restrict__ col_idx, int32_t *__restrict__ bad) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// triple 3c+1, 3c+2, 3c+3 (1-based) -- the layout ZeroMatrix(DOF=3) produces.  One SUB-lane
+// subgroup per node, lanes over the node's column triples.
+template <int SUB>
+__global__ void __launch_bounds__(256) k_block_check(int64_t nn, const int64_t *__restrict__ row_ptr,
+                                                     const int32_t *__restrict__ col_idx, int32_t *__restrict__ bad) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = g / SUB;
+  const int lane = (int)(g % SUB);
   if (i >= nn) return;
   const int64_t r0 = row_ptr[3 * i] - 1;
   const int64_t L = row_ptr[3 * i + 1] - 1 - r0;
@@ -49,12 +53,12 @@ __global__ void k_block_check(int64_t nn, const int64_t *__restrict__ row_ptr,
   for (int d = 1; d < 3 && ok; ++d)
     ok = (row_ptr[3 * i + d + 1] - row_ptr[3 * i + d]) == L;
   const int64_t m = L / 3;
-  for (int d = 0; d < 3 && ok; ++d) {
-    const int64_t rb = row_ptr[3 * i + d] - 1;
-    for (int64_t t = 0; t < m && ok; ++t) {
-      const int32_t c0 = col_idx[rb + 3 * t];
-      ok = ((c0 - 1) % 3 == 0) && col_idx[rb + 3 * t + 1] == c0 + 1 && col_idx[rb + 3 * t + 2] == c0 + 2 &&
-           (d == 0 || c0 == col_idx[r0 + 3 * t]);
+  for (int64_t t = lane; t < m && ok; t += SUB) {
+    const int32_t c = col_idx[r0 + 3 * t];
+    ok = (c - 1) % 3 == 0;
+    for (int d = 0; d < 3 && ok; ++d) {
+      const int64_t rb = row_ptr[3 * i + d] - 1 + 3 * t;
+      ok = col_idx[rb] == c && col_idx[rb + 1] == c + 1 && col_idx[rb + 2] == c + 2;
     }
   }
   if (!ok) atomicOr(bad, 1);
@@ -226,7 +230,7 @@ int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, con
     return PF_OK;
   }
   PF_HIP(hipMemsetAsync(ok_dev, 0, sizeof(int32_t), ctx->stream));
-  k_block_check<<<pf_blocks(nn, 256), 256, 0, ctx->stream>>>(nn, row_ptr, col_idx, ok_dev);
+  k_block_check<16><<<pf_blocks(nn * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(nn, row_ptr, col_idx, ok_dev);
   int32_t bad = 1;
   PF_HIP(hipMemcpyAsync(&bad, ok_dev, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   PF_HIP(hipStreamSynchronize(ctx->stream));

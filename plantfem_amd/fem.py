@@ -148,6 +148,8 @@ def circle2d(x_num: int = 10, y_num: int = 10, x_len: float | None = None, y_len
     xn = x_num // 2 + 1
     yn = y_num // 2 + 1
     xn = yn = max(xn, yn)
+    if xn < 2:  # the ring layers would index past the nodes that exist
+        raise ValueError(f"Circle2D / Cylinder: x_num or y_num must be >= 2 (got {x_num}, {y_num})")
     nod, el = _rect2d(2 * xn, 2 * yn, 2.0, 2.0)
     nod = nod - 1.0
     x, y = nod[:, 0].copy(), nod[:, 1].copy()
@@ -376,8 +378,13 @@ class FEMDomain:
         args = (x_num or 10, y_num or 10, z_num or 10, x_len or 1.0, y_len or 1.0, z_len or 1.0)
         if meshtype in ("Cube3D", "Cube", "cube3D", "rectangular3D"):
             self.nodcoord, self.elemnod = cube3d(*args)
-        elif meshtype in ("Cylinder", "Cylinder3D"):
+        elif meshtype == "Cylinder":
             self.nodcoord, self.elemnod = cylinder3d(*args)
+        elif meshtype == "Cylinder3D":
+            # FEMDomain_%create sends "Cylinder3D" to to_cylinder (FEMDomainClass.f90:7279-7282,
+            # :16136-16157), a different mesh from Mesh%create("Cylinder"); not built here
+            raise ValueError("meshtype 'Cylinder3D' (to_cylinder) is not on this path; 'Cylinder' is StemClass's "
+                             "Mesh%create route")
         elif meshtype == "Leaf3D":
             self.nodcoord, self.elemnod = leaf3d(*args, shaperatio=shaperatio, species=species,
                                                  SoyWidthRatio=SoyWidthRatio)

@@ -453,8 +453,12 @@ contains
       case ("Leaf3D")
          call cube3d_mesh(this%mesh, xn, yn, zn, lx, ly, lz)
          call leaf3d_shape(this%mesh, shaperatio, species, SoyWidthRatio)
-      case ("Cylinder", "Cylinder3D")
+      case ("Cylinder")
          call cylinder3d_mesh(this%mesh, xn, yn, zn, lx, ly, lz)
+      case ("Cylinder3D")
+         ! FEMDomain_%create sends "Cylinder3D" to to_cylinder (FEMDomainClass.f90:7279-7282,
+         ! :16136-16157), a different mesh from Mesh%create("Cylinder"); not built here
+         stop "plantfem_gpu: meshtype Cylinder3D (to_cylinder) is not on this path; use Cylinder"
       case default
          stop "plantfem_gpu: meshtype is Cube3D, Cube, Cylinder or Leaf3D on this path"
       end select
@@ -584,6 +588,8 @@ contains
       real(real64), parameter :: pi = 3.14159265358979323846d0
       integer(int32) :: xn, m, sq, nsq, nring, i, j, k, n, ini, nn2, ne2, e, a
       xn = max(x_num/2 + 1, y_num/2 + 1)
+      ! the ring layers index nodes up to nsq + 8*xn*(xn+2), which exist only from xn = 2 on
+      if (xn < 2) stop "plantfem_gpu: Cylinder needs x_num or y_num >= 2"
       m = 2*xn
       nsq = (m + 1)*(m + 1)
       nring = m*4*m

@@ -14,5 +14,9 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
+    try:  # tests hand torch device arrays to the library: torch's HIP runtime starts first
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     from plantfem_amd import _lib as L
     return L.default_context()

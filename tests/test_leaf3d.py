@@ -107,3 +107,30 @@ def test_cylinder_stem_mesh():
     assert min_detj(nod, el) > 0
     d = FEMDomain().create("Cylinder", x_num=10, y_num=10, z_num=6, x_len=0.08, y_len=0.08, z_len=0.6)
     assert np.array_equal(d.nodcoord, nod) and np.array_equal(d.elemnod, el)
+
+
+def test_cylinder_refuses_what_it_cannot_build():
+    """Circle2D with x_num, y_num <= 1 would index ring nodes that do not exist; and "Cylinder3D"
+    is FEMDomain_%create's to_cylinder route (FEMDomainClass.f90:7279-7282), not this mesh."""
+    import pytest
+    from plantfem_amd.fem import FEMDomain, cylinder3d
+    for xy in ((1, 1), (0, 1), (1, 0)):
+        with pytest.raises(ValueError, match="x_num or y_num"):
+            cylinder3d(xy[0], xy[1], 2)
+    assert cylinder3d(2, 1, 2)[1].shape[0] > 0
+    with pytest.raises(ValueError, match="Cylinder3D"):
+        FEMDomain().create("Cylinder3D", x_num=4, y_num=4, z_num=2)
+
+
+def test_fortran_cylinder_refuses_small_counts(tmp_path):
+    """The Fortran FEMDomain_%create("Cylinder") stops with a message at x_num, y_num <= 1
+    instead of writing past its ring arrays (host only)."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent.parent / "plantfem_amd" / "lib" / "bin" / "organ_meshes"
+    out = tmp_path / "m.bin"
+    res = subprocess.run([str(exe), str(out), "1", "1"], capture_output=True, text=True, timeout=60)
+    assert "Cylinder needs x_num or y_num >= 2" in res.stdout + res.stderr
+    assert not out.exists()
+    ok = subprocess.run([str(exe), str(out), "2", "1"], capture_output=True, text=True, timeout=60)
+    assert ok.returncode == 0 and out.exists()
