@@ -7,10 +7,14 @@ whole device path with the mesh and materials already in HBM: CRS graph build, e
 stiffness assembly, traction RHS, Dirichlet elimination and BiCGSTAB to a relative
 residual ||b - Ax|| / ||b|| of 1e-8 (FP64).  value = seconds per step (time to solve).
 
-The same line reports the mixed-precision solve on the same matrix, the SpMV kernels'
-bandwidth, the roofline of the dominant kernel (y = A p fused with r0.y, timed live with
-hipEvents around every launch) and the CPU baseline (the oracle's BiCGSTAB iterations on
-the host cores, extrapolated to the GPU iteration count).
+The same line reports: the iteration count under the reference's own stop rule; the
+mixed-precision steps (precision 32: FP32 matrix and vectors inside; precision 48: FP32 matrix,
+FP64 vectors) and, on this system and on the config 3 system (sigma 2.0 clipped to a 1e4
+contrast), every form against a certified FP64 reference at equal scaling (plain FP64, FP64 on
+D^-1 A, both mixed forms swept to the first stop target within an error of 1e-8); the SpMV
+kernels' bandwidth; the roofline of the dominant kernel (y = A p fused with r0.y, timed live
+with hipEvents around every launch); and the CPU baseline (the oracle on the host cores, per
+phase, with the residual certification of the references).
 
     python bench.py [--steps K] [--warmup W] [--size 100] [--no-cpu]
 
@@ -157,14 +161,14 @@ def main(argv=None):
         log("profile: 3 x 512 MiB device copies done")
         return
 
-    def step(precision=64, timing=0, target=1e-8, stop_mode=1):
+    def step(precision=64, timing=0, target=1e-8, stop_mode=1, **kw):
         s.setCRS(DOF=3)                       # ZeroMatrix graph build (device)
         s.setStiffness()                      # element stiffness assembly (device)
         s.addTraction(w.traction_box, w.pressure)
         s.fix(1, w.fixed_dofs, FixValue=0.0)
         s.relative_er = target
         s.solve(precision=precision, stop_mode=stop_mode, download=False, kernel_timing=timing,
-                target_relres=target, raise_on_breakdown=False, use_graph=0 if a.no_graph else 1)
+                target_relres=target, raise_on_breakdown=False, use_graph=0 if a.no_graph else 1, **kw)
         return dict(s.last_stats)
 
     # ---- FP64 time to solve (timed steps carry per-kernel hipEvent timing) -------------
@@ -196,26 +200,28 @@ def main(argv=None):
     log(f"reference criterion: {st_rc['iterations']} it, relres {st_rc['true_relres']:.2e}")
 
     # ---- mixed precision on the same matrix --------------------------------------------
-    # FP32 BiCGSTAB on D^-1 A, restarted at every FP64 true-residual replacement; the step
-    # is the same as the FP64 one (graph build, assembly, BCs, solve to relres 1e-8)
+    # precision 32 (the FP32 inner BiCGSTAB on D^-1 A, FP32 matrix and vectors, restarted at
+    # every FP64 true-residual replacement) and precision 48 (the same with FP64 vectors over
+    # the FP32 matrix); each step is the FP64 step's graph build, assembly, BCs and solve
     x64 = s.getX()
-    mixed_times, st32 = [], None
-    for i in range(max(1, min(a.steps, a.mixed_repeats))):
-        ctx.synchronize()
-        t = time.perf_counter()
-        st32 = step(32, 0)
-        ctx.synchronize()
-        mixed_times.append(time.perf_counter() - t)
-        log(f"mixed {i}: {mixed_times[-1]:.3f} s, status {st32['status']}, outer {st32['outer_iterations']}, "
-            f"inner {st32['iterations']}, relres {st32['true_relres']:.2e}")
-    t_mixed = float(np.median(mixed_times))
-    x32 = s.getX()
-    ctx.synchronize()
-    t = time.perf_counter()
-    st32t = step(32, 0, target=1e-10)
-    ctx.synchronize()
-    t_mixed_tight = time.perf_counter() - t
-    x32t = s.getX()
+
+    def mixed_steps(prec, target, reps, **kw):
+        times, st = [], None
+        for i in range(reps):
+            ctx.synchronize()
+            t = time.perf_counter()
+            st = step(prec, 0, target=target, **kw)
+            ctx.synchronize()
+            times.append(time.perf_counter() - t)
+            log(f"precision {prec} to {target:g} ({i}): {times[-1]:.3f} s, status {st['status']}, outer "
+                f"{st['outer_iterations']}, inner {st['iterations']}, relres {st['true_relres']:.2e}")
+        return float(np.median(times)), st, s.getX()
+
+    reps = max(1, min(a.steps, a.mixed_repeats))
+    t_mixed, st32, x32 = mixed_steps(32, 1e-8, reps)
+    t_mixed_tight, st32t, x32t = mixed_steps(32, 1e-10, 1)
+    t48, st48, x48 = mixed_steps(48, 1e-8, reps, **P48)
+    t48t, st48t, x48t = mixed_steps(48, 1e-10, 1, **P48)
     # reference: FP64 BiCGSTAB on the PointJacobi-scaled system to a scaled relres of 1e-13; its
     # residual on the unscaled system is certified on the CPU (oracle elimination + product)
     s.setCRS(DOF=3)
@@ -235,11 +241,17 @@ def main(argv=None):
 
     mixed_err = rel(x32)
     log(f"mixed: {t_mixed:.3f} s to 1e-8 (err {mixed_err:.2e}), {t_mixed_tight:.3f} s to 1e-10 "
-        f"(err {rel(x32t):.2e}); FP64 err {rel(x64):.2e}; reference: {st_ref['iterations']} PointJacobi "
+        f"(err {rel(x32t):.2e}); precision 48 {t48:.3f} s (err {rel(x48):.2e}), {t48t:.3f} s to 1e-10 "
+        f"(err {rel(x48t):.2e}); FP64 err {rel(x64):.2e}; reference: {st_ref['iterations']} PointJacobi "
         f"iterations, scaled relres {st_ref['true_relres']:.2e}")
+    # solve-only comparison on this system at equal scaling, every form swept to the error bar
+    comparison2 = compare_forms(ctx, s, lambda: refresh(s, w), x_ref, plain=(t_step, st64, x64))
 
     # ---- SpMV kernels on the eliminated matrix ----------------------------------------
-    step(64, 0)  # the unscaled eliminated system again
+    refresh(s, w)  # the unscaled system, eliminated by a one-iteration solve
+    s.itrmax = 1
+    s.solve(stop_mode=1, download=False, raise_on_breakdown=False)
+    s.itrmax = 40000
     spmv = {}
     xr = np.random.default_rng(1).standard_normal(n)
     for kind, name in ((1, "crs_f64"), (2, "blocked_f64"), (3, "blocked_f32")):
@@ -304,122 +316,140 @@ def main(argv=None):
                                       "true_relres": st32t["true_relres"], "rel_err_vs_fp64_reference": rel(x32t),
                                       "speedup_vs_fp64": round(t_step / t_mixed_tight, 3)
                                       if st32t["status"] == 0 else None},
+                  "precision48": {"scheme": "the same refinement with FP64 vectors over the FP32 matrix copy "
+                                            f"(inner tolerance {P48['inner_tol']}, at most {P48['inner_max']} inner "
+                                            "iterations)",
+                                  "status": st48["status"], "time_to_solve_s": round(t48, 4),
+                                  "inner_iterations": st48["iterations"], "outer": st48["outer_iterations"],
+                                  "true_relres": st48["true_relres"], "rel_err_vs_fp64_reference": rel(x48),
+                                  "speedup_vs_fp64": round(t_step / t48, 3) if st48["status"] == 0 else None,
+                                  "to_relres_1e-10": {"status": st48t["status"], "time_to_solve_s": round(t48t, 4),
+                                                      "true_relres": st48t["true_relres"],
+                                                      "rel_err_vs_fp64_reference": rel(x48t),
+                                                      "speedup_vs_fp64": round(t_step / t48t, 3)
+                                                      if st48t["status"] == 0 else None}},
                   "fp64_rel_err_vs_reference": rel(x64),
                   "reference": f"FP64 BiCGSTAB on the PointJacobi-scaled system, {st_ref['iterations']} iterations "
                                f"to scaled relres {st_ref['true_relres']:.2e}; unscaled residual certified in "
-                               f"cpu_baseline.certified"},
+                               f"cpu_baseline.certified",
+                  "comparison": comparison2},
         "config3": c3,
         "spmv": spmv, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
 
-def bench_c3(a, ctx):
-    """Config 3: the 100^3 box with E = 1e7 lognormal(2.0) clipped to a 1e4 contrast.  Plain FP64,
-    FP64 on the PointJacobi-scaled system D^-1 A (the scaling the mixed form uses) swept over
-    scaled stop targets, and the mixed form swept over FP64 relres targets, each to the first
-    target whose error against a tight FP64 reference is <= 1e-8."""
-    from plantfem_amd import workloads as W
-    from plantfem_amd.fem import FEMDomain, FEMSolver
-    w3 = W.config(3, seed=a.seed)
-    log(f"config 3: E contrast {w3.E.max() / w3.E.min():.3g}")
+# precision 48's refinement parameters (replacement after a 100x drop of the recursive
+# residual, or 1000 inner iterations), chosen on configs 2 and 3 by a sweep on MI355X
+P48 = {"inner_tol": 1e-2, "inner_max": 1000}
+TARGETS = (1e-8, 1e-9, 1e-10, 1e-11, 1e-12)
+ERROR_BAR = 1e-8
 
-    def build(s=None):
-        if s is None:
-            s = FEMSolver(ctx)
-            s.setDomain(FEMDomain(w3.nodcoord, w3.elemnod, ctx))
-            s.setCRS(DOF=3)
-            s.setStiffness(w3.E, w3.nu)
-        else:
-            s.setCRS(DOF=3)
-            s.setStiffness()
-        s.addTraction(w3.traction_box, w3.pressure)
-        s.fix(1, w3.fixed_dofs, FixValue=0.0)
-        return s
 
-    def timed(s, **kw):
+def refresh(s, w):
+    """graph, assembly, traction and fixes again (the PointJacobi scaling acts in place)"""
+    s.setCRS(DOF=3)
+    s.setStiffness()
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+
+
+def compare_forms(ctx, s, rebuild, x_ref, plain=None):
+    """Solve-only times of one system: plain FP64 to relres 1e-8, FP64 on the PointJacobi-scaled
+    system D^-1 A (the scaling both mixed forms use), precision 32 and precision 48, the last
+    three swept over their stop targets up to the first whose error against x_ref is within
+    ERROR_BAR.  `rebuild` restores the unscaled system.  plain = (seconds, stats, x) to reuse."""
+
+    def timed(**kw):
         ctx.synchronize()
         t = time.perf_counter()
         s.solve(download=False, raise_on_breakdown=False, stop_mode=1, **kw)
         ctx.synchronize()
         return time.perf_counter() - t, dict(s.last_stats)
 
-    sa = build()                     # unscaled system (plain FP64, mixed, residual checks)
-    crs3_pre, rhs3_pre = (sa.getCRS(), sa.getRHS()) if not a.no_cpu else (None, None)
-    sp = build()                     # PointJacobi-scaled solves (the scaling is in place)
-    sp.itrmax = 100000
-    sp.relative_er = 1e-13
-    t, st = timed(sp, preconditioning="PointJacobi")
-    x_ref = sp.getX()
-    ref = {"iterations": st["iterations"], "scaled_relres": st["true_relres"], "status": st["status"]}
-    log(f"c3 reference: {json.dumps(ref)}")
-
-    def unscaled_relres(x):
-        y, _ = sa.spmv(2, x)
-        b = sa.getRHS()
-        return float(np.linalg.norm(b - y) / np.linalg.norm(b))
-
     def err(x):
         return float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
 
-    out = {"workload": "hex box 100^3, E=1e7*lognormal(2.0) clipped to a 1e4 contrast, nu~U[0.25,0.35], clamped "
-                       "bottom, 1e4 Pa top traction", "e_contrast": float(w3.E.max() / w3.E.min()),
-           "error_bar": 1e-8, "reference": ref}
-    # plain FP64 (bicgstab_CRS_2 recurrence, unscaled) to relres 1e-8
-    sa.itrmax = 1
-    sa.relative_er = 1e-8
-    sa.solve(stop_mode=1, download=False, raise_on_breakdown=False)  # elimination + layouts
-    sa.itrmax = 100000
-    tp, st = timed(sa)
-    xp = sa.getX()
+    out = {"error_bar": ERROR_BAR, "times": "solve() wall seconds: elimination, layouts, scaling and the solve"}
+    rebuild()
+    s.itrmax = 100000
+    if plain is None:
+        s.relative_er = 1e-8
+        tp, st = timed()
+        x = s.getX()
+    else:
+        tp, st, x = plain
     out["fp64_plain"] = {"relres_target": 1e-8, "s": round(tp, 4), "iterations": st["iterations"],
-                         "status": st["status"], "true_relres": st["true_relres"], "rel_err": err(xp)}
-    log(f"c3 fp64 plain: {json.dumps(out['fp64_plain'])}")
-    # FP64 on D^-1 A: the same scaling as the mixed form, swept
+                         "status": st["status"], "true_relres": st["true_relres"], "rel_err": err(x),
+                         "note": "whole step (graph, assembly, BCs, solve)" if plain is not None else None}
+    log(f"fp64 plain: {json.dumps(out['fp64_plain'])}")
     legs = []
-    for tgt in (1e-8, 1e-9, 1e-10, 1e-11, 1e-12):
-        build(sp)
-        sp.relative_er = tgt
-        tj, st = timed(sp, preconditioning="PointJacobi")
-        x = sp.getX()
-        leg = {"scaled_relres_target": tgt, "s": round(tj, 4), "iterations": st["iterations"],
-               "status": st["status"], "true_relres": unscaled_relres(x), "rel_err": err(x)}
+    for tgt in TARGETS:
+        rebuild()
+        s.relative_er = tgt
+        t, st = timed(preconditioning="PointJacobi")
+        leg = {"scaled_relres_target": tgt, "s": round(t, 4), "iterations": st["iterations"], "status": st["status"],
+               "rel_err": err(s.getX())}
         legs.append(leg)
-        log(f"c3 fp64 PointJacobi: {json.dumps(leg)}")
-        if leg["rel_err"] <= 1e-8 and leg["true_relres"] <= 1e-8:
+        log(f"fp64 PointJacobi: {json.dumps(leg)}")
+        if leg["status"] == 0 and leg["rel_err"] <= ERROR_BAR:
             break
     out["fp64_pointjacobi"] = legs
-    # mixed: FP32 inner BiCGSTAB on D^-1 A, FP64 refinement to the relres target
-    mlegs = []
-    for tgt in (1e-8, 1e-9, 1e-10, 1e-11, 1e-12):
-        times, st = [], None
-        for _ in range(2):
-            tm, st = timed(sa, precision=32, target_relres=tgt)
-            times.append(tm)
-        x = sa.getX()
-        leg = {"relres_target": tgt, "s": round(min(times), 4), "status": st["status"],
-               "inner_iterations": st["iterations"], "outer": st["outer_iterations"],
-               "true_relres": st["true_relres"], "rel_err": err(x)}
-        mlegs.append(leg)
-        log(f"c3 mixed: {json.dumps(leg)}")
-        if leg["status"] == 0 and leg["rel_err"] <= 1e-8:
-            break
-    out["mixed"] = mlegs
-    ok_m = [m for m in mlegs if m["status"] == 0 and m["rel_err"] <= 1e-8]
-    ok_j = [j for j in legs if j["status"] == 0 and j["rel_err"] <= 1e-8 and j["true_relres"] <= 1e-8]
-    out["error_bar_met"] = bool(ok_m)
-    if ok_m:
-        m = ok_m[0]
-        out["mixed_at_error_bar"] = {"relres_target": m["relres_target"], "s": m["s"]}
-        if ok_j:
-            out["fp32_speedup_at_equal_scaling"] = round(ok_j[0]["s"] / m["s"], 3)
-        if out["fp64_plain"]["status"] == 0:
-            out["speedup_vs_plain_fp64_at_1e-8"] = round(out["fp64_plain"]["s"] / m["s"], 3)
-    m8 = mlegs[0]
-    if m8["status"] == 0 and legs[0]["status"] == 0:
-        out["fp32_speedup_at_relres_1e-8_equal_scaling"] = round(legs[0]["s"] / m8["s"], 3)
-    sa.close()
-    sp.close()
+    rebuild()
+    s.relative_er = 1e-8
+    for prec, name, kw in ((32, "precision32", {}), (48, "precision48", P48)):
+        mlegs = []
+        for tgt in TARGETS:
+            t, st = timed(precision=prec, target_relres=tgt, **kw)
+            leg = {"relres_target": tgt, "s": round(t, 4), "status": st["status"], "inner_iterations": st["iterations"],
+                   "outer": st["outer_iterations"], "true_relres": st["true_relres"], "rel_err": err(s.getX())}
+            mlegs.append(leg)
+            log(f"{name}: {json.dumps(leg)}")
+            if leg["status"] == 0 and leg["rel_err"] <= ERROR_BAR:
+                break
+        out[name] = mlegs
+    ok_j = [j for j in legs if j["status"] == 0 and j["rel_err"] <= ERROR_BAR]
+    summary = {}
+    for name in ("precision32", "precision48"):
+        ok = [m for m in out[name] if m["status"] == 0 and m["rel_err"] <= ERROR_BAR]
+        if ok:
+            summary[name] = {"relres_target": ok[0]["relres_target"], "s": ok[0]["s"], "rel_err": ok[0]["rel_err"],
+                             "speedup_vs_fp64_same_scaling": round(ok_j[0]["s"] / ok[0]["s"], 3) if ok_j else None,
+                             "speedup_vs_fp64_plain_at_1e-8": round(out["fp64_plain"]["s"] / ok[0]["s"], 3)}
+        else:
+            summary[name] = None
+    if ok_j:
+        summary["fp64_pointjacobi"] = {"scaled_relres_target": ok_j[0]["scaled_relres_target"], "s": ok_j[0]["s"],
+                                       "rel_err": ok_j[0]["rel_err"]}
+    out["at_error_bar"] = summary
+    return out
+
+
+def bench_c3(a, ctx):
+    """Config 3: the 100^3 box with E = 1e7 lognormal(2.0) clipped to a 1e4 contrast, all forms
+    against a tight FP64 reference (FP64 on the PointJacobi-scaled system to 1e-13)."""
+    from plantfem_amd import workloads as W
+    from plantfem_amd.fem import FEMDomain, FEMSolver
+    w3 = W.config(3, seed=a.seed)
+    log(f"config 3: E contrast {w3.E.max() / w3.E.min():.3g}")
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w3.nodcoord, w3.elemnod, ctx))
+    s.setCRS(DOF=3)
+    s.setStiffness(w3.E, w3.nu)
+    s.addTraction(w3.traction_box, w3.pressure)
+    s.fix(1, w3.fixed_dofs, FixValue=0.0)
+    crs3_pre, rhs3_pre = (s.getCRS(), s.getRHS()) if not a.no_cpu else (None, None)
+    s.itrmax = 100000
+    s.relative_er = 1e-13
+    s.solve(preconditioning="PointJacobi", stop_mode=1, download=False, raise_on_breakdown=False)
+    st = s.last_stats
+    x_ref = s.getX()
+    ref = {"iterations": st["iterations"], "scaled_relres": st["true_relres"], "status": st["status"]}
+    log(f"c3 reference: {json.dumps(ref)}")
+    out = {"workload": "hex box 100^3, E=1e7*lognormal(2.0) clipped to a 1e4 contrast, nu~U[0.25,0.35], clamped "
+                       "bottom, 1e4 Pa top traction", "e_contrast": float(w3.E.max() / w3.E.min()), "reference": ref}
+    out.update(compare_forms(ctx, s, lambda: refresh(s, w3), x_ref))
+    s.close()
     out["_pre"] = (crs3_pre, rhs3_pre, w3, x_ref)
     return out
 

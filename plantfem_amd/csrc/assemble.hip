@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) k_ke_elem(int64_t nn, int64_t ne, const d
 // sums Ke_e(3a+d, 3b+d') over the row node's incident elements e in ascending id, local nodes
 // a (= i) and b (= j) ascending -- the order of the reference's element loop -- starting from
 // 0.0, and is written once.
-template <int SUB>
+template <int SUB, int EPL>
 __global__ void __launch_bounds__(256) k_gather_rows(int64_t nn, int64_t ne, const int32_t *__restrict__ en,
                                                      const int64_t *__restrict__ eoff,
                                                      const int32_t *__restrict__ elist,
@@ -364,30 +364,43 @@ __global__ void __launch_bounds__(256) k_gather_rows(int64_t nn, int64_t ne, con
   const int64_t base = row_ptr[r] - 1;
   const int64_t len = row_ptr[r + 1] - 1 - base;
   const int64_t s0 = eoff[i], s1 = eoff[i + 1];
-  for (int64_t q0 = 0; q0 < len; q0 += SUB) {
-    const int64_t q = q0 + lane;
-    const bool on = q < len;
-    // the row is the node's neighbour list expanded by DOF: entry q is DOF q % 3 of neighbour q / 3
-    const int32_t j1 = on ? ncol[noff[i] + q / 3] + 1 : 0;
-    const int dc = (int)(q % 3);
-    double v = 0.0;
+  // up to SUB * EPL entries per pass (one pass for the <= 81 entries of a hex8 row): lane
+  // entry t is q = q0 + t * SUB + lane, accumulated in registers over the element loop
+  for (int64_t q0 = 0; q0 < len; q0 += SUB * EPL) {
+    int32_t j1[EPL];
+    int dc[EPL];
+    double v[EPL];
+#pragma unroll
+    for (int t = 0; t < EPL; ++t) {
+      const int64_t q = q0 + t * SUB + lane;
+      // the row is the node's neighbour list expanded by DOF: entry q is DOF q % 3 of
+      // neighbour q / 3 (node ids are >= 1, so 0 marks an entry past the row)
+      j1[t] = q < len ? ncol[noff[i] + q / 3] + 1 : 0;
+      dc[t] = (int)(q % 3);
+      v[t] = 0.0;
+    }
     for (int64_t s = s0; s < s1; ++s) {
       const int64_t e = elist[s];
       if (s > s0 && elist[s - 1] == e) continue;  // node listed twice in one element
       int32_t nodes[8];
 #pragma unroll
       for (int b = 0; b < 8; ++b) nodes[b] = en[(int64_t)b * ne + e];
-      const double *ke_e = ke + e * 576;
 #pragma unroll 1
       for (int a = 0; a < 8; ++a) {
         if (nodes[a] != (int32_t)(i + 1)) continue;
-        const double *kr = ke_e + (3 * a + d) * 24;
+        const double *kr = ke + e * 576 + (3 * a + d) * 24;
 #pragma unroll
-        for (int b = 0; b < 8; ++b)
-          if (nodes[b] == j1) v = v + kr[3 * b + dc];
+        for (int t = 0; t < EPL; ++t)
+#pragma unroll
+          for (int b = 0; b < 8; ++b)
+            if (nodes[b] == j1[t]) v[t] = v[t] + kr[3 * b + dc[t]];
       }
     }
-    if (on) val[base + q] = v;
+#pragma unroll
+    for (int t = 0; t < EPL; ++t) {
+      const int64_t q = q0 + t * SUB + lane;
+      if (q < len) val[base + q] = v[t];
+    }
   }
 }
 
@@ -870,7 +883,7 @@ int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const dou
   if (g.ne > 0 && g.kebuf.ensure((size_t)g.ne * 576) == PF_OK) {
     k_ke_elem<false><<<pf_blocks(g.ne * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
         g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, 0, g.ne, g.kebuf.p);
-    k_gather_rows<16><<<pf_blocks(3 * g.nn * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+    k_gather_rows<16, 6><<<pf_blocks(3 * g.nn * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(
         g.nn, g.ne, g.elemnod.p, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p, row_ptr, g.kebuf.p, val);
   } else {
     k_assemble_rows<<<pf_blocks(3 * g.nn, 128), 128, 0, ctx->stream>>>(

@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "spmv_dev.hpp"
@@ -59,31 +60,33 @@ struct Scal {
   int32_t restart;     // reliable updates: also restart the recurrence (r0 = p = r) at a replacement
 };
 
-template <typename V>
+// V: vector and accumulation type of the products; M: stored matrix values (M = V, or FP32
+// values with FP64 vectors for precision 48)
+template <typename V, typename M = V>
 struct Mat {
   int64_t n;
   const int64_t *row_ptr;
   const int32_t *col_idx;
-  const V *val;
+  const M *val;
   int64_t nn, nslices;
   const int64_t *soff;
   const int32_t *width;
   const int32_t *col;
-  const V *sval;
+  const M *sval;
 };
 
 // Visit every row of y = A*xf with the calling thread's share of the grid; op(row, sum) is
 // called by exactly one thread per row.  L = 1: Sell3 (3 rows per lane), L = 0: CRS rows
 // reduced by SUB-lane subgroups.
-template <typename V, int L, int SUB, typename XF, typename OP>
-__device__ inline void for_rows(const Mat<V> &m, XF xf, OP op) {
+template <typename V, int L, int SUB, typename M, typename XF, typename OP>
+__device__ inline void for_rows(const Mat<V, M> &m, XF xf, OP op) {
   if (L == 1) {
     const int64_t wave = (blockIdx.x * (int64_t)BT + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = ((int64_t)gridDim.x * BT) >> 6;
     for (int64_t s = wave; s < m.nslices; s += nwaves) {
       V y0, y1, y2;
-      pfdev::sell3_node<V, V>(m.soff, m.width, m.col, m.sval, s, lane, xf, y0, y1, y2);
+      pfdev::sell3_node<M, V>(m.soff, m.width, m.col, m.sval, s, lane, xf, y0, y1, y2);
       const int64_t i = s * 64 + lane;
       if (i < m.nn) {
         op(3 * i, y0);
@@ -96,7 +99,7 @@ __device__ inline void for_rows(const Mat<V> &m, XF xf, OP op) {
     const int sub = threadIdx.x % SUB;
     const int64_t ng = (int64_t)gridDim.x * BT / SUB;
     for (int64_t row = g; row < m.n; row += ng) {
-      V s = pfdev::crs_row<V, V, SUB>(m.row_ptr, m.col_idx, m.val, row, sub, xf);
+      V s = pfdev::crs_row<M, V, SUB>(m.row_ptr, m.col_idx, m.val, row, sub, xf);
       if (sub == 0) op(row, s);
     }
   }
@@ -141,8 +144,8 @@ __device__ inline void read_partials(const double *part, int G, double (&out)[NV
 }
 
 // ---- K0: r = b - A x (or r = b when x0 = 0); r0 = p = r  -> partials r.r, b.b ----------
-template <typename V, int L, int SUB, bool ZEROX>
-__global__ void __launch_bounds__(BT) k_init(Mat<V> m, const V *__restrict__ b, V *__restrict__ x,
+template <typename V, int L, int SUB, bool ZEROX, typename M>
+__global__ void __launch_bounds__(BT) k_init(Mat<V, M> m, const V *__restrict__ b, V *__restrict__ x,
                                              V *__restrict__ r, V *__restrict__ r0, V *__restrict__ p,
                                              double *__restrict__ part) {
   double acc[2] = {0.0, 0.0};
@@ -195,8 +198,8 @@ __global__ void __launch_bounds__(BT) k_init_finish(const double *__restrict__ p
 }
 
 // ---- K1: y = A p -> partials r0.y ------------------------------------------------------
-template <typename V, int L, int SUB>
-__global__ void __launch_bounds__(BT) k_spmv_p(Mat<V> m, const V *__restrict__ p, V *__restrict__ y,
+template <typename V, int L, int SUB, typename M>
+__global__ void __launch_bounds__(BT) k_spmv_p(Mat<V, M> m, const V *__restrict__ p, V *__restrict__ y,
                                                const V *__restrict__ r0, const Scal *__restrict__ sc,
                                                double *__restrict__ part) {
   if (sc->done) return;
@@ -218,8 +221,8 @@ struct EFromRY {
 };
 
 // ---- K2: c2, alpha ; v = A e, e = r - alpha y -> partials e.v, v.v ----------------------
-template <typename V, int L, int SUB>
-__global__ void __launch_bounds__(BT) k_spmv_e(Mat<V> m, const V *__restrict__ r, const V *__restrict__ y,
+template <typename V, int L, int SUB, typename M>
+__global__ void __launch_bounds__(BT) k_spmv_e(Mat<V, M> m, const V *__restrict__ r, const V *__restrict__ y,
                                                V *__restrict__ v, Scal *__restrict__ sc,
                                                const double *__restrict__ pin, int Gin,
                                                double *__restrict__ part) {
@@ -280,8 +283,8 @@ __global__ void __launch_bounds__(BT) k_evec(int64_t n, const V *__restrict__ r,
   for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) e[i] = ef(i);
 }
 
-template <typename V, int L, int SUB>
-__global__ void __launch_bounds__(BT) k_spmv_ev(Mat<V> m, const V *__restrict__ e, V *__restrict__ v,
+template <typename V, int L, int SUB, typename M>
+__global__ void __launch_bounds__(BT) k_spmv_ev(Mat<V, M> m, const V *__restrict__ e, V *__restrict__ v,
                                                 Scal *__restrict__ sc, double *__restrict__ part) {
   if (sc->done) return;
   const pfdev::PlainX<V> ef{e};
@@ -576,12 +579,12 @@ __global__ void k_rel_setz(int64_t n, const double *__restrict__ x, const double
 // r' = S (b - A C z_hi) in FP64 (S row scale, C column scale), stored as FP32 r (INIT: also
 // r0 = p = r, z_lo = 0; a restarting replacement also sets r0 = p = r)
 // partials: |r'|^2, |b - A x|^2, r0.r', |b|^2
-template <int L, int SUB, bool INIT>
+template <int L, int SUB, bool INIT, typename VI>
 __global__ void __launch_bounds__(BT) k_rel_residual(Mat<double> m, const double *__restrict__ b,
                                                      const double *__restrict__ scale,
                                                      const double *__restrict__ cscale, const double *__restrict__ zhi,
-                                                     float *__restrict__ r, float *__restrict__ r0,
-                                                     float *__restrict__ p, float *__restrict__ zlo,
+                                                     VI *__restrict__ r, VI *__restrict__ r0,
+                                                     VI *__restrict__ p, VI *__restrict__ zlo,
                                                      const Scal *__restrict__ sc, double *__restrict__ part) {
   if (!INIT && !sc->pending) return;
   const bool restart = !INIT && sc->restart;
@@ -589,7 +592,7 @@ __global__ void __launch_bounds__(BT) k_rel_residual(Mat<double> m, const double
   auto op = [&](int64_t row, double ax) {
     const double bu = b[row] - ax;
     const double rp = scale[row] * bu;
-    const float rf = (float)rp;
+    const VI rf = (VI)rp;
     r[row] = rf;
     if (restart) {
       r0[row] = rf;
@@ -598,7 +601,7 @@ __global__ void __launch_bounds__(BT) k_rel_residual(Mat<double> m, const double
     } else if (INIT) {
       r0[row] = rf;
       p[row] = rf;
-      zlo[row] = 0.0f;
+      zlo[row] = (VI)0;
       acc[2] += (double)rf * (double)rf;
       acc[3] += b[row] * b[row];
     } else {
@@ -643,7 +646,8 @@ __global__ void __launch_bounds__(BT) k_rel_finish(const double *__restrict__ pa
 }
 
 // x = S (z_hi + z_lo)
-__global__ void k_rel_final(int64_t n, const double *__restrict__ zhi, const float *__restrict__ zlo,
+template <typename VI>
+__global__ void k_rel_final(int64_t n, const double *__restrict__ zhi, const VI *__restrict__ zlo,
                             const double *__restrict__ scale, double *__restrict__ x) {
   for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT)
     x[i] = scale[i] * (zhi[i] + (double)zlo[i]);
@@ -670,6 +674,7 @@ struct SolverWork {
   int64_t n = 0;
   DevBuf<double> r, r0, p, y, v, rref;  // FP64 recurrence vectors (+ FP64 residual)
   DevBuf<double> zhi;                   // mixed: FP64 part of the scaled solution
+  DevBuf<double> zlo64;                 // precision 48: the inner (FP64) correction
   DevBuf<float> x32, r32, r032, p32, y32, v32, val32;
   DevBuf<double> part, hist;
   DevBuf<Scal> sc;  // [0] FP64 / outer, [1] FP32 inner
@@ -771,17 +776,19 @@ int choose_sub(double avg_row) {  // ~five nonzeros per lane, 4..64 lanes per CR
     default: { constexpr int S_ = 64; __VA_ARGS__; } break; \
   }
 
-// FP64 side of the mixed solve: matrix, launch shape and vectors of the replacements
+// FP64 side of the mixed solve: matrix, launch shape and vectors of the replacements; VI is
+// the inner recurrence's vector type (FP32 for precision 32, FP64 for precision 48)
+template <typename VI>
 struct Rel {
   Mat<double> m;
   Launch L;
-  const double *b, *scale, *cscale;  // row and column scale of the FP32 system
+  const double *b, *scale, *cscale;  // row and column scale of the inner system
   double *zhi;
-  float *r, *r0, *p, *zlo;
+  VI *r, *r0, *p, *zlo;
 };
 
-template <bool INIT>
-int enqueue_replacement(pf_ctx *ctx, const Rel &R, Scal *sc, double *part) {
+template <bool INIT, typename VI>
+int enqueue_replacement(pf_ctx *ctx, const Rel<VI> &R, Scal *sc, double *part) {
   static_assert(PART_PI_ROWS >= 4, "k_rel_residual writes 4 partial rows");
   PF_CHECK(check_grid(ctx, R.L));
   hipStream_t st = ctx->stream;
@@ -797,8 +804,8 @@ int enqueue_replacement(pf_ctx *ctx, const Rel &R, Scal *sc, double *part) {
   return PF_OK;
 }
 
-template <typename V>
-int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V> &m, const V *b, V *x, V *r, V *r0, V *p, Scal *sc,
+template <typename V, typename M>
+int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, const V *b, V *x, V *r, V *r0, V *p, Scal *sc,
                  double *part, bool zerox) {
   PF_CHECK(check_grid(ctx, L));
   hipStream_t st = ctx->stream;
@@ -816,9 +823,9 @@ int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V> &m, const V *b, V *x
   return PF_OK;
 }
 
-template <typename V>
-int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V> &m, V *x, V *r, const V *r0, V *p, V *y, V *v,
-                  Scal *sc, double *part, double *hist, int count, const Rel *rel, V *ebuf,
+template <typename V, typename M>
+int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, V *x, V *r, const V *r0, V *p, V *y, V *v,
+                  Scal *sc, double *part, double *hist, int count, const Rel<V> *rel, V *ebuf,
                   hipEvent_t *ev = nullptr) {
   PF_CHECK(check_grid(ctx, L));
   hipStream_t st = ctx->stream;
@@ -855,9 +862,9 @@ int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V> &m, V *x, V *r, con
 }
 
 // Run the iteration loop of one BiCGSTAB (scalars slot `slot`) to its stop.
-template <typename V>
-int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V> &m, V *x, V *r, const V *r0,
-             V *p, V *y, V *v, double *hist, const pf_solve_opts &o, const Rel *rel = nullptr, V *ebuf = nullptr) {
+template <typename V, typename M>
+int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V, M> &m, V *x, V *r, const V *r0,
+             V *p, V *y, V *v, double *hist, const pf_solve_opts &o, const Rel<V> *rel = nullptr, V *ebuf = nullptr) {
   Scal *sc = w->sc.p + slot;
   Scal *hs = w->h_sc + slot;
   int chunk = o.check_every > 0 ? o.check_every : 32;
@@ -970,9 +977,15 @@ int check_blocks(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, 
   return PF_OK;
 }
 
-template <typename V>
-Mat<V> make_mat(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const V *val, const Sell3<V> *s) {
-  Mat<V> m{n, row_ptr, col_idx, val, 0, 0, nullptr, nullptr, nullptr, nullptr};
+template <typename T>
+struct NonDeduced {
+  using type = T;
+};
+
+template <typename V, typename M = V>
+Mat<V, M> make_mat(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const M *val,
+                   const typename NonDeduced<Sell3<M>>::type *s) {
+  Mat<V, M> m{n, row_ptr, col_idx, val, 0, 0, nullptr, nullptr, nullptr, nullptr};
   if (s) {
     m.nn = s->nn;
     m.nslices = s->nslices;
@@ -1076,6 +1089,75 @@ int run_loop_ilu(pf_ctx *ctx, SolverWork *w, const Launch &L, const Mat<double> 
   return PF_OK;
 }
 
+// Iterative refinement with an inner BiCGSTAB on the FP32 copy of the (scaled) matrix: VI =
+// float keeps the inner vectors and sums in FP32 (precision 32), VI = double keeps them in
+// FP64 and only the matrix values in FP32 (precision 48).  The FP64 side holds the solution
+// z_hi, the true residual and the stop test on ||b - A x|| / ||b||.
+template <typename VI>
+int solve_refined(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                  const double *val, const double *b, double *x, const pf_solve_opts &o, int layout, int sub,
+                  int smode, const double *scale, const double *cscale, SolveResult &res) {
+  const size_t G = (size_t)ctx->num_cus * 8;
+  const unsigned ge = fused_grid(ctx, n);
+  const int64_t sell_threads = ((n / 3 + 63) / 64) * 64;
+  PF_CHECK(w->zhi.ensure(n));
+  VI *zlo, *r, *r0, *p, *y, *v;
+  if constexpr (std::is_same<VI, float>::value) {
+    PF_CHECK(w->x32.ensure(n));
+    PF_CHECK(w->r32.ensure(n));
+    PF_CHECK(w->r032.ensure(n));
+    PF_CHECK(w->p32.ensure(n));
+    PF_CHECK(w->y32.ensure(n));
+    PF_CHECK(w->v32.ensure(n));
+    zlo = w->x32.p;
+    r = w->r32.p;
+    r0 = w->r032.p;
+    p = w->p32.p;
+    y = w->y32.p;
+    v = w->v32.p;
+  } else {
+    PF_CHECK(w->zlo64.ensure(n));
+    zlo = w->zlo64.p;
+    r = w->r.p;
+    r0 = w->r0.p;
+    p = w->p.p;
+    y = w->y.p;
+    v = w->v.p;
+  }
+  const Mat<VI, float> mi = make_mat<VI, float>(n, row_ptr, col_idx, w->have_v32 ? w->val32.p : nullptr,
+                                                layout == 1 ? &w->s32 : nullptr);
+  const Launch Li{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), ge, sub, layout};
+  Rel<VI> R;
+  R.m = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
+  R.L = Launch{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), ge, sub, layout};
+  R.b = b;
+  R.scale = scale;
+  R.cscale = cscale;
+  R.zhi = w->zhi.p;
+  R.r = r;
+  R.r0 = r0;
+  R.p = p;
+  R.zlo = zlo;
+  PF_CHECK(prepare_scalars(ctx, w, 1, o, o.itrmax, 0.0, 2, 0, 1, o.inner_max > 0 ? o.inner_max : 0));
+  w->h_sc[1].target = o.target_relres;
+  w->h_sc[1].delta = o.inner_tol > 0.0 && o.inner_tol < 1.0 ? o.inner_tol : 0.1;
+  w->h_sc[1].restart = smode == 2 ? 1 : 0;
+  PF_HIP(hipMemcpyAsync(w->sc.p + 1, w->h_sc + 1, sizeof(Scal), hipMemcpyHostToDevice, ctx->stream));
+  k_rel_setz<<<ge, BT, 0, ctx->stream>>>(n, x, cscale, w->zhi.p);
+  PF_HIP(hipGetLastError());
+  PF_CHECK(enqueue_replacement<true>(ctx, R, w->sc.p + 1, w->part.p + PART_PI * G));
+  PF_CHECK((run_loop<VI, float>(ctx, w, 1, Li, mi, zlo, r, r0, p, y, v, nullptr, o, &R)));
+  k_rel_final<<<ge, BT, 0, ctx->stream>>>(n, w->zhi.p, zlo, cscale, x);
+  PF_HIP(hipGetLastError());
+  const Scal &hi = w->h_sc[1];
+  res.status = hi.status;
+  res.iterations = hi.itr_last;
+  res.outer = hi.nupd;
+  res.rr = hi.rr;
+  res.rr1 = 0.0;
+  return PF_OK;
+}
+
 }  // namespace
 
 int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
@@ -1090,7 +1172,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     pf_set_error("stop_mode must be 0 (bicgstab_CRS_2), 1 (|r| < relative_er |b|) or 2 (bicgstab_CRS)");
     return PF_INVALID;
   }
-  if (o.stop_mode == 2 && (o.precision == 32 || o.preconditioning == 2)) {
+  if (o.stop_mode == 2 && (o.precision != 64 || o.preconditioning == 2)) {
     pf_set_error("stop_mode 2 (bicgstab_CRS) applies to the unpreconditioned FP64 solver");
     return PF_INVALID;
   }
@@ -1127,9 +1209,14 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
       if (nnz >= ((int64_t)1 << 20) && (double)w->s64.nslots * 64.0 * 9.0 > 1.5 * (double)nnz) layout = 0;
     }
   }
+  if (o.precision != 64 && o.precision != 32 && o.precision != 48) {
+    pf_set_error("precision must be 64 (FP64), 32 (FP32 values and vectors) or 48 (FP32 values, FP64 vectors)");
+    return PF_INVALID;
+  }
   const bool mixed = o.precision == 32;
+  const bool refined = o.precision != 64;  // precision 32 or 48: refinement over an FP32 matrix copy
   // FP32 system: 0 A, 1 S A S (S = diag^-1/2, unknown scaled), 2 D^-1 A (rows only, restarted)
-  const int smode = mixed ? (o.jacobi == 2 ? 2 : (o.jacobi != 0 ? 1 : 0)) : 0;
+  const int smode = refined ? (o.jacobi == 2 ? 2 : (o.jacobi != 0 ? 1 : 0)) : 0;
   if (smode != 0 && (!w->have_scale || w->scale_mode != smode)) {
     PF_CHECK(w->jscale.ensure(n));
     if (smode == 1) {
@@ -1150,12 +1237,12 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
       PF_CHECK(pf_sell3_build<double>(ctx, n, row_ptr, col_idx, val, w->s64));
       w->have_s64 = true;
     }
-    if (mixed && (!w->have_s32 || w->s32_mode != smode)) {
+    if (refined && (!w->have_s32 || w->s32_mode != smode)) {
       PF_CHECK(pf_sell3_build<float>(ctx, n, row_ptr, col_idx, val, w->s32, scale, nullptr, cscale));
       w->have_s32 = true;
       w->s32_mode = smode;
     }
-  } else if (mixed && (!w->have_v32 || w->v32_mode != smode)) {
+  } else if (refined && (!w->have_v32 || w->v32_mode != smode)) {
     PF_CHECK(w->val32.ensure((size_t)nnz + 1));
     PF_CHECK(pf_crs_to_f32(ctx, n, row_ptr, col_idx, val, scale, w->val32.p, cscale));
     w->have_v32 = true;
@@ -1163,7 +1250,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   }
   const bool ilu = o.preconditioning == 2;
   if (ilu) {
-    if (mixed) {
+    if (refined) {
       pf_set_error("ILU(0) preconditioning runs in FP64 only (precision 64)");
       return PF_INVALID;
     }
@@ -1204,7 +1291,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     res.iterations = h.itr_last + (h.status == PF_BREAKDOWN_VV ? 1 : 0);
     res.rr = h.rr;
     res.rr1 = h.rr1;
-  } else if (!mixed) {
+  } else if (!refined) {
     Mat<double> m64 = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
     Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), fused_grid(ctx, n), sub, layout};
     const int guard = o.stop_mode == 1 ? 1 : 0;  // modes 0 and 2 keep the reference's unguarded arithmetic
@@ -1226,55 +1313,22 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     res.rr = h.rr;
     res.rr1 = h.rr1;
   } else {
-    // FP32 BiCGSTAB on S A C (S = C = Jacobi scaling or identity; or S = D^-1, C = I with a
-    // restart at every replacement) with FP64 reliable updates
-    PF_CHECK(w->zhi.ensure(n));
-    PF_CHECK(w->x32.ensure(n));
-    PF_CHECK(w->r32.ensure(n));
-    PF_CHECK(w->r032.ensure(n));
-    PF_CHECK(w->p32.ensure(n));
-    PF_CHECK(w->y32.ensure(n));
-    PF_CHECK(w->v32.ensure(n));
-    const unsigned ge = fused_grid(ctx, n);
+    // BiCGSTAB on S A C with the FP32 copy of the matrix (S = C = Jacobi scaling or identity;
+    // or S = D^-1, C = I with a restart at every replacement) and FP64 reliable updates:
+    // FP32 vectors for precision 32, FP64 vectors for precision 48
     if (!scale) {  // unscaled: S = C = I
       PF_CHECK(w->jscale.ensure(n));
-      k_ones<<<ge, BT, 0, ctx->stream>>>(n, w->jscale.p);
+      k_ones<<<fused_grid(ctx, n), BT, 0, ctx->stream>>>(n, w->jscale.p);
       PF_HIP(hipGetLastError());
       scale = cscale = w->jscale.p;
       w->have_scale = false;
     }
-    Mat<float> m32 = make_mat<float>(n, row_ptr, col_idx, w->have_v32 ? w->val32.p : nullptr,
-                                     layout == 1 ? &w->s32 : nullptr);
-    Launch L32{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), ge, sub, layout};
-    Rel R;
-    R.m = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
-    R.L = Launch{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), ge, sub, layout};
-    R.b = b;
-    R.scale = scale;
-    R.cscale = cscale;
-    R.zhi = w->zhi.p;
-    R.r = w->r32.p;
-    R.r0 = w->r032.p;
-    R.p = w->p32.p;
-    R.zlo = w->x32.p;
-    PF_CHECK(prepare_scalars(ctx, w, 1, o, o.itrmax, 0.0, 2, 0, 1, o.inner_max > 0 ? o.inner_max : 0));
-    w->h_sc[1].target = o.target_relres;
-    w->h_sc[1].delta = o.inner_tol > 0.0 && o.inner_tol < 1.0 ? o.inner_tol : 0.1;
-    w->h_sc[1].restart = smode == 2 ? 1 : 0;
-    PF_HIP(hipMemcpyAsync(w->sc.p + 1, w->h_sc + 1, sizeof(Scal), hipMemcpyHostToDevice, ctx->stream));
-    k_rel_setz<<<ge, BT, 0, ctx->stream>>>(n, x, cscale, w->zhi.p);
-    PF_HIP(hipGetLastError());
-    PF_CHECK(enqueue_replacement<true>(ctx, R, w->sc.p + 1, w->part.p + PART_PI * G));
-    PF_CHECK(run_loop<float>(ctx, w, 1, L32, m32, w->x32.p, w->r32.p, w->r032.p, w->p32.p, w->y32.p, w->v32.p,
-                             nullptr, o, &R));
-    k_rel_final<<<ge, BT, 0, ctx->stream>>>(n, w->zhi.p, w->x32.p, cscale, x);
-    PF_HIP(hipGetLastError());
-    const Scal &hi = w->h_sc[1];
-    res.status = hi.status;
-    res.iterations = hi.itr_last;
-    res.outer = hi.nupd;
-    res.rr = hi.rr;
-    res.rr1 = 0.0;
+    if (mixed)
+      PF_CHECK(solve_refined<float>(ctx, w, n, row_ptr, col_idx, val, b, x, o, layout, sub, smode, scale, cscale,
+                                    res));
+    else
+      PF_CHECK(solve_refined<double>(ctx, w, n, row_ptr, col_idx, val, b, x, o, layout, sub, smode, scale, cscale,
+                                     res));
   }
   PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   PF_HIP(hipEventSynchronize(ctx->ev1));

@@ -23,7 +23,7 @@ module plantfem_gpu
    private
 
    public :: Mesh_, CRS_, FEMDomain_, FEMSolver_, pf_solve_opts, pf_stats
-   public :: pf_gpu_bicgstab, pf_gpu_bicgstab_func, pf_gpu_spmv, pf_gpu_version, FEMDomainp_
+   public :: pf_gpu_bicgstab, pf_gpu_bicgstab_func, pf_gpu_bicgstab_crs, pf_gpu_spmv, pf_gpu_version, FEMDomainp_
    integer(int32), parameter, public :: FEMDomain_Overset_GPP = 1, FEMDomain_Overset_P2P = 2
 
    integer(c_int), parameter, public :: PF_OK = 0, PF_BREAKDOWN_VV = 1, PF_BREAKDOWN_C2C3 = 2, &
@@ -1379,6 +1379,31 @@ contains
       x = x0
       call pf_gpu_bicgstab(row_ptr, col_idx, val, rhs, x)
    end function
+
+   ! bicgstab_CRS(a, ptr_i, index_j, x, b, itrmax, er, debug) of LinearSolver_
+   ! (LinearSolverClass.f90:1493-1569) on the GPU: the same arguments, x holds x0 on entry; a
+   ! return before the loop when r0.r0 < er, the exit on r.r / r0.r0 < er (stop_mode 2)
+   subroutine pf_gpu_bicgstab_crs(a, ptr_i, index_j, x, b, itrmax, er, debug)
+      real(real64), intent(in) :: a(:), b(:)
+      real(real64), intent(inout) :: x(:)
+      integer(int32), intent(in) :: index_j(:), itrmax
+      integer(int64), intent(in) :: ptr_i(:)
+      real(real64), intent(in) :: er
+      logical, optional, intent(in) :: debug
+      type(pf_solve_opts) :: o
+      type(pf_stats) :: st
+      integer(c_int) :: code
+      o = default_opts()
+      o%stop_mode = 2
+      o%er0 = er
+      o%itrmax = itrmax
+      if (present(debug)) then
+         if (debug) o%debug = 1
+      end if
+      code = c_pf_bicgstab_crs(ctx(), int(size(ptr_i) - 1, c_int64_t), ptr_i, index_j, a, b, x, o, st)
+      if (code == PF_BREAKDOWN_VV .or. code == PF_BREAKDOWN_C2C3) stop "Bicgstab devide by zero"
+      if (code /= PF_NOT_CONVERGED) call check(code, "pf_gpu_bicgstab_crs")
+   end subroutine
 
    ! sub_crs_matvec (LinearSolverClass.f90:3728-3776): new_vector = A old_vector
    subroutine pf_gpu_spmv(row_ptr, col_idx, val, old_vector, new_vector)

@@ -53,9 +53,13 @@ def test_fortran_plugin_solver():
     print("fortran plugin_solver", flush=True)
     res = subprocess.run([str(BIN / "plugin_solver")], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
-    x = np.array([float(v) for v in res.stdout.split("x=")[1].split("\n")[0].split()])
+    x = np.array([float(v) for v in res.stdout.split("x=", 1)[1].split("\n")[0].split()])
     A = np.array([[2, -1, 0, 0], [-1, 2, -1, 0], [0, 2, 3, 1], [0, 0, 1, 2]], float)
-    assert np.allclose(x, np.linalg.solve(A, [1, 2, 3, 4]), rtol=1e-9, atol=1e-9)
+    want = np.linalg.solve(A, [1, 2, 3, 4])
+    assert np.allclose(x, want, rtol=1e-9, atol=1e-9)
+    # LinearSolver_'s bicgstab_CRS argument list (stop r.r / r0.r0 < 1e-22)
+    x_crs = np.array([float(v) for v in res.stdout.split("x_crs=")[1].split("\n")[0].split()])
+    assert np.allclose(x_crs, want, rtol=1e-9, atol=1e-9)
 
 
 @pytest.mark.gpu

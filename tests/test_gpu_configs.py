@@ -136,4 +136,12 @@ def test_c3_generator_matches_oracle(ctx, reduced):
         "relres", st["true_relres"], "err", err)
     assert st["status"] == 0 and st["true_relres"] < 1e-10
     assert err <= 1e-8
+    # FP32 matrix values with FP64 vectors (precision 48) on the same system
+    x48 = s.solve(precision=48, target_relres=1e-10)
+    st = s.last_stats
+    err = np.linalg.norm(x48 - xo) / np.linalg.norm(xo)
+    say("C3 generator precision 48: replacements", st["outer_iterations"], "iterations", st["iterations"],
+        "relres", st["true_relres"], "err", err)
+    assert st["status"] == 0 and st["true_relres"] < 1e-10
+    assert err <= 1e-8
     s.close()

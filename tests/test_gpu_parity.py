@@ -419,6 +419,30 @@ def test_mixed_precision_c1(ctx):
     assert err <= 1e-8
 
 
+@pytest.mark.parametrize("jacobi", [2, 1])
+def test_fp32_matrix_precision_48_c1(ctx, jacobi):
+    """precision 48: the matrix values in FP32 (the bytes the products stream), vectors, sums and
+    the BiCGSTAB recurrence in FP64, FP64 refinement on ||b - A x||; against the oracle."""
+    w = W.config(1)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    say("C1 precision 48, jacobi", jacobi)
+    x = s.solve(precision=48, target_relres=1e-10, jacobi=jacobi)
+    st = s.last_stats
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    xo, *_ = O.solve(rp, ci, val, rhs, fixed, ub, relative_er=1e-14, er0=1e-30)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    say("outer", st["outer_iterations"], "inner", st["iterations"], "relres", st["true_relres"], "err", err)
+    assert st["status"] == 0 and st["true_relres"] < 1e-10
+    assert err <= 1e-8
+    # invalid precisions are refused
+    with pytest.raises(L.PFError):
+        s.solve(precision=16)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("jacobi", [2, 1])
 def test_mixed_precision_full_grid(ctx, jacobi):
