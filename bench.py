@@ -141,6 +141,9 @@ def main(argv=None):
         s.relative_er = 1e-30
         s.solve(stop_mode=1, download=False, use_graph=0, raise_on_breakdown=False)
         log(f"profile: {s.last_stats['iterations']} FP64 iterations")
+        s.solve(precision=48, target_relres=1e-30, stop_mode=1, download=False, use_graph=0,
+                raise_on_breakdown=False, **P48)
+        log(f"profile: {s.last_stats['iterations']} precision 48 iterations")
         xr = np.random.default_rng(1).standard_normal(n)
         for kind in (1, 2, 3):
             _, ms = s.spmv(kind, xr, repeat=5, download=False)
