@@ -30,6 +30,9 @@ extern "C" {
 #define PF_BREAKDOWN_C2C3 2  /* c2*c3 == 0: same stop (FEMSolverClass.f90:2295) */
 #define PF_NOT_CONVERGED 3   /* itrmax reached; the reference ends silently (FEMSolverClass.f90:2241) */
 #define PF_INVALID 4         /* bad ids/sizes, missing CRS slot ("fill-in request" :1040-1047), call order */
+#define PF_NOT_SPD 5         /* direct solve: a non-positive pivot in the Cholesky factorization; the
+                                reference's gauss_jordan_crs stops "A is singular" on a zero pivot
+                                (FEMSolverClass.f90:2455) */
 #define PF_ERR_HIP -1        /* HIP runtime error */
 #define PF_ERR_NOMEM -2      /* device allocation failed */
 
@@ -420,6 +423,42 @@ int pf_system_batch_solve(pf_system *sys, const pf_solve_opts *opts, const pf_ba
  * Overwrites the system's x. */
 int pf_system_batch_autotune(pf_system *sys, const pf_solve_opts *opts, int32_t iters, pf_batch_tune *best,
                              int32_t max_results, pf_batch_tune *tried, double *tried_ms, int32_t *ntried);
+
+/* ---- direct solve: solve(algorithm="GaussJordan") ------------------------------------
+ * The reference's solve() routes algorithm="GaussJordan" (FEMSolverClass.f90:1795-1798) to
+ * gauss_jordan_crs (FEMSolverClass.f90:2413-2460), which stops "Under construction" after its
+ * first pivot test.  Here that branch solves the eliminated system directly: each mesh's
+ * nodes (DOF=3 node blocks) or DOFs are renumbered by the candidate of smallest bandwidth b
+ * (natural, reverse Cuthill-McKee, coordinate sorts), the symmetric positive definite matrix
+ * is factored L L^T in band storage (n x (b+1) doubles per mesh, PF_ERR_NOMEM when that does
+ * not fit the device), then forward/backward substitution and `refine` steps of iterative
+ * refinement on the FP64 residual b - A x of the CRS values.  The factor is kept while the
+ * matrix is unchanged: a later solve with a new right-hand side only substitutes.  A
+ * non-positive pivot gives PF_NOT_SPD and x = 0 for that mesh. */
+
+/* Whole system as one mesh.  stats: status, iterations = 1 + refine substitutions,
+ * outer_iterations = refine, true_relres (FP64, eliminated system), setup_ms (band assembly,
+ * device), solve_ms (factorization, substitutions and residuals, device). */
+int pf_system_solve_direct(pf_system *sys, int32_t refine, double *x_out, pf_stats *stats);
+
+/* Every mesh of a batch (pf_system_batch_set) in one persistent kernel: one workgroup per mesh
+ * from a largest-first queue (cost = n b^2).  mesh_stats (may be NULL): status, iterations
+ * (1 + refine), true_relres, start_ms/end_ms of the mesh's factorization and first
+ * substitution.  Returns PF_OK, or the status of the first mesh (in mesh order) that failed. */
+int pf_system_batch_direct(pf_system *sys, int32_t refine, double *x_out, pf_batch_mesh_stats *mesh_stats,
+                           pf_stats *total);
+
+/* Ordering and factor of mesh `mesh` (0 for pf_system_solve_direct) of the last direct solve:
+ * *n DOFs, half bandwidth *bandwidth, *ordering 0 natural, 1 reverse Cuthill-McKee, 2/3/4
+ * coordinate sort with x/y/z the slowest key; perm (n entries, may be NULL): mesh-local DOF
+ * (0-based) of band row q; factor (n*(b+1), may be NULL): L of the permuted matrix in LAPACK
+ * lower band layout, factor[c*(b+1) + r - c] = L[r][c]. */
+int pf_system_direct_info(pf_system *sys, int32_t mesh, int64_t *n, int32_t *bandwidth, int32_t *ordering,
+                          int32_t *perm, double *factor);
+
+/* Tile products of the factorization: 1 (default) FP64 MFMA 16x16x4, 0 the same products on the
+ * vector units.  Drops a held factor. */
+int pf_system_direct_kernel(pf_system *sys, int32_t mfma);
 
 /* ---- body force and stress recovery ----------------------------------------------------- */
 

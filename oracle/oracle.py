@@ -55,6 +55,9 @@ _SIG = {
     "orc_multi_crs": (I64, [I64, I64, P32, I32, I64, P32, P64, P32]),
     "orc_connect_coeffs_all": (None, [I64, I64, PD, P32, I64, P32, P32, P32, PD, PD, PD]),
     "orc_ebom": (I32, [I64, P64, P32, PD, I64, P32, PD, PD, D]),
+    "orc_pbtrf_lower": (I32, [I64, I32, PD, I32]),
+    "orc_pbtrs_lower": (None, [I64, I32, PD, I32, PD]),
+    "orc_band_from_crs": (I32, [I64, P64, P32, PD, P32, I32, PD, I32]),
 }
 
 
@@ -483,3 +486,48 @@ def ebom(nodcoord, elemnod, row_ptr, col_idx, val, elem, gp, pair_elem, position
     if st != 0:
         raise ValueError("connect entry outside the pattern")
     return val
+
+
+def band_cholesky(ab):
+    """orc_pbtrf_lower (LAPACK dpbtf2, lower) of a band matrix given as an (n, kd+1) array whose
+    row j is column j of the band (ab[j, d] = A[j+d][j]).  Returns (info, L in the same layout);
+    info = j+1 when the pivot of column j is not positive."""
+    a = np.array(ab, dtype=np.float64, order="C", copy=True)
+    n, ldab = a.shape
+    info = load().orc_pbtrf_lower(n, ldab - 1, _p(a, C.c_double), ldab)
+    return int(info), a
+
+
+def band_solve(lfac, b):
+    """orc_pbtrs_lower: (L L^T)^-1 b with L from band_cholesky."""
+    a = np.ascontiguousarray(lfac, dtype=np.float64)
+    x = np.array(b, dtype=np.float64, copy=True)
+    n, ldab = a.shape
+    load().orc_pbtrs_lower(n, ldab - 1, _p(a, C.c_double), ldab, _p(x, C.c_double))
+    return x
+
+
+def band_from_crs(row_ptr, col_idx, val, perm, kd):
+    """Lower band (n, kd+1) of P A P^T, perm[q] = original row of band row q (0-based)."""
+    rp, ci, va = _c(row_ptr, np.int64), _c(col_idx, np.int32), _c(val, np.float64)
+    pm = _c(perm, np.int32)
+    n = rp.size - 1
+    ab = np.zeros((n, kd + 1))
+    bad = load().orc_band_from_crs(n, _p(rp, C.c_int64), _p(ci, C.c_int32), _p(va, C.c_double),
+                                   _p(pm, C.c_int32), kd, _p(ab, C.c_double), kd + 1)
+    if bad:
+        raise ValueError("band_from_crs: an entry lies outside the band")
+    return ab
+
+
+def direct_solve(row_ptr, col_idx, val, rhs, perm, kd):
+    """Direct solve of A x = rhs through the band Cholesky of P A P^T (lower triangle used)."""
+    ab = band_from_crs(row_ptr, col_idx, val, perm, kd)
+    info, lf = band_cholesky(ab)
+    if info:
+        raise ValueError(f"direct_solve: pivot of band column {info - 1} is not positive")
+    pm = np.asarray(perm)
+    y = band_solve(lf, np.asarray(rhs, dtype=np.float64)[pm])
+    x = np.empty_like(y)
+    x[pm] = y
+    return x

@@ -1315,3 +1315,72 @@ int orc_ebom(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *
                     }
     return ORC_OK;
 }
+
+/* ---- direct solve: band Cholesky (LAPACK dpbtf2 / dpbtrs, lower) ---------------------- */
+
+int orc_pbtrf_lower(int64_t n, int32_t kd, double *ab, int32_t ldab)
+{
+    for (int64_t j = 0; j < n; ++j) {
+        double *cj = ab + j * (int64_t)ldab;
+        double ajj = cj[0];
+        if (!(ajj > 0.0)) return (int)(j + 1);
+        ajj = sqrt(ajj);
+        cj[0] = ajj;
+        const int64_t kn = (n - 1 - j) < kd ? (n - 1 - j) : kd;
+        if (kn > 0) {
+            const double r = 1.0 / ajj; /* dscal(kn, 1/ajj, ...) */
+            for (int64_t i = 1; i <= kn; ++i) cj[i] *= r;
+            /* dsyr('L', kn, -1, x, A(j+1:j+kn, j+1:j+kn)): column c, rows c..kn-1 */
+            for (int64_t c = 0; c < kn; ++c) {
+                const double temp = -cj[1 + c];
+                if (temp == 0.0) continue;
+                double *cc = ab + (j + 1 + c) * (int64_t)ldab;
+                for (int64_t r2 = c; r2 < kn; ++r2) cc[r2 - c] += cj[1 + r2] * temp;
+            }
+        }
+    }
+    return 0;
+}
+
+void orc_pbtrs_lower(int64_t n, int32_t kd, const double *ab, int32_t ldab, double *b)
+{
+    /* dtbsv('L', 'N'): L y = b */
+    for (int64_t j = 0; j < n; ++j) {
+        const double *cj = ab + j * (int64_t)ldab;
+        if (b[j] != 0.0) {
+            b[j] /= cj[0];
+            const double t = b[j];
+            const int64_t kn = (n - 1 - j) < kd ? (n - 1 - j) : kd;
+            for (int64_t i = 1; i <= kn; ++i) b[j + i] -= t * cj[i];
+        }
+    }
+    /* dtbsv('L', 'T'): L^T x = y */
+    for (int64_t j = n - 1; j >= 0; --j) {
+        const double *cj = ab + j * (int64_t)ldab;
+        const int64_t kn = (n - 1 - j) < kd ? (n - 1 - j) : kd;
+        double t = b[j];
+        for (int64_t i = kn; i >= 1; --i) t -= cj[i] * b[j + i];
+        b[j] = t / cj[0];
+    }
+}
+
+int orc_band_from_crs(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                      const int32_t *perm, int32_t kd, double *ab, int32_t ldab)
+{
+    int32_t *inv = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    if (!inv) return 1;
+    for (int64_t q = 0; q < n; ++q) inv[perm[q]] = (int32_t)q;
+    memset(ab, 0, sizeof(double) * (size_t)n * (size_t)ldab);
+    int bad = 0;
+    for (int64_t r = 0; r < n; ++r) {
+        const int64_t pr = inv[r];
+        for (int64_t k = row_ptr[r] - 1; k < row_ptr[r + 1] - 1; ++k) {
+            const int64_t pc = inv[col_idx[k] - 1];
+            const int64_t d = pr - pc;
+            if (d > kd || -d > kd) bad = 1;
+            else if (d >= 0) ab[pc * ldab + d] = val[k];
+        }
+    }
+    free(inv);
+    return bad;
+}

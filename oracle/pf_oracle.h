@@ -178,6 +178,20 @@ int64_t orc_multi_crs(int64_t nn, int64_t ne, const int32_t *en, int any_overset
 int orc_ebom(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val, int64_t ncon,
              const int32_t *nodes16, const double *c16, const double *detJ, double penalty);
 
+/* Direct solve behind solve(algorithm="GaussJordan") (FEMSolverClass.f90:1795-1798; the
+ * reference's gauss_jordan_crs, FEMSolverClass.f90:2413-2460, stops "Under construction", so
+ * the oracle restates the textbook method the GPU path uses: Cholesky of a symmetric positive
+ * definite band matrix).  Band layout of LAPACK's lower form: ab[j*ldab + (i - j)] = A[i][j]
+ * for j <= i <= j + kd, ldab >= kd + 1.
+ * orc_pbtrf_lower: LAPACK dpbtf2 ('L'), L L^T in place; 0, or j+1 when the pivot of column j
+ * (0-based) is not positive.  orc_pbtrs_lower: LAPACK dpbtrs ('L'), b := (L L^T)^-1 b.
+ * orc_band_from_crs: lower band of P A P^T from a CRS (1-based), perm[q] = original row of
+ * band row q (0-based); 1 when an entry falls outside the band. */
+int orc_pbtrf_lower(int64_t n, int32_t kd, double *ab, int32_t ldab);
+void orc_pbtrs_lower(int64_t n, int32_t kd, const double *ab, int32_t ldab, double *b);
+int orc_band_from_crs(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
+                      const int32_t *perm, int32_t kd, double *ab, int32_t ldab);
+
 #ifdef __cplusplus
 }
 #endif

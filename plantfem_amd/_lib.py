@@ -18,6 +18,7 @@ PF_BREAKDOWN_VV = 1
 PF_BREAKDOWN_C2C3 = 2
 PF_NOT_CONVERGED = 3
 PF_INVALID = 4
+PF_NOT_SPD = 5
 PF_ERR_HIP = -1
 PF_ERR_NOMEM = -2
 PF_NOALLOC_MAX_ROWS = 524288  # include/pfgpu.h
@@ -28,6 +29,7 @@ STATUS_NAMES = {
     PF_BREAKDOWN_C2C3: "breakdown c2*c3 == 0 (Bicgstab devide by zero)",
     PF_NOT_CONVERGED: "itrmax reached",
     PF_INVALID: "invalid input",
+    PF_NOT_SPD: "non-positive pivot: the matrix is not symmetric positive definite (A is singular)",
     PF_ERR_HIP: "HIP error",
     PF_ERR_NOMEM: "device out of memory",
 }
@@ -184,6 +186,10 @@ SIGNATURES = {
     "pf_system_stiffness_dev": (C.c_int, [P, P, P, I32]),
     "pf_system_solve_multi_dev": (C.c_int, [P, I32, P, P, C.POINTER(SolveOpts), P, C.POINTER(Stats)]),
     "pf_system_elem_sensitivity_dev": (C.c_int, [P, I32, P, P, P]),
+    "pf_system_solve_direct": (C.c_int, [P, I32, PF64, C.POINTER(Stats)]),
+    "pf_system_batch_direct": (C.c_int, [P, I32, PF64, P, C.POINTER(Stats)]),
+    "pf_system_direct_info": (C.c_int, [P, I32, PI64, PI32, PI32, PI32, PF64]),
+    "pf_system_direct_kernel": (C.c_int, [P, I32]),
 }
 
 

@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FC", "/opt/rocm/llvm/bin/flang")
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["scan.hip", "graph.hip", "assemble.hip", "bc.hip", "spmv.hip", "ilu.hip", "bicgstab.hip", "batch.hip", "multi.hip", "eig.hip", "noalloc.hip", "capi.hip"]
+HIP_SOURCES = ["scan.hip", "graph.hip", "assemble.hip", "bc.hip", "spmv.hip", "ilu.hip", "bicgstab.hip", "batch.hip", "multi.hip", "eig.hip", "noalloc.hip", "direct.hip", "capi.hip"]
 # files whose arithmetic must follow the reference's operation order exactly
 NO_CONTRACT = {"assemble.hip", "bc.hip", "ilu.hip"}
 

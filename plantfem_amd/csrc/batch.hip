@@ -470,6 +470,16 @@ struct BatchWork {
 
 void pf_batch_work_destroy(BatchWork *b) { delete b; }
 
+int pf_batch_dof_offsets(const BatchWork *b, std::vector<int64_t> &off) {
+  if (!b || b->nmesh < 1) {
+    pf_set_error("batch: call pf_system_batch_set first");
+    return PF_INVALID;
+  }
+  off.resize(b->node_off.size());
+  for (size_t i = 0; i < off.size(); ++i) off[i] = 3 * b->node_off[i];
+  return PF_OK;
+}
+
 int pf_batch_configure(pf_ctx *ctx, BatchWork **pb, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                        int32_t nmesh, const int64_t *node_off, const double *cost) {
   if (nmesh < 1 || !node_off || n % 3 != 0) {

@@ -638,6 +638,9 @@ struct pf_system {
   uint64_t bc_mat = 0, bc_rhs = 0, bc_fix = 0;  // versions the elimination was last applied to
   BatchWork *batch = nullptr;
   MultiWork *multi = nullptr;
+  DirectWork *direct = nullptr;
+  uint64_t pattern_version = 1;  // bumped with every change of the CRS structure
+  int32_t direct_mfma = 1;       // direct solve: 1 MFMA tile products, 0 the VALU form
   int32_t jmode = 0;          // J mode of the last assembly (reused by the sensitivities)
   bool ubar_nonzero = false;  // some fixed DOF has a nonzero prescribed value
   // keepThisMatrixAs: named copies of the CRS values (same pattern) and the fix_eig list
@@ -652,6 +655,7 @@ struct pf_system {
     pf_solver_work_destroy(work);
     pf_batch_work_destroy(batch);
     pf_multi_work_destroy(multi);
+    pf_direct_work_destroy(direct);
   }
 };
 
@@ -732,6 +736,7 @@ extern "C" int pf_system_set_mesh(pf_system *s, int64_t nn, int64_t ne, int32_t 
   s->has_mat = false;
   s->graph_crs = false;
   s->multi_crs = false;
+  ++s->pattern_version;
   return PF_OK;
 }
 
@@ -760,6 +765,7 @@ extern "C" int pf_system_zero_matrix(pf_system *s, int32_t dof, int64_t *nnz) {
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->graph_crs = true;
   s->multi_crs = false;
+  ++s->pattern_version;
   mark_changed(s);
   if (nnz) *nnz = s->nnz;
   return PF_OK;
@@ -787,6 +793,7 @@ extern "C" int pf_system_set_crs(pf_system *s, int64_t n, int64_t nnz, const int
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->graph_crs = false;
   s->multi_crs = false;
+  ++s->pattern_version;
   mark_changed(s);
   return PF_OK;
 }
@@ -1299,6 +1306,70 @@ extern "C" int pf_system_batch_autotune(pf_system *s, const pf_solve_opts *opts,
 }
 
 // ============================================================================ multi-RHS / adjoint
+// ---- direct solve: solve(algorithm="GaussJordan") (FEMSolverClass.f90:1795-1798) ----------
+
+static int direct_finish(pf_system *s, int rc, double *x_out) {
+  pf_ctx *ctx = s->ctx;
+  if (rc < 0 || rc == PF_INVALID) return rc;
+  if (s->has_fix) PF_CHECK(pf_bc_set_fixed_x(ctx, s->n, s->fixed.p, s->ubar.p, s->x.p));
+  if (x_out) PF_CHECK(download(ctx, x_out, s->x.p, (size_t)s->n));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return rc;
+}
+
+static const double *direct_coords(pf_system *s) {
+  return (s->g.has_coords && 3 * s->g.nn == s->n) ? s->g.nodcoord.p : nullptr;
+}
+
+extern "C" int pf_system_solve_direct(pf_system *s, int32_t refine, double *x_out, pf_stats *stats) {
+  PF_CHECK(sys_check(s, true));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_CHECK(apply_bc(s));
+  const int64_t off[2] = {0, s->n};
+  PF_CHECK(pf_direct_analyse(ctx, &s->direct, s->n, s->row_ptr.p, s->col_idx.p, 1, off, direct_coords(s), s->g.nn,
+                             s->pattern_version));
+  pf_stats st;
+  const int rc = pf_direct_solve_device(ctx, s->direct, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz,
+                                        s->rhs.p, s->x.p, s->mat_version, refine, s->direct_mfma, nullptr, &st);
+  if (stats && rc >= 0 && rc != PF_INVALID) *stats = st;
+  return direct_finish(s, rc, x_out);
+}
+
+extern "C" int pf_system_batch_direct(pf_system *s, int32_t refine, double *x_out, pf_batch_mesh_stats *mesh_stats,
+                                      pf_stats *total) {
+  PF_CHECK(sys_check(s, true));
+  pf_ctx *ctx = s->ctx;
+  DeviceGuard guard(ctx);
+  PF_CHECK(batch_prepare(s));
+  std::vector<int64_t> off;
+  PF_CHECK(pf_batch_dof_offsets(s->batch, off));
+  const int32_t nmesh = (int32_t)off.size() - 1;
+  PF_CHECK(pf_direct_analyse(ctx, &s->direct, s->n, s->row_ptr.p, s->col_idx.p, nmesh, off.data(), direct_coords(s),
+                             s->g.nn, s->pattern_version));
+  pf_stats st;
+  const int rc = pf_direct_solve_device(ctx, s->direct, s->n, s->row_ptr.p, s->col_idx.p, s->val.p, s->nnz,
+                                        s->rhs.p, s->x.p, s->mat_version, refine, s->direct_mfma, mesh_stats, &st);
+  if (total && rc >= 0 && rc != PF_INVALID) *total = st;
+  return direct_finish(s, rc, x_out);
+}
+
+extern "C" int pf_system_direct_info(pf_system *s, int32_t mesh, int64_t *n, int32_t *bandwidth, int32_t *ordering,
+                                     int32_t *perm, double *factor) {
+  PF_CHECK(sys_check(s, false));
+  DeviceGuard guard(s->ctx);
+  PF_CHECK(pf_direct_info(s->direct, mesh, n, bandwidth, ordering, perm));
+  if (factor) PF_CHECK(pf_direct_factor_get(s->ctx, s->direct, mesh, factor));
+  return PF_OK;
+}
+
+extern "C" int pf_system_direct_kernel(pf_system *s, int32_t mfma) {
+  PF_CHECK(sys_check(s, false));
+  s->direct_mfma = mfma ? 1 : 0;
+  pf_direct_invalidate(s->direct);
+  return PF_OK;
+}
+
 extern "C" int pf_system_solve_multi_dev(pf_system *s, int32_t nrhs, const double *B_dev, double *X_dev,
                                          const pf_solve_opts *opts, pf_stats *per_rhs, pf_stats *total) {
   PF_CHECK(sys_check(s, true));
@@ -1494,6 +1565,7 @@ extern "C" int pf_system_multi_crs(pf_system *s, int32_t dof, int32_t any_overse
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->graph_crs = false;
   s->multi_crs = true;
+  ++s->pattern_version;
   mark_changed(s);
   if (nnz) *nnz = s->nnz;
   return PF_OK;

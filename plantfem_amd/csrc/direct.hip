@@ -1,0 +1,928 @@
+// direct.hip -- direct solve of the eliminated system: banded Cholesky (L L^T) per mesh.
+//
+// The reference's solve() has a direct branch, solve(algorithm="GaussJordan")
+// (FEMSolverClass.f90:1795-1798) -> gauss_jordan_crs (FEMSolverClass.f90:2413-2460), whose body
+// stops with "Under construction" after the first pivot test ("A is singular" on a zero pivot,
+// :2455).  This file gives that branch a working direct method for the symmetric positive
+// definite systems the elimination produces (FEMSolverClass.f90:1707-1756 keeps K symmetric:
+// fixed rows and columns are zeroed with a unit diagonal).
+//
+// Per mesh (one mesh = the whole system, or every mesh of a block-diagonal batch):
+//  * ordering (host, once per sparsity pattern): the node (DOF=3 block) or DOF graph is
+//    renumbered by the candidate of smallest bandwidth among the natural order, reverse
+//    Cuthill-McKee and lexicographic coordinate sorts along each axis;
+//  * band storage (LAPACK dpbtrf 'L' layout): entry (r, c), 0 <= r - c <= b, of the permuted
+//    matrix at band[c*ld + r - c], ld >= b + 1; the order is padded to a multiple of 16 with
+//    unit diagonal rows;
+//  * factorization: one workgroup per mesh from a largest-first queue (persistent kernel),
+//    left-looking by blocks of 16 columns: the block's (b+16) x 16 rows are updated by the
+//    previous b columns with FP64 MFMA 16x16x4 tiles, the 16 x 16 diagonal tile is factored
+//    with cross-lane operations and the tiles below are solved against it in registers;
+//  * triangular solves: blocks of 16, the diagonal solve in one wave, the off-diagonal
+//    update with one thread per row (forward) or column (backward).
+// Fixed-precision iterative refinement on the true residual b - A x (the CRS values, in
+// FP64) follows as requested.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "pf_internal.hpp"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int DBT = 512;   // threads per workgroup of the factor/solve kernel
+constexpr int MAXT = 2;    // 16 x 16 tiles a wave holds at once
+
+struct DMesh {
+  int64_t d0;    // first DOF of the mesh in the system
+  int64_t boff;  // first band entry
+  int64_t woff;  // first entry of the mesh's permutation and work vector
+  int32_t n;     // DOFs
+  int32_t npad;  // n rounded up to a multiple of 16
+  int32_t b;     // half bandwidth of the permuted matrix (DOFs)
+  int32_t ld;    // leading dimension of the band (>= b + 1, even)
+};
+
+struct DArgs {
+  const DMesh *mesh;
+  const int32_t *order;  // queue order (largest first)
+  int32_t *queue;
+  int32_t nmesh;
+  int32_t mode;          // 0: factor, then solve x = A^-1 rhs; 1: solve only, x += A^-1 rhs
+  double *band;
+  const int32_t *perm;   // band index q of mesh m -> local DOF (perm[woff + q], q < n)
+  double *w;             // work vector (npad per mesh)
+  const double *rhs;
+  double *x;
+  int32_t *status;       // per mesh
+  int64_t *t_raw;        // per mesh: wall-clock start, end of the factor+solve
+};
+
+__device__ inline double shfl(double v, int src) { return __shfl(v, src, 64); }
+
+// Cholesky of the 16 x 16 diagonal tile held in the FP64 MFMA accumulator layout (lane l,
+// element v: row 4*v + l/16, column l%16).  Lower part becomes L11; bad is set on a
+// non-positive pivot.
+__device__ inline void chol16(d4 &a, int q, int jl, bool &bad) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const double piv = shfl(a[k >> 2], (k & 3) * 16 + k);
+    bad = bad || !(piv > 0.0);
+    const double s = sqrt(piv);
+    const double rs = 1.0 / s;
+    if (jl == k) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 4 * v + q;
+        if (i > k) a[v] *= rs;
+        else if (i == k) a[v] = s;
+      }
+    }
+    // L[i][k] of this lane's rows (same lane group, column k) and L[jl][k]
+    double lik[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) lik[v] = shfl(a[v], 16 * q + k);
+    const int src = 16 * (jl & 3) + k;
+    const double e0 = shfl(a[0], src), e1 = shfl(a[1], src), e2 = shfl(a[2], src), e3 = shfl(a[3], src);
+    const int sel = jl >> 2;
+    const double ljk = sel == 0 ? e0 : (sel == 1 ? e1 : (sel == 2 ? e2 : e3));
+    if (jl > k) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (4 * v + q >= jl) a[v] -= lik[v] * ljk;
+    }
+  }
+}
+
+// acc += A B for the 16x4 fragment A (lane l holds A[l%16][l/16]) and the 4x16 fragment B
+// (lane l holds B[l/16][l%16]); accumulator layout as chol16 (the register layout of
+// v_mfma_f64_16x16x4f64).  MF: the matrix core; else the
+// same product from cross-lane reads and FMAs.
+template <bool MF>
+__device__ inline void tile_fma(d4 &acc, double af, double bf, int q, int jl) {
+  if constexpr (MF) {
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc, 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double bk = shfl(bf, 16 * k + jl);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] += shfl(af, 16 * k + 4 * v + q) * bk;
+    }
+  }
+}
+
+// Left-looking blocked band Cholesky of one mesh by the whole workgroup.
+template <int BT, bool MF>
+__device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL, int32_t *sflag) {
+  constexpr int W = BT / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = lane >> 4, jl = lane & 15;
+  const int b = M.b, npad = M.npad;
+  const int64_t ld = M.ld;
+  if (threadIdx.x == 0) *sflag = 0;
+  __syncthreads();
+  double Lrow[16];
+  for (int j0 = 0; j0 < npad; j0 += 16) {
+    const int rows = min(16 + b, npad - j0);  // rows j0 .. j0+rows-1 hold the block's nonzeros
+    const int T = (rows + 15) >> 4;
+    const int P = (T + W * MAXT - 1) / (W * MAXT);
+    for (int p = 0; p < P; ++p) {
+      const int tb = wv + W * MAXT * p;  // this wave's tiles: tb, tb + W, ..., tb + (MAXT-1) W
+      d4 acc[MAXT];
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i) {
+        const int t = tb + W * i;
+        const int64_t cc = j0 + jl;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = j0 + 16 * t + 4 * v + q;
+          const int d = r - (int)cc;
+          acc[i][v] = (t < T && r < npad && d >= 0 && d <= b) ? band[cc * ld + d] : 0.0;
+        }
+      }
+      if (tb < T) {
+        // A[block rows, j0..j0+15] -= L[block rows, c] L[j0..j0+15, c]^T over the previous columns
+        const int clo = max(0, j0 + 16 * tb - b) & ~3;
+        for (int c = clo; c < j0; c += 4) {
+          const int64_t ck = c + q;
+          const int db = j0 + jl - (int)ck;
+          const double bf = db <= b ? band[ck * ld + db] : 0.0;
+#pragma unroll
+          for (int i = 0; i < MAXT; ++i) {
+            const int t = tb + W * i;
+            const int R = j0 + 16 * t;
+            if (t < T && c + 3 >= R - b) {
+              const int ra = R + jl;
+              const int da = ra - (int)ck;
+              const double af = (ra < npad && da <= b) ? -band[ck * ld + da] : 0.0;
+              tile_fma<MF>(acc[i], af, bf, q, jl);
+            }
+          }
+        }
+      }
+      if (p == 0) {
+        // diagonal tile (tile 0 of wave 0): L11, to LDS and the band
+        if (wv == 0) {
+          bool bad = false;
+          chol16(acc[0], q, jl, bad);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 4 * v + q;
+            sL[i * 16 + jl] = acc[0][v];
+            const int d = i - jl;
+            if (d >= 0 && d <= b) band[(int64_t)(j0 + jl) * ld + d] = acc[0][v];
+          }
+          if (bad && lane == 0) *sflag = 1;
+        }
+        __syncthreads();
+        if (*sflag) return PF_NOT_SPD;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) Lrow[k] = sL[jl * 16 + k];
+      }
+      // tiles below the diagonal: X L11^T = C, column by column across the lanes
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i) {
+        const int t = tb + W * i;
+        if (t >= 1 && t < T) {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) {
+            if (jl == k) {
+              const double rk = 1.0 / sL[k * 16 + k];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) acc[i][v] *= rk;
+            }
+            double xk[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) xk[v] = shfl(acc[i][v], 16 * q + k);
+            if (jl > k) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) acc[i][v] -= xk[v] * Lrow[k];
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = j0 + 16 * t + 4 * v + q;
+            const int d = r - (j0 + jl);
+            if (r < npad && d <= b) band[(int64_t)(j0 + jl) * ld + d] = acc[i][v];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  return PF_OK;
+}
+
+// L L^T z = w in place (w: npad entries of the mesh's work vector).
+template <int BT>
+__device__ void band_solve(const DMesh &M, const double *__restrict__ band, double *__restrict__ w, double *sY) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int b = M.b, npad = M.npad;
+  const int64_t ld = M.ld;
+  // forward: L y = w
+  for (int j0 = 0; j0 < npad; j0 += 16) {
+    if (wv == 0) {
+      double acc = lane < 16 ? w[j0 + lane] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (lane == k) acc /= band[(int64_t)(j0 + k) * ld];
+        const double yk = shfl(acc, k);
+        if (lane > k && lane < 16 && lane - k <= b) acc -= band[(int64_t)(j0 + k) * ld + (lane - k)] * yk;
+      }
+      if (lane < 16) {
+        w[j0 + lane] = acc;
+        sY[lane] = acc;
+      }
+    }
+    __syncthreads();
+    const int rend = min(npad, j0 + 16 + b);
+    for (int r = j0 + 16 + (int)threadIdx.x; r < rend; r += BT) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int d = r - j0 - j;
+        if (d <= b) s += band[(int64_t)(j0 + j) * ld + d] * sY[j];
+      }
+      w[r] -= s;
+    }
+    __syncthreads();
+  }
+  // backward: L^T z = y
+  for (int j0 = npad - 16; j0 >= 0; j0 -= 16) {
+    if (wv == 0) {
+      double acc = lane < 16 ? w[j0 + lane] : 0.0;
+#pragma unroll
+      for (int k = 15; k >= 0; --k) {
+        if (lane == k) acc /= band[(int64_t)(j0 + k) * ld];
+        const double xk = shfl(acc, k);
+        if (lane < k && k - lane <= b) acc -= band[(int64_t)(j0 + lane) * ld + (k - lane)] * xk;
+      }
+      if (lane < 16) {
+        w[j0 + lane] = acc;
+        sY[lane] = acc;
+      }
+    }
+    __syncthreads();
+    const int ibeg = max(0, j0 - b);
+    for (int i = ibeg + (int)threadIdx.x; i < j0; i += BT) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int d = j0 + j - i;
+        if (d <= b) s += band[(int64_t)i * ld + d] * sY[j];
+      }
+      w[i] -= s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_direct_queue_reset(int32_t *q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) q[0] = 0;
+}
+
+__global__ void k_flag_clear(int32_t *f) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) f[0] = 0;
+}
+
+template <int BT, bool MF>
+__global__ void __launch_bounds__(BT) k_direct(DArgs a) {
+  __shared__ double sL[256];
+  __shared__ double sY[16];
+  __shared__ int32_t next, sflag;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int32_t qi = next;
+    if (qi >= a.nmesh) break;
+    const int32_t m = a.order[qi];
+    const DMesh M = a.mesh[m];
+    const int64_t t0 = wall_clock64();
+    double *band = a.band + M.boff;
+    int st = a.mode == 0 ? factor_mesh<BT, MF>(M, band, sL, &sflag) : a.status[m];
+    double *w = a.w + M.woff;
+    const int32_t *perm = a.perm + M.woff;
+    if (st == PF_OK) {
+      for (int q = threadIdx.x; q < M.npad; q += BT) w[q] = q < M.n ? a.rhs[M.d0 + perm[q]] : 0.0;
+      __syncthreads();
+      band_solve<BT>(M, band, w, sY);
+      for (int q = threadIdx.x; q < M.n; q += BT) {
+        const int64_t r = M.d0 + perm[q];
+        a.x[r] = a.mode == 0 ? w[q] : a.x[r] + w[q];
+      }
+    } else if (a.mode == 0) {
+      for (int q = threadIdx.x; q < M.n; q += BT) a.x[M.d0 + q] = 0.0;
+    }
+    if (threadIdx.x == 0 && a.mode == 0) {
+      a.status[m] = st;
+      a.t_raw[2 * m] = t0;
+      a.t_raw[2 * m + 1] = wall_clock64();
+    }
+  }
+}
+
+__global__ void k_band_zero(double *__restrict__ band, int64_t count) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += stride) band[i] = 0.0;
+}
+
+// lower triangle of P A P^T into the band: one SUB-lane subgroup per row
+template <int SUB>
+__global__ void __launch_bounds__(256) k_band_scatter(int64_t n, const int64_t *__restrict__ row_ptr,
+                                                      const int32_t *__restrict__ col_idx,
+                                                      const double *__restrict__ val, const DMesh *__restrict__ mesh,
+                                                      const int32_t *__restrict__ dmesh,
+                                                      const int32_t *__restrict__ bidx, double *__restrict__ band,
+                                                      int32_t *__restrict__ bad) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t r = g / SUB;
+  const int lane = (int)(g % SUB);
+  if (r >= n) return;
+  const int32_t m = dmesh[r];
+  const DMesh M = mesh[m];
+  const int64_t pr = bidx[r];
+  const int64_t e = row_ptr[r + 1] - 1;
+  for (int64_t k = row_ptr[r] - 1 + lane; k < e; k += SUB) {
+    const int64_t c = col_idx[k] - 1;
+    const int64_t pc = bidx[c];
+    // an entry outside the mesh or the analysed band means a stale analysis: flag, never write
+    if (dmesh[c] != m || pr - pc > M.b || pc - pr > M.b) {
+      atomicOr(bad, 1);
+    } else if (pr >= pc) {
+      band[M.boff + pc * M.ld + (pr - pc)] = val[k];
+    }
+  }
+}
+
+// unit diagonal of the padding rows n .. npad-1 of every mesh
+__global__ void k_band_pad(int32_t nmesh, const DMesh *__restrict__ mesh, double *__restrict__ band) {
+  const int32_t m = blockIdx.x;
+  if (m >= nmesh) return;
+  const DMesh M = mesh[m];
+  for (int q = M.n + (int)threadIdx.x; q < M.npad; q += blockDim.x) band[M.boff + (int64_t)q * M.ld] = 1.0;
+}
+
+// r = rhs - y over every mesh's rows, and per mesh (|r|^2, |rhs|^2); one workgroup per mesh
+__global__ void __launch_bounds__(256) k_direct_resid(int32_t nmesh, const DMesh *__restrict__ mesh,
+                                                      const double *__restrict__ rhs, const double *__restrict__ y,
+                                                      double *__restrict__ r, double *__restrict__ norms) {
+  __shared__ double sh[2][4];
+  const int32_t m = blockIdx.x;
+  if (m >= nmesh) return;
+  const DMesh M = mesh[m];
+  double t0 = 0.0, t1 = 0.0;
+  for (int64_t i = M.d0 + threadIdx.x; i < M.d0 + M.n; i += 256) {
+    const double bi = rhs[i];
+    const double ri = bi - y[i];
+    r[i] = ri;
+    t0 += ri * ri;
+    t1 += bi * bi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    t0 += __shfl_xor(t0, o, 64);
+    t1 += __shfl_xor(t1, o, 64);
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][wv] = t0;
+    sh[1][wv] = t1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    norms[2 * m] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    norms[2 * m + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
+// node (DOF group) neighbours from a DOF=3 blocked CRS: every third column of row 3i
+__global__ void k_group_cols(int64_t nn, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
+                             const int64_t *__restrict__ goff, int32_t *__restrict__ gcol) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int64_t r0 = row_ptr[3 * i] - 1;
+  const int64_t m = goff[i + 1] - goff[i];
+  for (int64_t t = 0; t < m; ++t) gcol[goff[i] + t] = (col_idx[r0 + 3 * t] - 1) / 3;
+}
+
+// ---------------------------------------------------------------------------------
+// orderings (host)
+
+struct LocalGraph {
+  int64_t g0 = 0, ng = 0;                 // groups g0 .. g0+ng-1
+  const int64_t *off = nullptr;           // global group adjacency offsets
+  const int32_t *adj = nullptr;           // global group ids
+};
+
+int64_t bandwidth_of(const LocalGraph &G, const std::vector<int32_t> &pos) {
+  int64_t bw = 0;
+  for (int64_t i = 0; i < G.ng; ++i) {
+    const int64_t pi = pos[i];
+    for (int64_t k = G.off[G.g0 + i]; k < G.off[G.g0 + i + 1]; ++k) {
+      const int64_t j = G.adj[k] - G.g0;
+      if (j < 0 || j >= G.ng) continue;  // outside the mesh (not expected in a checked partition)
+      const int64_t d = pi > pos[j] ? pi - pos[j] : pos[j] - pi;
+      if (d > bw) bw = d;
+    }
+  }
+  return bw;
+}
+
+// reverse Cuthill-McKee from a pseudo-peripheral start (George-Liu), components in turn
+std::vector<int32_t> rcm_order(const LocalGraph &G) {
+  const int64_t n = G.ng;
+  std::vector<int32_t> deg(n);
+  for (int64_t i = 0; i < n; ++i) deg[i] = (int32_t)(G.off[G.g0 + i + 1] - G.off[G.g0 + i]);
+  std::vector<int32_t> order;
+  order.reserve(n);
+  std::vector<char> done(n, 0);
+  std::vector<int32_t> level(n, -1), queue;
+  queue.reserve(n);
+  auto bfs_levels = [&](int32_t s, int32_t &last_first, int32_t &nlev) {
+    queue.clear();
+    queue.push_back(s);
+    level[s] = 0;
+    size_t h = 0;
+    while (h < queue.size()) {
+      const int32_t u = queue[h++];
+      for (int64_t k = G.off[G.g0 + u]; k < G.off[G.g0 + u + 1]; ++k) {
+        const int64_t v = G.adj[k] - G.g0;
+        if (v < 0 || v >= n || done[v] || level[v] >= 0) continue;
+        level[v] = level[u] + 1;
+        queue.push_back((int32_t)v);
+      }
+    }
+    nlev = level[queue.back()] + 1;
+    last_first = queue.back();
+    int32_t best = queue.back();
+    for (size_t t = queue.size(); t-- > 0;) {
+      const int32_t u = queue[t];
+      if (level[u] != nlev - 1) break;
+      if (deg[u] < deg[best]) best = u;
+    }
+    last_first = best;
+    for (int32_t u : queue) level[u] = -1;
+  };
+  for (;;) {
+    int32_t s = -1;
+    for (int64_t i = 0; i < n; ++i)
+      if (!done[i] && (s < 0 || deg[i] < deg[s])) s = (int32_t)i;
+    if (s < 0) break;
+    int32_t cand = s, nlev = 0, nlev2 = 0;
+    bfs_levels(s, cand, nlev);
+    for (int it = 0; it < 8; ++it) {
+      int32_t c2 = cand;
+      bfs_levels(cand, c2, nlev2);
+      if (nlev2 <= nlev) break;
+      s = cand;
+      nlev = nlev2;
+      cand = c2;
+    }
+    // Cuthill-McKee from s: neighbours by increasing degree
+    const size_t first = order.size();
+    order.push_back(s);
+    done[s] = 1;
+    std::vector<int32_t> nb;
+    for (size_t h = first; h < order.size(); ++h) {
+      const int32_t u = order[h];
+      nb.clear();
+      for (int64_t k = G.off[G.g0 + u]; k < G.off[G.g0 + u + 1]; ++k) {
+        const int64_t v = G.adj[k] - G.g0;
+        if (v < 0 || v >= n || done[v]) continue;
+        done[v] = 1;
+        nb.push_back((int32_t)v);
+      }
+      std::stable_sort(nb.begin(), nb.end(), [&](int32_t x, int32_t y) { return deg[x] < deg[y]; });
+      order.insert(order.end(), nb.begin(), nb.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+struct MeshOrder {
+  std::vector<int32_t> order;  // new position -> local group
+  int64_t bw = 0;              // in groups
+  int32_t kind = 0;            // 0 natural, 1 RCM, 2/3/4 coordinate sort with x/y/z primary
+};
+
+MeshOrder best_order(const LocalGraph &G, const double *coords, int64_t ncoord) {
+  MeshOrder best;
+  best.order.resize(G.ng);
+  std::iota(best.order.begin(), best.order.end(), 0);
+  std::vector<int32_t> pos(G.ng);
+  auto eval = [&](const std::vector<int32_t> &ord) {
+    for (int64_t k = 0; k < G.ng; ++k) pos[ord[k]] = (int32_t)k;
+    return bandwidth_of(G, pos);
+  };
+  best.bw = eval(best.order);
+  best.kind = 0;
+  {
+    std::vector<int32_t> o = rcm_order(G);
+    const int64_t bw = eval(o);
+    if (bw < best.bw) {
+      best.bw = bw;
+      best.order = std::move(o);
+      best.kind = 1;
+    }
+  }
+  if (coords) {
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = 1e300;
+      hi[d] = -1e300;
+      for (int64_t i = 0; i < G.ng; ++i) {
+        const double v = coords[d * ncoord + G.g0 + i];
+        lo[d] = std::min(lo[d], v);
+        hi[d] = std::max(hi[d], v);
+      }
+    }
+    for (int p = 0; p < 3; ++p) {
+      int s = (p + 1) % 3, t = (p + 2) % 3;
+      if (hi[t] - lo[t] > hi[s] - lo[s]) std::swap(s, t);
+      std::vector<int32_t> o(G.ng);
+      std::iota(o.begin(), o.end(), 0);
+      auto key = [&](int32_t i, int d) { return coords[d * ncoord + G.g0 + i]; };
+      std::stable_sort(o.begin(), o.end(), [&](int32_t x, int32_t y) {
+        if (key(x, p) != key(y, p)) return key(x, p) < key(y, p);
+        if (key(x, s) != key(y, s)) return key(x, s) < key(y, s);
+        return key(x, t) < key(y, t);
+      });
+      const int64_t bw = eval(o);
+      if (bw < best.bw) {
+        best.bw = bw;
+        best.order = std::move(o);
+        best.kind = 2 + p;
+      }
+    }
+  }
+  return best;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+
+struct DirectWork {
+  // analysis (sparsity pattern and partition)
+  uint64_t pattern_key = 0;
+  std::vector<int64_t> dof_off;
+  int32_t nmesh = 0;
+  int32_t group = 1;
+  int64_t n = 0;
+  std::vector<DMesh> hmesh;
+  std::vector<int32_t> kinds;
+  std::vector<int32_t> hperm;  // band index -> local DOF, per mesh at woff
+  DevBuf<DMesh> mesh;
+  DevBuf<int32_t> perm, bidx, dmesh, order, queue, status, flag;
+  DevBuf<int64_t> t_raw;
+  DevBuf<double> band, w, r, y, norms;
+  int64_t band_entries = 0, work_entries = 0;
+  double analyse_ms = 0.0;
+  // factor held in band: the matrix version it was computed from (0: none)
+  uint64_t factor_version = 0;
+  bool factor_ok = false;
+};
+
+void pf_direct_work_destroy(DirectWork *d) { delete d; }
+
+void pf_direct_invalidate(DirectWork *d) {
+  if (d) d->factor_version = 0;
+}
+
+int pf_direct_analyse(pf_ctx *ctx, DirectWork **pd, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                      int32_t nmesh, const int64_t *dof_off, const double *coords_dev, int64_t ncoord,
+                      uint64_t pattern_key) {
+  if (nmesh < 1 || n < 1 || dof_off[0] != 0 || dof_off[nmesh] != n) {
+    pf_set_error("direct: meshes must partition the %lld DOFs", (long long)n);
+    return PF_INVALID;
+  }
+  for (int32_t m = 0; m < nmesh; ++m)
+    if (dof_off[m + 1] <= dof_off[m]) {
+      pf_set_error("direct: mesh %d is empty", m);
+      return PF_INVALID;
+    }
+  if (!*pd) *pd = new DirectWork();
+  DirectWork *D = *pd;
+  if (D->pattern_key == pattern_key && D->nmesh == nmesh && D->n == n &&
+      (int64_t)D->dof_off.size() == nmesh + 1 && std::equal(D->dof_off.begin(), D->dof_off.end(), dof_off))
+    return PF_OK;
+  const auto tstart = std::chrono::steady_clock::now();
+  D->pattern_key = 0;
+  D->factor_version = 0;
+  D->nmesh = 0;
+  PF_CHECK(D->flag.ensure(1));
+  int blocked3 = 0;
+  if (n % 3 == 0) PF_CHECK(pf_block_structure_check(ctx, n, row_ptr, col_idx, n / 3, D->flag.p, &blocked3));
+  const int32_t g = blocked3 ? 3 : 1;
+  for (int32_t m = 0; m <= nmesh; ++m)
+    if (dof_off[m] % g != 0) {
+      pf_set_error("direct: mesh offsets must be whole DOF=3 blocks");
+      return PF_INVALID;
+    }
+  const int64_t ng = n / g;
+  // group adjacency on the host
+  std::vector<int64_t> hrp(n + 1);
+  PF_HIP(hipMemcpyAsync(hrp.data(), row_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int64_t> goff(ng + 1, 0);
+  std::vector<int32_t> gadj;
+  if (g == 3) {
+    for (int64_t i = 0; i < ng; ++i) goff[i + 1] = goff[i] + (hrp[3 * i + 1] - hrp[3 * i]) / 3;
+    DevBuf<int64_t> doff;
+    DevBuf<int32_t> dcol;
+    PF_CHECK(doff.ensure(ng + 1));
+    PF_CHECK(dcol.ensure(std::max<int64_t>(goff[ng], 1)));
+    PF_HIP(hipMemcpyAsync(doff.p, goff.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, ctx->stream));
+    k_group_cols<<<pf_blocks(ng, 256, INT32_MAX), 256, 0, ctx->stream>>>(ng, row_ptr, col_idx, doff.p, dcol.p);
+    PF_HIP(hipGetLastError());
+    gadj.resize(goff[ng]);
+    PF_HIP(hipMemcpyAsync(gadj.data(), dcol.p, sizeof(int32_t) * goff[ng], hipMemcpyDeviceToHost, ctx->stream));
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+  } else {
+    const int64_t nnz = hrp[n] - 1;
+    gadj.resize(nnz);
+    PF_HIP(hipMemcpyAsync(gadj.data(), col_idx, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, ctx->stream));
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i <= n; ++i) goff[i] = hrp[i] - 1;
+    for (auto &c : gadj) c -= 1;
+  }
+  std::vector<double> coords;
+  if (coords_dev && g == 3 && ncoord == ng) {
+    coords.resize(3 * ncoord);
+    PF_HIP(hipMemcpyAsync(coords.data(), coords_dev, sizeof(double) * 3 * ncoord, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  // per-mesh orderings, meshes spread over host threads
+  std::vector<MeshOrder> ord(nmesh);
+  {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nt = (unsigned)std::min<int64_t>(nt, nmesh);
+    std::vector<std::thread> th;
+    std::atomic<int32_t> next{0};
+    for (unsigned t = 0; t < nt; ++t)
+      th.emplace_back([&]() {
+        for (;;) {
+          const int32_t m = next.fetch_add(1);
+          if (m >= nmesh) break;
+          LocalGraph G;
+          G.g0 = dof_off[m] / g;
+          G.ng = (dof_off[m + 1] - dof_off[m]) / g;
+          G.off = goff.data();
+          G.adj = gadj.data();
+          ord[m] = best_order(G, coords.empty() ? nullptr : coords.data(), ncoord);
+        }
+      });
+    for (auto &t : th) t.join();
+  }
+  // band layout
+  D->hmesh.assign(nmesh, DMesh{});
+  D->kinds.assign(nmesh, 0);
+  int64_t boff = 0, woff = 0;
+  for (int32_t m = 0; m < nmesh; ++m) {
+    DMesh &M = D->hmesh[m];
+    const int64_t nm = dof_off[m + 1] - dof_off[m];
+    if (nm > (int64_t)INT32_MAX - 16) {
+      pf_set_error("direct: mesh %d has %lld DOFs, more than the band kernels index", m, (long long)nm);
+      return PF_INVALID;
+    }
+    M.d0 = dof_off[m];
+    M.n = (int32_t)nm;
+    M.npad = (int32_t)((nm + 15) / 16 * 16);
+    M.b = (int32_t)(g * ord[m].bw + g - 1);
+    M.ld = (M.b + 2) & ~1;
+    M.boff = boff;
+    M.woff = woff;
+    boff += (int64_t)M.npad * M.ld;
+    woff += M.npad;
+    D->kinds[m] = ord[m].kind;
+  }
+  size_t free_b = 0, total_b = 0;
+  PF_HIP(hipMemGetInfo(&free_b, &total_b));
+  const double need = 8.0 * (double)boff;
+  const double have = (double)free_b + 8.0 * (double)D->band.n;
+  if (need > have - 2.0e9) {
+    pf_set_error("direct: the band factors need %.2f GB of device memory, %.2f GB are free", need * 1e-9,
+                 have * 1e-9);
+    return PF_ERR_NOMEM;
+  }
+  D->hperm.assign(woff, 0);
+  std::vector<int32_t> hbidx(n), hdm(n);
+  for (int32_t m = 0; m < nmesh; ++m) {
+    const DMesh &M = D->hmesh[m];
+    const std::vector<int32_t> &o = ord[m].order;
+    for (int64_t k = 0; k < (int64_t)o.size(); ++k)
+      for (int d = 0; d < g; ++d) {
+        const int32_t q = (int32_t)(g * k + d);
+        const int32_t loc = g * o[k] + d;
+        D->hperm[M.woff + q] = loc;
+        hbidx[M.d0 + loc] = q;
+        hdm[M.d0 + loc] = m;
+      }
+  }
+  std::vector<int32_t> qorder(nmesh);
+  std::iota(qorder.begin(), qorder.end(), 0);
+  std::stable_sort(qorder.begin(), qorder.end(), [&](int32_t x, int32_t y) {
+    const DMesh &a = D->hmesh[x], &b = D->hmesh[y];
+    return (double)a.npad * a.b * a.b > (double)b.npad * b.b * b.b;
+  });
+  PF_CHECK(D->band.ensure(std::max<int64_t>(boff, 1)));
+  PF_CHECK(D->mesh.ensure(nmesh));
+  PF_CHECK(D->perm.ensure(std::max<int64_t>(woff, 1)));
+  PF_CHECK(D->w.ensure(std::max<int64_t>(woff, 1)));
+  PF_CHECK(D->bidx.ensure(n));
+  PF_CHECK(D->dmesh.ensure(n));
+  PF_CHECK(D->order.ensure(nmesh));
+  PF_CHECK(D->queue.ensure(1));
+  PF_CHECK(D->status.ensure(nmesh));
+  PF_CHECK(D->t_raw.ensure(2 * (size_t)nmesh));
+  PF_CHECK(D->norms.ensure(2 * (size_t)nmesh));
+  PF_CHECK(D->r.ensure(n + 1));
+  PF_CHECK(D->y.ensure(n + 1));
+  PF_HIP(hipMemcpyAsync(D->mesh.p, D->hmesh.data(), sizeof(DMesh) * nmesh, hipMemcpyHostToDevice, ctx->stream));
+  PF_HIP(hipMemcpyAsync(D->perm.p, D->hperm.data(), sizeof(int32_t) * woff, hipMemcpyHostToDevice, ctx->stream));
+  PF_HIP(hipMemcpyAsync(D->bidx.p, hbidx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+  PF_HIP(hipMemcpyAsync(D->dmesh.p, hdm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+  PF_HIP(hipMemcpyAsync(D->order.p, qorder.data(), sizeof(int32_t) * nmesh, hipMemcpyHostToDevice, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  D->band_entries = boff;
+  D->work_entries = woff;
+  D->group = g;
+  D->n = n;
+  D->dof_off.assign(dof_off, dof_off + nmesh + 1);
+  D->nmesh = nmesh;
+  D->pattern_key = pattern_key;
+  D->analyse_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
+  return PF_OK;
+}
+
+namespace {
+int launch_direct(pf_ctx *ctx, DirectWork *D, int mode, const double *rhs, double *x, bool mfma) {
+  DArgs a;
+  a.mesh = D->mesh.p;
+  a.order = D->order.p;
+  a.queue = D->queue.p;
+  a.nmesh = D->nmesh;
+  a.mode = mode;
+  a.band = D->band.p;
+  a.perm = D->perm.p;
+  a.w = D->w.p;
+  a.rhs = rhs;
+  a.x = x;
+  a.status = D->status.p;
+  a.t_raw = D->t_raw.p;
+  // two workgroups per CU at most; no more workgroups than meshes
+  const int grid = std::max(1, std::min(D->nmesh, 2 * ctx->num_cus));
+  k_direct_queue_reset<<<1, 64, 0, ctx->stream>>>(D->queue.p);
+  PF_HIP(hipGetLastError());
+  if (mfma)
+    k_direct<DBT, true><<<grid, DBT, 0, ctx->stream>>>(a);
+  else
+    k_direct<DBT, false><<<grid, DBT, 0, ctx->stream>>>(a);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+// r = rhs - A x and the per-mesh norms
+int residual(pf_ctx *ctx, DirectWork *D, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+             const double *val, double avg_row, const double *rhs, const double *x) {
+  PF_CHECK(pf_spmv_crs_launch<double>(ctx, n, row_ptr, col_idx, val, x, D->y.p, avg_row));
+  k_direct_resid<<<D->nmesh, 256, 0, ctx->stream>>>(D->nmesh, D->mesh.p, rhs, D->y.p, D->r.p, D->norms.p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+}  // namespace
+
+int pf_direct_solve_device(pf_ctx *ctx, DirectWork *D, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                           const double *val, int64_t nnz, const double *rhs, double *x, uint64_t mat_version,
+                           int32_t refine, int32_t mfma, pf_batch_mesh_stats *mesh_stats, pf_stats *total) {
+  if (!D || D->nmesh < 1 || D->n != n) {
+    pf_set_error("direct: no analysis for this system");
+    return PF_INVALID;
+  }
+  if (refine < 0 || refine > 100) {
+    pf_set_error("direct: refinement steps must be in [0, 100]");
+    return PF_INVALID;
+  }
+  const double avg_row = n > 0 ? (double)nnz / (double)n : 1.0;
+  float setup_ms = 0.f, solve_ms = 0.f;
+  const bool refactor = D->factor_version != mat_version || mat_version == 0;
+  PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  if (refactor) {
+    D->factor_version = 0;
+    k_band_zero<<<pf_blocks(D->band_entries, 256, 8 * 2048), 256, 0, ctx->stream>>>(D->band.p, D->band_entries);
+    PF_HIP(hipGetLastError());
+    k_flag_clear<<<1, 64, 0, ctx->stream>>>(D->flag.p);
+    k_band_scatter<16><<<pf_blocks(n * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+        n, row_ptr, col_idx, val, D->mesh.p, D->dmesh.p, D->bidx.p, D->band.p, D->flag.p);
+    PF_HIP(hipGetLastError());
+    int32_t bad = 0;
+    PF_HIP(hipMemcpyAsync(&bad, D->flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    if (bad) {
+      pf_set_error("direct: the matrix has entries outside the analysed meshes or band (pattern changed?)");
+      return PF_INVALID;
+    }
+    k_band_pad<<<D->nmesh, 64, 0, ctx->stream>>>(D->nmesh, D->mesh.p, D->band.p);
+    PF_HIP(hipGetLastError());
+  }
+  PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  PF_HIP(hipEventSynchronize(ctx->ev1));
+  PF_HIP(hipEventElapsedTime(&setup_ms, ctx->ev0, ctx->ev1));
+  PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  if (refactor) {
+    PF_CHECK(launch_direct(ctx, D, 0, rhs, x, mfma != 0));
+  } else {
+    // factor kept: solve only (x = 0 + A^-1 rhs)
+    PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, ctx->stream));
+    PF_CHECK(launch_direct(ctx, D, 1, rhs, x, mfma != 0));
+  }
+  for (int32_t s = 0; s < refine; ++s) {
+    PF_CHECK(residual(ctx, D, n, row_ptr, col_idx, val, avg_row, rhs, x));
+    PF_CHECK(launch_direct(ctx, D, 1, D->r.p, x, mfma != 0));
+  }
+  PF_CHECK(residual(ctx, D, n, row_ptr, col_idx, val, avg_row, rhs, x));
+  PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  PF_HIP(hipEventSynchronize(ctx->ev1));
+  PF_HIP(hipEventElapsedTime(&solve_ms, ctx->ev0, ctx->ev1));
+  const int32_t nm = D->nmesh;
+  std::vector<int32_t> st(nm);
+  std::vector<int64_t> raw(2 * (size_t)nm);
+  std::vector<double> nrm(2 * (size_t)nm);
+  PF_HIP(hipMemcpyAsync(st.data(), D->status.p, sizeof(int32_t) * nm, hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipMemcpyAsync(raw.data(), D->t_raw.p, sizeof(int64_t) * 2 * nm, hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipMemcpyAsync(nrm.data(), D->norms.p, sizeof(double) * 2 * nm, hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  int rate_khz = 0;
+  PF_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+  const double tick_ms = rate_khz > 0 ? 1.0 / (double)rate_khz : 0.0;
+  int64_t tmin = raw[0];
+  for (int32_t m = 0; m < nm; ++m) tmin = std::min(tmin, raw[2 * m]);
+  int status = PF_OK;
+  double worst = 0.0;
+  bool all_ok = true;
+  for (int32_t m = 0; m < nm; ++m) {
+    const double relres = nrm[2 * m + 1] > 0.0 ? std::sqrt(nrm[2 * m] / nrm[2 * m + 1]) : std::sqrt(nrm[2 * m]);
+    if (st[m] != PF_OK) all_ok = false;
+    if (status == PF_OK && st[m] != PF_OK) status = st[m];
+    if (st[m] == PF_OK) worst = std::max(worst, relres);
+    if (mesh_stats) {
+      pf_batch_mesh_stats &s = mesh_stats[m];
+      std::memset(&s, 0, sizeof(s));
+      s.status = st[m];
+      s.iterations = st[m] == PF_OK ? 1 + refine : 0;
+      s.replacements = st[m] == PF_OK ? refine : 0;
+      s.workgroup = -1;
+      s.true_relres = relres;
+      s.start_ms = (double)(raw[2 * m] - tmin) * tick_ms;
+      s.end_ms = (double)(raw[2 * m + 1] - tmin) * tick_ms;
+    }
+  }
+  if (refactor && all_ok) D->factor_version = mat_version;
+  if (total) {
+    std::memset(total, 0, sizeof(*total));
+    total->status = status;
+    total->iterations = 1 + refine;
+    total->outer_iterations = refine;
+    total->true_relres = worst;
+    total->solve_ms = solve_ms;
+    total->setup_ms = setup_ms;
+  }
+  return status;
+}
+
+int pf_direct_info(DirectWork *D, int32_t mesh, int64_t *n, int32_t *bandwidth, int32_t *kind, int32_t *perm) {
+  if (!D || D->nmesh < 1 || mesh < 0 || mesh >= D->nmesh) {
+    pf_set_error("direct: no analysis, or mesh %d out of range", mesh);
+    return PF_INVALID;
+  }
+  const DMesh &M = D->hmesh[mesh];
+  if (n) *n = M.n;
+  if (bandwidth) *bandwidth = M.b;
+  if (kind) *kind = D->kinds[mesh];
+  if (perm) std::memcpy(perm, D->hperm.data() + M.woff, sizeof(int32_t) * M.n);
+  return PF_OK;
+}
+
+int pf_direct_factor_get(pf_ctx *ctx, DirectWork *D, int32_t mesh, double *ab) {
+  if (!D || D->nmesh < 1 || mesh < 0 || mesh >= D->nmesh || D->factor_version == 0) {
+    pf_set_error("direct: no factor held for mesh %d", mesh);
+    return PF_INVALID;
+  }
+  const DMesh &M = D->hmesh[mesh];
+  // LAPACK lower band layout with ldab = b + 1, columns 0 .. n-1
+  PF_HIP(hipMemcpy2DAsync(ab, sizeof(double) * (M.b + 1), D->band.p + M.boff, sizeof(double) * M.ld,
+                          sizeof(double) * (M.b + 1), M.n, hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}

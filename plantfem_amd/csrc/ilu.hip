@@ -452,25 +452,38 @@ int pf_ilu_sweep_enqueue(pf_ctx *ctx, IluFactor &F, int mode, const IluSweepArgs
 // last stored entry of the column, getCRSval :2488-2503) is nonzero is multiplied by its
 // reciprocal, the right-hand side entry too
 namespace {
-__global__ void k_jacobi_rows(int64_t n, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
-                              double *__restrict__ val, double *__restrict__ rhs) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t a = row_ptr[i] - 1, b = row_ptr[i + 1] - 1;
+// one SUB-lane subgroup per row: the lanes search the row for its diagonal (the one lane that
+// finds it contributes A_kk, the others 0, so the subgroup sum is A_kk exactly), then scale
+// their entries; lane 0 scales the right-hand side
+template <int SUB>
+__global__ void __launch_bounds__(256) k_jacobi_rows(int64_t n, const int64_t *__restrict__ row_ptr,
+                                                     const int32_t *__restrict__ col_idx, double *__restrict__ val,
+                                                     double *__restrict__ rhs) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = g / SUB;
+  const int lane = (int)(g % SUB);
+  const bool live = i < n;
+  int64_t a = 0, b = 0;
+  if (live) {
+    a = row_ptr[i] - 1;
+    b = row_ptr[i + 1] - 1;
+  }
   double akk = 0.0;
-  for (int64_t k = a; k < b; ++k)
+  for (int64_t k = a + lane; k < b; k += SUB)
     if (col_idx[k] == i + 1) akk = val[k];
-  if (akk == 0.0) return;
+#pragma unroll
+  for (int o = SUB / 2; o > 0; o >>= 1) akk += __shfl_xor(akk, o, SUB);
+  if (!live || akk == 0.0) return;
   const double inv = 1.0 / akk;
-  if (rhs) rhs[i] = rhs[i] * inv;
-  for (int64_t k = a; k < b; ++k) val[k] = val[k] * inv;
+  if (lane == 0 && rhs) rhs[i] = rhs[i] * inv;
+  for (int64_t k = a + lane; k < b; k += SUB) val[k] = val[k] * inv;
 }
 }  // namespace
 
 int pf_jacobi_rows(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, double *val,
                    double *rhs) {
   if (n <= 0) return PF_OK;
-  k_jacobi_rows<<<pf_blocks(n, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs);
+  k_jacobi_rows<16><<<pf_blocks(n * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, rhs);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

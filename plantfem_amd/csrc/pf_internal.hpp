@@ -302,6 +302,26 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
 // batch.hip: many independent meshes of one block-diagonal system, one persistent kernel
 struct BatchWork;
 void pf_batch_work_destroy(BatchWork *b);
+// DOF offsets (3 x node offsets, nmesh + 1 entries) of a configured batch
+int pf_batch_dof_offsets(const BatchWork *b, std::vector<int64_t> &off);
+
+// direct.hip: banded Cholesky direct solve per mesh (solve(algorithm="GaussJordan"))
+struct DirectWork;
+void pf_direct_work_destroy(DirectWork *d);
+void pf_direct_invalidate(DirectWork *d);
+// ordering and band layout of the meshes dof_off[m] .. dof_off[m+1]-1 (host offsets); node
+// orderings when the CRS has DOF=3 node blocks (coords: nn x 3 device coordinates or NULL),
+// DOF orderings otherwise; kept while pattern_key, n and the offsets are unchanged
+int pf_direct_analyse(pf_ctx *ctx, DirectWork **pd, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                      int32_t nmesh, const int64_t *dof_off, const double *coords_dev, int64_t ncoord,
+                      uint64_t pattern_key);
+// band assembly (when mat_version differs from the held factor), factorization, substitution,
+// `refine` refinement steps and per-mesh FP64 residuals; x receives the solution
+int pf_direct_solve_device(pf_ctx *ctx, DirectWork *D, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                           const double *val, int64_t nnz, const double *rhs, double *x, uint64_t mat_version,
+                           int32_t refine, int32_t mfma, pf_batch_mesh_stats *mesh_stats, pf_stats *total);
+int pf_direct_info(DirectWork *D, int32_t mesh, int64_t *n, int32_t *bandwidth, int32_t *kind, int32_t *perm);
+int pf_direct_factor_get(pf_ctx *ctx, DirectWork *D, int32_t mesh, double *ab);
 // node_off (host, nmesh+1): mesh m owns nodes node_off[m] .. node_off[m+1]-1; cost (host,
 // may be NULL) orders the queue, largest first (default: the mesh's nonzeros)
 int pf_batch_configure(pf_ctx *ctx, BatchWork **b, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,

@@ -851,17 +851,31 @@ class FEMSolver:
 
     # (7-2)
     def solve(self, x0=None, precision: int = 64, raise_on_breakdown: bool = True, download: bool = True,
-              preconditioning: str | None = None, jacobi_solve: bool = True, **opts):
-        """solve([preconditioning], [x0]) (FEMSolverClass.f90:1697-1825).  download=False keeps x
-        on the device (returns None).
+              preconditioning: str | None = None, jacobi_solve: bool = True, algorithm: str | None = None,
+              refine: int = 1, **opts):
+        """solve([algorithm], [preconditioning], [x0]) (FEMSolverClass.f90:1697-1825).
+        download=False keeps x on the device (returns None).
 
         preconditioning (FEMSolverClass.f90:1768-1790): None -> bicgstab_CRS_2; "ILU" or
         "incompleteLU" -> bicgstab_CRS_ILU with the ILU(0) factor kept from the first such solve
         (iluReset() drops it); "PointJacobi" -> JacobiPreconditionerCRS on the eliminated system,
         then bicgstab_CRS_2 on the scaled system.  The reference's PointJacobi branch calls no
         solver after the scaling (its x is x0 with the fixed values); jacobi_solve=False does
-        exactly that."""
+        exactly that.
+
+        algorithm (FEMSolverClass.f90:1793-1808, used when no preconditioning is given):
+        "GaussJordan" -> a direct solve (the reference's gauss_jordan_crs, FEMSolverClass.f90:
+        2413-2460, stops "Under construction"; here a banded Cholesky factorization after a
+        bandwidth-reducing renumbering, kept while the matrix is unchanged, with `refine` steps
+        of FP64 iterative refinement); any other value -> bicgstab_CRS_2."""
         self._flush()
+        if preconditioning is None and algorithm == "GaussJordan":
+            st = L.Stats()
+            x = np.empty(self.n) if download else None
+            code = L.load().pf_system_solve_direct(self.h, int(refine), L.ptr(x, C.c_double), C.byref(st))
+            self.last_stats = st.as_dict()
+            L.check(code, "solve (GaussJordan)")
+            return x
         if preconditioning is not None:
             if preconditioning == "PointJacobi":
                 opts["preconditioning"] = 1 if jacobi_solve else 3
@@ -1030,11 +1044,24 @@ class FEMSolver:
         return best.as_dict(), [(tried[i].as_dict(), float(ms[i])) for i in range(min(nt.value, 32))]
 
     def solveBatch(self, precision: int = 64, tune: dict | None = None, download: bool = True,
-                   raise_on_breakdown: bool = True, preconditioning: str | None = None, **opts):
+                   raise_on_breakdown: bool = True, preconditioning: str | None = None,
+                   algorithm: str | None = None, refine: int = 1, **opts):
         """Solve every mesh of the batch in one persistent kernel; returns (x, per-mesh stats).
         preconditioning="PointJacobi" scales the eliminated rows first (JacobiPreconditionerCRS,
-        in place) and solves the scaled systems."""
+        in place) and solves the scaled systems.  algorithm="GaussJordan": the direct solve of
+        solve(algorithm="GaussJordan") for every mesh, one workgroup per mesh."""
         self._flush()
+        if preconditioning is None and algorithm == "GaussJordan":
+            st = np.zeros(self._nmesh, dtype=L.BATCH_STATS_DTYPE)
+            tot = L.Stats()
+            x = np.empty(self.n) if download else None
+            code = L.load().pf_system_batch_direct(self.h, int(refine), L.ptr(x, C.c_double),
+                                                   st.ctypes.data_as(C.c_void_p), C.byref(tot))
+            self.last_stats = tot.as_dict()
+            L.check(code, "solveBatch (GaussJordan)",
+                    ok=(L.PF_OK,) if raise_on_breakdown else (L.PF_OK, L.PF_NOT_SPD))
+            return x, st
+
         if preconditioning is not None:
             if preconditioning != "PointJacobi":
                 raise ValueError(f"solveBatch :: invalid preconditioning {preconditioning!r}")
@@ -1052,6 +1079,29 @@ class FEMSolver:
             L.PF_OK, L.PF_NOT_CONVERGED, L.PF_BREAKDOWN_VV, L.PF_BREAKDOWN_C2C3)
         L.check(code, "solveBatch", ok=ok)
         return x, st
+
+    def directInfo(self, mesh: int = 0, factor: bool = False) -> dict:
+        """Ordering of mesh `mesh` of the last direct solve (0: the whole system) and, with
+        factor=True, its Cholesky factor L in LAPACK lower band layout ((b+1) x n, row d of
+        column c = L[c+d][c] of the renumbered matrix)."""
+        n = C.c_int64(0)
+        b = C.c_int32(0)
+        kind = C.c_int32(0)
+        L.check(L.load().pf_system_direct_info(self.h, int(mesh), C.byref(n), C.byref(b), C.byref(kind), None,
+                                               None), "directInfo")
+        perm = np.empty(n.value, dtype=np.int32)
+        ab = np.empty((n.value, b.value + 1)) if factor else None
+        L.check(L.load().pf_system_direct_info(self.h, int(mesh), None, None, None, L.ptr(perm, C.c_int32),
+                                               L.ptr(ab, C.c_double)), "directInfo")
+        out = {"n": n.value, "bandwidth": b.value, "ordering": ("natural", "rcm", "x", "y", "z")[kind.value],
+               "perm": perm}
+        if factor:
+            out["factor"] = ab.T
+        return out
+
+    def setDirectKernel(self, mfma: bool = True):
+        """Tile products of the direct factorization: MFMA (default) or vector FMAs."""
+        L.check(L.load().pf_system_direct_kernel(self.h, int(bool(mfma))), "setDirectKernel")
 
 
 def bicgstab_crs(row_ptr, col_idx, val, rhs, x0=None, ctx: L.Context | None = None, **opts):

@@ -3,9 +3,11 @@
 ! (Tutorial/fem/simpleDeformationByFEMSolver.f90).
 !
 !   cube_static --version            library version (no device needed)
-!   cube_static N [out.bin] [mode]   N^3 hex box; mode "elem" assembles with per-element
+!   cube_static N [out.bin] [mode] [algorithm]
+!                                    N^3 hex box; mode "elem" assembles with per-element
 !                                    StiffnessMatrix + setMatrix, "crs" (default) with the
-!                                    device assembly; writes x (real64 stream) to out.bin
+!                                    device assembly; algorithm "GaussJordan" solves directly
+!                                    (default bicgstab_CRS_2); writes x (real64 stream) to out.bin
 program cube_static
    use plantfem_gpu
    use iso_fortran_env, only: int32, int64, real64
@@ -16,7 +18,7 @@ program cube_static
    real(real64), allocatable :: x(:), E(:), v(:)
    real(real64) :: box(6), eps
    integer(int32) :: n, i, u
-   character(len=256) :: arg, outfile, mode
+   character(len=256) :: arg, outfile, mode, algorithm
 
    call get_command_argument(1, arg)
    if (trim(arg) == "--version") then
@@ -30,6 +32,9 @@ program cube_static
    mode = "crs"
    call get_command_argument(3, arg)
    if (len_trim(arg) > 0) mode = arg
+   algorithm = "bicgstab"
+   call get_command_argument(4, arg)
+   if (len_trim(arg) > 0) algorithm = arg
 
    call cube%create("Cube3D", x_num=n, y_num=n, z_num=n)
    allocate (E(cube%ne()), v(cube%ne()))
@@ -65,7 +70,7 @@ program cube_static
 
    solver%relative_er = 1.0d-14
    solver%er0 = 1.0d-30
-   x = solver%solve()
+   x = solver%solve(algorithm=trim(algorithm))
    print '(a,i0,a,i0,a,es12.5,a,es12.5)', "n_dof=", size(x), " iterations=", solver%stats%iterations, &
       " relres=", solver%stats%true_relres, " min_uz=", minval(x(3::3))
    if (len_trim(outfile) > 0) then

@@ -27,7 +27,7 @@ module plantfem_gpu
    integer(int32), parameter, public :: FEMDomain_Overset_GPP = 1, FEMDomain_Overset_P2P = 2
 
    integer(c_int), parameter, public :: PF_OK = 0, PF_BREAKDOWN_VV = 1, PF_BREAKDOWN_C2C3 = 2, &
-                                        PF_NOT_CONVERGED = 3, PF_INVALID = 4
+                                        PF_NOT_CONVERGED = 3, PF_INVALID = 4, PF_NOT_SPD = 5
 
    type, bind(C) :: pf_solve_opts
       integer(c_int32_t) :: itrmax, precision
@@ -198,6 +198,13 @@ module plantfem_gpu
          type(c_ptr), value :: sys
          type(pf_solve_opts), intent(in) :: opts
          type(c_ptr), value :: x0, x_out
+         type(pf_stats), intent(out) :: stats
+      end function
+      integer(c_int) function c_pf_system_solve_direct(sys, refine, x_out, stats) bind(C, name="pf_system_solve_direct")
+         import :: c_int, c_ptr, c_int32_t, pf_stats
+         type(c_ptr), value :: sys
+         integer(c_int32_t), value :: refine
+         type(c_ptr), value :: x_out
          type(pf_stats), intent(out) :: stats
       end function
       integer(c_int) function c_pf_hex8_mass_matrix(ctx, nn, ne, nodcoord, elemnod, density, ndensity, first, &
@@ -1204,8 +1211,11 @@ contains
    ! solve(algorithm, preconditioning, x0) (FEMSolverClass.f90:1697-1825): elimination, then
    ! bicgstab_CRS_2, or with preconditioning="PointJacobi" the Jacobi-scaled system (the
    ! reference calls no solver after the scaling; "PointJacobi-noSolve" does the same), or with
-   ! "ILU"/"incompleteLU" bicgstab_CRS_ILU; x_fixed = ubar.  algorithm is accepted for the
-   ! interface; every algorithm runs BiCGSTAB here.
+   ! "ILU"/"incompleteLU" bicgstab_CRS_ILU; x_fixed = ubar.  Without preconditioning,
+   ! algorithm="GaussJordan" (FEMSolverClass.f90:1795-1798, whose gauss_jordan_crs stops "Under
+   ! construction") is a direct solve: banded Cholesky after a bandwidth-reducing renumbering,
+   ! one step of FP64 refinement; a non-positive pivot stops " A is singular " as the
+   ! reference's pivot test does (:2455).  Other algorithm values run bicgstab_CRS_2.
    function solveFEMSolver(this, algorithm, preconditioning, x0, precision) result(x)
       class(FEMSolver_), intent(inout) :: this
       character(*), optional, intent(in) :: algorithm, preconditioning
@@ -1223,7 +1233,17 @@ contains
       o%relative_er = this%relative_er
       o%debug = merge(1, 0, this%debug)
       if (present(precision)) o%precision = precision
-      if (present(algorithm)) continue
+      if (.not. present(preconditioning) .and. present(algorithm)) then
+         if (algorithm == "GaussJordan") then
+            allocate (xl(this%n))
+            code = c_pf_system_solve_direct(this%sys, 1_c_int32_t, c_loc(xl), this%stats)
+            if (code == PF_NOT_SPD) stop " A is singular "
+            call check(code, "solve")
+            this%CRS_x = xl
+            call move_alloc(xl, x)
+            return
+         end if
+      end if
       if (present(preconditioning)) then
          select case (preconditioning)
          case ("PointJacobi")

@@ -35,11 +35,11 @@ def _oracle_cube(n):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,mode", [(8, "crs"), (5, "elem")])
-def test_fortran_cube_static_matches_oracle(tmp_path, n, mode):
+@pytest.mark.parametrize("n,mode,alg", [(8, "crs", "bicgstab"), (5, "elem", "bicgstab"), (7, "crs", "GaussJordan")])
+def test_fortran_cube_static_matches_oracle(tmp_path, n, mode, alg):
     out = tmp_path / "x.bin"
-    print("fortran cube_static", n, mode, flush=True)
-    res = subprocess.run([str(BIN / "cube_static"), str(n), str(out), mode], capture_output=True, text=True,
+    print("fortran cube_static", n, mode, alg, flush=True)
+    res = subprocess.run([str(BIN / "cube_static"), str(n), str(out), mode, alg], capture_output=True, text=True,
                          timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     x = np.fromfile(out, dtype=np.float64)
