@@ -17,7 +17,7 @@
 //  * factorization: one workgroup per mesh from a largest-first queue (persistent kernel),
 //    left-looking by blocks of 16 columns: the block's (b+16) x 16 rows are updated by the
 //    previous b columns with FP64 MFMA 16x16x4 tiles, the 16 x 16 diagonal tile is factored
-//    with cross-lane operations and the tiles below are solved against it in registers;
+//    with cross-lane operations and the tiles below are multiplied by its inverse transpose;
 //  * triangular solves: blocks of 16, the diagonal solve in one wave, the off-diagonal
 //    update with one thread per row (forward) or column (backward).
 // Fixed-precision iterative refinement on the true residual b - A x (the CRS values, in
@@ -39,6 +39,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int DBT = 512;   // threads per workgroup of the factor/solve kernel
 constexpr int MAXT = 2;    // 16 x 16 tiles a wave holds at once
+constexpr int DBMAX = 1020;  // largest half bandwidth whose block panel (16 x (b+4)) is staged in LDS
 
 struct DMesh {
   int64_t d0;    // first DOF of the mesh in the system
@@ -66,6 +67,14 @@ struct DArgs {
 };
 
 __device__ inline double shfl(double v, int src) { return __shfl(v, src, 64); }
+
+// The lane index, re-read inside a loop so that the compiler rebuilds the lane masks derived
+// from it there instead of hoisting hundreds of them out of the loop and spilling them.
+__device__ inline int lane_id() {
+  int l = (int)(threadIdx.x & 63);
+  asm volatile("" : "+v"(l));
+  return l;
+}
 
 // Cholesky of the 16 x 16 diagonal tile held in the FP64 MFMA accumulator layout (lane l,
 // element v: row 4*v + l/16, column l%16).  Lower part becomes L11; bad is set on a
@@ -119,58 +128,143 @@ __device__ inline void tile_fma(d4 &acc, double af, double bf, int q, int jl) {
   }
 }
 
-// Left-looking blocked band Cholesky of one mesh by the whole workgroup.
-template <int BT, bool MF>
-__device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL, int32_t *sflag) {
+// Fragments of U consecutive 4-column chunks of the previous columns: the B fragment
+// L[j0 + l%16][c + l/16] and, for each of the wave's MAXT tiles, the A fragment
+// L[R + l%16][c + l/16].  Every load has an index clamped into the mesh's band, so the loads
+// are unconditional and stay in flight together; entries outside the band or past the last
+// row are replaced by zero where the fragments are used.
+template <int U, bool LB>
+struct Frags {
+  double b[LB ? 1 : U];
+  double a[U][MAXT];
+};
+
+struct BlockGeom {
+  int j0, b, npad, T, tb, W;
+  int64_t ld;
+};
+
+template <int U, bool LB>
+__device__ inline void load_frags(Frags<U, LB> &f, const double *__restrict__ band, const BlockGeom &g, int c0,
+                                  int q, int jl) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int c = c0 + 4 * u;
+    const int64_t ck = c + q;
+    const bool chunk = c < g.j0;
+    if (!chunk) continue;  // wave-uniform
+    if constexpr (!LB) {
+      const int db = g.j0 + jl - (int)ck;  // >= 1
+      f.b[u] = band[ck * g.ld + min(db, g.b)];
+    }
+#pragma unroll
+    for (int i = 0; i < MAXT; ++i) {
+      const int t = g.tb + g.W * i;
+      const int da = g.j0 + 16 * t + jl - (int)ck;  // >= 1
+      f.a[u][i] = band[ck * g.ld + min(da, g.b)];
+    }
+  }
+}
+
+// LB: the B fragments come from the block panel in LDS, sB[(c - cbase) * 16 + j] = L[j0 + j][c]
+template <bool MF, int U, bool LB>
+__device__ inline void mfma_frags(const Frags<U, LB> &f, d4 *acc, const BlockGeom &g, int c0, int q, int jl,
+                                  const double *sB, int cbase) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int c = c0 + 4 * u;
+    const int ck = c + q;
+    double bf;
+    if constexpr (LB) bf = sB[(ck - cbase) * 16 + jl];
+    else bf = (g.j0 + jl - ck <= g.b) ? f.b[u] : 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXT; ++i) {
+      const int t = g.tb + g.W * i;
+      const int R = g.j0 + 16 * t;
+      if (c < g.j0 && t < g.T && c + 3 >= R - g.b) {  // wave-uniform
+        const int ra = R + jl;
+        const double af = (ra < g.npad && ra - ck <= g.b) ? -f.a[u][i] : 0.0;
+        tile_fma<MF>(acc[i], af, bf, q, jl);
+      }
+    }
+  }
+}
+
+// Left-looking blocked band Cholesky of one mesh by the whole workgroup.  Per block of 16
+// columns j0..j0+15: every wave updates its 16 x 16 tiles of rows j0..j0+15+b with the
+// previous b columns (MFMA; loads of the next group of chunks overlap the products of the
+// current one), wave 0 factors the diagonal tile (L11) and forms L11^-T, and every wave
+// turns its tiles below into L21 = C L11^-T with four products per tile.
+template <int BT, bool MF, bool LB>
+__device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL, double *sLi, double *sC,
+                           double *sB, int32_t *sflag) {
   constexpr int W = BT / 64;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int q = lane >> 4, jl = lane & 15;
+  constexpr int U = MF ? (LB ? 4 : 6) : 4;
+  const int wv = threadIdx.x >> 6;
   const int b = M.b, npad = M.npad;
   const int64_t ld = M.ld;
+  double *myC = sC + 256 * wv;
   if (threadIdx.x == 0) *sflag = 0;
   __syncthreads();
-  double Lrow[16];
   for (int j0 = 0; j0 < npad; j0 += 16) {
+    const int lane = lane_id();
+    const int q = lane >> 4, jl = lane & 15;
     const int rows = min(16 + b, npad - j0);  // rows j0 .. j0+rows-1 hold the block's nonzeros
     const int T = (rows + 15) >> 4;
     const int P = (T + W * MAXT - 1) / (W * MAXT);
+    const int cbase = max(0, j0 - b) & ~3;  // first column of the block panel
+    double bL[4];  // B fragments of L11^-T (set in pass 0)
     for (int p = 0; p < P; ++p) {
-      const int tb = wv + W * MAXT * p;  // this wave's tiles: tb, tb + W, ..., tb + (MAXT-1) W
+      BlockGeom g{j0, b, npad, T, wv + W * MAXT * p, W, ld};
       d4 acc[MAXT];
 #pragma unroll
       for (int i = 0; i < MAXT; ++i) {
-        const int t = tb + W * i;
+        const int t = g.tb + W * i;
         const int64_t cc = j0 + jl;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = j0 + 16 * t + 4 * v + q;
           const int d = r - (int)cc;
-          acc[i][v] = (t < T && r < npad && d >= 0 && d <= b) ? band[cc * ld + d] : 0.0;
+          acc[i][v] = band[cc * ld + min(max(d, 0), b)];  // clamped: always inside the band
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = j0 + 16 * t + 4 * v + q;
+          const int d = r - (int)cc;
+          if (!(t < T && r < npad && d >= 0 && d <= b)) acc[i][v] = 0.0;
         }
       }
-      if (tb < T) {
-        // A[block rows, j0..j0+15] -= L[block rows, c] L[j0..j0+15, c]^T over the previous columns
-        const int clo = max(0, j0 + 16 * tb - b) & ~3;
-        for (int c = clo; c < j0; c += 4) {
-          const int64_t ck = c + q;
-          const int db = j0 + jl - (int)ck;
-          const double bf = db <= b ? band[ck * ld + db] : 0.0;
-#pragma unroll
-          for (int i = 0; i < MAXT; ++i) {
-            const int t = tb + W * i;
-            const int R = j0 + 16 * t;
-            if (t < T && c + 3 >= R - b) {
-              const int ra = R + jl;
-              const int da = ra - (int)ck;
-              const double af = (ra < npad && da <= b) ? -band[ck * ld + da] : 0.0;
-              tile_fma<MF>(acc[i], af, bf, q, jl);
-            }
-          }
+      if (LB && p == 0) {
+        // block panel L[j0 .. j0+15][cbase .. j0-1] into LDS (zero outside the band)
+        const int cnt = (j0 - cbase) * 16;
+        for (int e = threadIdx.x; e < cnt; e += BT) {
+          const int c = cbase + (e >> 4), j = e & 15;
+          const int d = j0 + j - c;  // >= 1
+          const double v = band[(int64_t)c * ld + min(d, b)];
+          sB[e] = d <= b ? v : 0.0;
+        }
+        __syncthreads();
+      }
+      if (g.tb < T) {
+        // A[block rows, j0..j0+15] -= L[block rows, c] L[j0..j0+15, c]^T, c < j0
+        const int clo = max(0, j0 + 16 * g.tb - b) & ~3;
+        Frags<U, LB> f0, f1;
+        int c = clo;
+        if (c < j0) load_frags<U, LB>(f0, band, g, c, q, jl);
+        for (; c < j0; c += 8 * U) {
+          const int c1 = c + 4 * U, c2 = c1 + 4 * U;
+          if (c1 < j0) load_frags<U, LB>(f1, band, g, c1, q, jl);
+          mfma_frags<MF, U, LB>(f0, acc, g, c, q, jl, sB, cbase);
+          if (c2 < j0) load_frags<U, LB>(f0, band, g, c2, q, jl);
+          if (c1 < j0) mfma_frags<MF, U, LB>(f1, acc, g, c1, q, jl, sB, cbase);
         }
       }
       if (p == 0) {
-        // diagonal tile (tile 0 of wave 0): L11, to LDS and the band
+        // diagonal tile (tile 0 of wave 0): L11 to the band and LDS, then L11^-T, the
+        // solution X of X L11^T = I column by column across the lanes, to LDS
         if (wv == 0) {
+          const int ln = lane_id();
+          const int q = ln >> 4, jl = ln & 15;
           bool bad = false;
           chol16(acc[0], q, jl, bad);
 #pragma unroll
@@ -180,38 +274,58 @@ __device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL
             const int d = i - jl;
             if (d >= 0 && d <= b) band[(int64_t)(j0 + jl) * ld + d] = acc[0][v];
           }
-          if (bad && lane == 0) *sflag = 1;
-        }
-        __syncthreads();
-        if (*sflag) return PF_NOT_SPD;
+          double Lrow[16];  // L11[jl][k]
 #pragma unroll
-        for (int k = 0; k < 16; ++k) Lrow[k] = sL[jl * 16 + k];
-      }
-      // tiles below the diagonal: X L11^T = C, column by column across the lanes
+          for (int k = 0; k < 16; ++k) Lrow[k] = sL[jl * 16 + k];
+          d4 x;
 #pragma unroll
-      for (int i = 0; i < MAXT; ++i) {
-        const int t = tb + W * i;
-        if (t >= 1 && t < T) {
+          for (int v = 0; v < 4; ++v) x[v] = (4 * v + q == jl) ? 1.0 : 0.0;
 #pragma unroll
           for (int k = 0; k < 16; ++k) {
             if (jl == k) {
-              const double rk = 1.0 / sL[k * 16 + k];
+              const double rk = 1.0 / Lrow[k];
 #pragma unroll
-              for (int v = 0; v < 4; ++v) acc[i][v] *= rk;
+              for (int v = 0; v < 4; ++v) x[v] *= rk;
             }
             double xk[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) xk[v] = shfl(acc[i][v], 16 * q + k);
+            for (int v = 0; v < 4; ++v) xk[v] = shfl(x[v], 16 * q + k);
             if (jl > k) {
 #pragma unroll
-              for (int v = 0; v < 4; ++v) acc[i][v] -= xk[v] * Lrow[k];
+              for (int v = 0; v < 4; ++v) x[v] -= xk[v] * Lrow[k];
             }
           }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) sLi[(4 * v + q) * 16 + jl] = x[v];
+          if (bad && ln == 0) *sflag = 1;
+        }
+        __syncthreads();
+        if (*sflag) return PF_NOT_SPD;
+        // B fragment of chunk cq of the product X = C L11^-T: B[kk][n] = L11^-T[4cq + kk][n]
+#pragma unroll
+        for (int cq = 0; cq < 4; ++cq) bL[cq] = sLi[(4 * cq + q) * 16 + jl];
+      }
+      // tiles below the diagonal: X = C L11^-T, 4 products per tile; C goes through the
+      // wave's LDS tile from the accumulator layout into A fragments
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i) {
+        const int t = g.tb + W * i;
+        if (t >= 1 && t < T) {
+          const int ln = lane_id();
+          const int q = ln >> 4, jl = ln & 15;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) myC[(4 * v + q) * 16 + jl] = acc[i][v];
+          double af[4];
+#pragma unroll
+          for (int cq = 0; cq < 4; ++cq) af[cq] = myC[jl * 16 + 4 * cq + q];
+          d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int cq = 0; cq < 4; ++cq) tile_fma<MF>(x, af[cq], bL[cq], q, jl);
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int r = j0 + 16 * t + 4 * v + q;
             const int d = r - (j0 + jl);
-            if (r < npad && d <= b) band[(int64_t)(j0 + jl) * ld + d] = acc[i][v];
+            if (r < npad && d <= b) band[(int64_t)(j0 + jl) * ld + d] = x[v];
           }
         }
       }
@@ -221,21 +335,30 @@ __device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL
   return PF_OK;
 }
 
-// L L^T z = w in place (w: npad entries of the mesh's work vector).
+// L L^T z = w in place (w: npad entries of the mesh's work vector).  The 16 x 16 diagonal
+// solves run in wave 0 with the block's L entries loaded before the serial steps.
 template <int BT>
 __device__ void band_solve(const DMesh &M, const double *__restrict__ band, double *__restrict__ w, double *sY) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wv = threadIdx.x >> 6;
   const int b = M.b, npad = M.npad;
   const int64_t ld = M.ld;
   // forward: L y = w
   for (int j0 = 0; j0 < npad; j0 += 16) {
     if (wv == 0) {
+      const int lane = lane_id();
+      const int jr = lane & 15;
+      double Lr[16];  // row jr of the diagonal block: L[j0+jr][j0+k]
+#pragma unroll
+      for (int k = 0; k < 16; ++k) Lr[k] = band[(int64_t)(j0 + k) * ld + min(max(jr - k, 0), b)];
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (!(k <= jr && jr - k <= b)) Lr[k] = 0.0;
       double acc = lane < 16 ? w[j0 + lane] : 0.0;
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        if (lane == k) acc /= band[(int64_t)(j0 + k) * ld];
+        if (lane == k) acc /= Lr[k];
         const double yk = shfl(acc, k);
-        if (lane > k && lane < 16 && lane - k <= b) acc -= band[(int64_t)(j0 + k) * ld + (lane - k)] * yk;
+        if (lane > k && lane < 16) acc -= Lr[k] * yk;
       }
       if (lane < 16) {
         w[j0 + lane] = acc;
@@ -245,12 +368,13 @@ __device__ void band_solve(const DMesh &M, const double *__restrict__ band, doub
     __syncthreads();
     const int rend = min(npad, j0 + 16 + b);
     for (int r = j0 + 16 + (int)threadIdx.x; r < rend; r += BT) {
+      double l[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) l[j] = band[(int64_t)(j0 + j) * ld + min(r - j0 - j, b)];  // d >= 1
       double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int d = r - j0 - j;
-        if (d <= b) s += band[(int64_t)(j0 + j) * ld + d] * sY[j];
-      }
+      for (int j = 0; j < 16; ++j)
+        if (r - j0 - j <= b) s += l[j] * sY[j];
       w[r] -= s;
     }
     __syncthreads();
@@ -258,12 +382,20 @@ __device__ void band_solve(const DMesh &M, const double *__restrict__ band, doub
   // backward: L^T z = y
   for (int j0 = npad - 16; j0 >= 0; j0 -= 16) {
     if (wv == 0) {
+      const int lane = lane_id();
+      const int jr = lane & 15;
+      double Lc[16];  // column jr of the diagonal block: L[j0+k][j0+jr]
+#pragma unroll
+      for (int k = 0; k < 16; ++k) Lc[k] = band[(int64_t)(j0 + jr) * ld + min(max(k - jr, 0), b)];
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (!(k >= jr && k - jr <= b)) Lc[k] = 0.0;
       double acc = lane < 16 ? w[j0 + lane] : 0.0;
 #pragma unroll
       for (int k = 15; k >= 0; --k) {
-        if (lane == k) acc /= band[(int64_t)(j0 + k) * ld];
+        if (lane == k) acc /= Lc[k];
         const double xk = shfl(acc, k);
-        if (lane < k && k - lane <= b) acc -= band[(int64_t)(j0 + lane) * ld + (k - lane)] * xk;
+        if (lane < k) acc -= Lc[k] * xk;
       }
       if (lane < 16) {
         w[j0 + lane] = acc;
@@ -273,12 +405,13 @@ __device__ void band_solve(const DMesh &M, const double *__restrict__ band, doub
     __syncthreads();
     const int ibeg = max(0, j0 - b);
     for (int i = ibeg + (int)threadIdx.x; i < j0; i += BT) {
+      double l[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) l[j] = band[(int64_t)i * ld + min(j0 + j - i, b)];  // d >= 1
       double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int d = j0 + j - i;
-        if (d <= b) s += band[(int64_t)i * ld + d] * sY[j];
-      }
+      for (int j = 0; j < 16; ++j)
+        if (j0 + j - i <= b) s += l[j] * sY[j];
       w[i] -= s;
     }
     __syncthreads();
@@ -293,9 +426,12 @@ __global__ void k_flag_clear(int32_t *f) {
   if (threadIdx.x == 0 && blockIdx.x == 0) f[0] = 0;
 }
 
-template <int BT, bool MF>
+template <int BT, bool MF, bool LB>
 __global__ void __launch_bounds__(BT) k_direct(DArgs a) {
+  __shared__ double sB[LB ? 16 * (DBMAX + 4) : 1];
   __shared__ double sL[256];
+  __shared__ double sLi[256];
+  __shared__ double sC[256 * (BT / 64)];
   __shared__ double sY[16];
   __shared__ int32_t next, sflag;
   for (;;) {
@@ -308,7 +444,7 @@ __global__ void __launch_bounds__(BT) k_direct(DArgs a) {
     const DMesh M = a.mesh[m];
     const int64_t t0 = wall_clock64();
     double *band = a.band + M.boff;
-    int st = a.mode == 0 ? factor_mesh<BT, MF>(M, band, sL, &sflag) : a.status[m];
+    int st = a.mode == 0 ? factor_mesh<BT, MF, LB>(M, band, sL, sLi, sC, sB, &sflag) : a.status[m];
     double *w = a.w + M.woff;
     const int32_t *perm = a.perm + M.woff;
     if (st == PF_OK) {
@@ -586,6 +722,7 @@ struct DirectWork {
   DevBuf<int64_t> t_raw;
   DevBuf<double> band, w, r, y, norms;
   int64_t band_entries = 0, work_entries = 0;
+  int32_t max_b = 0;  // largest half bandwidth of the meshes
   double analyse_ms = 0.0;
   // factor held in band: the matrix version it was computed from (0: none)
   uint64_t factor_version = 0;
@@ -756,6 +893,8 @@ int pf_direct_analyse(pf_ctx *ctx, DirectWork **pd, int64_t n, const int64_t *ro
   PF_HIP(hipStreamSynchronize(ctx->stream));
   D->band_entries = boff;
   D->work_entries = woff;
+  D->max_b = 0;
+  for (const DMesh &M : D->hmesh) D->max_b = std::max(D->max_b, M.b);
   D->group = g;
   D->n = n;
   D->dof_off.assign(dof_off, dof_off + nmesh + 1);
@@ -785,10 +924,16 @@ int launch_direct(pf_ctx *ctx, DirectWork *D, int mode, const double *rhs, doubl
   const int grid = std::max(1, std::min(D->nmesh, 2 * ctx->num_cus));
   k_direct_queue_reset<<<1, 64, 0, ctx->stream>>>(D->queue.p);
   PF_HIP(hipGetLastError());
-  if (mfma)
-    k_direct<DBT, true><<<grid, DBT, 0, ctx->stream>>>(a);
+  // the block panel goes through LDS when every mesh's bandwidth fits it
+  const bool lb = D->max_b <= DBMAX;
+  if (mfma && lb)
+    k_direct<DBT, true, true><<<grid, DBT, 0, ctx->stream>>>(a);
+  else if (mfma)
+    k_direct<DBT, true, false><<<grid, DBT, 0, ctx->stream>>>(a);
+  else if (lb)
+    k_direct<DBT, false, true><<<grid, DBT, 0, ctx->stream>>>(a);
   else
-    k_direct<DBT, false><<<grid, DBT, 0, ctx->stream>>>(a);
+    k_direct<DBT, false, false><<<grid, DBT, 0, ctx->stream>>>(a);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
