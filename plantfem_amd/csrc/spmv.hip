@@ -143,32 +143,53 @@ __global__ void k_sell3_vals(SliceNodes sn, int64_t nslices, int64_t ncells, con
   }
 }
 
-// s_i = 1/sqrt(|a_ii|) (1 where the diagonal is missing or zero)
-__global__ void k_jacobi_scale(int64_t n, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
-                               const double *__restrict__ val, double *__restrict__ scale) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double d = 0.0;
-  for (int64_t k = row_ptr[i] - 1; k < row_ptr[i + 1] - 1; ++k)
-    if (col_idx[k] == i + 1) {
-      d = val[k];
-      break;
-    }
+// Position of the diagonal entry of row i (the first one in storage order), or -1: a SUB-lane
+// subgroup per row, each lane its strided share of the row, then the smallest position found.
+template <int SUB>
+__device__ inline int64_t diag_pos(int64_t i, bool live, const int64_t *__restrict__ row_ptr,
+                                   const int32_t *__restrict__ col_idx, int lane) {
+  int64_t kmin = INT64_MAX;
+  if (live) {
+    const int64_t e = row_ptr[i + 1] - 1;
+    for (int64_t k = row_ptr[i] - 1 + lane; k < e; k += SUB)
+      if (col_idx[k] == i + 1) {
+        kmin = k;
+        break;
+      }
+  }
+#pragma unroll
+  for (int o = SUB / 2; o > 0; o >>= 1) {
+    const int64_t other = __shfl_xor(kmin, o, SUB);
+    kmin = other < kmin ? other : kmin;
+  }
+  return kmin == INT64_MAX ? -1 : kmin;
+}
+
+// s_i = 1/sqrt|a_ii| (1 where the diagonal is missing or zero)
+template <int SUB>
+__global__ void __launch_bounds__(256) k_jacobi_scale(int64_t n, const int64_t *__restrict__ row_ptr,
+                                                      const int32_t *__restrict__ col_idx,
+                                                      const double *__restrict__ val, double *__restrict__ scale) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = g / SUB;
+  const int lane = (int)(g % SUB);
+  const int64_t k = diag_pos<SUB>(i, i < n, row_ptr, col_idx, lane);
+  if (i >= n || lane != 0) return;
+  const double d = k >= 0 ? val[k] : 0.0;
   scale[i] = d != 0.0 ? 1.0 / sqrt(fabs(d)) : 1.0;
 }
 
 // d_i = 1/a_ii (1 where the diagonal is missing or zero)
-__global__ void k_jacobi_row_scale(int64_t n, const int64_t *__restrict__ row_ptr,
-                                   const int32_t *__restrict__ col_idx, const double *__restrict__ val,
-                                   double *__restrict__ scale) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double d = 0.0;
-  for (int64_t k = row_ptr[i] - 1; k < row_ptr[i + 1] - 1; ++k)
-    if (col_idx[k] == i + 1) {
-      d = val[k];
-      break;
-    }
+template <int SUB>
+__global__ void __launch_bounds__(256) k_jacobi_row_scale(int64_t n, const int64_t *__restrict__ row_ptr,
+                                                          const int32_t *__restrict__ col_idx,
+                                                          const double *__restrict__ val, double *__restrict__ scale) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = g / SUB;
+  const int lane = (int)(g % SUB);
+  const int64_t k = diag_pos<SUB>(i, i < n, row_ptr, col_idx, lane);
+  if (i >= n || lane != 0) return;
+  const double d = k >= 0 ? val[k] : 0.0;
   scale[i] = d != 0.0 ? 1.0 / d : 1.0;
 }
 
@@ -241,7 +262,7 @@ int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, con
 int pf_jacobi_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
                     double *scale) {
   if (n <= 0) return PF_OK;
-  k_jacobi_scale<<<pf_blocks(n, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, scale);
+  k_jacobi_scale<16><<<pf_blocks(n * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, scale);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -249,7 +270,8 @@ int pf_jacobi_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_
 int pf_jacobi_row_scale(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *val,
                         double *scale) {
   if (n <= 0) return PF_OK;
-  k_jacobi_row_scale<<<pf_blocks(n, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val, scale);
+  k_jacobi_row_scale<16><<<pf_blocks(n * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(n, row_ptr, col_idx, val,
+                                                                                     scale);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
