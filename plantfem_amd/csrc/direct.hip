@@ -68,6 +68,12 @@ struct DArgs {
 
 __device__ inline double shfl(double v, int src) { return __shfl(v, src, 64); }
 
+// The wave index within the workgroup as a scalar: the compiler cannot tell that
+// threadIdx.x / 64 is the same in every lane of a wave, and without this every tile index,
+// chunk bound and loop counter derived from it would be kept per lane, with masked branches
+// and full waits for outstanding loads where they join.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // The lane index, re-read inside a loop so that the compiler rebuilds the lane masks derived
 // from it there instead of hoisting hundreds of them out of the loop and spilling them.
 __device__ inline int lane_id() {
@@ -130,9 +136,10 @@ __device__ inline void tile_fma(d4 &acc, double af, double bf, int q, int jl) {
 
 // Fragments of U consecutive 4-column chunks of the previous columns: the B fragment
 // L[j0 + l%16][c + l/16] and, for each of the wave's MAXT tiles, the A fragment
-// L[R + l%16][c + l/16].  Every load has an index clamped into the mesh's band, so the loads
-// are unconditional and stay in flight together; entries outside the band or past the last
-// row are replaced by zero where the fragments are used.
+// L[R + l%16][c + l/16].  This is the form for the first chunks of a tile (where some lanes'
+// entries lie outside the band) and the last rows: every load has an index clamped into the
+// band, so the loads are unconditional and stay in flight together, and the entries outside
+// the band are replaced by zero where they are used.  Chunks c0 + 4u < cend are loaded.
 template <int U, bool LB>
 struct Frags {
   double b[LB ? 1 : U];
@@ -146,13 +153,12 @@ struct BlockGeom {
 
 template <int U, bool LB>
 __device__ inline void load_frags(Frags<U, LB> &f, const double *__restrict__ band, const BlockGeom &g, int c0,
-                                  int q, int jl) {
+                                  int cend, int q, int jl) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int c = c0 + 4 * u;
     const int64_t ck = c + q;
-    const bool chunk = c < g.j0;
-    if (!chunk) continue;  // wave-uniform
+    if (c >= cend) continue;  // wave-uniform
     if constexpr (!LB) {
       const int db = g.j0 + jl - (int)ck;  // >= 1
       f.b[u] = band[ck * g.ld + min(db, g.b)];
@@ -166,27 +172,78 @@ __device__ inline void load_frags(Frags<U, LB> &f, const double *__restrict__ ba
   }
 }
 
-// LB: the B fragments come from the block panel in LDS, sB[(c - cbase) * 16 + j] = L[j0 + j][c]
+// acc[i] += A (-B) over the group's chunks.  LB: the B fragments come from the block panel in
+// LDS, stored negated: sB[(c - cbase) * 16 + j] = -L[j0 + j][c].
 template <bool MF, int U, bool LB>
-__device__ inline void mfma_frags(const Frags<U, LB> &f, d4 *acc, const BlockGeom &g, int c0, int q, int jl,
-                                  const double *sB, int cbase) {
+__device__ inline void mfma_frags(const Frags<U, LB> &f, d4 *acc, const BlockGeom &g, int c0, int cend, int q,
+                                  int jl, const double *sB, int cbase) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int c = c0 + 4 * u;
+    if (c >= cend) continue;  // wave-uniform
     const int ck = c + q;
-    double bf;
-    if constexpr (LB) bf = sB[(ck - cbase) * 16 + jl];
-    else bf = (g.j0 + jl - ck <= g.b) ? f.b[u] : 0.0;
+    double bn;
+    if constexpr (LB) bn = sB[(ck - cbase) * 16 + jl];
+    else bn = (g.j0 + jl - ck <= g.b) ? -f.b[u] : 0.0;
 #pragma unroll
     for (int i = 0; i < MAXT; ++i) {
       const int t = g.tb + g.W * i;
       const int R = g.j0 + 16 * t;
-      if (c < g.j0 && t < g.T && c + 3 >= R - g.b) {  // wave-uniform
+      if (t < g.T && c + 3 >= R - g.b) {  // wave-uniform
         const int ra = R + jl;
-        const double af = (ra < g.npad && ra - ck <= g.b) ? -f.a[u][i] : 0.0;
-        tile_fma<MF>(acc[i], af, bf, q, jl);
+        const double af = (ra < g.npad && ra - ck <= g.b) ? f.a[u][i] : 0.0;
+        tile_fma<MF>(acc[i], af, bn, q, jl);
       }
     }
+  }
+}
+
+// The steady part of a pass: chunks cs <= c < j0, where every lane's entry of each of the
+// wave's nt existing tiles lies inside the band and the tiles' rows end before the padding.
+// Branch-free: per tile a pointer that advances by 4 (ld - 1) doubles per chunk, the B
+// fragment (negated) from the LDS panel or, without it, from the band (also inside the band
+// there), two groups of chunks in flight.
+template <bool MF, int U, bool LB>
+__device__ inline void steady_update(d4 *acc, const double *__restrict__ band, const BlockGeom &g, int cs, int nt,
+                                     int q, int jl, const double *sB, int cbase) {
+  const int64_t step = g.ld - 1;
+  const double *pa[MAXT];
+#pragma unroll
+  for (int i = 0; i < MAXT; ++i) pa[i] = band + (int64_t)(cs + q) * step + g.j0 + 16 * (g.tb + g.W * i) + jl;
+  const double *pb = band + (int64_t)(cs + q) * step + g.j0 + jl;  // B fragment without the LDS panel
+  const double *lb = sB + (cs + q - cbase) * 16 + jl;
+  double a0[U][MAXT], a1[U][MAXT], b0[LB ? 1 : U], b1[LB ? 1 : U];
+  auto load = [&](double (&a)[U][MAXT], double (&bb)[LB ? 1 : U], int c) {
+    const int64_t off = (int64_t)(c - cs) * step;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (c + 4 * u >= g.j0) continue;  // wave-uniform
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i)
+        if (i < nt) a[u][i] = pa[i][off + (int64_t)(4 * u) * step];
+      if constexpr (!LB) bb[u] = pb[off + (int64_t)(4 * u) * step];
+    }
+  };
+  auto prod = [&](const double (&a)[U][MAXT], const double (&bb)[LB ? 1 : U], int c) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (c + 4 * u >= g.j0) continue;  // wave-uniform
+      double bn;
+      if constexpr (LB) bn = lb[(c - cs + 4 * u) * 16];
+      else bn = -bb[u];
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i)
+        if (i < nt) tile_fma<MF>(acc[i], a[u][i], bn, q, jl);
+    }
+  };
+  int c = cs;
+  if (c < g.j0) load(a0, b0, c);
+  for (; c < g.j0; c += 8 * U) {
+    const int c1 = c + 4 * U, c2 = c1 + 4 * U;
+    if (c1 < g.j0) load(a1, b1, c1);
+    prod(a0, b0, c);
+    if (c2 < g.j0) load(a0, b0, c2);
+    if (c1 < g.j0) prod(a1, b1, c1);
   }
 }
 
@@ -199,8 +256,9 @@ template <int BT, bool MF, bool LB>
 __device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL, double *sLi, double *sC,
                            double *sB, int32_t *sflag) {
   constexpr int W = BT / 64;
-  constexpr int U = MF ? (LB ? 4 : 6) : 4;
-  const int wv = threadIdx.x >> 6;
+  constexpr int U = 4;   // chunks per group of the steady part
+  constexpr int UR = 4;                      // chunks per group of the ramp
+  const int wv = wave_id();
   const int b = M.b, npad = M.npad;
   const int64_t ld = M.ld;
   double *myC = sC + 256 * wv;
@@ -235,29 +293,30 @@ __device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL
         }
       }
       if (LB && p == 0) {
-        // block panel L[j0 .. j0+15][cbase .. j0-1] into LDS (zero outside the band)
+        // block panel -L[j0 .. j0+15][cbase .. j0-1] into LDS (zero outside the band)
         const int cnt = (j0 - cbase) * 16;
         for (int e = threadIdx.x; e < cnt; e += BT) {
           const int c = cbase + (e >> 4), j = e & 15;
           const int d = j0 + j - c;  // >= 1
           const double v = band[(int64_t)c * ld + min(d, b)];
-          sB[e] = d <= b ? v : 0.0;
+          sB[e] = d <= b ? -v : 0.0;
         }
         __syncthreads();
       }
       if (g.tb < T) {
-        // A[block rows, j0..j0+15] -= L[block rows, c] L[j0..j0+15, c]^T, c < j0
+        // A[block rows, j0..j0+15] -= L[block rows, c] L[j0..j0+15, c]^T, c < j0: a ramp with
+        // predicates up to cs, where the wave's last existing tile is inside the band
+        // everywhere, then the branch-free steady part
         const int clo = max(0, j0 + 16 * g.tb - b) & ~3;
-        Frags<U, LB> f0, f1;
-        int c = clo;
-        if (c < j0) load_frags<U, LB>(f0, band, g, c, q, jl);
-        for (; c < j0; c += 8 * U) {
-          const int c1 = c + 4 * U, c2 = c1 + 4 * U;
-          if (c1 < j0) load_frags<U, LB>(f1, band, g, c1, q, jl);
-          mfma_frags<MF, U, LB>(f0, acc, g, c, q, jl, sB, cbase);
-          if (c2 < j0) load_frags<U, LB>(f0, band, g, c2, q, jl);
-          if (c1 < j0) mfma_frags<MF, U, LB>(f1, acc, g, c1, q, jl, sB, cbase);
+        const int nt = min(MAXT, (T - 1 - g.tb) / W + 1);  // existing tiles of this wave
+        const int Rl = j0 + 16 * (g.tb + W * (nt - 1));
+        const int cs = Rl + 16 <= npad ? min(j0, max(clo, (Rl + 15 - b + 3) & ~3)) : j0;
+        for (int c = clo; c < cs; c += 4 * UR) {
+          Frags<UR, LB> f;
+          load_frags<UR, LB>(f, band, g, c, cs, q, jl);
+          mfma_frags<MF, UR, LB>(f, acc, g, c, cs, q, jl, sB, cbase);
         }
+        steady_update<MF, U, LB>(acc, band, g, cs, nt, q, jl, sB, cbase);
       }
       if (p == 0) {
         // diagonal tile (tile 0 of wave 0): L11 to the band and LDS, then L11^-T, the
@@ -339,7 +398,7 @@ __device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL
 // solves run in wave 0 with the block's L entries loaded before the serial steps.
 template <int BT>
 __device__ void band_solve(const DMesh &M, const double *__restrict__ band, double *__restrict__ w, double *sY) {
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_id();
   const int b = M.b, npad = M.npad;
   const int64_t ld = M.ld;
   // forward: L y = w
@@ -527,7 +586,7 @@ __global__ void __launch_bounds__(256) k_direct_resid(int32_t nmesh, const DMesh
     t0 += __shfl_xor(t0, o, 64);
     t1 += __shfl_xor(t1, o, 64);
   }
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_id();
   if ((threadIdx.x & 63) == 0) {
     sh[0][wv] = t0;
     sh[1][wv] = t1;
@@ -924,8 +983,9 @@ int launch_direct(pf_ctx *ctx, DirectWork *D, int mode, const double *rhs, doubl
   const int grid = std::max(1, std::min(D->nmesh, 2 * ctx->num_cus));
   k_direct_queue_reset<<<1, 64, 0, ctx->stream>>>(D->queue.p);
   PF_HIP(hipGetLastError());
-  // the block panel goes through LDS when every mesh's bandwidth fits it
-  const bool lb = D->max_b <= DBMAX;
+  // the block panel through LDS is kept for reference but not used: under the memory
+  // contention of a full batch its per-block fill costs more than it saves
+  const bool lb = false;
   if (mfma && lb)
     k_direct<DBT, true, true><<<grid, DBT, 0, ctx->stream>>>(a);
   else if (mfma)
