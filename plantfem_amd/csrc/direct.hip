@@ -12,12 +12,13 @@
 //    renumbered by the candidate of smallest bandwidth among the natural order, reverse
 //    Cuthill-McKee and lexicographic coordinate sorts along each axis;
 //  * band storage (LAPACK dpbtrf 'L' layout): entry (r, c), 0 <= r - c <= b, of the permuted
-//    matrix at band[c*ld + r - c], ld >= b + 1; the order is padded to a multiple of 16 with
+//    matrix at band[c*ld + r - c], ld >= b + 1; the order is padded to a multiple of 32 with
 //    unit diagonal rows;
 //  * factorization: one workgroup per mesh from a largest-first queue (persistent kernel),
-//    left-looking by blocks of 16 columns: the block's (b+16) x 16 rows are updated by the
-//    previous b columns with FP64 MFMA 16x16x4 tiles, the 16 x 16 diagonal tile is factored
-//    with cross-lane operations and the tiles below are multiplied by its inverse transpose;
+//    left-looking by blocks of 32 columns: the block's (b+32) x 32 rows are updated by the
+//    previous b columns with FP64 MFMA 16x16x4 tiles, the two 16 x 16 diagonal tiles are
+//    factored with cross-lane operations and the tiles below are multiplied by their inverse
+//    transposes;
 //  * triangular solves: blocks of 16, the diagonal solve in one wave, the off-diagonal
 //    update with one thread per row (forward) or column (backward).
 // Fixed-precision iterative refinement on the true residual b - A x (the CRS values, in
@@ -38,15 +39,14 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int DBT = 512;   // threads per workgroup of the factor/solve kernel
-constexpr int MAXT = 2;    // 16 x 16 tiles a wave holds at once
-constexpr int DBMAX = 1020;  // largest half bandwidth whose block panel (16 x (b+4)) is staged in LDS
+constexpr int MAXT = 2;    // row tiles a wave holds at once (each 16 rows x 32 columns)
 
 struct DMesh {
   int64_t d0;    // first DOF of the mesh in the system
   int64_t boff;  // first band entry
   int64_t woff;  // first entry of the mesh's permutation and work vector
   int32_t n;     // DOFs
-  int32_t npad;  // n rounded up to a multiple of 16
+  int32_t npad;  // n rounded up to a multiple of 32
   int32_t b;     // half bandwidth of the permuted matrix (DOFs)
   int32_t ld;    // leading dimension of the band (>= b + 1, even)
 };
@@ -134,57 +134,60 @@ __device__ inline void tile_fma(d4 &acc, double af, double bf, int q, int jl) {
   }
 }
 
-// Fragments of U consecutive 4-column chunks of the previous columns: the B fragment
-// L[j0 + l%16][c + l/16] and, for each of the wave's MAXT tiles, the A fragment
-// L[R + l%16][c + l/16].  This is the form for the first chunks of a tile (where some lanes'
-// entries lie outside the band) and the last rows: every load has an index clamped into the
-// band, so the loads are unconditional and stay in flight together, and the entries outside
-// the band are replaced by zero where they are used.  Chunks c0 + 4u < cend are loaded.
-template <int U, bool LB>
-struct Frags {
-  double b[LB ? 1 : U];
-  double a[U][MAXT];
-};
-
+// Geometry of one block of 32 columns j0 .. j0+31 and of a wave's tiles in one pass: row
+// tiles t (rows R = j0 + 16 t .. R+15, t < T) of the rows j0 .. j0+31+b, column tiles
+// ct = 0 (columns j0 .. j0+15) and ct = 1 (j0+16 .. j0+31); the wave holds tiles
+// tb, tb + W, ..., tb + (MAXT-1) W.
 struct BlockGeom {
   int j0, b, npad, T, tb, W;
   int64_t ld;
 };
 
-template <int U, bool LB>
-__device__ inline void load_frags(Frags<U, LB> &f, const double *__restrict__ band, const BlockGeom &g, int c0,
+// Fragments of U consecutive 4-column chunks c0, c0+4, ... of the previous columns: A
+// fragments L[R + l%16][c + l/16] of the wave's tiles and B fragments L[j0 + 16 ct + l%16]
+// [c + l/16] of both column tiles.  This is the form for the first chunks of a tile (where
+// some lanes' entries lie outside the band) and the last rows: every load has an index
+// clamped into the band, so the loads are unconditional and stay in flight together, and the
+// entries outside the band are replaced by zero where they are used.  Chunks below cend.
+template <int U>
+struct Frags {
+  double b[U][2];
+  double a[U][MAXT];
+};
+
+template <int U>
+__device__ inline void load_frags(Frags<U> &f, const double *__restrict__ band, const BlockGeom &g, int c0,
                                   int cend, int q, int jl) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int c = c0 + 4 * u;
-    const int64_t ck = c + q;
     if (c >= cend) continue;  // wave-uniform
-    if constexpr (!LB) {
-      const int db = g.j0 + jl - (int)ck;  // >= 1
-      f.b[u] = band[ck * g.ld + min(db, g.b)];
+    const int64_t ck = c + q;
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const int db = g.j0 + 16 * ct + jl - (int)ck;  // >= 1
+      f.b[u][ct] = band[ck * g.ld + min(db, g.b)];
     }
 #pragma unroll
     for (int i = 0; i < MAXT; ++i) {
-      const int t = g.tb + g.W * i;
-      const int da = g.j0 + 16 * t + jl - (int)ck;  // >= 1
+      const int da = g.j0 + 16 * (g.tb + g.W * i) + jl - (int)ck;  // >= 1
       f.a[u][i] = band[ck * g.ld + min(da, g.b)];
     }
   }
 }
 
-// acc[i] += A (-B) over the group's chunks.  LB: the B fragments come from the block panel in
-// LDS, stored negated: sB[(c - cbase) * 16 + j] = -L[j0 + j][c].
-template <bool MF, int U, bool LB>
-__device__ inline void mfma_frags(const Frags<U, LB> &f, d4 *acc, const BlockGeom &g, int c0, int cend, int q,
-                                  int jl, const double *sB, int cbase) {
+// acc[i][ct] += A (-B_ct) over the group's chunks (tile 0 has no column tile 1)
+template <bool MF, int U>
+__device__ inline void mfma_frags(const Frags<U> &f, d4 (*acc)[2], const BlockGeom &g, int c0, int cend, int q,
+                                  int jl) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int c = c0 + 4 * u;
     if (c >= cend) continue;  // wave-uniform
     const int ck = c + q;
-    double bn;
-    if constexpr (LB) bn = sB[(ck - cbase) * 16 + jl];
-    else bn = (g.j0 + jl - ck <= g.b) ? -f.b[u] : 0.0;
+    double bn[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) bn[ct] = (g.j0 + 16 * ct + jl - ck <= g.b) ? -f.b[u][ct] : 0.0;
 #pragma unroll
     for (int i = 0; i < MAXT; ++i) {
       const int t = g.tb + g.W * i;
@@ -192,28 +195,28 @@ __device__ inline void mfma_frags(const Frags<U, LB> &f, d4 *acc, const BlockGeo
       if (t < g.T && c + 3 >= R - g.b) {  // wave-uniform
         const int ra = R + jl;
         const double af = (ra < g.npad && ra - ck <= g.b) ? f.a[u][i] : 0.0;
-        tile_fma<MF>(acc[i], af, bn, q, jl);
+        tile_fma<MF>(acc[i][0], af, bn[0], q, jl);
+        if (t >= 1) tile_fma<MF>(acc[i][1], af, bn[1], q, jl);
       }
     }
   }
 }
 
 // The steady part of a pass: chunks cs <= c < j0, where every lane's entry of each of the
-// wave's nt existing tiles lies inside the band and the tiles' rows end before the padding.
-// Branch-free: per tile a pointer that advances by 4 (ld - 1) doubles per chunk, the B
-// fragment (negated) from the LDS panel or, without it, from the band (also inside the band
-// there), two groups of chunks in flight.
-template <bool MF, int U, bool LB>
-__device__ inline void steady_update(d4 *acc, const double *__restrict__ band, const BlockGeom &g, int cs, int nt,
-                                     int q, int jl, const double *sB, int cbase) {
+// wave's nt existing tiles and of both column tiles lies inside the band and the tiles' rows
+// end before the padding.  Branch-free: per tile a pointer that advances by 4 (ld - 1)
+// doubles per chunk, two groups of chunks in flight, each A fragment used by both column
+// tiles.
+template <bool MF, int U>
+__device__ inline void steady_update(d4 (*acc)[2], const double *__restrict__ band, const BlockGeom &g, int cs,
+                                     int nt, int q, int jl) {
   const int64_t step = g.ld - 1;
   const double *pa[MAXT];
 #pragma unroll
   for (int i = 0; i < MAXT; ++i) pa[i] = band + (int64_t)(cs + q) * step + g.j0 + 16 * (g.tb + g.W * i) + jl;
-  const double *pb = band + (int64_t)(cs + q) * step + g.j0 + jl;  // B fragment without the LDS panel
-  const double *lb = sB + (cs + q - cbase) * 16 + jl;
-  double a0[U][MAXT], a1[U][MAXT], b0[LB ? 1 : U], b1[LB ? 1 : U];
-  auto load = [&](double (&a)[U][MAXT], double (&bb)[LB ? 1 : U], int c) {
+  const double *pb = band + (int64_t)(cs + q) * step + g.j0 + jl;
+  double a0[U][MAXT], a1[U][MAXT], b0[U][2], b1[U][2];
+  auto load = [&](double (&a)[U][MAXT], double (&bb)[U][2], int c) {
     const int64_t off = (int64_t)(c - cs) * step;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -221,19 +224,21 @@ __device__ inline void steady_update(d4 *acc, const double *__restrict__ band, c
 #pragma unroll
       for (int i = 0; i < MAXT; ++i)
         if (i < nt) a[u][i] = pa[i][off + (int64_t)(4 * u) * step];
-      if constexpr (!LB) bb[u] = pb[off + (int64_t)(4 * u) * step];
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) bb[u][ct] = pb[off + (int64_t)(4 * u) * step + 16 * ct];
     }
   };
-  auto prod = [&](const double (&a)[U][MAXT], const double (&bb)[LB ? 1 : U], int c) {
+  auto prod = [&](const double (&a)[U][MAXT], const double (&bb)[U][2], int c) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (c + 4 * u >= g.j0) continue;  // wave-uniform
-      double bn;
-      if constexpr (LB) bn = lb[(c - cs + 4 * u) * 16];
-      else bn = -bb[u];
+      const double bn0 = -bb[u][0], bn1 = -bb[u][1];
 #pragma unroll
       for (int i = 0; i < MAXT; ++i)
-        if (i < nt) tile_fma<MF>(acc[i], a[u][i], bn, q, jl);
+        if (i < nt) {
+          tile_fma<MF>(acc[i][0], a[u][i], bn0, q, jl);
+          if (g.tb + g.W * i >= 1) tile_fma<MF>(acc[i][1], a[u][i], bn1, q, jl);
+        }
     }
   };
   int c = cs;
@@ -247,145 +252,191 @@ __device__ inline void steady_update(d4 *acc, const double *__restrict__ band, c
   }
 }
 
-// Left-looking blocked band Cholesky of one mesh by the whole workgroup.  Per block of 16
-// columns j0..j0+15: every wave updates its 16 x 16 tiles of rows j0..j0+15+b with the
-// previous b columns (MFMA; loads of the next group of chunks overlap the products of the
-// current one), wave 0 factors the diagonal tile (L11) and forms L11^-T, and every wave
-// turns its tiles below into L21 = C L11^-T with four products per tile.
-template <int BT, bool MF, bool LB>
+// Diagonal tile of column tile ct (held by the calling wave in accumulator layout): Cholesky
+// factor L_d to the band (columns col0 .. col0+15) and LDS sL, then L_d^-T (the solution X of
+// X L_d^T = I, column by column across the lanes) to sLi; *sflag on a non-positive pivot.
+template <bool MF>
+__device__ inline void diag_block(d4 &a, double *__restrict__ band, int64_t ld, int b, int col0, double *sL,
+                                  double *sLi, int32_t *sflag) {
+  const int ln = lane_id();
+  const int q = ln >> 4, jl = ln & 15;
+  bool bad = false;
+  chol16(a, q, jl, bad);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int i = 4 * v + q;
+    sL[i * 16 + jl] = a[v];
+    const int d = i - jl;
+    if (d >= 0 && d <= b) band[(int64_t)(col0 + jl) * ld + d] = a[v];
+  }
+  double Lrow[16];  // L_d[jl][k]
+#pragma unroll
+  for (int k = 0; k < 16; ++k) Lrow[k] = sL[jl * 16 + k];
+  d4 x;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) x[v] = (4 * v + q == jl) ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (jl == k) {
+      const double rk = 1.0 / Lrow[k];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) x[v] *= rk;
+    }
+    double xk[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) xk[v] = shfl(x[v], 16 * q + k);
+    if (jl > k) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) x[v] -= xk[v] * Lrow[k];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) sLi[(4 * v + q) * 16 + jl] = x[v];
+  if (bad && ln == 0) *sflag = 1;
+}
+
+// x = c m^T for 16 x 16 tiles: c in accumulator layout (through the wave's LDS tile myC into
+// A fragments), m given by its B fragments mb[cq] = m[jl][4 cq + q]... i.e. B[kk][n] =
+// m^T[4 cq + kk][n]; sign -1 subtracts the product from acc instead
+template <bool MF>
+__device__ inline void tile_times(d4 &acc, const d4 &c, const double *mb, double *myC) {
+  const int ln = lane_id();
+  const int q = ln >> 4, jl = ln & 15;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) myC[(4 * v + q) * 16 + jl] = c[v];
+  double af[4];
+#pragma unroll
+  for (int cq = 0; cq < 4; ++cq) af[cq] = myC[jl * 16 + 4 * cq + q];
+#pragma unroll
+  for (int cq = 0; cq < 4; ++cq) tile_fma<MF>(acc, af[cq], mb[cq], q, jl);
+}
+
+// store a tile (accumulator layout) of column tile starting at col0 into the band
+__device__ inline void store_tile(const d4 &x, double *__restrict__ band, int64_t ld, int b, int npad, int R,
+                                  int col0) {
+  const int ln = lane_id();
+  const int q = ln >> 4, jl = ln & 15;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = R + 4 * v + q;
+    const int d = r - (col0 + jl);
+    if (r < npad && d >= 0 && d <= b) band[(int64_t)(col0 + jl) * ld + d] = x[v];
+  }
+}
+
+// Left-looking blocked band Cholesky of one mesh by the whole workgroup, blocks of 32 columns.
+// Per block: every wave updates its tiles of rows j0 .. j0+31+b in both column tiles with the
+// previous b columns (MFMA; each A fragment feeds both column tiles; loads of the next group
+// of chunks overlap the products of the current one).  Then wave 0 factors the first
+// diagonal tile (L00, and L00^-T); every wave forms its column-0 tiles L(t,0) = C L00^-T;
+// the owner of tile 1 publishes L10; every wave subtracts L(t,0) L10^T from its column-1
+// tiles; wave 1 factors the second diagonal tile (L11, L11^-T); every wave forms
+// L(t,1) = C L11^-T.  Four barriers per block.
+template <int BT, bool MF>
 __device__ int factor_mesh(const DMesh &M, double *__restrict__ band, double *sL, double *sLi, double *sC,
-                           double *sB, int32_t *sflag) {
+                           int32_t *sflag) {
   constexpr int W = BT / 64;
-  constexpr int U = 4;   // chunks per group of the steady part
-  constexpr int UR = 4;                      // chunks per group of the ramp
+  constexpr int U = 2;   // chunks per group of the steady part
+  constexpr int UR = 2;  // chunks per group of the ramp
   const int wv = wave_id();
   const int b = M.b, npad = M.npad;
   const int64_t ld = M.ld;
   double *myC = sC + 256 * wv;
+  double *sLi0 = sLi, *sL10 = sLi + 256, *sLi1 = sLi + 512;
   if (threadIdx.x == 0) *sflag = 0;
   __syncthreads();
-  for (int j0 = 0; j0 < npad; j0 += 16) {
+  for (int j0 = 0; j0 < npad; j0 += 32) {
     const int lane = lane_id();
     const int q = lane >> 4, jl = lane & 15;
-    const int rows = min(16 + b, npad - j0);  // rows j0 .. j0+rows-1 hold the block's nonzeros
-    const int T = (rows + 15) >> 4;
+    const int rows = min(32 + b, npad - j0);  // rows j0 .. j0+rows-1 hold the block's nonzeros
+    const int T = (rows + 15) >> 4;           // >= 2
     const int P = (T + W * MAXT - 1) / (W * MAXT);
-    const int cbase = max(0, j0 - b) & ~3;  // first column of the block panel
-    double bL[4];  // B fragments of L11^-T (set in pass 0)
+    double mb0[4], mb10[4], mb1[4];  // B fragments of L00^-T, -L10^T, L11^-T (set in pass 0)
     for (int p = 0; p < P; ++p) {
       BlockGeom g{j0, b, npad, T, wv + W * MAXT * p, W, ld};
-      d4 acc[MAXT];
+      d4 acc[MAXT][2];
 #pragma unroll
       for (int i = 0; i < MAXT; ++i) {
         const int t = g.tb + W * i;
-        const int64_t cc = j0 + jl;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = j0 + 16 * t + 4 * v + q;
-          const int d = r - (int)cc;
-          acc[i][v] = band[cc * ld + min(max(d, 0), b)];  // clamped: always inside the band
-        }
+        for (int ct = 0; ct < 2; ++ct) {
+          const int64_t cc = j0 + 16 * ct + jl;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = j0 + 16 * t + 4 * v + q;
-          const int d = r - (int)cc;
-          if (!(t < T && r < npad && d >= 0 && d <= b)) acc[i][v] = 0.0;
+          for (int v = 0; v < 4; ++v) {
+            const int r = j0 + 16 * t + 4 * v + q;
+            acc[i][ct][v] = band[cc * ld + min(max(r - (int)cc, 0), b)];  // clamped: inside the band
+          }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = j0 + 16 * t + 4 * v + q;
+            const int d = r - (int)cc;
+            if (!(t < T && r < npad && d >= 0 && d <= b)) acc[i][ct][v] = 0.0;
+          }
         }
-      }
-      if (LB && p == 0) {
-        // block panel -L[j0 .. j0+15][cbase .. j0-1] into LDS (zero outside the band)
-        const int cnt = (j0 - cbase) * 16;
-        for (int e = threadIdx.x; e < cnt; e += BT) {
-          const int c = cbase + (e >> 4), j = e & 15;
-          const int d = j0 + j - c;  // >= 1
-          const double v = band[(int64_t)c * ld + min(d, b)];
-          sB[e] = d <= b ? -v : 0.0;
-        }
-        __syncthreads();
       }
       if (g.tb < T) {
-        // A[block rows, j0..j0+15] -= L[block rows, c] L[j0..j0+15, c]^T, c < j0: a ramp with
-        // predicates up to cs, where the wave's last existing tile is inside the band
-        // everywhere, then the branch-free steady part
+        // A[block rows, j0..j0+31] -= L[block rows, c] L[j0..j0+31, c]^T, c < j0: a ramp with
+        // predicates up to cs, then the branch-free steady part
         const int clo = max(0, j0 + 16 * g.tb - b) & ~3;
         const int nt = min(MAXT, (T - 1 - g.tb) / W + 1);  // existing tiles of this wave
         const int Rl = j0 + 16 * (g.tb + W * (nt - 1));
-        const int cs = Rl + 16 <= npad ? min(j0, max(clo, (Rl + 15 - b + 3) & ~3)) : j0;
+        const int cs = Rl + 16 <= npad ? min(j0, max(clo, (max(Rl + 15, j0 + 31) - b + 3) & ~3)) : j0;
         for (int c = clo; c < cs; c += 4 * UR) {
-          Frags<UR, LB> f;
-          load_frags<UR, LB>(f, band, g, c, cs, q, jl);
-          mfma_frags<MF, UR, LB>(f, acc, g, c, cs, q, jl, sB, cbase);
+          Frags<UR> f;
+          load_frags<UR>(f, band, g, c, cs, q, jl);
+          mfma_frags<MF, UR>(f, acc, g, c, cs, q, jl);
         }
-        steady_update<MF, U, LB>(acc, band, g, cs, nt, q, jl, sB, cbase);
+        steady_update<MF, U>(acc, band, g, cs, nt, q, jl);
       }
       if (p == 0) {
-        // diagonal tile (tile 0 of wave 0): L11 to the band and LDS, then L11^-T, the
-        // solution X of X L11^T = I column by column across the lanes, to LDS
-        if (wv == 0) {
-          const int ln = lane_id();
-          const int q = ln >> 4, jl = ln & 15;
-          bool bad = false;
-          chol16(acc[0], q, jl, bad);
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int i = 4 * v + q;
-            sL[i * 16 + jl] = acc[0][v];
-            const int d = i - jl;
-            if (d >= 0 && d <= b) band[(int64_t)(j0 + jl) * ld + d] = acc[0][v];
-          }
-          double Lrow[16];  // L11[jl][k]
-#pragma unroll
-          for (int k = 0; k < 16; ++k) Lrow[k] = sL[jl * 16 + k];
-          d4 x;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) x[v] = (4 * v + q == jl) ? 1.0 : 0.0;
-#pragma unroll
-          for (int k = 0; k < 16; ++k) {
-            if (jl == k) {
-              const double rk = 1.0 / Lrow[k];
-#pragma unroll
-              for (int v = 0; v < 4; ++v) x[v] *= rk;
-            }
-            double xk[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) xk[v] = shfl(x[v], 16 * q + k);
-            if (jl > k) {
-#pragma unroll
-              for (int v = 0; v < 4; ++v) x[v] -= xk[v] * Lrow[k];
-            }
-          }
-#pragma unroll
-          for (int v = 0; v < 4; ++v) sLi[(4 * v + q) * 16 + jl] = x[v];
-          if (bad && ln == 0) *sflag = 1;
-        }
+        if (wv == 0) diag_block<MF>(acc[0][0], band, ld, b, j0, sL, sLi0, sflag);
         __syncthreads();
         if (*sflag) return PF_NOT_SPD;
-        // B fragment of chunk cq of the product X = C L11^-T: B[kk][n] = L11^-T[4cq + kk][n]
 #pragma unroll
-        for (int cq = 0; cq < 4; ++cq) bL[cq] = sLi[(4 * cq + q) * 16 + jl];
+        for (int cq = 0; cq < 4; ++cq) mb0[cq] = sLi0[(4 * cq + q) * 16 + jl];
       }
-      // tiles below the diagonal: X = C L11^-T, 4 products per tile; C goes through the
-      // wave's LDS tile from the accumulator layout into A fragments
+      // column tile 0 below the diagonal: L(t,0) = C L00^-T; tile 1 publishes L10
 #pragma unroll
       for (int i = 0; i < MAXT; ++i) {
         const int t = g.tb + W * i;
         if (t >= 1 && t < T) {
-          const int ln = lane_id();
-          const int q = ln >> 4, jl = ln & 15;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) myC[(4 * v + q) * 16 + jl] = acc[i][v];
-          double af[4];
-#pragma unroll
-          for (int cq = 0; cq < 4; ++cq) af[cq] = myC[jl * 16 + 4 * cq + q];
           d4 x = {0.0, 0.0, 0.0, 0.0};
+          tile_times<MF>(x, acc[i][0], mb0, myC);
+          acc[i][0] = x;
+          store_tile(x, band, ld, b, npad, j0 + 16 * t, j0);
+          if (t == 1) {
 #pragma unroll
-          for (int cq = 0; cq < 4; ++cq) tile_fma<MF>(x, af[cq], bL[cq], q, jl);
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int r = j0 + 16 * t + 4 * v + q;
-            const int d = r - (j0 + jl);
-            if (r < npad && d <= b) band[(int64_t)(j0 + jl) * ld + d] = x[v];
+            for (int v = 0; v < 4; ++v) sL10[(4 * v + q) * 16 + jl] = x[v];
           }
+        }
+      }
+      if (p == 0) {
+        __syncthreads();
+        // B fragment of -L10^T: B[kk][n] = -L10[n][4 cq + kk]
+#pragma unroll
+        for (int cq = 0; cq < 4; ++cq) mb10[cq] = -sL10[jl * 16 + 4 * cq + q];
+      }
+      // column tile 1: C(t,1) -= L(t,0) L10^T
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i) {
+        const int t = g.tb + W * i;
+        if (t >= 1 && t < T) tile_times<MF>(acc[i][1], acc[i][0], mb10, myC);
+      }
+      if (p == 0) {
+        if (wv == 1) diag_block<MF>(acc[0][1], band, ld, b, j0 + 16, sL, sLi1, sflag);
+        __syncthreads();
+        if (*sflag) return PF_NOT_SPD;
+#pragma unroll
+        for (int cq = 0; cq < 4; ++cq) mb1[cq] = sLi1[(4 * cq + q) * 16 + jl];
+      }
+      // column tile 1 below its diagonal: L(t,1) = C L11^-T
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i) {
+        const int t = g.tb + W * i;
+        if (t >= 2 && t < T) {
+          d4 x = {0.0, 0.0, 0.0, 0.0};
+          tile_times<MF>(x, acc[i][1], mb1, myC);
+          store_tile(x, band, ld, b, npad, j0 + 16 * t, j0 + 16);
         }
       }
     }
@@ -485,11 +536,10 @@ __global__ void k_flag_clear(int32_t *f) {
   if (threadIdx.x == 0 && blockIdx.x == 0) f[0] = 0;
 }
 
-template <int BT, bool MF, bool LB>
+template <int BT, bool MF>
 __global__ void __launch_bounds__(BT) k_direct(DArgs a) {
-  __shared__ double sB[LB ? 16 * (DBMAX + 4) : 1];
   __shared__ double sL[256];
-  __shared__ double sLi[256];
+  __shared__ double sLi[768];
   __shared__ double sC[256 * (BT / 64)];
   __shared__ double sY[16];
   __shared__ int32_t next, sflag;
@@ -503,7 +553,7 @@ __global__ void __launch_bounds__(BT) k_direct(DArgs a) {
     const DMesh M = a.mesh[m];
     const int64_t t0 = wall_clock64();
     double *band = a.band + M.boff;
-    int st = a.mode == 0 ? factor_mesh<BT, MF, LB>(M, band, sL, sLi, sC, sB, &sflag) : a.status[m];
+    int st = a.mode == 0 ? factor_mesh<BT, MF>(M, band, sL, sLi, sC, &sflag) : a.status[m];
     double *w = a.w + M.woff;
     const int32_t *perm = a.perm + M.woff;
     if (st == PF_OK) {
@@ -781,7 +831,6 @@ struct DirectWork {
   DevBuf<int64_t> t_raw;
   DevBuf<double> band, w, r, y, norms;
   int64_t band_entries = 0, work_entries = 0;
-  int32_t max_b = 0;  // largest half bandwidth of the meshes
   double analyse_ms = 0.0;
   // factor held in band: the matrix version it was computed from (0: none)
   uint64_t factor_version = 0;
@@ -893,7 +942,7 @@ int pf_direct_analyse(pf_ctx *ctx, DirectWork **pd, int64_t n, const int64_t *ro
     }
     M.d0 = dof_off[m];
     M.n = (int32_t)nm;
-    M.npad = (int32_t)((nm + 15) / 16 * 16);
+    M.npad = (int32_t)((nm + 31) / 32 * 32);
     M.b = (int32_t)(g * ord[m].bw + g - 1);
     M.ld = (M.b + 2) & ~1;
     M.boff = boff;
@@ -952,8 +1001,6 @@ int pf_direct_analyse(pf_ctx *ctx, DirectWork **pd, int64_t n, const int64_t *ro
   PF_HIP(hipStreamSynchronize(ctx->stream));
   D->band_entries = boff;
   D->work_entries = woff;
-  D->max_b = 0;
-  for (const DMesh &M : D->hmesh) D->max_b = std::max(D->max_b, M.b);
   D->group = g;
   D->n = n;
   D->dof_off.assign(dof_off, dof_off + nmesh + 1);
@@ -983,17 +1030,10 @@ int launch_direct(pf_ctx *ctx, DirectWork *D, int mode, const double *rhs, doubl
   const int grid = std::max(1, std::min(D->nmesh, 2 * ctx->num_cus));
   k_direct_queue_reset<<<1, 64, 0, ctx->stream>>>(D->queue.p);
   PF_HIP(hipGetLastError());
-  // the block panel through LDS is kept for reference but not used: under the memory
-  // contention of a full batch its per-block fill costs more than it saves
-  const bool lb = false;
-  if (mfma && lb)
-    k_direct<DBT, true, true><<<grid, DBT, 0, ctx->stream>>>(a);
-  else if (mfma)
-    k_direct<DBT, true, false><<<grid, DBT, 0, ctx->stream>>>(a);
-  else if (lb)
-    k_direct<DBT, false, true><<<grid, DBT, 0, ctx->stream>>>(a);
+  if (mfma)
+    k_direct<DBT, true><<<grid, DBT, 0, ctx->stream>>>(a);
   else
-    k_direct<DBT, false, false><<<grid, DBT, 0, ctx->stream>>>(a);
+    k_direct<DBT, false><<<grid, DBT, 0, ctx->stream>>>(a);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
