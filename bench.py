@@ -87,6 +87,8 @@ def main(argv=None):
                     help="overset plant of Cube3D boxes, or of Cylinder stem / box petioles / Leaf3D blades")
     ap.add_argument("--precond-size", type=int, default=60)
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batch-legs", default="direct,fp64,mixed,fp64_pointjacobi",
+                    help="batch workload legs to run (comma list; e.g. 'direct' alone for a profile)")
     ap.add_argument("--batch-itrmax", type=int, default=3000,
                     help="iteration budget per mesh of the batched solve")
     ap.add_argument("--adjoint-size", type=int, default=60)
@@ -555,9 +557,12 @@ def bench_batch(a):
                                   "nu 0.3; clamped at one end/edge; pressure 1e2..1e4 Pa",
                       "n_dof": int(ndof.sum()), "nnz": int(s.nnz), "itrmax_per_mesh": a.batch_itrmax,
                       "stop": "||b - A x|| / ||b|| < 1e-8 per mesh"}}
+    legs = set(a.batch_legs.split(","))
     # direct: solve(algorithm="GaussJordan") for every mesh (banded Cholesky, one workgroup per
     # mesh); the first call renumbers the meshes (host, kept per sparsity pattern), the timed
     # call assembles the bands and factors anew (setDirectKernel drops the held factor)
+    if "direct" not in legs:
+        raise SystemExit("the batch workload line is the direct leg: include 'direct' in --batch-legs")
     t0 = time.perf_counter()
     s.solveBatch(algorithm="GaussJordan", refine=1)
     first_s = time.perf_counter() - t0
@@ -592,6 +597,8 @@ def bench_batch(a):
     log(f"batch direct: {json.dumps(leg)}")
     out["direct"] = leg
     for leg_name, prec, pc in (("fp64", 64, None), ("mixed", 32, None), ("fp64_pointjacobi", 64, "PointJacobi")):
+        if leg_name not in legs:
+            continue
         if pc is not None:  # the scaling is destructive: a fresh system
             s.close()
             s = B.build_solver(u)
