@@ -98,8 +98,9 @@ def main(argv=None):
     ap.add_argument("--fd-h", type=float, default=1e-3)
     ap.add_argument("--fd-reduce", type=float, default=1e-11)
     ap.add_argument("--profile", action="store_true",
-                    help="short profiling workload: 40 FP64 iterations, the SpMV kernels and a 512 MiB "
-                         "device copy (the reference kernel for HBM counter calibration)")
+                    help="short profiling workload: 40 FP64 iterations, the element-matrix kernels, the "
+                         "SpMV kernels and a 512 MiB device copy (the reference kernel for HBM counter "
+                         "calibration)")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the multi-GPU (z-slab sharded) path is a design "
@@ -148,6 +149,15 @@ def main(argv=None):
         s.solve(precision=48, target_relres=1e-30, stop_mode=1, download=False, use_graph=0,
                 raise_on_breakdown=False, **P48)
         log(f"profile: {s.last_stats['iterations']} precision 48 iterations")
+        # element-matrix kernels: the reference-order one (kind 0, in the assembly above) against
+        # the isotropic split on the vector units (1) and on the FP64 matrix cores (2)
+        for kind in (1, 2, 0):
+            s.setKeKernel(kind)
+            for _ in range(3):
+                t1 = time.perf_counter()
+                s.setStiffness()
+                ms = (time.perf_counter() - t1) * 1e3
+            log(f"profile: stiffness with element kernel {kind} {ms:.2f} ms (host clock, with the row gather)")
         xr = np.random.default_rng(1).standard_normal(n)
         for kind in (1, 2, 3):
             _, ms = s.spmv(kind, xr, repeat=5, download=False)

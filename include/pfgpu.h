@@ -237,6 +237,15 @@ int pf_system_get_crs(pf_system *sys, int64_t *row_ptr, int32_t *col_idx, double
  * E == nu == NULL reuses the resident arrays of the previous call. */
 int pf_system_stiffness(pf_system *sys, const double *E, const double *nu, int32_t jmode);
 
+/* Element-matrix kernel of later pf_system_stiffness / _dev calls on a single-domain CRS:
+ * 0 (default) the reference's operation order of StiffnessMatrixFEMDomain
+ * (FEMDomainClass.f90:11457-11515, CRS values bitwise); 1 the isotropic split
+ * Ke(3a+d,3b+e) = lam M_ab(d,e) + mu M_ab(e,d) + mu [d==e] tr M_ab with
+ * M_ab(p,q) = sum_gp detJ G(p,a) G(q,b), contraction on the vector units; 2 the same with the
+ * contraction on the FP64 matrix cores.  1 and 2 agree with 0 to rounding (not bitwise).  No
+ * counterpart in the reference: it selects how the element contraction is evaluated. */
+int pf_system_ke_kernel(pf_system *sys, int32_t kind);
+
 /* setMatrix(DomainID=1,ElementID,DOF,Matrix) for a batch (FEMSolverClass.f90:901-909,
  * :1020-1057): elem_ids 1-based, ke count matrices of (nne*dof)^2 column-major.  Adds into
  * val; a missing slot returns PF_INVALID like the reference's fill-in stop. */

@@ -190,6 +190,7 @@ SIGNATURES = {
     "pf_system_batch_direct": (C.c_int, [P, I32, PF64, P, C.POINTER(Stats)]),
     "pf_system_direct_info": (C.c_int, [P, I32, PI64, PI32, PI32, PI32, PF64]),
     "pf_system_direct_kernel": (C.c_int, [P, I32]),
+    "pf_system_ke_kernel": (C.c_int, [P, I32]),
 }
 
 

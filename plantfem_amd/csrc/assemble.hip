@@ -341,6 +341,140 @@ __global__ void __launch_bounds__(256) k_ke_elem(int64_t nn, int64_t ne, const d
   }
 }
 
+// ---- element matrices through the isotropic split (contraction on MFMA or on the VALU) ----
+// For the isotropic D of DMatrix (lam, mu), B^T D B summed over the Gauss points splits
+// per node pair (a, b) into
+//   Ke(3a+d, 3b+e) = lam M_ab(d, e) + mu M_ab(e, d) + mu [d == e] (M_ab(0,0) + M_ab(1,1) + M_ab(2,2)),
+//   M_ab(p, q) = sum_gp detJ G(p, a) G(q, b),
+// so the only dense contraction is the 24 x 24 x 8 product M = Gt^T diag(detJ) Gt with
+// Gt(gp, 8p + a) = G(p, a) at Gauss point gp.  The geometry (J, det J, G) is bitwise the one
+// of hex8_gp_G; the sums are in another order than the reference's matmul chain, so the entries
+// agree with k_ke_elem to rounding, not bitwise.  MF: the product on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64, tiles (0,0), (0,1), (1,1) of the 32 x 32 padded M, two steps of 4
+// Gauss points each, M(1,0) = M(0,1)^T through LDS), else each lane forms the 3 x 3 block M_ab
+// of its node pair in registers.  One wavefront per 8 elements: lane 8el + gp evaluates the
+// geometry of element el at Gauss point gp into the wave's LDS; then per element, lane
+// 8a + b builds and stores the 3 x 3 block (a, b) of Ke (row-major, as k_ke_elem<false>).
+typedef double ke_d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void wave_lds_sync() {
+  // LDS operations of one wavefront are performed in issue order; the fences keep the compiler
+  // from moving LDS accesses across this point
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int KI_W = 4;     // wavefronts per block
+constexpr int KI_EL = 8;    // elements per wavefront
+constexpr int KI_GS = 25;   // G(3,8) and det J per Gauss point
+constexpr int KI_MS = 25;   // row stride of M in LDS
+
+template <bool MF>
+__global__ void __launch_bounds__(256) k_ke_iso(int64_t nn, int64_t ne, const double *__restrict__ xyz,
+                                                const int32_t *__restrict__ en, const double *__restrict__ E,
+                                                const double *__restrict__ nu, int jmode, int64_t first,
+                                                int64_t count, double *__restrict__ ke) {
+  __shared__ double sG[KI_W][KI_EL][8][KI_GS];
+  __shared__ double sLM[KI_W][KI_EL][2];
+  __shared__ double sM[MF ? KI_W : 1][MF ? 24 * KI_MS : 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t q0 = ((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6) * KI_EL;
+  if (q0 >= count) return;  // whole wavefronts; no block-wide barrier follows
+  const int nel = (int)(count - q0 < KI_EL ? count - q0 : KI_EL);
+  {
+    const int el = lane >> 3, gp = lane & 7;
+    if (el < nel) {
+      const int64_t e = first + q0 + el;
+      int32_t nodes[8];
+      double X[8][3];
+      load_element(nn, ne, xyz, en, e, nodes, X);
+      double G[3][8];
+      const double dj = hex8_gp_G(X, gp, jmode, G, true);
+      double *g = sG[w][el][gp];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) g[p * 8 + b] = G[p][b];
+      g[24] = dj;
+      if (gp == 0) {
+        const double Ee = E[e], ve = nu[e];
+        sLM[w][el][0] = ve * Ee / (1.0 + ve) / (1.0 - 2.0 * ve);
+        sLM[w][el][1] = Ee / 2.0 / (1.0 + ve);
+      }
+    }
+  }
+  wave_lds_sync();
+  const int a = lane >> 3, b = lane & 7;
+  for (int el = 0; el < nel; ++el) {
+    double m[3][3];
+    if constexpr (MF) {
+      const int li = lane & 15, lk = lane >> 4;
+      ke_d4 t00 = {0.0, 0.0, 0.0, 0.0}, t01 = t00, t11 = t00;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const double *g = sG[w][el][4 * s + lk];
+        const double dj = g[24];
+        const double g0 = g[li];
+        const double g1 = li < 8 ? g[16 + (li & 7)] : 0.0;
+        const double a0 = dj * g0, a1 = dj * g1;
+        t00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, g0, t00, 0, 0, 0);
+        t01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, g1, t01, 0, 0, 0);
+        t11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, g1, t11, 0, 0, 0);
+      }
+      // accumulator layout: lane l, element v holds row 4v + l/16, column l%16 of its tile
+      double *M = sM[w];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 4 * v + lk;
+        M[r * KI_MS + li] = t00[v];
+        if (li < 8) {
+          M[r * KI_MS + 16 + li] = t01[v];
+          M[(16 + li) * KI_MS + r] = t01[v];
+          if (r < 8) M[(16 + r) * KI_MS + 16 + li] = t11[v];
+        }
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) m[p][q] = M[(8 * p + a) * KI_MS + 8 * q + b];
+      wave_lds_sync();  // the next element rewrites M
+    } else {
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) m[p][q] = 0.0;
+#pragma unroll
+      for (int gp = 0; gp < 8; ++gp) {
+        const double *g = sG[w][el][gp];
+        const double dj = g[24];
+        const double ga0 = dj * g[a], ga1 = dj * g[8 + a], ga2 = dj * g[16 + a];
+        const double gb0 = g[b], gb1 = g[8 + b], gb2 = g[16 + b];
+        m[0][0] = __builtin_fma(ga0, gb0, m[0][0]);
+        m[0][1] = __builtin_fma(ga0, gb1, m[0][1]);
+        m[0][2] = __builtin_fma(ga0, gb2, m[0][2]);
+        m[1][0] = __builtin_fma(ga1, gb0, m[1][0]);
+        m[1][1] = __builtin_fma(ga1, gb1, m[1][1]);
+        m[1][2] = __builtin_fma(ga1, gb2, m[1][2]);
+        m[2][0] = __builtin_fma(ga2, gb0, m[2][0]);
+        m[2][1] = __builtin_fma(ga2, gb1, m[2][1]);
+        m[2][2] = __builtin_fma(ga2, gb2, m[2][2]);
+      }
+    }
+    const double lam = sLM[w][el][0], mu = sLM[w][el][1];
+    const double tr = (m[0][0] + m[1][1]) + m[2][2];
+    double *out = ke + (q0 + el) * 576 + (3 * a) * 24 + 3 * b;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const double sh = (d == e) ? m[e][d] + tr : m[e][d];
+        out[d * 24 + e] = __builtin_fma(lam, m[d][e], mu * sh);
+      }
+  }
+}
+
 // CRS values of a hex8 / DOF=3 ZeroMatrix pattern (rows = node neighbour lists ncol expanded
 // by DOF, FEMDomainClass.f90:10780-10836) from row-major element matrices: one
 // SUB-lane subgroup per CRS row r = 3i + d, lanes over the row's entries; entry (r, 3j+d')
@@ -872,17 +1006,30 @@ __global__ void k_traction(int64_t nn, int64_t ne, const double *__restrict__ xy
 }  // namespace
 
 int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
-                     const int64_t *row_ptr, double *val) {
+                     const int64_t *row_ptr, double *val, int ke_kernel) {
   if (g.nne != 8 || !g.cols_built || !g.has_coords) {
     pf_set_error("hex8 assembly needs an 8-node mesh with coordinates and its graph built");
+    return PF_INVALID;
+  }
+  if (ke_kernel < 0 || ke_kernel > 2) {
+    pf_set_error("hex8 assembly: element kernel must be 0 (reference order), 1 (isotropic split, VALU) "
+                 "or 2 (isotropic split, MFMA)");
     return PF_INVALID;
   }
   if (g.nn == 0) return PF_OK;
   // element matrices once per element (row-major scratch of 4.6 KB per element), then one
   // gather per CRS row; without room for the scratch, the one-pass row kernel
   if (g.ne > 0 && g.kebuf.ensure((size_t)g.ne * 576) == PF_OK) {
-    k_ke_elem<false><<<pf_blocks(g.ne * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
-        g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, 0, g.ne, g.kebuf.p);
+    const int64_t waves = (g.ne + KI_EL - 1) / KI_EL;
+    if (ke_kernel == 0)
+      k_ke_elem<false><<<pf_blocks(g.ne * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+          g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, 0, g.ne, g.kebuf.p);
+    else if (ke_kernel == 1)
+      k_ke_iso<false><<<pf_blocks(waves * 64, 64 * KI_W, INT32_MAX), 64 * KI_W, 0, ctx->stream>>>(
+          g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, 0, g.ne, g.kebuf.p);
+    else
+      k_ke_iso<true><<<pf_blocks(waves * 64, 64 * KI_W, INT32_MAX), 64 * KI_W, 0, ctx->stream>>>(
+          g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, 0, g.ne, g.kebuf.p);
     k_gather_rows<16, 6><<<pf_blocks(3 * g.nn * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(
         g.nn, g.ne, g.elemnod.p, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p, row_ptr, g.kebuf.p, val);
   } else {

@@ -641,6 +641,7 @@ struct pf_system {
   DirectWork *direct = nullptr;
   uint64_t pattern_version = 1;  // bumped with every change of the CRS structure
   int32_t direct_mfma = 1;       // direct solve: 1 MFMA tile products, 0 the VALU form
+  int32_t ke_kernel = 0;         // element matrices: 0 reference order, 1/2 isotropic split VALU/MFMA
   int32_t jmode = 0;          // J mode of the last assembly (reused by the sensitivities)
   bool ubar_nonzero = false;  // some fixed DOF has a nonzero prescribed value
   // keepThisMatrixAs: named copies of the CRS values (same pattern) and the fix_eig list
@@ -853,7 +854,7 @@ extern "C" int pf_system_stiffness(pf_system *s, const double *E, const double *
     s->has_mat = true;
   }
   if (s->multi_crs) PF_CHECK(multi_stiffness(s, jmode));
-  else PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
+  else PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p, s->ke_kernel));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->jmode = jmode;
   mark_changed(s);
@@ -876,7 +877,7 @@ extern "C" int pf_system_stiffness_dev(pf_system *s, const double *E_dev, const 
   PF_HIP(hipMemcpyAsync(s->nu.p, nu_dev, sizeof(double) * s->g.ne, hipMemcpyDeviceToDevice, ctx->stream));
   s->has_mat = true;
   if (s->multi_crs) PF_CHECK(multi_stiffness(s, jmode));
-  else PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p));
+  else PF_CHECK(pf_assemble_hex8(ctx, s->g, s->E.p, s->nu.p, jmode, s->row_ptr.p, s->val.p, s->ke_kernel));
   PF_HIP(hipStreamSynchronize(ctx->stream));
   s->jmode = jmode;
   mark_changed(s);
@@ -1367,6 +1368,17 @@ extern "C" int pf_system_direct_kernel(pf_system *s, int32_t mfma) {
   PF_CHECK(sys_check(s, false));
   s->direct_mfma = mfma ? 1 : 0;
   pf_direct_invalidate(s->direct);
+  return PF_OK;
+}
+
+extern "C" int pf_system_ke_kernel(pf_system *s, int32_t kind) {
+  PF_CHECK(sys_check(s, false));
+  if (kind < 0 || kind > 2) {
+    pf_set_error("pf_system_ke_kernel: kind must be 0 (reference order), 1 (isotropic split, VALU) or 2 "
+                 "(isotropic split, MFMA)");
+    return PF_INVALID;
+  }
+  s->ke_kernel = kind;
   return PF_OK;
 }
 

@@ -135,8 +135,10 @@ int pf_graph_merged_cols(pf_ctx *ctx, NodeGraph &g, int64_t ncon, const int32_t 
 int pf_sort_u64(pf_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int end_bit, bool *in_tmp);
 
 // assemble.hip
+// ke_kernel: element matrices in the reference's operation order (0, bitwise), or through the
+// isotropic split with the contraction on the VALU (1) or the FP64 matrix cores (2)
 int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
-                     const int64_t *row_ptr, double *val);
+                     const int64_t *row_ptr, double *val, int ke_kernel = 0);
 int pf_hex8_ke_batch(pf_ctx *ctx, const NodeGraph &g, const double *E, const double *nu, int jmode,
                      int64_t first, int64_t count, double *ke);
 // batched setMatrix / setVector: node gathers over a batch in call order

@@ -1103,6 +1103,13 @@ class FEMSolver:
         """Tile products of the direct factorization: MFMA (default) or vector FMAs."""
         L.check(L.load().pf_system_direct_kernel(self.h, int(bool(mfma))), "setDirectKernel")
 
+    def setKeKernel(self, kind: int = 0):
+        """Element-matrix kernel of later setStiffness calls: 0 the reference's operation order of
+        StiffnessMatrixFEMDomain (FEMDomainClass.f90:11457-11515, bitwise CRS values), 1 the
+        isotropic split Ke = lam M + mu M^T + mu tr(M) I per node pair (M = sum detJ G G^T) on
+        the vector units, 2 the same contraction on the FP64 matrix cores (1, 2: to rounding)."""
+        L.check(L.load().pf_system_ke_kernel(self.h, int(kind)), "setKeKernel")
+
 
 def bicgstab_crs(row_ptr, col_idx, val, rhs, x0=None, ctx: L.Context | None = None, **opts):
     """Plug-in LinearSolver(row_ptr, col_idx, val, rhs, x) (FEMSolverClass.f90:1638-1648) on the GPU.
