@@ -62,7 +62,7 @@ def spmv_bytes(kind: int, n: int, nnz: int) -> float:
     nn = n // 3
     nblocks = nnz // 9
     nslices = (nn + 63) // 64
-    if kind == 1:  # CRS: int64 row_ptr, int32 col_idx, FP64 values
+    if kind in (1, 4):  # CRS: int64 row_ptr, int32 col_idx, FP64 values (4: the same product, x staged)
         return 12.0 * nnz + 8.0 * (n + 1) + 16.0 * n
     if kind == 2:  # blocked FP64: one int32 node index per 3x3 block
         return 8.0 * nnz + 4.0 * nblocks + 12.0 * nslices + 8 + 16.0 * n
@@ -98,9 +98,9 @@ def main(argv=None):
     ap.add_argument("--fd-h", type=float, default=1e-3)
     ap.add_argument("--fd-reduce", type=float, default=1e-11)
     ap.add_argument("--profile", action="store_true",
-                    help="short profiling workload: 40 FP64 iterations, the element-matrix kernels, the "
-                         "SpMV kernels and a 512 MiB device copy (the reference kernel for HBM counter "
-                         "calibration)")
+                    help="short profiling workload: 40 FP64 iterations (blocked, CRS rows, CRS rows with x "
+                         "staged), the element-matrix kernels, the SpMV kernels and a 512 MiB device copy "
+                         "(the reference kernel for HBM counter calibration)")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the multi-GPU (z-slab sharded) path is a design "
@@ -149,6 +149,9 @@ def main(argv=None):
         s.solve(precision=48, target_relres=1e-30, stop_mode=1, download=False, use_graph=0,
                 raise_on_breakdown=False, **P48)
         log(f"profile: {s.last_stats['iterations']} precision 48 iterations")
+        for kind in (1, 4):  # the fused iteration on CRS rows, plain and with x staged in LDS
+            s.solve(stop_mode=1, download=False, use_graph=0, raise_on_breakdown=False, spmv_kind=kind)
+            log(f"profile: {s.last_stats['iterations']} FP64 iterations, spmv kind {s.last_stats['spmv_kind_used']}")
         # element-matrix kernels: the reference-order one (kind 0, in the assembly above) against
         # the isotropic split on the vector units (1) and on the FP64 matrix cores (2)
         for kind in (1, 2, 0):
@@ -159,7 +162,7 @@ def main(argv=None):
                 ms = (time.perf_counter() - t1) * 1e3
             log(f"profile: stiffness with element kernel {kind} {ms:.2f} ms (host clock, with the row gather)")
         xr = np.random.default_rng(1).standard_normal(n)
-        for kind in (1, 2, 3):
+        for kind in (1, 4, 2, 3):
             _, ms = s.spmv(kind, xr, repeat=5, download=False)
             log(f"profile: spmv kind {kind} {ms:.3f} ms")
         # HBM counter calibration: a plain 512 MiB device-to-device copy through the HIP
@@ -271,7 +274,7 @@ def main(argv=None):
     s.itrmax = 40000
     spmv = {}
     xr = np.random.default_rng(1).standard_normal(n)
-    for kind, name in ((1, "crs_f64"), (2, "blocked_f64"), (3, "blocked_f32")):
+    for kind, name in ((1, "crs_f64"), (4, "crs_f64_staged"), (2, "blocked_f64"), (3, "blocked_f32")):
         s.spmv(kind, xr, repeat=3, download=False)
         _, ms = s.spmv(kind, None, repeat=20, download=False)
         gbs = spmv_bytes(kind, n, nnz) / (ms * 1e-3) / 1e9
@@ -874,6 +877,14 @@ def bench_overset(a):
                        "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                        "kernel": f"k_spmv_p ({'CRS' if kind == 1 else 'blocked'} FP64 y=A p fused with r0.y)",
                        "bytes_per_launch": k1_bytes, "avg_launch_ms": round(stt["spmv_p_ms"], 4)}
+    # the CRS products standalone on the coupled matrix: row kernel and x staged in LDS
+    xr = np.random.default_rng(1).standard_normal(int(crs.n))
+    out["spmv"] = {}
+    for kind, name in ((1, "crs_f64"), (4, "crs_f64_staged")):
+        s.spmv(kind, xr, repeat=3, download=False)
+        _, ms = s.spmv(kind, None, repeat=20, download=False)
+        gbs = spmv_bytes(kind, int(crs.n), int(crs.nnz)) / (ms * 1e-3) / 1e9
+        out["spmv"][name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)}
     log(f"overset: {json.dumps(out)}")
     # parity of the assembled system with the oracle (pattern and values bitwise)
     nn = [m[0].shape[0] for m in P.meshes]

@@ -60,7 +60,9 @@ typedef struct {
                                 matrix, computed at the first such solve and kept (ILU_MATRIX);
                               3 "PointJacobi" as the reference's branch runs it: the scaling and no
                                 solver call, so x = x0 (zeros) with the fixed values */
-  int32_t spmv_kind;     /* 0 auto (blocked when the CRS has DOF=3 node blocks), 1 CRS row kernel, 2 blocked */
+  int32_t spmv_kind;     /* 0 auto (blocked when the CRS has DOF=3 node blocks, else CRS rows with x staged
+                            in LDS for the unpreconditioned FP64 solve), 1 CRS row kernel, 2 blocked,
+                            4 CRS rows with x staged in LDS per row block (FP64, no ILU) */
   int32_t check_every;   /* iterations enqueued between host polls of the stop flag (default 32) */
   int32_t use_graph;     /* capture the polled chunk in a hipGraph (default 1) */
   int32_t debug;         /* print sqrt(r.r) per poll */
@@ -85,7 +87,7 @@ typedef struct {
   int32_t status;           /* PF_* code of the solve */
   int32_t iterations;       /* BiCGSTAB iterations (FP64 or, for precision 32, FP32) */
   int32_t outer_iterations; /* FP64 residual replacements (mixed), 0 for FP64 */
-  int32_t spmv_kind_used;   /* 1 CRS row kernel, 2 blocked */
+  int32_t spmv_kind_used;   /* 1 CRS row kernel, 2 blocked, 4 CRS rows with x staged in LDS */
   double rr;                /* final r.r of the recurrence (FP64 form) */
   double rr1;               /* r.r after iteration 1 */
   double true_relres;       /* ||b - A x|| / ||b|| in FP64 after the solve (eliminated system) */
@@ -357,7 +359,9 @@ int pf_system_ilu_reset(pf_system *sys);
 
 /* y = A x with the system's current CRS (elimination included once solve() has run).
  * kind 1: CRS row kernel, FP64; 2: blocked kernel, FP64; 3: blocked kernel, FP32 values and
- * vectors (x rounded to FP32, y widened back).  x, y are host arrays of length n (x NULL:
+ * vectors (x rounded to FP32, y widened back); 4: CRS row kernel with x staged in LDS per
+ * block of rows (the distinct columns of the block gathered once, 16-bit positions per entry;
+ * the same sums as kind 1, bitwise).  x, y are host arrays of length n (x NULL:
  * reuse the last x; y NULL: keep on device).  The kernel is launched `repeat` times and *ms
  * (may be NULL) receives the average device time of one launch (hipEvents). */
 int pf_system_spmv(pf_system *sys, int32_t kind, const double *x, double *y, int32_t repeat, double *ms);

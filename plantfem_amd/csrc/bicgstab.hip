@@ -38,7 +38,7 @@
 namespace {
 
 constexpr int BT = 256;  // threads per block of every fused kernel (4 wavefronts)
-constexpr int KEY_WORDS = 40;      // words of the key of a captured iteration chunk
+constexpr int KEY_WORDS = 45;      // words of the key of a captured iteration chunk
 constexpr int ILU_KEY_WORDS = 32;  // the same for the ILU chunk
 
 struct Scal {
@@ -73,14 +73,20 @@ struct Mat {
   const int32_t *width;
   const int32_t *col;
   const M *sval;
+  pfdev::Stage st;  // L = 2: the row blocks' staged columns
 };
 
 // Visit every row of y = A*xf with the calling thread's share of the grid; op(row, sum) is
 // called by exactly one thread per row.  L = 1: Sell3 (3 rows per lane), L = 0: CRS rows
-// reduced by SUB-lane subgroups.
+// reduced by SUB-lane subgroups, L = 2: the same CRS sums with x staged in LDS per row block
+// (blocks over row blocks, every thread of a block takes part).
 template <typename V, int L, int SUB, typename M, typename XF, typename OP>
 __device__ inline void for_rows(const Mat<V, M> &m, XF xf, OP op) {
-  if (L == 1) {
+  if constexpr (L == 2) {
+    __shared__ V sx[pfdev::ST_UMAX];
+    for (int64_t b = blockIdx.x; b < m.st.nb; b += gridDim.x)
+      pfdev::staged_block<M, V, SUB, BT>(m.row_ptr, m.col_idx, m.val, m.st, m.n, b, sx, xf, op);
+  } else if constexpr (L == 1) {
     const int64_t wave = (blockIdx.x * (int64_t)BT + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = ((int64_t)gridDim.x * BT) >> 6;
@@ -681,6 +687,7 @@ struct SolverWork {
   Scal *h_sc = nullptr;
   Sell3<double> s64;
   Sell3<float> s32;
+  CrsStage stage;  // CRS product with x staged in LDS per row block
   DevBuf<int32_t> flag;
   DevBuf<double> jscale;  // FP32 system row scale: 1/sqrt|a_ii| (symmetric) or 1/a_ii (row)
   DevBuf<double> jones;   // column scale of the row-scaled form
@@ -730,7 +737,7 @@ struct Launch {
   unsigned grid;    // SpMV kernels
   unsigned grid_e;  // element-wise kernels
   int sub;
-  int layout;  // 0 CRS, 1 Sell3
+  int layout;  // 0 CRS, 1 Sell3, 2 CRS with x staged in LDS per row block
 };
 
 // partial-sum rows of G doubles each: K1 (1 dot), K2 (2), K3 (2), and one region shared by
@@ -815,6 +822,8 @@ int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, const V *b, V
     k_init<V, 0, 4, true><<<g, BT, 0, st>>>(m, b, x, r, r0, p, pi);
   } else if (L.layout == 1) {
     k_init<V, 1, 64, false><<<g, BT, 0, st>>>(m, b, x, r, r0, p, pi);
+  } else if (L.layout == 2) {
+    PF_DISPATCH_SUB(L.sub, (k_init<V, 2, S_, false><<<g, BT, 0, st>>>(m, b, x, r, r0, p, pi)));
   } else {
     PF_DISPATCH_SUB(L.sub, (k_init<V, 0, S_, false><<<g, BT, 0, st>>>(m, b, x, r, r0, p, pi)));
   }
@@ -828,6 +837,10 @@ int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, V *x, V *r, 
                   Scal *sc, double *part, double *hist, int count, const Rel<V> *rel, V *ebuf,
                   hipEvent_t *ev = nullptr) {
   PF_CHECK(check_grid(ctx, L));
+  if (L.layout == 2 && !ebuf) {
+    pf_set_error("internal: the staged CRS layout needs the e buffer");
+    return PF_INVALID;
+  }
   hipStream_t st = ctx->stream;
   const size_t G = (size_t)ctx->num_cus * 8;
   double *p1 = part + PART_P1 * G, *p2 = part + PART_P2 * G, *p3 = part + PART_P3 * G;
@@ -839,6 +852,11 @@ int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, V *x, V *r, 
       k_spmv_p<V, 1, 64><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, p1);
       if (ev) PF_HIP(hipEventRecord(ev[5 * it + 1], st));
       k_spmv_e<V, 1, 64><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, p1, g1, p2);
+    } else if (L.layout == 2) {  // e streamed, then staged per row block like p
+      PF_DISPATCH_SUB(L.sub, (k_spmv_p<V, 2, S_><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, p1)));
+      if (ev) PF_HIP(hipEventRecord(ev[5 * it + 1], st));
+      k_evec<V><<<L.grid_e, BT, 0, st>>>(m.n, r, y, ebuf, sc, p1, g1);
+      PF_DISPATCH_SUB(L.sub, (k_spmv_ev<V, 2, S_><<<L.grid, BT, 0, st>>>(m, ebuf, v, sc, p2)));
     } else {
       PF_DISPATCH_SUB(L.sub, (k_spmv_p<V, 0, S_><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, p1)));
       if (ev) PF_HIP(hipEventRecord(ev[5 * it + 1], st));
@@ -894,7 +912,9 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V,
       rel ? (uint64_t)(uintptr_t)rel->b : 0, rel ? (uint64_t)(uintptr_t)rel->scale : 0,
       rel ? (uint64_t)(uintptr_t)rel->cscale : 0, rel ? (uint64_t)(uintptr_t)rel->zhi : 0,
       rel ? (uint64_t)(uintptr_t)rel->r : 0, rel ? (uint64_t)(uintptr_t)rel->r0 : 0,
-      rel ? (uint64_t)(uintptr_t)rel->p : 0, rel ? (uint64_t)(uintptr_t)rel->zlo : 0};
+      rel ? (uint64_t)(uintptr_t)rel->p : 0, rel ? (uint64_t)(uintptr_t)rel->zlo : 0,
+      (uint64_t)(uintptr_t)m.st.lidx, (uint64_t)(uintptr_t)m.st.ucnt, (uint64_t)(uintptr_t)m.st.ubase,
+      (uint64_t)(uintptr_t)m.st.ucol, (uint64_t)m.st.nb * 4099 + (uint64_t)m.st.rpb};
   if (use_graph && (w->gexec[slot] == nullptr || std::memcmp(w->gkey[slot], key, sizeof(key)) != 0)) {
     if (w->gexec[slot]) {
       PF_HIP(hipGraphExecDestroy(w->gexec[slot]));
@@ -963,6 +983,7 @@ void note_matrix(SolverWork *w, int64_t n, const double *val, bool &matrix_chang
   w->key_val = val;
   if (matrix_changed) {
     w->have_s64 = w->have_s32 = w->have_v32 = w->have_scale = false;
+    w->stage.ready = false;
     w->block_ok = -1;
     std::memset(w->gkey, 0, sizeof(w->gkey));
   }
@@ -1190,7 +1211,11 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   const double avg_row = (double)nnz / (double)n;
   int layout = 0;
-  if (o.spmv_kind != 1) {
+  if (o.spmv_kind < 0 || o.spmv_kind > 4 || o.spmv_kind == 3) {
+    pf_set_error("spmv_kind must be 0 (auto), 1 (CRS rows), 2 (blocked) or 4 (CRS rows, x staged in LDS)");
+    return PF_INVALID;
+  }
+  if (o.spmv_kind != 1 && o.spmv_kind != 4) {
     PF_CHECK(check_blocks(ctx, w, n, row_ptr, col_idx));
     layout = w->block_ok ? 1 : 0;
     if (o.spmv_kind == 2 && !layout) {
@@ -1261,6 +1286,15 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     PF_CHECK(w->ps.ensure(n));
     PF_CHECK(w->e.ensure(n));
   }
+  // the FP64 solve on CRS rows stages x per row block (auto, or spmv_kind 4)
+  if (o.spmv_kind == 4 && (refined || ilu)) {
+    pf_set_error("spmv_kind 4 (x staged in LDS) runs the FP64 solve without ILU only");
+    return PF_INVALID;
+  }
+  if (layout == 0 && !refined && !ilu && (o.spmv_kind == 0 || o.spmv_kind == 4)) {
+    if (!w->stage.ready) PF_CHECK(pf_crs_stage_build(ctx, n, row_ptr, col_idx, nnz, w->stage));
+    layout = 2;
+  }
   PF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   PF_HIP(hipEventSynchronize(ctx->ev1));
   float setup_ms = 0.f;
@@ -1272,7 +1306,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   // FP64 CRS rows: refinement residual and the final true residual
   Mat<double> mcrs = make_mat<double>(n, row_ptr, col_idx, val, nullptr);
   Launch Lres{fused_grid(ctx, n * (int64_t)sub), fused_grid(ctx, n), sub, 0};
-  res.kind = layout == 1 ? 2 : 1;
+  res.kind = layout == 1 ? 2 : (layout == 2 ? 4 : 1);
   const size_t G = (size_t)ctx->num_cus * 8;
   double *pres = w->part.p + PART_PI * G;
 
@@ -1294,13 +1328,19 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   } else if (!refined) {
     Mat<double> m64 = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
     Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), fused_grid(ctx, n), sub, layout};
+    if (layout == 2) {
+      m64.st = pfdev::Stage{w->stage.lidx.p, w->stage.ucnt.p, w->stage.ubase.p, w->stage.ucol.p, w->stage.nb,
+                            w->stage.rpb};
+      L64.grid = fused_grid(ctx, w->stage.nb * (int64_t)BT);
+      L64.sub = w->stage.sub;
+    }
     const int guard = o.stop_mode == 1 ? 1 : 0;  // modes 0 and 2 keep the reference's unguarded arithmetic
     // kernel stop rules: 0 bicgstab_CRS_2, 1 |r| < relative_er |b|, 3 bicgstab_CRS (er = er0)
     const int kmode = o.stop_mode == 2 ? 3 : o.stop_mode;
     PF_CHECK(prepare_scalars(ctx, w, 0, o, o.itrmax, o.relative_er, kmode, 0, guard));
     PF_CHECK(enqueue_init<double>(ctx, L64, m64, b, x, w->r.p, w->r0.p, w->p.p, w->sc.p, w->part.p, false));
     double *ebuf = nullptr;
-    if (layout == 0) {  // CRS: e as a stream, then the product gathers e alone
+    if (layout != 1) {  // CRS rows: e as a stream, then the product gathers (or stages) e alone
       PF_CHECK(w->e.ensure(n));
       ebuf = w->e.p;
     }
@@ -1462,13 +1502,14 @@ int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, 
                  int repeat, double *ms) {
   if (ms) *ms = 0.0;
   if (n <= 0) return PF_OK;
-  if (kind < 1 || kind > 3) {
-    pf_set_error("spmv kind must be 1, 2 or 3");
+  if (kind < 1 || kind > 4) {
+    pf_set_error("spmv kind must be 1, 2, 3 or 4");
     return PF_INVALID;
   }
   note_matrix(w, n, val, matrix_changed);
   PF_CHECK(w->flag.ensure(1));
-  if (kind != 1) {
+  if (kind == 4 && !w->stage.ready) PF_CHECK(pf_crs_stage_build(ctx, n, row_ptr, col_idx, nnz, w->stage));
+  if (kind == 2 || kind == 3) {
     PF_CHECK(check_blocks(ctx, w, n, row_ptr, col_idx));
     if (!w->block_ok) {
       pf_set_error("blocked SpMV requested but the CRS has no DOF=3 node-block structure");
@@ -1496,6 +1537,8 @@ int pf_work_spmv(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr, 
   for (int rep = 0; rep < repeat; ++rep) {
     if (kind == 1)
       PF_CHECK(pf_spmv_crs_launch<double>(ctx, n, row_ptr, col_idx, val, x, y, (double)nnz / (double)n));
+    else if (kind == 4)
+      PF_CHECK(pf_spmv_crs_stage_launch<double>(ctx, w->stage, row_ptr, col_idx, val, x, y));
     else if (kind == 2)
       PF_CHECK(pf_spmv_sell3_launch<double>(ctx, w->s64, x, y));
     else

@@ -77,7 +77,7 @@ extern "C" int pf_dev_bicgstab_crs_noalloc(pf_ctx *ctx, int64_t n, const int64_t
   }
   pf_solve_opts o;
   if (opts) o = *opts; else pf_solve_opts_default(&o);
-  if (o.precision != 64 || o.preconditioning != 0 || o.spmv_kind == 2) {
+  if (o.precision != 64 || o.preconditioning != 0 || (o.spmv_kind != 0 && o.spmv_kind != 1)) {
     pf_set_error("pf_dev_bicgstab_crs_noalloc: FP64, no preconditioning, CRS layout only");
     return PF_INVALID;
   }

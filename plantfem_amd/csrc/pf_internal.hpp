@@ -199,6 +199,29 @@ int pf_bc_set_fixed_x(pf_ctx *ctx, int64_t n, const uint8_t *fixed, const double
 template <typename V>
 int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                        const V *val, const V *x, V *y, double avg_row, bool accumulate = false);
+// CRS product with x staged in LDS per block of rpb rows: for block b the sorted distinct
+// 0-based columns of its entries (ucol[ubase[b] ..], ucnt[b] of them) and for every entry the
+// position of its column in that list (lidx, 16-bit).  ucnt[b] = -1 marks a block whose
+// entries or distinct columns exceed the staging limits; it gathers x through col_idx.
+// Depends on the pattern only (row_ptr, col_idx).
+struct CrsStage {
+  int64_t n = 0, nnz = 0, nb = 0;
+  int sub = 0, rpb = 0;
+  int64_t staged = 0, unique = 0;  // staged blocks; distinct columns summed over the blocks
+  DevBuf<int32_t> ucnt, usz, ucol;
+  DevBuf<int64_t> ubase, tmp;
+  DevBuf<uint16_t> lidx;
+  bool ready = false;
+};
+int pf_crs_stage_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, int64_t nnz,
+                       CrsStage &st);
+// y = A x (accumulate: y += A x), the sums of pf_spmv_crs_launch bitwise (same subgroup width,
+// same per-lane order), x read from the staged copies
+template <typename V>
+int pf_spmv_crs_stage_launch(pf_ctx *ctx, const CrsStage &st, const int64_t *row_ptr, const int32_t *col_idx,
+                             const V *val, const V *x, V *y, bool accumulate = false);
+// lanes per CRS row for an average row length (both CRS products use it)
+int pf_crs_sub(double avg_row);
 int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                              int64_t nn, int32_t *ok_dev, int *ok_host);
 // scale (may be NULL): values stored as s_i a_ij c_j, c = cscale (NULL: c = s, the symmetric form)

@@ -1,4 +1,6 @@
 // spmv.hip -- y = A x kernels and the blocked (Sell3) layout built from a DOF=3 CRS.
+#include <vector>
+
 #include "spmv_dev.hpp"
 
 namespace {
@@ -206,16 +208,137 @@ __global__ void k_crs_f32(int64_t n, const int64_t *__restrict__ row_ptr, const 
   }
 }
 
+// ---- CRS product with x staged in LDS per row block (pfdev::staged_block) -----------
+constexpr int ST_T = 256;      // threads of the staging and product kernels
+constexpr int ST_HASH = 4096;  // LDS hash slots of the staging (twice the distinct-column limit)
+constexpr int ST_PROBE = 64;   // probes after which a block is left unstaged
+
+__device__ inline int st_hash(int32_t c) { return (int)(((uint32_t)c * 2654435761u) >> 20); }  // 12 bits
+
+__device__ inline int block_excl_scan(int v, int *sh, int &total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < ST_T; off <<= 1) {
+    const int w = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += w;
+    __syncthreads();
+  }
+  total = sh[ST_T - 1];
+  const int incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+// One workgroup per row block.  The block's 0-based columns go into an LDS hash set (linear
+// probing); COUNT writes the number of distinct ones (ucnt[b]; -1 above ST_UMAX or after a
+// probe run of ST_PROBE).  FILL sorts them (bitonic, <= ST_UMAX), writes them to
+// ucol[ubase[b] ..] and every entry's rank among them to lidx; a block whose insertion now
+// needs too long a probe run is marked unstaged instead (slot placement depends on the order
+// of the atomics; the set of columns and hence the sorted list do not).
+template <bool FILL>
+__global__ void __launch_bounds__(ST_T) k_stage_build(int64_t n, int rpb, const int64_t *__restrict__ row_ptr,
+                                                      const int32_t *__restrict__ col_idx, int32_t *__restrict__ ucnt,
+                                                      const int64_t *__restrict__ ubase, int32_t *__restrict__ ucol,
+                                                      uint16_t *__restrict__ lidx) {
+  __shared__ int32_t key[ST_HASH];
+  __shared__ int32_t rank[FILL ? ST_HASH : 1];
+  __shared__ int32_t srt[FILL ? pfdev::ST_UMAX : 1];
+  __shared__ int sh[ST_T];
+  __shared__ int ovf;
+  const int tid = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  if (FILL && ucnt[b] < 0) return;  // whole block
+  const int64_t r0 = b * rpb;
+  const int64_t r1 = r0 + rpb < n ? r0 + rpb : n;
+  const int64_t k0 = row_ptr[r0] - 1, k1 = row_ptr[r1] - 1;
+  for (int i = tid; i < ST_HASH; i += ST_T) key[i] = -1;
+  if (tid == 0) ovf = 0;
+  __syncthreads();
+  for (int64_t q = k0 + tid; q < k1; q += ST_T) {
+    const int32_t c = col_idx[q] - 1;
+    int h = st_hash(c), t = 0;
+    for (; t < ST_PROBE; ++t) {
+      const int32_t prev = atomicCAS(&key[h], -1, c);
+      if (prev == -1 || prev == c) break;
+      h = (h + 1) & (ST_HASH - 1);
+    }
+    if (t == ST_PROBE) ovf = 1;
+  }
+  __syncthreads();
+  constexpr int CH = ST_HASH / ST_T;
+  int cnt = 0;
+  for (int i = 0; i < CH; ++i) cnt += key[tid * CH + i] >= 0;
+  int total = 0;
+  const int ex = block_excl_scan(cnt, sh, total);
+  const bool staged = !ovf && total <= pfdev::ST_UMAX;
+  if (!FILL || !staged) {  // COUNT, or FILL of a block that no longer fits
+    if (tid == 0) ucnt[b] = staged ? total : -1;
+    return;
+  }
+  int pos = ex;
+  for (int i = 0; i < CH; ++i) {
+    const int32_t c = key[tid * CH + i];
+    if (c >= 0) srt[pos++] = c;
+  }
+  int P = 1;
+  while (P < total) P <<= 1;
+  for (int i = total + tid; i < P; i += ST_T) srt[i] = INT32_MAX;
+  __syncthreads();
+  for (int kk = 2; kk <= P; kk <<= 1)
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < P; i += ST_T) {
+        const int l = i ^ j;
+        if (l > i) {
+          const int32_t x = srt[i], y = srt[l];
+          if ((i & kk) == 0 ? x > y : x < y) {
+            srt[i] = y;
+            srt[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  int32_t *uc = ucol + ubase[b];
+  for (int i = tid; i < total; i += ST_T) {
+    const int32_t c = srt[i];
+    uc[i] = c;
+    int h = st_hash(c);
+    while (key[h] != c) h = (h + 1) & (ST_HASH - 1);  // c is in the set
+    rank[h] = i;
+  }
+  __syncthreads();
+  for (int64_t q = k0 + tid; q < k1; q += ST_T) {
+    const int32_t c = col_idx[q] - 1;
+    int h = st_hash(c);
+    while (key[h] != c) h = (h + 1) & (ST_HASH - 1);
+    lidx[q] = (uint16_t)rank[h];
+  }
+}
+
+__global__ void k_stage_sizes(int64_t nb, const int32_t *__restrict__ ucnt, int32_t *__restrict__ usz) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b < nb) usz[b] = ucnt[b] > 0 ? ucnt[b] : 0;
+}
+
+template <typename V, int SUB, bool ACC>
+__global__ void __launch_bounds__(ST_T) k_spmv_crs_lds(int64_t n, pfdev::Stage st, const int64_t *__restrict__ row_ptr,
+                                                       const int32_t *__restrict__ col_idx, const V *__restrict__ val,
+                                                       const V *__restrict__ x, V *__restrict__ y) {
+  __shared__ V sx[pfdev::ST_UMAX];
+  auto op = [&](int64_t row, V s) { y[row] = ACC ? y[row] + s : s; };
+  pfdev::staged_block<V, V, SUB, ST_T>(row_ptr, col_idx, val, st, n, (int64_t)blockIdx.x, sx, pfdev::PlainX<V>{x},
+                                       op);
+}
+
 }  // namespace
 
 template <typename V>
 int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                        const V *val, const V *x, V *y, double avg_row, bool accumulate) {
   if (n <= 0) return PF_OK;
-  // subgroup of ~avg_row/5 lanes (4..64): about five nonzeros per lane (measured at 100^3,
-  // 79 nonzeros per row: 16 lanes 0.82 ms, 64 lanes 1.15 ms)
-  int sub = 4;
-  while (sub < 64 && sub * 1.4142 * 5.0 < avg_row) sub *= 2;
+  const int sub = pf_crs_sub(avg_row);
   const int64_t threads = n * sub;
   const unsigned blocks = pf_blocks(threads, 256, INT32_MAX);
 #define PF_SPMV_CRS_CASE(S)                                                                  \
@@ -241,6 +364,96 @@ template int pf_spmv_crs_launch<double>(pf_ctx *, int64_t, const int64_t *, cons
                                         const double *, double *, double, bool);
 template int pf_spmv_crs_launch<float>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const float *,
                                        const float *, float *, double, bool);
+
+int pf_crs_sub(double avg_row) {
+  // subgroup of ~avg_row/5 lanes (4..64): about five nonzeros per lane (measured at 100^3,
+  // 79 nonzeros per row: 16 lanes 0.82 ms, 64 lanes 1.15 ms)
+  int sub = 4;
+  while (sub < 64 && sub * 1.4142 * 5.0 < avg_row) sub *= 2;
+  return sub;
+}
+
+int pf_crs_stage_build(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx, int64_t nnz,
+                       CrsStage &st) {
+  st.ready = false;
+  st.n = n;
+  st.nnz = nnz;
+  st.sub = pf_crs_sub(n > 0 ? (double)nnz / (double)n : 1.0);
+  // ST_RPS rows per subgroup, halved while the matrix has fewer than 8 blocks per CU: every
+  // block starts with a dependent gather of its x entries, which other blocks must overlap
+  int rps = pfdev::ST_RPS;
+  st.rpb = rps * (ST_T / st.sub);
+  while (rps > 1 && n > 0 && (n + st.rpb - 1) / st.rpb < 8 * (int64_t)ctx->num_cus) {
+    rps /= 2;
+    st.rpb = rps * (ST_T / st.sub);
+  }
+  st.nb = n > 0 ? (n + st.rpb - 1) / st.rpb : 0;
+  st.staged = st.unique = 0;
+  if (st.nb == 0) {
+    st.ready = true;
+    return PF_OK;
+  }
+  if (st.nb > INT32_MAX) {
+    pf_set_error("staged CRS product: too many row blocks");
+    return PF_INVALID;
+  }
+  PF_CHECK(st.ucnt.ensure((size_t)st.nb));
+  PF_CHECK(st.usz.ensure((size_t)st.nb));
+  PF_CHECK(st.ubase.ensure((size_t)st.nb + 1));
+  PF_CHECK(st.lidx.ensure((size_t)nnz + 1));
+  k_stage_build<false><<<(unsigned)st.nb, ST_T, 0, ctx->stream>>>(n, st.rpb, row_ptr, col_idx, st.ucnt.p, nullptr,
+                                                                 nullptr, nullptr);
+  k_stage_sizes<<<pf_blocks(st.nb, 256, INT32_MAX), 256, 0, ctx->stream>>>(st.nb, st.ucnt.p, st.usz.p);
+  PF_HIP(hipGetLastError());
+  PF_CHECK(pf_scan_i32_to_i64(ctx, st.usz.p, st.nb, st.ubase.p, st.tmp));
+  int64_t total = 0;
+  PF_CHECK(pf_read_i64(ctx, st.ubase.p + st.nb, &total));
+  PF_CHECK(st.ucol.ensure((size_t)total + 1));
+  k_stage_build<true><<<(unsigned)st.nb, ST_T, 0, ctx->stream>>>(n, st.rpb, row_ptr, col_idx, st.ucnt.p,
+                                                                st.ubase.p, st.ucol.p, st.lidx.p);
+  PF_HIP(hipGetLastError());
+  std::vector<int32_t> h((size_t)st.nb);
+  PF_HIP(hipMemcpyAsync(h.data(), st.ucnt.p, sizeof(int32_t) * st.nb, hipMemcpyDeviceToHost, ctx->stream));
+  PF_HIP(hipStreamSynchronize(ctx->stream));
+  for (int32_t c : h) st.staged += c >= 0;
+  st.unique = total;
+  st.ready = true;
+  return PF_OK;
+}
+
+template <typename V>
+int pf_spmv_crs_stage_launch(pf_ctx *ctx, const CrsStage &st, const int64_t *row_ptr, const int32_t *col_idx,
+                             const V *val, const V *x, V *y, bool accumulate) {
+  if (!st.ready) {
+    pf_set_error("staged CRS product: staging not built");
+    return PF_INVALID;
+  }
+  if (st.nb == 0) return PF_OK;
+  const unsigned blocks = (unsigned)st.nb;
+  const pfdev::Stage sv{st.lidx.p, st.ucnt.p, st.ubase.p, st.ucol.p, st.nb, st.rpb};
+#define PF_SPMV_LDS_CASE(S)                                                                                   \
+  case S:                                                                                                    \
+    if (accumulate)                                                                                          \
+      k_spmv_crs_lds<V, S, true><<<blocks, ST_T, 0, ctx->stream>>>(st.n, sv, row_ptr, col_idx, val, x, y);  \
+    else                                                                                                     \
+      k_spmv_crs_lds<V, S, false><<<blocks, ST_T, 0, ctx->stream>>>(st.n, sv, row_ptr, col_idx, val, x, y); \
+    break;
+  switch (st.sub) {
+    PF_SPMV_LDS_CASE(4)
+    PF_SPMV_LDS_CASE(8)
+    PF_SPMV_LDS_CASE(16)
+    PF_SPMV_LDS_CASE(32)
+    default:
+    PF_SPMV_LDS_CASE(64)
+  }
+#undef PF_SPMV_LDS_CASE
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+template int pf_spmv_crs_stage_launch<double>(pf_ctx *, const CrsStage &, const int64_t *, const int32_t *,
+                                              const double *, const double *, double *, bool);
+template int pf_spmv_crs_stage_launch<float>(pf_ctx *, const CrsStage &, const int64_t *, const int32_t *,
+                                             const float *, const float *, float *, bool);
 
 int pf_block_structure_check(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                              int64_t nn, int32_t *ok_dev, int *ok_host) {

@@ -31,10 +31,12 @@ __device__ inline V subgroup_sum(V v) {
   return v;
 }
 
-// sum_k val[k] * xf(col[k]-1) over row `row`, lanes of a SUB-wide subgroup split the row
-template <typename V, typename A, int SUB, typename XF>
-__device__ inline A crs_row(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
-                            const V *__restrict__ val, int64_t row, int sub, XF xf) {
+// sum_k val[k] * xf(ix(k)) over row `row`, lanes of a SUB-wide subgroup split the row; ix(k)
+// is the index of entry k's column in whatever holds x (the 0-based column for x itself, a
+// position in LDS for a staged copy), so every form sums the same products in the same order
+template <typename V, typename A, int SUB, typename IX, typename XF>
+__device__ inline A crs_row_ix(const int64_t *__restrict__ row_ptr, IX ix, const V *__restrict__ val,
+                               int64_t row, int sub, XF xf) {
   const int64_t k0 = row_ptr[row] - 1, k1 = row_ptr[row + 1] - 1;
   A acc = 0;
   // plain loads: neighbouring subgroups share cache lines of val/col_idx (rows are not
@@ -43,13 +45,31 @@ __device__ inline A crs_row(const int64_t *__restrict__ row_ptr, const int32_t *
   A acc1 = 0, acc2 = 0, acc3 = 0;
   int64_t k = k0 + sub;
   for (; k + 3 * SUB < k1; k += 4 * SUB) {
-    acc += (A)val[k] * (A)xf((int64_t)col_idx[k] - 1);
-    acc1 += (A)val[k + SUB] * (A)xf((int64_t)col_idx[k + SUB] - 1);
-    acc2 += (A)val[k + 2 * SUB] * (A)xf((int64_t)col_idx[k + 2 * SUB] - 1);
-    acc3 += (A)val[k + 3 * SUB] * (A)xf((int64_t)col_idx[k + 3 * SUB] - 1);
+    acc += (A)val[k] * (A)xf(ix(k));
+    acc1 += (A)val[k + SUB] * (A)xf(ix(k + SUB));
+    acc2 += (A)val[k + 2 * SUB] * (A)xf(ix(k + 2 * SUB));
+    acc3 += (A)val[k + 3 * SUB] * (A)xf(ix(k + 3 * SUB));
   }
-  for (; k < k1; k += SUB) acc += (A)val[k] * (A)xf((int64_t)col_idx[k] - 1);
+  for (; k < k1; k += SUB) acc += (A)val[k] * (A)xf(ix(k));
   return subgroup_sum<A, SUB>((acc + acc1) + (acc2 + acc3));
+}
+
+// 0-based column of entry k of a 1-based CRS
+struct ColIx {
+  const int32_t *__restrict__ col_idx;
+  __device__ int64_t operator()(int64_t k) const { return (int64_t)col_idx[k] - 1; }
+};
+
+// position of entry k's column in its row block's staged copy of x
+struct LocalIx {
+  const uint16_t *__restrict__ lidx;
+  __device__ int operator()(int64_t k) const { return lidx[k]; }
+};
+
+template <typename V, typename A, int SUB, typename XF>
+__device__ inline A crs_row(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
+                            const V *__restrict__ val, int64_t row, int sub, XF xf) {
+  return crs_row_ix<V, A, SUB>(row_ptr, ColIx{col_idx}, val, row, sub, xf);
 }
 
 // the three row sums of node (slice s, lane l); xf(j) returns x_j
@@ -85,5 +105,53 @@ struct PlainX {
   const V *__restrict__ x;
   __device__ V operator()(int64_t j) const { return x[j]; }
 };
+
+// x staged in LDS (index = position in the row block's list of distinct columns)
+template <typename V>
+struct LdsX {
+  const V *x;
+  __device__ V operator()(int j) const { return x[j]; }
+};
+
+// ---- CRS rows with x staged in LDS per row block ------------------------------------
+// A row block is rpb consecutive rows.  Its entries reference few distinct columns (hex8
+// rows: ~12% of the entries), so the block gathers each distinct x entry once into LDS and
+// its rows then read only streams (values, 16-bit positions) and LDS instead of one
+// dependent col_idx -> x gather per entry.  ucnt[b] < 0: block b was not staged (above the
+// limits) and gathers through col_idx.
+constexpr int ST_UMAX = 2048;  // distinct columns of a staged block (16 KB of FP64 LDS)
+constexpr int ST_RPS = 4;      // rows per subgroup in a row block (fewer on small matrices)
+
+struct Stage {
+  const uint16_t *lidx;  // per entry: position of its column in the block's list
+  const int32_t *ucnt;   // per block: distinct columns, -1 = not staged
+  const int64_t *ubase;  // per block: offset of its list in ucol
+  const int32_t *ucol;   // the lists: sorted 0-based columns
+  int64_t nb;
+  int rpb;
+};
+
+// Rows of block b: op(row, sum) by one lane per row; sums bitwise those of crs_row (xf's values
+// are staged unchanged).  Every thread of the block calls this (it holds two block barriers).
+template <typename V, typename A, int SUB, int NT, typename XF, typename OP>
+__device__ inline void staged_block(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
+                                    const V *__restrict__ val, const Stage &st, int64_t n, int64_t b, A *sx, XF xf,
+                                    OP op) {
+  const int U = st.ucnt[b];
+  if (U > 0) {
+    const int32_t *uc = st.ucol + st.ubase[b];
+    for (int i = threadIdx.x; i < U; i += NT) sx[i] = (A)xf((int64_t)uc[i]);
+  }
+  __syncthreads();
+  const int sg = threadIdx.x / SUB, sub = threadIdx.x % SUB;
+  const int64_t r0 = b * st.rpb;
+  const int64_t r1 = r0 + st.rpb < n ? r0 + st.rpb : n;
+  for (int64_t row = r0 + sg; row < r1; row += NT / SUB) {  // whole subgroups
+    const A s = U >= 0 ? crs_row_ix<V, A, SUB>(row_ptr, LocalIx{st.lidx}, val, row, sub, LdsX<A>{sx})
+                       : crs_row<V, A, SUB>(row_ptr, col_idx, val, row, sub, xf);
+    if (sub == 0) op(row, s);
+  }
+  __syncthreads();  // the next block rewrites sx
+}
 
 }  // namespace pfdev

@@ -765,7 +765,8 @@ class FEMSolver:
         return out
 
     def spmv(self, kind: int, x=None, repeat: int = 1, download: bool = True):
-        """y = A x with kernel kind (1 CRS FP64, 2 blocked FP64, 3 blocked FP32); returns (y, ms/launch)."""
+        """y = A x with kernel kind (1 CRS FP64, 2 blocked FP64, 3 blocked FP32, 4 CRS FP64 with x staged
+        in LDS per row block, bitwise kind 1); returns (y, ms/launch)."""
         y = np.empty(self.n) if download else None
         ms = C.c_double(0)
         L.check(L.load().pf_system_spmv(self.h, int(kind), L.ptr(None if x is None else _f64(x), C.c_double),

@@ -1,0 +1,83 @@
+"""CRS product with x staged in LDS per row block (pf_system_spmv kind 4) against the CRS row
+kernel (kind 1, sub_crs_matvec, LinearSolverClass.f90:3728-3776): the same products summed in
+the same order per lane, so the results must be bitwise equal; both against the oracle's
+product.  Patterns: the hex8 ZeroMatrix (every block staged), and a general pattern with empty
+rows and blocks, repeated columns, a block above the staging's entry limit and one above its
+distinct-column limit (those blocks gather x through the column indices)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from plantfem_amd import workloads as W
+from plantfem_amd.fem import CRS, FEMDomain, FEMSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_staged_product_box(ctx):
+    w = W.box(12, sigma=1.0, seed=8)
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    for seed in (3, 4):
+        x = np.random.default_rng(seed).standard_normal(rhs.size)
+        say("staged spmv box, seed", seed)
+        y1, _ = s.spmv(1, x)
+        y4, _ = s.spmv(4, x)
+        assert np.array_equal(y1, y4)
+        ref = O.spmv(rp, ci, val, x)
+        assert np.abs(y4 - ref).max() <= 1e-13 * np.abs(ref).max()
+    # new values on the same pattern: the staging is rebuilt and stays exact
+    s.setStiffness(2.0 * w.E, w.nu)
+    x = np.random.default_rng(5).standard_normal(rhs.size)
+    y1, _ = s.spmv(1, x)
+    y4, _ = s.spmv(4, x)
+    assert np.array_equal(y1, y4)
+    s.close()
+
+
+def general_pattern(n, seed):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 120, n)
+    lens[rng.random(n) < 0.1] = 0       # empty rows
+    lens[256:384] = 0                   # empty row blocks
+    lens[500] = 9000                    # one row above the staging's entry limit
+    lens[1024:1280] = 40                # ~40 distinct columns per row over the whole range
+    cols = []
+    for i, m in enumerate(lens):
+        if 1024 <= i < 1280:
+            c = rng.integers(1, n + 1, m)
+        else:
+            lo = max(1, i - 300)
+            c = rng.integers(lo, min(n, i + 300) + 1, m)
+            if m > 4:
+                c[: m // 5] = c[0]      # repeated columns within the row
+        cols.append(c)
+    rp = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int64)
+    ci = np.concatenate(cols).astype(np.int32)
+    val = rng.standard_normal(ci.size)
+    return rp, ci, val
+
+
+def test_staged_product_general_pattern(ctx):
+    n = 20000
+    rp, ci, val = general_pattern(n, 11)
+    s = FEMSolver(ctx)
+    s.setCRS(CRS=CRS(rp, ci, val))
+    x = np.random.default_rng(12).standard_normal(n)
+    say("staged spmv general pattern: n", n, "nnz", ci.size)
+    y1, _ = s.spmv(1, x)
+    y4, _ = s.spmv(4, x)
+    assert np.array_equal(y1, y4)
+    ref = O.spmv(rp, ci, val, x)
+    assert np.abs(y4 - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.all(y4[np.diff(rp) == 0] == 0.0)
+    s.close()

@@ -102,7 +102,7 @@ def reduced():
     return out
 
 
-@pytest.mark.parametrize("kind", [2, 1])
+@pytest.mark.parametrize("kind", [2, 1, 4])
 def test_c2_generator_fp64_matches_oracle(ctx, reduced, kind):
     w, xo, ito = reduced[2]
     s = solver_for(w, ctx)

@@ -359,9 +359,10 @@ def test_breakdown_and_itrmax_statuses(ctx):
     assert st["status"] == L.PF_NOT_CONVERGED and st["iterations"] == 3
 
 
-@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("kind", [1, 2, 4])
 def test_c1_solve_matches_oracle(ctx, kind):
-    """Config 1 end to end on the GPU vs the oracle, both converged tightly: 1e-10 relative."""
+    """Config 1 end to end on the GPU vs the oracle, both converged tightly: 1e-10 relative (CRS rows,
+    blocked, CRS rows with x staged in LDS)."""
     w = W.config(1)
     say("C1 solve kind", kind)
     s = FEMSolver(ctx)
