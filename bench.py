@@ -875,7 +875,8 @@ def bench_overset(a):
     k1_gbs = k1_bytes / (stt["spmv_p_ms"] * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                       "kernel": f"k_spmv_p ({'CRS' if kind == 1 else 'blocked'} FP64 y=A p fused with r0.y)",
+                       "kernel": "k_spmv_p (" + {1: "CRS rows", 2: "blocked", 4: "CRS rows, x staged in LDS"}[kind]
+                                 + " FP64 y=A p fused with r0.y)",
                        "bytes_per_launch": k1_bytes, "avg_launch_ms": round(stt["spmv_p_ms"], 4)}
     # the CRS products standalone on the coupled matrix: row kernel and x staged in LDS
     xr = np.random.default_rng(1).standard_normal(int(crs.n))

@@ -757,6 +757,18 @@ unsigned fused_grid(pf_ctx *ctx, int64_t work_threads) {
   return (unsigned)b;
 }
 
+// element-wise kernels (K3, K4, the e stream, initialisation): every block of the next kernel
+// re-sums all partials of the previous one, so these stream with fewer, fuller blocks (measured
+// at 100^3: K3 0.042 ms with 8 blocks per CU, 0.035 ms with 4)
+constexpr int ELEM_BLOCKS_PER_CU = 4;
+unsigned elem_grid(pf_ctx *ctx, int64_t n) {
+  int64_t b = (n + BT - 1) / BT;
+  const int64_t cap = (int64_t)ctx->num_cus * ELEM_BLOCKS_PER_CU;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
 // every partial row holds one double per block of the writing kernel: grids above G would
 // write past w->part
 int check_grid(pf_ctx *ctx, const Launch &L) {
@@ -1119,7 +1131,7 @@ int solve_refined(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr,
                   const double *val, const double *b, double *x, const pf_solve_opts &o, int layout, int sub,
                   int smode, const double *scale, const double *cscale, SolveResult &res) {
   const size_t G = (size_t)ctx->num_cus * 8;
-  const unsigned ge = fused_grid(ctx, n);
+  const unsigned ge = elem_grid(ctx, n);
   const int64_t sell_threads = ((n / 3 + 63) / 64) * 64;
   PF_CHECK(w->zhi.ensure(n));
   VI *zlo, *r, *r0, *p, *y, *v;
@@ -1230,8 +1242,9 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
       // the sliced copy pads every 64-node slice to its widest row: when row lengths vary a
       // lot (the rows coupled by overset penalties) it outgrows the CRS, which then streams
       // fewer bytes (matrices below a million entries stay on the sliced copy: launch
-      // latency, not bandwidth, sets their time)
-      if (nnz >= ((int64_t)1 << 20) && (double)w->s64.nslots * 64.0 * 9.0 > 1.5 * (double)nnz) layout = 0;
+      // latency, not bandwidth, sets their time).  Measured on overset plants: values padded
+      // 1.48x, y = A p + v = A e 0.495 ms blocked against 0.384 ms on CRS rows.
+      if (nnz >= ((int64_t)1 << 20) && (double)w->s64.nslots * 64.0 * 9.0 > 1.2 * (double)nnz) layout = 0;
     }
   }
   if (o.precision != 64 && o.precision != 32 && o.precision != 48) {
@@ -1286,12 +1299,16 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     PF_CHECK(w->ps.ensure(n));
     PF_CHECK(w->e.ensure(n));
   }
-  // the FP64 solve on CRS rows stages x per row block (auto, or spmv_kind 4)
+  // the FP64 solve on CRS rows stages x per row block (spmv_kind 4; auto for rows of at most
+  // ~110 entries, i.e. 16 lanes per row: 1.23x per product on the hex8 box, while on the
+  // overset plants, rows of 24 to ~950 entries at 32 lanes, the fused staged products ran ~7%
+  // slower than plain rows: subgroups of one block wait at its barriers for its longest row)
   if (o.spmv_kind == 4 && (refined || ilu)) {
     pf_set_error("spmv_kind 4 (x staged in LDS) runs the FP64 solve without ILU only");
     return PF_INVALID;
   }
-  if (layout == 0 && !refined && !ilu && (o.spmv_kind == 0 || o.spmv_kind == 4)) {
+  if (layout == 0 && !refined && !ilu &&
+      (o.spmv_kind == 4 || (o.spmv_kind == 0 && choose_sub(avg_row) <= 16))) {
     if (!w->stage.ready) PF_CHECK(pf_crs_stage_build(ctx, n, row_ptr, col_idx, nnz, w->stage));
     layout = 2;
   }
@@ -1305,7 +1322,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   const int64_t sell_threads = ((n / 3 + 63) / 64) * 64;
   // FP64 CRS rows: refinement residual and the final true residual
   Mat<double> mcrs = make_mat<double>(n, row_ptr, col_idx, val, nullptr);
-  Launch Lres{fused_grid(ctx, n * (int64_t)sub), fused_grid(ctx, n), sub, 0};
+  Launch Lres{fused_grid(ctx, n * (int64_t)sub), elem_grid(ctx, n), sub, 0};
   res.kind = layout == 1 ? 2 : (layout == 2 ? 4 : 1);
   const size_t G = (size_t)ctx->num_cus * 8;
   double *pres = w->part.p + PART_PI * G;
@@ -1313,7 +1330,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
   PF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   if (ilu) {
     Mat<double> m64 = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
-    Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), fused_grid(ctx, n), sub, layout};
+    Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), elem_grid(ctx, n), sub, layout};
     const int guard = o.stop_mode == 1 ? 1 : 0;
     PF_CHECK(prepare_scalars(ctx, w, 0, o, o.itrmax, o.relative_er, o.stop_mode == 1 ? 1 : 0, 0, guard));
     PF_CHECK(enqueue_init<double>(ctx, L64, m64, b, x, w->r.p, w->r0.p, w->p.p, w->sc.p, w->part.p, false));
@@ -1327,7 +1344,7 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     res.rr1 = h.rr1;
   } else if (!refined) {
     Mat<double> m64 = make_mat<double>(n, row_ptr, col_idx, val, layout == 1 ? &w->s64 : nullptr);
-    Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), fused_grid(ctx, n), sub, layout};
+    Launch L64{fused_grid(ctx, layout == 1 ? sell_threads : n * (int64_t)sub), elem_grid(ctx, n), sub, layout};
     if (layout == 2) {
       m64.st = pfdev::Stage{w->stage.lidx.p, w->stage.ucnt.p, w->stage.ubase.p, w->stage.ucol.p, w->stage.nb,
                             w->stage.rpb};
@@ -1433,7 +1450,7 @@ int pf_bicgstab_crs_workspace(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, co
 
   const int sub = choose_sub((double)nnz / (double)n);
   const Mat<double> m = make_mat<double>(n, row_ptr, col_idx, val, nullptr);
-  const Launch L{fused_grid(ctx, n * (int64_t)sub), fused_grid(ctx, n), sub, 0};
+  const Launch L{fused_grid(ctx, n * (int64_t)sub), elem_grid(ctx, n), sub, 0};
   Scal h;
   std::memset(&h, 0, sizeof(h));
   h.er0 = o.er0;
