@@ -47,23 +47,43 @@ __global__ void __launch_bounds__(256) k_bc_rows(int64_t n, const int64_t *__res
   if (i >= n) return;  // whole subgroups leave together (SUB divides 64)
   const int64_t k0 = row_ptr[i] - 1, k1 = row_ptr[i + 1] - 1;
   const bool fr = fixed[i] != 0;
+  // four column loads, then four flag gathers, in flight per lane
   int anyf = 0;
-  for (int64_t k = k0 + lane; k < k1; k += SUB) anyf |= fixed[col_idx[k] - 1];
+  int64_t k = k0 + lane;
+  for (; k + 3 * SUB < k1; k += 4 * SUB) {
+    const int32_t c0 = col_idx[k], c1 = col_idx[k + SUB], c2 = col_idx[k + 2 * SUB], c3 = col_idx[k + 3 * SUB];
+    anyf |= (fixed[c0 - 1] | fixed[c1 - 1]) | (fixed[c2 - 1] | fixed[c3 - 1]);
+  }
+  for (; k < k1; k += SUB) anyf |= fixed[col_idx[k] - 1];
 #pragma unroll
   for (int off = SUB / 2; off > 0; off >>= 1) anyf |= __shfl_xor(anyf, off, SUB);
-  if (lane == 0) {
-    if (fr) {
-      rhs[i] = ubar[i];
-    } else if (anyf) {
-      double b = rhs[i];
-      for (int64_t k = k0; k < k1; ++k) {
+  if (fr) {
+    if (lane == 0) rhs[i] = ubar[i];
+  } else if (anyf) {
+    // b = b - a_ik ubar_k over the fixed columns in CRS order: the lanes load SUB terms at a
+    // time, then every lane subtracts them in entry order from shuffles (the same roundings as
+    // one sequential loop)
+    double b = rhs[i];
+    for (int64_t base = k0; base < k1; base += SUB) {
+      const int64_t k = base + lane;
+      double term = 0.0;
+      int has = 0;
+      if (k < k1) {
         const int64_t c = col_idx[k] - 1;
-        if (fixed[c]) b = b - val[k] * ubar[c];
+        if (fixed[c]) {
+          term = val[k] * ubar[c];
+          has = 1;
+        }
       }
-      rhs[i] = b;
+      const int cnt = k1 - base < SUB ? (int)(k1 - base) : SUB;
+      for (int t = 0; t < cnt; ++t) {
+        const double tt = __shfl(term, t, SUB);
+        if (__shfl(has, t, SUB)) b = b - tt;
+      }
     }
+    if (lane == 0) rhs[i] = b;
   }
-  // the first lane's reads of val above complete before these stores (one wavefront, in order)
+  // the reads of val above have been consumed (shuffled) before these stores
   if (fr || anyf)
     for (int64_t k = k0 + lane; k < k1; k += SUB) {
       const int64_t c = col_idx[k] - 1;

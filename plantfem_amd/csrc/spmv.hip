@@ -49,19 +49,21 @@ __global__ void __launch_bounds__(256) k_block_check(int64_t nn, const int64_t *
   const int64_t i = g / SUB;
   const int lane = (int)(g % SUB);
   if (i >= nn) return;
-  const int64_t r0 = row_ptr[3 * i] - 1;
-  const int64_t L = row_ptr[3 * i + 1] - 1 - r0;
-  bool ok = (L % 3) == 0;
-  for (int d = 1; d < 3 && ok; ++d)
-    ok = (row_ptr[3 * i + d + 1] - row_ptr[3 * i + d]) == L;
-  const int64_t m = L / 3;
-  for (int64_t t = lane; t < m && ok; t += SUB) {
-    const int32_t c = col_idx[r0 + 3 * t];
-    ok = (c - 1) % 3 == 0;
-    for (int d = 0; d < 3 && ok; ++d) {
-      const int64_t rb = row_ptr[3 * i + d] - 1 + 3 * t;
-      ok = col_idx[rb] == c && col_idx[rb + 1] == c + 1 && col_idx[rb + 2] == c + 2;
-    }
+  const int64_t p0 = row_ptr[3 * i], p1 = row_ptr[3 * i + 1], p2 = row_ptr[3 * i + 2], p3 = row_ptr[3 * i + 3];
+  const int64_t L = p1 - p0;
+  if (L % 3 != 0 || p2 - p1 != L || p3 - p2 != L) {
+    if (lane == 0) atomicOr(bad, 1);
+    return;
+  }
+  // the three rows have L entries each: all nine loads of a triple are in bounds and in flight
+  const int32_t *a = col_idx + (p0 - 1), *b = col_idx + (p1 - 1), *e = col_idx + (p2 - 1);
+  bool ok = true;
+  for (int64_t t = 3 * (int64_t)lane; t < L; t += 3 * SUB) {
+    const int32_t a0 = a[t], a1 = a[t + 1], a2 = a[t + 2];
+    const int32_t b0 = b[t], b1 = b[t + 1], b2 = b[t + 2];
+    const int32_t e0 = e[t], e1 = e[t + 1], e2 = e[t + 2];
+    ok = ok && (a0 - 1) % 3 == 0 && a1 == a0 + 1 && a2 == a0 + 2 && b0 == a0 && b1 == a0 + 1 && b2 == a0 + 2 &&
+         e0 == a0 && e1 == a0 + 1 && e2 == a0 + 2;
   }
   if (!ok) atomicOr(bad, 1);
 }

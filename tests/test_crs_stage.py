@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from plantfem_amd import _lib as L
 from plantfem_amd import workloads as W
 from plantfem_amd.fem import CRS, FEMDomain, FEMSolver
 
@@ -81,3 +82,50 @@ def test_staged_product_general_pattern(ctx):
     assert np.abs(y4 - ref).max() <= 1e-12 * np.abs(ref).max()
     assert np.all(y4[np.diff(rp) == 0] == 0.0)
     s.close()
+
+
+def test_block_check_accepts_and_rejects(ctx):
+    """The DOF=3 node-block check that gates the blocked layout (every node's three rows share one
+    list of aligned column triples, the layout ZeroMatrix(DOF=3) produces): the hex8 pattern
+    passes and the blocked product equals the CRS one; each broken variant is refused with an
+    error instead of being read as blocks."""
+    w = W.box(4, sigma=0.5, seed=2)
+    rp, ci, val, rhs, fixed, ub = O.build_system(w)
+    n = rp.size - 1
+    x = np.random.default_rng(7).standard_normal(n)
+
+    def blocked(rp_, ci_, val_):
+        s = FEMSolver(ctx)
+        s.setCRS(CRS=CRS(rp_, ci_, val_))
+        try:
+            y2, _ = s.spmv(2, x)
+            y1, _ = s.spmv(1, x)
+            return y2, y1
+        finally:
+            s.close()
+
+    y2, y1 = blocked(rp, ci, val)
+    assert np.abs(y2 - y1).max() <= 1e-13 * np.abs(y1).max()
+    i = 7  # a node; its rows 3i, 3i+1, 3i+2 (0-based)
+    r0, r1, r2 = rp[3 * i] - 1, rp[3 * i + 1] - 1, rp[3 * i + 2] - 1
+    variants = {}
+    c = ci.copy()                                   # a triple out of order in one row
+    c[r1], c[r1 + 1] = ci[r1 + 1], ci[r1]
+    variants["swapped"] = (rp, c, val)
+    c = ci.copy()                                   # the middle row holds another node's triple
+    j = 3 if ci[r1] != ci[r1 + 3] else 6
+    c[r1:r1 + 3] = ci[r1 + j:r1 + j + 3]
+    variants["foreign"] = (rp, c, val)
+    c = ci.copy()                                   # a misaligned triple in all three rows
+    for rs in (r0, r1, r2):
+        c[rs:rs + 3] = (ci[r0:r0 + 3] % n) + 1 if ci[r0 + 2] < n else ci[r0:r0 + 3] - 1
+    variants["misaligned"] = (rp, c, val)
+    keep = np.ones(ci.size, bool)                   # the first row of the node one triple short
+    keep[r0:r0 + 3] = False
+    rp_short = rp.copy()
+    rp_short[3 * i + 1:] -= 3
+    variants["short"] = (rp_short, ci[keep], val[keep])
+    for name, (rp_, ci_, val_) in variants.items():
+        say("block check variant", name)
+        with pytest.raises(L.PFError):
+            blocked(rp_, ci_, val_)
