@@ -91,8 +91,9 @@ struct NodeGraph {
 };
 
 // blocked sliced-ELL copy of a DOF=3 CRS: 64 nodes per slice (one wavefront), slot t of
-// node l of slice s at block index soff[s] + t*64 + l; 3x3 values stored component-major
-// inside the slot (q = 3*row_dof + col_dof): val[(soff[s] + t*64)*9 + q*64 + l].
+// node l of slice s at block index (soff[s] + t)*64 + l; the 3x3 values of the slot's 64
+// blocks (q = 3*row_dof + col_dof) at val[(soff[s] + t)*576 + sell3_value_index<V>(q, l)]:
+// 16-byte groups per lane for q < 8, the ninth value after them.
 template <typename V>
 struct Sell3 {
   int64_t nn = 0, nslices = 0, nslots = 0;  // nslots = soff[nslices]

@@ -132,7 +132,7 @@ __global__ void k_sell3_vals(SliceNodes sn, int64_t nslices, int64_t ncells, con
     }
     const int64_t i = sn.first(lo) + lane;
     const int64_t t = slot - soff[lo];
-    V *o = out + slot * 64 * 9 + lane;
+    V *o = out + slot * 64 * 9;
     bool live = false;
     if (i < sn.end(lo)) {
       const int64_t len = row_ptr[3 * i + 1] - row_ptr[3 * i];
@@ -142,7 +142,7 @@ __global__ void k_sell3_vals(SliceNodes sn, int64_t nslices, int64_t ncells, con
       for (int k = 0; k < 3; ++k) {
         double a = live ? val[row_ptr[3 * i + d] - 1 + 3 * t + k] : 0.0;
         if (live && scale) a = (a * scale[3 * i + d]) * cscale[3 * (int64_t)col[g] + k];
-        o[(3 * d + k) * 64] = (V)a;
+        o[pfdev::sell3_value_index<V>(3 * d + k, lane)] = (V)a;
       }
   }
 }
