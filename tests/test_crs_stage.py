@@ -129,3 +129,25 @@ def test_block_check_accepts_and_rejects(ctx):
         say("block check variant", name)
         with pytest.raises(L.PFError):
             blocked(rp_, ci_, val_)
+
+
+def test_staged_layout_option_errors(ctx):
+    """spmv_kind 4 (CRS rows with x staged) is an FP64 layout without ILU; kind 3 is no solver
+    layout.  Each refusal is an error before any kernel runs, and the system stays usable."""
+    w = W.box(5, sigma=0.5, seed=3)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    for kw in ({"spmv_kind": 3}, {"spmv_kind": 4, "precision": 32}, {"spmv_kind": 4, "preconditioning": "ILU"},
+               {"spmv_kind": 7}):
+        say("refused", kw)
+        with pytest.raises(L.PFError):
+            s.solve(**kw)
+    s.relative_er = 1e-12
+    x4 = s.solve(spmv_kind=4)
+    assert s.last_stats["spmv_kind_used"] == 4 and s.last_stats["status"] == 0
+    x1 = s.solve(spmv_kind=1)
+    assert np.linalg.norm(x4 - x1) <= 1e-9 * np.linalg.norm(x1)
+    s.close()
