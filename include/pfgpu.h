@@ -61,9 +61,9 @@ typedef struct {
                               3 "PointJacobi" as the reference's branch runs it: the scaling and no
                                 solver call, so x = x0 (zeros) with the fixed values */
   int32_t spmv_kind;     /* 0 auto (blocked when the CRS has DOF=3 node blocks and pads little, else CRS
-                            rows, with x staged in LDS for the unpreconditioned FP64 solve when rows
-                            average up to ~110 entries), 1 CRS row kernel, 2 blocked, 4 CRS rows with
-                            x staged in LDS per row block (FP64, no ILU) */
+                            rows, with x staged in LDS for the unpreconditioned FP64 solve from 2^18
+                            rows on), 1 CRS row kernel, 2 blocked, 4 CRS rows with x staged in LDS per
+                            row block (FP64, no ILU) */
   int32_t check_every;   /* iterations enqueued between host polls of the stop flag (default 32) */
   int32_t use_graph;     /* capture the polled chunk in a hipGraph (default 1) */
   int32_t debug;         /* print sqrt(r.r) per poll */

@@ -289,8 +289,11 @@ __global__ void __launch_bounds__(BT) k_evec(int64_t n, const V *__restrict__ r,
   for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) e[i] = ef(i);
 }
 
+// at most 64 VGPRs (8 waves per SIMD): the grid of 8 blocks per CU is then resident at once
+// (the staged form at 32 lanes per row took 66 and ran 0.241 instead of 0.177 ms on a 465k-row
+// overset system)
 template <typename V, int L, int SUB, typename M>
-__global__ void __launch_bounds__(BT) k_spmv_ev(Mat<V, M> m, const V *__restrict__ e, V *__restrict__ v,
+__global__ void __launch_bounds__(BT, 8) k_spmv_ev(Mat<V, M> m, const V *__restrict__ e, V *__restrict__ v,
                                                 Scal *__restrict__ sc, double *__restrict__ part) {
   if (sc->done) return;
   const pfdev::PlainX<V> ef{e};
@@ -1299,16 +1302,15 @@ int pf_bicgstab_device(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row
     PF_CHECK(w->ps.ensure(n));
     PF_CHECK(w->e.ensure(n));
   }
-  // the FP64 solve on CRS rows stages x per row block (spmv_kind 4; auto for rows of at most
-  // ~110 entries, i.e. 16 lanes per row: 1.23x per product on the hex8 box, while on the
-  // overset plants, rows of 24 to ~950 entries at 32 lanes, the fused staged products ran ~7%
-  // slower than plain rows: subgroups of one block wait at its barriers for its longest row)
+  // the FP64 solve on CRS rows stages x per row block (spmv_kind 4; auto from 2^18 rows: y = A p
+  // plus v = A e 1.458 -> 1.188 ms on the 3.09M-row hex box and 0.383 -> 0.346 ms on a 465k-row
+  // overset system, but 0.095 -> 0.100 ms on a 148k-row one, whose few row blocks per
+  // workgroup leave the staging gathers exposed)
   if (o.spmv_kind == 4 && (refined || ilu)) {
     pf_set_error("spmv_kind 4 (x staged in LDS) runs the FP64 solve without ILU only");
     return PF_INVALID;
   }
-  if (layout == 0 && !refined && !ilu &&
-      (o.spmv_kind == 4 || (o.spmv_kind == 0 && choose_sub(avg_row) <= 16))) {
+  if (layout == 0 && !refined && !ilu && (o.spmv_kind == 4 || (o.spmv_kind == 0 && n >= ((int64_t)1 << 18)))) {
     if (!w->stage.ready) PF_CHECK(pf_crs_stage_build(ctx, n, row_ptr, col_idx, nnz, w->stage));
     layout = 2;
   }
