@@ -11,9 +11,10 @@
 // rounding matches a plain IEEE evaluation of the reference formulas.
 //
 // Assembly (StiffnessMatrix_as_CRS_FEMDomain, FEMDomainClass.f90:11193-11229) is a
-// deterministic gather instead of the reference's OpenMP reduction: one thread owns one
-// CRS row, walks the row node's incident elements in ascending id and adds their Ke rows
-// into its own row, so every entry is summed in ascending element order with no atomics.
+// deterministic gather instead of the reference's OpenMP reduction: the element matrices are
+// formed once into a scratch array, then one wavefront per node (k_gather_nodes) walks the
+// node's incident elements in ascending id and adds their Ke rows into the node's three CRS
+// rows, so every entry is summed in ascending element order with no atomics.
 #include <algorithm>
 
 #include "pf_internal.hpp"
@@ -538,6 +539,99 @@ __global__ void __launch_bounds__(256) k_gather_rows(int64_t nn, int64_t ne, con
   }
 }
 
+// The same sums per node instead of per row: one wavefront per node i gathers its three CRS
+// rows (3 x 3 deg entries, lane l holding entries l, l + 64, ..., EPN of them), so the node's
+// incident elements and their connectivity are loaded once (lane 8 el + b holds node b of
+// element el of a chunk of 8, shared by shuffles) and the element-matrix loads of all three
+// rows are in flight together.  Every entry still sums Ke_e(3a+d, 3b+d') over the incident
+// elements in ascending id, local nodes a and b ascending, from 0.0: bitwise k_gather_rows.
+// Needs 9 deg <= 64 EPN.
+template <int EPN>
+__global__ void __launch_bounds__(256) k_gather_nodes(int64_t nn, int64_t ne, const int32_t *__restrict__ en,
+                                                      const int64_t *__restrict__ eoff,
+                                                      const int32_t *__restrict__ elist,
+                                                      const int64_t *__restrict__ noff,
+                                                      const int32_t *__restrict__ ncol,
+                                                      const int64_t *__restrict__ row_ptr,
+                                                      const double *__restrict__ ke, double *__restrict__ val) {
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (i >= nn) return;  // whole wavefronts
+  const int64_t n0 = noff[i];
+  const int len = 3 * (int)(noff[i + 1] - n0);  // entries of each of the node's rows
+  const int64_t s0 = eoff[i], s1 = eoff[i + 1];
+  int32_t jn[EPN];
+  int dd[EPN], dq[EPN], dc[EPN];
+  double v[EPN];
+#pragma unroll
+  for (int k = 0; k < EPN; ++k) {
+    const int P = lane + 64 * k;
+    const bool live = P < 3 * len;
+    dd[k] = live ? P / len : 0;
+    dq[k] = live ? P - dd[k] * len : 0;
+    dc[k] = dq[k] % 3;
+    jn[k] = live ? ncol[n0 + dq[k] / 3] + 1 : 0;  // node ids are >= 1: 0 matches no node
+    v[k] = 0.0;
+  }
+  for (int64_t c0 = s0; c0 < s1; c0 += 8) {
+    const int mc = (int)(s1 - c0 < 8 ? s1 - c0 : 8);
+    int32_t el_id = 0, nv = 0;
+    if (lane < 8 * mc) {
+      el_id = elist[c0 + (lane >> 3)];
+      nv = en[(int64_t)(lane & 7) * ne + el_id];
+    }
+    const int32_t prev0 = c0 > s0 ? elist[c0 - 1] : -1;  // the listing before this chunk
+    const uint64_t self = __ballot(nv == (int32_t)(i + 1));  // (element, local node) pairs of node i
+    for (int el = 0; el < mc; ++el) {
+      const int32_t e = __shfl(el_id, 8 * el, 64);
+      const int32_t ep = el > 0 ? __shfl(el_id, 8 * (el - 1), 64) : prev0;
+      if (ep == e) continue;  // node listed twice in one element
+      int32_t nodes[8];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) nodes[b] = __shfl(nv, 8 * el + b, 64);
+      const unsigned am = (unsigned)(self >> (8 * el)) & 0xffu;
+      // each entry's local node b in this element (an element lists a node once but a
+      // degenerate one may repeat it: then every match adds, in ascending b)
+      int bk[EPN];
+      bool multi = false;
+#pragma unroll
+      for (int k = 0; k < EPN; ++k) {
+        int bb = -1, cnt = 0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+          if (nodes[b] == jn[k]) {
+            bb = bb < 0 ? b : bb;
+            ++cnt;
+          }
+        bk[k] = bb;
+        multi = multi || cnt > 1;
+      }
+#pragma unroll 1
+      for (int a = 0; a < 8; ++a) {
+        if (!((am >> a) & 1u)) continue;
+        const double *kr = ke + (int64_t)e * 576 + (3 * a) * 24;
+        if (!multi) {
+          double ld[EPN];  // loads from in-bounds addresses, all in flight, then selected
+#pragma unroll
+          for (int k = 0; k < EPN; ++k) ld[k] = kr[dd[k] * 24 + 3 * (bk[k] < 0 ? 0 : bk[k]) + dc[k]];
+#pragma unroll
+          for (int k = 0; k < EPN; ++k)
+            if (bk[k] >= 0) v[k] = v[k] + ld[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < EPN; ++k)
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+              if (nodes[b] == jn[k]) v[k] = v[k] + kr[dd[k] * 24 + 3 * b + dc[k]];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < EPN; ++k)
+    if (lane + 64 * k < 3 * len) val[row_ptr[3 * i + dd[k]] - 1 + dq[k]] = v[k];
+}
+
 // N_a = 0.1250d0*(1 -+ xi)*(1 -+ eta)*(1 -+ zeta) at Gauss point gp (ShapeFunctionClass.f90:781-788)
 __device__ inline double hex8_N(int a, int gp) {
   const double gx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? GPV : -GPV;
@@ -1030,8 +1124,12 @@ int pf_assemble_hex8(pf_ctx *ctx, const NodeGraph &g, const double *E, const dou
     else
       k_ke_iso<true><<<pf_blocks(waves * 64, 64 * KI_W, INT32_MAX), 64 * KI_W, 0, ctx->stream>>>(
           g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, 0, g.ne, g.kebuf.p);
-    k_gather_rows<16, 6><<<pf_blocks(3 * g.nn * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(
-        g.nn, g.ne, g.elemnod.p, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p, row_ptr, g.kebuf.p, val);
+    if (9 * g.maxdeg <= 64 * 4)  // a hex8 node has at most 27 neighbours
+      k_gather_nodes<4><<<pf_blocks(g.nn * 64, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+          g.nn, g.ne, g.elemnod.p, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p, row_ptr, g.kebuf.p, val);
+    else
+      k_gather_rows<16, 6><<<pf_blocks(3 * g.nn * 16, 256, INT32_MAX), 256, 0, ctx->stream>>>(
+          g.nn, g.ne, g.elemnod.p, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p, row_ptr, g.kebuf.p, val);
   } else {
     k_assemble_rows<<<pf_blocks(3 * g.nn, 128), 128, 0, ctx->stream>>>(
         g.nn, g.ne, g.nodcoord.p, g.elemnod.p, E, nu, jmode, g.eoff.p, g.elist.p, g.noff.p, g.ncol.p,
