@@ -109,6 +109,50 @@ def test_assembly_bit_exact(ctx, jmode):
     assert np.array_equal(crs.val, val), np.abs(crs.val - val).max()
 
 
+def pie_mesh(sectors, layers, seed):
+    """Wedges around the z axis as degenerate hexes: element (c, r_k, r_k+1, c, c', r'_k,
+    r'_k+1, c') lists each axis node twice, and an axis node neighbours every rim node of
+    its own and the adjacent levels (3 (sectors + 1) nodes in the middle levels)."""
+    rng = np.random.default_rng(seed)
+    per = sectors + 1
+    nod = []
+    for z in range(layers + 1):
+        nod.append((0.0, 0.0, 0.1 * z))
+        for k in range(sectors):
+            th = 2.0 * np.pi * k / sectors
+            nod.append((0.1 * np.cos(th), 0.1 * np.sin(th), 0.1 * z))
+    nod = np.asfortranarray(np.array(nod) + 0.005 * rng.standard_normal((len(nod), 3)))
+    el = []
+    for z in range(layers):
+        c, ct = z * per + 1, (z + 1) * per + 1
+        for k in range(sectors):
+            r, r1 = c + 1 + k, c + 1 + (k + 1) % sectors
+            el.append((c, r, r1, c, ct, r + per, r1 + per, ct))
+    return nod, np.asfortranarray(np.array(el, dtype=np.int32))
+
+
+@pytest.mark.parametrize("sectors", [6, 16])
+def test_assembly_degenerate_and_high_degree(ctx, sectors):
+    """Elements that list a node twice (every match of a local node adds, in ascending local
+    index) and, with 16 sectors, axis nodes of 51 neighbours (the per-row gather taken when
+    a node's three rows do not fit one wavefront): CRS values bitwise vs the oracle."""
+    nod, el = pie_mesh(sectors, 2, sectors)
+    rng = np.random.default_rng(sectors)
+    ne = el.shape[0]
+    E = 1e7 * np.exp(0.5 * rng.standard_normal(ne))
+    nu = rng.uniform(0.25, 0.35, ne)
+    say("assembly pie", sectors)
+    crs = FEMDomain(nod, el, ctx).StiffnessMatrix(YoungModulus=E, PoissonRatio=nu)
+    rp, ci = O.zero_matrix(el, nod.shape[0], 3)
+    deg = np.diff(rp)[::3] // 3
+    assert deg.max() == 3 * (sectors + 1)
+    assert (9 * deg.max() > 256) == (sectors == 16)  # 28 neighbours fill one wavefront's 4 x 64 lanes
+    val = O.assemble(nod, el, E, nu, rp, ci, 0)
+    assert np.array_equal(crs.row_ptr, rp) and np.array_equal(crs.col_idx, ci)
+    assert np.all(np.isfinite(val))
+    assert np.array_equal(crs.val, val), np.abs(crs.val - val).max()
+
+
 def test_setmatrix_setvector_call_order(ctx):
     """setMatrix/setVector batches add in call order, like setValueFEMSolver."""
     nod, el = cube3d(3, 3, 2)
