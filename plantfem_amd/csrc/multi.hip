@@ -65,31 +65,68 @@ __device__ inline void case_partials(double (&v)[NV], int R, double *part) {
   }
 }
 
-// every node row of y = A x for the calling thread's case lane; op(dof, case, sum)
+// blocks per step of the row loop: their node columns, x gathers and values are all in
+// flight before the first product of the step (a hex8 node row holds at most 27 blocks).
+// The node-block arrays end with MCHUNK_MAX zero blocks, so a step may read past its row.
+constexpr int MCHUNK_MAX = 9;
+template <int RP>
+constexpr int mchunk() {
+  return RP == 64 ? 9 : 3;  // block values in scalar registers with one node per wavefront
+}
+
+// every node row of y = A x for the calling thread's case lane; op(dof, case, sum).  The
+// blocks are summed in stored order.  Lanes past the last case load case 0 (in bounds) and
+// drop the result, so no load waits behind a lane condition.
 template <int RP, typename XF, typename OP>
 __device__ inline void node_rows(const Blk &B, int R, XF xf, OP op) {
+  constexpr int MC = mchunk<RP>();
+  static_assert(MC <= MCHUNK_MAX, "row step longer than the zero blocks after the last row");
   const int l = threadIdx.x % RP;
-  const int64_t g0 = (blockIdx.x * (int64_t)MBT + threadIdx.x) / RP;
-  const int64_t ng = (int64_t)gridDim.x * (MBT / RP);
   const bool live = l < R;
-  for (int64_t i0 = g0; i0 < B.nn; i0 += ng) {
+  const int lc = live ? l : 0;
+  // Workgroups are dispatched to the 8 XCDs in turn (block b on XCD b % 8).  With a grid of
+  // a multiple of 8 blocks, XCD k walks its own eighth of the nodes in order, so the nodes in
+  // flight on one XCD are consecutive and the R-wide x rows they gather (up to 27 per node,
+  // 1.5 KB each with 64 cases) are shared in that XCD's L2 instead of fetched by several.
+  int64_t i_lo = 0, i_hi = B.nn, g0, ng;
+  if (gridDim.x % 8 == 0) {
+    const int64_t k = blockIdx.x % 8;
+    i_lo = k * B.nn / 8;
+    i_hi = (k + 1) * B.nn / 8;
+    g0 = i_lo + (blockIdx.x / 8) * (MBT / RP) + threadIdx.x / RP;
+    ng = (int64_t)(gridDim.x / 8) * (MBT / RP);
+  } else {
+    g0 = (blockIdx.x * (int64_t)MBT + threadIdx.x) / RP;
+    ng = (int64_t)gridDim.x * (MBT / RP);
+  }
+  for (int64_t i0 = g0; i0 < i_hi; i0 += ng) {
     // with 64 cases a wavefront owns one node: make the node (and so every matrix address
     // below) wave-uniform so the block values come through scalar loads
     const int64_t i = RP == 64 ? (int64_t)__builtin_amdgcn_readfirstlane((int)i0) : i0;
     const int64_t b0 = B.noff[i], b1 = B.noff[i + 1];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int64_t blk = b0; blk < b1; ++blk) {
-      const int64_t j = 3 * (int64_t)B.bcol[blk];
-      const double *q = B.bval + blk * 9;
-      double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-      if (live) {
-        x0 = xf(j, l);
-        x1 = xf(j + 1, l);
-        x2 = xf(j + 2, l);
+    for (int64_t blk = b0; blk < b1; blk += MC) {
+      int64_t j[MC];
+#pragma unroll
+      for (int u = 0; u < MC; ++u) j[u] = 3 * (int64_t)B.bcol[blk + u];
+      double x[MC][3];
+#pragma unroll
+      for (int u = 0; u < MC; ++u) {
+        x[u][0] = xf(j[u], lc);
+        x[u][1] = xf(j[u] + 1, lc);
+        x[u][2] = xf(j[u] + 2, lc);
       }
-      a0 += q[0] * x0 + q[1] * x1 + q[2] * x2;
-      a1 += q[3] * x0 + q[4] * x1 + q[5] * x2;
-      a2 += q[6] * x0 + q[7] * x1 + q[8] * x2;
+#pragma unroll
+      for (int u = 0; u < MC; ++u) {
+        const double *q = B.bval + (blk + u) * 9;
+        const double t0 = q[0] * x[u][0] + q[1] * x[u][1] + q[2] * x[u][2];
+        const double t1 = q[3] * x[u][0] + q[4] * x[u][1] + q[5] * x[u][2];
+        const double t2 = q[6] * x[u][0] + q[7] * x[u][1] + q[8] * x[u][2];
+        const bool own = blk + u < b1;  // a select, not a branch: no load waits behind it
+        a0 = own ? a0 + t0 : a0;
+        a1 = own ? a1 + t1 : a1;
+        a2 = own ? a2 + t2 : a2;
+      }
     }
     if (live) {
       op(3 * i, l, a0);
@@ -393,7 +430,13 @@ __global__ void k_bcsr_build(int64_t nn, const int64_t *__restrict__ row_ptr, co
   if (i > nn) return;
   const int64_t o = (row_ptr[3 * i] - 1) / 9;
   noff[i] = o;
-  if (i == nn) return;
+  if (i == nn) {  // the zero blocks after the last row
+    for (int t = 0; t < MCHUNK_MAX; ++t) {
+      bcol[o + t] = 0;
+      for (int k = 0; k < 9; ++k) bval[(o + t) * 9 + k] = 0.0;
+    }
+    return;
+  }
   const int64_t w = (row_ptr[3 * i + 1] - row_ptr[3 * i]) / 3;
   for (int64_t t = 0; t < w; ++t) {
     bcol[o + t] = (col_idx[row_ptr[3 * i] - 1 + 3 * t] - 1) / 3;
@@ -513,8 +556,8 @@ int pf_multi_solve_device(pf_ctx *ctx, MultiWork **pw, int64_t n, const int64_t 
     PF_CHECK(pf_read_i64(ctx, row_ptr + n, &nnz1));
     const int64_t nblk = (nnz1 - 1) / 9;
     PF_CHECK(w->noff.ensure((size_t)nn + 1));
-    PF_CHECK(w->bcol.ensure((size_t)nblk + 1));
-    PF_CHECK(w->bval.ensure((size_t)nblk * 9 + 1));
+    PF_CHECK(w->bcol.ensure((size_t)nblk + MCHUNK_MAX));
+    PF_CHECK(w->bval.ensure(((size_t)nblk + MCHUNK_MAX) * 9));
     k_bcsr_build<<<pf_blocks(nn + 1, 256, INT32_MAX), 256, 0, ctx->stream>>>(nn, row_ptr, col_idx, val, w->noff.p,
                                                                            w->bcol.p, w->bval.p);
     PF_HIP(hipGetLastError());

@@ -76,6 +76,29 @@ def test_multi_rhs_matches_oracle():
         assert np.all(U[fixed == 1, l] == 0.0)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("ncases", [40, 64])
+def test_multi_rhs_wavefront_of_cases_matches_oracle(ncases):
+    """33-64 cases: one node per wavefront (block values in scalar registers, nine blocks
+    per step of the row loop, the last step of a row reading into the next row or the zero
+    blocks after the last one); 40 cases leave 24 lanes without a case."""
+    p = W.inverse_problem(n=5, ncases=ncases, nobs=20, seed=5)
+    sysm = _system(p, relative_er=1e-13)
+    sysm.assemble(torch.tensor(p.E_hidden, device="cuda"))
+    U = sysm.solve(torch.tensor(p.F, device="cuda")).cpu().numpy()
+    st = sysm.last_stats
+    assert st["total"]["status"] == 0 and st["total"]["true_relres"] < 1e-12
+    rp, ci, val, fixed = _oracle_eliminated(p, p.E_hidden)
+    worst = 0.0
+    for l in range(ncases):
+        b = np.where(fixed == 1, 0.0, p.F[:, l])
+        xo, status, it, *_ = O.solve(rp, ci, val, b, fixed, np.zeros(b.size), relative_er=1e-14, er0=1e-300)
+        assert status == 0
+        worst = max(worst, np.linalg.norm(U[:, l] - xo) / np.linalg.norm(xo))
+    say("cases", ncases, "worst error vs oracle", worst)
+    assert worst <= 1e-10
+
+
 def _loss_and_grad(p, sysm, logE, D, obs):
     from plantfem_amd.autograd import static_solve
     logE = logE.clone().requires_grad_(True)
