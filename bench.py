@@ -191,16 +191,16 @@ def main(argv=None):
                 target_relres=target, raise_on_breakdown=False, use_graph=0 if a.no_graph else 1, **kw)
         return dict(s.last_stats)
 
-    # ---- FP64 time to solve (timed steps carry per-kernel hipEvent timing) -------------
+    # ---- FP64 time to solve (timed steps carry hipEvents around every y = A p) ----------
     for i in range(a.warmup):
-        st = step(64, 1)
+        st = step(64, 2)
         log(f"warmup {i}: {st['iterations']} it, {st['solve_ms']:.1f} ms, relres {st['true_relres']:.2e}")
     ctx.synchronize()
     times, stats = [], []
     for i in range(a.steps):
         ctx.synchronize()
         t = time.perf_counter()
-        st = step(64, 1)
+        st = step(64, 2)
         ctx.synchronize()
         times.append(time.perf_counter() - t)
         stats.append(st)
@@ -209,7 +209,7 @@ def main(argv=None):
     t_step = float(np.median(times))
     st64 = stats[-1]
 
-    # graph-captured solve without per-kernel events, for the overhead of the timing mode
+    # graph-captured solve without events, for the overhead of the timing mode
     t = time.perf_counter()
     st_graph = step(64, 0)
     ctx.synchronize()

@@ -73,7 +73,8 @@ typedef struct {
                                return before the loop when r0.r0 < er (:1527), stop when
                                r.r / r0.r0 < er, no square root (:1561); FP64, no ILU */
   int32_t kernel_timing; /* 1: bracket every kernel of the iteration loop with hipEvents (no hipGraph)
-                            and report the average device time of each in pf_stats */
+                            and report the average device time of each in pf_stats;
+                            2: the same for y = A p only (two events per iteration) */
   int32_t jacobi;        /* precision 32, scaling of the FP32 inner system (D = diag(A)):
                             2 (default): rows, D^-1 A, and every replacement restarts the inner
                               BiCGSTAB from the FP64 residual (iterative refinement);
@@ -95,7 +96,8 @@ typedef struct {
   double solve_ms;          /* device time of the solve, hipEvents on the context stream */
   double setup_ms;          /* device time of per-solve setup (blocked layout, FP32 copy) */
   /* kernel_timing = 1: average device ms of y = A p, v = A e, the x/r update and the p update
-     (FP32 inner kernels for precision 32); 0 otherwise */
+     (FP32 inner kernels for precision 32); kernel_timing = 2: y = A p only, the others 0;
+     0 otherwise */
   double spmv_p_ms, spmv_e_ms, update_ms, pupdate_ms;
   int64_t timed_iterations;
 } pf_stats;

@@ -850,8 +850,10 @@ int enqueue_init(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, const V *b, V
 template <typename V, typename M>
 int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, V *x, V *r, const V *r0, V *p, V *y, V *v,
                   Scal *sc, double *part, double *hist, int count, const Rel<V> *rel, V *ebuf,
-                  hipEvent_t *ev = nullptr) {
+                  hipEvent_t *ev = nullptr, bool ev_all = true) {
   PF_CHECK(check_grid(ctx, L));
+  // ev: events 0 and 1 bracket y = A p; with ev_all, events 2-4 follow the other kernels
+  hipEvent_t *eva = ev_all ? ev : nullptr;
   if (L.layout == 2 && !ebuf) {
     pf_set_error("internal: the staged CRS layout needs the e buffer");
     return PF_INVALID;
@@ -882,12 +884,12 @@ int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, V *x, V *r, 
         PF_DISPATCH_SUB(L.sub, (k_spmv_e<V, 0, S_><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, p1, g1, p2)));
       }
     }
-    if (ev) PF_HIP(hipEventRecord(ev[5 * it + 2], st));
+    if (eva) PF_HIP(hipEventRecord(eva[5 * it + 2], st));
     k_update<V><<<L.grid_e, BT, 0, st>>>(m.n, x, r, r0, p, y, v, sc, p2, g1, p3);
-    if (ev) PF_HIP(hipEventRecord(ev[5 * it + 3], st));
+    if (eva) PF_HIP(hipEventRecord(eva[5 * it + 3], st));
     k_pupdate<V><<<L.grid_e, BT, 0, st>>>(m.n, p, r, y, sc, p3, ge, par, hist, rel ? rel->zhi : nullptr,
                                           rel ? x : nullptr);
-    if (ev) PF_HIP(hipEventRecord(ev[5 * it + 4], st));
+    if (eva) PF_HIP(hipEventRecord(eva[5 * it + 4], st));
     if (rel) PF_CHECK(enqueue_replacement<false>(ctx, *rel, sc, part + PART_PI * G));
   }
   PF_HIP(hipGetLastError());
@@ -949,15 +951,16 @@ int run_loop(pf_ctx *ctx, SolverWork *w, int slot, const Launch &L, const Mat<V,
       PF_HIP(hipGraphLaunch(w->gexec[slot], ctx->stream));
     } else {
       PF_CHECK(enqueue_iters<V>(ctx, L, m, x, r, r0, p, y, v, sc, w->part.p, hist, chunk, rel, ebuf,
-                                timing ? w->evs.data() : nullptr));
+                                timing ? w->evs.data() : nullptr, o.kernel_timing != 2));
     }
     PF_HIP(hipMemcpyAsync(hs, sc, sizeof(Scal), hipMemcpyDeviceToHost, ctx->stream));
     PF_HIP(hipStreamSynchronize(ctx->stream));
     if (timing) {
       // iterations whose four kernels ran (kernels after a stop return at entry)
       const int live = hs->itr_last - itr_before;
+      const int nk = o.kernel_timing == 2 ? 1 : 4;  // 2: y = A p only
       for (int it = 0; it < live && it < chunk; ++it)
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < nk; ++k) {
           float ms = 0.f;
           PF_HIP(hipEventElapsedTime(&ms, w->evs[5 * it + k], w->evs[5 * it + k + 1]));
           w->t_sum[k] += ms;

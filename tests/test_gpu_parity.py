@@ -404,6 +404,30 @@ def test_breakdown_and_itrmax_statuses(ctx):
 
 
 @pytest.mark.parametrize("kind", [1, 2, 4])
+def test_kernel_timing_modes(ctx, kind):
+    """kernel_timing 1 (hipEvents around every kernel, direct launches) and 2 (around y = A p
+    only) give the graph-launched solve's solution bitwise and report the averages they time."""
+    w = W.box(10, seed=3)
+    s = FEMSolver(ctx)
+    s.setDomain(FEMDomain(w.nodcoord, w.elemnod, ctx))
+    s.setStiffness(w.E, w.nu)
+    s.addTraction(w.traction_box, w.pressure)
+    s.fix(1, w.fixed_dofs, FixValue=0.0)
+    xs, sts = [], []
+    for mode in (0, 1, 2):
+        say("timing mode", mode, "kind", kind)
+        xs.append(s.solve(spmv_kind=kind, kernel_timing=mode))
+        sts.append(dict(s.last_stats))
+    assert all(st["status"] == 0 and st["iterations"] == sts[0]["iterations"] for st in sts)
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+    names = ("spmv_p_ms", "spmv_e_ms", "update_ms", "pupdate_ms")
+    assert all(sts[0][k] == 0.0 for k in names)
+    assert all(sts[1][k] > 0.0 for k in names) and sts[1]["timed_iterations"] == sts[1]["iterations"]
+    assert sts[2]["spmv_p_ms"] > 0.0 and all(sts[2][k] == 0.0 for k in names[1:])
+    assert sts[2]["timed_iterations"] == sts[2]["iterations"]
+
+
+@pytest.mark.parametrize("kind", [1, 2, 4])
 def test_c1_solve_matches_oracle(ctx, kind):
     """Config 1 end to end on the GPU vs the oracle, both converged tightly: 1e-10 relative (CRS rows,
     blocked, CRS rows with x staged in LDS)."""
