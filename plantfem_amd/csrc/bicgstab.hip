@@ -289,11 +289,11 @@ __global__ void __launch_bounds__(BT) k_evec(int64_t n, const V *__restrict__ r,
   for (int64_t i = blockIdx.x * (int64_t)BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT) e[i] = ef(i);
 }
 
-// at most 64 VGPRs (8 waves per SIMD): the grid of 8 blocks per CU is then resident at once
-// (the staged form at 32 lanes per row took 66 and ran 0.241 instead of 0.177 ms on a 465k-row
-// overset system)
+// CRS forms: at most 64 VGPRs (8 waves per SIMD), so the grid of 8 blocks per CU is resident at
+// once (the staged form at 32 lanes per row took 66 and ran 0.241 instead of 0.177 ms on a
+// 465k-row overset system); the blocked form keeps its registers
 template <typename V, int L, int SUB, typename M>
-__global__ void __launch_bounds__(BT, 8) k_spmv_ev(Mat<V, M> m, const V *__restrict__ e, V *__restrict__ v,
+__global__ void __launch_bounds__(BT, L == 1 ? 1 : 8) k_spmv_ev(Mat<V, M> m, const V *__restrict__ e, V *__restrict__ v,
                                                 Scal *__restrict__ sc, double *__restrict__ part) {
   if (sc->done) return;
   const pfdev::PlainX<V> ef{e};
@@ -868,7 +868,12 @@ int enqueue_iters(pf_ctx *ctx, const Launch &L, const Mat<V, M> &m, V *x, V *r, 
     if (L.layout == 1) {
       k_spmv_p<V, 1, 64><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, p1);
       if (ev) PF_HIP(hipEventRecord(ev[5 * it + 1], st));
-      k_spmv_e<V, 1, 64><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, p1, g1, p2);
+      if (ebuf) {  // e streamed, then v = A e gathering e alone
+        k_evec<V><<<L.grid_e, BT, 0, st>>>(m.n, r, y, ebuf, sc, p1, g1);
+        k_spmv_ev<V, 1, 64><<<L.grid, BT, 0, st>>>(m, ebuf, v, sc, p2);
+      } else {
+        k_spmv_e<V, 1, 64><<<L.grid, BT, 0, st>>>(m, r, y, v, sc, p1, g1, p2);
+      }
     } else if (L.layout == 2) {  // e streamed, then staged per row block like p
       PF_DISPATCH_SUB(L.sub, (k_spmv_p<V, 2, S_><<<L.grid, BT, 0, st>>>(m, p, y, r0, sc, p1)));
       if (ev) PF_HIP(hipEventRecord(ev[5 * it + 1], st));
@@ -1185,7 +1190,16 @@ int solve_refined(pf_ctx *ctx, SolverWork *w, int64_t n, const int64_t *row_ptr,
   k_rel_setz<<<ge, BT, 0, ctx->stream>>>(n, x, cscale, w->zhi.p);
   PF_HIP(hipGetLastError());
   PF_CHECK(enqueue_replacement<true>(ctx, R, w->sc.p + 1, w->part.p + PART_PI * G));
-  PF_CHECK((run_loop<VI, float>(ctx, w, 1, Li, mi, zlo, r, r0, p, y, v, nullptr, o, &R)));
+  // precision 48 on the blocked layout: e = r - alpha y streamed once, then v = A e gathers e
+  // alone (its FP32 values make the two FP64 gathers of the fused form a larger share)
+  VI *ebuf = nullptr;
+  if constexpr (std::is_same<VI, double>::value) {
+    if (layout == 1) {
+      PF_CHECK(w->e.ensure(n));
+      ebuf = w->e.p;
+    }
+  }
+  PF_CHECK((run_loop<VI, float>(ctx, w, 1, Li, mi, zlo, r, r0, p, y, v, nullptr, o, &R, ebuf)));
   k_rel_final<<<ge, BT, 0, ctx->stream>>>(n, w->zhi.p, zlo, cscale, x);
   PF_HIP(hipGetLastError());
   const Scal &hi = w->h_sc[1];
