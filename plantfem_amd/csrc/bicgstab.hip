@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(BT) k_pupdate(int64_t n, V *__restrict__ p, co
   } else if (sc->stop_mode == 3) {  // bicgstab_CRS: rr / init_rr < er, no square root (:1561)
     stop = rr / sc->rr0 < sc->er0;
   } else {
-    // reliable updates: convergence is judged on the FP64 true residual after a
+    // reliable updates: convergence is decided on the FP64 true residual after a
     // replacement; here only decide whether one is due (|r| fell by delta since its max)
     stop = false;
     const double nr = sqrt(rr);
