@@ -376,10 +376,9 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   k_dinv<<<pf_blocks(n, 256, INT32_MAX), 256, 0, st>>>(n, row_ptr, col_idx, A, red.p, dinv.p);
   k_init_block<<<grid_for(n * m), 256, 0, st>>>(n, m, red.p, X);
   PF_HIP(hipGetLastError());
+  // every column of a block in one launch (the matrix streamed once, not once per column)
   auto spmv_block = [&](const double *M, const double *in, double *out, int64_t cols) -> int {
-    for (int64_t c = 0; c < cols; ++c)
-      PF_CHECK(pf_spmv_crs_launch<double>(ctx, n, row_ptr, col_idx, M, in + c * n, out + c * n, avg_row));
-    return PF_OK;
+    return pf_spmm_crs_launch(ctx, n, cols, row_ptr, col_idx, M, in, out, avg_row);
   };
   // Rayleigh-Ritz on the first k columns of S; keeps the lowest m in X (and, with k > m, the
   // W/P part of the combination in P).  Returns 1 when dsygvd fails.

@@ -200,6 +200,10 @@ int pf_bc_set_fixed_x(pf_ctx *ctx, int64_t n, const uint8_t *fixed, const double
 template <typename V>
 int pf_spmv_crs_launch(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                        const V *val, const V *x, V *y, double avg_row, bool accumulate = false);
+// Y = A X for cols column-major vectors (leading dimension n), every column's sums those of
+// pf_spmv_crs_launch bitwise, the matrix streamed once
+int pf_spmm_crs_launch(pf_ctx *ctx, int64_t n, int64_t cols, const int64_t *row_ptr, const int32_t *col_idx,
+                       const double *val, const double *x, double *y, double avg_row);
 // CRS product with x staged in LDS per block of rpb rows: for block b the sorted distinct
 // 0-based columns of its entries (ucol[ubase[b] ..], ucnt[b] of them) and for every entry the
 // position of its column in that list (lidx, 16-bit).  ucnt[b] = -1 marks a block whose

@@ -20,6 +20,24 @@ __global__ void __launch_bounds__(256) k_spmv_crs(int64_t n, const int64_t *__re
   if (sub == 0) y[row] = ACC ? y[row] + s : s;
 }
 
+// the same row sums for several vectors (X, Y column-major, leading dimension n): a subgroup
+// takes one row and every column in turn, so the row's values and column indices come from
+// HBM once and from cache for the other columns
+template <typename V, int SUB>
+__global__ void __launch_bounds__(256) k_spmm_crs(int64_t n, int64_t cols, const int64_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ col_idx,
+                                                  const V *__restrict__ val, const V *__restrict__ x,
+                                                  V *__restrict__ y) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = g / SUB;
+  const int sub = (int)(g % SUB);
+  if (row >= n) return;  // whole subgroups leave together (SUB divides 64)
+  for (int64_t c = 0; c < cols; ++c) {
+    const V s = pfdev::crs_row<V, V, SUB>(row_ptr, col_idx, val, row, sub, pfdev::PlainX<V>{x + c * n});
+    if (sub == 0) y[row + c * n] = s;
+  }
+}
+
 template <typename V>
 __global__ void __launch_bounds__(256) k_spmv_sell3(int64_t nn, int64_t nslices,
                                                     const int64_t *__restrict__ soff,
@@ -366,6 +384,22 @@ template int pf_spmv_crs_launch<double>(pf_ctx *, int64_t, const int64_t *, cons
                                         const double *, double *, double, bool);
 template int pf_spmv_crs_launch<float>(pf_ctx *, int64_t, const int64_t *, const int32_t *, const float *,
                                        const float *, float *, double, bool);
+
+int pf_spmm_crs_launch(pf_ctx *ctx, int64_t n, int64_t cols, const int64_t *row_ptr, const int32_t *col_idx,
+                       const double *val, const double *x, double *y, double avg_row) {
+  if (n <= 0 || cols <= 0) return PF_OK;
+  const int sub = pf_crs_sub(avg_row);
+  const unsigned blocks = pf_blocks(n * sub, 256, INT32_MAX);
+  switch (sub) {
+    case 4: k_spmm_crs<double, 4><<<blocks, 256, 0, ctx->stream>>>(n, cols, row_ptr, col_idx, val, x, y); break;
+    case 8: k_spmm_crs<double, 8><<<blocks, 256, 0, ctx->stream>>>(n, cols, row_ptr, col_idx, val, x, y); break;
+    case 16: k_spmm_crs<double, 16><<<blocks, 256, 0, ctx->stream>>>(n, cols, row_ptr, col_idx, val, x, y); break;
+    case 32: k_spmm_crs<double, 32><<<blocks, 256, 0, ctx->stream>>>(n, cols, row_ptr, col_idx, val, x, y); break;
+    default: k_spmm_crs<double, 64><<<blocks, 256, 0, ctx->stream>>>(n, cols, row_ptr, col_idx, val, x, y); break;
+  }
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
 
 int pf_crs_sub(double avg_row) {
   // subgroup of ~avg_row/5 lanes (4..64): about five nonzeros per lane (measured at 100^3,
