@@ -113,6 +113,41 @@ __global__ void k_copy(int64_t count, const double *__restrict__ a, double *__re
     b[t] = a[t];
 }
 
+// Rayleigh-Ritz update of the search block, in place: for M = S, AS, BS (leading dimension
+// n, k columns [X | W | P]) and the lowest m Ritz vectors C (k x m, leading dimension ldc),
+// X_M = M C and, with k > m, P_M = [W P]_M C[m:k, :] into columns 2m.. of M.  One thread per
+// row reads the row's k entries before it writes the row's new ones.
+template <int MAXM>
+__global__ void __launch_bounds__(256) k_rr_update(int64_t n, int k, int m, const double *__restrict__ C, int ldc,
+                                                   double *S, double *AS, double *BS) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int w = 0; w < 3; ++w) {
+      double *M = w == 0 ? S : (w == 1 ? AS : BS);
+      double xo[MAXM], po[MAXM];
+#pragma unroll
+      for (int j = 0; j < MAXM; ++j) xo[j] = po[j] = 0.0;
+      for (int c = 0; c < k; ++c) {
+        const double v = M[i + (int64_t)c * n];
+        if (c < m) {
+#pragma unroll
+          for (int j = 0; j < MAXM; ++j)
+            if (j < m) xo[j] += v * C[c + j * ldc];
+        } else {
+#pragma unroll
+          for (int j = 0; j < MAXM; ++j)
+            if (j < m) po[j] += v * C[c + j * ldc];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < MAXM; ++j)
+        if (j < m) {
+          M[i + (int64_t)j * n] = xo[j] + po[j];
+          if (k > m) M[i + (int64_t)(2 * m + j) * n] = po[j];
+        }
+    }
+  }
+}
+
 // out[c] = sum_i X[i, c] Y[i, c], one block per column
 __global__ void __launch_bounds__(1024) k_coldots(int64_t n, const double *__restrict__ X,
                                                   const double *__restrict__ Y, double *__restrict__ out) {
@@ -352,12 +387,11 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   }
   *count = nev;
   const int64_t K = 3 * m;
-  DevBuf<double> S, AS, BS, T, R, dinv, GA, GB, th, E, sc;
+  DevBuf<double> S, AS, BS, R, dinv, GA, GB, th, E, sc;
   DevBuf<rocblas_int> info;
   PF_CHECK(S.ensure((size_t)(n * K)));
   PF_CHECK(AS.ensure((size_t)(n * K)));
   PF_CHECK(BS.ensure((size_t)(n * K)));
-  PF_CHECK(T.ensure((size_t)(n * m * 3)));
   PF_CHECK(R.ensure((size_t)(n * m)));
   PF_CHECK(dinv.ensure((size_t)n));
   PF_CHECK(GA.ensure((size_t)(K * K)));
@@ -370,9 +404,9 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   const unsigned gram_blocks = (unsigned)std::min<int64_t>((n + GR - 1) / GR, (int64_t)std::max(ctx->num_cus / 2, 1));
   DevBuf<double> gpart;
   PF_CHECK(gpart.ensure((size_t)gram_blocks * K * K + 1));
-  double *X = S.p, *Wb = S.p + n * m, *P = S.p + 2 * n * m;
-  double *AX = AS.p, *AW = AS.p + n * m, *AP = AS.p + 2 * n * m;
-  double *BX = BS.p, *BW = BS.p + n * m, *BP = BS.p + 2 * n * m;
+  double *X = S.p, *Wb = S.p + n * m;  // P: columns 2m .. 3m
+  double *AX = AS.p, *AW = AS.p + n * m;
+  double *BX = BS.p, *BW = BS.p + n * m;
   k_dinv<<<pf_blocks(n, 256, INT32_MAX), 256, 0, st>>>(n, row_ptr, col_idx, A, red.p, dinv.p);
   k_init_block<<<grid_for(n * m), 256, 0, st>>>(n, m, red.p, X);
   PF_HIP(hipGetLastError());
@@ -384,7 +418,6 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   // W/P part of the combination in P).  Returns 1 when dsygvd fails.
   std::vector<double> h_th((size_t)K);
   auto rayleigh_ritz = [&](int64_t k, int *failed) -> int {
-    const double one = 1.0, zero = 0.0;
     // Gram matrices S^T (A S) and S^T (B S) (a transposed tall-skinny product, which the
     // library GEMM runs on a handful of workgroups: 21 ms for k = 30, n = 134k; this: one pass)
     k_gram<<<gram_blocks, 1024, 0, st>>>(n, (int)k, S.p, AS.p, gpart.p);
@@ -400,31 +433,15 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
     if (*failed) return PF_OK;
     for (int64_t c = 0; c < k; ++c) h_th[c] = hth[c];
     PF_HIP(hipMemcpyAsync(GA.p, hZ.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, st));
-    // T = [S C | AS C | BS C] for the lowest m vectors C = GA[:, 0:m]
-    double *Tx = T.p, *Ta = T.p + n * m, *Tb = T.p + 2 * n * m;
-    PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
-                        (rocblas_int)k, &one, S.p, (rocblas_int)n, GA.p, (rocblas_int)k, &zero, Tx, (rocblas_int)n));
-    PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
-                        (rocblas_int)k, &one, AS.p, (rocblas_int)n, GA.p, (rocblas_int)k, &zero, Ta, (rocblas_int)n));
-    PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
-                        (rocblas_int)k, &one, BS.p, (rocblas_int)n, GA.p, (rocblas_int)k, &zero, Tb, (rocblas_int)n));
-    if (k > m) {  // P = [W P] C[m:k, :] (written into the P slot after X is replaced)
-      PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
-                          (rocblas_int)(k - m), &one, S.p + n * m, (rocblas_int)n, GA.p + m, (rocblas_int)k, &zero, R.p,
-                          (rocblas_int)n));
-      k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, R.p, P);
-      PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
-                          (rocblas_int)(k - m), &one, AS.p + n * m, (rocblas_int)n, GA.p + m, (rocblas_int)k, &zero,
-                          R.p, (rocblas_int)n));
-      k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, R.p, AP);
-      PF_RB(rocblas_dgemm(H.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)m,
-                          (rocblas_int)(k - m), &one, BS.p + n * m, (rocblas_int)n, GA.p + m, (rocblas_int)k, &zero,
-                          R.p, (rocblas_int)n));
-      k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, R.p, BP);
-    }
-    k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, Tx, X);
-    k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, Ta, AX);
-    k_copy<<<grid_for(n * m), 256, 0, st>>>(n * m, Tb, BX);
+    // X = S C, AX = AS C, BX = BS C for the lowest m vectors C = GA[:, 0:m], and with k > m
+    // P = [W P] C[m:k, :] (the same for AS, BS): one pass over the block, in place
+    const unsigned rg = pf_blocks(n, 256, (int64_t)ctx->num_cus * 8);
+    if (m <= 8)
+      k_rr_update<8><<<rg, 256, 0, st>>>(n, (int)k, (int)m, GA.p, (int)k, S.p, AS.p, BS.p);
+    else if (m <= 16)
+      k_rr_update<16><<<rg, 256, 0, st>>>(n, (int)k, (int)m, GA.p, (int)k, S.p, AS.p, BS.p);
+    else
+      k_rr_update<32><<<rg, 256, 0, st>>>(n, (int)k, (int)m, GA.p, (int)k, S.p, AS.p, BS.p);
     PF_HIP(hipGetLastError());
     return PF_OK;
   };
