@@ -148,21 +148,38 @@ __global__ void __launch_bounds__(256) k_rr_update(int64_t n, int k, int m, cons
   }
 }
 
-// out[c] = sum_i X[i, c] Y[i, c], one block per column
-__global__ void __launch_bounds__(1024) k_coldots(int64_t n, const double *__restrict__ X,
-                                                  const double *__restrict__ Y, double *__restrict__ out) {
-  __shared__ double sh[16];
-  const int64_t c = blockIdx.x;
-  double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += X[c * n + i] * Y[c * n + i];
+// out[c] = sum_i X[i, c] Y[i, c] in two steps: CD_NB blocks per column (blockIdx.y = column)
+// leave one partial each, with four loads in flight per thread; then one thread per column
+// adds its CD_NB partials in order
+constexpr int CD_NB = 64;
+__global__ void __launch_bounds__(256) k_coldots_part(int64_t n, const double *__restrict__ X,
+                                                      const double *__restrict__ Y, double *__restrict__ part) {
+  __shared__ double sh[4];
+  const int64_t c = blockIdx.y;
+  const double *x = X + c * n, *y = Y + c * n;
+  const int64_t stride = (int64_t)CD_NB * 256;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int64_t i = blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    a0 += x[i] * y[i];
+    a1 += x[i + stride] * y[i + stride];
+    a2 += x[i + 2 * stride] * y[i + 2 * stride];
+    a3 += x[i + 3 * stride] * y[i + 3 * stride];
+  }
+  for (; i < n; i += stride) a0 += x[i] * y[i];
+  double acc = (a0 + a1) + (a2 + a3);
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < 16; ++w) t += sh[w];
-    out[c] = t;
-  }
+  if (threadIdx.x == 0) part[c * CD_NB + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+__global__ void k_coldots_sum(int cols, const double *__restrict__ part, double *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  double t = 0.0;
+  for (int b = 0; b < CD_NB; ++b) t += part[(int64_t)c * CD_NB + b];
+  out[c] = t;
 }
 
 // partial Gram matrices G = S^T T over row tiles (S, T column-major n x k, k <= GK): 32 rows
@@ -414,6 +431,15 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   auto spmv_block = [&](const double *M, const double *in, double *out, int64_t cols) -> int {
     return pf_spmm_crs_launch(ctx, n, cols, row_ptr, col_idx, M, in, out, avg_row);
   };
+  // out[0 .. cols) = column dots of X and Y (column-major, leading dimension n)
+  DevBuf<double> cdpart;
+  PF_CHECK(cdpart.ensure((size_t)K * CD_NB));
+  auto coldots = [&](const double *Xc, const double *Yc, int64_t cols, double *out) -> int {
+    k_coldots_part<<<dim3(CD_NB, (unsigned)cols), 256, 0, st>>>(n, Xc, Yc, cdpart.p);
+    k_coldots_sum<<<(unsigned)((cols + 63) / 64), 64, 0, st>>>((int)cols, cdpart.p, out);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+  };
   // Rayleigh-Ritz on the first k columns of S; keeps the lowest m in X (and, with k > m, the
   // W/P part of the combination in P).  Returns 1 when dsygvd fails.
   std::vector<double> h_th((size_t)K);
@@ -463,8 +489,8 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
     PF_HIP(hipMemcpyAsync(th.p, h_th.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
     k_residual_block<<<grid_for(n * m), 256, 0, st>>>(n, m, AX, BX, th.p, dinv.p, red.p, R.p, Wb);
     PF_HIP(hipGetLastError());
-    k_coldots<<<(unsigned)m, 1024, 0, st>>>(n, R.p, R.p, sc.p);
-    k_coldots<<<(unsigned)m, 1024, 0, st>>>(n, BX, BX, sc.p + m);
+    PF_CHECK(coldots(R.p, R.p, m, sc.p));
+    PF_CHECK(coldots(BX, BX, m, sc.p + m));
     PF_HIP(hipGetLastError());
     std::vector<double> nrm((size_t)(2 * m));
     PF_HIP(hipMemcpyAsync(nrm.data(), sc.p, sizeof(double) * 2 * m, hipMemcpyDeviceToHost, st));
@@ -507,7 +533,7 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
     PF_CHECK(spmv_block(B, Wb, BW, na));
     // unit B-norm columns of W (and P)
     const int64_t ncols = have_p ? 2 * na : na;
-    k_coldots<<<(unsigned)ncols, 1024, 0, st>>>(n, S.p + m * n, BS.p + m * n, sc.p);
+    PF_CHECK(coldots(S.p + m * n, BS.p + m * n, ncols, sc.p));
     PF_HIP(hipGetLastError());
     PF_HIP(hipMemcpyAsync(wn.data(), sc.p, sizeof(double) * ncols, hipMemcpyDeviceToHost, st));
     PF_HIP(hipStreamSynchronize(st));
