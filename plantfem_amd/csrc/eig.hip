@@ -217,16 +217,22 @@ __global__ void __launch_bounds__(1024) k_gram(int64_t n, int k, const double *_
 #pragma unroll
   for (int q = 0; q < GPER; ++q) {
     const int p = threadIdx.x + q * 1024;
-    if (p < npair) part[(int64_t)blockIdx.x * npair + p] = acc[q];
+    if (p < npair) part[(int64_t)p * gridDim.x + blockIdx.x] = acc[q];
   }
 }
 
-__global__ void k_gram_sum(int npair, int nblocks, const double *__restrict__ part, double *__restrict__ G) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npair) return;
+// G[p] = sum over the blocks of k_gram of their partials of entry p (part[p * nblocks + b]):
+// one wavefront per entry, lanes over the blocks, then a fixed shuffle tree
+__global__ void __launch_bounds__(256) k_gram_sum(int npair, int nblocks, const double *__restrict__ part,
+                                                  double *__restrict__ G) {
+  const int p = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= npair) return;  // whole wavefronts
   double t = 0.0;
-  for (int b = 0; b < nblocks; ++b) t += part[(int64_t)b * npair + p];
-  G[p] = t;
+  for (int b = lane; b < nblocks; b += 64) t += part[(int64_t)p * nblocks + b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if (lane == 0) G[p] = t;
 }
 
 // symmetric-definite pencil GA z = theta GB z (k x k, column-major, upper parts used) on the
@@ -417,8 +423,8 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
   PF_CHECK(E.ensure((size_t)K));
   PF_CHECK(sc.ensure((size_t)K));
   PF_CHECK(info.ensure(1));
-  // few partial Gram matrices: their sum (k_gram_sum, one thread per entry) reads them all
-  const unsigned gram_blocks = (unsigned)std::min<int64_t>((n + GR - 1) / GR, (int64_t)std::max(ctx->num_cus / 2, 1));
+  // one workgroup per CU for the partial Gram matrices (a wavefront per entry sums them)
+  const unsigned gram_blocks = (unsigned)std::min<int64_t>((n + GR - 1) / GR, (int64_t)std::max(ctx->num_cus, 1));
   DevBuf<double> gpart;
   PF_CHECK(gpart.ensure((size_t)gram_blocks * K * K + 1));
   double *X = S.p, *Wb = S.p + n * m;  // P: columns 2m .. 3m
@@ -447,9 +453,9 @@ int pf_eig_device(pf_ctx *ctx, int64_t n, const int64_t *row_ptr, const int32_t 
     // Gram matrices S^T (A S) and S^T (B S) (a transposed tall-skinny product, which the
     // library GEMM runs on a handful of workgroups: 21 ms for k = 30, n = 134k; this: one pass)
     k_gram<<<gram_blocks, 1024, 0, st>>>(n, (int)k, S.p, AS.p, gpart.p);
-    k_gram_sum<<<(unsigned)((k * k + 255) / 256), 256, 0, st>>>((int)(k * k), (int)gram_blocks, gpart.p, GA.p);
+    k_gram_sum<<<(unsigned)((k * k * 64 + 255) / 256), 256, 0, st>>>((int)(k * k), (int)gram_blocks, gpart.p, GA.p);
     k_gram<<<gram_blocks, 1024, 0, st>>>(n, (int)k, S.p, BS.p, gpart.p);
-    k_gram_sum<<<(unsigned)((k * k + 255) / 256), 256, 0, st>>>((int)(k * k), (int)gram_blocks, gpart.p, GB.p);
+    k_gram_sum<<<(unsigned)((k * k * 64 + 255) / 256), 256, 0, st>>>((int)(k * k), (int)gram_blocks, gpart.p, GB.p);
     PF_HIP(hipGetLastError());
     std::vector<double> hA((size_t)k * k), hB((size_t)k * k), hth, hZ;
     PF_HIP(hipMemcpyAsync(hA.data(), GA.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, st));
